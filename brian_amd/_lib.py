@@ -1,0 +1,280 @@
+"""ctypes binding of libbrianmi.so (declarations in include/brianmi.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``python -m brian_amd.build``).  There is no CPU fallback: if the shared
+library is missing or no gfx950 device is visible, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbrianmi.so")
+
+_lib = None
+
+i32, i64, u64, dbl = C.c_int32, C.c_int64, C.c_uint64, C.c_double
+P = C.c_void_p
+Pd = C.POINTER(C.c_double)
+Pi32 = C.POINTER(C.c_int32)
+Pi64 = C.POINTER(C.c_int64)
+
+
+class BrianMIError(RuntimeError):
+    """Raised when a libbrianmi call returns a non-zero status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"libbrianmi error {code}: {msg}")
+        self.code = code
+
+
+# name -> argtypes; every function returns int (status) unless listed in _RESTYPE
+_SIGS = {
+    "bm_version": [],
+    "bm_net_create": [dbl, C.c_int, C.POINTER(P)],
+    "bm_net_destroy": [P],
+    "bm_net_add_group": [P, i64, C.c_int, Pd, C.POINTER(C.c_int)],
+    "bm_net_set_linear": [P, C.c_int, Pd, Pd],
+    "bm_net_set_linear_per_neuron": [P, C.c_int, Pd, Pd],
+    "bm_net_set_program": [P, C.c_int, C.c_int, C.c_int, C.c_int, Pi32, C.c_int, Pd],
+    "bm_net_set_threshold": [P, C.c_int, C.c_int, dbl, C.c_int],
+    "bm_net_add_reset_op": [P, C.c_int, C.c_int, C.c_int, dbl, C.c_int],
+    "bm_net_set_empirical_threshold": [P, C.c_int, C.c_int, dbl, C.c_int],
+    "bm_net_set_refractory": [P, C.c_int, C.c_int, dbl, Pd],
+    "bm_net_set_timed_input": [P, C.c_int, C.c_int, i64, C.c_int, Pd],
+    "bm_net_add_generator": [P, i64, i64, Pi32, Pi32, C.POINTER(C.c_int)],
+    "bm_net_add_poisson": [P, i64, Pd, u64, C.POINTER(C.c_int)],
+    "bm_net_add_connection": [P, C.c_int, i64, i64, C.c_int, i64, i64, C.c_int, i64, Pi64, Pi32,
+                              Pd, C.c_int, C.c_int, Pi32, C.c_int, C.POINTER(C.c_int)],
+    "bm_net_add_stdp": [P, C.c_int, dbl, dbl, dbl, dbl, dbl, dbl, C.c_int, C.c_int,
+                        C.POINTER(C.c_int)],
+    "bm_net_record": [P, C.c_int, i64],
+    "bm_net_prepare": [P, C.c_int, C.c_int, C.c_int],
+    "bm_net_run": [P, i64],
+    "bm_net_sync": [P],
+    "bm_net_current_step": [P, Pi64],
+    "bm_net_get_state": [P, C.c_int, C.c_int, Pd],
+    "bm_net_set_state": [P, C.c_int, C.c_int, Pd],
+    "bm_net_spike_count": [P, C.c_int, Pi64],
+    "bm_net_get_spikes": [P, C.c_int, Pi32, Pi32, i64, Pi64],
+    "bm_net_clear_spikes": [P, C.c_int],
+    "bm_net_get_weights": [P, C.c_int, Pd],
+    "bm_net_get_stdp_traces": [P, C.c_int, Pd, Pd],
+    "bm_net_set_profiling": [P, C.c_int],
+    "bm_net_get_profile": [P, Pi64, Pd, Pd],
+}
+_RESTYPE = {"bm_last_error": C.c_char_p}
+
+
+def load():
+    """Load libbrianmi.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m brian_amd.build` "
+            "(there is no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    lib.bm_last_error.restype = C.c_char_p
+    lib.bm_last_error.argtypes = []
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names this binding expects the library to export."""
+    return ["bm_last_error"] + list(_SIGS)
+
+
+def check(code):
+    if code != 0:
+        raise BrianMIError(code, load().bm_last_error().decode(errors="replace"))
+
+
+def call(name, *args):
+    check(getattr(load(), name)(*args))
+
+
+def dptr(a):
+    return a.ctypes.data_as(Pd) if a is not None else None
+
+
+def i32ptr(a):
+    return a.ctypes.data_as(Pi32) if a is not None else None
+
+
+def i64ptr(a):
+    return a.ctypes.data_as(Pi64) if a is not None else None
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def as_i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def as_i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+class NetHandle:
+    """Owning wrapper around a ``bm_net*``."""
+
+    def __init__(self, dt, precision=64):
+        self._lib = load()
+        h = P()
+        call("bm_net_create", dt, int(precision), C.byref(h))
+        self.h = h
+        self.dt = dt
+        self.precision = precision
+        self._group_nvars = {}
+
+    def close(self):
+        if self.h:
+            self._lib.bm_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- construction
+    def add_group(self, S0):
+        S0 = f64(S0)
+        m, n = S0.shape
+        gid = C.c_int()
+        call("bm_net_add_group", self.h, n, m, dptr(S0), C.byref(gid))
+        self._group_nvars[gid.value] = m
+        return gid.value
+
+    def set_linear(self, gid, A, Cv):
+        A, Cv = f64(A), f64(Cv)
+        call("bm_net_set_linear", self.h, gid, dptr(A), dptr(Cv))
+
+    def set_linear_per_neuron(self, gid, A, Cv):
+        A, Cv = f64(A), f64(Cv)
+        call("bm_net_set_linear_per_neuron", self.h, gid, dptr(A), dptr(Cv))
+
+    def set_program(self, gid, scheme, nregs, code, consts):
+        code = as_i32(code)
+        consts = f64(consts if len(consts) else [0.0])
+        ninstr = (len(code) - self._nvars(gid)) // 4
+        call("bm_net_set_program", self.h, gid, scheme, nregs, ninstr, i32ptr(code), len(consts),
+             dptr(consts))
+
+    def _nvars(self, gid):
+        return self._group_nvars[gid]
+
+    def set_threshold(self, gid, var, vt, vt_var=-1):
+        call("bm_net_set_threshold", self.h, gid, var, float(vt), vt_var)
+
+    def set_empirical_threshold(self, gid, var, vt, period):
+        call("bm_net_set_empirical_threshold", self.h, gid, var, float(vt), int(period))
+
+    def add_reset_op(self, gid, var, op, value=0.0, value_var=-1):
+        call("bm_net_add_reset_op", self.h, gid, var, op, float(value), value_var)
+
+    def set_refractory(self, gid, period_bins, refractory_time, per_neuron=None):
+        pn = f64(per_neuron) if per_neuron is not None else None
+        call("bm_net_set_refractory", self.h, gid, int(period_bins), float(refractory_time), dptr(pn))
+
+    def set_timed_input(self, gid, var, values, per_neuron):
+        v = f64(values)
+        call("bm_net_set_timed_input", self.h, gid, var, v.shape[0], 1 if per_neuron else 0, dptr(v))
+
+    def add_generator(self, N, steps, idx):
+        steps, idx = as_i32(steps), as_i32(idx)
+        gid = C.c_int()
+        call("bm_net_add_generator", self.h, N, len(steps), i32ptr(steps), i32ptr(idx), C.byref(gid))
+        return gid.value
+
+    def add_poisson(self, N, rates, seed):
+        r = f64(np.broadcast_to(np.asarray(rates, dtype=np.float64), (N,)))
+        gid = C.c_int()
+        call("bm_net_add_poisson", self.h, N, dptr(r), C.c_uint64(seed), C.byref(gid))
+        return gid.value
+
+    def add_connection(self, src, src_off, src_n, tgt, tgt_off, tgt_n, var, indptr, indices, w,
+                       delay_steps=0, syn_delay=None, max_delay_steps=0):
+        indptr, indices, w = as_i64(indptr), as_i32(indices), f64(w)
+        sd = as_i32(syn_delay) if syn_delay is not None else None
+        cid = C.c_int()
+        call("bm_net_add_connection", self.h, src, src_off, src_n, tgt, tgt_off, tgt_n, var,
+             len(indices), i64ptr(indptr), i32ptr(indices), dptr(w), 1 if sd is not None else 0,
+             int(delay_steps), i32ptr(sd), int(max_delay_steps), C.byref(cid))
+        return cid.value
+
+    def add_stdp(self, cid, decay_pre, decay_post, Ap, Am, wmin, wmax, update=0, interactions=0):
+        sid = C.c_int()
+        call("bm_net_add_stdp", self.h, cid, decay_pre, decay_post, Ap, Am, wmin, wmax, update,
+             interactions, C.byref(sid))
+        return sid.value
+
+    def record(self, gid, capacity):
+        call("bm_net_record", self.h, gid, int(capacity))
+
+    def prepare(self, window=0, neurons_per_thread=0, use_graph=True):
+        call("bm_net_prepare", self.h, int(window), int(neurons_per_thread), 1 if use_graph else 0)
+
+    # ---- execution
+    def run(self, nsteps):
+        call("bm_net_run", self.h, int(nsteps))
+
+    def sync(self):
+        call("bm_net_sync", self.h)
+
+    def get_state(self, gid, var, N):
+        out = np.empty(N, dtype=np.float64)
+        call("bm_net_get_state", self.h, gid, var, dptr(out))
+        return out
+
+    def set_state(self, gid, var, values):
+        v = f64(values)
+        call("bm_net_set_state", self.h, gid, var, dptr(v))
+
+    def spike_count(self, gid):
+        n = C.c_int64()
+        call("bm_net_spike_count", self.h, gid, C.byref(n))
+        return n.value
+
+    def get_spikes(self, gid):
+        n = self.spike_count(gid)
+        steps = np.empty(n, dtype=np.int32)
+        idx = np.empty(n, dtype=np.int32)
+        got = C.c_int64()
+        call("bm_net_get_spikes", self.h, gid, i32ptr(steps), i32ptr(idx), n, C.byref(got))
+        return steps[:got.value].astype(np.int64), idx[:got.value].astype(np.int64)
+
+    def clear_spikes(self, gid):
+        call("bm_net_clear_spikes", self.h, gid)
+
+    def get_weights(self, cid, nnz):
+        out = np.empty(nnz, dtype=np.float64)
+        call("bm_net_get_weights", self.h, cid, dptr(out))
+        return out
+
+    def get_stdp_traces(self, sid, n_pre, n_post):
+        a = np.empty(n_pre, dtype=np.float64)
+        b = np.empty(n_post, dtype=np.float64)
+        call("bm_net_get_stdp_traces", self.h, sid, dptr(a), dptr(b))
+        return a, b
+
+    def set_profiling(self, on):
+        call("bm_net_set_profiling", self.h, 1 if on else 0)
+
+    def get_profile(self):
+        n, ms, w = C.c_int64(), C.c_double(), C.c_double()
+        call("bm_net_get_profile", self.h, C.byref(n), C.byref(ms), C.byref(w))
+        return n.value, ms.value, w.value

@@ -1,0 +1,1151 @@
+// Host side of the network engine and its C-ABI (include/brianmi.h).
+#include "../../include/brianmi.h"
+#include "net_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+namespace bm {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+struct HostGroup {
+  int64_t N = 0;
+  int M = 0;
+  int kind = GK_NEURON;
+  std::vector<double> S0;
+  int upd = UPD_NONE;
+  std::vector<double> A, C;          // shared
+  std::vector<double> Apn, Cpn;      // per neuron
+  std::vector<int32_t> prog_code;
+  std::vector<double> prog_consts;
+  int prog_scheme = 0, prog_nregs = 0, prog_ninstr = 0;
+  int th_kind = TH_NONE, th_var = 0, th_vtvar = -1, th_period = 0;
+  double vt = 0;
+  std::vector<int> rs_var, rs_op, rs_vvar;
+  std::vector<double> rs_val;
+  int refr = REFR_NONE, period = 0;
+  double refr_time = 0;
+  std::vector<double> refr_arr;
+  int ti_var = -1, ti_per_neuron = 0;
+  int64_t ti_nt = 0;
+  std::vector<double> ti_vals;
+  std::vector<double> rates;
+  uint64_t seed = 0;
+  std::vector<int32_t> gen_steps, gen_idx;
+  int64_t raster_cap = 0;
+  bool need_hist = false;
+  int H = 0;
+  // device
+  DevBuf S, last, Apn_d, Cpn_d, code_d, consts_d, refr_d, ti_d, pthr_d, gen_idx_d, gen_off_d,
+      raster_d, raster_count_d, hist_d, hist_count_d;
+  std::vector<DevBuf> rings;
+  std::vector<RingDev> ring_desc;
+  std::vector<ChanDev> chans;
+  int D = 1;
+  int64_t gen_nsteps = 0;
+  int64_t host_raster_read = 0;
+  GroupDev dev{};
+};
+
+struct HostConn {
+  int src = 0, tgt = 0, var = 0;
+  int64_t src_off = 0, src_n = 0, tgt_off = 0, tgt_n = 0, nnz = 0;
+  std::vector<int64_t> row_ptr;
+  std::vector<int32_t> col;
+  std::vector<double> w;
+  int delay_kind = 0, hom_delay = 0, max_delay = 0, min_delay = 0;
+  std::vector<int32_t> syn_delay;
+  bool count_mode = false;
+  double wval = 0;
+  int ring = -1, ch = 0;
+  int stdp = -1;
+  int64_t max_indeg = 0;
+  DevBuf row_ptr_d, col_d, w_d, dly_d;
+  int dly_bytes = 0;
+};
+
+struct HostStdp {
+  int conn = 0;
+  double decay_pre = 0, decay_post = 0, Ap = 0, Am = 0, wmin = 0, wmax = 0;
+  int update = 0, interactions = 0;
+  DevBuf col_ptr_d, col_perm_d, col_row_d, a_pre_d, a_post_d;
+  StdpDev dev{};
+};
+
+}  // namespace bm
+
+using namespace bm;
+
+struct bm_net {
+  double dt = 1e-4;
+  int precision = 64;
+  std::vector<std::unique_ptr<HostGroup>> groups;
+  std::vector<std::unique_ptr<HostConn>> conns;
+  std::vector<std::unique_ptr<HostStdp>> stdps;
+  bool prepared = false;
+  int W = 1, K = 4;
+  bool use_graph = true;
+  int64_t step = 0;       // next step to run
+  hipStream_t stream = nullptr;
+  DevBuf d_base;          // int64 device step base
+  DevBuf hist_ptrs_d, hist_H_d;
+  int nhist = 0;
+  // graph cache
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_windows = 0;
+  // profiling
+  bool profiling = false;
+  std::vector<hipEvent_t> ev;
+  int64_t prof_launches = 0;
+  double prof_ms = 0;
+  int device = 0;
+
+  size_t tsize() const { return precision == 64 ? 8 : 4; }
+  ~bm_net() {
+    if (graph_exec) hipGraphExecDestroy(graph_exec);
+    for (auto e : ev) hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+template <typename T>
+void upload_as(DevBuf &buf, const double *src, size_t n, hipStream_t st) {
+  std::vector<T> tmp(n);
+  for (size_t i = 0; i < n; ++i) tmp[i] = (T)src[i];
+  buf.alloc(n * sizeof(T));
+  if (n) BM_HIP(hipMemcpyAsync(buf.p, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice, st));
+  BM_HIP(hipStreamSynchronize(st));
+}
+
+void upload_T(bm_net *net, DevBuf &buf, const double *src, size_t n) {
+  if (net->precision == 64) upload_as<double>(buf, src, n, net->stream);
+  else upload_as<float>(buf, src, n, net->stream);
+}
+
+template <typename R>
+void upload_raw(DevBuf &buf, const R *src, size_t n, hipStream_t st) {
+  buf.alloc(n * sizeof(R));
+  if (n) {
+    BM_HIP(hipMemcpyAsync(buf.p, src, n * sizeof(R), hipMemcpyHostToDevice, st));
+    BM_HIP(hipStreamSynchronize(st));
+  }
+}
+
+HostGroup &grp(bm_net *net, int gid) {
+  BM_REQUIRE(gid >= 0 && gid < (int)net->groups.size(), "bad group id");
+  return *net->groups[gid];
+}
+
+void require_unprepared(bm_net *net) {
+  BM_REQUIRE(!net->prepared, "network already prepared: structure is frozen");
+}
+
+using SweepFn = void (*)(const GroupDev, const int64_t *, int64_t, int, int, int);
+
+template <typename T, int M>
+SweepFn pick_gen(int K) {
+  switch (K) {
+    case 1: return k_sweep<T, M, 1, F_GEN>;
+    default: return k_sweep<T, M, 4, F_GEN>;
+  }
+}
+
+template <typename T, int M>
+SweepFn pick_fast(int K) {
+  switch (K) {
+    case 1: return k_sweep<T, M, 1, 0>;
+    case 2: return k_sweep<T, M, 2, 0>;
+    case 4: return k_sweep<T, M, 4, 0>;
+    default: return k_sweep<T, M, 8, 0>;
+  }
+}
+
+template <typename T>
+SweepFn pick_sweep(int M, int K, bool fast) {
+  if (fast) {
+    switch (M) {
+      case 1: return pick_fast<T, 1>(K);
+      case 2: return pick_fast<T, 2>(K);
+      case 3: return pick_fast<T, 3>(K);
+      default: return pick_fast<T, 4>(K);
+    }
+  }
+  switch (M) {
+    case 0: return pick_gen<T, 0>(K);
+    case 1: return pick_gen<T, 1>(K);
+    case 2: return pick_gen<T, 2>(K);
+    case 3: return pick_gen<T, 3>(K);
+    case 4: return pick_gen<T, 4>(K);
+    case 5: return pick_gen<T, 5>(K);
+    case 6: return pick_gen<T, 6>(K);
+    case 7: return pick_gen<T, 7>(K);
+    default: return pick_gen<T, 8>(K);
+  }
+}
+
+// The lean kernel covers: shared linear propagator, 1..4 variables, constant
+// or variable threshold, reset ops without refractoriness, at most one
+// (packed count) ring, no timed input.
+bool fast_eligible(const HostGroup &g) {
+  if (g.kind != GK_NEURON || g.upd != UPD_LINEAR || g.M < 1 || g.M > 4) return false;
+  if (g.refr != REFR_NONE || g.th_kind == TH_EMPIRICAL || g.ti_nt > 0) return false;
+  if (g.ring_desc.size() > 1) return false;
+  if (g.ring_desc.size() == 1 && g.ring_desc[0].kind != RING_COUNT) return false;
+  return true;
+}
+
+int group_K(bm_net *net, const HostGroup &g) {
+  if (fast_eligible(g)) return net->K;
+  if (g.upd == UPD_PROGRAM) return 1;
+  if (g.upd == UPD_PERNEURON && g.M > 3) return 1;
+  return net->K >= 4 ? 4 : 1;
+}
+
+void launch_group(bm_net *net, HostGroup &g, int64_t rel0, int W, int skip_first_tail,
+                  int tail_only) {
+  const int64_t *dbase = net->d_base.as<int64_t>();
+  if (g.kind == GK_GENERATOR) {
+    if (tail_only) return;
+    const int blocks = 64;
+    if (net->precision == 64)
+      hipLaunchKernelGGL(k_generator<double>, dim3(blocks), dim3(BLOCK), 0, net->stream, g.dev, dbase, rel0, W);
+    else
+      hipLaunchKernelGGL(k_generator<float>, dim3(blocks), dim3(BLOCK), 0, net->stream, g.dev, dbase, rel0, W);
+    return;
+  }
+  const int K = group_K(net, g);
+  const int64_t per_block = (int64_t)BLOCK * K;
+  const int64_t blocks = (g.N + per_block - 1) / per_block;
+  if (blocks == 0) return;
+  size_t shmem = sizeof(uint16_t) * BLOCK * K * std::max(W, 1) + 16;
+  if (g.upd == UPD_PROGRAM) shmem += (size_t)g.prog_nregs * BLOCK * net->tsize();
+  const bool fast = fast_eligible(g);
+  SweepFn fn = net->precision == 64 ? pick_sweep<double>(g.M, K, fast) : pick_sweep<float>(g.M, K, fast);
+  hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(BLOCK), shmem, net->stream, g.dev, dbase, rel0,
+                     W, skip_first_tail, tail_only);
+}
+
+void launch_stdp_step(bm_net *net, int64_t rel0) {
+  const int64_t *dbase = net->d_base.as<int64_t>();
+  for (auto &sp : net->stdps) {
+    HostStdp &s = *sp;
+    const int64_t tot = s.dev.src_n + s.dev.tgt_n;
+    const unsigned b = (unsigned)((tot + BLOCK - 1) / BLOCK);
+    const unsigned wb = 128;
+    if (net->precision == 64) {
+      hipLaunchKernelGGL(k_stdp_decay<double>, dim3(b), dim3(BLOCK), 0, net->stream, s.dev);
+      hipLaunchKernelGGL(k_stdp_pre<double>, dim3(wb), dim3(BLOCK), 0, net->stream, s.dev, dbase, rel0);
+      hipLaunchKernelGGL(k_stdp_post<double>, dim3(wb), dim3(BLOCK), 0, net->stream, s.dev, dbase, rel0);
+    } else {
+      hipLaunchKernelGGL(k_stdp_decay<float>, dim3(b), dim3(BLOCK), 0, net->stream, s.dev);
+      hipLaunchKernelGGL(k_stdp_pre<float>, dim3(wb), dim3(BLOCK), 0, net->stream, s.dev, dbase, rel0);
+      hipLaunchKernelGGL(k_stdp_post<float>, dim3(wb), dim3(BLOCK), 0, net->stream, s.dev, dbase, rel0);
+    }
+  }
+  if (net->nhist > 0)
+    hipLaunchKernelGGL(k_hist_clear, dim3(1), dim3(64), 0, net->stream,
+                       net->hist_ptrs_d.as<unsigned int *>(), net->hist_H_d.as<int>(), net->nhist, dbase,
+                       rel0 + 1);
+}
+
+// One window of W steps starting at base+rel0 for every group (+ STDP when W==1).
+void launch_window(bm_net *net, int64_t rel0, int W, int skip_first_tail) {
+  for (auto &gp : net->groups) launch_group(net, *gp, rel0, W, skip_first_tail, 0);
+  if (!net->stdps.empty()) launch_stdp_step(net, rel0);
+}
+
+void check_launch() { BM_HIP(hipGetLastError()); }
+
+void build_stdp(bm_net *net, HostStdp &s) {
+  HostConn &c = *net->conns[s.conn];
+  // column permutation: stable by source (connectionmatrix.py:317-333)
+  std::vector<int64_t> cnt(c.tgt_n + 1, 0);
+  for (int64_t q = 0; q < c.nnz; ++q) cnt[c.col[q] + 1]++;
+  for (int64_t j = 0; j < c.tgt_n; ++j) cnt[j + 1] += cnt[j];
+  std::vector<int64_t> perm(c.nnz), pos(cnt.begin(), cnt.end() - 1);
+  std::vector<int32_t> rowof(c.nnz);
+  for (int64_t i = 0; i < c.src_n; ++i)
+    for (int64_t q = c.row_ptr[i]; q < c.row_ptr[i + 1]; ++q) {
+      const int64_t p = pos[c.col[q]]++;
+      perm[p] = q;
+      rowof[p] = (int32_t)i;
+    }
+  upload_raw(s.col_ptr_d, cnt.data(), cnt.size(), net->stream);
+  upload_raw(s.col_perm_d, perm.data(), perm.size(), net->stream);
+  upload_raw(s.col_row_d, rowof.data(), rowof.size(), net->stream);
+  std::vector<double> z1(c.src_n, 0.0), z2(c.tgt_n, 0.0);
+  upload_T(net, s.a_pre_d, z1.data(), z1.size());
+  upload_T(net, s.a_post_d, z2.data(), z2.size());
+}
+
+void prepare(bm_net *net, int window, int K, int use_graph) {
+  BM_REQUIRE(!net->prepared, "already prepared");
+  BM_REQUIRE(K == 0 || K == 1 || K == 2 || K == 4 || K == 8, "neurons_per_thread must be 0, 1, 2, 4 or 8");
+  net->K = K == 0 ? 4 : K;
+  net->use_graph = use_graph != 0;
+  // ---- window
+  int dmin = 1 << 30;
+  for (auto &cp : net->conns) dmin = std::min(dmin, cp->min_delay);
+  int W;
+  if (window > 0) {
+    W = window;
+  } else {
+    W = net->conns.empty() ? WMAX : std::min(WMAX, dmin + 1);
+  }
+  if (!net->stdps.empty()) W = 1;
+  BM_REQUIRE(W >= 1 && W <= WMAX, "window out of range [1,16]");
+  BM_REQUIRE(net->conns.empty() || W <= dmin + 1,
+             "window exceeds (minimum synaptic delay + 1) steps");
+  net->W = W;
+
+  // ---- per-connection storage and count/float mode
+  for (size_t ci = 0; ci < net->conns.size(); ++ci) {
+    HostConn &c = *net->conns[ci];
+    bool uniform = c.nnz > 0;
+    for (int64_t q = 1; q < c.nnz && uniform; ++q) uniform = (c.w[q] == c.w[0]);
+    std::vector<int64_t> indeg(c.tgt_n, 0);
+    for (int64_t q = 0; q < c.nnz; ++q) indeg[c.col[q]]++;
+    c.max_indeg = c.tgt_n ? *std::max_element(indeg.begin(), indeg.end()) : 0;
+    c.count_mode = uniform && c.stdp < 0 && c.max_indeg <= 65535;
+    c.wval = c.nnz ? c.w[0] : 0.0;
+    upload_raw(c.row_ptr_d, c.row_ptr.data(), c.row_ptr.size(), net->stream);
+    upload_raw(c.col_d, c.col.data(), c.col.size(), net->stream);
+    if (!c.count_mode) upload_T(net, c.w_d, c.w.data(), c.w.size());
+    if (c.delay_kind == 1) {
+      if (c.max_delay <= 255) {
+        std::vector<uint8_t> d8(c.syn_delay.begin(), c.syn_delay.end());
+        upload_raw(c.dly_d, d8.data(), d8.size(), net->stream);
+        c.dly_bytes = 1;
+      } else {
+        std::vector<uint16_t> d16(c.syn_delay.begin(), c.syn_delay.end());
+        upload_raw(c.dly_d, d16.data(), d16.size(), net->stream);
+        c.dly_bytes = 2;
+      }
+    }
+  }
+  // ---- rings per target group
+  for (size_t gi = 0; gi < net->groups.size(); ++gi) {
+    HostGroup &g = *net->groups[gi];
+    int dmax = 0;
+    std::vector<int> incoming;
+    for (size_t ci = 0; ci < net->conns.size(); ++ci)
+      if (net->conns[ci]->tgt == (int)gi) {
+        incoming.push_back((int)ci);
+        dmax = std::max(dmax, net->conns[ci]->max_delay);
+      }
+    // application order: Connections (connections slot) before DelayConnection
+    // flushes (after_connections), each in creation order.
+    std::stable_sort(incoming.begin(), incoming.end(), [&](int a, int b) {
+      return net->conns[a]->delay_kind < net->conns[b]->delay_kind;
+    });
+    g.D = W + dmax + 1;
+    // count channels packed per ring: 8-bit counts when every in-degree <= 255
+    std::vector<int> cnt_chans, flt_chans;
+    for (int ci : incoming) (net->conns[ci]->count_mode ? cnt_chans : flt_chans).push_back(ci);
+    bool small = true;
+    for (int ci : cnt_chans) small = small && net->conns[ci]->max_indeg <= 255;
+    const int bits = small ? 8 : 16;
+    const int per_ring = small ? 4 : 2;
+    g.rings.clear();
+    g.ring_desc.clear();
+    for (size_t a = 0; a < cnt_chans.size(); a += per_ring) {
+      const int nch = (int)std::min<size_t>(per_ring, cnt_chans.size() - a);
+      // nch must divide counts-per-word so each neuron's counts share one word
+      const int per_word = 32 / bits;
+      int nch_p = nch;
+      while (per_word % nch_p) ++nch_p;
+      RingDev rd{};
+      rd.kind = RING_COUNT;
+      rd.nch = nch_p;
+      rd.bits = bits;
+      rd.sw = nch_p * bits / 8;
+      rd.slot_bytes = (g.N * rd.sw + 255) / 256 * 256;
+      const int ridx = (int)g.rings.size();
+      g.rings.emplace_back();
+      g.rings.back().alloc((size_t)rd.slot_bytes * g.D);
+      BM_HIP(hipMemsetAsync(g.rings.back().p, 0, g.rings.back().bytes, net->stream));
+      rd.base = g.rings.back().p;
+      g.ring_desc.push_back(rd);
+      for (int q = 0; q < nch; ++q) {
+        HostConn &c = *net->conns[cnt_chans[a + q]];
+        c.ring = ridx;
+        c.ch = q;
+      }
+    }
+    for (int ci : flt_chans) {
+      RingDev rd{};
+      rd.kind = RING_FLOAT;
+      rd.nch = 1;
+      rd.bits = 0;
+      rd.sw = (int)net->tsize();
+      rd.slot_bytes = (g.N * rd.sw + 255) / 256 * 256;
+      const int ridx = (int)g.rings.size();
+      g.rings.emplace_back();
+      g.rings.back().alloc((size_t)rd.slot_bytes * g.D);
+      BM_HIP(hipMemsetAsync(g.rings.back().p, 0, g.rings.back().bytes, net->stream));
+      rd.base = g.rings.back().p;
+      g.ring_desc.push_back(rd);
+      net->conns[ci]->ring = ridx;
+      net->conns[ci]->ch = 0;
+    }
+    BM_REQUIRE(g.rings.size() <= (size_t)MAXRING, "too many incoming ring buffers for one group");
+    g.chans.clear();
+    for (int ci : incoming) {
+      HostConn &c = *net->conns[ci];
+      ChanDev cd{};
+      cd.ring = c.ring;
+      cd.ch = c.ch;
+      cd.var = c.var;
+      cd.is_delay = c.delay_kind == 1;
+      cd.w = c.wval;
+      g.chans.push_back(cd);
+    }
+    BM_REQUIRE(g.chans.size() <= (size_t)MAXCHAN, "too many incoming connections for one group");
+  }
+  // ---- STDP structures and histories
+  for (auto &sp : net->stdps) {
+    HostConn &c = *net->conns[sp->conn];
+    net->groups[c.src]->need_hist = true;
+    net->groups[c.tgt]->need_hist = true;
+    net->groups[c.src]->H = std::max(net->groups[c.src]->H, c.hom_delay + 2);
+    net->groups[c.tgt]->H = std::max(net->groups[c.tgt]->H, 2);
+    build_stdp(net, *sp);
+  }
+  // ---- group device data
+  std::vector<unsigned int *> hptrs;
+  std::vector<int> hH;
+  for (size_t gi = 0; gi < net->groups.size(); ++gi) {
+    HostGroup &g = *net->groups[gi];
+    GroupDev &d = g.dev;
+    std::memset(&d, 0, sizeof(d));
+    d.N = g.N;
+    d.M = g.M;
+    d.kind = g.kind;
+    d.upd = g.upd;
+    d.dt = net->dt;
+    if (g.M > 0) {
+      upload_T(net, g.S, g.S0.data(), g.S0.size());
+      d.S = g.S.p;
+    }
+    std::vector<int32_t> lastv(g.N, NEVER);
+    upload_raw(g.last, lastv.data(), lastv.size(), net->stream);
+    d.last = g.last.as<int32_t>();
+    for (int j = 0; j < g.M * g.M && j < (int)g.A.size(); ++j) d.A[j] = g.A[j];
+    for (int j = 0; j < g.M && j < (int)g.C.size(); ++j) d.C[j] = g.C[j];
+    if (g.upd == UPD_PERNEURON) {
+      upload_T(net, g.Apn_d, g.Apn.data(), g.Apn.size());
+      upload_T(net, g.Cpn_d, g.Cpn.data(), g.Cpn.size());
+      d.Apn = g.Apn_d.p;
+      d.Cpn = g.Cpn_d.p;
+    }
+    if (g.upd == UPD_PROGRAM) {
+      upload_raw(g.code_d, g.prog_code.data(), g.prog_code.size(), net->stream);
+      upload_raw(g.consts_d, g.prog_consts.data(), g.prog_consts.size(), net->stream);
+      d.prog.code = g.code_d.as<int32_t>();
+      d.prog.consts = g.consts_d.as<double>();
+      d.prog.ninstr = g.prog_ninstr;
+      d.prog.nregs = g.prog_nregs;
+      d.prog.scheme = g.prog_scheme;
+    }
+    d.th_kind = g.th_kind;
+    d.th_var = g.th_var;
+    d.th_vtvar = g.th_vtvar;
+    d.th_period = g.th_period;
+    d.vt = g.vt;
+    d.nreset = (int)g.rs_var.size();
+    for (int r = 0; r < d.nreset; ++r) {
+      d.rs_var[r] = g.rs_var[r];
+      d.rs_op[r] = g.rs_op[r];
+      d.rs_vvar[r] = g.rs_vvar[r];
+      d.rs_val[r] = g.rs_val[r];
+    }
+    d.refr = g.refr;
+    d.period = g.period;
+    d.refr_time = g.refr_time;
+    d.use_next_allowed = g.refr != REFR_NONE;
+    if (g.refr == REFR_ARRAY) {
+      upload_raw(g.refr_d, g.refr_arr.data(), g.refr_arr.size(), net->stream);
+      d.refr_arr = g.refr_d.as<double>();
+    }
+    d.D = g.D;
+    d.nring = (int)g.ring_desc.size();
+    for (int r = 0; r < d.nring; ++r) d.rings[r] = g.ring_desc[r];
+    d.nchan = (int)g.chans.size();
+    for (int c = 0; c < d.nchan; ++c) d.chans[c] = g.chans[c];
+    // outgoing
+    int nout = 0;
+    for (size_t ci = 0; ci < net->conns.size(); ++ci) {
+      HostConn &c = *net->conns[ci];
+      if (c.src != (int)gi) continue;
+      BM_REQUIRE(nout < MAXOUT, "too many outgoing connections for one group");
+      HostGroup &tg = *net->groups[c.tgt];
+      OutDev &o = d.outs[nout++];
+      o.src_off = c.src_off;
+      o.src_n = c.src_n;
+      o.row_ptr = c.row_ptr_d.as<int64_t>();
+      o.col = c.col_d.as<int32_t>();
+      o.w = c.w_d.p;
+      o.dly = c.dly_d.p;
+      o.dly_bytes = c.delay_kind == 1 ? c.dly_bytes : 0;
+      o.hom_delay = c.delay_kind == 0 ? c.hom_delay : 0;
+      const RingDev &rd = tg.ring_desc[c.ring];
+      o.ring = rd.base;
+      o.slot_bytes = rd.slot_bytes;
+      o.kind = rd.kind;
+      o.bits = rd.bits;
+      o.nch = rd.nch;
+      o.ch = c.ch;
+      o.D = tg.D;
+      o.tgt_off = c.tgt_off;
+    }
+    d.nout = nout;
+    if (g.raster_cap > 0) {
+      g.raster_d.alloc((size_t)g.raster_cap * 8);
+      g.raster_count_d.alloc(8);
+      BM_HIP(hipMemsetAsync(g.raster_count_d.p, 0, 8, net->stream));
+      d.raster = g.raster_d.as<unsigned long long>();
+      d.raster_count = g.raster_count_d.as<unsigned long long>();
+      d.raster_cap = g.raster_cap;
+    }
+    if (g.need_hist) {
+      g.hist_d.alloc((size_t)g.H * g.N * 4);
+      g.hist_count_d.alloc((size_t)g.H * 4);
+      BM_HIP(hipMemsetAsync(g.hist_count_d.p, 0, (size_t)g.H * 4, net->stream));
+      d.hist = g.hist_d.as<int32_t>();
+      d.hist_count = g.hist_count_d.as<unsigned int>();
+      d.H = g.H;
+      hptrs.push_back(d.hist_count);
+      hH.push_back(g.H);
+    }
+    if (g.kind == GK_POISSON) {
+      std::vector<uint64_t> thr(g.N);
+      for (int64_t i = 0; i < g.N; ++i) {
+        double t = std::floor(g.rates[i] * net->dt * 4294967296.0);
+        t = std::min(std::max(t, 0.0), 4294967295.0);
+        thr[i] = (uint64_t)t;
+      }
+      upload_raw(g.pthr_d, thr.data(), thr.size(), net->stream);
+      d.pthr = g.pthr_d.as<uint64_t>();
+      d.seed = g.seed;
+    }
+    if (g.ti_nt > 0) {
+      upload_T(net, g.ti_d, g.ti_vals.data(), g.ti_vals.size());
+      d.ti_vals = g.ti_d.p;
+      d.ti_nt = g.ti_nt;
+      d.ti_var = g.ti_var;
+      d.ti_per_neuron = g.ti_per_neuron;
+    } else {
+      d.ti_var = -1;
+    }
+    if (g.kind == GK_GENERATOR) {
+      int64_t nst = g.gen_steps.empty() ? 0 : (int64_t)g.gen_steps.back() + 1;
+      std::vector<int64_t> off(nst + 1, 0);
+      for (int32_t st : g.gen_steps) off[st + 1]++;
+      for (int64_t k = 0; k < nst; ++k) off[k + 1] += off[k];
+      upload_raw(g.gen_off_d, off.data(), off.size(), net->stream);
+      upload_raw(g.gen_idx_d, g.gen_idx.data(), g.gen_idx.size(), net->stream);
+      d.gen_idx = g.gen_idx_d.as<int32_t>();
+      d.gen_off = g.gen_off_d.as<int64_t>();
+      d.gen_nsteps = nst;
+      g.gen_nsteps = nst;
+    }
+  }
+  net->nhist = (int)hptrs.size();
+  if (net->nhist) {
+    upload_raw(net->hist_ptrs_d, hptrs.data(), hptrs.size(), net->stream);
+    upload_raw(net->hist_H_d, hH.data(), hH.size(), net->stream);
+  }
+  // STDP device descriptors (need group hist pointers)
+  for (auto &sp : net->stdps) {
+    HostStdp &s = *sp;
+    HostConn &c = *net->conns[s.conn];
+    StdpDev &d = s.dev;
+    std::memset(&d, 0, sizeof(d));
+    d.src_off = c.src_off;
+    d.src_n = c.src_n;
+    d.tgt_off = c.tgt_off;
+    d.tgt_n = c.tgt_n;
+    d.row_ptr = c.row_ptr_d.as<int64_t>();
+    d.col = c.col_d.as<int32_t>();
+    d.w = c.w_d.p;
+    d.col_ptr = s.col_ptr_d.as<int64_t>();
+    d.col_perm = s.col_perm_d.as<int64_t>();
+    d.col_row = s.col_row_d.as<int32_t>();
+    d.a_pre = s.a_pre_d.p;
+    d.a_post = s.a_post_d.p;
+    d.decay_pre = s.decay_pre;
+    d.decay_post = s.decay_post;
+    d.Ap = s.Ap;
+    d.Am = s.Am;
+    d.wmin = s.wmin;
+    d.wmax = s.wmax;
+    d.update = s.update;
+    d.interactions = s.interactions;
+    d.pre_delay = c.hom_delay;
+    HostGroup &sg = *net->groups[c.src];
+    HostGroup &tg = *net->groups[c.tgt];
+    d.pre_hist = sg.hist_d.as<int32_t>();
+    d.pre_hist_count = sg.hist_count_d.as<unsigned int>();
+    d.pre_H = sg.H;
+    d.pre_N = sg.N;
+    d.post_hist = tg.hist_d.as<int32_t>();
+    d.post_hist_count = tg.hist_count_d.as<unsigned int>();
+    d.post_H = tg.H;
+    d.post_N = tg.N;
+  }
+  net->d_base.alloc(8);
+  int64_t zero = 0;
+  BM_HIP(hipMemcpyAsync(net->d_base.p, &zero, 8, hipMemcpyHostToDevice, net->stream));
+  BM_HIP(hipStreamSynchronize(net->stream));
+  net->prepared = true;
+}
+
+void set_base(bm_net *net, int64_t v) {
+  BM_HIP(hipMemcpyAsync(net->d_base.p, &v, 8, hipMemcpyHostToDevice, net->stream));
+}
+
+void build_graph(bm_net *net, int windows) {
+  if (net->graph_exec && net->graph_windows == windows) return;
+  if (net->graph_exec) {
+    hipGraphExecDestroy(net->graph_exec);
+    net->graph_exec = nullptr;
+  }
+  hipGraph_t graph;
+  BM_HIP(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < windows; ++k) launch_window(net, (int64_t)k * net->W, net->W, 0);
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, net->stream, net->d_base.as<int64_t>(),
+                     (int64_t)windows * net->W);
+  hipError_t e = hipStreamEndCapture(net->stream, &graph);
+  BM_HIP(e);
+  e = hipGraphInstantiate(&net->graph_exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  BM_HIP(e);
+  net->graph_windows = windows;
+}
+
+void run(bm_net *net, int64_t nsteps) {
+  BM_REQUIRE(net->prepared, "bm_net_prepare must be called before bm_net_run");
+  BM_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
+  if (nsteps == 0) return;
+  const int W = net->W;
+  int64_t done = 0;
+  set_base(net, net->step);
+  auto prof_begin = [&]() {
+    if (!net->profiling) return;
+    hipEvent_t a, b;
+    BM_HIP(hipEventCreate(&a));
+    BM_HIP(hipEventCreate(&b));
+    net->ev.push_back(a);
+    net->ev.push_back(b);
+    BM_HIP(hipEventRecord(a, net->stream));
+  };
+  auto prof_end = [&]() {
+    if (!net->profiling) return;
+    BM_HIP(hipEventRecord(net->ev.back(), net->stream));
+  };
+  // first window: the previous run already applied the tail of its last step
+  {
+    const int w = (int)std::min<int64_t>(W, nsteps);
+    prof_begin();
+    launch_window(net, 0, w, 1);
+    prof_end();
+    check_launch();
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, net->stream, net->d_base.as<int64_t>(), (int64_t)w);
+    done += w;
+  }
+  const int64_t remaining_windows = (nsteps - done) / W;
+  int64_t wdone = 0;
+  if (net->use_graph && !net->profiling && remaining_windows >= 8) {
+    const int chunk = (int)std::min<int64_t>(remaining_windows, std::max<int64_t>(1, 512 / W));
+    build_graph(net, chunk);
+    while (remaining_windows - wdone >= chunk) {
+      BM_HIP(hipGraphLaunch(net->graph_exec, net->stream));
+      wdone += chunk;
+    }
+  }
+  for (; wdone < remaining_windows; ++wdone) {
+    prof_begin();
+    launch_window(net, 0, W, 0);
+    prof_end();
+    check_launch();
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, net->stream, net->d_base.as<int64_t>(), (int64_t)W);
+  }
+  done += remaining_windows * W;
+  if (done < nsteps) {
+    const int w = (int)(nsteps - done);
+    prof_begin();
+    launch_window(net, 0, w, 0);
+    prof_end();
+    check_launch();
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, net->stream, net->d_base.as<int64_t>(), (int64_t)w);
+    done += w;
+  }
+  // tail of the last step: deliveries and resets so the state matches Brian's
+  // state after run()
+  for (auto &gp : net->groups) launch_group(net, *gp, 0, 0, 0, 1);
+  check_launch();
+  net->step += nsteps;
+}
+
+template <typename T>
+void get_state_T(bm_net *net, HostGroup &g, int var, double *out) {
+  std::vector<T> tmp(g.N);
+  BM_HIP(hipMemcpyAsync(tmp.data(), static_cast<T *>(g.S.p) + (size_t)var * g.N, g.N * sizeof(T),
+                        hipMemcpyDeviceToHost, net->stream));
+  BM_HIP(hipStreamSynchronize(net->stream));
+  for (int64_t i = 0; i < g.N; ++i) out[i] = (double)tmp[i];
+}
+
+template <typename T>
+void set_state_T(bm_net *net, HostGroup &g, int var, const double *in) {
+  std::vector<T> tmp(g.N);
+  for (int64_t i = 0; i < g.N; ++i) tmp[i] = (T)in[i];
+  BM_HIP(hipMemcpyAsync(static_cast<T *>(g.S.p) + (size_t)var * g.N, tmp.data(), g.N * sizeof(T),
+                        hipMemcpyHostToDevice, net->stream));
+  BM_HIP(hipStreamSynchronize(net->stream));
+}
+
+}  // namespace
+
+// ====================================================================== C-ABI
+extern "C" {
+
+const char *bm_last_error(void) { return g_last_error.c_str(); }
+int bm_version(void) { return 1; }
+
+int bm_net_create(double dt, int precision, bm_net **out) {
+  return guard([&] {
+    BM_REQUIRE(out != nullptr, "out is null");
+    BM_REQUIRE(dt > 0, "dt must be > 0");
+    BM_REQUIRE(precision == 32 || precision == 64, "precision must be 32 or 64");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(BM_ENODEV, "no HIP device");
+    auto net = std::make_unique<bm_net>();
+    net->dt = dt;
+    net->precision = precision;
+    BM_HIP(hipGetDevice(&net->device));
+    BM_HIP(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking));
+    *out = net.release();
+  });
+}
+
+int bm_net_destroy(bm_net *net) {
+  return guard([&] {
+    if (!net) return;
+    if (net->stream) hipStreamSynchronize(net->stream);
+    delete net;
+  });
+}
+
+int bm_net_add_group(bm_net *net, int64_t N, int nvars, const double *S0, int *gid) {
+  return guard([&] {
+    require_unprepared(net);
+    BM_REQUIRE(N >= 0 && N < (int64_t(1) << 31), "N out of range");
+    BM_REQUIRE(nvars >= 0 && nvars <= MAXM, "nvars out of range [0,8]");
+    auto g = std::make_unique<HostGroup>();
+    g->N = N;
+    g->M = nvars;
+    g->S0.assign(S0 ? S0 : nullptr, S0 ? S0 + (size_t)N * nvars : nullptr);
+    if (!S0) g->S0.assign((size_t)N * nvars, 0.0);
+    net->groups.push_back(std::move(g));
+    *gid = (int)net->groups.size() - 1;
+  });
+}
+
+int bm_net_set_linear(bm_net *net, int gid, const double *A, const double *C) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    g.upd = UPD_LINEAR;
+    g.A.assign(A, A + g.M * g.M);
+    g.C.assign(C, C + g.M);
+  });
+}
+
+int bm_net_set_linear_per_neuron(bm_net *net, int gid, const double *A, const double *C) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    g.upd = UPD_PERNEURON;
+    g.Apn.assign(A, A + (size_t)g.M * g.M * g.N);
+    g.Cpn.assign(C, C + (size_t)g.M * g.N);
+  });
+}
+
+int bm_net_set_program(bm_net *net, int gid, int scheme, int nregs, int ninstr, const int32_t *code,
+                       int nconst, const double *consts) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(scheme >= 1 && scheme <= 3, "scheme must be 1 (Euler), 2 (RK2) or 3 (exp-Euler)");
+    BM_REQUIRE(nregs >= g.M && nregs <= 64, "nregs out of range");
+    g.upd = UPD_PROGRAM;
+    g.prog_scheme = scheme;
+    g.prog_nregs = nregs;
+    g.prog_ninstr = ninstr;
+    g.prog_code.assign(code, code + g.M + 4 * (size_t)ninstr);
+    for (int q = 0; q < ninstr; ++q) {
+      const int32_t *ins = code + g.M + 4 * q;
+      BM_REQUIRE(ins[1] >= 0 && ins[1] < nregs && ins[2] >= 0 && ins[3] >= 0,
+                 "program register index out of range");
+      BM_REQUIRE(ins[0] == OP_CONST ? ins[2] < nconst : (ins[2] < nregs && ins[3] < nregs),
+                 "program operand out of range");
+    }
+    for (int j = 0; j < g.M; ++j) BM_REQUIRE(code[j] < nregs, "output register out of range");
+    g.prog_consts.assign(consts, consts + nconst);
+    if (g.prog_consts.empty()) g.prog_consts.push_back(0.0);
+  });
+}
+
+int bm_net_set_threshold(bm_net *net, int gid, int var, double vt, int vt_var) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(var >= 0 && var < g.M, "threshold variable out of range");
+    BM_REQUIRE(vt_var < g.M, "threshold vt variable out of range");
+    g.th_kind = vt_var >= 0 ? TH_VAR : TH_CONST;
+    g.th_var = var;
+    g.th_vtvar = vt_var;
+    g.vt = vt;
+  });
+}
+
+int bm_net_add_reset_op(bm_net *net, int gid, int var, int op, double value, int value_var) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(var >= 0 && var < g.M, "reset variable out of range");
+    BM_REQUIRE(op == 0 || op == 1, "reset op must be 0 (set) or 1 (add)");
+    BM_REQUIRE(value_var < g.M, "reset value variable out of range");
+    BM_REQUIRE(g.rs_var.size() < (size_t)MAXRESET, "too many reset ops");
+    g.rs_var.push_back(var);
+    g.rs_op.push_back(op);
+    g.rs_val.push_back(value);
+    g.rs_vvar.push_back(value_var);
+  });
+}
+
+int bm_net_set_refractory(bm_net *net, int gid, int period_bins, double refractory_time,
+                          const double *per_neuron) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(period_bins >= 1, "period_bins must be >= 1");
+    g.refr = per_neuron ? REFR_ARRAY : REFR_CONST;
+    g.period = period_bins;
+    g.refr_time = refractory_time;
+    if (per_neuron) g.refr_arr.assign(per_neuron, per_neuron + g.N);
+  });
+}
+
+int bm_net_set_empirical_threshold(bm_net *net, int gid, int var, double vt, int period) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(var >= 0 && var < g.M, "threshold variable out of range");
+    g.th_kind = TH_EMPIRICAL;
+    g.th_var = var;
+    g.vt = vt;
+    g.th_period = period;
+  });
+}
+
+int bm_net_set_timed_input(bm_net *net, int gid, int var, int64_t nt, int per_neuron,
+                           const double *values) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(var >= 0 && var < g.M, "input variable out of range");
+    BM_REQUIRE(nt > 0, "nt must be > 0");
+    g.ti_var = var;
+    g.ti_nt = nt;
+    g.ti_per_neuron = per_neuron ? 1 : 0;
+    g.ti_vals.assign(values, values + (size_t)nt * (per_neuron ? g.N : 1));
+  });
+}
+
+int bm_net_add_generator(bm_net *net, int64_t N, int64_t nspikes, const int32_t *steps,
+                         const int32_t *idx, int *gid) {
+  return guard([&] {
+    require_unprepared(net);
+    BM_REQUIRE(N > 0 && N < (int64_t(1) << 31), "N out of range");
+    auto g = std::make_unique<HostGroup>();
+    g->N = N;
+    g->kind = GK_GENERATOR;
+    g->gen_steps.assign(steps, steps + nspikes);
+    g->gen_idx.assign(idx, idx + nspikes);
+    for (int64_t q = 0; q < nspikes; ++q) {
+      BM_REQUIRE(idx[q] >= 0 && idx[q] < N, "generator index out of range");
+      BM_REQUIRE(steps[q] >= 0, "generator step must be >= 0");
+      if (q) BM_REQUIRE(steps[q] > steps[q - 1] || (steps[q] == steps[q - 1] && idx[q] > idx[q - 1]),
+                        "generator spikes must be sorted by (step, index) without duplicates");
+    }
+    net->groups.push_back(std::move(g));
+    *gid = (int)net->groups.size() - 1;
+  });
+}
+
+int bm_net_add_poisson(bm_net *net, int64_t N, const double *rates, uint64_t seed, int *gid) {
+  return guard([&] {
+    require_unprepared(net);
+    BM_REQUIRE(N > 0 && N < (int64_t(1) << 31), "N out of range");
+    auto g = std::make_unique<HostGroup>();
+    g->N = N;
+    g->kind = GK_POISSON;
+    g->rates.assign(rates, rates + N);
+    g->seed = seed;
+    net->groups.push_back(std::move(g));
+    *gid = (int)net->groups.size() - 1;
+  });
+}
+
+int bm_net_add_connection(bm_net *net, int src_gid, int64_t src_off, int64_t src_n, int tgt_gid,
+                          int64_t tgt_off, int64_t tgt_n, int tgt_var, int64_t nnz,
+                          const int64_t *row_ptr, const int32_t *col, const double *w, int delay_kind,
+                          int hom_delay_steps, const int32_t *syn_delay, int max_delay_steps, int *cid) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &sg = grp(net, src_gid);
+    HostGroup &tg = grp(net, tgt_gid);
+    BM_REQUIRE(src_off >= 0 && src_n >= 0 && src_off + src_n <= sg.N, "source range out of bounds");
+    BM_REQUIRE(tgt_off >= 0 && tgt_n >= 0 && tgt_off + tgt_n <= tg.N, "target range out of bounds");
+    BM_REQUIRE(tg.kind == GK_NEURON && tgt_var >= 0 && tgt_var < tg.M, "target variable out of range");
+    BM_REQUIRE(delay_kind == 0 || delay_kind == 1, "delay_kind must be 0 or 1");
+    BM_REQUIRE(row_ptr[0] == 0 && row_ptr[src_n] == nnz, "row_ptr inconsistent with nnz");
+    auto c = std::make_unique<HostConn>();
+    c->src = src_gid;
+    c->tgt = tgt_gid;
+    c->var = tgt_var;
+    c->src_off = src_off;
+    c->src_n = src_n;
+    c->tgt_off = tgt_off;
+    c->tgt_n = tgt_n;
+    c->nnz = nnz;
+    c->row_ptr.assign(row_ptr, row_ptr + src_n + 1);
+    for (int64_t i = 0; i < src_n; ++i) BM_REQUIRE(row_ptr[i + 1] >= row_ptr[i], "row_ptr not monotone");
+    c->col.assign(col, col + nnz);
+    for (int64_t q = 0; q < nnz; ++q) BM_REQUIRE(col[q] >= 0 && col[q] < tgt_n, "column index out of range");
+    c->w.assign(w, w + nnz);
+    c->delay_kind = delay_kind;
+    if (delay_kind == 0) {
+      BM_REQUIRE(hom_delay_steps >= 0 && hom_delay_steps < 65536, "delay out of range");
+      c->hom_delay = hom_delay_steps;
+      c->max_delay = c->min_delay = hom_delay_steps;
+    } else {
+      BM_REQUIRE(max_delay_steps >= 0 && max_delay_steps < 65536, "max_delay out of range");
+      c->syn_delay.assign(syn_delay, syn_delay + nnz);
+      int mn = max_delay_steps, mx = 0;
+      for (int64_t q = 0; q < nnz; ++q) {
+        BM_REQUIRE(syn_delay[q] >= 0 && syn_delay[q] <= max_delay_steps,
+                   "synaptic delay outside [0, max_delay]");
+        mn = std::min(mn, syn_delay[q]);
+        mx = std::max(mx, syn_delay[q]);
+      }
+      c->max_delay = max_delay_steps;
+      c->min_delay = nnz ? mn : max_delay_steps;
+      (void)mx;
+    }
+    net->conns.push_back(std::move(c));
+    *cid = (int)net->conns.size() - 1;
+  });
+}
+
+int bm_net_add_stdp(bm_net *net, int cid, double decay_pre, double decay_post, double Ap, double Am,
+                    double wmin, double wmax, int update, int interactions, int *sid) {
+  return guard([&] {
+    require_unprepared(net);
+    BM_REQUIRE(cid >= 0 && cid < (int)net->conns.size(), "bad connection id");
+    HostConn &c = *net->conns[cid];
+    BM_REQUIRE(c.delay_kind == 0, "STDP on DelayConnection is not supported");
+    BM_REQUIRE(c.stdp < 0, "connection already has STDP");
+    BM_REQUIRE(update == 0 || update == 1, "update must be 0 (additive) or 1 (multiplicative)");
+    BM_REQUIRE(interactions >= 0 && interactions <= 3, "interactions out of range");
+    auto s = std::make_unique<HostStdp>();
+    s->conn = cid;
+    s->decay_pre = decay_pre;
+    s->decay_post = decay_post;
+    s->Ap = Ap;
+    s->Am = Am;
+    s->wmin = wmin;
+    s->wmax = wmax;
+    s->update = update;
+    s->interactions = interactions;
+    net->stdps.push_back(std::move(s));
+    c.stdp = (int)net->stdps.size() - 1;
+    *sid = c.stdp;
+  });
+}
+
+int bm_net_record(bm_net *net, int gid, int64_t capacity) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(capacity > 0, "capacity must be > 0");
+    g.raster_cap = capacity;
+  });
+}
+
+int bm_net_prepare(bm_net *net, int window, int neurons_per_thread, int use_graph) {
+  return guard([&] { prepare(net, window, neurons_per_thread, use_graph); });
+}
+
+int bm_net_run(bm_net *net, int64_t nsteps) {
+  return guard([&] { run(net, nsteps); });
+}
+
+int bm_net_sync(bm_net *net) {
+  return guard([&] {
+    BM_HIP(hipStreamSynchronize(net->stream));
+    if (net->profiling) {
+      for (size_t q = 0; q + 1 < net->ev.size(); q += 2) {
+        float ms = 0;
+        BM_HIP(hipEventElapsedTime(&ms, net->ev[q], net->ev[q + 1]));
+        net->prof_ms += ms;
+        net->prof_launches++;
+      }
+      for (auto e : net->ev) hipEventDestroy(e);
+      net->ev.clear();
+    }
+  });
+}
+
+int bm_net_current_step(bm_net *net, int64_t *step) {
+  return guard([&] { *step = net->step; });
+}
+
+int bm_net_get_state(bm_net *net, int gid, int var, double *out) {
+  return guard([&] {
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(var >= 0 && var < g.M, "variable out of range");
+    if (!net->prepared) {
+      std::memcpy(out, g.S0.data() + (size_t)var * g.N, g.N * sizeof(double));
+      return;
+    }
+    if (net->precision == 64) get_state_T<double>(net, g, var, out);
+    else get_state_T<float>(net, g, var, out);
+  });
+}
+
+int bm_net_set_state(bm_net *net, int gid, int var, const double *in) {
+  return guard([&] {
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(var >= 0 && var < g.M, "variable out of range");
+    if (!net->prepared) {
+      std::memcpy(g.S0.data() + (size_t)var * g.N, in, g.N * sizeof(double));
+      return;
+    }
+    if (net->precision == 64) set_state_T<double>(net, g, var, in);
+    else set_state_T<float>(net, g, var, in);
+  });
+}
+
+int bm_net_spike_count(bm_net *net, int gid, int64_t *count) {
+  return guard([&] {
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.raster_cap > 0 && net->prepared, "group is not recorded or network not prepared");
+    unsigned long long c = 0;
+    BM_HIP(hipMemcpyAsync(&c, g.raster_count_d.p, 8, hipMemcpyDeviceToHost, net->stream));
+    BM_HIP(hipStreamSynchronize(net->stream));
+    *count = (int64_t)c;
+  });
+}
+
+int bm_net_get_spikes(bm_net *net, int gid, int32_t *steps, int32_t *idx, int64_t cap, int64_t *n_out) {
+  return guard([&] {
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.raster_cap > 0 && net->prepared, "group is not recorded or network not prepared");
+    unsigned long long c = 0;
+    BM_HIP(hipMemcpyAsync(&c, g.raster_count_d.p, 8, hipMemcpyDeviceToHost, net->stream));
+    BM_HIP(hipStreamSynchronize(net->stream));
+    if ((int64_t)c > g.raster_cap)
+      throw Error(BM_EOVERFLOW, "spike raster overflow: " + std::to_string(c) + " spikes > capacity " +
+                                    std::to_string(g.raster_cap));
+    BM_REQUIRE((int64_t)c <= cap, "output buffer too small");
+    std::vector<uint64_t> tmp(c);
+    if (c)
+      BM_HIP(hipMemcpyAsync(tmp.data(), g.raster_d.p, c * 8, hipMemcpyDeviceToHost, net->stream));
+    BM_HIP(hipStreamSynchronize(net->stream));
+    std::sort(tmp.begin(), tmp.end());
+    for (size_t q = 0; q < tmp.size(); ++q) {
+      steps[q] = (int32_t)(tmp[q] >> 32);
+      idx[q] = (int32_t)(tmp[q] & 0xFFFFFFFFu);
+    }
+    *n_out = (int64_t)c;
+  });
+}
+
+int bm_net_clear_spikes(bm_net *net, int gid) {
+  return guard([&] {
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.raster_cap > 0 && net->prepared, "group is not recorded or network not prepared");
+    BM_HIP(hipMemsetAsync(g.raster_count_d.p, 0, 8, net->stream));
+    BM_HIP(hipStreamSynchronize(net->stream));
+  });
+}
+
+int bm_net_get_weights(bm_net *net, int cid, double *w) {
+  return guard([&] {
+    BM_REQUIRE(cid >= 0 && cid < (int)net->conns.size(), "bad connection id");
+    HostConn &c = *net->conns[cid];
+    if (!net->prepared || c.count_mode) {
+      std::memcpy(w, c.w.data(), c.nnz * sizeof(double));
+      return;
+    }
+    if (net->precision == 64) {
+      BM_HIP(hipMemcpyAsync(w, c.w_d.p, c.nnz * 8, hipMemcpyDeviceToHost, net->stream));
+      BM_HIP(hipStreamSynchronize(net->stream));
+    } else {
+      std::vector<float> tmp(c.nnz);
+      BM_HIP(hipMemcpyAsync(tmp.data(), c.w_d.p, c.nnz * 4, hipMemcpyDeviceToHost, net->stream));
+      BM_HIP(hipStreamSynchronize(net->stream));
+      for (int64_t q = 0; q < c.nnz; ++q) w[q] = tmp[q];
+    }
+  });
+}
+
+int bm_net_get_stdp_traces(bm_net *net, int sid, double *a_pre, double *a_post) {
+  return guard([&] {
+    BM_REQUIRE(sid >= 0 && sid < (int)net->stdps.size(), "bad stdp id");
+    BM_REQUIRE(net->prepared, "network not prepared");
+    HostStdp &s = *net->stdps[sid];
+    const int64_t n1 = s.dev.src_n, n2 = s.dev.tgt_n;
+    if (net->precision == 64) {
+      BM_HIP(hipMemcpyAsync(a_pre, s.a_pre_d.p, n1 * 8, hipMemcpyDeviceToHost, net->stream));
+      BM_HIP(hipMemcpyAsync(a_post, s.a_post_d.p, n2 * 8, hipMemcpyDeviceToHost, net->stream));
+      BM_HIP(hipStreamSynchronize(net->stream));
+    } else {
+      std::vector<float> t1(n1), t2(n2);
+      BM_HIP(hipMemcpyAsync(t1.data(), s.a_pre_d.p, n1 * 4, hipMemcpyDeviceToHost, net->stream));
+      BM_HIP(hipMemcpyAsync(t2.data(), s.a_post_d.p, n2 * 4, hipMemcpyDeviceToHost, net->stream));
+      BM_HIP(hipStreamSynchronize(net->stream));
+      for (int64_t i = 0; i < n1; ++i) a_pre[i] = t1[i];
+      for (int64_t i = 0; i < n2; ++i) a_post[i] = t2[i];
+    }
+  });
+}
+
+int bm_net_set_profiling(bm_net *net, int enable) {
+  return guard([&] {
+    net->profiling = enable != 0;
+    net->prof_launches = 0;
+    net->prof_ms = 0;
+  });
+}
+
+int bm_net_get_profile(bm_net *net, int64_t *launches, double *total_ms, double *window_steps) {
+  return guard([&] {
+    *launches = net->prof_launches;
+    *total_ms = net->prof_ms;
+    *window_steps = net->W;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,863 @@
+// Device side of the network engine: the fused multi-step sweep kernel
+// (state update, delayed-input consumption, threshold, refractoriness, reset,
+// wavefront-ballot spike compaction, raster append and CSR propagation into
+// the target rings), the spike-generator kernel and the STDP kernels.
+//
+// Step semantics follow brian/network.py:365-380: within step n the groups
+// run (state update -> threshold -> refractory filter -> push), then the
+// connections (delivery of spikes from step n-d), then the DelayConnection
+// flush (after_connections), then the resets.  The engine keeps that order
+// but cuts it differently: kernel step n runs the *tail* of step n-1
+// (deliveries due at step n-1, then resets) followed by the *head* of step n
+// (state update, threshold).  Every arithmetic operation on a neuron happens
+// in the same order as in the reference, only in a later kernel.
+//
+// Deliveries travel through a ring of D slots per target group: spikes of
+// step s with delay d are added into slot (s+d) % D during the kernel that
+// produced them; slot k is consumed (and zeroed) by the tail of step k.  With
+// every synaptic delay >= d_min steps, a launch may advance W <= d_min+1 steps
+// without any inter-workgroup synchronisation: nothing it produces is
+// consumed before the next launch.
+#pragma once
+#include "common.hpp"
+
+namespace bm {
+
+constexpr int MAXM = 8;
+constexpr int MAXRING = 4;
+constexpr int MAXCHAN = 8;
+constexpr int MAXOUT = 8;
+constexpr int MAXRESET = 4;
+constexpr int BLOCK = 256;
+constexpr int WMAX = 16;
+constexpr int32_t NEVER = INT32_MIN;
+
+enum RingKind { RING_COUNT = 0, RING_FLOAT = 1 };
+enum UpdKind { UPD_NONE = 0, UPD_LINEAR = 1, UPD_PERNEURON = 2, UPD_PROGRAM = 3 };
+enum GroupKind { GK_NEURON = 0, GK_POISSON = 1, GK_GENERATOR = 2 };
+enum ThKind { TH_NONE = 0, TH_CONST = 1, TH_VAR = 2, TH_EMPIRICAL = 3 };
+enum RefrKind { REFR_NONE = 0, REFR_CONST = 1, REFR_ARRAY = 2 };
+enum Op {
+  OP_MOV = 0, OP_CONST, OP_ADD, OP_SUB, OP_MUL, OP_DIV, OP_NEG, OP_EXP, OP_LOG,
+  OP_SQRT, OP_POW, OP_ABS, OP_TIME, OP_SIN, OP_COS, OP_TANH, OP_CLIPLO, OP_CLIPHI,
+  OP_GT, OP_LT, OP_WHERE
+};
+
+struct RingDev {
+  void *base;           // D slots of slot_bytes each
+  int64_t slot_bytes;   // bytes per slot (multiple of 256)
+  int kind;             // RingKind
+  int nch;              // COUNT: channels packed per neuron
+  int bits;             // COUNT: bits per count (8 or 16)
+  int sw;               // bytes per neuron in a slot (1, 2, 4 for COUNT; sizeof(T) FLOAT)
+};
+
+struct ChanDev {        // one incoming connection, applied in reference order
+  int ring;
+  int ch;
+  int var;
+  int is_delay;         // DelayConnection: sum into the slot, then one add
+  double w;             // COUNT weight
+};
+
+struct OutDev {         // one outgoing connection of this group
+  int64_t src_off, src_n;
+  const int64_t *row_ptr;
+  const int32_t *col;
+  const void *w;        // T[nnz] (FLOAT rings)
+  const void *dly;      // per-synapse delay steps (u8 / u16) or null
+  int dly_bytes;        // 0 homogeneous, 1 u8, 2 u16
+  int hom_delay;
+  void *ring;
+  int64_t slot_bytes;
+  int kind, bits, nch, ch, D, pad;
+  int64_t tgt_off;
+};
+
+struct ProgDev {
+  const int32_t *code;  // [M dreg][ninstr x 4]
+  const double *consts;
+  int ninstr, nregs, scheme, pad;
+};
+
+struct GroupDev {
+  int64_t N;
+  int M, kind, upd, pad0;
+  void *S;               // [M][N] T
+  int32_t *last;         // [N] last spike step (NEVER if none)
+  double A[MAXM * MAXM];
+  double C[MAXM];
+  const void *Apn;       // [M][M][N] T
+  const void *Cpn;       // [M][N] T
+  ProgDev prog;
+  int th_kind, th_var, th_vtvar, th_period;
+  double vt;
+  int nreset, pad1;
+  int rs_var[MAXRESET], rs_op[MAXRESET], rs_vvar[MAXRESET];
+  double rs_val[MAXRESET];
+  int refr, period;
+  double refr_time;
+  const double *refr_arr;
+  int use_next_allowed, D;
+  int nring, nchan;
+  RingDev rings[MAXRING];
+  ChanDev chans[MAXCHAN];
+  int nout, H;
+  OutDev outs[MAXOUT];
+  unsigned long long *raster;
+  unsigned long long *raster_count;
+  int64_t raster_cap;
+  int32_t *hist;              // [H][N] spike lists (STDP), null if unused
+  unsigned int *hist_count;   // [H]
+  const uint64_t *pthr;       // Poisson thresholds [N]
+  uint64_t seed;
+  int ti_var, ti_per_neuron;
+  int64_t ti_nt;
+  const void *ti_vals;        // T[nt] or T[nt][N]
+  double dt;
+  // generator
+  const int32_t *gen_idx;
+  const int64_t *gen_off;     // [gen_nsteps+1]
+  int64_t gen_nsteps;
+};
+
+// ------------------------------------------------------------------ helpers
+__device__ inline int lane_id() { return __lane_id(); }
+
+template <typename T>
+__device__ inline T dev_exp(T x) { return exp(x); }
+template <typename T>
+__device__ inline T dev_log(T x) { return log(x); }
+
+// Linear update S <- A.S + C.  The FP64 check mode evaluates each row as the
+// left fold A[j,0]*S0 + A[j,1]*S1 + ... then + C[j] with no contraction
+// (brian/stateupdater.py:413-416 fixed to that order); FP32 lets the
+// compiler contract.
+template <int M, int K>
+__device__ inline void linear_exact(double (&s)[M][K], const double *a, const double *c) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double tmp[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      double r = a[j * M] * s[0][k];
+#pragma unroll
+      for (int q = 1; q < M; ++q) r = r + a[j * M + q] * s[q][k];
+      tmp[j] = r + c[j];
+    }
+#pragma unroll
+    for (int j = 0; j < M; ++j) s[j][k] = tmp[j];
+  }
+}
+template <int M, int K>
+__device__ inline void linear_exact(float (&s)[M][K], const float *a, const float *c) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    float tmp[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      float r = a[j * M] * s[0][k];
+#pragma unroll
+      for (int q = 1; q < M; ++q) r = fmaf(a[j * M + q], s[q][k], r);
+      tmp[j] = r + c[j];
+    }
+#pragma unroll
+    for (int j = 0; j < M; ++j) s[j][k] = tmp[j];
+  }
+}
+
+// Interpreter for the nonlinear right-hand sides.  Registers live in LDS,
+// one column per thread; the instruction stream is wavefront-uniform.
+template <typename T>
+__device__ inline void run_program(const ProgDev &p, T *reg, int tid, double t) {
+#pragma clang fp contract(off)
+  const int32_t *ins = p.code;
+  for (int q = 0; q < p.ninstr; ++q) {
+    const int op = ins[4 * q], d = ins[4 * q + 1], a = ins[4 * q + 2], b = ins[4 * q + 3];
+    T *rd = reg + d * BLOCK + tid;
+    const T ra = reg[a * BLOCK + tid];
+    const T rb = reg[b * BLOCK + tid];
+    switch (op) {
+      case OP_MOV: *rd = ra; break;
+      case OP_CONST: *rd = (T)p.consts[a]; break;
+      case OP_ADD: *rd = ra + rb; break;
+      case OP_SUB: *rd = ra - rb; break;
+      case OP_MUL: *rd = ra * rb; break;
+      case OP_DIV: *rd = ra / rb; break;
+      case OP_NEG: *rd = -ra; break;
+      case OP_EXP: *rd = exp(ra); break;
+      case OP_LOG: *rd = log(ra); break;
+      case OP_SQRT: *rd = sqrt(ra); break;
+      case OP_POW: *rd = pow(ra, rb); break;
+      case OP_ABS: *rd = fabs(ra); break;
+      case OP_TIME: *rd = (T)t; break;
+      case OP_SIN: *rd = sin(ra); break;
+      case OP_COS: *rd = cos(ra); break;
+      case OP_TANH: *rd = tanh(ra); break;
+      case OP_CLIPLO: *rd = ra < rb ? rb : ra; break;
+      case OP_CLIPHI: *rd = ra > rb ? rb : ra; break;
+      case OP_GT: *rd = ra > rb ? (T)1 : (T)0; break;
+      case OP_LT: *rd = ra < rb ? (T)1 : (T)0; break;
+      default: break;
+    }
+  }
+}
+
+// One nonlinear step (brian/equations.py:655-670 Euler, :714-739 RK2,
+// :741-789 exponential Euler) for K=1 neuron per thread.
+template <typename T, int M>
+__device__ inline void program_step(const GroupDev &g, T (&s)[M][1], T *reg, int tid, double t) {
+#pragma clang fp contract(off)
+  const ProgDev &p = g.prog;
+  const int32_t *dreg = p.code;          // M output registers (-1 = parameter)
+  ProgDev body = p;
+  body.code = p.code + M;
+  const T dt = (T)g.dt;
+  if (p.scheme == 1 || p.scheme == 2) {
+#pragma unroll
+    for (int j = 0; j < M; ++j) reg[j * BLOCK + tid] = s[j][0];
+    run_program<T>(body, reg, tid, t);
+    T d1[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) d1[j] = dreg[j] >= 0 ? reg[dreg[j] * BLOCK + tid] : (T)0;
+    if (p.scheme == 1) {
+#pragma unroll
+      for (int j = 0; j < M; ++j)
+        if (dreg[j] >= 0) s[j][0] = s[j][0] + dt * d1[j];
+      return;
+    }
+    const T half = (T)0.5 * dt;
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+      reg[j * BLOCK + tid] = dreg[j] >= 0 ? s[j][0] + half * d1[j] : s[j][0];
+    run_program<T>(body, reg, tid, t);
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+      if (dreg[j] >= 0) s[j][0] = s[j][0] + dt * reg[dreg[j] * BLOCK + tid];
+    return;
+  }
+  // exponential Euler: per variable the affine coefficients with all other
+  // variables held at their current value, then S = (S+B)*exp(A dt) - B.
+  T Acoef[M], Bcoef[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    Acoef[j] = (T)0;
+    Bcoef[j] = (T)0;
+    if (dreg[j] < 0) continue;
+#pragma unroll
+    for (int q = 0; q < M; ++q) reg[q * BLOCK + tid] = (q == j) ? (T)0 : s[q][0];
+    run_program<T>(body, reg, tid, t);
+    T b = reg[dreg[j] * BLOCK + tid];
+#pragma unroll
+    for (int q = 0; q < M; ++q) reg[q * BLOCK + tid] = (q == j) ? (T)1 : s[q][0];
+    run_program<T>(body, reg, tid, t);
+    T a = reg[dreg[j] * BLOCK + tid] - b;
+    Acoef[j] = a;
+    Bcoef[j] = b / a;
+  }
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    if (dreg[j] < 0) continue;
+    T x = s[j][0] + Bcoef[j];
+    x = x * exp(Acoef[j] * dt);
+    s[j][0] = x - Bcoef[j];
+  }
+}
+
+template <typename T>
+__device__ inline T load_T(const void *p, int64_t i) { return static_cast<const T *>(p)[i]; }
+
+// ------------------------------------------------------------------ sweep
+// Compile-time feature set of a sweep instantiation.  F_GEN is the general
+// kernel (any number of count/float rings, per-neuron propagators, nonlinear
+// programs, refractoriness, timed input, Poisson sources); F = 0 is the lean
+// kernel for the common case (shared linear propagator, one packed count
+// ring, plain reset), which prefetches the ring slices of up to PF steps at
+// once so a window pays the HBM latency once per PF steps.
+enum Feat { F_GEN = 1 };
+
+__device__ inline uint32_t load_slice(const unsigned char *slot, int i, int sw) {
+  if (sw == 2) return reinterpret_cast<const uint16_t *>(slot)[i];
+  if (sw == 1) return slot[i];
+  return reinterpret_cast<const uint32_t *>(slot)[i];
+}
+__device__ inline void zero_slice(unsigned char *slot, int i, int sw) {
+  if (sw == 2) reinterpret_cast<uint16_t *>(slot)[i] = 0;
+  else if (sw == 1) slot[i] = 0;
+  else reinterpret_cast<uint32_t *>(slot)[i] = 0u;
+}
+
+// Applies `cnt` deliveries of weight w to x in the reference order:
+// Connection adds each weight to the target (connection.py:240-241);
+// DelayConnection sums into its ring slot, then adds once (delayconnection.py:112-119).
+template <typename T>
+__device__ inline T apply_count(T x, uint32_t cnt, T w, int is_delay) {
+#pragma clang fp contract(off)
+  if (is_delay) {
+    T acc = (T)0;
+    for (uint32_t q = 0; q < cnt; ++q) acc = acc + w;
+    return x + acc;
+  }
+  for (uint32_t q = 0; q < cnt; ++q) x = x + w;
+  return x;
+}
+
+template <typename T, int M, int K, int F>
+__global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t *d_base,
+                                                 int64_t rel0, int W, int skip_first_tail,
+                                                 int tail_only) {
+  constexpr bool GEN = (F & F_GEN) != 0;
+  constexpr int PF = GEN ? 1 : 8;                 // ring prefetch depth (steps)
+  constexpr int MM = M > 0 ? M : 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint16_t *sh_list = reinterpret_cast<uint16_t *>(smem);   // (w << 12) | local index
+  __shared__ int sh_n;
+  __shared__ unsigned long long sh_base;
+  __shared__ unsigned int sh_hbase;
+
+  const int tid = threadIdx.x;
+  const int64_t n0 = *d_base + rel0;
+  const int N = (int)g.N;
+  const int blk0 = blockIdx.x * (BLOCK * K);
+  const int D = g.D;
+
+  T s[MM][K];
+  int32_t last[K];
+  bool valid[K];
+  T a_sh[MM * MM], c_sh[MM];
+#pragma unroll
+  for (int j = 0; j < MM * MM; ++j) a_sh[j] = (T)g.A[j];
+#pragma unroll
+  for (int j = 0; j < MM; ++j) c_sh[j] = (T)g.C[j];
+  T apn[GEN ? MM * MM : 1][K];
+  T cpn[GEN ? MM : 1][K];
+
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = blk0 + k * BLOCK + tid;
+    valid[k] = i < N;
+    last[k] = NEVER;
+#pragma unroll
+    for (int j = 0; j < MM; ++j) s[j][k] = (T)0;
+    if (valid[k]) {
+      last[k] = g.last[i];
+      if constexpr (M > 0) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) s[j][k] = static_cast<const T *>(g.S)[(int64_t)j * N + i];
+      }
+      if constexpr (GEN) {
+        if (g.upd == UPD_PERNEURON) {
+#pragma unroll
+          for (int j = 0; j < MM * MM; ++j) apn[j][k] = load_T<T>(g.Apn, (int64_t)j * N + i);
+#pragma unroll
+          for (int j = 0; j < MM; ++j) cpn[j][k] = load_T<T>(g.Cpn, (int64_t)j * N + i);
+        }
+      }
+    }
+  }
+  if (tid == 0) sh_n = 0;
+  __syncthreads();
+
+  T *reg = reinterpret_cast<T *>(smem + sizeof(uint16_t) * BLOCK * K * (W > 0 ? W : 1) + 16);
+  const int nsteps = tail_only ? 1 : W;
+  // ring slot of the tail at window offset w is (n0 - 1 + w) mod D
+  int slot0 = (int)((n0 - 1) % D);
+  if (slot0 < 0) slot0 += D;
+
+  for (int w0 = 0; w0 < nsteps; w0 += PF) {
+    uint32_t cw[PF][K];
+    if constexpr (!GEN) {
+      // lean path: one packed count ring; prefetch PF slots, then zero them
+      if (g.nring == 1) {
+        const RingDev &rd = g.rings[0];
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+          const int w = w0 + u;
+          int sl = slot0 + w;
+          if (sl >= D) sl -= D;
+          const unsigned char *slot = static_cast<const unsigned char *>(rd.base) + (int64_t)sl * rd.slot_bytes;
+          const bool on = w < nsteps && !(w == 0 && skip_first_tail);
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            cw[u][k] = (on && valid[k]) ? load_slice(slot, blk0 + k * BLOCK + tid, rd.sw) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+          const int w = w0 + u;
+          int sl = slot0 + w;
+          if (sl >= D) sl -= D;
+          unsigned char *slot = static_cast<unsigned char *>(rd.base) + (int64_t)sl * rd.slot_bytes;
+          const bool on = w < nsteps && !(w == 0 && skip_first_tail);
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            if (on && valid[k]) zero_slice(slot, blk0 + k * BLOCK + tid, rd.sw);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int w = w0 + u;
+      if (w >= nsteps) break;
+      const int64_t n = n0 + w;
+      int sl = slot0 + w;
+      if (sl >= D) sl -= D;
+      // ---------------- tail of step n-1: deliveries, then resets
+      if (!(w == 0 && skip_first_tail)) {
+        const int64_t kst = n - 1;
+        if constexpr (!GEN) {
+          if (g.nring == 1) {
+            const RingDev &rd = g.rings[0];
+            const uint32_t mask = (1u << rd.bits) - 1u;
+            for (int c = 0; c < g.nchan; ++c) {
+              const ChanDev &ch = g.chans[c];
+              const T wv = (T)ch.w;
+#pragma unroll
+              for (int k = 0; k < K; ++k) {
+                const uint32_t cnt = (cw[u][k] >> (ch.ch * rd.bits)) & mask;
+#pragma unroll
+                for (int j = 0; j < MM; ++j)
+                  if (j == ch.var) s[j][k] = apply_count<T>(s[j][k], cnt, wv, ch.is_delay);
+              }
+            }
+          }
+        } else {
+          uint32_t words[MAXRING][K];
+          T fval[MAXRING][K];
+#pragma unroll
+          for (int r = 0; r < MAXRING; ++r) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              words[r][k] = 0u;
+              fval[r][k] = (T)0;
+            }
+            if (r >= g.nring) continue;
+            const RingDev &rd = g.rings[r];
+            unsigned char *slot = static_cast<unsigned char *>(rd.base) + (int64_t)sl * rd.slot_bytes;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              if (!valid[k]) continue;
+              const int i = blk0 + k * BLOCK + tid;
+              if (rd.kind == RING_COUNT) {
+                words[r][k] = load_slice(slot, i, rd.sw);
+                zero_slice(slot, i, rd.sw);
+              } else {
+                T *fp = reinterpret_cast<T *>(slot) + i;
+                fval[r][k] = *fp;
+                *fp = (T)0;
+              }
+            }
+          }
+          for (int c = 0; c < g.nchan; ++c) {
+            const ChanDev &ch = g.chans[c];
+            const RingDev &rd = g.rings[ch.ring];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              if (!valid[k]) continue;
+              uint32_t wsel = 0u;
+              T fsel = (T)0;
+#pragma unroll
+              for (int r = 0; r < MAXRING; ++r)
+                if (r == ch.ring) { wsel = words[r][k]; fsel = fval[r][k]; }
+#pragma unroll
+              for (int j = 0; j < MM; ++j) {
+                if (j != ch.var) continue;
+                if (rd.kind == RING_COUNT) {
+                  const uint32_t cnt = (wsel >> (ch.ch * rd.bits)) & ((1u << rd.bits) - 1u);
+                  s[j][k] = apply_count<T>(s[j][k], cnt, (T)ch.w, ch.is_delay);
+                } else {
+                  s[j][k] = s[j][k] + fsel;
+                }
+              }
+            }
+          }
+        }
+        // resets (brian/reset.py:139-147) / refractory clamp (:311-332)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (!valid[k] || last[k] == NEVER) continue;
+          if (GEN && g.refr != REFR_NONE) {
+            if (g.nreset > 0 && kst - last[k] < g.period) {
+              bool clamp = true;
+              if (g.refr == REFR_ARRAY) {
+                const int i = blk0 + k * BLOCK + tid;
+                const double na = (double)last[k] * g.dt + g.refr_arr[i];
+                clamp = na > ((double)kst * g.dt - g.dt * 0.25);
+              }
+              if (clamp) {
+                T v = (T)g.rs_val[0];
+                if (g.rs_vvar[0] >= 0) {
+#pragma unroll
+                  for (int q = 0; q < MM; ++q)
+                    if (q == g.rs_vvar[0]) v = s[q][k];
+                }
+#pragma unroll
+                for (int j = 0; j < MM; ++j)
+                  if (j == g.rs_var[0]) s[j][k] = v;
+              }
+            }
+          } else if (last[k] == kst) {
+            for (int r = 0; r < g.nreset; ++r) {
+              T v = (T)g.rs_val[r];
+              if (g.rs_vvar[r] >= 0) {
+#pragma unroll
+                for (int q = 0; q < MM; ++q)
+                  if (q == g.rs_vvar[r]) v = s[q][k];
+              }
+#pragma unroll
+              for (int j = 0; j < MM; ++j)
+                if (j == g.rs_var[r]) s[j][k] = (g.rs_op[r] == 0) ? v : s[j][k] + v;
+            }
+          }
+        }
+      }
+      if (tail_only) break;
+
+      // ---------------- head of step n
+      bool spk[K];
+      if constexpr (M > 0) {
+        if constexpr (GEN) {
+          if (g.ti_nt > 0) {  // TimedArraySetter at 'start' (brian/timedarray.py:287-303)
+            const int64_t row = n < g.ti_nt ? n : g.ti_nt - 1;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              if (!valid[k]) continue;
+              const int i = blk0 + k * BLOCK + tid;
+              const T v = g.ti_per_neuron ? load_T<T>(g.ti_vals, row * N + i) : load_T<T>(g.ti_vals, row);
+#pragma unroll
+              for (int j = 0; j < MM; ++j)
+                if (j == g.ti_var) s[j][k] = v;
+            }
+          }
+        }
+        if (!GEN || g.upd == UPD_LINEAR) {
+          linear_exact<MM, K>(s, a_sh, c_sh);
+        } else if (g.upd == UPD_PERNEURON) {
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            T one[MM][1];
+            T aa[MM * MM], cc[MM];
+#pragma unroll
+            for (int j = 0; j < MM; ++j) { one[j][0] = s[j][k]; cc[j] = cpn[j][k]; }
+#pragma unroll
+            for (int j = 0; j < MM * MM; ++j) aa[j] = apn[j][k];
+            linear_exact<MM, 1>(one, aa, cc);
+#pragma unroll
+            for (int j = 0; j < MM; ++j) s[j][k] = one[j][0];
+          }
+        } else if (g.upd == UPD_PROGRAM) {
+          if constexpr (K == 1) {
+            const double t = (double)n * g.dt + 0.0;  // Clock._t (brian/clock.py:186)
+            T one[MM][1];
+#pragma unroll
+            for (int j = 0; j < MM; ++j) one[j][0] = s[j][0];
+            program_step<T, MM>(g, one, reg, tid, t);
+#pragma unroll
+            for (int j = 0; j < MM; ++j) s[j][0] = one[j][0];
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        spk[k] = false;
+        if (!valid[k]) continue;
+        const int i = blk0 + k * BLOCK + tid;
+        if (GEN && g.kind == GK_POISSON) {
+          spk[k] = (uint64_t)counter_u32(g.seed, (uint64_t)i, (uint64_t)n) < g.pthr[i];
+        } else if (g.th_kind != TH_NONE) {
+          T v = (T)0, vt = (T)g.vt;
+#pragma unroll
+          for (int j = 0; j < MM; ++j) {
+            if (j == g.th_var) v = s[j][k];
+            if (g.th_kind == TH_VAR && j == g.th_vtvar) vt = s[j][k];
+          }
+          bool c = v > vt;
+          if constexpr (GEN) {
+            if (c && g.th_kind == TH_EMPIRICAL && last[k] != NEVER && n - last[k] <= g.th_period) c = false;
+            if (c && g.use_next_allowed && last[k] != NEVER) {
+              const double t = (double)n * g.dt + 0.0;
+              const double lt = (double)last[k] * g.dt + 0.0;
+              const double na = lt + (g.refr == REFR_ARRAY ? g.refr_arr[i] : g.refr_time);
+              c = na <= t;
+            }
+          }
+          spk[k] = c;
+        }
+        if (spk[k]) last[k] = (int32_t)n;
+      }
+      // wavefront ballot + prefix compaction into the block's LDS list
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const unsigned long long mask = __ballot(spk[k]);
+        if (mask == 0ull) continue;
+        const int leader = __ffsll((long long)mask) - 1;
+        int base = 0;
+        if (lane_id() == leader) base = atomicAdd(&sh_n, __popcll(mask));
+        base = __shfl(base, leader);
+        if (spk[k]) {
+          const unsigned long long below = mask & ((1ull << lane_id()) - 1ull);
+          sh_list[base + __popcll(below)] = (uint16_t)((w << 12) | (k * BLOCK + tid));
+        }
+      }
+    }
+  }
+
+  // ---------------- write back
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (!valid[k]) continue;
+    const int i = blk0 + k * BLOCK + tid;
+    g.last[i] = last[k];
+    if constexpr (M > 0) {
+#pragma unroll
+      for (int j = 0; j < M; ++j) static_cast<T *>(g.S)[(int64_t)j * N + i] = s[j][k];
+    }
+  }
+  __syncthreads();
+  const int nsp = sh_n;
+  if (nsp == 0) return;
+
+  // ---------------- raster append (SpikeMonitor, brian/monitor.py:156-166)
+  if (g.raster != nullptr) {
+    if (tid == 0) sh_base = atomicAdd(g.raster_count, (unsigned long long)nsp);
+    __syncthreads();
+    for (int p = tid; p < nsp; p += BLOCK) {
+      const uint32_t e = sh_list[p];
+      const unsigned long long pos = sh_base + p;
+      if (pos < (unsigned long long)g.raster_cap) {
+        const uint64_t step = (uint64_t)(n0 + (e >> 12));
+        const uint64_t idx = (uint64_t)(blk0 + (e & 0xFFFu));
+        g.raster[pos] = (step << 32) | idx;
+      }
+    }
+  }
+  // ---------------- spike history for STDP (only with W == 1)
+  if (g.hist != nullptr) {
+    const int64_t hs = n0 % g.H;
+    if (tid == 0) sh_hbase = atomicAdd(&g.hist_count[hs], (unsigned int)nsp);
+    __syncthreads();
+    for (int p = tid; p < nsp; p += BLOCK)
+      g.hist[hs * N + sh_hbase + p] = (int32_t)(blk0 + (sh_list[p] & 0xFFFu));
+  }
+  // ---------------- propagation: one wavefront per spike, lanes over the row
+  const int wave = tid >> 6, lane = tid & 63;
+  for (int p = wave; p < nsp; p += BLOCK / 64) {
+    const uint32_t e = sh_list[p];
+    const int64_t n = n0 + (e >> 12);
+    const int i = blk0 + (int)(e & 0xFFFu);
+    for (int o = 0; o < g.nout; ++o) {
+      const OutDev &od = g.outs[o];
+      const int64_t r = (int64_t)i - od.src_off;
+      if (r < 0 || r >= od.src_n) continue;
+      int base_slot = (int)(n % od.D);
+      const int64_t lo = od.row_ptr[r], hi = od.row_ptr[r + 1];
+      const int pbits = od.nch * od.bits;
+      for (int64_t q = lo + lane; q < hi; q += 64) {
+        const int64_t tj = od.tgt_off + od.col[q];
+        int d = od.hom_delay;
+        if (od.dly_bytes == 1) d = static_cast<const uint8_t *>(od.dly)[q];
+        else if (od.dly_bytes == 2) d = static_cast<const uint16_t *>(od.dly)[q];
+        int sl = base_slot + d;
+        while (sl >= od.D) sl -= od.D;
+        unsigned char *slot = static_cast<unsigned char *>(od.ring) + (int64_t)sl * od.slot_bytes;
+        if (od.kind == RING_COUNT) {
+          const int64_t pos = tj * pbits + od.ch * od.bits;
+          atomicAdd(reinterpret_cast<unsigned int *>(slot) + (pos >> 5), 1u << (int)(pos & 31));
+        } else {
+          unsafeAtomicAdd(reinterpret_cast<T *>(slot) + tj, static_cast<const T *>(od.w)[q]);
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ generator
+// FastSpikeGeneratorThreshold (brian/directcontrol.py:219-268): the spikes of
+// steps [n0, n0+W) are gen_idx[gen_off[n0] .. gen_off[n0+W]).  One thread per
+// emitted spike; grid-stride over the window's entries.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_generator(const GroupDev g, const int64_t *d_base,
+                                                     int64_t rel0, int W) {
+  const int64_t n0 = *d_base + rel0;
+  if (n0 >= g.gen_nsteps) return;
+  const int64_t n1 = (n0 + W < g.gen_nsteps) ? n0 + W : g.gen_nsteps;
+  const int64_t e0 = g.gen_off[n0], e1 = g.gen_off[n1];
+  const int64_t total = e1 - e0;
+  for (int64_t q0 = (int64_t)blockIdx.x * BLOCK; q0 < total; q0 += (int64_t)gridDim.x * BLOCK) {
+    const int64_t q = q0 + threadIdx.x;
+    const bool act = q < total;
+    int64_t e = e0 + q;
+    // step of entry e: binary search in gen_off[n0..n1]
+    int64_t lo = n0, hi = n1;  // gen_off[lo] <= e < gen_off[hi]
+    if (act) {
+      while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (g.gen_off[mid] <= e) lo = mid; else hi = mid;
+      }
+    }
+    const int64_t n = lo;
+    const int64_t i = act ? (int64_t)g.gen_idx[e] : 0;
+    if (g.raster != nullptr) {
+      __shared__ unsigned long long sh_b2;
+      __shared__ int sh_cnt;
+      if (threadIdx.x == 0) sh_cnt = 0;
+      __syncthreads();
+      int pos = 0;
+      if (act) pos = atomicAdd(&sh_cnt, 1);
+      __syncthreads();
+      if (threadIdx.x == 0) sh_b2 = atomicAdd(g.raster_count, (unsigned long long)sh_cnt);
+      __syncthreads();
+      if (act && sh_b2 + pos < (unsigned long long)g.raster_cap)
+        g.raster[sh_b2 + pos] = ((uint64_t)n << 32) | (uint64_t)i;
+      __syncthreads();
+    }
+    if (!act) continue;
+    if (g.hist != nullptr) {
+      const int64_t hs = n % g.H;
+      const unsigned int hp = atomicAdd(&g.hist_count[hs], 1u);
+      g.hist[hs * g.N + hp] = (int32_t)i;
+    }
+    for (int o = 0; o < g.nout; ++o) {
+      const OutDev &od = g.outs[o];
+      const int64_t r = i - od.src_off;
+      if (r < 0 || r >= od.src_n) continue;
+      const int64_t rlo = od.row_ptr[r], rhi = od.row_ptr[r + 1];
+      for (int64_t qq = rlo; qq < rhi; ++qq) {
+        const int64_t tj = od.tgt_off + od.col[qq];
+        int d = od.hom_delay;
+        if (od.dly_bytes == 1) d = static_cast<const uint8_t *>(od.dly)[qq];
+        else if (od.dly_bytes == 2) d = static_cast<const uint16_t *>(od.dly)[qq];
+        const int64_t sl = (n + d) % od.D;
+        unsigned char *slot = static_cast<unsigned char *>(od.ring) + sl * od.slot_bytes;
+        if (od.kind == RING_COUNT) {
+          const int64_t pos = tj * (od.nch * od.bits) + od.ch * od.bits;
+          atomicAdd(reinterpret_cast<unsigned int *>(slot) + (pos >> 5), 1u << (int)(pos & 31));
+        } else {
+          unsafeAtomicAdd(reinterpret_cast<T *>(slot) + tj, static_cast<const T *>(od.w)[qq]);
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ STDP
+struct StdpDev {
+  int64_t src_off, src_n, tgt_off, tgt_n;
+  const int64_t *row_ptr;
+  const int32_t *col;
+  void *w;                    // T[nnz]
+  const int64_t *col_ptr;     // [tgt_n+1]
+  const int64_t *col_perm;    // [nnz] index into w, ascending source within column
+  const int32_t *col_row;     // [nnz] source row of that synapse
+  void *a_pre, *a_post;       // T[src_n], T[tgt_n]
+  double decay_pre, decay_post, Ap, Am, wmin, wmax;
+  int update, interactions, pre_delay, pad;
+  const int32_t *pre_hist;    // source group's history [H][Nsrc]
+  const unsigned int *pre_hist_count;
+  int pre_H;
+  int post_H;
+  const int32_t *post_hist;
+  const unsigned int *post_hist_count;
+  int64_t pre_N, post_N;      // group sizes (history row stride)
+};
+
+// Trace decay: the 1x1 LinearStateUpdater of the STDP trace groups
+// (brian/stdp.py:282-287): A = decay*A + 0.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_stdp_decay(const StdpDev sd) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i < sd.src_n) {
+    T *a = static_cast<T *>(sd.a_pre);
+    a[i] = (T)sd.decay_pre * a[i] + (T)0;
+  } else if (i < sd.src_n + sd.tgt_n) {
+    T *a = static_cast<T *>(sd.a_post);
+    const int64_t j = i - sd.src_n;
+    a[j] = (T)sd.decay_post * a[j] + (T)0;
+  }
+}
+
+// Presynaptic updater (brian/stdp.py:371-393): for each pre spike i,
+// A_pre[i] += Ap; w[i,:] += A_post[cols]; clip.  One wavefront per spike.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_stdp_pre(const StdpDev sd, const int64_t *d_base, int64_t rel0) {
+#pragma clang fp contract(off)
+  const int64_t n = *d_base + rel0 - sd.pre_delay;
+  if (n < 0) return;
+  const int64_t hs = n % sd.pre_H;
+  const unsigned int cnt = sd.pre_hist_count[hs];
+  const int wave = (blockIdx.x * BLOCK + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  const int nwaves = gridDim.x * (BLOCK / 64);
+  T *w = static_cast<T *>(sd.w);
+  T *apre = static_cast<T *>(sd.a_pre);
+  const T *apost = static_cast<const T *>(sd.a_post);
+  for (unsigned int p = wave; p < cnt; p += nwaves) {
+    const int64_t i = (int64_t)sd.pre_hist[hs * sd.pre_N + p] - sd.src_off;
+    if (i < 0 || i >= sd.src_n) continue;
+    if (lane == 0) {
+      if (sd.interactions == 0 || sd.interactions == 3) apre[i] = apre[i] + (T)sd.Ap;
+      else apre[i] = (T)sd.Ap;
+    }
+    const int64_t lo = sd.row_ptr[i], hi = sd.row_ptr[i + 1];
+    for (int64_t q = lo + lane; q < hi; q += 64) {
+      T x = w[q];
+      const T ap = apost[sd.col[q]];
+      if (sd.update == 0) x = x + ap;
+      else if (sd.Am < 0) x = x * ((T)1 + ap);
+      else x = x + ((T)sd.wmax - x) * ap;
+      x = x < (T)sd.wmin ? (T)sd.wmin : (x > (T)sd.wmax ? (T)sd.wmax : x);
+      w[q] = x;
+    }
+  }
+}
+
+// Postsynaptic updater: for each post spike j, A_post[j] += Am;
+// w[:,j] += A_pre[rows]; clip, through the column permutation
+// (brian/connections/connectionmatrix.py:317-333, :410-449).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_stdp_post(const StdpDev sd, const int64_t *d_base, int64_t rel0) {
+#pragma clang fp contract(off)
+  const int64_t n = *d_base + rel0;
+  const int64_t hs = n % sd.post_H;
+  const unsigned int cnt = sd.post_hist_count[hs];
+  const int wave = (blockIdx.x * BLOCK + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  const int nwaves = gridDim.x * (BLOCK / 64);
+  T *w = static_cast<T *>(sd.w);
+  const T *apre = static_cast<const T *>(sd.a_pre);
+  T *apost = static_cast<T *>(sd.a_post);
+  for (unsigned int p = wave; p < cnt; p += nwaves) {
+    const int64_t j = (int64_t)sd.post_hist[hs * sd.post_N + p] - sd.tgt_off;
+    if (j < 0 || j >= sd.tgt_n) continue;
+    if (lane == 0) {
+      if (sd.interactions == 0 || sd.interactions == 2) apost[j] = apost[j] + (T)sd.Am;
+      else apost[j] = (T)sd.Am;
+    }
+    const int64_t lo = sd.col_ptr[j], hi = sd.col_ptr[j + 1];
+    for (int64_t q = lo + lane; q < hi; q += 64) {
+      const int64_t pw = sd.col_perm[q];
+      T x = w[pw];
+      const T a = apre[sd.col_row[q]];
+      if (sd.update == 0) x = x + a;
+      else if (sd.Ap < 0) x = x * ((T)1 + a);
+      else x = x + ((T)sd.wmax - x) * a;
+      x = x < (T)sd.wmin ? (T)sd.wmin : (x > (T)sd.wmax ? (T)sd.wmax : x);
+      w[pw] = x;
+    }
+  }
+}
+
+// Clears the history counter of the slot the next step will write.
+__global__ void k_hist_clear(unsigned int *const *counts, const int *H, int ngroups,
+                             const int64_t *d_base, int64_t rel_next) {
+  const int gi = threadIdx.x;
+  if (gi < ngroups) {
+    const int64_t n = *d_base + rel_next;
+    counts[gi][n % H[gi]] = 0u;
+  }
+}
+
+__global__ void k_advance(int64_t *d_base, int64_t by) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *d_base += by;
+}
+
+}  // namespace bm

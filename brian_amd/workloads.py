@@ -1,0 +1,120 @@
+"""Seeded synthetic workloads for the CUBA benchmark network.
+
+Everything here is host-side input generation (numpy ``Generator(PCG64)``):
+connectivity, initial state and input arrays that are fed identically to the
+device engine and to the CPU oracle.  Model constants follow
+``examples/misc/CUBA.py`` and ``dev/benchmarking/cuba/cuba.py:7-62`` of the
+reference (Benchmark 2 of Brette et al. 2007), with units as plain floats
+(``brian_no_units``).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .units import mV, ms
+
+# CUBA constants (examples/misc/CUBA.py:27-43)
+taum = 20 * ms
+taue = 5 * ms
+taui = 10 * ms
+Vt = -50 * mV
+Vr = -60 * mV
+El = -49 * mV
+we = (60 * 0.27 / 10) * mV
+wi = (-20 * 4.5 / 10) * mV
+CUBA_EQS = """
+dv/dt  = (ge+gi-(v-El))/taum : volt
+dge/dt = -ge/taue : volt
+dgi/dt = -gi/taui : volt
+"""
+
+
+def cuba_rhs(x):
+    """Right-hand side of the CUBA equations on a state vector [v, ge, gi]."""
+    v, ge, gi = x[0], x[1], x[2]
+    return np.array([(ge + gi - (v - El)) / taum, -ge / taue, -gi / taui])
+
+
+def csr_from_coo(src, tgt, nsrc):
+    """Source-row CSR with ascending column indices within each row."""
+    src = np.asarray(src, dtype=np.int64)
+    tgt = np.asarray(tgt, dtype=np.int64)
+    order = np.lexsort((tgt, src))
+    counts = np.bincount(src, minlength=nsrc)
+    indptr = np.zeros(nsrc + 1, dtype=np.int64)
+    np.cumsum(counts, out=indptr[1:])
+    return indptr, tgt[order].astype(np.int32)
+
+
+def bernoulli_csr(rng, nsrc, ntgt, p):
+    """Bernoulli(p) over every (i, j) pair, row by row (self-pairs allowed)."""
+    rows = []
+    counts = np.zeros(nsrc, dtype=np.int64)
+    chunk = max(1, int(2 ** 24 // max(ntgt, 1)))
+    for a in range(0, nsrc, chunk):
+        b = min(nsrc, a + chunk)
+        mask = rng.random((b - a, ntgt)) < p
+        r, c = np.nonzero(mask)
+        counts[a:b] = np.bincount(r, minlength=b - a)
+        rows.append(c.astype(np.int32))
+    indptr = np.zeros(nsrc + 1, dtype=np.int64)
+    np.cumsum(counts, out=indptr[1:])
+    indices = np.concatenate(rows) if rows else np.zeros(0, np.int32)
+    return indptr, indices
+
+
+def fixed_indegree_sources(rng, N, k):
+    """For each target, k distinct presynaptic ids uniform over [0, N)."""
+    pre = rng.integers(0, N, size=(N, k), dtype=np.int64)
+    pre.sort(axis=1)
+    while True:
+        dup = np.nonzero((pre[:, 1:] == pre[:, :-1]).any(axis=1))[0]
+        if len(dup) == 0:
+            return pre
+        fresh = rng.integers(0, N, size=(len(dup), k), dtype=np.int64)
+        fresh.sort(axis=1)
+        pre[dup] = fresh
+
+
+def cuba(N=4000, seed=1, connectivity="bernoulli", p=0.02, indegree=80, delay_steps=0):
+    """CUBA network description: dict with v0 and two CSR blocks.
+
+    connectivity 'bernoulli': Bernoulli(p) over all pairs (C1 form);
+    'indegree': fixed in-degree with distinct uniform sources (C2 form).
+    The first int(0.8 N) neurons are excitatory (weight we onto ge), the
+    rest inhibitory (wi onto gi).
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    Ne = int(0.8 * N)
+    Ni = N - Ne
+    v0 = Vr + rng.random(N) * (Vt - Vr)
+    if connectivity == "bernoulli":
+        ipe, ixe = bernoulli_csr(rng, Ne, N, p)
+        ipi, ixi = bernoulli_csr(rng, Ni, N, p)
+    elif connectivity == "indegree":
+        pre = fixed_indegree_sources(rng, N, indegree)
+        tgt = np.repeat(np.arange(N, dtype=np.int64), indegree)
+        src = pre.reshape(-1)
+        # stable sort by source keeps targets ascending within each row
+        order = np.argsort(src, kind="stable")
+        src_s, tgt_s = src[order], tgt[order]
+        counts = np.bincount(src_s, minlength=N)
+        indptr = np.zeros(N + 1, dtype=np.int64)
+        np.cumsum(counts, out=indptr[1:])
+        cols = tgt_s.astype(np.int32)
+        e_end = indptr[Ne]
+        ipe = indptr[:Ne + 1].copy()
+        ixe = cols[:e_end]
+        ipi = indptr[Ne:] - e_end
+        ixi = cols[e_end:]
+    else:
+        raise ValueError(connectivity)
+    S0 = np.zeros((3, N))
+    S0[0] = v0
+    return {
+        "N": N, "Ne": Ne, "Ni": Ni, "S0": S0,
+        "exc": (ipe, ixe, np.full(len(ixe), we)),
+        "inh": (ipi, ixi, np.full(len(ixi), wi)),
+        "delay_steps": int(delay_steps),
+        "Vt": Vt, "Vr": Vr,
+    }

@@ -1,0 +1,170 @@
+/*
+ * brianmi.h -- C-ABI of libbrianmi.so, the MI355X (gfx950) backend for the
+ * clock-driven network step of Brian 1.4.4.
+ *
+ * Conventions (all entry points):
+ *   - return int: 0 = OK, otherwise a BM_E* code; bm_last_error() gives text
+ *     (thread-local, valid until the next call on the same thread);
+ *   - never throw across the ABI;
+ *   - the caller owns host buffers, the library owns device buffers; readback
+ *     fills caller buffers by copy (no views into library memory);
+ *   - a handle is not thread-safe and owns one HIP stream (default priority).
+ *
+ * Reference interfaces replaced (paths relative to brian-team/brian 1.4.4):
+ *   groups      brian/neurongroup.py:257-549 (NeuronGroup update/reset/get_spikes),
+ *               brian/stateupdater.py:328-487 (LinearStateUpdater),
+ *               brian/experimental/multilinearstateupdater.py:112-158,
+ *               brian/stateupdater.py:490-633 (Euler / RK2 / exponential Euler),
+ *               brian/threshold.py:129-200, :337-406, brian/reset.py:115-335
+ *   connections brian/connections/connection.py:224-296 (Connection.propagate),
+ *               brian/connections/delayconnection.py:85-206 (DelayConnection),
+ *               brian/connections/connectionmatrix.py:196-455 (SparseConnectionMatrix)
+ *   stdp        brian/stdp.py:30-54, :181-539 (STDPUpdater, STDP, ExponentialSTDP)
+ *   inputs      brian/directcontrol.py:219-268 (FastSpikeGeneratorThreshold),
+ *               brian/timedarray.py:287-303 (TimedArraySetter)
+ *   monitors    brian/monitor.py:156-199 (SpikeMonitor), :590-638 (PopulationRateMonitor)
+ *   run loop    brian/network.py:541-586 (Network.run), brian/clock.py:162-212
+ *   fitting     brian/library/modelfitting/gpu_modelfitting.py:202-401 (GPUModelFitting)
+ *
+ * Brian 1 has no C-ABI for this path (its objects are Python callables); the
+ * functions below are the calls its Python protocols map onto, and the
+ * ctypes binding in brian_amd/_lib.py is the reference-side stub.
+ */
+#ifndef BRIANMI_H
+#define BRIANMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  BM_OK = 0,
+  BM_EINVAL = 1,     /* bad argument (shape, index, range) */
+  BM_EHIP = 2,       /* HIP runtime error */
+  BM_ENOMEM = 3,     /* device allocation failed */
+  BM_ESTATE = 4,     /* call not allowed in the current state */
+  BM_EOVERFLOW = 5,  /* a bounded device buffer overflowed (raster, spike list) */
+  BM_ENODEV = 6      /* no usable gfx950 device */
+};
+
+typedef struct bm_net bm_net;
+
+const char *bm_last_error(void);
+int bm_version(void);
+
+/* ---------------------------------------------------------------- network
+ * Network + Clock (brian/network.py:62-648, brian/clock.py:52-224).
+ * precision: 32 (FP32 fast path) or 64 (FP64 check mode, no contraction,
+ * Brian's arithmetic order). */
+int bm_net_create(double dt, int precision, bm_net **out);
+int bm_net_destroy(bm_net *net);
+
+/* NeuronGroup with nvars state variables, S0 is SoA [nvars][N]
+ * (brian/group.py:24, brian/neurongroup.py:362-366). */
+int bm_net_add_group(bm_net *net, int64_t N, int nvars, const double *S0, int *gid);
+
+/* LinearStateUpdater: S <- A.S + C with A = expm(M dt), C = B - A.B computed
+ * on the host in FP64 (brian/stateupdater.py:253-282, :351-416).
+ * A is row-major [nvars][nvars], C is [nvars]. */
+int bm_net_set_linear(bm_net *net, int gid, const double *A, const double *C);
+
+/* Per-neuron propagator (brian/experimental/multilinearstateupdater.py:132-158):
+ * A is [nvars][nvars][N], C is [nvars][N]. */
+int bm_net_set_linear_per_neuron(bm_net *net, int gid, const double *A, const double *C);
+
+/* Nonlinear state updater (Euler / RK2 midpoint / exponential Euler,
+ * brian/equations.py:655-789) given as a register bytecode program, see
+ * brian_amd/equations.py for the encoding.  scheme: 1 Euler, 2 RK2, 3 exp-Euler. */
+int bm_net_set_program(bm_net *net, int gid, int scheme, int nregs,
+                       int ninstr, const int32_t *code, int nconst, const double *consts);
+
+/* Threshold (brian/threshold.py:160-191): spike iff S[var] > vt, or
+ * S[var] > S[vt_var] when vt_var >= 0 (VariableThreshold, :373-396). */
+int bm_net_set_threshold(bm_net *net, int gid, int var, double vt, int vt_var);
+
+/* Reset op applied to neurons that spiked (brian/reset.py:139-147, :196-257):
+ * op 0: S[var] = value (or S[value_var]); op 1: S[var] += value (or S[value_var]).
+ * Ops are applied in the order added. */
+int bm_net_add_reset_op(bm_net *net, int gid, int var, int op, double value, int value_var);
+
+/* EmpiricalThreshold (brian/threshold.py:409-457): spike iff S[var] > vt and
+ * the neuron did not spike in the last `period` steps (period = int(refr/dt),
+ * minus one when refr is a whole number of steps). */
+int bm_net_set_empirical_threshold(bm_net *net, int gid, int var, double vt, int period);
+
+/* Refractoriness (brian/neurongroup.py:368-429, :506-515, brian/reset.py:311-332):
+ * period_bins = int(max_refractory/dt)+1, refractory_time = R - 0.5*dt;
+ * per_neuron (nullable) gives variable refractoriness [N] in seconds. */
+int bm_net_set_refractory(bm_net *net, int gid, int period_bins, double refractory_time,
+                          const double *per_neuron);
+
+/* TimedArraySetter (brian/timedarray.py:287-303): at the start of step n,
+ * S[var][i] = values[min(n, nt-1)][i] (per_neuron) or values[min(n,nt-1)]. */
+int bm_net_set_timed_input(bm_net *net, int gid, int var, int64_t nt, int per_neuron,
+                           const double *values);
+
+/* SpikeGeneratorGroup (FastSpikeGeneratorThreshold, brian/directcontrol.py:219-268):
+ * nspikes (step, index) pairs sorted by (step, index). */
+int bm_net_add_generator(bm_net *net, int64_t N, int64_t nspikes, const int32_t *steps,
+                         const int32_t *idx, int *gid);
+
+/* Poisson source with a counter-based RNG: neuron i spikes at step n iff
+ * u32(seed, i, n) < floor(rate_i*dt*2^32) (splitmix64 construction). */
+int bm_net_add_poisson(bm_net *net, int64_t N, const double *rates, uint64_t seed, int *gid);
+
+/* Connection / DelayConnection (brian/connections/connection.py:224-296,
+ * delayconnection.py:85-206) as CSR over the source range [src_off, src_off+src_n)
+ * into target range [tgt_off, tgt_off+tgt_n), target variable tgt_var.
+ * row_ptr[src_n+1] (int64), col[nnz] (ascending within row, relative to tgt_off),
+ * w[nnz].  delay_kind 0: Connection with homogeneous delay hom_delay_steps
+ * (int(delay/dt)); delay_kind 1: DelayConnection with per-synapse steps
+ * syn_delay[nnz] = int(d*invdt) and max_delay_steps = int(max_delay/dt). */
+int bm_net_add_connection(bm_net *net, int src_gid, int64_t src_off, int64_t src_n,
+                          int tgt_gid, int64_t tgt_off, int64_t tgt_n, int tgt_var,
+                          int64_t nnz, const int64_t *row_ptr, const int32_t *col,
+                          const double *w, int delay_kind, int hom_delay_steps,
+                          const int32_t *syn_delay, int max_delay_steps, int *cid);
+
+/* ExponentialSTDP on a delay_kind 0 connection (brian/stdp.py:452-539):
+ * decay_pre/decay_post are the 1x1 propagators exp(-dt/tau) (FP64, host);
+ * Ap, Am already scaled as in stdp.py:509-513; update 0 additive,
+ * 1 multiplicative; interactions 0 all, 1 nearest, 2 nearest_pre, 3 nearest_post. */
+int bm_net_add_stdp(bm_net *net, int cid, double decay_pre, double decay_post,
+                    double Ap, double Am, double wmin, double wmax, int update,
+                    int interactions, int *sid);
+
+/* SpikeMonitor on a group: record (step, index) into a device raster of
+ * capacity entries (brian/monitor.py:156-166). */
+int bm_net_record(bm_net *net, int gid, int64_t capacity);
+
+/* Freeze the structure, allocate rings, choose the step window.
+ * window 0 = automatic (min synaptic delay + 1, capped), else forced (>=1).
+ * neurons_per_thread 0 = automatic. */
+int bm_net_prepare(bm_net *net, int window, int neurons_per_thread, int use_graph);
+
+/* Advance nsteps (brian/network.py:568-584).  Asynchronous until bm_net_sync. */
+int bm_net_run(bm_net *net, int64_t nsteps);
+int bm_net_sync(bm_net *net);
+int bm_net_current_step(bm_net *net, int64_t *step);
+
+int bm_net_get_state(bm_net *net, int gid, int var, double *out);
+int bm_net_set_state(bm_net *net, int gid, int var, const double *in);
+int bm_net_spike_count(bm_net *net, int gid, int64_t *count);
+/* Raster sorted by (step, index); n_out = number of entries written. */
+int bm_net_get_spikes(bm_net *net, int gid, int32_t *steps, int32_t *idx, int64_t cap,
+                      int64_t *n_out);
+int bm_net_clear_spikes(bm_net *net, int gid);
+int bm_net_get_weights(bm_net *net, int cid, double *w);
+int bm_net_get_stdp_traces(bm_net *net, int sid, double *a_pre, double *a_post);
+
+/* Timing of the sweep kernel launches of the last run (HIP events on the
+ * handle's stream), when enabled before bm_net_run. */
+int bm_net_set_profiling(bm_net *net, int enable);
+int bm_net_get_profile(bm_net *net, int64_t *launches, double *total_ms, double *window_steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BRIANMI_H */
