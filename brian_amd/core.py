@@ -1,0 +1,1174 @@
+"""Brian 1 compatible objects backed by libbrianmi.
+
+The classes keep the reference's constructor signatures and attribute names
+(brian/neurongroup.py, threshold.py, reset.py, connections/, stdp.py,
+monitor.py, directcontrol.py, timedarray.py, network.py, clock.py) so a
+Brian 1 script runs unchanged after ``from brian_amd import *``.  Nothing
+here integrates anything: the objects describe the network on the host and
+``Network.run`` compiles it once into a device network (``_lib.NetHandle``)
+and advances it there.  Spike rasters, rates and state are read back on
+demand.
+"""
+from __future__ import annotations
+
+import inspect
+import re
+import weakref
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from .equations import Equations
+from .units import ms, mV, second
+
+__all__ = [
+    "Clock", "defaultclock", "reinit_default_clock", "get_default_clock",
+    "Threshold", "VariableThreshold", "EmpiricalThreshold", "NoThreshold", "StringThreshold",
+    "Reset", "VariableReset", "StringReset", "Refractoriness", "NoReset",
+    "NeuronGroup", "SpikeGeneratorGroup", "PoissonGroup", "TimedArray",
+    "Connection", "DelayConnection", "STDP", "ExponentialSTDP",
+    "SpikeMonitor", "PopulationRateMonitor", "PopulationSpikeCounter", "SpikeCounter",
+    "StateMonitor", "Network", "MagicNetwork", "run", "reinit", "stop", "network_operation",
+    "NetworkOperation", "set_global_preferences", "get_global_preference", "forget",
+]
+
+_PREFS = {"precision": 64, "window": 0, "neurons_per_thread": 0, "use_graph": True,
+          "max_rate_hz": 250.0}
+
+
+def set_global_preferences(**kw):
+    """brian/globalprefs.py:62-89.  Backend keys: precision (32 | 64),
+    window (0 = automatic), neurons_per_thread, use_graph, max_rate_hz
+    (raster capacity sizing).  Unknown Brian keys are accepted and ignored."""
+    _PREFS.update(kw)
+
+
+def get_global_preference(k):
+    return _PREFS[k]
+
+
+# ------------------------------------------------------------------ tracking
+_COMPREHENSIONS = ("<listcomp>", "<genexpr>", "<dictcomp>", "<setcomp>")
+
+class _Tracked:
+    """Records the frame that created an object (brian/magic.py:173-197) so
+    that ``run()`` can find the objects of the calling scope."""
+    _instances = []
+
+    def _track(self, level=1):
+        frame = inspect.currentframe()
+        try:
+            f = frame.f_back
+            # skip this module and Python 3 comprehension frames (in Python 2,
+            # which Brian's magic assumes, comprehensions ran in the caller)
+            while f is not None and (f.f_code.co_filename == __file__ or
+                                     f.f_code.co_name in _COMPREHENSIONS):
+                f = f.f_back
+            self._frame_id = id(f) if f is not None else None
+        finally:
+            del frame
+        _Tracked._instances.append(weakref.ref(self))
+
+
+def forget(*objs):
+    ids = set(id(o) for o in objs)
+    _Tracked._instances = [r for r in _Tracked._instances if r() is not None and id(r()) not in ids]
+
+
+# ------------------------------------------------------------------ clock
+class Clock:
+    """Integer-tick clock, t = n*dt (brian/clock.py:52-224)."""
+
+    def __init__(self, dt=0.1 * ms, t=0 * ms, order=0, makedefaultclock=False):
+        self._dt = float(dt)
+        self._n = int(float(t) / self._dt)
+        self.order = order
+
+    @property
+    def dt(self):
+        return self._dt
+
+    @dt.setter
+    def dt(self, v):
+        self._dt = float(v)
+
+    @property
+    def t(self):
+        return self._n * self._dt + 0.0
+
+    @t.setter
+    def t(self, v):
+        self._n = int(float(v) / self._dt)
+
+    def reinit(self, t=0 * ms):
+        self._n = int(float(t) / self._dt)
+
+
+defaultclock = Clock()
+
+
+def get_default_clock():
+    return defaultclock
+
+
+def reinit_default_clock(t=0 * ms):
+    defaultclock.reinit(t)
+
+
+# ------------------------------------------------------------------ threshold / reset
+class Threshold:
+    """``(P.state_(state) > threshold).nonzero()`` (brian/threshold.py:129-191)."""
+
+    def __init__(self, threshold=0 * mV, state=0):
+        self.threshold = threshold
+        self.state = state
+
+
+class VariableThreshold:
+    """Spike iff state > threshold_state (brian/threshold.py:337-406)."""
+
+    def __init__(self, threshold_state=1, state=0):
+        self.threshold_state = threshold_state
+        self.state = state
+
+
+class EmpiricalThreshold:
+    """brian/threshold.py:409-457: no re-trigger within ``refractory``."""
+
+    def __init__(self, threshold=1 * mV, refractory=1 * ms, state=0, clock=None):
+        self.threshold = threshold
+        self.state = state
+        clock = clock or defaultclock
+        self.refractory = int(refractory / clock.dt)
+        if abs(self.refractory * clock.dt - refractory) <= 1e-8 * abs(refractory) and self.refractory > 0:
+            self.refractory -= 1
+
+
+class NoThreshold:
+    pass
+
+
+class StringThreshold:
+    """'v > expr' where expr is a constant or a variable."""
+
+    def __init__(self, expr, level=0):
+        self.expr = expr
+
+
+class Reset:
+    """``V[lastspikes] = resetvalue`` (brian/reset.py:115-147)."""
+
+    def __init__(self, resetvalue=0 * mV, state=0):
+        self.resetvalue = resetvalue
+        self.state = state
+
+
+class VariableReset:
+    """brian/reset.py:220-257: V = S[resetvaluestate] at spikes."""
+
+    def __init__(self, resetvaluestate=1, state=0):
+        self.resetvaluestate = resetvaluestate
+        self.state = state
+
+
+class StringReset:
+    """Statements such as 'v = Vr; Vt += dVt' (brian/reset.py:155-213)."""
+
+    def __init__(self, expr, level=0):
+        self.expr = expr
+        self.namespace = dict(_caller_ns(level + 1))
+
+
+class Refractoriness(Reset):
+    """brian/reset.py:288-335: hold ``state`` at ``resetvalue`` for ``period``."""
+
+    def __init__(self, resetvalue=0 * mV, period=5 * ms, state=0):
+        Reset.__init__(self, resetvalue, state)
+        self.period = period
+
+
+class NoReset:
+    pass
+
+
+def _caller_ns(level):
+    frame = inspect.currentframe()
+    try:
+        f = frame
+        for _ in range(level + 1):
+            f = f.f_back
+        ns = dict(f.f_globals)
+        ns.update(f.f_locals)
+        return ns
+    finally:
+        del frame
+
+
+# ------------------------------------------------------------------ groups
+class _GroupBase(_Tracked):
+    clock = None
+
+    def __len__(self):
+        return self.N
+
+    def subgroup(self, N):
+        """Consecutive slices of the owner (brian/neurongroup.py:551-599)."""
+        owner = self._owner
+        start = self._origin + self._next_sub
+        if self._next_sub + N > self.N:
+            raise ValueError("subgroup exceeds group size")
+        self._next_sub += N
+        return _SubGroup(owner, start, N)
+
+    def __getitem__(self, sl):
+        if isinstance(sl, slice):
+            start, stop, step = sl.indices(self.N)
+            if step != 1:
+                raise ValueError("subgroup slices must be contiguous")
+            return _SubGroup(self._owner, self._origin + start, stop - start)
+        raise TypeError("index a group with a slice")
+
+
+class NeuronGroup(_GroupBase):
+    """brian/neurongroup.py:257-549.
+
+    ``NeuronGroup(N, model, threshold=None, reset=None, refractory=0*ms,
+    clock=None, method=None, order=1, implicit=False, ...)``.  Linear models
+    use the exact propagator; others Euler (order=1), RK2 (order=2) or
+    exponential Euler (implicit=True), as magic_state_updater chooses
+    (brian/stateupdater.py:70-172).
+    """
+
+    def __init__(self, N, model=None, threshold=None, reset=NoReset(), init=None,
+                 refractory=0 * ms, level=0, clock=None, order=1, implicit=False,
+                 unit_checking=True, max_delay=0 * ms, compile=False, freeze=False,
+                 method=None, max_refractory=None, **kw):
+        self._track(level + 1)
+        self.N = int(N)
+        self._owner = self
+        self._origin = 0
+        self._next_sub = 0
+        self.clock = clock or defaultclock
+        if isinstance(model, str):
+            model = Equations(model, level=level + 1)
+        self._eqs = model
+        self.var_names = model.variables() if model is not None else []
+        self.var_index = {n: j for j, n in enumerate(self.var_names)}
+        for j in range(len(self.var_names)):
+            self.var_index[j] = j
+        self._S = np.zeros((len(self.var_names), self.N))
+        if init is not None:
+            for j, v in enumerate(init):
+                self._S[j, :] = v
+        self._method = method
+        self._order = order
+        self._implicit = implicit or method == "exponential_Euler"
+        if method == "Euler":
+            self._order = 1
+        elif method == "RK":
+            self._order = 2
+        self._threshold = self._make_threshold(threshold, level + 1)
+        self._reset, self._refractory = self._make_reset(reset, refractory, max_refractory)
+        self._net = None
+
+    # ---- construction helpers
+    def _var(self, v):
+        if isinstance(v, str):
+            return self.var_index[v]
+        return int(v)
+
+    def _make_threshold(self, th, level):
+        if th is None or isinstance(th, NoThreshold):
+            return None
+        if isinstance(th, (int, float, np.floating)):
+            return ("const", 0, float(th))
+        if isinstance(th, Threshold):
+            return ("const", self._var(th.state), float(th.threshold))
+        if isinstance(th, VariableThreshold):
+            return ("var", self._var(th.state), self._var(th.threshold_state))
+        if isinstance(th, EmpiricalThreshold):
+            return ("empirical", self._var(th.state), float(th.threshold), th.refractory)
+        if isinstance(th, (str, StringThreshold)):
+            expr = th if isinstance(th, str) else th.expr
+            m = re.match(r"^\s*(\w+)\s*>\s*(.+?)\s*$", expr)
+            if not m or m.group(1) not in self.var_index:
+                raise ValueError(f"unsupported threshold expression {expr!r}")
+            rhs = m.group(2)
+            if rhs in self.var_index:
+                return ("var", self.var_index[m.group(1)], self.var_index[rhs])
+            ns = dict(self._eqs._namespace)
+            ns.update(_caller_ns(level + 1))
+            return ("const", self.var_index[m.group(1)], float(eval(rhs, ns)))
+        raise TypeError(f"unsupported threshold {th!r}")
+
+    def _make_reset(self, reset, refractory, max_refractory):
+        dt = self.clock.dt
+        ops = []
+        refr = None
+        if isinstance(reset, NoReset) or reset is None:
+            return [], None
+        if isinstance(reset, Refractoriness):
+            var = self._var(reset.state)
+            ops = [(var, 0, float(reset.resetvalue), -1)]
+            return ops, ("const", float(reset.period))
+        if isinstance(reset, (int, float, np.floating)) or type(reset) is Reset:
+            if type(reset) is Reset:
+                var, val = self._var(reset.state), float(reset.resetvalue)
+            else:
+                var, val = 0, float(reset)
+            ops = [(var, 0, val, -1)]
+            if isinstance(refractory, (int, float, np.floating)):
+                if refractory > 0.9 * dt:
+                    refr = ("const", float(refractory))
+            else:
+                arr = np.asarray(refractory, dtype=np.float64)
+                refr = ("array", np.broadcast_to(arr, (self.N,)).copy())
+            return ops, refr
+        if isinstance(reset, VariableReset):
+            return [(self._var(reset.state), 0, 0.0, self._var(reset.resetvaluestate))], None
+        if isinstance(reset, (str, StringReset)):
+            expr = reset if isinstance(reset, str) else reset.expr
+            ns = dict(self._eqs._namespace)
+            if isinstance(reset, StringReset):
+                ns.update(reset.namespace)
+            else:
+                ns.update(_caller_ns(3))
+            for stmt in re.split(r"[;\n]", expr):
+                stmt = stmt.strip()
+                if not stmt:
+                    continue
+                m = re.match(r"^(\w+)\s*(\+?=)\s*(.+)$", stmt)
+                if not m or m.group(1) not in self.var_index:
+                    raise ValueError(f"unsupported reset statement {stmt!r}")
+                var = self.var_index[m.group(1)]
+                op = 0 if m.group(2) == "=" else 1
+                rhs = m.group(3).strip()
+                if rhs in self.var_index:
+                    ops.append((var, op, 0.0, self.var_index[rhs]))
+                else:
+                    ops.append((var, op, float(eval(rhs, ns)), -1))
+            if isinstance(refractory, (int, float)) and refractory > 0.9 * dt:
+                raise ValueError("refractoriness with string resets is not supported")
+            return ops, None
+        raise TypeError(f"unsupported reset {reset!r}")
+
+    # ---- state access (brian/group.py state_/__getattr__)
+    def num_states(self):
+        return len(self.var_names)
+
+    def state_(self, name):
+        return self._get_state(self._var(name), 0, self.N)
+
+    state = state_
+
+    def _get_state(self, var, off, n):
+        net = self._owner._net
+        if net is not None and net.handle is not None:
+            full = net.handle.get_state(net.gid[id(self._owner)], var, self._owner.N)
+            return full[off:off + n]
+        return self._owner._S[var, off:off + n]
+
+    def _set_state(self, var, off, n, value):
+        owner = self._owner
+        net = owner._net
+        if net is not None and net.handle is not None:
+            gid = net.gid[id(owner)]
+            full = net.handle.get_state(gid, var, owner.N)
+            full[off:off + n] = value
+            net.handle.set_state(gid, var, full)
+        else:
+            owner._S[var, off:off + n] = value
+
+    def __getattr__(self, name):
+        if name.startswith("_") or name in ("N", "var_index", "var_names", "clock"):
+            raise AttributeError(name)
+        vi = self.__dict__.get("var_index", {})
+        if name in vi:
+            return self._get_state(vi[name], self._origin, self.N)
+        if name.endswith("_") and name[:-1] in vi:
+            return self._get_state(vi[name[:-1]], self._origin, self.N)
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        vi = self.__dict__.get("var_index")
+        if vi is not None and not name.startswith("_"):
+            key = name[:-1] if name.endswith("_") and name[:-1] in vi else name
+            if key in vi and isinstance(value, TimedArray):
+                if self._owner is not self:
+                    raise NotImplementedError("TimedArray on a subgroup is not supported")
+                arr = value.arr
+                if arr.ndim == 2 and arr.shape[1] != self.N:
+                    raise ValueError("TimedArray columns must match the group size")
+                object.__setattr__(self, "_timed", (vi[key], arr))
+                return
+            if key in vi:
+                self._set_state(vi[key], self._origin, self.N, value)
+                return
+        object.__setattr__(self, name, value)
+
+    def rest(self):
+        A, C = self._eqs.linear_propagator(self.clock.dt)
+        raise NotImplementedError("rest() is not part of the accelerated path")
+
+
+class _SubGroup(NeuronGroup):
+    """A contiguous slice of an owner group (shares its state and spikes)."""
+
+    def __init__(self, owner, origin, N):
+        object.__setattr__(self, "N", int(N))
+        object.__setattr__(self, "_owner", owner)
+        object.__setattr__(self, "_origin", int(origin))
+        object.__setattr__(self, "_next_sub", 0)
+        object.__setattr__(self, "clock", owner.clock)
+        object.__setattr__(self, "var_index", owner.var_index)
+        object.__setattr__(self, "var_names", owner.var_names)
+        object.__setattr__(self, "_eqs", getattr(owner, "_eqs", None))
+
+    def _get_state(self, var, off, n):
+        return NeuronGroup._get_state(self, var, off, n)
+
+
+class SpikeGeneratorGroup(_GroupBase):
+    """brian/directcontrol.py:63-215 with FastSpikeGeneratorThreshold
+    (:219-268): spikes (i, t) are emitted at step ceil(t/dt)."""
+
+    def __init__(self, N, spiketimes, clock=None, period=None, sort=True, gather=False, level=0):
+        self._track(level + 1)
+        self.N = int(N)
+        self._owner = self
+        self._origin = 0
+        self._next_sub = 0
+        self.clock = clock or defaultclock
+        if period is not None:
+            raise NotImplementedError("periodic spike generators are not supported")
+        if isinstance(spiketimes, tuple) and len(spiketimes) == 2 and not np.isscalar(spiketimes[0]):
+            I, T = np.asarray(spiketimes[0]), np.asarray(spiketimes[1], dtype=np.float64)
+        else:
+            st = list(spiketimes)
+            I = np.array([int(i) for i, _ in st], dtype=np.int64)
+            T = np.array([float(t) for _, t in st], dtype=np.float64)
+        steps = np.array(np.ceil(T / self.clock.dt), dtype=np.int64)
+        order = np.lexsort((I, steps))
+        self._steps = steps[order]
+        self._idx = I[order]
+        self._net = None
+        self.var_index = {}
+        self.var_names = []
+
+
+class PoissonGroup(_GroupBase):
+    """Poisson sources (brian/directcontrol.py:406-447) drawn with the
+    counter-based RNG of the backend: neuron i spikes at step n iff
+    u32(seed, i, n) < floor(rate_i*dt*2^32)."""
+
+    def __init__(self, N, rates=0, clock=None, seed=None, level=0):
+        self._track(level + 1)
+        self.N = int(N)
+        self._owner = self
+        self._origin = 0
+        self._next_sub = 0
+        self.clock = clock or defaultclock
+        self.rates = np.broadcast_to(np.asarray(rates, dtype=np.float64), (self.N,)).copy()
+        self.seed = int(np.random.randint(0, 2 ** 31)) if seed is None else int(seed)
+        self._net = None
+        self.var_index = {}
+        self.var_names = []
+
+
+class TimedArray:
+    """Values per time step (brian/timedarray.py:15-234); assign as
+    ``G.I = TimedArray(values)`` or ``set_group_var_by_array``."""
+
+    def __init__(self, arr, times=None, clock=None, start=None, dt=None):
+        self.arr = np.asarray(arr, dtype=np.float64)
+        if times is not None or (start not in (None, 0)):
+            raise NotImplementedError("TimedArray with explicit times/start is not supported")
+
+
+# ------------------------------------------------------------------ connections
+def _full_csr(W, nsrc, ntgt):
+    W = sp.csr_matrix(W, shape=(nsrc, ntgt))
+    W.sum_duplicates()
+    W.sort_indices()
+    return W
+
+
+class Connection(_Tracked):
+    """brian/connections/connection.py:15-462 (sparse structure).
+
+    ``Connection(source, target, state=0, delay=0*ms, modulation=None,
+    structure='sparse', weight=None, sparseness=None, max_delay=5*ms)``.
+    A non-float ``delay`` turns it into a :class:`DelayConnection`.
+    """
+
+    def __new__(cls, source=None, target=None, state=0, delay=0 * ms, *a, **kw):
+        if cls is Connection and not isinstance(delay, float):
+            return super().__new__(DelayConnection)
+        return super().__new__(cls)
+
+    def __init__(self, source, target, state=0, delay=0 * ms, modulation=None,
+                 structure="sparse", weight=None, sparseness=None, max_delay=5 * ms, **kwds):
+        self._track(1)
+        self._init_common(source, target, state, modulation, weight, sparseness)
+        self.delay = int(float(delay) / source.clock.dt)      # connection.py:200
+        self._delay_kind = 0
+
+    def _init_common(self, source, target, state, modulation, weight, sparseness):
+        self.source = source
+        self.target = target
+        self.state = state
+        self.nstate = target._var(state) if not isinstance(state, int) else state
+        if modulation is not None:
+            raise NotImplementedError("modulation is not supported by the device backend")
+        self._W = sp.lil_matrix((len(source), len(target)))
+        self._compiled_W = None
+        self._net = None
+        self.cid = None
+        if weight is not None or sparseness is not None:
+            if weight is None:
+                weight = 1.0
+            if sparseness is None:
+                sparseness = 1
+            if sp.issparse(weight) or isinstance(weight, np.ndarray):
+                self.connect(W=weight)
+            elif sparseness == 1:
+                self.connect_full(weight=weight)
+            else:
+                self.connect_random(weight=weight, p=sparseness)
+
+    # ---- construction (brian/connections/connection.py:298-461, construction.py)
+    def _origin(self, P, Q):
+        return P._origin - self.source._origin, Q._origin - self.target._origin
+
+    def connect(self, source=None, target=None, W=None):
+        P = source if source is not None else self.source
+        Q = target if target is not None else self.target
+        i0, j0 = self._origin(P, Q)
+        W = sp.lil_matrix(W) if not sp.issparse(W) else W.tolil()
+        if W.shape != (len(P), len(Q)):
+            raise ValueError("W has the wrong shape")
+        self._W[i0:i0 + len(P), j0:j0 + len(Q)] = W
+
+    def connect_full(self, source=None, target=None, weight=1.0):
+        P = source if source is not None else self.source
+        Q = target if target is not None else self.target
+        self.connect(P, Q, np.full((len(P), len(Q)), float(weight)))
+
+    def connect_random(self, source=None, target=None, p=0.1, weight=1.0, fixed=False, seed=None,
+                       sparseness=None):
+        """Bernoulli(p) connectivity drawn on the host with numpy (the
+        reference's random_matrix stream is not reproduced)."""
+        P = source if source is not None else self.source
+        Q = target if target is not None else self.target
+        if sparseness is not None:
+            p = sparseness
+        rng = np.random.default_rng(seed) if seed is not None else np.random
+        i0, j0 = self._origin(P, Q)
+        rows, cols = [], []
+        for a in range(0, len(P), 4096):
+            b = min(len(P), a + 4096)
+            if fixed:
+                k = int(p * len(Q))
+                for i in range(a, b):
+                    c = np.sort(rng.permutation(len(Q))[:k]) if hasattr(rng, "permutation") else None
+                    rows.append(np.full(k, i))
+                    cols.append(c)
+                continue
+            mask = rng.random((b - a, len(Q))) < p
+            r, c = np.nonzero(mask)
+            rows.append(r + a)
+            cols.append(c)
+        r = np.concatenate(rows) if rows else np.zeros(0, int)
+        c = np.concatenate(cols) if cols else np.zeros(0, int)
+        if callable(weight):
+            w = np.array([weight(i, j) for i, j in zip(r, c)], dtype=np.float64)
+        else:
+            w = np.full(len(r), float(weight))
+        M = sp.coo_matrix((w, (r + i0, c + j0)), shape=self._W.shape).tolil()
+        self._W = (self._W + M).tolil() if self._W.nnz else M
+
+    def connect_one_to_one(self, source=None, target=None, weight=1.0):
+        P = source if source is not None else self.source
+        Q = target if target is not None else self.target
+        if len(P) != len(Q):
+            raise ValueError("one-to-one needs equal sizes")
+        self.connect(P, Q, sp.identity(len(P)) * float(weight))
+
+    def connect_from_sparse(self, W, column_access=True):
+        self._W = sp.lil_matrix(W)
+
+    # ---- matrix access
+    @property
+    def W(self):
+        if self._net is not None and self._net.handle is not None and self.cid is not None:
+            M = self._compiled_W.copy()
+            M.data = self._net.handle.get_weights(self.cid, M.nnz)
+            return M
+        return self._W
+
+    def __getitem__(self, ij):
+        return self.W[ij]
+
+    def __setitem__(self, ij, v):
+        if self._net is not None and self._net.handle is not None:
+            raise RuntimeError("the connection matrix is frozen once the network runs")
+        self._W[ij] = v
+
+    def compress(self):
+        if self._compiled_W is None:
+            self._compiled_W = _full_csr(self._W, len(self.source), len(self.target))
+        return self._compiled_W
+
+
+class _DelayMatrix:
+    """Dense-indexable store of per-synapse delays (seconds)."""
+
+    def __init__(self, conn, init):
+        self.conn = conn
+        self.M = sp.lil_matrix((len(conn.source), len(conn.target)))
+        self.default = float(init) if isinstance(init, (int, float)) else 0.0
+
+    def __setitem__(self, ij, v):
+        self.M[ij] = v
+
+    def __getitem__(self, ij):
+        return self.M[ij]
+
+    def values_for(self, rows, cols):
+        """Delay of each listed synapse; entries never set take the default."""
+        D = self.M.tocsr()
+        D.sort_indices()
+        out = np.full(len(rows), self.default)
+        for q, (i, j) in enumerate(zip(rows, cols)):
+            lo, hi = D.indptr[i], D.indptr[i + 1]
+            k = np.searchsorted(D.indices[lo:hi], j)
+            if k < hi - lo and D.indices[lo + k] == j:
+                out[q] = D.data[lo + k]
+        return out
+
+
+class DelayConnection(Connection):
+    """brian/connections/delayconnection.py:17-337: per-synapse delays,
+    delivered through a ring of int(max_delay/dt)+1 slots."""
+
+    def __init__(self, source, target, state=0, modulation=None, structure="sparse",
+                 weight=None, sparseness=None, delay=None, max_delay=5 * ms, **kwds):
+        self._track(1)
+        self._max_delay = float(max_delay)
+        self._delay_kind = 1
+        self._delays = None
+        self._delay_spec = delay
+        self._init_common(source, target, state, modulation, weight, sparseness)
+
+    def set_delays(self, source=None, target=None, delay=None):
+        self._delay_spec = delay
+
+    @property
+    def delay(self):
+        """Per-synapse delays in seconds, settable like ``C.delay[i, :] = d``
+        (the delayvec matrix of delayconnection.py:101-106)."""
+        if not isinstance(self._delay_spec, _DelayMatrix):
+            self._delay_spec = _DelayMatrix(self, self._delay_spec)
+        return self._delay_spec
+
+    @delay.setter
+    def delay(self, value):
+        if value is None or getattr(self, "_delay_kind", 0) == 0:
+            object.__setattr__(self, "_delay_value", value)
+            return
+        self._delay_spec = value
+
+    def _delay_steps(self, W):
+        """int(invdt * d) per synapse (delayconnection.py:123, :152)."""
+        invdt = 1 / self.target.clock._dt
+        spec = self._delay_spec
+        n = W.nnz
+        rows = np.repeat(np.arange(W.shape[0]), np.diff(W.indptr))
+        if isinstance(spec, _DelayMatrix):
+            d = spec.values_for(rows, W.indices) if n else np.zeros(0)
+        elif spec is None:
+            d = np.zeros(n)
+        elif isinstance(spec, (int, float)):
+            d = np.full(n, float(spec))
+        elif isinstance(spec, tuple) and len(spec) == 2:
+            lo, hi = float(spec[0]), float(spec[1])
+            d = lo + np.random.rand(n) * (hi - lo)
+        elif callable(spec):
+            d = np.array([float(spec(i, j)) for i, j in zip(rows, W.indices)], dtype=np.float64)
+        else:
+            D = sp.csr_matrix(spec) if sp.issparse(spec) else None
+            if D is not None:
+                d = np.asarray(D[rows, W.indices]).ravel()
+            else:
+                d = np.asarray(spec, dtype=np.float64)[rows, W.indices]
+        self._delays = d
+        return np.array(invdt * d, dtype=np.int64)
+
+    @property
+    def max_delay_steps(self):
+        return int(self._max_delay / self.target.clock.dt)
+
+
+# ------------------------------------------------------------------ STDP
+class STDP(_Tracked):
+    """Generic STDP is restricted to the exponential rule; see ExponentialSTDP."""
+
+    def __init__(self, *a, **k):
+        raise NotImplementedError("only ExponentialSTDP is supported by the device backend")
+
+
+class ExponentialSTDP(_Tracked):
+    """brian/stdp.py:452-539 (with the trace groups and updaters of
+    :181-424): traces decay with the exact 1x1 propagator, pre spikes
+    (delay = connection delay) update rows, post spikes update columns,
+    weights clipped to [wmin, wmax]."""
+
+    def __init__(self, C, taup, taum, Ap, Am, interactions="all", wmin=0, wmax=None,
+                 update="additive", delay_pre=None, delay_post=None, clock=None):
+        self._track(1)
+        if wmax is None:
+            raise AttributeError("You must specify the maximum synaptic weight")
+        if isinstance(C, DelayConnection):
+            raise NotImplementedError("STDP on DelayConnection is not supported")
+        if delay_pre is not None or delay_post is not None:
+            raise NotImplementedError("explicit delay_pre/delay_post are not supported")
+        wmax = float(wmax)
+        self.C = C
+        self.taup, self.taum = float(taup), float(taum)
+        self.interactions = {"all": 0, "nearest": 1, "nearest_pre": 2, "nearest_post": 3}[interactions]
+        if update == "additive":
+            Ap *= wmax
+            Am *= wmax
+            self.update = 0
+        elif update == "multiplicative":
+            self.update = 1
+        else:
+            raise NotImplementedError("mixed STDP updates are not supported")
+        self.Ap, self.Am = float(Ap), float(Am)
+        self.wmin, self.wmax = float(wmin), wmax
+        self.sid = None
+        self._net = None
+
+    def decays(self, dt):
+        ep = Equations("dA/dt = -A/taup : 1", taup=self.taup)
+        em = Equations("dA/dt = -A/taum : 1", taum=self.taum)
+        return ep.linear_propagator(dt)[0][0, 0], em.linear_propagator(dt)[0][0, 0]
+
+    @property
+    def traces(self):
+        net = self._net
+        n_pre, n_post = len(self.C.source), len(self.C.target)
+        return net.handle.get_stdp_traces(self.sid, n_pre, n_post)
+
+
+# ------------------------------------------------------------------ monitors
+class SpikeMonitor(_Tracked):
+    """brian/monitor.py:84-218: ``spikes`` list of (i, t), ``nspikes``,
+    ``it``, ``spiketimes``."""
+
+    def __init__(self, source, record=True, delay=0, function=None):
+        self._track(1)
+        self.source = source
+        self.record = record
+        self.function = function
+        self._net = None
+        self._base_count = 0
+
+    def _raster(self):
+        net = self._net
+        if net is None or net.handle is None:
+            return np.zeros(0, np.int64), np.zeros(0, np.int64)
+        owner = self.source._owner
+        steps, idx = net.raster(owner)
+        o, n = self.source._origin, self.source.N
+        m = (idx >= o) & (idx < o + n)
+        return steps[m], idx[m] - o
+
+    @property
+    def nspikes(self):
+        return len(self._raster()[0])
+
+    @property
+    def spikes(self):
+        s, i = self._raster()
+        dt = self.source.clock.dt
+        return list(zip(i.tolist(), (s * dt + 0.0).tolist()))
+
+    @property
+    def it(self):
+        s, i = self._raster()
+        return i, s * self.source.clock.dt + 0.0
+
+    @property
+    def spiketimes(self):
+        s, i = self._raster()
+        dt = self.source.clock.dt
+        out = {k: [] for k in range(self.source.N)}
+        for k, t in zip(i.tolist(), (s * dt).tolist()):
+            out[k].append(t)
+        return {k: np.array(v) for k, v in out.items()}
+
+    def __getitem__(self, i):
+        return self.spiketimes[i]
+
+    def reinit(self):
+        if self._net is not None:
+            self._net.clear_raster(self.source._owner)
+
+
+class PopulationSpikeCounter(SpikeMonitor):
+    """brian/monitor.py:267-295."""
+
+    def __init__(self, source, delay=0):
+        SpikeMonitor.__init__(self, source, record=False, delay=delay)
+
+
+class SpikeCounter(SpikeMonitor):
+    """Per-neuron spike counts (brian/monitor.py:225-264)."""
+
+    @property
+    def count(self):
+        s, i = self._raster()
+        return np.bincount(i, minlength=self.source.N)
+
+    def __getitem__(self, i):
+        return self.count[i]
+
+
+class PopulationRateMonitor(SpikeMonitor):
+    """brian/monitor.py:590-680: rate per bin of int(bin/dt) steps, each
+    step adding len(spikes)/(bin*dt*N) in step order."""
+
+    def __init__(self, source, bin=None):
+        SpikeMonitor.__init__(self, source)
+        dt = source.clock.dt
+        self._bin = int(bin / dt) if bin else 1
+        self._factor = 1. / float(self._bin * dt * len(source))
+
+    def _bins(self):
+        net = self._net
+        s, _ = self._raster()
+        nsteps = net.steps_run if net is not None else 0
+        counts = np.bincount(s, minlength=nsteps)[:nsteps] if nsteps else np.zeros(0, np.int64)
+        nb = (nsteps + self._bin - 1) // self._bin
+        rate = np.zeros(nb)
+        for b in range(nb):
+            r = 0.0
+            for c in counts[b * self._bin:(b + 1) * self._bin]:
+                r += c * self._factor
+            rate[b] = r
+        times = np.arange(nb) * self._bin * self.source.clock.dt
+        return times, rate
+
+    @property
+    def rate(self):
+        return self._bins()[1]
+
+    rate_ = rate
+
+    @property
+    def times(self):
+        return self._bins()[0]
+
+    times_ = times
+
+
+class StateMonitor(_Tracked):
+    """brian/monitor.py:683-990.  Recording state forces the network to run
+    one step per launch with a read back after each step; use it for short
+    diagnostic runs."""
+
+    def __init__(self, P, varname, clock=None, record=False, timestep=1, when="end"):
+        self._track(1)
+        self.P = P
+        self.varname = varname
+        self.var = P._var(varname)
+        self.timestep = int(timestep)
+        if record is True:
+            self.record = np.arange(len(P))
+        elif record is False:
+            self.record = np.zeros(0, np.int64)
+        else:
+            self.record = np.atleast_1d(np.asarray(record, dtype=np.int64))
+        self._values = []
+        self._times = []
+        self._count = 0
+
+    def _sample(self, t):
+        if self._count % self.timestep == 0:
+            v = self.P._get_state(self.var, self.P._origin, self.P.N)
+            self._values.append(v[self.record].copy())
+            self._times.append(t)
+        self._count += 1
+
+    @property
+    def values(self):
+        return np.array(self._values).T if self._values else np.zeros((len(self.record), 0))
+
+    @property
+    def times(self):
+        return np.array(self._times)
+
+    def __getitem__(self, i):
+        k = int(np.nonzero(self.record == i)[0][0])
+        return self.values[k]
+
+    def reinit(self):
+        self._values, self._times, self._count = [], [], 0
+
+
+# ------------------------------------------------------------------ network
+class NetworkOperation(_Tracked):
+    """A host callback run after every step (brian/network.py:694-721);
+    networks that contain one run one step per launch."""
+
+    def __init__(self, function, clock=None, when="end"):
+        self._track(1)
+        self.function = function
+        self.when = when
+
+    def __call__(self):
+        return self.function()
+
+
+def network_operation(*args, **kw):
+    def deco(f):
+        op = NetworkOperation(f, **kw)
+        op._track(1)
+        return op
+    if len(args) == 1 and callable(args[0]) and not kw:
+        op = NetworkOperation(args[0])
+        op._track(1)
+        return op
+    return deco
+
+
+class _StopFlag:
+    stop = False
+
+
+def stop():
+    _StopFlag.stop = True
+
+
+class Network:
+    """brian/network.py:62-648.  Objects are compiled into one device
+    network at the first ``run``; later runs continue from the current
+    step (brian/clock.py:203-206)."""
+
+    def __init__(self, *args, **kw):
+        self.groups, self.conns, self.stdps, self.monitors, self.ops = [], [], [], [], []
+        self.handle = None
+        self.gid = {}
+        self.steps_run = 0
+        self._rasters = {}
+        self.precision = kw.pop("precision", None)
+        for a in args:
+            self.add(a)
+
+    def add(self, *objs):
+        for o in objs:
+            if isinstance(o, (list, tuple, set)):
+                self.add(*o)
+            elif isinstance(o, (NeuronGroup, SpikeGeneratorGroup, PoissonGroup)):
+                g = o._owner
+                if g not in self.groups:
+                    self.groups.append(g)
+            elif isinstance(o, ExponentialSTDP):
+                if o not in self.stdps:
+                    self.stdps.append(o)
+            elif isinstance(o, Connection):
+                if o not in self.conns:
+                    self.conns.append(o)
+            elif isinstance(o, (SpikeMonitor, StateMonitor)):
+                if o not in self.monitors:
+                    self.monitors.append(o)
+            elif isinstance(o, NetworkOperation):
+                self.ops.append(o)
+            elif isinstance(o, Network):
+                self.add(o.groups, o.conns, o.stdps, o.monitors, o.ops)
+        return self
+
+    # ---- compilation
+    def _compile(self):
+        prec = self.precision or _PREFS["precision"]
+        dt = defaultclock.dt
+        for g in self.groups:
+            if abs(g.clock.dt - dt) > 1e-15:
+                raise NotImplementedError("all objects must share one clock")
+        for c in self.conns:
+            for P in (c.source._owner, c.target._owner):
+                if P not in self.groups:
+                    self.groups.append(P)
+        for m in self.monitors:
+            src = m.source._owner if isinstance(m, SpikeMonitor) else m.P._owner
+            if src not in self.groups:
+                self.groups.append(src)
+        h = _lib.NetHandle(dt, prec)
+        self.handle = h
+        state_mon = any(isinstance(m, StateMonitor) for m in self.monitors)
+        for g in self.groups:
+            if isinstance(g, SpikeGeneratorGroup):
+                gid = h.add_generator(g.N, g._steps, g._idx)
+            elif isinstance(g, PoissonGroup):
+                gid = h.add_poisson(g.N, g.rates, g.seed)
+            else:
+                gid = h.add_group(g._S)
+                self._compile_neurons(h, gid, g, dt)
+            self.gid[id(g)] = gid
+            g._net = self
+        recorded = set()
+        for m in self.monitors:
+            if isinstance(m, SpikeMonitor):
+                recorded.add(id(m.source._owner))
+                m._net = self
+        for c in self.conns:
+            W = c.compress()
+            src, tgt = c.source._owner, c.target._owner
+            kw = {}
+            if c._delay_kind == 1:
+                kw = {"syn_delay": c._delay_steps(W), "max_delay_steps": c.max_delay_steps}
+            else:
+                kw = {"delay_steps": c.delay}
+            c.cid = h.add_connection(self.gid[id(src)], c.source._origin, len(c.source),
+                                     self.gid[id(tgt)], c.target._origin, len(c.target), c.nstate,
+                                     W.indptr, W.indices, W.data, **kw)
+            c._net = self
+        for s in self.stdps:
+            dp, dm = s.decays(dt)
+            s.sid = h.add_stdp(s.C.cid, dp, dm, s.Ap, s.Am, s.wmin, s.wmax, s.update, s.interactions)
+            s._net = self
+        self._capacity = {}
+        for g in self.groups:
+            if id(g) in recorded:
+                cap = max(1 << 20, int(g.N * self._planned_steps * dt * _PREFS["max_rate_hz"]))
+                self._capacity[id(g)] = cap
+                h.record(self.gid[id(g)], cap)
+        window = 1 if (state_mon or self.ops) else _PREFS["window"]
+        h.prepare(window=window, neurons_per_thread=_PREFS["neurons_per_thread"],
+                  use_graph=_PREFS["use_graph"])
+
+    def _compile_neurons(self, h, gid, g, dt):
+        eqs = g._eqs
+        implicit = g._implicit
+        linear = (g._method in (None, "linear")) and not implicit and eqs.is_linear()
+        if linear:
+            A, C = eqs.linear_propagator(dt)
+            h.set_linear(gid, A, C)
+        elif len(g.var_names):
+            code, consts, nregs = eqs.compile_program()
+            scheme = 3 if implicit else (2 if g._order == 2 else 1)
+            h.set_program(gid, scheme, nregs, code, consts)
+        th = g._threshold
+        if th is not None:
+            if th[0] == "const":
+                h.set_threshold(gid, th[1], th[2])
+            elif th[0] == "var":
+                h.set_threshold(gid, th[1], 0.0, th[2])
+            else:
+                h.set_empirical_threshold(gid, th[1], th[2], th[3])
+        for var, op, val, vvar in g._reset:
+            h.add_reset_op(gid, var, op, val, vvar)
+        if g._refractory is not None and g._reset:
+            kind, val = g._refractory
+            if kind == "const":
+                h.set_refractory(gid, int(val / dt) + 1, val - 0.5 * dt)
+            else:
+                h.set_refractory(gid, int(np.max(val) / dt) + 1, np.max(val) - 0.5 * dt, per_neuron=val)
+        ti = getattr(g, "_timed", None)
+        if ti is not None:
+            var, arr = ti
+            h.set_timed_input(gid, var, arr, per_neuron=arr.ndim == 2)
+
+    # ---- running
+    def run(self, duration, report=None, report_period=10 * second, threads=1):
+        dt = defaultclock.dt
+        nsteps = int(np.ceil(float(duration) / dt))    # clock.py:203-206
+        if self.handle is None:
+            self._planned_steps = max(nsteps, 1)
+            self._compile()
+        state_mons = [m for m in self.monitors if isinstance(m, StateMonitor)]
+        _StopFlag.stop = False
+        if state_mons or self.ops:
+            for _ in range(nsteps):
+                t = defaultclock.t
+                self.handle.run(1)
+                self.steps_run += 1
+                defaultclock._n += 1
+                for m in state_mons:
+                    m._sample(t)
+                for op in self.ops:
+                    op()
+                if _StopFlag.stop:
+                    break
+        else:
+            self.handle.run(nsteps)
+            self.handle.sync()
+            self.steps_run += nsteps
+            defaultclock._n += nsteps
+        self._rasters = {}
+
+    def raster(self, owner):
+        key = id(owner)
+        if key not in self._rasters:
+            self._rasters[key] = self.handle.get_spikes(self.gid[key])
+        return self._rasters[key]
+
+    def clear_raster(self, owner):
+        self.handle.clear_spikes(self.gid[id(owner)])
+        self._rasters.pop(id(owner), None)
+
+    def reinit(self):
+        raise NotImplementedError("reinit: build a new Network")
+
+    def stop(self):
+        _StopFlag.stop = True
+
+
+class MagicNetwork(Network):
+    """Collects the tracked objects created in the calling frame
+    (brian/network.py:843-898, brian/magic.py:310-320)."""
+
+    def __init__(self, verbose=False, level=1):
+        frame = inspect.currentframe()
+        try:
+            f = frame
+            for _ in range(level + 1):
+                f = f.f_back
+            fid = id(f)
+        finally:
+            del frame
+        objs = []
+        for r in _Tracked._instances:
+            o = r()
+            if o is not None and getattr(o, "_frame_id", None) == fid:
+                objs.append(o)
+        Network.__init__(self, *objs)
+
+
+_magic_cache = {}
+
+
+def run(duration, threads=1, report=None, report_period=10 * second):
+    """brian/network.py:901-938: build a MagicNetwork from the caller's frame
+    (reused by later run() calls from the same frame) and run it."""
+    frame = inspect.currentframe().f_back
+    fid = id(frame)
+    del frame
+    objs = [r() for r in _Tracked._instances]
+    objs = [o for o in objs if o is not None and getattr(o, "_frame_id", None) == fid]
+    key = (fid, tuple(sorted(id(o) for o in objs)))
+    net = _magic_cache.get(key)
+    if net is None:
+        net = Network(*objs)
+        _magic_cache.clear()
+        _magic_cache[key] = net
+    net.run(duration, report=report, report_period=report_period)
+    return net
+
+
+def reinit(states=True):
+    """Forget the magic network; the next run() rebuilds it."""
+    _magic_cache.clear()
+    reinit_default_clock()

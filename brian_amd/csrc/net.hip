@@ -1,6 +1,6 @@
 // Host side of the network engine and its C-ABI (include/brianmi.h).
 #include "../../include/brianmi.h"
-#include "net_kernels.hpp"
+#include "net_launch.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -148,49 +148,6 @@ void require_unprepared(bm_net *net) {
   BM_REQUIRE(!net->prepared, "network already prepared: structure is frozen");
 }
 
-using SweepFn = void (*)(const GroupDev, const int64_t *, int64_t, int, int, int);
-
-template <typename T, int M>
-SweepFn pick_gen(int K) {
-  switch (K) {
-    case 1: return k_sweep<T, M, 1, F_GEN>;
-    default: return k_sweep<T, M, 4, F_GEN>;
-  }
-}
-
-template <typename T, int M>
-SweepFn pick_fast(int K) {
-  switch (K) {
-    case 1: return k_sweep<T, M, 1, 0>;
-    case 2: return k_sweep<T, M, 2, 0>;
-    case 4: return k_sweep<T, M, 4, 0>;
-    default: return k_sweep<T, M, 8, 0>;
-  }
-}
-
-template <typename T>
-SweepFn pick_sweep(int M, int K, bool fast) {
-  if (fast) {
-    switch (M) {
-      case 1: return pick_fast<T, 1>(K);
-      case 2: return pick_fast<T, 2>(K);
-      case 3: return pick_fast<T, 3>(K);
-      default: return pick_fast<T, 4>(K);
-    }
-  }
-  switch (M) {
-    case 0: return pick_gen<T, 0>(K);
-    case 1: return pick_gen<T, 1>(K);
-    case 2: return pick_gen<T, 2>(K);
-    case 3: return pick_gen<T, 3>(K);
-    case 4: return pick_gen<T, 4>(K);
-    case 5: return pick_gen<T, 5>(K);
-    case 6: return pick_gen<T, 6>(K);
-    case 7: return pick_gen<T, 7>(K);
-    default: return pick_gen<T, 8>(K);
-  }
-}
-
 // The lean kernel covers: shared linear propagator, 1..4 variables, constant
 // or variable threshold, reset ops without refractoriness, at most one
 // (packed count) ring, no timed input.
@@ -214,11 +171,8 @@ void launch_group(bm_net *net, HostGroup &g, int64_t rel0, int W, int skip_first
   const int64_t *dbase = net->d_base.as<int64_t>();
   if (g.kind == GK_GENERATOR) {
     if (tail_only) return;
-    const int blocks = 64;
-    if (net->precision == 64)
-      hipLaunchKernelGGL(k_generator<double>, dim3(blocks), dim3(BLOCK), 0, net->stream, g.dev, dbase, rel0, W);
-    else
-      hipLaunchKernelGGL(k_generator<float>, dim3(blocks), dim3(BLOCK), 0, net->stream, g.dev, dbase, rel0, W);
+    if (net->precision == 64) launch_generator<double>(g.dev, net->stream, dbase, rel0, W);
+    else launch_generator<float>(g.dev, net->stream, dbase, rel0, W);
     return;
   }
   const int K = group_K(net, g);
@@ -228,32 +182,20 @@ void launch_group(bm_net *net, HostGroup &g, int64_t rel0, int W, int skip_first
   size_t shmem = sizeof(uint16_t) * BLOCK * K * std::max(W, 1) + 16;
   if (g.upd == UPD_PROGRAM) shmem += (size_t)g.prog_nregs * BLOCK * net->tsize();
   const bool fast = fast_eligible(g);
-  SweepFn fn = net->precision == 64 ? pick_sweep<double>(g.M, K, fast) : pick_sweep<float>(g.M, K, fast);
-  hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(BLOCK), shmem, net->stream, g.dev, dbase, rel0,
-                     W, skip_first_tail, tail_only);
+  SweepArgs a{dbase, rel0, W, skip_first_tail, tail_only};
+  if (net->precision == 64) launch_sweep<double>(g.dev, fast, g.M, K, (unsigned)blocks, shmem, net->stream, a);
+  else launch_sweep<float>(g.dev, fast, g.M, K, (unsigned)blocks, shmem, net->stream, a);
 }
 
 void launch_stdp_step(bm_net *net, int64_t rel0) {
   const int64_t *dbase = net->d_base.as<int64_t>();
   for (auto &sp : net->stdps) {
-    HostStdp &s = *sp;
-    const int64_t tot = s.dev.src_n + s.dev.tgt_n;
-    const unsigned b = (unsigned)((tot + BLOCK - 1) / BLOCK);
-    const unsigned wb = 128;
-    if (net->precision == 64) {
-      hipLaunchKernelGGL(k_stdp_decay<double>, dim3(b), dim3(BLOCK), 0, net->stream, s.dev);
-      hipLaunchKernelGGL(k_stdp_pre<double>, dim3(wb), dim3(BLOCK), 0, net->stream, s.dev, dbase, rel0);
-      hipLaunchKernelGGL(k_stdp_post<double>, dim3(wb), dim3(BLOCK), 0, net->stream, s.dev, dbase, rel0);
-    } else {
-      hipLaunchKernelGGL(k_stdp_decay<float>, dim3(b), dim3(BLOCK), 0, net->stream, s.dev);
-      hipLaunchKernelGGL(k_stdp_pre<float>, dim3(wb), dim3(BLOCK), 0, net->stream, s.dev, dbase, rel0);
-      hipLaunchKernelGGL(k_stdp_post<float>, dim3(wb), dim3(BLOCK), 0, net->stream, s.dev, dbase, rel0);
-    }
+    if (net->precision == 64) launch_stdp<double>(sp->dev, net->stream, dbase, rel0);
+    else launch_stdp<float>(sp->dev, net->stream, dbase, rel0);
   }
   if (net->nhist > 0)
-    hipLaunchKernelGGL(k_hist_clear, dim3(1), dim3(64), 0, net->stream,
-                       net->hist_ptrs_d.as<unsigned int *>(), net->hist_H_d.as<int>(), net->nhist, dbase,
-                       rel0 + 1);
+    launch_hist_clear(net->hist_ptrs_d.as<unsigned int *>(), net->hist_H_d.as<int>(), net->nhist, dbase,
+                      rel0 + 1, net->stream);
 }
 
 // One window of W steps starting at base+rel0 for every group (+ STDP when W==1).
@@ -621,8 +563,7 @@ void build_graph(bm_net *net, int windows) {
   hipGraph_t graph;
   BM_HIP(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < windows; ++k) launch_window(net, (int64_t)k * net->W, net->W, 0);
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, net->stream, net->d_base.as<int64_t>(),
-                     (int64_t)windows * net->W);
+  launch_advance(net->d_base.as<int64_t>(), (int64_t)windows * net->W, net->stream);
   hipError_t e = hipStreamEndCapture(net->stream, &graph);
   BM_HIP(e);
   e = hipGraphInstantiate(&net->graph_exec, graph, nullptr, nullptr, 0);
@@ -658,7 +599,7 @@ void run(bm_net *net, int64_t nsteps) {
     launch_window(net, 0, w, 1);
     prof_end();
     check_launch();
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, net->stream, net->d_base.as<int64_t>(), (int64_t)w);
+    launch_advance(net->d_base.as<int64_t>(), (int64_t)w, net->stream);
     done += w;
   }
   const int64_t remaining_windows = (nsteps - done) / W;
@@ -676,7 +617,7 @@ void run(bm_net *net, int64_t nsteps) {
     launch_window(net, 0, W, 0);
     prof_end();
     check_launch();
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, net->stream, net->d_base.as<int64_t>(), (int64_t)W);
+    launch_advance(net->d_base.as<int64_t>(), (int64_t)W, net->stream);
   }
   done += remaining_windows * W;
   if (done < nsteps) {
@@ -685,7 +626,7 @@ void run(bm_net *net, int64_t nsteps) {
     launch_window(net, 0, w, 0);
     prof_end();
     check_launch();
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, net->stream, net->d_base.as<int64_t>(), (int64_t)w);
+    launch_advance(net->d_base.as<int64_t>(), (int64_t)w, net->stream);
     done += w;
   }
   // tail of the last step: deliveries and resets so the state matches Brian's

@@ -19,107 +19,9 @@
 // without any inter-workgroup synchronisation: nothing it produces is
 // consumed before the next launch.
 #pragma once
-#include "common.hpp"
+#include "net_types.hpp"
 
 namespace bm {
-
-constexpr int MAXM = 8;
-constexpr int MAXRING = 4;
-constexpr int MAXCHAN = 8;
-constexpr int MAXOUT = 8;
-constexpr int MAXRESET = 4;
-constexpr int BLOCK = 256;
-constexpr int WMAX = 16;
-constexpr int32_t NEVER = INT32_MIN;
-
-enum RingKind { RING_COUNT = 0, RING_FLOAT = 1 };
-enum UpdKind { UPD_NONE = 0, UPD_LINEAR = 1, UPD_PERNEURON = 2, UPD_PROGRAM = 3 };
-enum GroupKind { GK_NEURON = 0, GK_POISSON = 1, GK_GENERATOR = 2 };
-enum ThKind { TH_NONE = 0, TH_CONST = 1, TH_VAR = 2, TH_EMPIRICAL = 3 };
-enum RefrKind { REFR_NONE = 0, REFR_CONST = 1, REFR_ARRAY = 2 };
-enum Op {
-  OP_MOV = 0, OP_CONST, OP_ADD, OP_SUB, OP_MUL, OP_DIV, OP_NEG, OP_EXP, OP_LOG,
-  OP_SQRT, OP_POW, OP_ABS, OP_TIME, OP_SIN, OP_COS, OP_TANH, OP_CLIPLO, OP_CLIPHI,
-  OP_GT, OP_LT, OP_WHERE
-};
-
-struct RingDev {
-  void *base;           // D slots of slot_bytes each
-  int64_t slot_bytes;   // bytes per slot (multiple of 256)
-  int kind;             // RingKind
-  int nch;              // COUNT: channels packed per neuron
-  int bits;             // COUNT: bits per count (8 or 16)
-  int sw;               // bytes per neuron in a slot (1, 2, 4 for COUNT; sizeof(T) FLOAT)
-};
-
-struct ChanDev {        // one incoming connection, applied in reference order
-  int ring;
-  int ch;
-  int var;
-  int is_delay;         // DelayConnection: sum into the slot, then one add
-  double w;             // COUNT weight
-};
-
-struct OutDev {         // one outgoing connection of this group
-  int64_t src_off, src_n;
-  const int64_t *row_ptr;
-  const int32_t *col;
-  const void *w;        // T[nnz] (FLOAT rings)
-  const void *dly;      // per-synapse delay steps (u8 / u16) or null
-  int dly_bytes;        // 0 homogeneous, 1 u8, 2 u16
-  int hom_delay;
-  void *ring;
-  int64_t slot_bytes;
-  int kind, bits, nch, ch, D, pad;
-  int64_t tgt_off;
-};
-
-struct ProgDev {
-  const int32_t *code;  // [M dreg][ninstr x 4]
-  const double *consts;
-  int ninstr, nregs, scheme, pad;
-};
-
-struct GroupDev {
-  int64_t N;
-  int M, kind, upd, pad0;
-  void *S;               // [M][N] T
-  int32_t *last;         // [N] last spike step (NEVER if none)
-  double A[MAXM * MAXM];
-  double C[MAXM];
-  const void *Apn;       // [M][M][N] T
-  const void *Cpn;       // [M][N] T
-  ProgDev prog;
-  int th_kind, th_var, th_vtvar, th_period;
-  double vt;
-  int nreset, pad1;
-  int rs_var[MAXRESET], rs_op[MAXRESET], rs_vvar[MAXRESET];
-  double rs_val[MAXRESET];
-  int refr, period;
-  double refr_time;
-  const double *refr_arr;
-  int use_next_allowed, D;
-  int nring, nchan;
-  RingDev rings[MAXRING];
-  ChanDev chans[MAXCHAN];
-  int nout, H;
-  OutDev outs[MAXOUT];
-  unsigned long long *raster;
-  unsigned long long *raster_count;
-  int64_t raster_cap;
-  int32_t *hist;              // [H][N] spike lists (STDP), null if unused
-  unsigned int *hist_count;   // [H]
-  const uint64_t *pthr;       // Poisson thresholds [N]
-  uint64_t seed;
-  int ti_var, ti_per_neuron;
-  int64_t ti_nt;
-  const void *ti_vals;        // T[nt] or T[nt][N]
-  double dt;
-  // generator
-  const int32_t *gen_idx;
-  const int64_t *gen_off;     // [gen_nsteps+1]
-  int64_t gen_nsteps;
-};
 
 // ------------------------------------------------------------------ helpers
 __device__ inline int lane_id() { return __lane_id(); }
@@ -741,25 +643,7 @@ __global__ __launch_bounds__(BLOCK) void k_generator(const GroupDev g, const int
 }
 
 // ------------------------------------------------------------------ STDP
-struct StdpDev {
-  int64_t src_off, src_n, tgt_off, tgt_n;
-  const int64_t *row_ptr;
-  const int32_t *col;
-  void *w;                    // T[nnz]
-  const int64_t *col_ptr;     // [tgt_n+1]
-  const int64_t *col_perm;    // [nnz] index into w, ascending source within column
-  const int32_t *col_row;     // [nnz] source row of that synapse
-  void *a_pre, *a_post;       // T[src_n], T[tgt_n]
-  double decay_pre, decay_post, Ap, Am, wmin, wmax;
-  int update, interactions, pre_delay, pad;
-  const int32_t *pre_hist;    // source group's history [H][Nsrc]
-  const unsigned int *pre_hist_count;
-  int pre_H;
-  int post_H;
-  const int32_t *post_hist;
-  const unsigned int *post_hist_count;
-  int64_t pre_N, post_N;      // group sizes (history row stride)
-};
+
 
 // Trace decay: the 1x1 LinearStateUpdater of the STDP trace groups
 // (brian/stdp.py:282-287): A = decay*A + 0.
@@ -846,6 +730,7 @@ __global__ __launch_bounds__(BLOCK) void k_stdp_post(const StdpDev sd, const int
   }
 }
 
+#ifdef BM_COMMON_KERNELS
 // Clears the history counter of the slot the next step will write.
 __global__ void k_hist_clear(unsigned int *const *counts, const int *H, int ngroups,
                              const int64_t *d_base, int64_t rel_next) {
@@ -859,5 +744,7 @@ __global__ void k_hist_clear(unsigned int *const *counts, const int *H, int ngro
 __global__ void k_advance(int64_t *d_base, int64_t by) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *d_base += by;
 }
+
+#endif  // BM_COMMON_KERNELS
 
 }  // namespace bm

@@ -96,7 +96,10 @@ def linear_propagator(f, n, dt):
         for j in range(n):
             M[j, i] = fi[j] + AB[j, 0]
     try:
-        B = scipy.linalg.solve(M, AB)
+        with np.errstate(all="ignore"):
+            B = scipy.linalg.solve(M, AB)
+        if not np.all(np.isfinite(B)):      # singular M (scipy may return inf/nan)
+            raise np.linalg.LinAlgError("singular matrix")
         A = scipy.linalg.expm(M * dt)
         C = -np.dot(A, B) + B
         return np.array(A, order="C"), np.array(C[:, 0])

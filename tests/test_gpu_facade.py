@@ -1,0 +1,179 @@
+"""Brian-API tests on the GPU, restating the reference's own known-answer
+tests (brian/tests/...) through ``from brian_amd import *``."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _fresh():
+    import brian_amd as b
+    b.reinit()
+    b.set_global_preferences(precision=64, window=0)
+    yield
+
+
+def test_tutorial1c_840_spikes():
+    """brian/tests/testcorrectness/test_from_tutorial1c.py:3-39."""
+    from brian_amd import NeuronGroup, SpikeMonitor, run, msecond, mvolt, second
+    tau = 20 * msecond
+    Vt = -50 * mvolt
+    Vr = -60 * mvolt
+    El = -49 * mvolt
+    G = NeuronGroup(N=40, model='dV/dt = -(V-El)/tau : volt # membrane potential',
+                    threshold=Vt, reset=Vr)
+    G.V = El
+    M = SpikeMonitor(G)
+    run(1 * second)
+    assert M.nspikes == 840
+
+
+def test_exponential_current_closed_form():
+    """brian/tests/testcorrectness/test_exponential_current.py:3-58."""
+    from brian_amd import (NeuronGroup, SpikeGeneratorGroup, Connection, StateMonitor, run,
+                           ms, mV, exp)
+    taum, taue, taui = 20 * ms, 1 * ms, 10 * ms
+    Vt, Vr = 10 * mV, 0 * mV
+    G1 = SpikeGeneratorGroup(2, [(0, 0 * ms)])
+    G2 = NeuronGroup(N=1, model='''
+                   dV/dt = (-V+ge-gi)/taum : volt
+                   dge/dt = -ge/taue : volt
+                   dgi/dt = -gi/taui : volt
+                   ''', threshold=Vt, reset=Vr)
+    G2.V = Vr
+    C1 = Connection(G1, G2, 'ge')
+    C2 = Connection(G1, G2, 'gi')
+    C1[0, 0] = 3 * mV
+    C2[1, 0] = 3 * mV
+    Mv = StateMonitor(G2, 'V', record=True)
+    run(100 * ms)
+    t = Mv.times
+    Vpred = (exp(-t / taum) - exp(-t / taue)) * taue * (3 * mV) / (taum - taue)
+    assert np.max(np.abs(Vpred - Mv[0])) < 0.00001
+
+
+def test_epsp_peak():
+    """brian/tests/testcorrectness/test_epsp.py:3-62."""
+    from brian_amd import (NeuronGroup, SpikeGeneratorGroup, Connection, StateMonitor, run,
+                           ms, mV, mvolt, msecond)
+    taum, taupsp, y0 = 10 * ms, 0.325 * ms, 4.86 * mV
+    P = NeuronGroup(N=1, model='''
+                  dV/dt = (-V+x)*(1./taum) : volt
+                  dx/dt = (-x+y)*(1./taupsp) : volt
+                  dy/dt = -y*(1./taupsp) : volt
+                  ''', threshold=100 * mV, reset=0 * mV)
+    Pinit = SpikeGeneratorGroup(1, [(0, 0 * ms)])
+    C = Connection(Pinit, P, 'y')
+    C.connect_full(Pinit, P, y0)
+    M = StateMonitor(P, 'V', record=0)
+    run(10 * ms)
+    V = M[0]
+    i = int(np.argmax(V))
+    assert abs(V[i] - 0.136889 * mvolt) < 0.001 * mvolt
+    assert abs(M.times[i] - 1.69735 * msecond) < 0.1 * ms
+
+
+def test_propagation_structures():
+    """brian/tests/testfeatures/test_propagation.py:3-35 (the non-modulation
+    cases: sparse/dense/dynamic, with and without delays)."""
+    from brian_amd import NeuronGroup, Connection, SpikeMonitor, run, ms, defaultclock, arange
+    H = NeuronGroup(1, 'V:1\nmod:1', reset=0, threshold=1)
+    H.V = 2
+    H.mod = 1
+    G = [NeuronGroup(10, 'V:1', reset=0, threshold=1) for _ in range(6)]
+    M = [SpikeMonitor(g) for g in G]
+    C0 = Connection(H, G[0], 'V', weight=2)
+    C1 = Connection(H, G[1], 'V', weight=2, structure='dense')
+    C2 = Connection(H, G[2], 'V', weight=2, structure='dynamic')
+    C6 = Connection(H, G[3], 'V', weight=2, delay=True)
+    C7 = Connection(H, G[4], 'V', weight=2, delay=True, structure='dense')
+    C8 = Connection(H, G[5], 'V', weight=2, delay=True, structure='dynamic')
+    for c in [C6, C7, C8]:
+        c.delay[0, :] = arange(10) * defaultclock.dt + defaultclock.dt / 2
+    run(2 * ms)
+    for k, m in enumerate(M):
+        assert len(m.spikes), k
+        i, j = zip(*m.spikes)
+        i = np.array(i)
+        j = np.array((np.array(j) + defaultclock.dt / 2) / defaultclock.dt, dtype=int)
+        assert (i == np.arange(10)).all(), k
+        if k < 3:
+            assert (j == 1).all(), k
+        else:
+            assert (j == (1 + np.arange(10))).all(), (k, j)
+
+
+def test_refractoriness_clamp():
+    """brian/tests/testinterface/test_reset.py:126-141."""
+    from brian_amd import NeuronGroup, Network, Refractoriness, msecond
+    G = NeuronGroup(1, model='dV/dt=-V/(.1*msecond):1.', threshold=1.,
+                    reset=Refractoriness(-.5, 1 * msecond))
+    G.V = 15.
+    net = Network(G)
+    net.run(0.5 * msecond)
+    assert np.all(G.state(0) == -.5)
+    net.run(1 * msecond)
+    v = G.state(0)
+    assert np.all((-0.4 < v) & (v < 0.1))
+
+
+def test_threshold_strict_and_ascending():
+    """brian/tests/testinterface/test_threshold.py:20-60: strict '>' and
+    ascending spike indices within a step."""
+    from brian_amd import NeuronGroup, SpikeMonitor, Network, ms
+    G = NeuronGroup(6, model='V : 1', threshold=1.0, reset=0.0)
+    G.V = np.array([1.0, 1.5, 0.5, 2.0, 1.0 + 1e-12, 1.0])
+    M = SpikeMonitor(G)
+    Network(G, M).run(0.1 * ms)
+    i, t = M.it
+    assert list(i) == [1, 3, 4]
+    assert np.all(t == 0.0)
+
+
+def test_stdp_potentiation_sign():
+    """brian/tests/testinterface/test_stdp.py:6-37 with ExponentialSTDP:
+    a pre spike ~2 ms before a post spike potentiates the synapse."""
+    from brian_amd import (NeuronGroup, SpikeGeneratorGroup, Connection, ExponentialSTDP, Network,
+                           ms, mV)
+    inp = SpikeGeneratorGroup(1, [(0, 1 * ms)])
+    Ee, El = 0 * mV, -80 * mV
+    G = NeuronGroup(1, model='''
+    dv/dt = (ge * (Ee - v) + (El - v)) / (5 * ms) : volt
+    dge/dt = -ge / (5 * ms) : 1
+    ''', threshold=-50 * mV, reset=El)
+    G.v = -51 * mV
+    con = Connection(inp, G, 'ge', weight=1)
+    stdp = ExponentialSTDP(con, 10 * ms, 10 * ms, 0.1, 0.1, wmax=10.0)
+    net = Network(inp, G, con, stdp)
+    net.run(4 * ms)
+    assert con.W[0, 0] > 1
+
+
+def test_facade_cuba_matches_oracle():
+    """examples/misc/CUBA.py through the Brian API (fixed host connectivity)
+    against the oracle: bit-exact raster in FP64 check mode."""
+    from brian_amd import NeuronGroup, Connection, SpikeMonitor, Network, ms, mV
+    from brian_amd import workloads as W
+    from tests.netbuild import oracle_cuba
+    import scipy.sparse as sps
+    d = W.cuba(4000, seed=21)
+    P = NeuronGroup(4000, model=W.CUBA_EQS, threshold=W.Vt, reset=W.Vr)
+    P.v = d["S0"][0]
+    Pe = P.subgroup(d["Ne"])
+    Pi = P.subgroup(d["Ni"])
+    ipe, ixe, we = d["exc"]
+    ipi, ixi, wi = d["inh"]
+    Ce = Connection(Pe, P, 'ge', weight=sps.csr_matrix((we, ixe, ipe), shape=(d["Ne"], 4000)))
+    Ci = Connection(Pi, P, 'gi', weight=sps.csr_matrix((wi, ixi, ipi), shape=(d["Ni"], 4000)))
+    M = SpikeMonitor(P)
+    net = Network(P, Ce, Ci, M)
+    net.run(300 * ms)
+    i, t = M.it
+    onet, gid = oracle_cuba(d)
+    onet.run(3000)
+    rs, ri = onet.raster(gid)
+    assert len(ri) > 1000
+    np.testing.assert_array_equal(i, ri)
+    np.testing.assert_array_equal(np.rint(t / 1e-4).astype(int), rs)
+    np.testing.assert_allclose(P.v, onet.groups[gid].S[0], rtol=1e-12, atol=1e-15)
