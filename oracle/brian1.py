@@ -153,7 +153,8 @@ class Group:
              ('euler', f) | ('rk2', f) | ('expeuler', f) | None
         f(S) -> list of derivative arrays (one per variable, None for a
         parameter with zero derivative); S is the (m, N) state.
-    threshold: None | ('const', var, Vt) | ('var', var, var_t)
+    threshold: None | ('const', var, Vt) | ('var', var, var_t) |
+               ('empirical', var, Vt, period_steps)
     reset: None | list of (var, op, value, is_var) with op in {'=', '+='}
     refractory: None | float seconds | per-neuron float array (variable)
     refractory_var/reset var: Refractoriness clamps ``reset[0]``'s variable.
@@ -236,7 +237,14 @@ class Group:
                 raise ValueError(kind)
         spikes = np.zeros(0, dtype=np.int64)
         th = self.threshold
-        if th is not None:                   # threshold.py:160-191, :373-396
+        if th is not None and th[0] == "empirical":   # threshold.py:409-457
+            cond = self.S[th[1]] > th[2]
+            recent = [self.get_spikes(k) for k in range(th[3])]
+            if recent:
+                cond[np.concatenate(recent)] = False
+            spikes = cond.nonzero()[0]
+            th = None
+        elif th is not None:                 # threshold.py:160-191, :373-396
             v = self.S[th[1]]
             vt = th[2] if th[0] == "const" else self.S[th[2]]
             spikes = (v > vt).nonzero()[0]

@@ -153,12 +153,13 @@ def test_stdp_potentiation_sign():
 def test_facade_cuba_matches_oracle():
     """examples/misc/CUBA.py through the Brian API (fixed host connectivity)
     against the oracle: bit-exact raster in FP64 check mode."""
-    from brian_amd import NeuronGroup, Connection, SpikeMonitor, Network, ms, mV
+    from brian_amd import NeuronGroup, Connection, SpikeMonitor, Network, Equations, ms, mV
     from brian_amd import workloads as W
     from tests.netbuild import oracle_cuba
     import scipy.sparse as sps
     d = W.cuba(4000, seed=21)
-    P = NeuronGroup(4000, model=W.CUBA_EQS, threshold=W.Vt, reset=W.Vr)
+    eqs = Equations(W.CUBA_EQS, El=W.El, taum=W.taum, taue=W.taue, taui=W.taui)
+    P = NeuronGroup(4000, model=eqs, threshold=W.Vt, reset=W.Vr)
     P.v = d["S0"][0]
     Pe = P.subgroup(d["Ne"])
     Pi = P.subgroup(d["Ni"])
@@ -177,3 +178,67 @@ def test_facade_cuba_matches_oracle():
     np.testing.assert_array_equal(i, ri)
     np.testing.assert_array_equal(np.rint(t / 1e-4).astype(int), rs)
     np.testing.assert_allclose(P.v, onet.groups[gid].S[0], rtol=1e-12, atol=1e-15)
+
+
+def _cobahh_facade(steps):
+    """dev/tests/correctness/test_optimizations.py:48-128 through the Brian API."""
+    import os
+    import scipy.sparse as sps
+    from brian_amd import (NeuronGroup, Connection, SpikeMonitor, Network, Equations,
+                           EmpiricalThreshold, umetre, ufarad, cm, siemens, msiemens, mV, ms, nS,
+                           exp, arange, second)
+    area = 20000 * umetre ** 2
+    Cm = (1 * ufarad * cm ** -2) * area
+    gl = (5e-5 * siemens * cm ** -2) * area
+    El, EK, ENa = -60 * mV, -90 * mV, 50 * mV
+    g_na = (100 * msiemens * cm ** -2) * area
+    g_kd = (30 * msiemens * cm ** -2) * area
+    VT = -63 * mV
+    taue, taui, Ee, Ei = 5 * ms, 10 * ms, 0 * mV, -80 * mV
+    eqs = Equations('''
+    dv/dt = (gl*(El-v)+ge*(Ee-v)+gi*(Ei-v)-g_na*(m*m*m)*h*(v-ENa)-g_kd*(n*n*n*n)*(v-EK))/Cm : volt
+    dm/dt = alpham*(1-m)-betam*m : 1
+    dn/dt = alphan*(1-n)-betan*n : 1
+    dh/dt = alphah*(1-h)-betah*h : 1
+    dge/dt = -ge*(1./taue) : siemens
+    dgi/dt = -gi*(1./taui) : siemens
+    alpham = 0.32*(mV**-1)*(13*mV-v+VT)/(exp((13*mV-v+VT)/(4*mV))-1.)/ms : Hz
+    betam = 0.28*(mV**-1)*(v-VT-40*mV)/(exp((v-VT-40*mV)/(5*mV))-1)/ms : Hz
+    alphah = 0.128*exp((17*mV-v+VT)/(18*mV))/ms : Hz
+    betah = 4./(1+exp((40*mV-v+VT)/(5*mV)))/ms : Hz
+    alphan = 0.032*(mV**-1)*(15*mV-v+VT)/(exp((15*mV-v+VT)/(5*mV))-1.)/ms : Hz
+    betan = .5*exp((10*mV-v+VT)/(40*mV))/ms : Hz
+    ''')
+    P = NeuronGroup(4000, model=eqs, threshold=EmpiricalThreshold(threshold=-20 * mV, refractory=3 * ms),
+                    implicit=True, freeze=True)
+    Pe = P.subgroup(3200)
+    Pi = P.subgroup(800)
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "cobahh.npz"))
+    Ce = Connection(Pe, P, 'ge')
+    Ce.connect(Pe, P, sps.coo_matrix((np.full(len(z["ce_r"]), 6 * nS), (z["ce_r"], z["ce_c"])),
+                                     shape=tuple(z["ce_shape"])))
+    Ci = Connection(Pi, P, 'gi')
+    Ci.connect(Pi, P, sps.coo_matrix((np.full(len(z["ci_r"]), 67 * nS), (z["ci_r"], z["ci_c"])),
+                                     shape=tuple(z["ci_shape"])))
+    ar = arange(len(P), dtype=float)
+    P.v = El + ((ar / len(P) - 0.5) * 20 - 5) * mV
+    P.ge = ((ar / len(P) - 0.5) * 6 + 4) * 10. * nS
+    P.gi = ((ar / len(P) - 0.5) * 48 + 20) * 10. * nS
+    mon = SpikeMonitor(P)
+    Network(P, Ce, Ci, mon).run(steps * 0.1 * ms)
+    i, t = mon.it
+    return np.rint(t / 1e-4).astype(np.int64), i, z
+
+
+def test_cobahh_golden_exp_euler():
+    """Nonlinear exp-Euler (device bytecode interpreter, FP64 check mode) +
+    EmpiricalThreshold + propagation on the reference's fixed connectivity:
+    the 1 s raster equals the reference's recorded one (156,843 spikes)."""
+    s, i, z = _cobahh_facade(10000)
+    gs, gi = z["spike_step"].astype(np.int64), z["spike_idx"].astype(np.int64)
+    k = min(len(s), len(gs))
+    diff = np.nonzero((s[:k] != gs[:k]) | (i[:k] != gi[:k]))[0]
+    first = int(min(s[diff[0]], gs[diff[0]])) if len(diff) else None
+    print(f"COBAHH device vs golden: {len(s)} vs {len(gs)} spikes, first divergence step {first}")
+    assert len(s) == len(gs) == 156843
+    assert first is None

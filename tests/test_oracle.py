@@ -1,0 +1,340 @@
+"""Pins the CPU oracle (oracle/brian1.py) before it is trusted as the checker.
+
+Each test fixes something a plausible restatement mistake would break:
+closed-form propagators, the reference suite's known answers, a brute-force
+pure-Python restatement on tiny networks, and the reference's golden COBAHH
+raster (dev/tests/correctness, stored as tests/golden/cobahh.npz).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import brian1 as O
+
+DT = 1e-4
+ms, mV = 1e-3, 1e-3
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_cuba_propagator_closed_form():
+    """A and C of the CUBA model against the exact solution of the triangular
+    system (docs_sphinx/developer-simulationprinciples.txt:112-114)."""
+    taum, taue, taui, El = 20 * ms, 5 * ms, 10 * ms, -49 * mV
+
+    def f(x):
+        v, ge, gi = x
+        return np.array([(ge + gi - (v - El)) / taum, -ge / taue, -gi / taui])
+
+    A, C = O.linear_propagator(f, 3, DT)
+    em, ee, ei = math.exp(-DT / taum), math.exp(-DT / taue), math.exp(-DT / taui)
+    exact = np.array([
+        [em, taue / (taum - taue) * (em - ee), taui / (taum - taui) * (em - ei)],
+        [0, ee, 0],
+        [0, 0, ei]])
+    np.testing.assert_allclose(A, exact, rtol=1e-12, atol=1e-17)
+    np.testing.assert_allclose(C, [El * (1 - em), 0, 0], rtol=1e-12, atol=1e-17)
+    # a transposed operand or sign error in M would break this invariant:
+    # the resting state is a fixed point of S <- A S + C
+    rest = np.array([El, 0.0, 0.0])
+    np.testing.assert_allclose(A @ rest + C, rest, rtol=1e-14)
+
+
+def test_numeric_propagator_singular():
+    """Parameters make M singular: the 100-step Euler product of
+    stateupdater.py:284-314 is used and keeps parameters constant."""
+    def f(x):
+        return np.array([-x[0] / 0.01 + x[1] / 0.01, 0.0])
+    A, C = O.linear_propagator(f, 2, DT)
+    assert A[1, 1] == 1.0 and A[1, 0] == 0.0
+    E = 1 - DT / 100 / 0.01
+    assert abs(A[0, 0] - E ** 100) < 1e-15
+
+
+def tutorial1c_net(n=40):
+    tau, El = 20 * ms, -49 * mV
+    A, C = O.linear_propagator(lambda x: np.array([-(x[0] - El) / tau]), 1, DT)
+    net = O.Net(DT)
+    g = O.Group(n, np.full((1, n), El), ("linear", A, C), ("const", 0, -50 * mV),
+                [(0, "=", -60 * mV, False)])
+    return net, net.add_group(g)
+
+
+def test_tutorial1c_840_spikes():
+    """brian/tests/testcorrectness/test_from_tutorial1c.py: 840 spikes."""
+    net, gid = tutorial1c_net()
+    net.run(10000)
+    s, i = net.raster(gid)
+    assert len(s) == 840
+    # the closed form: first spike at step 0, period ceil(48.0 ms / dt)
+    t_isi = 20 * ms * math.log((-60 - -49) / (-50 - -49))
+    assert np.all(np.diff(np.unique(s)) == math.ceil(t_isi / DT))
+
+
+def _exp_current_net(taum=20 * ms, taue=1 * ms, taui=10 * ms):
+    def f(x):
+        V, ge, gi = x
+        return np.array([(-V + ge - gi) / taum, -ge / taue, -gi / taui])
+    A, C = O.linear_propagator(f, 3, DT)
+    net = O.Net(DT)
+    src = O.SpikeSource.from_times(2, [0], [0.0], DT)
+    sid = net.add_group(src)
+    g = O.Group(1, np.zeros((3, 1)), ("linear", A, C), ("const", 0, 10 * mV), [(0, "=", 0.0, False)])
+    gid = net.add_group(g)
+    net.add_connection((sid, 0, 2), (gid, 0, 1), 1, [0, 1, 1], [0], [3 * mV])
+    net.add_connection((sid, 0, 2), (gid, 0, 1), 2, [0, 0, 1], [0], [3 * mV])
+    return net, gid
+
+
+def test_exponential_current_closed_form():
+    """brian/tests/testcorrectness/test_exponential_current.py."""
+    taum, taue = 20 * ms, 1 * ms
+    net, gid = _exp_current_net()
+    vs = []
+    for n in range(1000):
+        net.step()
+        vs.append(net.groups[gid].S[0, 0])
+    t = np.arange(1000) * DT
+    pred = (np.exp(-t / taum) - np.exp(-t / taue)) * taue * (3 * mV) / (taum - taue)
+    assert np.max(np.abs(pred - np.array(vs))) < 1e-5
+
+
+def test_epsp_peak():
+    """brian/tests/testcorrectness/test_epsp.py: Vmax 0.136889 mV at 1.69735 ms."""
+    taum, taupsp, y0 = 10 * ms, 0.325 * ms, 4.86 * mV
+
+    def f(x):
+        V, xx, y = x
+        return np.array([(-V + xx) * (1. / taum), (-xx + y) * (1. / taupsp), -y * (1. / taupsp)])
+    A, C = O.linear_propagator(f, 3, DT)
+    net = O.Net(DT)
+    sid = net.add_group(O.SpikeSource.from_times(1, [0], [0.0], DT))
+    gid = net.add_group(O.Group(1, np.zeros((3, 1)), ("linear", A, C), ("const", 0, 100 * mV),
+                                [(0, "=", 0.0, False)]))
+    net.add_connection((sid, 0, 1), (gid, 0, 1), 2, [0, 1], [0], [y0])
+    V = []
+    for _ in range(100):
+        net.step()
+        V.append(net.groups[gid].S[0, 0])
+    k = int(np.argmax(V))
+    assert abs(V[k] - 0.136889 * mV) < 0.001 * mV
+    assert abs(k * DT - 1.69735 * ms) <= DT
+
+
+@pytest.mark.parametrize("delayed", [False, True])
+def test_propagation_arrival_steps(delayed):
+    """brian/tests/testfeatures/test_propagation.py: a spike at step 0 makes
+    the targets fire at step 1 (no delay) or 1+k for delay k*dt + dt/2."""
+    net = O.Net(DT)
+    A = np.eye(2)
+    C = np.zeros(2)
+    H = O.Group(1, np.array([[2.0], [1.0]]), ("linear", A, C), ("const", 0, 1.0), [(0, "=", 0.0, False)])
+    hid = net.add_group(H)
+    G = O.Group(10, np.zeros((1, 10)), ("linear", np.eye(1), np.zeros(1)), ("const", 0, 1.0),
+                [(0, "=", 0.0, False)])
+    gid = net.add_group(G)
+    indptr, idx, w = [0, 10], np.arange(10), np.full(10, 2.0)
+    if delayed:
+        d = np.arange(10) * DT + DT / 2
+        steps = np.array(10000.0 * d, dtype=np.int64)           # int(invdt*d)
+        assert list(steps) == list(range(10))
+        net.add_connection((hid, 0, 1), (gid, 0, 10), 0, indptr, idx, w, syn_delay_steps=steps,
+                           max_delay_steps=50)
+    else:
+        net.add_connection((hid, 0, 1), (gid, 0, 10), 0, indptr, idx, w)
+    net.run(20)
+    s, i = net.raster(gid)
+    assert list(i) == list(range(10))
+    assert list(s) == ([1] * 10 if not delayed else list(1 + np.arange(10)))
+
+
+def test_refractoriness_clamp():
+    """brian/tests/testinterface/test_reset.py:126-141."""
+    A, C = O.linear_propagator(lambda x: np.array([-x[0] / (.1 * ms)]), 1, DT)
+    net = O.Net(DT)
+    gid = net.add_group(O.Group(1, [[15.0]], ("linear", A, C), ("const", 0, 1.0),
+                                [(0, "=", -.5, False)], refractory=1 * ms))
+    net.run(5)
+    assert net.groups[gid].S[0, 0] == -.5
+    net.run(10)
+    assert -0.4 < net.groups[gid].S[0, 0] < 0.1
+    # the clamp lasts int(1ms/dt)+1 = 11 bins including the spike step
+    net2 = O.Net(DT)
+    g2 = net2.add_group(O.Group(1, [[15.0]], ("linear", A, C), ("const", 0, 1.0),
+                                [(0, "=", -.5, False)], refractory=1 * ms))
+    vals = []
+    for _ in range(14):
+        net2.step()
+        vals.append(net2.groups[g2].S[0, 0])
+    assert all(v == -.5 for v in vals[:11]) and vals[11] != -.5
+
+
+def test_threshold_strict_ascending():
+    net = O.Net(DT)
+    gid = net.add_group(O.Group(5, [[1.0, 1.5, 0.5, 2.0, 1.0 + 1e-12]], None, ("const", 0, 1.0),
+                                [(0, "=", 0.0, False)]))
+    net.run(1)
+    s, i = net.raster(gid)
+    assert list(i) == [1, 3, 4]
+
+
+def _bruteforce(N, S0, A, C, W, delay, Vt, Vr, steps):
+    """Pure-Python restatement: dense W[i][j], sequential adds, lists."""
+    m = len(S0)
+    S = [list(row) for row in S0]
+    hist = []
+    raster = []
+    for n in range(steps):
+        new = [[0.0] * N for _ in range(m)]
+        for j in range(m):
+            for k in range(N):
+                r = A[j][0] * S[0][k]
+                for q in range(1, m):
+                    r = r + A[j][q] * S[q][k]
+                new[j][k] = r + C[j]
+        S = new
+        sp = [k for k in range(N) if S[0][k] > Vt]
+        hist.append(sp)
+        raster += [(n, k) for k in sp]
+        if n - delay >= 0:
+            for i in hist[n - delay]:
+                for k in range(N):
+                    if W[i][k] != 0.0:
+                        var = 1 if i < N // 2 else 2
+                        S[var][k] = S[var][k] + W[i][k]
+        for k in hist[n]:
+            S[0][k] = Vr
+    return raster, S
+
+
+@pytest.mark.parametrize("delay", [0, 3])
+def test_bruteforce_tiny_network(delay):
+    rng = np.random.default_rng(3 + delay)
+    N = 24
+    taum, taue, taui, El = 20 * ms, 5 * ms, 10 * ms, -45 * mV
+
+    def f(x):
+        v, ge, gi = x
+        return np.array([(ge + gi - (v - El)) / taum, -ge / taue, -gi / taui])
+    A, C = O.linear_propagator(f, 3, DT)
+    S0 = np.zeros((3, N))
+    S0[0] = -60 * mV + rng.random(N) * 10 * mV
+    Wd = (rng.random((N, N)) < 0.3) * rng.uniform(0.5, 4.0, (N, N)) * mV
+    Wd[N // 2:] *= -1.5
+    raster, S = _bruteforce(N, S0.tolist(), A.tolist(), C.tolist(), Wd.tolist(), delay,
+                            -50 * mV, -60 * mV, 600)
+    net = O.Net(DT)
+    gid = net.add_group(O.Group(N, S0, ("linear", A, C), ("const", 0, -50 * mV),
+                                [(0, "=", -60 * mV, False)]))
+    import scipy.sparse as sps
+    for lo, hi, var in ((0, N // 2, 1), (N // 2, N, 2)):
+        M = sps.csr_matrix(Wd[lo:hi])
+        net.add_connection((gid, lo, hi - lo), (gid, 0, N), var, M.indptr, M.indices, M.data,
+                           delay_steps=delay)
+    net.run(600)
+    s, i = net.raster(gid)
+    assert len(raster) > 20
+    assert list(zip(s.tolist(), i.tolist())) == raster
+    assert np.array_equal(net.groups[gid].S, np.array(S))
+
+
+def test_stdp_sign_and_bounds():
+    """brian/tests/testinterface/test_stdp.py (pre before post potentiates)
+    plus clipping to [wmin, wmax]."""
+    Ee, El = 0.0, -80 * mV
+    net = O.Net(DT)
+    sid = net.add_group(O.SpikeSource.from_times(1, [0], [1 * ms], DT))
+
+    def f(S):
+        v, ge = S
+        return [(ge * (Ee - v) + (El - v)) / (5 * ms), -ge / (5 * ms)]
+    gid = net.add_group(O.Group(1, [[-51 * mV], [0.0]], ("euler", f), ("const", 0, -50 * mV),
+                                [(0, "=", El, False)]))
+    c = net.add_connection((sid, 0, 1), (gid, 0, 1), 1, [0, 1], [0], [1.0])
+    net.add_stdp(c, taup=10 * ms, taum=10 * ms, Ap=0.1, Am=0.1, wmin=0.0, wmax=10.0)
+    net.run(40)
+    assert c.data[0] > 1.0
+    assert 0.0 <= c.data[0] <= 10.0
+
+
+def test_counter_rng_splitmix64_vector():
+    """The mixing function is splitmix64's finaliser: seed 0 -> first output
+    0xE220A8397B1DCDAF (published splitmix64 sequence)."""
+    G = np.uint64(0x9E3779B97F4A7C15)
+    assert int(O._mix64(np.array([G], dtype=np.uint64))[0]) == 0xE220A8397B1DCDAF
+
+
+def cobahh_oracle(steps):
+    """The reference's COBAHH golden model (dev/tests/correctness/test_optimizations.py:48-128)."""
+    z = np.load(os.path.join(HERE, "golden", "cobahh.npz"))
+    umetre, ufarad, cm, siemens, msiemens, nS = 1e-6, 1e-6, 1e-2, 1.0, 1e-3, 1e-9
+    area = 20000 * umetre ** 2
+    Cm = (1 * ufarad * cm ** -2) * area
+    gl = (5e-5 * siemens * cm ** -2) * area
+    El, EK, ENa = -60 * mV, -90 * mV, 50 * mV
+    g_na = (100 * msiemens * cm ** -2) * area
+    g_kd = (30 * msiemens * cm ** -2) * area
+    VT = -63 * mV
+    taue, taui, Ee, Ei = 5 * ms, 10 * ms, 0 * mV, -80 * mV
+    exp = np.exp
+
+    def rhs(S):
+        v, m, n, h, ge, gi = S
+        alpham = 0.32 * (mV ** -1) * (13 * mV - v + VT) / (exp((13 * mV - v + VT) / (4 * mV)) - 1.) / ms
+        betam = 0.28 * (mV ** -1) * (v - VT - 40 * mV) / (exp((v - VT - 40 * mV) / (5 * mV)) - 1) / ms
+        alphah = 0.128 * exp((17 * mV - v + VT) / (18 * mV)) / ms
+        betah = 4. / (1 + exp((40 * mV - v + VT) / (5 * mV))) / ms
+        alphan = 0.032 * (mV ** -1) * (15 * mV - v + VT) / (exp((15 * mV - v + VT) / (5 * mV)) - 1.) / ms
+        betan = .5 * exp((10 * mV - v + VT) / (40 * mV)) / ms
+        return [(gl * (El - v) + ge * (Ee - v) + gi * (Ei - v) - g_na * (m * m * m) * h * (v - ENa)
+                 - g_kd * (n * n * n * n) * (v - EK)) / Cm,
+                alpham * (1 - m) - betam * m,
+                alphan * (1 - n) - betan * n,
+                alphah * (1 - h) - betah * h,
+                -ge * (1. / taue),
+                -gi * (1. / taui)]
+
+    N = 4000
+    ar = np.arange(N, dtype=np.float64)
+    S0 = np.zeros((6, N))
+    S0[0] = El + ((ar / N - 0.5) * 20 - 5) * mV
+    S0[4] = ((ar / N - 0.5) * 6 + 4) * 10. * nS
+    S0[5] = ((ar / N - 0.5) * 48 + 20) * 10. * nS
+    # EmpiricalThreshold(-20 mV, 3 ms): int(3ms/dt)=30, minus one (threshold.py:446-449)
+    net = O.Net(DT)
+    g = O.Group(N, S0, ("expeuler", rhs), ("empirical", 0, -20 * mV, 29), None)
+    gid = net.add_group(g)
+    we, wi = 6 * nS, 67 * nS
+    import scipy.sparse as sps
+    Ce = sps.csr_matrix((np.full(len(z["ce_r"]), we), (z["ce_r"], z["ce_c"])), shape=tuple(z["ce_shape"]))
+    Ci = sps.csr_matrix((np.full(len(z["ci_r"]), wi), (z["ci_r"], z["ci_c"])), shape=tuple(z["ci_shape"]))
+    Ce.sort_indices()
+    Ci.sort_indices()
+    net.add_connection((gid, 0, 3200), (gid, 0, N), 4, Ce.indptr, Ce.indices, Ce.data)
+    net.add_connection((gid, 3200, 800), (gid, 0, N), 5, Ci.indptr, Ci.indices, Ci.data)
+    net.run(steps)
+    return net, gid, z
+
+
+def test_cobahh_golden_raster_prefix():
+    """The golden COBAHH raster (exp-Euler, EmpiricalThreshold, fixed
+    connectivity) is reproduced exactly by the oracle.  200 ms here; the full
+    1 s (156,843 spikes) also matches exactly (test_cobahh_golden_raster_full,
+    marked slow)."""
+    _cobahh_exact(2000)
+
+
+@pytest.mark.slow
+def test_cobahh_golden_raster_full():
+    _cobahh_exact(10000)
+
+
+def _cobahh_exact(steps):
+    net, gid, z = cobahh_oracle(steps)
+    s, i = net.raster(gid)
+    gs, gi = z["spike_step"].astype(np.int64), z["spike_idx"].astype(np.int64)
+    m = gs < steps
+    assert len(s) == int(m.sum()) and len(s) > 1000
+    np.testing.assert_array_equal(s, gs[m])
+    np.testing.assert_array_equal(i, gi[m])
