@@ -133,7 +133,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--npt", type=int, default=0)
+    ap.add_argument("--probe-copy", type=int, default=0,
+                    help="only run the byte-counter calibration copy of this many bytes")
     args = ap.parse_args()
+    if args.probe_copy:
+        from brian_amd import _lib
+        ms = _lib.probe_copy(args.probe_copy, reps=10)
+        print(json.dumps({"probe_copy_bytes": args.probe_copy, "avg_ms": ms,
+                          "GBps_read_plus_write": 2 * args.probe_copy / ms / 1e6}))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
               "(the sharding design is described in DESIGN.md).", file=sys.stderr)

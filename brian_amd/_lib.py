@@ -65,6 +65,7 @@ _SIGS = {
     "bm_net_get_stdp_traces": [P, C.c_int, Pd, Pd],
     "bm_net_set_profiling": [P, C.c_int],
     "bm_net_get_profile": [P, Pi64, Pd, Pd],
+    "bm_probe_copy": [i64, C.c_int, Pd],
 }
 _RESTYPE = {"bm_last_error": C.c_char_p}
 
@@ -87,6 +88,13 @@ def load():
         fn.restype = C.c_int
     _lib = lib
     return lib
+
+
+def probe_copy(nbytes, reps=20):
+    """Average duration (ms) of a streaming copy of nbytes."""
+    ms = C.c_double()
+    call("bm_probe_copy", int(nbytes), int(reps), C.byref(ms))
+    return ms.value
 
 
 def exported_symbols():

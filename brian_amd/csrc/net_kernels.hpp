@@ -205,12 +205,138 @@ __device__ inline T apply_count(T x, uint32_t cnt, T w, int is_delay) {
   return x;
 }
 
+// Block-cooperative CSR propagation of the block's spike list into the
+// target rings.  Up to BLOCK spikes at a time: every thread loads the row
+// bounds of one spike (one memory round trip for all rows), a block scan
+// turns row lengths into offsets, then all threads stride over the
+// concatenated synapses (one more round trip for the column indices) and
+// add one count (or the weight) into slot (n + delay) mod D of the target.
+__device__ inline int block_exclusive_scan(int v, int tid, int *sh_wsum, int *total) {
+  const int lane = tid & 63, wv = tid >> 6;
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) sh_wsum[wv] = x;
+  __syncthreads();
+  int before = 0, tot = 0;
+#pragma unroll
+  for (int q = 0; q < BLOCK / 64; ++q) {
+    const int t = sh_wsum[q];
+    if (q < wv) before += t;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return before + x - v;
+}
+
+// Few spikes: one wavefront per spike, lanes stride over its row (two
+// dependent round trips per spike, no block barriers).
+template <typename T>
+__device__ inline void propagate_waves(const GroupDev &g, const uint16_t *sh_list, int nsp, int64_t n0,
+                                       int blk0, int tid) {
+  const int wave = tid >> 6, lane = tid & 63;
+  for (int p = wave; p < nsp; p += BLOCK / 64) {
+    const uint32_t e = sh_list[p];
+    const int64_t n = n0 + (e >> 12);
+    const int i = blk0 + (int)(e & 0xFFFu);
+    for (int o = 0; o < g.nout; ++o) {
+      const OutDev &od = g.outs[o];
+      const int64_t r = (int64_t)i - od.src_off;
+      if (r < 0 || r >= od.src_n) continue;
+      const int base_slot = (int)(n % od.D);
+      const int64_t lo = od.row_ptr[r], hi = od.row_ptr[r + 1];
+      const int pbits = od.nch * od.bits;
+      for (int64_t q = lo + lane; q < hi; q += 64) {
+        const int64_t tj = od.tgt_off + od.col[q];
+        int d = od.hom_delay;
+        if (od.dly_bytes == 1) d = static_cast<const uint8_t *>(od.dly)[q];
+        else if (od.dly_bytes == 2) d = static_cast<const uint16_t *>(od.dly)[q];
+        int sl = base_slot + d;
+        while (sl >= od.D) sl -= od.D;
+        unsigned char *slot = static_cast<unsigned char *>(od.ring) + (int64_t)sl * od.slot_bytes;
+        if (od.kind == RING_COUNT) {
+          const int64_t pos = tj * pbits + od.ch * od.bits;
+          atomicAdd(reinterpret_cast<unsigned int *>(slot) + (pos >> 5), 1u << (int)(pos & 31));
+        } else {
+          unsafeAtomicAdd(reinterpret_cast<T *>(slot) + tj, static_cast<const T *>(od.w)[q]);
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+__device__ inline void propagate_block(const GroupDev &g, const uint16_t *sh_list, int nsp, int64_t n0,
+                                       int blk0, int tid) {
+  if (nsp <= BLOCK / 64) {
+    propagate_waves<T>(g, sh_list, nsp, n0, blk0, tid);
+    return;
+  }
+  __shared__ int64_t sh_lo[BLOCK];
+  __shared__ int sh_off[BLOCK + 1];
+  __shared__ int sh_sb[BLOCK];
+  __shared__ int sh_wsum[BLOCK / 64];
+  for (int o = 0; o < g.nout; ++o) {
+    const OutDev &od = g.outs[o];
+    const int pbits = od.nch * od.bits;
+    for (int base = 0; base < nsp; base += BLOCK) {
+      const int cnt = nsp - base < BLOCK ? nsp - base : BLOCK;
+      int64_t lo = 0;
+      int len = 0, sb = 0;
+      if (tid < cnt) {
+        const uint32_t e = sh_list[base + tid];
+        const int64_t n = n0 + (e >> 12);
+        const int64_t r = (int64_t)(blk0 + (int)(e & 0xFFFu)) - od.src_off;
+        if (r >= 0 && r < od.src_n) {
+          lo = od.row_ptr[r];
+          len = (int)(od.row_ptr[r + 1] - lo);
+        }
+        sb = (int)(n % od.D);
+      }
+      int total = 0;
+      const int off = block_exclusive_scan(len, tid, sh_wsum, &total);
+      sh_lo[tid] = lo;
+      sh_off[tid] = off;
+      sh_sb[tid] = sb;
+      if (tid == 0) sh_off[cnt] = total;
+      __syncthreads();
+      for (int q = tid; q < total; q += BLOCK) {
+        // spike p owning synapse q: largest p with sh_off[p] <= q
+        int a = 0, b = cnt - 1;
+        while (a < b) {
+          const int mid = (a + b + 1) >> 1;
+          if (sh_off[mid] <= q) a = mid; else b = mid - 1;
+        }
+        const int64_t syn = sh_lo[a] + (q - sh_off[a]);
+        const int64_t tj = od.tgt_off + od.col[syn];
+        int d = od.hom_delay;
+        if (od.dly_bytes == 1) d = static_cast<const uint8_t *>(od.dly)[syn];
+        else if (od.dly_bytes == 2) d = static_cast<const uint16_t *>(od.dly)[syn];
+        int sl = sh_sb[a] + d;
+        while (sl >= od.D) sl -= od.D;
+        unsigned char *slot = static_cast<unsigned char *>(od.ring) + (int64_t)sl * od.slot_bytes;
+        if (od.kind == RING_COUNT) {
+          const int64_t pos = tj * pbits + od.ch * od.bits;
+          atomicAdd(reinterpret_cast<unsigned int *>(slot) + (pos >> 5), 1u << (int)(pos & 31));
+        } else {
+          unsafeAtomicAdd(reinterpret_cast<T *>(slot) + tj, static_cast<const T *>(od.w)[syn]);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 template <typename T, int M, int K, int F>
 __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t *d_base,
                                                  int64_t rel0, int W, int skip_first_tail,
                                                  int tail_only) {
   constexpr bool GEN = (F & F_GEN) != 0;
-  constexpr int PF = GEN ? 1 : 8;                 // ring prefetch depth (steps)
+  constexpr int PF = GEN ? 1 : (K >= 4 ? 8 : 16);  // ring prefetch depth (steps)
   constexpr int MM = M > 0 ? M : 1;
   extern __shared__ __align__(16) unsigned char smem[];
   uint16_t *sh_list = reinterpret_cast<uint16_t *>(smem);   // (w << 12) | local index
@@ -258,9 +384,6 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
       }
     }
   }
-  if (tid == 0) sh_n = 0;
-  __syncthreads();
-
   T *reg = reinterpret_cast<T *>(smem + sizeof(uint16_t) * BLOCK * K * (W > 0 ? W : 1) + 16);
   const int nsteps = tail_only ? 1 : W;
   // ring slot of the tail at window offset w is (n0 - 1 + w) mod D
@@ -284,6 +407,17 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
           for (int k = 0; k < K; ++k)
             cw[u][k] = (on && valid[k]) ? load_slice(slot, blk0 + k * BLOCK + tid, rd.sw) : 0u;
         }
+      }
+    }
+    if (w0 == 0) {
+      // one barrier for the LDS counter; the state and first ring loads are
+      // in flight together and complete here
+      if (tid == 0) sh_n = 0;
+      __syncthreads();
+    }
+    if constexpr (!GEN) {
+      if (g.nring == 1) {
+        const RingDev &rd = g.rings[0];
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
           const int w = w0 + u;
@@ -504,6 +638,34 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
     }
   }
 
+  __syncthreads();
+  const int nsp = sh_n;
+  if (nsp > 0) {
+    // ---------------- raster append (SpikeMonitor, brian/monitor.py:156-166)
+    if (g.raster != nullptr) {
+      if (tid == 0) sh_base = atomicAdd(g.raster_count, (unsigned long long)nsp);
+      __syncthreads();
+      for (int p = tid; p < nsp; p += BLOCK) {
+        const uint32_t e = sh_list[p];
+        const unsigned long long pos = sh_base + p;
+        if (pos < (unsigned long long)g.raster_cap) {
+          const uint64_t step = (uint64_t)(n0 + (e >> 12));
+          const uint64_t idx = (uint64_t)(blk0 + (e & 0xFFFu));
+          g.raster[pos] = (step << 32) | idx;
+        }
+      }
+    }
+    // ---------------- spike history for STDP (only with W == 1)
+    if (g.hist != nullptr) {
+      const int64_t hs = n0 % g.H;
+      if (tid == 0) sh_hbase = atomicAdd(&g.hist_count[hs], (unsigned int)nsp);
+      __syncthreads();
+      for (int p = tid; p < nsp; p += BLOCK)
+        g.hist[hs * N + sh_hbase + p] = (int32_t)(blk0 + (sh_list[p] & 0xFFFu));
+    }
+    propagate_block<T>(g, sh_list, nsp, n0, blk0, tid);
+  }
+
   // ---------------- write back
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -513,62 +675,6 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
     if constexpr (M > 0) {
 #pragma unroll
       for (int j = 0; j < M; ++j) static_cast<T *>(g.S)[(int64_t)j * N + i] = s[j][k];
-    }
-  }
-  __syncthreads();
-  const int nsp = sh_n;
-  if (nsp == 0) return;
-
-  // ---------------- raster append (SpikeMonitor, brian/monitor.py:156-166)
-  if (g.raster != nullptr) {
-    if (tid == 0) sh_base = atomicAdd(g.raster_count, (unsigned long long)nsp);
-    __syncthreads();
-    for (int p = tid; p < nsp; p += BLOCK) {
-      const uint32_t e = sh_list[p];
-      const unsigned long long pos = sh_base + p;
-      if (pos < (unsigned long long)g.raster_cap) {
-        const uint64_t step = (uint64_t)(n0 + (e >> 12));
-        const uint64_t idx = (uint64_t)(blk0 + (e & 0xFFFu));
-        g.raster[pos] = (step << 32) | idx;
-      }
-    }
-  }
-  // ---------------- spike history for STDP (only with W == 1)
-  if (g.hist != nullptr) {
-    const int64_t hs = n0 % g.H;
-    if (tid == 0) sh_hbase = atomicAdd(&g.hist_count[hs], (unsigned int)nsp);
-    __syncthreads();
-    for (int p = tid; p < nsp; p += BLOCK)
-      g.hist[hs * N + sh_hbase + p] = (int32_t)(blk0 + (sh_list[p] & 0xFFFu));
-  }
-  // ---------------- propagation: one wavefront per spike, lanes over the row
-  const int wave = tid >> 6, lane = tid & 63;
-  for (int p = wave; p < nsp; p += BLOCK / 64) {
-    const uint32_t e = sh_list[p];
-    const int64_t n = n0 + (e >> 12);
-    const int i = blk0 + (int)(e & 0xFFFu);
-    for (int o = 0; o < g.nout; ++o) {
-      const OutDev &od = g.outs[o];
-      const int64_t r = (int64_t)i - od.src_off;
-      if (r < 0 || r >= od.src_n) continue;
-      int base_slot = (int)(n % od.D);
-      const int64_t lo = od.row_ptr[r], hi = od.row_ptr[r + 1];
-      const int pbits = od.nch * od.bits;
-      for (int64_t q = lo + lane; q < hi; q += 64) {
-        const int64_t tj = od.tgt_off + od.col[q];
-        int d = od.hom_delay;
-        if (od.dly_bytes == 1) d = static_cast<const uint8_t *>(od.dly)[q];
-        else if (od.dly_bytes == 2) d = static_cast<const uint16_t *>(od.dly)[q];
-        int sl = base_slot + d;
-        while (sl >= od.D) sl -= od.D;
-        unsigned char *slot = static_cast<unsigned char *>(od.ring) + (int64_t)sl * od.slot_bytes;
-        if (od.kind == RING_COUNT) {
-          const int64_t pos = tj * pbits + od.ch * od.bits;
-          atomicAdd(reinterpret_cast<unsigned int *>(slot) + (pos >> 5), 1u << (int)(pos & 31));
-        } else {
-          unsafeAtomicAdd(reinterpret_cast<T *>(slot) + tj, static_cast<const T *>(od.w)[q]);
-        }
-      }
     }
   }
 }

@@ -22,5 +22,6 @@ void launch_stdp(const StdpDev &sd, hipStream_t st, const int64_t *d_base, int64
 void launch_hist_clear(unsigned int *const *counts, const int *H, int ngroups, const int64_t *d_base,
                        int64_t rel_next, hipStream_t st);
 void launch_advance(int64_t *d_base, int64_t by, hipStream_t st);
+void launch_probe_copy(const void *src, void *dst, int64_t bytes, hipStream_t st);
 
 }  // namespace bm

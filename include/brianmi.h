@@ -164,6 +164,10 @@ int bm_net_get_stdp_traces(bm_net *net, int sid, double *a_pre, double *a_post);
 int bm_net_set_profiling(bm_net *net, int enable);
 int bm_net_get_profile(bm_net *net, int64_t *launches, double *total_ms, double *window_steps);
 
+/* Streaming copy of nbytes (reps launches) used to calibrate hardware byte
+ * counters against known traffic; no counterpart in the reference. */
+int bm_probe_copy(int64_t nbytes, int reps, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif
