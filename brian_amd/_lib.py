@@ -66,6 +66,23 @@ _SIGS = {
     "bm_net_set_profiling": [P, C.c_int],
     "bm_net_get_profile": [P, Pi64, Pd, Pd],
     "bm_probe_copy": [i64, C.c_int, Pd],
+    "bm_stdp_batch_create": [C.c_int, dbl, C.c_int, C.c_int, C.c_int, Pd, Pd, Pd, C.c_int, dbl,
+                             C.c_int, dbl, C.c_int, Pd, dbl, u64, dbl, dbl, dbl, dbl, dbl, dbl,
+                             C.c_int, C.c_int, i64, C.POINTER(P)],
+    "bm_stdp_batch_destroy": [P],
+    "bm_fit_create": [C.c_int, dbl, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_char_p,
+                      C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, i64, Pd, Pi32,
+                      Pi32, Pi32, C.c_int, C.c_int, C.POINTER(P)],
+    "bm_fit_destroy": [P],
+    "bm_fit_set_candidates": [P, Pd, Pd, Pi32, Pi32],
+    "bm_fit_run": [P, i64, i64],
+    "bm_fit_sync": [P],
+    "bm_fit_results": [P, Pi32, Pi32],
+    "bm_fit_get_state": [P, Pd],
+    "bm_stdp_batch_run": [P, i64, C.c_int],
+    "bm_stdp_batch_sync": [P],
+    "bm_stdp_batch_get": [P, C.c_int, Pd],
+    "bm_stdp_batch_get_spikes": [P, Pi32, Pi32, i64, Pi64],
 }
 _RESTYPE = {"bm_last_error": C.c_char_p}
 
@@ -286,3 +303,63 @@ class NetHandle:
         n, ms, w = C.c_int64(), C.c_double(), C.c_double()
         call("bm_net_get_profile", self.h, C.byref(n), C.byref(ms), C.byref(w))
         return n.value, ms.value, w.value
+
+
+class StdpBatchHandle:
+    """Owning wrapper of a bm_stdp_batch (independent STDP instances)."""
+
+    def __init__(self, precision, dt, ninst, nin, A, Cv, S0, th_var, vt, reset_var, vr, tgt_var,
+                 w0, rate, seed, decay_pre, decay_post, Ap, Am, wmin, wmax, update=0,
+                 interactions=0, raster_cap=1 << 22):
+        self._lib = load()
+        A, Cv, S0, w0 = f64(A), f64(Cv), f64(S0), f64(w0)
+        self.ninst, self.nin, self.M = int(ninst), int(nin), len(Cv)
+        h = P()
+        call("bm_stdp_batch_create", int(precision), float(dt), self.ninst, self.nin, self.M,
+             dptr(A), dptr(Cv), dptr(S0), int(th_var), float(vt), int(reset_var), float(vr),
+             int(tgt_var), dptr(w0), float(rate), C.c_uint64(seed), float(decay_pre),
+             float(decay_post), float(Ap), float(Am), float(wmin), float(wmax), int(update),
+             int(interactions), int(raster_cap), C.byref(h))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self._lib.bm_stdp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nsteps, steps_per_launch=0):
+        call("bm_stdp_batch_run", self.h, int(nsteps), int(steps_per_launch))
+
+    def sync(self):
+        call("bm_stdp_batch_sync", self.h)
+
+    def _get(self, what, n):
+        out = np.empty(n, dtype=np.float64)
+        call("bm_stdp_batch_get", self.h, what, dptr(out))
+        return out
+
+    def weights(self):
+        return self._get(0, self.ninst * self.nin).reshape(self.ninst, self.nin)
+
+    def a_pre(self):
+        return self._get(1, self.ninst * self.nin).reshape(self.ninst, self.nin)
+
+    def a_post(self):
+        return self._get(2, self.ninst)
+
+    def state(self):
+        return self._get(3, self.ninst * self.M).reshape(self.ninst, self.M)
+
+    def spikes(self):
+        n = C.c_int64()
+        call("bm_stdp_batch_get_spikes", self.h, None, None, 0, C.byref(n))
+        st = np.empty(n.value, dtype=np.int32)
+        ix = np.empty(n.value, dtype=np.int32)
+        call("bm_stdp_batch_get_spikes", self.h, i32ptr(st), i32ptr(ix), n.value, C.byref(n))
+        return st.astype(np.int64), ix.astype(np.int64)

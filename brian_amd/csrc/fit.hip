@@ -1,0 +1,328 @@
+// Batched model fitting: every candidate neuron of every target integrates
+// the model against its target's input current and counts spikes and
+// exclusive coincidences with the target spike train (the role of the
+// reference's PyCUDA runsim, brian/library/modelfitting/gpu_modelfitting.py:39-199,
+// with the RESET1/RESET2 split of brian/experimental/modelfitting/gpu_modelfitting.py:139-148).
+//
+// As in the reference, the kernel body is generated from the model
+// equations (state update scheme, threshold, resets) by the host side and
+// compiled for the device at run time, here with hipRTC for gfx950.  One
+// thread per candidate; state, parameters and the coincidence-counter state
+// stay in registers for the whole launch.
+#include "common.hpp"
+#include "../../include/brianmi.h"
+
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace bm {
+
+static const char *kFitTemplate = R"KERNEL(
+typedef %SCALAR% T;
+extern "C" __global__ __launch_bounds__(256) void fit_runsim(
+    const T *__restrict__ I_arr, long long I_stride, const int *__restrict__ cand_target,
+    T *state, const T *__restrict__ params, int *spikecount, int *ncoinc,
+    const int *__restrict__ spiketimes, const int *__restrict__ train_off, int *st_index,
+    unsigned char *allowed, const int *__restrict__ delay_arr, const int *__restrict__ refr_arr,
+    int *next_allowed_arr, int Tstart, int Tend, int onset, int delta, int ncand, T dt) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncand) return;
+%LOAD%
+  const int tgt = cand_target[c];
+  const T *I_t = I_arr + (long long)tgt * I_stride;
+  const int *train = spiketimes + train_off[tgt];
+  int idx = st_index[c];
+  int last_spike_time = train[idx];
+  int next_spike_time = train[idx + 1];
+  bool last_spike_allowed = (allowed[c] & 1) != 0;
+  bool next_spike_allowed = (allowed[c] & 2) != 0;
+  int nc = ncoinc[c], ns = spikecount[c];
+  const int spikedelay = delay_arr[c];
+  const int refractory = refr_arr[c];
+  int next_allowed = next_allowed_arr[c];
+  for (int T_ = Tstart; T_ < Tend; ++T_) {
+    const T t = (T)T_ * dt;
+    (void)t;
+    const T %INPUT% = I_t[T_];
+%UPDATE%
+    const bool is_refractory = (T_ <= next_allowed);
+    const bool has_spiked = (%THRESHOLD%) && !is_refractory;
+    ns += (has_spiked && (T_ >= onset)) ? 1 : 0;
+    if (has_spiked || is_refractory) {
+%RESET1%
+    }
+    if (has_spiked) {
+%RESET2%
+      next_allowed = T_ + refractory;
+    }
+    const int Tspike = T_ + spikedelay;
+    bool near_last = last_spike_time + delta >= Tspike;
+    bool near_next = next_spike_time - delta <= Tspike;
+    near_last = near_last && has_spiked;
+    near_next = near_next && has_spiked;
+    nc += ((near_last && last_spike_allowed) || (near_next && next_spike_allowed)) ? 1 : 0;
+    const bool near_both = (near_last && last_spike_allowed) && (near_next && next_spike_allowed);
+    last_spike_allowed = last_spike_allowed && !near_last;
+    next_spike_allowed = (next_spike_allowed && !near_next) || near_both;
+    if (Tspike >= next_spike_time) {
+      ++idx;
+      last_spike_time = next_spike_time;
+      next_spike_time = train[idx + 1];
+      last_spike_allowed = next_spike_allowed;
+      next_spike_allowed = true;
+    }
+  }
+%STORE%
+  st_index[c] = idx;
+  allowed[c] = (unsigned char)((last_spike_allowed ? 1 : 0) | (next_spike_allowed ? 2 : 0));
+  ncoinc[c] = nc;
+  spikecount[c] = ns;
+  next_allowed_arr[c] = next_allowed;
+}
+)KERNEL";
+
+struct FitModule {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+};
+
+// compiled modules are cached per source text for the life of the process
+static std::mutex g_fit_mu;
+static std::map<std::string, FitModule> g_fit_cache;
+
+static void replace_all(std::string &s, const std::string &a, const std::string &b) {
+  size_t p = 0;
+  while ((p = s.find(a, p)) != std::string::npos) {
+    s.replace(p, a.size(), b);
+    p += b.size();
+  }
+}
+
+static FitModule compile_fit(const std::string &src) {
+  std::lock_guard<std::mutex> lk(g_fit_mu);
+  auto it = g_fit_cache.find(src);
+  if (it != g_fit_cache.end()) return it->second;
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "fit_runsim.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+    throw Error(BM_EHIP, "hiprtcCreateProgram failed");
+  // packed FP32 instructions stay disabled, as for the rest of the library
+  const char *opts[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-Xclang",
+                        "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  hiprtcResult rc = hiprtcCompileProgram(prog, 7, opts);
+  if (rc != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    hiprtcGetProgramLogSize(prog, &n);
+    std::string log(n, '\0');
+    if (n) hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    throw Error(BM_EINVAL, "fitting kernel does not compile:\n" + log);
+  }
+  size_t sz = 0;
+  hiprtcGetCodeSize(prog, &sz);
+  std::vector<char> code(sz);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  FitModule m;
+  BM_HIP(hipModuleLoadData(&m.mod, code.data()));
+  BM_HIP(hipModuleGetFunction(&m.fn, m.mod, "fit_runsim"));
+  g_fit_cache[src] = m;
+  return m;
+}
+
+}  // namespace bm
+
+using namespace bm;
+
+struct bm_fit {
+  int precision = 32;
+  int ncand = 0, ntarget = 0, nvars = 0, nparams = 0;
+  int64_t nsteps_input = 0;
+  int delta = 0, onset = 0;
+  double dt = 1e-4;
+  FitModule mod;
+  DevBuf I, cand_target, state, params, spikecount, ncoinc, spiketimes, train_off, st_index, allowed,
+      delay, refr, next_allowed;
+  std::vector<double> init_state;     // [nvars][ncand]
+  std::vector<int32_t> train_start;   // per target: offset of its padded train
+  std::vector<int32_t> cand_tgt;
+  hipStream_t stream = nullptr;
+  size_t tsize() const { return precision == 64 ? 8 : 4; }
+  ~bm_fit() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+void upload_scalar(bm_fit *h, DevBuf &b, const double *src, size_t n) {
+  b.alloc(n * h->tsize());
+  if (!n) return;
+  if (h->precision == 64) {
+    BM_HIP(hipMemcpyAsync(b.p, src, n * 8, hipMemcpyHostToDevice, h->stream));
+    BM_HIP(hipStreamSynchronize(h->stream));
+  } else {
+    std::vector<float> t(src, src + n);
+    BM_HIP(hipMemcpyAsync(b.p, t.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+    BM_HIP(hipStreamSynchronize(h->stream));
+  }
+}
+template <typename R>
+void upload(bm_fit *h, DevBuf &b, const R *src, size_t n) {
+  b.alloc(n * sizeof(R));
+  if (!n) return;
+  BM_HIP(hipMemcpyAsync(b.p, src, n * sizeof(R), hipMemcpyHostToDevice, h->stream));
+  BM_HIP(hipStreamSynchronize(h->stream));
+}
+
+// Puts every candidate back at the start of the run: state, counters,
+// coincidence state (next_spike_allowed=1, last_spike_allowed=0 as in
+// gpu_modelfitting.py:364-365), refractoriness.
+void reset_counters(bm_fit *h) {
+  upload_scalar(h, h->state, h->init_state.data(), h->init_state.size());
+  std::vector<int32_t> zeros(h->ncand, 0), neg(h->ncand, -1);
+  std::vector<uint8_t> allow(h->ncand, 2);
+  upload(h, h->spikecount, zeros.data(), zeros.size());
+  upload(h, h->ncoinc, zeros.data(), zeros.size());
+  upload(h, h->st_index, zeros.data(), zeros.size());
+  upload(h, h->allowed, allow.data(), allow.size());
+  upload(h, h->next_allowed, neg.data(), neg.size());
+}
+}  // namespace
+
+extern "C" {
+
+int bm_fit_create(int precision, double dt, int ncand, int ntarget, int nvars, int nparams,
+                  const char *kernel_load, const char *kernel_update, const char *kernel_threshold,
+                  const char *kernel_reset1, const char *kernel_reset2, const char *kernel_store,
+                  const char *input_name, int64_t nsteps_input, const double *I,
+                  const int32_t *cand_target, const int32_t *train_len, const int32_t *trains,
+                  int delta_steps, int onset_steps, bm_fit **out) {
+  return guard([&] {
+    BM_REQUIRE(precision == 32 || precision == 64, "precision must be 32 or 64");
+    BM_REQUIRE(ncand > 0 && ntarget > 0 && nsteps_input > 0, "sizes must be positive");
+    auto h = std::make_unique<bm_fit>();
+    h->precision = precision;
+    h->ncand = ncand;
+    h->ntarget = ntarget;
+    h->nvars = nvars;
+    h->nparams = nparams;
+    h->nsteps_input = nsteps_input;
+    h->delta = delta_steps;
+    h->onset = onset_steps;
+    h->dt = dt;
+    BM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    std::string src = kFitTemplate;
+    replace_all(src, "%SCALAR%", precision == 64 ? "double" : "float");
+    replace_all(src, "%LOAD%", kernel_load);
+    replace_all(src, "%UPDATE%", kernel_update);
+    replace_all(src, "%THRESHOLD%", kernel_threshold);
+    replace_all(src, "%RESET1%", kernel_reset1);
+    replace_all(src, "%RESET2%", kernel_reset2);
+    replace_all(src, "%STORE%", kernel_store);
+    replace_all(src, "%INPUT%", input_name);
+    h->mod = compile_fit(src);
+    upload_scalar(h.get(), h->I, I, (size_t)ntarget * nsteps_input);
+    for (int c = 0; c < ncand; ++c)
+      BM_REQUIRE(cand_target[c] >= 0 && cand_target[c] < ntarget, "candidate target out of range");
+    h->cand_tgt.assign(cand_target, cand_target + ncand);
+    upload(h.get(), h->cand_target, cand_target, ncand);
+    // padded trains: [-inf, s1..sk, +inf] with sentinels beyond any spike time
+    std::vector<int32_t> padded, off(ntarget);
+    int64_t q = 0;
+    const int32_t lo = -(1 << 30), hi = (1 << 30);
+    for (int j = 0; j < ntarget; ++j) {
+      off[j] = (int32_t)padded.size();
+      padded.push_back(lo);
+      for (int k = 0; k < train_len[j]; ++k) {
+        BM_REQUIRE(k == 0 || trains[q + k] >= trains[q + k - 1], "target trains must be sorted");
+        padded.push_back(trains[q + k]);
+      }
+      q += train_len[j];
+      padded.push_back(hi);
+    }
+    upload(h.get(), h->spiketimes, padded.data(), padded.size());
+    upload(h.get(), h->train_off, off.data(), off.size());
+    h->init_state.assign((size_t)nvars * ncand, 0.0);
+    std::vector<int32_t> zeros(ncand, 0);
+    upload(h.get(), h->delay, zeros.data(), zeros.size());
+    upload(h.get(), h->refr, zeros.data(), zeros.size());
+    std::vector<double> pz((size_t)std::max(nparams, 1) * ncand, 0.0);
+    upload_scalar(h.get(), h->params, pz.data(), pz.size());
+    reset_counters(h.get());
+    *out = h.release();
+  });
+}
+
+int bm_fit_destroy(bm_fit *h) {
+  return guard([&] {
+    if (!h) return;
+    (void)hipStreamSynchronize(h->stream);
+    delete h;
+  });
+}
+
+int bm_fit_set_candidates(bm_fit *h, const double *params, const double *init_state,
+                          const int32_t *refractory_steps, const int32_t *delay_steps) {
+  return guard([&] {
+    if (params && h->nparams > 0) upload_scalar(h, h->params, params, (size_t)h->nparams * h->ncand);
+    if (init_state) h->init_state.assign(init_state, init_state + (size_t)h->nvars * h->ncand);
+    if (refractory_steps) upload(h, h->refr, refractory_steps, h->ncand);
+    if (delay_steps) upload(h, h->delay, delay_steps, h->ncand);
+    reset_counters(h);
+  });
+}
+
+// Advance every candidate over steps [t0, t1) (t1 <= nsteps_input); the
+// counters continue from the previous call (the reference's stepsize chunks).
+int bm_fit_run(bm_fit *h, int64_t t0, int64_t t1) {
+  return guard([&] {
+    BM_REQUIRE(t0 >= 0 && t1 >= t0 && t1 <= h->nsteps_input, "time range outside the input");
+    void *I = h->I.p, *ct = h->cand_target.p, *st = h->state.p, *pa = h->params.p, *sc = h->spikecount.p,
+         *nc = h->ncoinc.p, *tr = h->spiketimes.p, *to = h->train_off.p, *si = h->st_index.p,
+         *al = h->allowed.p, *de = h->delay.p, *re = h->refr.p, *na = h->next_allowed.p;
+    long long stride = (long long)h->nsteps_input;
+    int T0 = (int)t0, T1 = (int)t1, onset = h->onset, delta = h->delta, n = h->ncand;
+    double dt64 = h->dt;
+    float dt32 = (float)h->dt;
+    void *args[] = {&I, &stride, &ct, &st, &pa, &sc, &nc, &tr, &to, &si, &al, &de, &re, &na,
+                    &T0, &T1, &onset, &delta, &n,
+                    h->precision == 64 ? (void *)&dt64 : (void *)&dt32};
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    BM_HIP(hipModuleLaunchKernel(h->mod.fn, blocks, 1, 1, 256, 1, 1, 0, h->stream, args, nullptr));
+  });
+}
+
+int bm_fit_sync(bm_fit *h) {
+  return guard([&] { BM_HIP(hipStreamSynchronize(h->stream)); });
+}
+
+int bm_fit_results(bm_fit *h, int32_t *ncoinc, int32_t *nspikes) {
+  return guard([&] {
+    BM_HIP(hipMemcpyAsync(ncoinc, h->ncoinc.p, (size_t)h->ncand * 4, hipMemcpyDeviceToHost, h->stream));
+    BM_HIP(hipMemcpyAsync(nspikes, h->spikecount.p, (size_t)h->ncand * 4, hipMemcpyDeviceToHost, h->stream));
+    BM_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int bm_fit_get_state(bm_fit *h, double *out) {
+  return guard([&] {
+    const size_t n = (size_t)h->nvars * h->ncand;
+    if (h->precision == 64) {
+      BM_HIP(hipMemcpyAsync(out, h->state.p, n * 8, hipMemcpyDeviceToHost, h->stream));
+      BM_HIP(hipStreamSynchronize(h->stream));
+    } else {
+      std::vector<float> t(n);
+      BM_HIP(hipMemcpyAsync(t.data(), h->state.p, n * 4, hipMemcpyDeviceToHost, h->stream));
+      BM_HIP(hipStreamSynchronize(h->stream));
+      for (size_t i = 0; i < n; ++i) out[i] = t[i];
+    }
+  });
+}
+
+}  // extern "C"

@@ -336,7 +336,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
                                                  int64_t rel0, int W, int skip_first_tail,
                                                  int tail_only) {
   constexpr bool GEN = (F & F_GEN) != 0;
-  constexpr int PF = GEN ? 1 : (K >= 4 ? 8 : 16);  // ring prefetch depth (steps)
+  constexpr int PF = GEN ? 1 : 8;                 // ring prefetch depth (steps)
   constexpr int MM = M > 0 ? M : 1;
   extern __shared__ __align__(16) unsigned char smem[];
   uint16_t *sh_list = reinterpret_cast<uint16_t *>(smem);   // (w << 12) | local index

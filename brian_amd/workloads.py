@@ -118,3 +118,47 @@ def cuba(N=4000, seed=1, connectivity="bernoulli", p=0.02, indegree=80, delay_st
         "delay_steps": int(delay_steps),
         "Vt": Vt, "Vr": Vr,
     }
+
+
+# ---------------------------------------------------------------- fitting
+# LIF with refractoriness and an adaptive threshold Vt0 + Vt: Vt jumps by
+# FIT_DVT at each spike and relaxes with FIT_TAU_T; V is relative to rest.
+# The adaptation constants keep every target of the hidden-parameter range
+# below 1/(2*delta) = 125 Hz, where the gamma factor's normalisation holds.
+FIT_EQS = """
+dV/dt = (R*I - V)/tau : 1
+dVt/dt = -Vt/tau_t : 1
+I : 1
+R : 1
+tau : second
+Vt0 : 1
+"""
+FIT_THRESHOLD = "V > Vt0 + Vt"
+FIT_TAU_T = 100 * ms
+FIT_DVT = 20 * mV
+FIT_RESET = "V = 0; Vt += 0.02"
+FIT_BOUNDS = {"R": (50e6, 500e6), "tau": (5 * ms, 50 * ms), "Vt0": (5 * mV, 20 * mV),
+              "refractory": (1 * ms, 5 * ms)}
+
+
+def ou_current(ntarget, nsteps, dt=1e-4, mu=0.5e-9, sigma=0.2e-9, tau=5e-3, seed=0):
+    """Ornstein-Uhlenbeck input with the exact one-step update, FP64, seeded."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a = np.exp(-dt / tau)
+    b = sigma * np.sqrt(1 - a * a)
+    I = np.empty((ntarget, nsteps))
+    x = mu + sigma * rng.standard_normal(ntarget)
+    for n in range(nsteps):
+        I[:, n] = x
+        x = mu + (x - mu) * a + b * rng.standard_normal(ntarget)
+    return I
+
+
+def log_uniform(rng, lo, hi, n):
+    return np.exp(np.log(lo) + rng.random(n) * (np.log(hi) - np.log(lo)))
+
+
+def fit_hidden_params(ntarget, seed=0):
+    """Hidden parameters of the target neurons, log-uniform in FIT_BOUNDS."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return {k: log_uniform(rng, lo, hi, ntarget) for k, (lo, hi) in FIT_BOUNDS.items()}

@@ -164,6 +164,58 @@ int bm_net_get_stdp_traces(bm_net *net, int sid, double *a_pre, double *a_post);
 int bm_net_set_profiling(bm_net *net, int enable);
 int bm_net_get_profile(bm_net *net, int64_t *launches, double *total_ms, double *window_steps);
 
+/* --------------------------------------------------------- batched STDP
+ * Many independent instances, each nin Poisson inputs (counter RNG, stream =
+ * inst*nin + i, the same streams as a PoissonGroup of ninst*nin neurons)
+ * driving one linear neuron (S <- A.S + C, nvars <= 4) through a plastic
+ * Connection onto S[tgt_var], with ExponentialSTDP (brian/stdp.py:452-539,
+ * updaters :371-393, order connection -> pre -> post) and Reset(vr).
+ * One wavefront per instance; FP32 weights carry Kahan compensation.
+ * w0 is [ninst][nin], S0 is [ninst][nvars]. */
+typedef struct bm_stdp_batch bm_stdp_batch;
+int bm_stdp_batch_create(int precision, double dt, int ninst, int nin, int nvars, const double *A,
+                         const double *C, const double *S0, int th_var, double vt, int reset_var,
+                         double vr, int tgt_var, const double *w0, double rate, uint64_t seed,
+                         double decay_pre, double decay_post, double Ap, double Am, double wmin,
+                         double wmax, int update, int interactions, int64_t raster_cap,
+                         bm_stdp_batch **out);
+int bm_stdp_batch_destroy(bm_stdp_batch *h);
+/* steps_per_launch 0 = default chunk (20000 steps per launch). */
+int bm_stdp_batch_run(bm_stdp_batch *h, int64_t nsteps, int steps_per_launch);
+int bm_stdp_batch_sync(bm_stdp_batch *h);
+/* what: 0 weights [ninst][nin], 1 A_pre [ninst][nin], 2 A_post [ninst], 3 state [ninst][nvars]. */
+int bm_stdp_batch_get(bm_stdp_batch *h, int what, double *out);
+/* Post-synaptic spikes sorted by (step, instance); steps == NULL only counts. */
+int bm_stdp_batch_get_spikes(bm_stdp_batch *h, int32_t *steps, int32_t *inst, int64_t cap, int64_t *n_out);
+
+/* ---------------------------------------------------------- model fitting
+ * GPUModelFitting (brian/library/modelfitting/gpu_modelfitting.py:202-401):
+ * ncand candidates, candidate c fits target cand_target[c] and reads input
+ * row I[cand_target[c]][0..nsteps_input).  The kernel body snippets are C
+ * generated from the model equations (load/store of state and parameters,
+ * state update scheme, threshold expression, RESET1 applied at spikes and
+ * while refractory, RESET2 at spikes only); the library compiles them for
+ * gfx950 with hipRTC and caches the module.  Target trains are integer
+ * steps: train_len[ntarget] and the concatenated sorted trains.  Exclusive
+ * coincidence counting with window delta_steps; spikes counted from onset. */
+typedef struct bm_fit bm_fit;
+int bm_fit_create(int precision, double dt, int ncand, int ntarget, int nvars, int nparams,
+                  const char *kernel_load, const char *kernel_update, const char *kernel_threshold,
+                  const char *kernel_reset1, const char *kernel_reset2, const char *kernel_store,
+                  const char *input_name, int64_t nsteps_input, const double *I,
+                  const int32_t *cand_target, const int32_t *train_len, const int32_t *trains,
+                  int delta_steps, int onset_steps, bm_fit **out);
+int bm_fit_destroy(bm_fit *h);
+/* params [nparams][ncand], init_state [nvars][ncand] (nullable = keep),
+ * refractory/delay steps [ncand] (nullable = keep); resets all counters. */
+int bm_fit_set_candidates(bm_fit *h, const double *params, const double *init_state,
+                          const int32_t *refractory_steps, const int32_t *delay_steps);
+/* Advance all candidates over steps [t0, t1); counters carry over. */
+int bm_fit_run(bm_fit *h, int64_t t0, int64_t t1);
+int bm_fit_sync(bm_fit *h);
+int bm_fit_results(bm_fit *h, int32_t *ncoinc, int32_t *nspikes);
+int bm_fit_get_state(bm_fit *h, double *out);
+
 /* Streaming copy of nbytes (reps launches) used to calibrate hardware byte
  * counters against known traffic; no counterpart in the reference. */
 int bm_probe_copy(int64_t nbytes, int reps, double *avg_ms);

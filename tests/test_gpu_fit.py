@@ -1,0 +1,103 @@
+"""Model fitting on the GPU: the generated fitting kernel against the oracle,
+and PSO recovery of hidden parameters through modelfitting()."""
+import numpy as np
+import pytest
+
+from brian_amd import workloads as W
+from brian_amd.equations import Equations
+from brian_amd.modelfitting import FitEngine, modelfitting, spike_steps
+from oracle import fitting as F
+
+pytestmark = pytest.mark.gpu
+DT = 1e-4
+
+
+def fit_eqs():
+    return Equations(W.FIT_EQS, tau_t=W.FIT_TAU_T)
+
+
+def oracle_step(tau_t):
+    def step(s, I, T):
+        V__tmp = (s["R"] * I - s["V"]) / s["tau"]
+        Vt__tmp = (-s["Vt"]) / tau_t
+        s["V"] += V__tmp * DT
+        s["Vt"] += Vt__tmp * DT
+    return step
+
+
+def oracle_counts(params, refr_steps, I, cand_target, trains, delta, steps):
+    st = {"V": np.zeros(len(cand_target)), "Vt": np.zeros(len(cand_target)), "R": params["R"],
+          "tau": params["tau"], "Vt0": params["Vt0"]}
+
+    def r1(s, m):
+        s["V"][m] = 0.0
+
+    def r2(s, m):
+        s["Vt"][m] += W.FIT_DVT
+    return F.run_fit(oracle_step(W.FIT_TAU_T), r1, r2, lambda s: s["V"] > s["Vt0"] + s["Vt"], st, I,
+                     cand_target, trains, delta, refr_steps, t1=steps)
+
+
+def oracle_trains(params, refr_steps, I, steps):
+    """Spike steps of the 'true' neurons (one per row of I)."""
+    n = I.shape[0]
+    V = np.zeros(n)
+    Vt = np.zeros(n)
+    nxt = -np.ones(n, dtype=np.int64)
+    out = [[] for _ in range(n)]
+    for T in range(steps):
+        V__tmp = (params["R"] * I[:, T] - V) / params["tau"]
+        Vt__tmp = (-Vt) / W.FIT_TAU_T
+        V += V__tmp * DT
+        Vt += Vt__tmp * DT
+        refr = T <= nxt
+        spk = (V > params["Vt0"] + Vt) & ~refr
+        V[spk | refr] = 0.0
+        Vt[spk] += W.FIT_DVT
+        nxt[spk] = T + refr_steps[spk]
+        for j in np.nonzero(spk)[0]:
+            out[j].append(T)
+    return [np.array(o, dtype=np.int32) for o in out]
+
+
+def test_fit_kernel_fp64_matches_oracle():
+    ntarget, pop, steps = 4, 64, 6000
+    I = W.ou_current(ntarget, steps, seed=3)
+    hidden = W.fit_hidden_params(ntarget, seed=4)
+    trains = oracle_trains(hidden, spike_steps(hidden["refractory"], DT), I, steps)
+    rng = np.random.default_rng(5)
+    cand_target = np.repeat(np.arange(ntarget), pop)
+    M = ntarget * pop
+    params = {k: W.log_uniform(rng, lo, hi, M) for k, (lo, hi) in W.FIT_BOUNDS.items()}
+    refr = spike_steps(params.pop("refractory"), DT)
+    eng = FitEngine(fit_eqs(), W.FIT_THRESHOLD, W.FIT_RESET, I, trains, cand_target,
+                    DT, 40, precision=64)
+    eng.set_candidates(params, init_state={"V": 0.0, "Vt": 0.0}, refractory_steps=refr)
+    eng.run(0, steps, stepsize=2500)          # three chunks: counters carry over
+    nc, ns = eng.results()
+    onc, ons = oracle_counts(params, refr, I, cand_target, trains, 40, steps)
+    assert ns.sum() > 100 and nc.sum() > 10
+    np.testing.assert_array_equal(ns, ons)
+    np.testing.assert_array_equal(nc, onc)
+    # FP32: the same model, counts within a few percent in aggregate
+    e32 = FitEngine(fit_eqs(), W.FIT_THRESHOLD, W.FIT_RESET, I, trains, cand_target,
+                    DT, 40, precision=32)
+    e32.set_candidates(params, init_state={"V": 0.0, "Vt": 0.0}, refractory_steps=refr)
+    e32.run(0, steps)
+    nc32, ns32 = e32.results()
+    assert abs(ns32.sum() - ns.sum()) / ns.sum() < 0.01
+
+
+def test_modelfitting_recovers_hidden_neuron():
+    ntarget, steps = 3, 10000
+    I = W.ou_current(ntarget, steps, seed=7)
+    hidden = W.fit_hidden_params(ntarget, seed=8)
+    trains = oracle_trains(hidden, spike_steps(hidden["refractory"], DT), I, steps)
+    data = [(j, s * DT) for j in range(ntarget) for s in trains[j]]
+    b = W.FIT_BOUNDS
+    res = modelfitting(model=fit_eqs(), reset=W.FIT_RESET, threshold=W.FIT_THRESHOLD,
+                       data=data, input=I, dt=DT, popsize=2000, maxiter=8, delta=4e-3,
+                       initial_values={"V": 0.0, "Vt": 0.0}, R=b["R"], tau=b["tau"], Vt0=b["Vt0"],
+                       refractory=b["refractory"], seed=1)
+    print("fitness", res["fitness"])
+    assert np.all(res["fitness"] > 0.5)
