@@ -72,8 +72,19 @@ _SIGS = {
     "bm_stdp_batch_destroy": [P],
     "bm_fit_create": [C.c_int, dbl, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_char_p,
                       C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, i64, Pd, Pi32,
-                      Pi32, Pi32, C.c_int, C.c_int, C.POINTER(P)],
+                      Pi32, Pi32, C.c_int, C.c_int, C.c_int, C.POINTER(P)],
+    "bm_fit_features": [P, C.POINTER(C.c_float)],
     "bm_fit_destroy": [P],
+    "bm_mlp_num_params": [C.c_int, C.c_int, C.c_int, Pi64],
+    "bm_mlp_create": [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(P)],
+    "bm_mlp_destroy": [P],
+    "bm_mlp_forward": [P, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)],
+    "bm_mlp_grad": [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, Pd, C.POINTER(C.c_float)],
+    "bm_mlp_set_data": [P, C.POINTER(C.c_float), C.POINTER(C.c_float), i64],
+    "bm_mlp_train": [P, C.c_int, C.c_int, C.c_float, u64, Pd],
+    "bm_mlp_get_params": [P, C.POINTER(C.c_float)],
+    "bm_mlp_gemm_test": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float),
+                         C.POINTER(C.c_float), C.POINTER(C.c_float)],
     "bm_fit_set_candidates": [P, Pd, Pd, Pi32, Pi32],
     "bm_fit_run": [P, i64, i64],
     "bm_fit_sync": [P],
@@ -130,6 +141,10 @@ def call(name, *args):
 
 def dptr(a):
     return a.ctypes.data_as(Pd) if a is not None else None
+
+
+def f32ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
 
 
 def i32ptr(a):

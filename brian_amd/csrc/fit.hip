@@ -30,7 +30,8 @@ extern "C" __global__ __launch_bounds__(256) void fit_runsim(
     T *state, const T *__restrict__ params, int *spikecount, int *ncoinc,
     const int *__restrict__ spiketimes, const int *__restrict__ train_off, int *st_index,
     unsigned char *allowed, const int *__restrict__ delay_arr, const int *__restrict__ refr_arr,
-    int *next_allowed_arr, int Tstart, int Tend, int onset, int delta, int ncand, T dt) {
+    int *next_allowed_arr, int Tstart, int Tend, int onset, int delta, int ncand, T dt,
+    float *feat, int *last_spk) {
 #pragma clang fp contract(off)
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncand) return;
@@ -47,6 +48,7 @@ extern "C" __global__ __launch_bounds__(256) void fit_runsim(
   const int spikedelay = delay_arr[c];
   const int refractory = refr_arr[c];
   int next_allowed = next_allowed_arr[c];
+%FEAT_LOAD%
   for (int T_ = Tstart; T_ < Tend; ++T_) {
     const T t = (T)T_ * dt;
     (void)t;
@@ -61,6 +63,7 @@ extern "C" __global__ __launch_bounds__(256) void fit_runsim(
     if (has_spiked) {
 %RESET2%
       next_allowed = T_ + refractory;
+%FEAT_SPIKE%
     }
     const int Tspike = T_ + spikedelay;
     bool near_last = last_spike_time + delta >= Tspike;
@@ -80,6 +83,7 @@ extern "C" __global__ __launch_bounds__(256) void fit_runsim(
     }
   }
 %STORE%
+%FEAT_STORE%
   st_index[c] = idx;
   allowed[c] = (unsigned char)((last_spike_allowed ? 1 : 0) | (next_spike_allowed ? 2 : 0));
   ncoinc[c] = nc;
@@ -87,6 +91,38 @@ extern "C" __global__ __launch_bounds__(256) void fit_runsim(
   next_allowed_arr[c] = next_allowed;
 }
 )KERNEL";
+
+// Feature mode (training data of the learned proposal): per candidate a row
+// of FEAT_STRIDE floats: [0,32) ISI counts in log-spaced bins from 10 to
+// 10^4 steps, [32,64) spike-triggered sums of the input at lags 0,10,..,310
+// steps (spikes at T >= 320), [64] number of triggered spikes.
+static const char *kFeatLoad = R"F(
+  int last_spike_T = last_spk[c];
+  float sta[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) sta[k] = feat[(size_t)c * 66 + 32 + k];
+  float nsta = feat[(size_t)c * 66 + 64];
+)F";
+static const char *kFeatSpike = R"F(
+      if (last_spike_T >= 0) {
+        const int isi = T_ - last_spike_T;
+        int bin = isi < 10 ? 0 : (int)(32.0f * logf((float)isi / 10.0f) / logf(1000.0f));
+        bin = bin < 0 ? 0 : (bin > 31 ? 31 : bin);
+        feat[(size_t)c * 66 + bin] += 1.0f;
+      }
+      last_spike_T = T_;
+      if (T_ >= 320) {
+#pragma unroll
+        for (int k = 0; k < 32; ++k) sta[k] += (float)I_t[T_ - 10 * k];
+        nsta += 1.0f;
+      }
+)F";
+static const char *kFeatStore = R"F(
+  last_spk[c] = last_spike_T;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) feat[(size_t)c * 66 + 32 + k] = sta[k];
+  feat[(size_t)c * 66 + 64] = nsta;
+)F";
 
 struct FitModule {
   hipModule_t mod = nullptr;
@@ -148,7 +184,8 @@ struct bm_fit {
   double dt = 1e-4;
   FitModule mod;
   DevBuf I, cand_target, state, params, spikecount, ncoinc, spiketimes, train_off, st_index, allowed,
-      delay, refr, next_allowed;
+      delay, refr, next_allowed, feat, last_spk;
+  bool features = false;
   std::vector<double> init_state;     // [nvars][ncand]
   std::vector<int32_t> train_start;   // per target: offset of its padded train
   std::vector<int32_t> cand_tgt;
@@ -192,6 +229,11 @@ void reset_counters(bm_fit *h) {
   upload(h, h->st_index, zeros.data(), zeros.size());
   upload(h, h->allowed, allow.data(), allow.size());
   upload(h, h->next_allowed, neg.data(), neg.size());
+  if (h->features) {
+    std::vector<float> fz((size_t)h->ncand * 66, 0.f);
+    upload(h, h->feat, fz.data(), fz.size());
+    upload(h, h->last_spk, neg.data(), neg.size());
+  }
 }
 }  // namespace
 
@@ -202,7 +244,7 @@ int bm_fit_create(int precision, double dt, int ncand, int ntarget, int nvars, i
                   const char *kernel_reset1, const char *kernel_reset2, const char *kernel_store,
                   const char *input_name, int64_t nsteps_input, const double *I,
                   const int32_t *cand_target, const int32_t *train_len, const int32_t *trains,
-                  int delta_steps, int onset_steps, bm_fit **out) {
+                  int delta_steps, int onset_steps, int features, bm_fit **out) {
   return guard([&] {
     BM_REQUIRE(precision == 32 || precision == 64, "precision must be 32 or 64");
     BM_REQUIRE(ncand > 0 && ntarget > 0 && nsteps_input > 0, "sizes must be positive");
@@ -226,6 +268,10 @@ int bm_fit_create(int precision, double dt, int ncand, int ntarget, int nvars, i
     replace_all(src, "%RESET2%", kernel_reset2);
     replace_all(src, "%STORE%", kernel_store);
     replace_all(src, "%INPUT%", input_name);
+    h->features = features != 0;
+    replace_all(src, "%FEAT_LOAD%", h->features ? kFeatLoad : "");
+    replace_all(src, "%FEAT_SPIKE%", h->features ? kFeatSpike : "");
+    replace_all(src, "%FEAT_STORE%", h->features ? kFeatStore : "");
     h->mod = compile_fit(src);
     upload_scalar(h.get(), h->I, I, (size_t)ntarget * nsteps_input);
     for (int c = 0; c < ncand; ++c)
@@ -290,9 +336,10 @@ int bm_fit_run(bm_fit *h, int64_t t0, int64_t t1) {
     int T0 = (int)t0, T1 = (int)t1, onset = h->onset, delta = h->delta, n = h->ncand;
     double dt64 = h->dt;
     float dt32 = (float)h->dt;
+    void *fe = h->features ? h->feat.p : nullptr, *ls = h->features ? h->last_spk.p : nullptr;
     void *args[] = {&I, &stride, &ct, &st, &pa, &sc, &nc, &tr, &to, &si, &al, &de, &re, &na,
                     &T0, &T1, &onset, &delta, &n,
-                    h->precision == 64 ? (void *)&dt64 : (void *)&dt32};
+                    h->precision == 64 ? (void *)&dt64 : (void *)&dt32, &fe, &ls};
     const unsigned blocks = (unsigned)((n + 255) / 256);
     BM_HIP(hipModuleLaunchKernel(h->mod.fn, blocks, 1, 1, 256, 1, 1, 0, h->stream, args, nullptr));
   });
@@ -306,6 +353,14 @@ int bm_fit_results(bm_fit *h, int32_t *ncoinc, int32_t *nspikes) {
   return guard([&] {
     BM_HIP(hipMemcpyAsync(ncoinc, h->ncoinc.p, (size_t)h->ncand * 4, hipMemcpyDeviceToHost, h->stream));
     BM_HIP(hipMemcpyAsync(nspikes, h->spikecount.p, (size_t)h->ncand * 4, hipMemcpyDeviceToHost, h->stream));
+    BM_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int bm_fit_features(bm_fit *h, float *out) {
+  return guard([&] {
+    BM_REQUIRE(h->features, "the fitting handle was created without feature mode");
+    BM_HIP(hipMemcpyAsync(out, h->feat.p, (size_t)h->ncand * 66 * 4, hipMemcpyDeviceToHost, h->stream));
     BM_HIP(hipStreamSynchronize(h->stream));
   });
 }

@@ -186,7 +186,7 @@ class FitEngine:
     """
 
     def __init__(self, eqs, threshold, reset, I, targets, cand_target, dt, delta_steps,
-                 onset_steps=0, input_var="I", method="Euler", precision=32):
+                 onset_steps=0, input_var="I", method="Euler", precision=32, features=False):
         self._lib = _lib.load()
         src = fit_kernel_source(eqs, threshold, reset, input_var, method, precision)
         self.state_names, self.param_names = src["state"], src["params"]
@@ -205,7 +205,8 @@ class FitEngine:
                   src["update"].encode(), src["threshold"].encode(), src["reset1"].encode(),
                   src["reset2"].encode(), src["store"].encode(), input_var.encode(),
                   self.nsteps, _lib.dptr(I), _lib.i32ptr(self.cand_target), _lib.i32ptr(lens),
-                  _lib.i32ptr(flat), int(delta_steps), int(onset_steps), C.byref(h))
+                  _lib.i32ptr(flat), int(delta_steps), int(onset_steps), 1 if features else 0,
+                  C.byref(h))
         self.h = h
         self.dt = dt
         self.src = src
@@ -254,6 +255,12 @@ class FitEngine:
         _lib.call("bm_fit_results", self.h, _lib.i32ptr(nc), _lib.i32ptr(ns))
         return nc, ns
 
+    def features(self):
+        """Raw feature rows [ncand][66] (feature mode)."""
+        out = np.empty(self.ncand * 66, dtype=np.float32)
+        _lib.call("bm_fit_features", self.h, _lib.f32ptr(out))
+        return out.reshape(self.ncand, 66)
+
     def state(self):
         out = np.empty(len(self.state_names) * self.ncand)
         _lib.call("bm_fit_get_state", self.h, _lib.dptr(out))
@@ -268,12 +275,15 @@ def gamma_from_counts(ncoinc, nspikes, target_len, target_rate_per_step, delta_s
 
 
 def particle_swarm(X0, fun, iterations, pso_params=(.9, 1.9, 1.9), min_values=None, max_values=None,
-                   group_size=None, rng=None, callback=None):
+                   group_size=None, rng=None, callback=None, evaluate_initial=False):
     """Grouped particle swarm maximiser (brian/tools/particle_swarm.py:6-111).
 
     X0: (N params, M particles); particles j*group_size..(j+1)*group_size-1
     form group j with its own global best.  fun(X) -> fitness[M].
     Returns (X_gbest [N, groups], fitness_gbest [groups]).
+    evaluate_initial: also evaluate X0 before the first move (the reference
+    moves every particle once before its first evaluation); used when X0
+    comes from a learned proposal.
     """
     rng = rng or np.random.default_rng()
     N, M = X0.shape
@@ -288,6 +298,13 @@ def particle_swarm(X0, fun, iterations, pso_params=(.9, 1.9, 1.9), min_values=No
     fit_lbest = -np.inf * np.ones(M)
     X_gbest = X[:, ::group_size].copy()
     fit_gbest = -np.inf * np.ones(G)
+    if evaluate_initial:
+        fit = np.asarray(fun(X), dtype=np.float64)
+        fit_lbest = fit.copy()
+        fg = fit.reshape(G, group_size)
+        best = fg.argmax(axis=1)
+        X_gbest = X[:, np.arange(G) * group_size + best].copy()
+        fit_gbest = fg[np.arange(G), best].copy()
     for k in range(iterations):
         R1 = rng.random((N, M))
         R2 = rng.random((N, M))
@@ -318,7 +335,7 @@ def modelfitting(model=None, reset=None, threshold=None, refractory=0 * ms, data
                  input=None, dt=None, popsize=1000, maxiter=10, delta=4 * ms, onset=0 * ms,
                  initial_values=None, stepsize=None, precision=32, method="Euler",
                  pso_params=(.9, 1.9, 1.9), seed=None, initial_params=None, return_engine=False,
-                 **params):
+                 evaluate_initial=None, **params):
     """Fit ``model`` to ``data`` (list of (target index, spike time) or one
     array of spike times); ``input`` is one current trace shared by all
     targets or one trace per target.  ``params`` give [min, max] bounds (or
@@ -377,10 +394,13 @@ def modelfitting(model=None, reset=None, threshold=None, refractory=0 * ms, data
             X0[k] = ilo + rng.random(M) * (ihi - ilo)
     lo = np.array([bounds[n][0] for n in names])
     hi = np.array([bounds[n][1] for n in names])
-    Xg, fg = particle_swarm(X0, fun, maxiter, pso_params, lo, hi, group_size=popsize, rng=rng)
+    if evaluate_initial is None:
+        evaluate_initial = initial_params is not None
+    Xg, fg = particle_swarm(X0, fun, maxiter, pso_params, lo, hi, group_size=popsize, rng=rng,
+                            evaluate_initial=evaluate_initial)
     res = FitResults({n: Xg[k] for k, n in enumerate(names)})
     res["fitness"] = fg
-    res["simulated_candidate_seconds"] = float(M) * duration * maxiter
+    res["simulated_candidate_seconds"] = float(M) * duration * (maxiter + (1 if evaluate_initial else 0))
     if return_engine:
         return res, engine
     engine.close()

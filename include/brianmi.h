@@ -204,7 +204,7 @@ int bm_fit_create(int precision, double dt, int ncand, int ntarget, int nvars, i
                   const char *kernel_reset1, const char *kernel_reset2, const char *kernel_store,
                   const char *input_name, int64_t nsteps_input, const double *I,
                   const int32_t *cand_target, const int32_t *train_len, const int32_t *trains,
-                  int delta_steps, int onset_steps, bm_fit **out);
+                  int delta_steps, int onset_steps, int features, bm_fit **out);
 int bm_fit_destroy(bm_fit *h);
 /* params [nparams][ncand], init_state [nvars][ncand] (nullable = keep),
  * refractory/delay steps [ncand] (nullable = keep); resets all counters. */
@@ -215,6 +215,31 @@ int bm_fit_run(bm_fit *h, int64_t t0, int64_t t1);
 int bm_fit_sync(bm_fit *h);
 int bm_fit_results(bm_fit *h, int32_t *ncoinc, int32_t *nspikes);
 int bm_fit_get_state(bm_fit *h, double *out);
+/* Feature mode only: [ncand][66] raw spike-train features (32 log-binned ISI
+ * counts, 32 spike-triggered input sums at 1 ms lags, triggered-spike count,
+ * unused) for the learned proposal; no counterpart in the reference. */
+int bm_fit_features(bm_fit *h, float *out);
+
+/* ------------------------------------------------------- learned proposal
+ * MLP nin -> nh -> nh -> 2*nu (ReLU), Gaussian NLL over nu normalised
+ * parameters (outputs: nu means, nu log-sigmas).  BF16 MFMA GEMMs with FP32
+ * accumulation; FP32 master weights and Adam.  No counterpart in the
+ * reference (its fitting starts from uniform random particles); it seeds the
+ * particle swarm of the fitting loop.  params: flat FP32 W1[nin][nh] b1[nh]
+ * W2[nh][nh] b2[nh] W3[nh][32] b3[32] (output layer padded to 32). */
+typedef struct bm_mlp bm_mlp;
+int bm_mlp_num_params(int nin, int nh, int nu, int64_t *n);
+int bm_mlp_create(int nin, int nh, int nu, const float *params, bm_mlp **out);
+int bm_mlp_destroy(bm_mlp *h);
+/* out [B][2*nu] */
+int bm_mlp_forward(bm_mlp *h, const float *X, int B, float *out);
+/* mean loss and its gradient (same flat layout as params); B multiple of 32 */
+int bm_mlp_grad(bm_mlp *h, const float *X, const float *U, int B, double *loss, float *grads);
+int bm_mlp_set_data(bm_mlp *h, const float *X, const float *U, int64_t n);
+int bm_mlp_train(bm_mlp *h, int epochs, int batch, float lr, uint64_t seed, double *last_loss);
+int bm_mlp_get_params(bm_mlp *h, float *out);
+/* C[M][N] = op(A) op(B) with BF16-rounded operands (layout check of the MFMA tile) */
+int bm_mlp_gemm_test(int M, int N, int K, int ta, int tb, const float *A, const float *B, float *C);
 
 /* Streaming copy of nbytes (reps launches) used to calibrate hardware byte
  * counters against known traffic; no counterpart in the reference. */
