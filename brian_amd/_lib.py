@@ -51,6 +51,11 @@ _SIGS = {
                               Pd, C.c_int, C.c_int, Pi32, C.c_int, C.POINTER(C.c_int)],
     "bm_net_add_stdp": [P, C.c_int, dbl, dbl, dbl, dbl, dbl, dbl, C.c_int, C.c_int,
                         C.POINTER(C.c_int)],
+    "bm_net_connect_random_blocks": [P, C.c_int, C.c_int, C.c_int, C.c_int, Pi64, Pi64, Pi64, Pi64,
+                                     dbl, dbl, u64, C.c_int, C.c_int, dbl, dbl, C.c_int,
+                                     C.POINTER(C.c_int)],
+    "bm_net_connection_nnz": [P, C.c_int, Pi64],
+    "bm_net_get_connection": [P, C.c_int, Pi64, Pi32, Pi32],
     "bm_net_record": [P, C.c_int, i64],
     "bm_net_prepare": [P, C.c_int, C.c_int, C.c_int],
     "bm_net_run": [P, i64],
@@ -255,6 +260,26 @@ class NetHandle:
              len(indices), i64ptr(indptr), i32ptr(indices), dptr(w), 1 if sd is not None else 0,
              int(delay_steps), i32ptr(sd), int(max_delay_steps), C.byref(cid))
         return cid.value
+
+    def connect_random_blocks(self, src, tgt, var, src_off, src_n, tgt_off, tgt_n, p, weight, seed,
+                              delay_steps=0, dmin=None, dmax=None, max_delay_steps=0):
+        """Device-built block-diagonal Bernoulli(p) connection (see brianmi.h)."""
+        a = [as_i64(x) for x in (src_off, src_n, tgt_off, tgt_n)]
+        cid = C.c_int()
+        kind = 0 if dmin is None else 1
+        call("bm_net_connect_random_blocks", self.h, src, tgt, var, len(a[0]), *[i64ptr(x) for x in a],
+             float(p), float(weight), C.c_uint64(seed), kind, int(delay_steps),
+             float(dmin or 0.0), float(dmax or 0.0), int(max_delay_steps), C.byref(cid))
+        return cid.value
+
+    def get_connection(self, cid, src_n):
+        n = C.c_int64()
+        call("bm_net_connection_nnz", self.h, cid, C.byref(n))
+        rp = np.empty(src_n + 1, np.int64)
+        col = np.empty(max(n.value, 1), np.int32)
+        dl = np.empty(max(n.value, 1), np.int32)
+        call("bm_net_get_connection", self.h, cid, i64ptr(rp), i32ptr(col), i32ptr(dl))
+        return rp, col[:n.value], dl[:n.value]
 
     def add_stdp(self, cid, decay_pre, decay_post, Ap, Am, wmin, wmax, update=0, interactions=0):
         sid = C.c_int()

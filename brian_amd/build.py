@@ -17,7 +17,8 @@ OUT = os.path.join(HERE, "libbrianmi.so")
 BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-UNITS = ["net.hip", "kernels_f32.hip", "kernels_f64.hip", "stdp_batch.hip", "fit.hip", "mlp.hip"]
+UNITS = ["net.hip", "kernels_f32.hip", "kernels_f64.hip", "stdp_batch.hip", "fit.hip", "mlp.hip",
+         "conngen.hip"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wno-unused-result",
          # packed FP32 instructions stay disabled (not written by hand, not generated)
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

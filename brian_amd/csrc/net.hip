@@ -68,6 +68,8 @@ struct HostConn {
   int64_t max_indeg = 0;
   DevBuf row_ptr_d, col_d, w_d, dly_d;
   int dly_bytes = 0;
+  bool generated = false;   // CSR built on the device at prepare time
+  GenSpec gen;
 };
 
 struct HostStdp {
@@ -251,6 +253,19 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
   // ---- per-connection storage and count/float mode
   for (size_t ci = 0; ci < net->conns.size(); ++ci) {
     HostConn &c = *net->conns[ci];
+    if (c.generated) {
+      // counts per slot and channel are bounded by the largest source block
+      int64_t mx = 0;
+      for (int64_t n : c.gen.src_n) mx = std::max(mx, n);
+      c.max_indeg = mx;
+      c.count_mode = c.stdp < 0 && mx <= 65535;
+      BM_REQUIRE(c.count_mode, "generated connections need a source block of at most 65535 neurons");
+      c.gen.dly_bytes = c.max_delay <= 255 ? 1 : 2;
+      c.dly_bytes = c.delay_kind == 1 ? c.gen.dly_bytes : 0;
+      c.gen.with_delay = c.delay_kind == 1;
+      c.nnz = generate_block_csr(c.gen, c.row_ptr_d, c.col_d, c.dly_d, net->stream);
+      continue;
+    }
     bool uniform = c.nnz > 0;
     for (int64_t q = 1; q < c.nnz && uniform; ++q) uniform = (c.w[q] == c.w[0]);
     std::vector<int64_t> indeg(c.tgt_n, 0);
@@ -899,6 +914,104 @@ int bm_net_add_connection(bm_net *net, int src_gid, int64_t src_off, int64_t src
   });
 }
 
+int bm_net_connect_random_blocks(bm_net *net, int src_gid, int tgt_gid, int tgt_var, int nblocks,
+                                 const int64_t *src_off, const int64_t *src_n, const int64_t *tgt_off,
+                                 const int64_t *tgt_n, double p, double weight, uint64_t seed, int delay_kind,
+                                 int hom_delay_steps, double dmin, double dmax, int max_delay_steps, int *cid) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &sg = grp(net, src_gid);
+    HostGroup &tg = grp(net, tgt_gid);
+    BM_REQUIRE(tg.kind == GK_NEURON && tgt_var >= 0 && tgt_var < tg.M, "target variable out of range");
+    BM_REQUIRE(nblocks > 0, "nblocks must be > 0");
+    BM_REQUIRE(p >= 0 && p < 1, "p must be in [0, 1)");
+    BM_REQUIRE(delay_kind == 0 || delay_kind == 1, "delay_kind must be 0 or 1");
+    auto c = std::make_unique<HostConn>();
+    c->generated = true;
+    GenSpec &gsp = c->gen;
+    int64_t lo_s = INT64_MAX, hi_s = 0, lo_t = INT64_MAX, hi_t = 0;
+    for (int b = 0; b < nblocks; ++b) {
+      BM_REQUIRE(src_off[b] >= 0 && src_n[b] >= 0 && src_off[b] + src_n[b] <= sg.N, "source block out of range");
+      BM_REQUIRE(tgt_off[b] >= 0 && tgt_n[b] >= 0 && tgt_off[b] + tgt_n[b] <= tg.N, "target block out of range");
+      if (b) BM_REQUIRE(src_off[b] >= src_off[b - 1] + src_n[b - 1], "source blocks must be sorted and disjoint");
+      lo_s = std::min(lo_s, src_off[b]);
+      hi_s = std::max(hi_s, src_off[b] + src_n[b]);
+      lo_t = std::min(lo_t, tgt_off[b]);
+      hi_t = std::max(hi_t, tgt_off[b] + tgt_n[b]);
+    }
+    BM_REQUIRE(hi_s < (int64_t(1) << 32) && hi_t < (int64_t(1) << 32), "indices must fit 32 bits");
+    gsp.src_off.assign(src_off, src_off + nblocks);
+    gsp.src_n.assign(src_n, src_n + nblocks);
+    gsp.tgt_off.assign(tgt_off, tgt_off + nblocks);
+    gsp.tgt_n.assign(tgt_n, tgt_n + nblocks);
+    gsp.src_lo = lo_s;
+    gsp.tgt_lo = lo_t;
+    gsp.nrows = hi_s - lo_s;
+    gsp.p = p;
+    gsp.seed = seed;
+    gsp.dmin = dmin;
+    gsp.dmax = dmax;
+    gsp.invdt = 1.0 / net->dt;
+    c->src = src_gid;
+    c->tgt = tgt_gid;
+    c->var = tgt_var;
+    c->src_off = lo_s;
+    c->src_n = hi_s - lo_s;
+    c->tgt_off = lo_t;
+    c->tgt_n = hi_t - lo_t;
+    c->w.assign(1, weight);
+    c->wval = weight;
+    c->delay_kind = delay_kind;
+    if (delay_kind == 0) {
+      BM_REQUIRE(hom_delay_steps >= 0 && hom_delay_steps < 65536, "delay out of range");
+      c->hom_delay = hom_delay_steps;
+      c->max_delay = c->min_delay = hom_delay_steps;
+    } else {
+      BM_REQUIRE(dmin >= 0 && dmax >= dmin, "delay range must satisfy 0 <= dmin <= dmax");
+      const int smin = (int)(gsp.invdt * dmin), smax = (int)(gsp.invdt * dmax);
+      BM_REQUIRE(smax <= max_delay_steps && max_delay_steps < 65536, "delays exceed max_delay");
+      c->max_delay = max_delay_steps;
+      c->min_delay = smin;
+    }
+    net->conns.push_back(std::move(c));
+    *cid = (int)net->conns.size() - 1;
+  });
+}
+
+int bm_net_connection_nnz(bm_net *net, int cid, int64_t *nnz) {
+  return guard([&] {
+    BM_REQUIRE(cid >= 0 && cid < (int)net->conns.size(), "bad connection id");
+    BM_REQUIRE(net->prepared || !net->conns[cid]->generated, "generated connections exist after prepare");
+    *nnz = net->conns[cid]->nnz;
+  });
+}
+
+int bm_net_get_connection(bm_net *net, int cid, int64_t *row_ptr, int32_t *col, int32_t *delay_steps) {
+  return guard([&] {
+    BM_REQUIRE(cid >= 0 && cid < (int)net->conns.size(), "bad connection id");
+    HostConn &c = *net->conns[cid];
+    BM_REQUIRE(net->prepared, "network not prepared");
+    BM_HIP(hipMemcpyAsync(row_ptr, c.row_ptr_d.p, (size_t)(c.src_n + 1) * 8, hipMemcpyDeviceToHost, net->stream));
+    if (c.nnz) BM_HIP(hipMemcpyAsync(col, c.col_d.p, (size_t)c.nnz * 4, hipMemcpyDeviceToHost, net->stream));
+    BM_HIP(hipStreamSynchronize(net->stream));
+    if (delay_steps) {
+      if (c.delay_kind == 0) {
+        for (int64_t q = 0; q < c.nnz; ++q) delay_steps[q] = c.hom_delay;
+      } else if (c.dly_bytes == 1) {
+        std::vector<uint8_t> t(c.nnz);
+        if (c.nnz) BM_HIP(hipMemcpyAsync(t.data(), c.dly_d.p, c.nnz, hipMemcpyDeviceToHost, net->stream));
+        BM_HIP(hipStreamSynchronize(net->stream));
+        for (int64_t q = 0; q < c.nnz; ++q) delay_steps[q] = t[q];
+      } else {
+        std::vector<uint16_t> t(c.nnz);
+        if (c.nnz) BM_HIP(hipMemcpyAsync(t.data(), c.dly_d.p, c.nnz * 2, hipMemcpyDeviceToHost, net->stream));
+        BM_HIP(hipStreamSynchronize(net->stream));
+        for (int64_t q = 0; q < c.nnz; ++q) delay_steps[q] = t[q];
+      }
+    }
+  });
+}
+
 int bm_net_add_stdp(bm_net *net, int cid, double decay_pre, double decay_post, double Ap, double Am,
                     double wmin, double wmax, int update, int interactions, int *sid) {
   return guard([&] {
@@ -906,6 +1019,7 @@ int bm_net_add_stdp(bm_net *net, int cid, double decay_pre, double decay_post, d
     BM_REQUIRE(cid >= 0 && cid < (int)net->conns.size(), "bad connection id");
     HostConn &c = *net->conns[cid];
     BM_REQUIRE(c.delay_kind == 0, "STDP on DelayConnection is not supported");
+    BM_REQUIRE(!c.generated, "STDP on device-generated connections is not supported");
     BM_REQUIRE(c.stdp < 0, "connection already has STDP");
     BM_REQUIRE(update == 0 || update == 1, "update must be 0 (additive) or 1 (multiplicative)");
     BM_REQUIRE(interactions >= 0 && interactions <= 3, "interactions out of range");
@@ -1036,6 +1150,10 @@ int bm_net_get_weights(bm_net *net, int cid, double *w) {
   return guard([&] {
     BM_REQUIRE(cid >= 0 && cid < (int)net->conns.size(), "bad connection id");
     HostConn &c = *net->conns[cid];
+    if (c.generated) {
+      for (int64_t q = 0; q < c.nnz; ++q) w[q] = c.wval;
+      return;
+    }
     if (!net->prepared || c.count_mode) {
       std::memcpy(w, c.w.data(), c.nnz * sizeof(double));
       return;

@@ -4,6 +4,8 @@
 #pragma once
 #include "net_types.hpp"
 
+#include <vector>
+
 namespace bm {
 
 struct SweepArgs {
@@ -23,5 +25,15 @@ void launch_hist_clear(unsigned int *const *counts, const int *H, int ngroups, c
                        int64_t rel_next, hipStream_t st);
 void launch_advance(int64_t *d_base, int64_t by, hipStream_t st);
 void launch_probe_copy(const void *src, void *dst, int64_t bytes, hipStream_t st);
+
+// block-diagonal random connectivity built on the device (conngen.hip)
+struct GenSpec {
+  std::vector<int64_t> src_off, src_n, tgt_off, tgt_n;   // blocks sorted by src_off
+  int64_t src_lo = 0, tgt_lo = 0, nrows = 0;
+  double p = 0, dmin = 0, dmax = 0, invdt = 0;
+  uint64_t seed = 0;
+  int with_delay = 0, dly_bytes = 1;
+};
+int64_t generate_block_csr(const GenSpec &s, DevBuf &row_ptr, DevBuf &col, DevBuf &dly, hipStream_t st);
 
 }  // namespace bm

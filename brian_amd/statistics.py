@@ -55,3 +55,61 @@ def gamma_factor(source, target, delta, normalize=True, dt=None):
         norm = .5 * (1 - 2 * target_rate * delta)
         return (num_coinc - NCoincAvg) / (norm * (len(source) + len(target)))
     return num_coinc
+
+
+def isi_cv_per_neuron(steps, idx, N):
+    """ISI CV of every neuron of a raster (step, index); NaN below 3 spikes."""
+    steps = np.asarray(steps, dtype=np.int64)
+    idx = np.asarray(idx, dtype=np.int64)
+    order = np.lexsort((steps, idx))
+    s, i = steps[order], idx[order]
+    isi = np.diff(s).astype(np.float64)
+    same = i[1:] == i[:-1]
+    isi, who = isi[same], i[1:][same]
+    cnt = np.bincount(who, minlength=N)
+    sm = np.bincount(who, weights=isi, minlength=N)
+    sq = np.bincount(who, weights=isi * isi, minlength=N)
+    cv = np.full(N, np.nan)
+    ok = cnt >= 2
+    mean = sm[ok] / cnt[ok]
+    cv[ok] = np.sqrt(np.maximum(sq[ok] / cnt[ok] - mean * mean, 0.0)) / mean
+    return cv
+
+
+def network_of(idx, offsets):
+    """Network number of each neuron index given offsets[k] = first neuron of network k."""
+    return np.searchsorted(np.asarray(offsets), np.asarray(idx), side="right") - 1
+
+
+def per_network_rates(idx, offsets, duration):
+    """Firing rate (Hz) of each network of a batch laid out by offsets (len nnet+1)."""
+    offsets = np.asarray(offsets)
+    nnet = len(offsets) - 1
+    cnt = np.bincount(network_of(idx, offsets), minlength=nnet)[:nnet]
+    return cnt / (np.diff(offsets) * duration)
+
+
+def per_network_first_divergence(a, b, offsets):
+    """First step at which the rasters a=(steps, idx) and b differ, per network
+    (-1 where identical).  Rasters are sorted by (step, index)."""
+    offsets = np.asarray(offsets)
+    nnet = len(offsets) - 1
+    out = np.full(nnet, -1, dtype=np.int64)
+    parts = []
+    for s, i in (a, b):
+        net = network_of(i, offsets)
+        order = np.argsort(net, kind="stable")
+        bounds = np.searchsorted(net[order], np.arange(nnet + 1))
+        parts.append((s[order], i[order], bounds))
+    (sa, ia, ba), (sb, ib, bb) = parts
+    for k in range(nnet):
+        xa, xb = slice(ba[k], ba[k + 1]), slice(bb[k], bb[k + 1])
+        pa, pb = sa[xa], sb[xb]
+        qa, qb = ia[xa], ib[xb]
+        n = min(len(pa), len(pb))
+        d = np.nonzero((pa[:n] != pb[:n]) | (qa[:n] != qb[:n]))[0]
+        if len(d):
+            out[k] = min(pa[d[0]], pb[d[0]])
+        elif len(pa) != len(pb):
+            out[k] = pa[n] if len(pa) > n else pb[n]
+    return out

@@ -120,6 +120,35 @@ def cuba(N=4000, seed=1, connectivity="bernoulli", p=0.02, indegree=80, delay_st
     }
 
 
+
+def uneven_batch(nnet=4096, seed=0, lo=500, hi=20000, p=0.02, dmin=0.1 * ms, dmax=5 * ms):
+    """Many independent CUBA networks laid out back to back in one group.
+
+    Sizes are round(exp(U[ln lo, ln hi])); network k occupies neurons
+    [off[k], off[k] + N[k]), its first int(0.8 N) neurons excitatory.  The
+    connectivity is described by blocks for the device generator (every pair
+    inside a network present with probability p, delays uniform in
+    [dmin, dmax]); exc blocks connect onto ge, inh blocks onto gi.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    N = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), nnet))).astype(np.int64)
+    Ne = (0.8 * N).astype(np.int64)
+    off = np.zeros(nnet + 1, dtype=np.int64)
+    np.cumsum(N, out=off[1:])
+    total = int(off[-1])
+    S0 = np.zeros((3, total))
+    S0[0] = Vr + rng.random(total) * (Vt - Vr)
+    exc = [(int(off[k]), int(Ne[k]), int(off[k]), int(N[k])) for k in range(nnet)]
+    inh = [(int(off[k] + Ne[k]), int(N[k] - Ne[k]), int(off[k]), int(N[k])) for k in range(nnet)]
+    s = int(seed) & 0xFFFFFFFF
+    return {
+        "N": total, "sizes": N, "offsets": off, "Ne": Ne, "S0": S0,
+        "exc_blocks": exc, "inh_blocks": inh, "p": p,
+        "exc_seed": (s << 32) | 0x243F6A88, "inh_seed": (s << 32) | 0x85A308D3,
+        "dmin": dmin, "dmax": dmax, "max_delay_steps": int(dmax / (0.1 * ms)),
+        "Vt": Vt, "Vr": Vr, "we": we, "wi": wi,
+    }
+
 # ---------------------------------------------------------------- fitting
 # LIF with refractoriness and an adaptive threshold Vt0 + Vt: Vt jumps by
 # FIT_DVT at each spike and relaxes with FIT_TAU_T; V is relative to rest.

@@ -127,6 +127,24 @@ int bm_net_add_connection(bm_net *net, int src_gid, int64_t src_off, int64_t src
                           const double *w, int delay_kind, int hom_delay_steps,
                           const int32_t *syn_delay, int max_delay_steps, int *cid);
 
+/* Block-diagonal random connectivity built on the device (the role of
+ * connect_random, brian/connections/construction.py:42-177, for many
+ * independent networks): block b connects source rows [src_off[b],
+ * +src_n[b]) to targets [tgt_off[b], +tgt_n[b]); pair (i, j) is present iff
+ * pair_u32(seed, i, j) < floor(p * 2^32) (32-bit integer hash, reproducible
+ * on the host); all weights equal `weight`.  delay_kind 0: homogeneous
+ * hom_delay_steps; 1: per-synapse d = dmin + (u + 0.5) 2^-32 (dmax - dmin)
+ * with u a second pair hash, steps int(d/dt), DelayConnection semantics.
+ * The CSR is built at bm_net_prepare; read it back with bm_net_get_connection. */
+int bm_net_connect_random_blocks(bm_net *net, int src_gid, int tgt_gid, int tgt_var, int nblocks,
+                                 const int64_t *src_off, const int64_t *src_n, const int64_t *tgt_off,
+                                 const int64_t *tgt_n, double p, double weight, uint64_t seed, int delay_kind,
+                                 int hom_delay_steps, double dmin, double dmax, int max_delay_steps, int *cid);
+int bm_net_connection_nnz(bm_net *net, int cid, int64_t *nnz);
+/* row_ptr [src_n+1] over the connection's source range, col [nnz] relative
+ * to its target range, delay_steps [nnz] (nullable). */
+int bm_net_get_connection(bm_net *net, int cid, int64_t *row_ptr, int32_t *col, int32_t *delay_steps);
+
 /* ExponentialSTDP on a delay_kind 0 connection (brian/stdp.py:452-539):
  * decay_pre/decay_post are the 1x1 propagators exp(-dt/tau) (FP64, host);
  * Ap, Am already scaled as in stdp.py:509-513; update 0 additive,

@@ -65,3 +65,53 @@ def first_divergence(a_steps, a_idx, b_steps, b_idx):
     if len(a_steps) != len(b_steps):
         return int(a_steps[n] if len(a_steps) > n else b_steps[n])
     return None
+
+
+def _blocks(blocks):
+    b = np.asarray(blocks, dtype=np.int64).reshape(-1, 4)
+    return b[:, 0], b[:, 1], b[:, 2], b[:, 3]
+
+
+def device_uneven(d, precision=32, window=0, record=True, use_graph=True, npt=0, capacity=None):
+    """Uneven batch description (workloads.uneven_batch) -> prepared NetHandle
+    with device-generated connectivity; returns (handle, group, exc cid, inh cid)."""
+    A, C = cuba_propagator()
+    h = NetHandle(1e-4, precision)
+    g = h.add_group(d["S0"])
+    h.set_linear(g, A, C)
+    h.set_threshold(g, 0, d["Vt"])
+    h.add_reset_op(g, 0, 0, d["Vr"])
+    cids = []
+    for key, var, w, seed in (("exc_blocks", 1, d["we"], d["exc_seed"]), ("inh_blocks", 2, d["wi"], d["inh_seed"])):
+        so, sn, to, tn = _blocks(d[key])
+        cids.append(h.connect_random_blocks(g, g, var, so, sn, to, tn, d["p"], w, seed, dmin=d["dmin"],
+                                            dmax=d["dmax"], max_delay_steps=d["max_delay_steps"]))
+    if record:
+        h.record(g, capacity or max(1 << 20, int(d["N"] * 40 * 0.5)))
+    h.prepare(window=window, neurons_per_thread=npt, use_graph=use_graph)
+    return h, g, cids[0], cids[1]
+
+
+def host_uneven_csr(d):
+    """The connectivity of device_uneven recomputed on the host."""
+    from oracle import conngen
+    out = {}
+    for key, seed in (("exc_blocks", d["exc_seed"]), ("inh_blocks", d["inh_seed"])):
+        out[key] = conngen.block_csr(d[key], d["p"], seed, d["dmin"], d["dmax"], 1e-4)
+    return out
+
+
+def oracle_uneven(d, csr):
+    A, C = cuba_propagator()
+    net = O.Net(1e-4)
+    grp = O.Group(d["N"], d["S0"], ("linear", A, C), ("const", 0, d["Vt"]),
+                  [(0, "=", d["Vr"], False)], dt=1e-4)
+    gid = net.add_group(grp)
+    for key, var, w in (("exc_blocks", 1, d["we"]), ("inh_blocks", 2, d["wi"])):
+        rp, col, dl = csr[key]
+        so, sn, to, tn = _blocks(d[key])
+        src_lo, src_hi = int(so.min()), int((so + sn).max())
+        tgt_lo, tgt_hi = int(to.min()), int((to + tn).max())
+        net.add_connection((gid, src_lo, src_hi - src_lo), (gid, tgt_lo, tgt_hi - tgt_lo), var, rp, col,
+                           np.full(len(col), w), syn_delay_steps=dl, max_delay_steps=d["max_delay_steps"])
+    return net, gid
