@@ -23,6 +23,11 @@ Pi32 = C.POINTER(C.c_int32)
 Pi64 = C.POINTER(C.c_int64)
 
 
+# bm_net_set_tuning keys (brianmi.h)
+TUNE = {"window": 1, "neurons_per_thread": 2, "wave_max": 3, "row_split": 4, "ring_bits": 5,
+        "ring_pack": 6, "use_graph": 7, "window_max": 8}
+
+
 class BrianMIError(RuntimeError):
     """Raised when a libbrianmi call returns a non-zero status."""
 
@@ -69,16 +74,24 @@ _SIGS = {
     "bm_net_get_weights": [P, C.c_int, Pd],
     "bm_net_get_stdp_traces": [P, C.c_int, Pd, Pd],
     "bm_net_set_profiling": [P, C.c_int],
+    "bm_net_set_tuning": [P, C.c_int, C.c_int],
+    "bm_net_add_state_monitor": [P, C.c_int, C.c_int, i64, Pi64, C.c_int, i64, C.POINTER(C.c_int)],
+    "bm_net_read_state_monitor": [P, C.c_int, Pd, i64, Pi64, Pi64],
+    "bm_net_get_tuning": [P, C.c_int, C.POINTER(C.c_int)],
     "bm_net_get_profile": [P, Pi64, Pd, Pd],
     "bm_probe_copy": [i64, C.c_int, Pd],
     "bm_stdp_batch_create": [C.c_int, dbl, C.c_int, C.c_int, C.c_int, Pd, Pd, Pd, C.c_int, dbl,
                              C.c_int, dbl, C.c_int, Pd, dbl, u64, dbl, dbl, dbl, dbl, dbl, dbl,
                              C.c_int, C.c_int, i64, C.POINTER(P)],
     "bm_stdp_batch_destroy": [P],
+    "bm_stdp_batch_set_profiling": [P, C.c_int],
+    "bm_stdp_batch_get_profile": [P, Pi64, Pd],
     "bm_fit_create": [C.c_int, dbl, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_char_p,
                       C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, i64, Pd, Pi32,
                       Pi32, Pi32, C.c_int, C.c_int, C.c_int, C.POINTER(P)],
     "bm_fit_features": [P, C.POINTER(C.c_float)],
+    "bm_fit_set_record": [P, i64],
+    "bm_fit_get_spikes": [P, Pi32, Pi32, i64, Pi64],
     "bm_fit_destroy": [P],
     "bm_mlp_num_params": [C.c_int, C.c_int, C.c_int, Pi64],
     "bm_mlp_create": [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(P)],
@@ -272,10 +285,15 @@ class NetHandle:
              float(dmin or 0.0), float(dmax or 0.0), int(max_delay_steps), C.byref(cid))
         return cid.value
 
-    def get_connection(self, cid, src_n):
+    def get_connection(self, cid, src_n, rows_only=False):
+        """(row_ptr, col, delay_steps) of a prepared connection; rows_only
+        reads back only the row pointers."""
         n = C.c_int64()
         call("bm_net_connection_nnz", self.h, cid, C.byref(n))
         rp = np.empty(src_n + 1, np.int64)
+        if rows_only:
+            call("bm_net_get_connection", self.h, cid, i64ptr(rp), None, None)
+            return rp
         col = np.empty(max(n.value, 1), np.int32)
         dl = np.empty(max(n.value, 1), np.int32)
         call("bm_net_get_connection", self.h, cid, i64ptr(rp), i32ptr(col), i32ptr(dl))
@@ -286,6 +304,32 @@ class NetHandle:
         call("bm_net_add_stdp", self.h, cid, decay_pre, decay_post, Ap, Am, wmin, wmax, update,
              interactions, C.byref(sid))
         return sid.value
+
+    def add_state_monitor(self, gid, var, indices=None, timestep=1, capacity=0):
+        """Device StateMonitor; indices None = record=False (sum / sum of squares)."""
+        mid = C.c_int()
+        if indices is None:
+            call("bm_net_add_state_monitor", self.h, gid, var, 0, None, int(timestep), 0, C.byref(mid))
+        else:
+            ix = as_i64(indices)
+            call("bm_net_add_state_monitor", self.h, gid, var, len(ix), i64ptr(ix), int(timestep),
+                 int(capacity), C.byref(mid))
+        return mid.value
+
+    def read_state_samples(self, mid, nrec):
+        """(first step, values [nsamples][nrec]) recorded since the last read."""
+        n, first = C.c_int64(), C.c_int64()
+        call("bm_net_read_state_monitor", self.h, mid, None, 0, C.byref(n), C.byref(first))
+        out = np.empty((n.value, nrec), dtype=np.float64)
+        call("bm_net_read_state_monitor", self.h, mid, dptr(out), n.value, C.byref(n), C.byref(first))
+        return first.value, out
+
+    def read_state_moments(self, mid, N):
+        """(steps accumulated, sum [N], sum of squares [N]) of a record=False monitor."""
+        n, first = C.c_int64(), C.c_int64()
+        out = np.empty((2, N), dtype=np.float64)
+        call("bm_net_read_state_monitor", self.h, mid, dptr(out), 0, C.byref(n), C.byref(first))
+        return n.value, out[0], out[1]
 
     def record(self, gid, capacity):
         call("bm_net_record", self.h, gid, int(capacity))
@@ -339,6 +383,14 @@ class NetHandle:
     def set_profiling(self, on):
         call("bm_net_set_profiling", self.h, 1 if on else 0)
 
+    def set_tuning(self, key, value):
+        call("bm_net_set_tuning", self.h, TUNE[key], int(value))
+
+    def get_tuning(self, key):
+        v = C.c_int()
+        call("bm_net_get_tuning", self.h, TUNE[key], C.byref(v))
+        return v.value
+
     def get_profile(self):
         n, ms, w = C.c_int64(), C.c_double(), C.c_double()
         call("bm_net_get_profile", self.h, C.byref(n), C.byref(ms), C.byref(w))
@@ -378,6 +430,14 @@ class StdpBatchHandle:
 
     def sync(self):
         call("bm_stdp_batch_sync", self.h)
+
+    def set_profiling(self, on):
+        call("bm_stdp_batch_set_profiling", self.h, 1 if on else 0)
+
+    def get_profile(self):
+        n, ms = C.c_int64(), C.c_double()
+        call("bm_stdp_batch_get_profile", self.h, C.byref(n), C.byref(ms))
+        return n.value, ms.value
 
     def _get(self, what, n):
         out = np.empty(n, dtype=np.float64)

@@ -34,13 +34,15 @@ __all__ = [
 ]
 
 _PREFS = {"precision": 64, "window": 0, "neurons_per_thread": 0, "use_graph": True,
-          "max_rate_hz": 250.0}
+          "max_rate_hz": 250.0, "autotune": False}
 
 
 def set_global_preferences(**kw):
     """brian/globalprefs.py:62-89.  Backend keys: precision (32 | 64),
     window (0 = automatic), neurons_per_thread, use_graph, max_rate_hz
-    (raster capacity sizing).  Unknown Brian keys are accepted and ignored."""
+    (raster capacity sizing), autotune (search the launch shape during the
+    first steps of the first long run; results are unchanged).  Unknown Brian
+    keys are accepted and ignored."""
     _PREFS.update(kw)
 
 
@@ -875,9 +877,11 @@ class PopulationRateMonitor(SpikeMonitor):
 
 
 class StateMonitor(_Tracked):
-    """brian/monitor.py:683-990.  Recording state forces the network to run
-    one step per launch with a read back after each step; use it for short
-    diagnostic runs."""
+    """brian/monitor.py:683-990.  Recorded on the device inside the sweep, at
+    the end of each step (after the resets), into a buffer that is drained
+    after every run; record=False keeps per-neuron running sums (mean, var,
+    std).  Samples are taken at steps 0, timestep, 2 timestep, ... of the
+    network's clock."""
 
     def __init__(self, P, varname, clock=None, record=False, timestep=1, when="end"):
         self._track(1)
@@ -885,37 +889,68 @@ class StateMonitor(_Tracked):
         self.varname = varname
         self.var = P._var(varname)
         self.timestep = int(timestep)
+        self._record_false = record is False
         if record is True:
             self.record = np.arange(len(P))
         elif record is False:
             self.record = np.zeros(0, np.int64)
         else:
             self.record = np.atleast_1d(np.asarray(record, dtype=np.int64))
-        self._values = []
-        self._times = []
-        self._count = 0
+        self._blocks = []
+        self._tblocks = []
+        self._net = None
+        self.mid = None
 
-    def _sample(self, t):
-        if self._count % self.timestep == 0:
-            v = self.P._get_state(self.var, self.P._origin, self.P.N)
-            self._values.append(v[self.record].copy())
-            self._times.append(t)
-        self._count += 1
+    def _drain(self):
+        net = self._net
+        if net is None or self.mid is None or self._record_false:
+            return
+        first, vals = net.handle.read_state_samples(self.mid, len(self.record))
+        if len(vals):
+            self._blocks.append(vals)
+            steps = first + np.arange(len(vals)) * self.timestep
+            self._tblocks.append((net._n0 + steps) * defaultclock.dt)
 
     @property
     def values(self):
-        return np.array(self._values).T if self._values else np.zeros((len(self.record), 0))
+        if not self._blocks:
+            return np.zeros((len(self.record), 0))
+        return np.concatenate(self._blocks, axis=0).T
 
     @property
     def times(self):
-        return np.array(self._times)
+        return np.concatenate(self._tblocks) if self._tblocks else np.zeros(0)
 
     def __getitem__(self, i):
         k = int(np.nonzero(self.record == i)[0][0])
         return self.values[k]
 
+    def _moments(self):
+        net = self._net
+        if net is None or self.mid is None:
+            raise RuntimeError("the monitor has not run")
+        owner = self.P._owner
+        n, sm, sq = net.handle.read_state_moments(self.mid, owner.N)
+        a, b = self.P._origin, self.P._origin + self.P.N
+        return n, sm[a:b], sq[a:b]
+
+    @property
+    def mean(self):
+        n, sm, _ = self._moments()
+        return sm / n
+
+    @property
+    def var(self):
+        n, sm, sq = self._moments()
+        m = sm / n
+        return sq / n - m * m
+
+    @property
+    def std(self):
+        return np.sqrt(np.maximum(self.var, 0.0))
+
     def reinit(self):
-        self._values, self._times, self._count = [], [], 0
+        self._blocks, self._tblocks = [], []
 
 
 # ------------------------------------------------------------------ network
@@ -1045,7 +1080,20 @@ class Network:
                 cap = max(1 << 20, int(g.N * self._planned_steps * dt * _PREFS["max_rate_hz"]))
                 self._capacity[id(g)] = cap
                 h.record(self.gid[id(g)], cap)
-        window = 1 if (state_mon or self.ops) else _PREFS["window"]
+        self._n0 = defaultclock._n
+        for m in self.monitors:
+            if not isinstance(m, StateMonitor):
+                continue
+            gid = self.gid[id(m.P._owner)]
+            m._net = self
+            if m._record_false:
+                m.mid = h.add_state_monitor(gid, m.var, None)
+            else:
+                nrec = max(len(m.record), 1)
+                budget = max(16, (256 << 20) // (nrec * (prec // 8)))
+                m._cap = int(min(self._planned_steps // m.timestep + 2, budget))
+                m.mid = h.add_state_monitor(gid, m.var, m.P._origin + m.record, m.timestep, m._cap)
+        window = 1 if self.ops else _PREFS["window"]
         h.prepare(window=window, neurons_per_thread=_PREFS["neurons_per_thread"],
                   use_graph=_PREFS["use_graph"])
 
@@ -1088,22 +1136,34 @@ class Network:
         if self.handle is None:
             self._planned_steps = max(nsteps, 1)
             self._compile()
-        state_mons = [m for m in self.monitors if isinstance(m, StateMonitor)]
+        state_mons = [m for m in self.monitors if isinstance(m, StateMonitor) and not m._record_false]
         _StopFlag.stop = False
-        if state_mons or self.ops:
+        if self.ops:
             for _ in range(nsteps):
-                t = defaultclock.t
                 self.handle.run(1)
                 self.steps_run += 1
                 defaultclock._n += 1
                 for m in state_mons:
-                    m._sample(t)
+                    m._drain()
                 for op in self.ops:
                     op()
                 if _StopFlag.stop:
                     break
         else:
-            self.handle.run(nsteps)
+            done = 0
+            if (_PREFS["autotune"] and not state_mons and not getattr(self, "_tuned", False)
+                    and nsteps >= 20000):
+                from . import autotune
+                self.tuning, done = autotune.tune(self.handle, trial_steps=128, max_steps=nsteps // 4)
+                self._tuned = True
+            # state monitor buffers hold _cap samples: drain at least that often
+            chunk = min([(m._cap - 1) * m.timestep for m in state_mons] + [nsteps - done]) or 1
+            while done < nsteps:
+                k = min(chunk, nsteps - done)
+                self.handle.run(k)
+                done += k
+                for m in state_mons:
+                    m._drain()
             self.handle.sync()
             self.steps_run += nsteps
             defaultclock._n += nsteps

@@ -31,7 +31,8 @@ extern "C" __global__ __launch_bounds__(256) void fit_runsim(
     const int *__restrict__ spiketimes, const int *__restrict__ train_off, int *st_index,
     unsigned char *allowed, const int *__restrict__ delay_arr, const int *__restrict__ refr_arr,
     int *next_allowed_arr, int Tstart, int Tend, int onset, int delta, int ncand, T dt,
-    float *feat, int *last_spk) {
+    float *feat, int *last_spk, unsigned long long *rec, unsigned long long *rec_count,
+    long long rec_cap) {
 #pragma clang fp contract(off)
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncand) return;
@@ -64,6 +65,7 @@ extern "C" __global__ __launch_bounds__(256) void fit_runsim(
 %RESET2%
       next_allowed = T_ + refractory;
 %FEAT_SPIKE%
+%REC_SPIKE%
     }
     const int Tspike = T_ + spikedelay;
     bool near_last = last_spike_time + delta >= Tspike;
@@ -122,6 +124,14 @@ static const char *kFeatStore = R"F(
 #pragma unroll
   for (int k = 0; k < 32; ++k) feat[(size_t)c * 66 + 32 + k] = sta[k];
   feat[(size_t)c * 66 + 64] = nsta;
+)F";
+
+// Record mode: every spike appends (step << 32 | candidate) to a raster.
+static const char *kRecSpike = R"F(
+      {
+        const unsigned long long pos = atomicAdd(rec_count, 1ull);
+        if (pos < (unsigned long long)rec_cap) rec[pos] = ((unsigned long long)T_ << 32) | (unsigned)c;
+      }
 )F";
 
 struct FitModule {
@@ -185,7 +195,9 @@ struct bm_fit {
   FitModule mod;
   DevBuf I, cand_target, state, params, spikecount, ncoinc, spiketimes, train_off, st_index, allowed,
       delay, refr, next_allowed, feat, last_spk;
-  bool features = false;
+  bool features = false, record = false;
+  DevBuf rec, rec_count;
+  int64_t rec_cap = 0;
   std::vector<double> init_state;     // [nvars][ncand]
   std::vector<int32_t> train_start;   // per target: offset of its padded train
   std::vector<int32_t> cand_tgt;
@@ -220,20 +232,25 @@ void upload(bm_fit *h, DevBuf &b, const R *src, size_t n) {
 // Puts every candidate back at the start of the run: state, counters,
 // coincidence state (next_spike_allowed=1, last_spike_allowed=0 as in
 // gpu_modelfitting.py:364-365), refractoriness.
+void fill_bytes(bm_fit *h, DevBuf &b, size_t n, int byte) {
+  b.alloc(n);
+  if (n) BM_HIP(hipMemsetAsync(b.p, byte, n, h->stream));
+}
+
 void reset_counters(bm_fit *h) {
   upload_scalar(h, h->state, h->init_state.data(), h->init_state.size());
-  std::vector<int32_t> zeros(h->ncand, 0), neg(h->ncand, -1);
-  std::vector<uint8_t> allow(h->ncand, 2);
-  upload(h, h->spikecount, zeros.data(), zeros.size());
-  upload(h, h->ncoinc, zeros.data(), zeros.size());
-  upload(h, h->st_index, zeros.data(), zeros.size());
-  upload(h, h->allowed, allow.data(), allow.size());
-  upload(h, h->next_allowed, neg.data(), neg.size());
+  const size_t n = (size_t)h->ncand;
+  fill_bytes(h, h->spikecount, n * 4, 0);
+  fill_bytes(h, h->ncoinc, n * 4, 0);
+  fill_bytes(h, h->st_index, n * 4, 0);
+  fill_bytes(h, h->allowed, n, 2);           // next_spike_allowed only
+  fill_bytes(h, h->next_allowed, n * 4, 0xFF);   // -1
   if (h->features) {
-    std::vector<float> fz((size_t)h->ncand * 66, 0.f);
-    upload(h, h->feat, fz.data(), fz.size());
-    upload(h, h->last_spk, neg.data(), neg.size());
+    fill_bytes(h, h->feat, n * 66 * 4, 0);
+    fill_bytes(h, h->last_spk, n * 4, 0xFF);
   }
+  if (h->record && h->rec_cap > 0) fill_bytes(h, h->rec_count, 8, 0);
+  BM_HIP(hipStreamSynchronize(h->stream));
 }
 }  // namespace
 
@@ -268,10 +285,13 @@ int bm_fit_create(int precision, double dt, int ncand, int ntarget, int nvars, i
     replace_all(src, "%RESET2%", kernel_reset2);
     replace_all(src, "%STORE%", kernel_store);
     replace_all(src, "%INPUT%", input_name);
-    h->features = features != 0;
+    BM_REQUIRE(features >= 0 && features <= 3, "features is a bit mask: 1 features, 2 spike raster");
+    h->features = (features & 1) != 0;
+    h->record = (features & 2) != 0;
     replace_all(src, "%FEAT_LOAD%", h->features ? kFeatLoad : "");
     replace_all(src, "%FEAT_SPIKE%", h->features ? kFeatSpike : "");
     replace_all(src, "%FEAT_STORE%", h->features ? kFeatStore : "");
+    replace_all(src, "%REC_SPIKE%", h->record ? kRecSpike : "");
     h->mod = compile_fit(src);
     upload_scalar(h.get(), h->I, I, (size_t)ntarget * nsteps_input);
     for (int c = 0; c < ncand; ++c)
@@ -337,9 +357,12 @@ int bm_fit_run(bm_fit *h, int64_t t0, int64_t t1) {
     double dt64 = h->dt;
     float dt32 = (float)h->dt;
     void *fe = h->features ? h->feat.p : nullptr, *ls = h->features ? h->last_spk.p : nullptr;
+    BM_REQUIRE(!h->record || h->rec_cap > 0, "record mode needs bm_fit_set_record before running");
+    void *rc = h->record ? h->rec.p : nullptr, *rn = h->record ? h->rec_count.p : nullptr;
+    long long cap = h->rec_cap;
     void *args[] = {&I, &stride, &ct, &st, &pa, &sc, &nc, &tr, &to, &si, &al, &de, &re, &na,
                     &T0, &T1, &onset, &delta, &n,
-                    h->precision == 64 ? (void *)&dt64 : (void *)&dt32, &fe, &ls};
+                    h->precision == 64 ? (void *)&dt64 : (void *)&dt32, &fe, &ls, &rc, &rn, &cap};
     const unsigned blocks = (unsigned)((n + 255) / 256);
     BM_HIP(hipModuleLaunchKernel(h->mod.fn, blocks, 1, 1, 256, 1, 1, 0, h->stream, args, nullptr));
   });
@@ -354,6 +377,41 @@ int bm_fit_results(bm_fit *h, int32_t *ncoinc, int32_t *nspikes) {
     BM_HIP(hipMemcpyAsync(ncoinc, h->ncoinc.p, (size_t)h->ncand * 4, hipMemcpyDeviceToHost, h->stream));
     BM_HIP(hipMemcpyAsync(nspikes, h->spikecount.p, (size_t)h->ncand * 4, hipMemcpyDeviceToHost, h->stream));
     BM_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int bm_fit_set_record(bm_fit *h, int64_t capacity) {
+  return guard([&] {
+    BM_REQUIRE(h->record, "the fitting handle was created without record mode");
+    BM_REQUIRE(capacity > 0, "capacity must be > 0");
+    h->rec.alloc((size_t)capacity * 8);
+    h->rec_cap = capacity;
+    fill_bytes(h, h->rec_count, 8, 0);
+    BM_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int bm_fit_get_spikes(bm_fit *h, int32_t *steps, int32_t *cand, int64_t cap, int64_t *n_out) {
+  return guard([&] {
+    BM_REQUIRE(h->record && h->rec_cap > 0, "no spike raster recorded");
+    unsigned long long cnt = 0;
+    BM_HIP(hipMemcpyAsync(&cnt, h->rec_count.p, 8, hipMemcpyDeviceToHost, h->stream));
+    BM_HIP(hipStreamSynchronize(h->stream));
+    BM_REQUIRE(cnt <= (unsigned long long)h->rec_cap, "spike raster capacity exceeded");
+    if (!steps) {
+      *n_out = (int64_t)cnt;
+      return;
+    }
+    std::vector<unsigned long long> r(cnt);
+    if (cnt) BM_HIP(hipMemcpyAsync(r.data(), h->rec.p, cnt * 8, hipMemcpyDeviceToHost, h->stream));
+    BM_HIP(hipStreamSynchronize(h->stream));
+    std::sort(r.begin(), r.end());
+    const int64_t n = std::min<int64_t>((int64_t)cnt, cap);
+    for (int64_t k = 0; k < n; ++k) {
+      steps[k] = (int32_t)(r[k] >> 32);
+      cand[k] = (int32_t)(r[k] & 0xFFFFFFFFull);
+    }
+    *n_out = n;
   });
 }
 

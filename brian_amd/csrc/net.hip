@@ -51,6 +51,14 @@ struct HostGroup {
   int64_t gen_nsteps = 0;
   int64_t host_raster_read = 0;
   GroupDev dev{};
+  struct Mon {
+    int var = 0, timestep = 1, mode = 0;
+    int64_t cap = 0;
+    std::vector<int64_t> idx;
+    DevBuf slot_d, buf_d, base_d;
+    int64_t base = 0;       // host copy of *base_d
+  };
+  std::vector<std::unique_ptr<Mon>> mons;
 };
 
 struct HostConn {
@@ -92,7 +100,12 @@ struct bm_net {
   std::vector<std::unique_ptr<HostStdp>> stdps;
   bool prepared = false;
   int W = 1, K = 4;
+  int Wcap = 1;           // largest window the rings are sized for
   bool use_graph = true;
+  std::vector<std::pair<int, int>> mon_ids;   // (group, monitor within group)
+  // tiling knobs (bm_net_set_tuning)
+  int wave_max = BLOCK / 64, row_split = 64;
+  int ring_bits = 0, ring_pack = 1;
   int64_t step = 0;       // next step to run
   hipStream_t stream = nullptr;
   DevBuf d_base;          // int64 device step base
@@ -155,6 +168,7 @@ void require_unprepared(bm_net *net) {
 // (packed count) ring, no timed input.
 bool fast_eligible(const HostGroup &g) {
   if (g.kind != GK_NEURON || g.upd != UPD_LINEAR || g.M < 1 || g.M > 4) return false;
+  if (!g.mons.empty()) return false;
   if (g.refr != REFR_NONE || g.th_kind == TH_EMPIRICAL || g.ti_nt > 0) return false;
   if (g.ring_desc.size() > 1) return false;
   if (g.ring_desc.size() == 1 && g.ring_desc[0].kind != RING_COUNT) return false;
@@ -166,6 +180,22 @@ int group_K(bm_net *net, const HostGroup &g) {
   if (g.upd == UPD_PROGRAM) return 1;
   if (g.upd == UPD_PERNEURON && g.M > 3) return 1;
   return net->K >= 4 ? 4 : 1;
+}
+
+size_t sweep_shmem(bm_net *net, const HostGroup &g, int K, int W) {
+  size_t shmem = sizeof(uint16_t) * BLOCK * K * std::max(W, 1) + 16;
+  if (g.upd == UPD_PROGRAM) shmem += (size_t)g.prog_nregs * BLOCK * net->tsize();
+  return shmem;
+}
+
+// Every group's sweep must fit the 64 KiB of dynamic LDS a launch gets
+// without raising the per-kernel limit.
+void check_launch_shapes(bm_net *net) {
+  for (auto &gp : net->groups) {
+    if (gp->kind == GK_GENERATOR) continue;
+    BM_REQUIRE(sweep_shmem(net, *gp, group_K(net, *gp), net->W) <= 65536,
+               "window x neurons_per_thread too large for the workgroup's spike list (64 KiB LDS)");
+  }
 }
 
 void launch_group(bm_net *net, HostGroup &g, int64_t rel0, int W, int skip_first_tail,
@@ -181,8 +211,7 @@ void launch_group(bm_net *net, HostGroup &g, int64_t rel0, int W, int skip_first
   const int64_t per_block = (int64_t)BLOCK * K;
   const int64_t blocks = (g.N + per_block - 1) / per_block;
   if (blocks == 0) return;
-  size_t shmem = sizeof(uint16_t) * BLOCK * K * std::max(W, 1) + 16;
-  if (g.upd == UPD_PROGRAM) shmem += (size_t)g.prog_nregs * BLOCK * net->tsize();
+  const size_t shmem = sweep_shmem(net, g, K, W);
   const bool fast = fast_eligible(g);
   SweepArgs a{dbase, rel0, W, skip_first_tail, tail_only};
   if (net->precision == 64) launch_sweep<double>(g.dev, fast, g.M, K, (unsigned)blocks, shmem, net->stream, a);
@@ -232,23 +261,23 @@ void build_stdp(bm_net *net, HostStdp &s) {
 
 void prepare(bm_net *net, int window, int K, int use_graph) {
   BM_REQUIRE(!net->prepared, "already prepared");
-  BM_REQUIRE(K == 0 || K == 1 || K == 2 || K == 4 || K == 8, "neurons_per_thread must be 0, 1, 2, 4 or 8");
+  BM_REQUIRE(K == 0 || K == 1 || K == 2 || K == 4, "neurons_per_thread must be 0, 1, 2 or 4");
   net->K = K == 0 ? 4 : K;
   net->use_graph = use_graph != 0;
   // ---- window
   int dmin = 1 << 30;
   for (auto &cp : net->conns) dmin = std::min(dmin, cp->min_delay);
-  int W;
-  if (window > 0) {
-    W = window;
-  } else {
-    W = net->conns.empty() ? WMAX : std::min(WMAX, dmin + 1);
-  }
+  int Wcap = net->conns.empty() ? WMAX : std::min(WMAX, dmin + 1);
+  if (!net->stdps.empty()) Wcap = 1;
+  int W = window > 0 ? window : Wcap;
   if (!net->stdps.empty()) W = 1;
   BM_REQUIRE(W >= 1 && W <= WMAX, "window out of range [1,16]");
   BM_REQUIRE(net->conns.empty() || W <= dmin + 1,
              "window exceeds (minimum synaptic delay + 1) steps");
   net->W = W;
+  // rings are sized for the largest admissible window so the window can be
+  // retuned between runs without moving pending deliveries
+  net->Wcap = std::max(W, Wcap);
 
   // ---- per-connection storage and count/float mode
   for (size_t ci = 0; ci < net->conns.size(); ++ci) {
@@ -303,14 +332,16 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     std::stable_sort(incoming.begin(), incoming.end(), [&](int a, int b) {
       return net->conns[a]->delay_kind < net->conns[b]->delay_kind;
     });
-    g.D = W + dmax + 1;
+    g.D = net->Wcap + dmax + 1;
     // count channels packed per ring: 8-bit counts when every in-degree <= 255
     std::vector<int> cnt_chans, flt_chans;
     for (int ci : incoming) (net->conns[ci]->count_mode ? cnt_chans : flt_chans).push_back(ci);
     bool small = true;
     for (int ci : cnt_chans) small = small && net->conns[ci]->max_indeg <= 255;
+    BM_REQUIRE(net->ring_bits != 8 || small, "8-bit ring counters need every in-degree <= 255");
+    if (net->ring_bits == 16) small = false;
     const int bits = small ? 8 : 16;
-    const int per_ring = small ? 4 : 2;
+    const int per_ring = net->ring_pack ? (small ? 4 : 2) : 1;
     g.rings.clear();
     g.ring_desc.clear();
     for (size_t a = 0; a < cnt_chans.size(); a += per_ring) {
@@ -433,6 +464,34 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       d.refr_arr = g.refr_d.as<double>();
     }
     d.D = g.D;
+    d.wave_max = net->wave_max;
+    d.row_split = net->row_split;
+    d.nmon = (int)g.mons.size();
+    for (int m = 0; m < d.nmon; ++m) {
+      HostGroup::Mon &hm = *g.mons[m];
+      MonDev &md = d.mons[m];
+      md.var = hm.var;
+      md.timestep = hm.timestep;
+      md.mode = hm.mode;
+      int64_t zero64 = 0;
+      upload_raw(hm.base_d, &zero64, 1, net->stream);
+      md.base = hm.base_d.as<int64_t>();
+      if (hm.mode == 0) {
+        std::vector<int32_t> slot(g.N, -1);
+        for (size_t q = 0; q < hm.idx.size(); ++q) slot[hm.idx[q]] = (int32_t)q;
+        upload_raw(hm.slot_d, slot.data(), slot.size(), net->stream);
+        hm.buf_d.alloc((size_t)std::max<int64_t>(hm.cap * (int64_t)hm.idx.size(), 1) * net->tsize());
+        md.slot = hm.slot_d.as<int32_t>();
+        md.nrec = (int64_t)hm.idx.size();
+        md.cap = hm.cap;
+      } else {
+        std::vector<double> z((size_t)2 * g.N, 0.0);
+        upload_raw(hm.buf_d, z.data(), z.size(), net->stream);
+        md.nrec = 0;
+        md.cap = 0;
+      }
+      md.buf = hm.buf_d.p;
+    }
     d.nring = (int)g.ring_desc.size();
     for (int r = 0; r < d.nring; ++r) d.rings[r] = g.ring_desc[r];
     d.nchan = (int)g.chans.size();
@@ -558,6 +617,7 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     d.post_H = tg.H;
     d.post_N = tg.N;
   }
+  check_launch_shapes(net);
   net->d_base.alloc(8);
   int64_t zero = 0;
   BM_HIP(hipMemcpyAsync(net->d_base.p, &zero, 8, hipMemcpyHostToDevice, net->stream));
@@ -992,7 +1052,7 @@ int bm_net_get_connection(bm_net *net, int cid, int64_t *row_ptr, int32_t *col, 
     HostConn &c = *net->conns[cid];
     BM_REQUIRE(net->prepared, "network not prepared");
     BM_HIP(hipMemcpyAsync(row_ptr, c.row_ptr_d.p, (size_t)(c.src_n + 1) * 8, hipMemcpyDeviceToHost, net->stream));
-    if (c.nnz) BM_HIP(hipMemcpyAsync(col, c.col_d.p, (size_t)c.nnz * 4, hipMemcpyDeviceToHost, net->stream));
+    if (c.nnz && col) BM_HIP(hipMemcpyAsync(col, c.col_d.p, (size_t)c.nnz * 4, hipMemcpyDeviceToHost, net->stream));
     BM_HIP(hipStreamSynchronize(net->stream));
     if (delay_steps) {
       if (c.delay_kind == 0) {
@@ -1039,6 +1099,80 @@ int bm_net_add_stdp(bm_net *net, int cid, double decay_pre, double decay_post, d
   });
 }
 
+int bm_net_add_state_monitor(bm_net *net, int gid, int var, int64_t nrec, const int64_t *indices,
+                             int timestep, int64_t capacity, int *mid) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.kind == GK_NEURON && var >= 0 && var < g.M, "monitored variable out of range");
+    BM_REQUIRE(g.mons.size() < (size_t)MAXMON, "too many state monitors on one group");
+    BM_REQUIRE(timestep >= 1, "timestep must be >= 1");
+    auto m = std::make_unique<HostGroup::Mon>();
+    m->var = var;
+    m->timestep = timestep;
+    if (indices == nullptr) {
+      BM_REQUIRE(nrec == 0, "mean/variance mode takes no indices");
+      m->mode = 1;
+    } else {
+      BM_REQUIRE(nrec > 0 && capacity > 0, "nrec and capacity must be > 0");
+      m->mode = 0;
+      m->cap = capacity;
+      std::vector<char> seen(g.N, 0);
+      for (int64_t q = 0; q < nrec; ++q) {
+        BM_REQUIRE(indices[q] >= 0 && indices[q] < g.N, "monitored neuron out of range");
+        BM_REQUIRE(!seen[indices[q]], "monitored neurons must be distinct");
+        seen[indices[q]] = 1;
+      }
+      m->idx.assign(indices, indices + nrec);
+    }
+    g.mons.push_back(std::move(m));
+    net->mon_ids.push_back({gid, (int)g.mons.size() - 1});
+    *mid = (int)net->mon_ids.size() - 1;
+  });
+}
+
+int bm_net_read_state_monitor(bm_net *net, int mid, double *out, int64_t max_samples, int64_t *nsamples,
+                              int64_t *first_step) {
+  return guard([&] {
+    BM_REQUIRE(net->prepared, "network not prepared");
+    BM_REQUIRE(mid >= 0 && mid < (int)net->mon_ids.size(), "bad monitor id");
+    HostGroup &g = *net->groups[net->mon_ids[mid].first];
+    HostGroup::Mon &m = *g.mons[net->mon_ids[mid].second];
+    BM_HIP(hipStreamSynchronize(net->stream));
+    if (m.mode == 1) {
+      // steps accumulated so far; out = [sum (N), sum of squares (N)]
+      *nsamples = net->step;
+      *first_step = 0;
+      if (out) BM_HIP(hipMemcpy(out, m.buf_d.p, (size_t)2 * g.N * 8, hipMemcpyDeviceToHost));
+      return;
+    }
+    // steps < net->step are complete; samples at steps 0, ts, 2 ts, ...
+    const int64_t total = (net->step + m.timestep - 1) / m.timestep;
+    const int64_t avail = total - m.base;
+    BM_REQUIRE(avail <= m.cap, "state monitor buffer overrun: read it back more often");
+    *first_step = m.base * m.timestep;
+    if (!out) {
+      *nsamples = avail;
+      return;
+    }
+    const int64_t n = std::min(avail, max_samples);
+    const int64_t nrec = (int64_t)m.idx.size();
+    if (n > 0) {
+      if (net->precision == 64) {
+        BM_HIP(hipMemcpy(out, m.buf_d.p, (size_t)(n * nrec) * 8, hipMemcpyDeviceToHost));
+      } else {
+        std::vector<float> t((size_t)(n * nrec));
+        BM_HIP(hipMemcpy(t.data(), m.buf_d.p, t.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < t.size(); ++q) out[q] = t[q];
+      }
+    }
+    BM_REQUIRE(n == avail, "max_samples smaller than the samples held: read all of them");
+    m.base += n;
+    BM_HIP(hipMemcpy(m.base_d.p, &m.base, 8, hipMemcpyHostToDevice));
+    *nsamples = n;
+  });
+}
+
 int bm_net_record(bm_net *net, int gid, int64_t capacity) {
   return guard([&] {
     require_unprepared(net);
@@ -1050,6 +1184,91 @@ int bm_net_record(bm_net *net, int gid, int64_t capacity) {
 
 int bm_net_prepare(bm_net *net, int window, int neurons_per_thread, int use_graph) {
   return guard([&] { prepare(net, window, neurons_per_thread, use_graph); });
+}
+
+int bm_net_set_tuning(bm_net *net, int key, int value) {
+  return guard([&] {
+    switch (key) {
+      case BM_TUNE_WINDOW: {
+        BM_REQUIRE(net->prepared, "the window is set by bm_net_prepare before preparation");
+        const int W = value == 0 ? net->Wcap : value;
+        BM_REQUIRE(W >= 1 && W <= net->Wcap, "window out of range [1, minimum delay + 1]");
+        BM_REQUIRE(net->stdps.empty() || W == 1, "networks with STDP run one step per launch");
+        const int old = net->W;
+        net->W = W;
+        try {
+          check_launch_shapes(net);
+        } catch (...) {
+          net->W = old;
+          throw;
+        }
+        break;
+      }
+      case BM_TUNE_NEURONS_PER_THREAD: {
+        BM_REQUIRE(value == 1 || value == 2 || value == 4, "neurons_per_thread must be 1, 2 or 4");
+        const int old = net->K;
+        net->K = value;
+        if (net->prepared) {
+          try {
+            check_launch_shapes(net);
+          } catch (...) {
+            net->K = old;
+            throw;
+          }
+        }
+        break;
+      }
+      case BM_TUNE_WAVE_MAX:
+        BM_REQUIRE(value >= 0, "wave_max must be >= 0");
+        net->wave_max = value;
+        for (auto &g : net->groups) g->dev.wave_max = value;
+        break;
+      case BM_TUNE_ROW_SPLIT:
+        BM_REQUIRE(value >= 1 && value <= 64 && (value & (value - 1)) == 0,
+                   "row_split must be a power of two in [1, 64]");
+        net->row_split = value;
+        for (auto &g : net->groups) g->dev.row_split = value;
+        break;
+      case BM_TUNE_RING_BITS:
+        require_unprepared(net);
+        BM_REQUIRE(value == 0 || value == 8 || value == 16, "ring_bits must be 0 (automatic), 8 or 16");
+        net->ring_bits = value;
+        break;
+      case BM_TUNE_RING_PACK:
+        require_unprepared(net);
+        BM_REQUIRE(value == 0 || value == 1, "ring_pack must be 0 or 1");
+        net->ring_pack = value;
+        break;
+      case BM_TUNE_USE_GRAPH:
+        net->use_graph = value != 0;
+        break;
+      default:
+        throw Error(BM_EINVAL, "unknown tuning key");
+    }
+    // captured launches hold the old configuration
+    if (net->graph_exec) {
+      BM_HIP(hipStreamSynchronize(net->stream));
+      hipGraphExecDestroy(net->graph_exec);
+      net->graph_exec = nullptr;
+      net->graph_windows = 0;
+    }
+  });
+}
+
+int bm_net_get_tuning(bm_net *net, int key, int *value) {
+  return guard([&] {
+    switch (key) {
+      case BM_TUNE_WINDOW: *value = net->W; break;
+      case BM_TUNE_NEURONS_PER_THREAD: *value = net->K; break;
+      case BM_TUNE_WAVE_MAX: *value = net->wave_max; break;
+      case BM_TUNE_ROW_SPLIT: *value = net->row_split; break;
+      case BM_TUNE_RING_BITS: *value = net->ring_bits; break;
+      case BM_TUNE_RING_PACK: *value = net->ring_pack; break;
+      case BM_TUNE_USE_GRAPH: *value = net->use_graph ? 1 : 0; break;
+      case BM_TUNE_WINDOW_MAX: *value = net->prepared ? net->Wcap : 0; break;
+      default: throw Error(BM_EINVAL, "unknown tuning key");
+    }
+  });
 }
 
 int bm_net_run(bm_net *net, int64_t nsteps) {

@@ -233,13 +233,17 @@ __device__ inline int block_exclusive_scan(int v, int tid, int *sh_wsum, int *to
   return before + x - v;
 }
 
-// Few spikes: one wavefront per spike, lanes stride over its row (two
-// dependent round trips per spike, no block barriers).
+// Few spikes: each wavefront takes 64/row_split spikes at a time, row_split
+// lanes stride over each row (two dependent round trips per spike, no block
+// barriers).
 template <typename T>
 __device__ inline void propagate_waves(const GroupDev &g, const uint16_t *sh_list, int nsp, int64_t n0,
                                        int blk0, int tid) {
   const int wave = tid >> 6, lane = tid & 63;
-  for (int p = wave; p < nsp; p += BLOCK / 64) {
+  const int rs = g.row_split;                       // power of two in [1, 64]
+  const int per_wave = 64 / rs;
+  const int sub = lane / rs, sl = lane & (rs - 1);
+  for (int p = wave * per_wave + sub; p < nsp; p += (BLOCK / 64) * per_wave) {
     const uint32_t e = sh_list[p];
     const int64_t n = n0 + (e >> 12);
     const int i = blk0 + (int)(e & 0xFFFu);
@@ -250,7 +254,7 @@ __device__ inline void propagate_waves(const GroupDev &g, const uint16_t *sh_lis
       const int base_slot = (int)(n % od.D);
       const int64_t lo = od.row_ptr[r], hi = od.row_ptr[r + 1];
       const int pbits = od.nch * od.bits;
-      for (int64_t q = lo + lane; q < hi; q += 64) {
+      for (int64_t q = lo + sl; q < hi; q += rs) {
         const int64_t tj = od.tgt_off + od.col[q];
         int d = od.hom_delay;
         if (od.dly_bytes == 1) d = static_cast<const uint8_t *>(od.dly)[q];
@@ -272,7 +276,7 @@ __device__ inline void propagate_waves(const GroupDev &g, const uint16_t *sh_lis
 template <typename T>
 __device__ inline void propagate_block(const GroupDev &g, const uint16_t *sh_list, int nsp, int64_t n0,
                                        int blk0, int tid) {
-  if (nsp <= BLOCK / 64) {
+  if (nsp <= g.wave_max) {
     propagate_waves<T>(g, sh_list, nsp, n0, blk0, tid);
     return;
   }
@@ -543,6 +547,33 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
 #pragma unroll
               for (int j = 0; j < MM; ++j)
                 if (j == g.rs_var[r]) s[j][k] = (g.rs_op[r] == 0) ? v : s[j][k] + v;
+            }
+          }
+        }
+        // StateMonitor at the end of step kst
+        if constexpr (GEN) {
+          for (int m = 0; m < g.nmon; ++m) {
+            const MonDev &md = g.mons[m];
+            if (md.mode == 0 && kst % md.timestep != 0) continue;
+            const int64_t sample = kst / md.timestep - *md.base;
+            if (md.mode == 0 && (sample < 0 || sample >= md.cap)) continue;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              if (!valid[k]) continue;
+              const int i = blk0 + k * BLOCK + tid;
+              T v = (T)0;
+#pragma unroll
+              for (int j = 0; j < MM; ++j)
+                if (j == md.var) v = s[j][k];
+              if (md.mode == 1) {
+                double *acc = static_cast<double *>(md.buf);
+                const double x = (double)v;
+                acc[i] = acc[i] + x;
+                acc[(int64_t)N + i] = acc[(int64_t)N + i] + x * x;
+              } else {
+                const int32_t col = md.slot[i];
+                if (col >= 0) static_cast<T *>(md.buf)[sample * md.nrec + col] = v;
+              }
             }
           }
         }

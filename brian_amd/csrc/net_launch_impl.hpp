@@ -9,9 +9,11 @@ using SweepFn = void (*)(const GroupDev, const int64_t *, int64_t, int, int, int
 
 template <typename T, int M>
 SweepFn pick_gen(int K) {
+  // the grid is sized by the caller for exactly this K
   switch (K) {
     case 1: return k_sweep<T, M, 1, F_GEN>;
-    default: return k_sweep<T, M, 4, F_GEN>;
+    case 4: return k_sweep<T, M, 4, F_GEN>;
+    default: throw Error(1 /* BM_EINVAL */, "general sweep kernel exists for 1 or 4 neurons per thread");
   }
 }
 
@@ -20,7 +22,8 @@ SweepFn pick_fast(int K) {
   switch (K) {
     case 1: return k_sweep<T, M, 1, 0>;
     case 2: return k_sweep<T, M, 2, 0>;
-    default: return k_sweep<T, M, 4, 0>;
+    case 4: return k_sweep<T, M, 4, 0>;
+    default: throw Error(1 /* BM_EINVAL */, "lean sweep kernel exists for 1, 2 or 4 neurons per thread");
   }
 }
 

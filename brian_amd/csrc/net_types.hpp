@@ -11,6 +11,7 @@ constexpr int MAXRING = 4;
 constexpr int MAXCHAN = 8;
 constexpr int MAXOUT = 8;
 constexpr int MAXRESET = 4;
+constexpr int MAXMON = 4;
 constexpr int BLOCK = 256;
 constexpr int WMAX = 16;
 constexpr int32_t NEVER = INT32_MIN;
@@ -63,6 +64,19 @@ struct ProgDev {
   int ninstr, nregs, scheme, pad;
 };
 
+// StateMonitor recorded inside the sweep (brian/monitor.py:683-990), after
+// the resets of each step ('end' slot).  mode 0: the variable of selected
+// neurons every timestep steps into buf [cap][nrec] (T), sample index
+// step/timestep - *base; mode 1 (record=False): per-neuron running sum and
+// sum of squares in double [2][N].
+struct MonDev {
+  int var, timestep, mode, pad;
+  const int32_t *slot;       // [N] column in buf, -1 if not recorded (mode 0)
+  void *buf;
+  int64_t nrec, cap;
+  const int64_t *base;       // device: sample index held in buf row 0
+};
+
 struct GroupDev {
   int64_t N;
   int M, kind, upd, pad0;
@@ -102,6 +116,13 @@ struct GroupDev {
   const int32_t *gen_idx;
   const int64_t *gen_off;     // [gen_nsteps+1]
   int64_t gen_nsteps;
+  // propagation tiling: a block whose window produced at most wave_max spikes
+  // propagates them wavefront-wise with row_split lanes per CSR row (so
+  // 64/row_split rows per wavefront); more spikes use the block-cooperative
+  // scan over the concatenated rows.
+  int wave_max, row_split;
+  int nmon, pad2;
+  MonDev mons[MAXMON];
 };
 
 

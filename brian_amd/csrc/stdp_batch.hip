@@ -219,7 +219,14 @@ struct bm_stdp_batch {
   DevBuf w, wc, apre, S, apost, raster, raster_count;
   hipStream_t stream = nullptr;
   int64_t step = 0;
+  // optional kernel timing with events on the handle's stream
+  bool profiling = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int64_t prof_launches = 0;
+  double prof_ms = 0;
   ~bm_stdp_batch() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -331,11 +338,39 @@ int bm_stdp_batch_run(bm_stdp_batch *h, int64_t nsteps, int steps_per_launch) {
     const int chunk = steps_per_launch > 0 ? steps_per_launch : 20000;
     for (int64_t done = 0; done < nsteps; done += chunk) {
       const int n = (int)std::min<int64_t>(chunk, nsteps - done);
+      if (h->profiling) BM_HIP(hipEventRecord(h->ev0, h->stream));
       if (h->precision == 64) launch_stdp_batch<double, false>(h->dev, h->step + done, n, h->stream);
       else launch_stdp_batch<float, true>(h->dev, h->step + done, n, h->stream);
       BM_HIP(hipGetLastError());
+      if (h->profiling) {
+        BM_HIP(hipEventRecord(h->ev1, h->stream));
+        BM_HIP(hipEventSynchronize(h->ev1));
+        float ms = 0;
+        BM_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        h->prof_ms += ms;
+        h->prof_launches += 1;
+      }
     }
     h->step += nsteps;
+  });
+}
+
+int bm_stdp_batch_set_profiling(bm_stdp_batch *h, int on) {
+  return guard([&] {
+    if (on && !h->ev0) {
+      BM_HIP(hipEventCreate(&h->ev0));
+      BM_HIP(hipEventCreate(&h->ev1));
+    }
+    h->profiling = on != 0;
+    h->prof_launches = 0;
+    h->prof_ms = 0;
+  });
+}
+
+int bm_stdp_batch_get_profile(bm_stdp_batch *h, int64_t *launches, double *total_ms) {
+  return guard([&] {
+    *launches = h->prof_launches;
+    *total_ms = h->prof_ms;
   });
 }
 

@@ -186,7 +186,8 @@ class FitEngine:
     """
 
     def __init__(self, eqs, threshold, reset, I, targets, cand_target, dt, delta_steps,
-                 onset_steps=0, input_var="I", method="Euler", precision=32, features=False):
+                 onset_steps=0, input_var="I", method="Euler", precision=32, features=False,
+                 record=0):
         self._lib = _lib.load()
         src = fit_kernel_source(eqs, threshold, reset, input_var, method, precision)
         self.state_names, self.param_names = src["state"], src["params"]
@@ -205,9 +206,11 @@ class FitEngine:
                   src["update"].encode(), src["threshold"].encode(), src["reset1"].encode(),
                   src["reset2"].encode(), src["store"].encode(), input_var.encode(),
                   self.nsteps, _lib.dptr(I), _lib.i32ptr(self.cand_target), _lib.i32ptr(lens),
-                  _lib.i32ptr(flat), int(delta_steps), int(onset_steps), 1 if features else 0,
-                  C.byref(h))
+                  _lib.i32ptr(flat), int(delta_steps), int(onset_steps),
+                  (1 if features else 0) | (2 if record else 0), C.byref(h))
         self.h = h
+        if record:
+            _lib.call("bm_fit_set_record", h, int(record))
         self.dt = dt
         self.src = src
 
@@ -254,6 +257,23 @@ class FitEngine:
         ns = np.empty(self.ncand, dtype=np.int32)
         _lib.call("bm_fit_results", self.h, _lib.i32ptr(nc), _lib.i32ptr(ns))
         return nc, ns
+
+    def spikes(self):
+        """Recorded spikes (record mode): (steps, candidates) sorted by step."""
+        n = C.c_int64()
+        _lib.call("bm_fit_get_spikes", self.h, None, None, 0, C.byref(n))
+        st = np.empty(n.value, dtype=np.int32)
+        cd = np.empty(n.value, dtype=np.int32)
+        _lib.call("bm_fit_get_spikes", self.h, _lib.i32ptr(st), _lib.i32ptr(cd), n.value, C.byref(n))
+        return st[:n.value], cd[:n.value]
+
+    def trains(self):
+        """Recorded spike trains as one sorted step array per candidate."""
+        st, cd = self.spikes()
+        order = np.argsort(cd, kind="stable")
+        bounds = np.searchsorted(cd[order], np.arange(self.ncand + 1))
+        s = st[order]
+        return [s[bounds[k]:bounds[k + 1]].astype(np.int32) for k in range(self.ncand)]
 
     def features(self):
         """Raw feature rows [ncand][66] (feature mode)."""
@@ -305,6 +325,8 @@ def particle_swarm(X0, fun, iterations, pso_params=(.9, 1.9, 1.9), min_values=No
         best = fg.argmax(axis=1)
         X_gbest = X[:, np.arange(G) * group_size + best].copy()
         fit_gbest = fg[np.arange(G), best].copy()
+        if callback is not None:
+            callback(-1, X, fit, X_gbest, fit_gbest)
     for k in range(iterations):
         R1 = rng.random((N, M))
         R2 = rng.random((N, M))
@@ -396,10 +418,18 @@ def modelfitting(model=None, reset=None, threshold=None, refractory=0 * ms, data
     hi = np.array([bounds[n][1] for n in names])
     if evaluate_initial is None:
         evaluate_initial = initial_params is not None
+    history = []
+
+    def record(k, X, fit, X_gbest, fit_gbest):
+        # (simulated candidate-seconds so far, median best gamma over targets)
+        evals = k + 1 + (1 if evaluate_initial else 0)
+        history.append((float(M) * duration * evals, float(np.median(fit_gbest))))
+
     Xg, fg = particle_swarm(X0, fun, maxiter, pso_params, lo, hi, group_size=popsize, rng=rng,
-                            evaluate_initial=evaluate_initial)
+                            evaluate_initial=evaluate_initial, callback=record)
     res = FitResults({n: Xg[k] for k, n in enumerate(names)})
     res["fitness"] = fg
+    res["history"] = history
     res["simulated_candidate_seconds"] = float(M) * duration * (maxiter + (1 if evaluate_initial else 0))
     if return_engine:
         return res, engine

@@ -149,6 +149,37 @@ def uneven_batch(nnet=4096, seed=0, lo=500, hi=20000, p=0.02, dmin=0.1 * ms, dma
         "Vt": Vt, "Vr": Vr, "we": we, "wi": wi,
     }
 
+
+# ---------------------------------------------------------------- STDP batch
+# examples/plasticity/STDP1.py: Poisson inputs at 15 Hz onto one LIF neuron
+# with an excitatory conductance, ExponentialSTDP on the input weights.
+STDP_TAUM, STDP_TAUPRE, STDP_TAUPOST = 10 * ms, 20 * ms, 20 * ms
+STDP_EE, STDP_VT, STDP_VR, STDP_EL, STDP_TAUE = 0 * mV, -54 * mV, -60 * mV, -74 * mV, 5 * ms
+STDP_GMAX = 0.01
+STDP_RATE = 15.0
+STDP_EQS = """
+dv/dt = (ge*(Ee-vr) + El - v)/taum : volt
+dge/dt = -ge/taue : 1
+"""
+
+
+def stdp_batch(ninst=1024, nin=1000, wseed=5, dt=1e-4):
+    """Constants and initial weights of a batch of STDP1 instances."""
+    from .equations import Equations
+    eqs = Equations(STDP_EQS, Ee=STDP_EE, vr=STDP_VR, El=STDP_EL, taum=STDP_TAUM, taue=STDP_TAUE)
+    A, C = eqs.linear_propagator(dt)
+    dp = float(np.exp(-dt / STDP_TAUPRE))
+    dm = float(np.exp(-dt / STDP_TAUPOST))
+    Ap = 0.01 * STDP_GMAX
+    Am = -0.01 * STDP_TAUPRE / STDP_TAUPOST * 1.05 * STDP_GMAX
+    rng = np.random.default_rng(wseed)
+    w0 = rng.random((ninst, nin)) * STDP_GMAX
+    S0 = np.zeros((ninst, 2))
+    S0[:, 0] = STDP_VR
+    return {"A": A, "C": C, "decay_pre": dp, "decay_post": dm, "Ap": Ap, "Am": Am, "w0": w0, "S0": S0,
+            "vt": STDP_VT, "vr": STDP_VR, "rate": STDP_RATE, "gmax": STDP_GMAX,
+            "ninst": ninst, "nin": nin}
+
 # ---------------------------------------------------------------- fitting
 # LIF with refractoriness and an adaptive threshold Vt0 + Vt: Vt jumps by
 # FIT_DVT at each spike and relaxes with FIT_TAU_T; V is relative to rest.

@@ -142,7 +142,7 @@ int bm_net_connect_random_blocks(bm_net *net, int src_gid, int tgt_gid, int tgt_
                                  int hom_delay_steps, double dmin, double dmax, int max_delay_steps, int *cid);
 int bm_net_connection_nnz(bm_net *net, int cid, int64_t *nnz);
 /* row_ptr [src_n+1] over the connection's source range, col [nnz] relative
- * to its target range, delay_steps [nnz] (nullable). */
+ * to its target range (nullable), delay_steps [nnz] (nullable). */
 int bm_net_get_connection(bm_net *net, int cid, int64_t *row_ptr, int32_t *col, int32_t *delay_steps);
 
 /* ExponentialSTDP on a delay_kind 0 connection (brian/stdp.py:452-539):
@@ -157,10 +157,54 @@ int bm_net_add_stdp(bm_net *net, int cid, double decay_pre, double decay_post,
  * capacity entries (brian/monitor.py:156-166). */
 int bm_net_record(bm_net *net, int gid, int64_t capacity);
 
+/* StateMonitor (brian/monitor.py:683-990, when='end'): recorded inside the
+ * sweep after the resets of each step, so windows of several steps keep
+ * running.  indices != NULL: variable `var` of neurons indices[0..nrec) at
+ * steps 0, timestep, 2 timestep, ... into a device buffer of `capacity`
+ * samples, drained by bm_net_read_state_monitor.  indices == NULL (nrec 0):
+ * record=False mode, per-neuron running sum and sum of squares in FP64. */
+int bm_net_add_state_monitor(bm_net *net, int gid, int var, int64_t nrec, const int64_t *indices,
+                             int timestep, int64_t capacity, int *mid);
+/* Samples recorded since the previous read: out [nsamples][nrec] (row q is
+ * step first_step + q*timestep), then the buffer restarts; out == NULL only
+ * counts.  Mode record=False: out = [sum (N), sum of squares (N)] over the
+ * nsamples steps run so far (not drained). */
+int bm_net_read_state_monitor(bm_net *net, int mid, double *out, int64_t max_samples, int64_t *nsamples,
+                              int64_t *first_step);
+
 /* Freeze the structure, allocate rings, choose the step window.
  * window 0 = automatic (min synaptic delay + 1, capped), else forced (>=1).
  * neurons_per_thread 0 = automatic. */
 int bm_net_prepare(bm_net *net, int window, int neurons_per_thread, int use_graph);
+
+/* Launch-shape knobs (no counterpart in the reference, whose step is a
+ * Python loop; they are what the autotuner in brian_amd/autotune.py
+ * searches).  None changes results: count rings make deliveries
+ * order-independent, and float rings are order-dependent in any case.
+ *   BM_TUNE_WINDOW             steps per launch, 1..minimum delay + 1 (after prepare)
+ *   BM_TUNE_NEURONS_PER_THREAD 1, 2, 4 (workgroup covers 256x that many neurons;
+ *                              general kernels use 1 or 4)
+ *   BM_TUNE_WAVE_MAX           spikes per workgroup-window up to which
+ *                              propagation is wavefront-wise (else block scan)
+ *   BM_TUNE_ROW_SPLIT          lanes per CSR row in wavefront-wise propagation
+ *                              (power of two, 1..64)
+ *   BM_TUNE_RING_BITS          0 automatic, 8 or 16-bit counters (before prepare)
+ *   BM_TUNE_RING_PACK          1 pack count channels into one ring word,
+ *                              0 one ring per channel (before prepare)
+ *   BM_TUNE_USE_GRAPH          replay captured launch chunks (hipGraph)
+ *   BM_TUNE_WINDOW_MAX         (get only) largest window the rings allow */
+enum {
+  BM_TUNE_WINDOW = 1,
+  BM_TUNE_NEURONS_PER_THREAD = 2,
+  BM_TUNE_WAVE_MAX = 3,
+  BM_TUNE_ROW_SPLIT = 4,
+  BM_TUNE_RING_BITS = 5,
+  BM_TUNE_RING_PACK = 6,
+  BM_TUNE_USE_GRAPH = 7,
+  BM_TUNE_WINDOW_MAX = 8
+};
+int bm_net_set_tuning(bm_net *net, int key, int value);
+int bm_net_get_tuning(bm_net *net, int key, int *value);
 
 /* Advance nsteps (brian/network.py:568-584).  Asynchronous until bm_net_sync. */
 int bm_net_run(bm_net *net, int64_t nsteps);
@@ -201,6 +245,10 @@ int bm_stdp_batch_destroy(bm_stdp_batch *h);
 /* steps_per_launch 0 = default chunk (20000 steps per launch). */
 int bm_stdp_batch_run(bm_stdp_batch *h, int64_t nsteps, int steps_per_launch);
 int bm_stdp_batch_sync(bm_stdp_batch *h);
+/* Kernel timing with HIP events on the handle's stream (each launch is
+ * waited for while on); get_profile returns launches and total ms since on. */
+int bm_stdp_batch_set_profiling(bm_stdp_batch *h, int on);
+int bm_stdp_batch_get_profile(bm_stdp_batch *h, int64_t *launches, double *total_ms);
 /* what: 0 weights [ninst][nin], 1 A_pre [ninst][nin], 2 A_post [ninst], 3 state [ninst][nvars]. */
 int bm_stdp_batch_get(bm_stdp_batch *h, int what, double *out);
 /* Post-synaptic spikes sorted by (step, instance); steps == NULL only counts. */
@@ -215,7 +263,10 @@ int bm_stdp_batch_get_spikes(bm_stdp_batch *h, int32_t *steps, int32_t *inst, in
  * while refractory, RESET2 at spikes only); the library compiles them for
  * gfx950 with hipRTC and caches the module.  Target trains are integer
  * steps: train_len[ntarget] and the concatenated sorted trains.  Exclusive
- * coincidence counting with window delta_steps; spikes counted from onset. */
+ * coincidence counting with window delta_steps; spikes counted from onset.
+ * features is a bit mask: 1 accumulates spike-train features (ISI histogram,
+ * spike-triggered input; bm_fit_features), 2 records every candidate's
+ * spikes into a raster (bm_fit_set_record, bm_fit_get_spikes). */
 typedef struct bm_fit bm_fit;
 int bm_fit_create(int precision, double dt, int ncand, int ntarget, int nvars, int nparams,
                   const char *kernel_load, const char *kernel_update, const char *kernel_threshold,
@@ -237,6 +288,10 @@ int bm_fit_get_state(bm_fit *h, double *out);
  * counts, 32 spike-triggered input sums at 1 ms lags, triggered-spike count,
  * unused) for the learned proposal; no counterpart in the reference. */
 int bm_fit_features(bm_fit *h, float *out);
+/* Record mode: raster of capacity entries, cleared by bm_fit_set_candidates;
+ * spikes come back sorted by (step, candidate); steps == NULL only counts. */
+int bm_fit_set_record(bm_fit *h, int64_t capacity);
+int bm_fit_get_spikes(bm_fit *h, int32_t *steps, int32_t *cand, int64_t cap, int64_t *n_out);
 
 /* ------------------------------------------------------- learned proposal
  * MLP nin -> nh -> nh -> 2*nu (ReLU), Gaussian NLL over nu normalised

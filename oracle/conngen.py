@@ -85,15 +85,18 @@ def block_csr(blocks, p, seed, dmin=None, dmax=None, dt=1e-4):
     counts = np.zeros(src_hi - src_lo, dtype=np.int64)
     cols, dls = [], []
     for so, sn, to, tn in blocks:
-        i = np.arange(so, so + sn, dtype=np.uint32)[:, None]
         j = np.arange(to, to + tn, dtype=np.uint32)[None, :]
-        hit = pair_u32(seed, i, j) < thr
-        counts[so - src_lo:so - src_lo + sn] = hit.sum(1)
-        r, c = np.nonzero(hit)            # row-major: columns ascending per row
-        cols.append((to + c - tgt_lo).astype(np.int32))
-        if dmin is not None:
-            u = pair_u32(seed ^ DELAY_SEED_XOR, so + r, to + c)
-            dls.append(delay_steps(u, dmin, dmax, dt).astype(np.int32))
+        step = max(1, (1 << 22) // max(tn, 1))          # bounded temporaries
+        for r0 in range(so, so + sn, step):
+            r1 = min(r0 + step, so + sn)
+            i = np.arange(r0, r1, dtype=np.uint32)[:, None]
+            hit = pair_u32(seed, i, j) < thr
+            counts[r0 - src_lo:r1 - src_lo] = hit.sum(1)
+            r, c = np.nonzero(hit)        # row-major: columns ascending per row
+            cols.append((to + c - tgt_lo).astype(np.int32))
+            if dmin is not None:
+                u = pair_u32(seed ^ DELAY_SEED_XOR, r0 + r, to + c)
+                dls.append(delay_steps(u, dmin, dmax, dt).astype(np.int32))
     row_ptr = np.zeros(len(counts) + 1, dtype=np.int64)
     np.cumsum(counts, out=row_ptr[1:])
     col = np.concatenate(cols) if cols else np.zeros(0, np.int32)

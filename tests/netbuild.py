@@ -75,21 +75,9 @@ def _blocks(blocks):
 def device_uneven(d, precision=32, window=0, record=True, use_graph=True, npt=0, capacity=None):
     """Uneven batch description (workloads.uneven_batch) -> prepared NetHandle
     with device-generated connectivity; returns (handle, group, exc cid, inh cid)."""
-    A, C = cuba_propagator()
-    h = NetHandle(1e-4, precision)
-    g = h.add_group(d["S0"])
-    h.set_linear(g, A, C)
-    h.set_threshold(g, 0, d["Vt"])
-    h.add_reset_op(g, 0, 0, d["Vr"])
-    cids = []
-    for key, var, w, seed in (("exc_blocks", 1, d["we"], d["exc_seed"]), ("inh_blocks", 2, d["wi"], d["inh_seed"])):
-        so, sn, to, tn = _blocks(d[key])
-        cids.append(h.connect_random_blocks(g, g, var, so, sn, to, tn, d["p"], w, seed, dmin=d["dmin"],
-                                            dmax=d["dmax"], max_delay_steps=d["max_delay_steps"]))
-    if record:
-        h.record(g, capacity or max(1 << 20, int(d["N"] * 40 * 0.5)))
-    h.prepare(window=window, neurons_per_thread=npt, use_graph=use_graph)
-    return h, g, cids[0], cids[1]
+    from brian_amd.batch import build_uneven
+    return build_uneven(d, precision, window=window, npt=npt, use_graph=use_graph, record=record,
+                        capacity=capacity)
 
 
 def host_uneven_csr(d):

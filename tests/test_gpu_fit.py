@@ -101,3 +101,30 @@ def test_modelfitting_recovers_hidden_neuron():
                        refractory=b["refractory"], seed=1)
     print("fitness", res["fitness"])
     assert np.all(res["fitness"] > 0.5)
+
+
+def test_fit_kernel_record_mode_trains_exact():
+    """Record mode (the spike trains of every candidate): FP64 trains equal
+    the oracle's step by step, across two run() chunks."""
+    from brian_amd.equations import Equations
+    from brian_amd.modelfitting import FitEngine
+    n, steps = 16, 8000
+    I = W.ou_current(n, steps, seed=31)
+    hidden = W.fit_hidden_params(n, seed=32)
+    refr = spike_steps(hidden["refractory"], DT)
+    ref = oracle_trains(hidden, refr, I, steps)
+    eqs = Equations(W.FIT_EQS, tau_t=W.FIT_TAU_T)
+    empty = [np.zeros(0, np.int32)] * n
+    eng = FitEngine(eqs, W.FIT_THRESHOLD, W.FIT_RESET, I, empty, np.arange(n), DT, 40, precision=64,
+                    record=1 << 16)
+    eng.set_candidates({k: v for k, v in hidden.items() if k != "refractory"},
+                       init_state={"V": 0.0, "Vt": 0.0}, refractory_steps=refr)
+    eng.run(0, 3000)
+    eng.run(3000, steps)
+    got = eng.trains()
+    assert sum(len(t) for t in ref) > 100
+    for j in range(n):
+        np.testing.assert_array_equal(got[j], ref[j])
+    _, ns = eng.results()
+    np.testing.assert_array_equal(ns, [len(t) for t in ref])
+    eng.close()

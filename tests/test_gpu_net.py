@@ -94,3 +94,107 @@ def test_c1_cuba_fp32_rate():
     rate32 = len(s) / 4000 / 1.0
     rate64 = len(rs) / 4000 / 1.0
     assert abs(rate32 - rate64) / rate64 < 0.05, (rate32, rate64)
+
+
+def test_launch_shape_knobs_keep_fp64_bit_exact():
+    """Retuning the window, neurons per thread, wavefront/block propagation
+    threshold and row-split width between runs leaves the raster bit-exact."""
+    d = W.cuba(4000, seed=13, delay_steps=10)
+    h, g = device_cuba(d, precision=64)
+    net, gid = oracle_cuba(d)
+    assert h.get_tuning("window_max") == 11
+    shapes = [{"window": 11}, {"window": 3, "neurons_per_thread": 1}, {"wave_max": 0},
+              {"wave_max": 256, "row_split": 8}, {"row_split": 16, "neurons_per_thread": 2},
+              {"window": 7, "wave_max": 16, "row_split": 32}, {"window": 1, "neurons_per_thread": 4}]
+    for k, cfg in enumerate(shapes):
+        for key, v in cfg.items():
+            h.set_tuning(key, v)
+        n = 300 + 37 * k
+        h.run(n)
+        net.run(n)
+    s, i = h.get_spikes(g)
+    rs, ri = net.raster(gid)
+    assert len(rs) > 1000
+    assert first_divergence(s, i, rs, ri) is None
+    _state_close(h, g, net, gid, 4000)
+    with pytest.raises(Exception):
+        h.set_tuning("window", 12)
+    with pytest.raises(Exception):
+        h.set_tuning("ring_bits", 16)          # layout is fixed once prepared
+
+
+@pytest.mark.parametrize("layout", [{"ring_bits": 16}, {"ring_pack": 0}])
+def test_ring_layouts_fp64_bit_exact(layout):
+    d = W.cuba(4000, seed=17, delay_steps=2)
+    from brian_amd._lib import NetHandle
+    from tests.netbuild import cuba_propagator
+    A, C = cuba_propagator()
+    h = NetHandle(1e-4, 64)
+    g = h.add_group(d["S0"])
+    h.set_linear(g, A, C)
+    h.set_threshold(g, 0, d["Vt"])
+    h.add_reset_op(g, 0, 0, d["Vr"])
+    for (ip, ix, w), off, n, var in ((d["exc"], 0, d["Ne"], 1), (d["inh"], d["Ne"], d["Ni"], 2)):
+        h.add_connection(g, off, n, g, 0, 4000, var, ip, ix, w, delay_steps=2)
+    h.record(g, 1 << 20)
+    for k, v in layout.items():
+        h.set_tuning(k, v)
+    h.prepare()
+    net, gid = oracle_cuba(d)
+    h.run(2000)
+    net.run(2000)
+    s, i = h.get_spikes(g)
+    rs, ri = net.raster(gid)
+    assert first_divergence(s, i, rs, ri) is None
+
+
+def test_state_monitor_on_device_windowed():
+    """StateMonitor recorded inside the sweep (window of 11 steps): samples
+    every 3 steps of 7 neurons equal the oracle's end-of-step state, across
+    several run() calls and buffer drains; record=False sums match too."""
+    from brian_amd._lib import NetHandle
+    from tests.netbuild import cuba_propagator
+    d = W.cuba(4000, seed=19, delay_steps=10)
+    A, C = cuba_propagator()
+    h = NetHandle(1e-4, 64)
+    g = h.add_group(d["S0"])
+    h.set_linear(g, A, C)
+    h.set_threshold(g, 0, d["Vt"])
+    h.add_reset_op(g, 0, 0, d["Vr"])
+    for (ip, ix, w), off, n, var in ((d["exc"], 0, d["Ne"], 1), (d["inh"], d["Ne"], d["Ni"], 2)):
+        h.add_connection(g, off, n, g, 0, 4000, var, ip, ix, w, delay_steps=10)
+    idx = np.array([0, 5, 17, 1999, 3200, 3999, 42])
+    m_v = h.add_state_monitor(g, 0, idx, timestep=3, capacity=400)
+    m_ge = h.add_state_monitor(g, 1, idx[:3], timestep=1, capacity=1200)
+    m_mom = h.add_state_monitor(g, 0, None)
+    h.prepare()
+    assert h.get_tuning("window") == 11
+    net, gid = oracle_cuba(d)
+    ref_v, ref_ge, sum_v, sq_v = {}, {}, np.zeros(4000), np.zeros(4000)
+    got_v, got_ge = [], []
+    step = 0
+    for chunk in (1, 500, 37, 1000):
+        h.run(chunk)
+        for _ in range(chunk):
+            net.step()
+            v = net.groups[gid].S[0]
+            if step % 3 == 0:
+                ref_v[step] = v[idx].copy()
+            ref_ge[step] = net.groups[gid].S[1][idx[:3]].copy()
+            sum_v += v
+            sq_v += v * v
+            step += 1
+        f, vals = h.read_state_samples(m_v, len(idx))
+        got_v += [(f + 3 * q, vals[q]) for q in range(len(vals))]
+        f, vals = h.read_state_samples(m_ge, 3)
+        got_ge += [(f + q, vals[q]) for q in range(len(vals))]
+    assert [s for s, _ in got_v] == sorted(ref_v)
+    for s, row in got_v:
+        np.testing.assert_allclose(row, ref_v[s], rtol=STATE_RTOL, atol=1e-15)
+    assert len(got_ge) == step
+    for s, row in got_ge:
+        np.testing.assert_allclose(row, ref_ge[s], rtol=STATE_RTOL, atol=1e-15)
+    n, sm, sq = h.read_state_moments(m_mom, 4000)
+    assert n == step
+    np.testing.assert_allclose(sm, sum_v, rtol=1e-12)
+    np.testing.assert_allclose(sq, sq_v, rtol=1e-12)
