@@ -553,6 +553,10 @@ def cpu_baseline_stdp(d, budget_s):
 
 
 # ---------------------------------------------------------------- fit
+def progress(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def bench_fit(args):
     """Model fitting: plain PSO against the learned proposal (C5 shape)."""
     from brian_amd import learned as L
@@ -578,6 +582,7 @@ def bench_fit(args):
     eng.run()
     trains = eng.trains()
     eng.close()
+    progress(f"targets simulated: {sum(len(t) for t in trains)} spikes")
     rates = np.array([len(t) for t in trains]) / dur
     data = np.concatenate([np.stack([np.full(len(t), j, dtype=np.float64), t * DT], 1)
                            for j, t in enumerate(trains) if len(t)])
@@ -599,6 +604,7 @@ def bench_fit(args):
     eng.sync()
     kernel_s = time.perf_counter() - t0
     eng.close()
+    progress(f"one evaluation of {M} candidates: {kernel_s:.2f} s")
 
     common = dict(model=eqs, reset=W.FIT_RESET, threshold=W.FIT_THRESHOLD, data=data, input=I, dt=DT,
                   popsize=pop, delta=4e-3, initial_values={"V": 0.0, "Vt": 0.0}, R=b["R"], tau=b["tau"],
@@ -606,6 +612,7 @@ def bench_fit(args):
     t0 = time.perf_counter()
     plain = modelfitting(maxiter=args.plain_iters, seed=1, **common)
     plain_s = time.perf_counter() - t0
+    progress(f"plain PSO: {plain_s:.1f} s, median gamma {plain['history'][-1][1]:.4f}")
 
     # learned proposal: simulator-generated training pairs, BF16-MFMA MLP
     bounds = {k: b[k] for k in ("R", "tau", "Vt0", "refractory")}
@@ -618,10 +625,12 @@ def bench_fit(args):
     t0 = time.perf_counter()
     Ft, Ut, _ = L.simulate_training_set(factory, I, bounds, per_input, lrng, dur, DT)
     sim_s = time.perf_counter() - t0
+    progress(f"training set: {len(Ft)} pairs in {sim_s:.1f} s")
     t0 = time.perf_counter()
     net = L.MLP(64, 256, 4, seed=7)
     last_loss = net.train(Ft, Ut, epochs=args.epochs, batch=1024, lr=1e-3, seed=2)
     train_s = time.perf_counter() - t0
+    progress(f"MLP trained: {train_s:.1f} s, loss {last_loss:.3f}")
     t0 = time.perf_counter()
     Ftarget = L.normalise(np.stack([L.raw_features(trains[j], I[j]) for j in range(ntarget)]),
                           [len(t) for t in trains], I.mean(1), I.std(1), dur)

@@ -887,7 +887,7 @@ class StateMonitor(_Tracked):
         self._track(1)
         self.P = P
         self.varname = varname
-        self.var = P._var(varname)
+        self._vidx = P._var(varname)
         self.timestep = int(timestep)
         self._record_false = record is False
         if record is True:
@@ -1087,12 +1087,12 @@ class Network:
             gid = self.gid[id(m.P._owner)]
             m._net = self
             if m._record_false:
-                m.mid = h.add_state_monitor(gid, m.var, None)
+                m.mid = h.add_state_monitor(gid, m._vidx, None)
             else:
                 nrec = max(len(m.record), 1)
                 budget = max(16, (256 << 20) // (nrec * (prec // 8)))
                 m._cap = int(min(self._planned_steps // m.timestep + 2, budget))
-                m.mid = h.add_state_monitor(gid, m.var, m.P._origin + m.record, m.timestep, m._cap)
+                m.mid = h.add_state_monitor(gid, m._vidx, m.P._origin + m.record, m.timestep, m._cap)
         window = 1 if self.ops else _PREFS["window"]
         h.prepare(window=window, neurons_per_thread=_PREFS["neurons_per_thread"],
                   use_graph=_PREFS["use_graph"])

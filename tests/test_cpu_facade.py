@@ -1,0 +1,21 @@
+"""Facade objects build on the host without a device (construction only)."""
+import numpy as np
+import pytest
+
+
+def test_build_objects_without_device():
+    from brian_amd import (NeuronGroup, Connection, StateMonitor, SpikeMonitor, PopulationRateMonitor,
+                           Network, ms, mV)
+    G = NeuronGroup(100, "dv/dt = -v/(10*ms) : volt\nge : volt", threshold=-50 * mV, reset=-60 * mV)
+    C = Connection(G, G, "ge", weight=1 * mV, sparseness=0.1)
+    m1 = StateMonitor(G, "v", record=True)
+    m2 = StateMonitor(G[10:20], "v", record=[0, 3])
+    m3 = StateMonitor(G, "ge", record=False, timestep=5)
+    S = SpikeMonitor(G)
+    R = PopulationRateMonitor(G)
+    assert m1.values.shape == (100, 0) and len(m1.times) == 0
+    assert list(m2.record) == [0, 3]
+    with pytest.raises(RuntimeError):
+        m3.mean
+    net = Network(G, C, m1, m2, m3, S, R)
+    assert net is not None
