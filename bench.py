@@ -530,7 +530,7 @@ def cpu_baseline_stdp(d, budget_s):
     from oracle import brian1 as O
     ninst, nin = 16, d["nin"]
     net = O.Net(1e-4)
-    sid = net.add_group(O.SpikeSource(ninst * nin, poisson=(np.full(ninst * nin, d["rate"]), 2024, 1e-4)),
+    sid = net.add_group(O.SpikeSource(ninst * nin, poisson=(np.full(ninst * nin, d["rate"]), 2024, 1e-4, "weyl")),
                         record=False)
     S0 = np.zeros((2, ninst))
     S0[0] = d["vr"]

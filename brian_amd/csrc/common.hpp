@@ -89,4 +89,21 @@ __host__ __device__ inline uint32_t counter_u32(uint64_t seed, uint64_t stream, 
   return (uint32_t)(mix64(key + counter * G) >> 32);
 }
 
+// Cheaper per-step stream for inputs drawn every step (batched STDP): a
+// 32-bit key per stream, then MurmurHash3's finaliser of a Weyl sequence
+// key + counter * 0x9E3779B9 (a bijection of the counter for each key).
+__host__ __device__ inline uint32_t stream_key32(uint64_t seed, uint64_t stream) {
+  const uint64_t G = 0x9E3779B97F4A7C15ull;
+  return (uint32_t)(mix64(seed * G + stream) >> 32);
+}
+__host__ __device__ inline uint32_t weyl_u32(uint32_t key, uint32_t counter) {
+  uint32_t h = key + counter * 0x9E3779B9u;
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
+}
+
 }  // namespace bm

@@ -4,7 +4,9 @@
 //
 // Reference semantics (brian/stdp.py:181-539, brian/network.py:365-380),
 // per instance and step n:
-//   groups:      inputs spike (counter RNG); neuron S <- A.S + C; spike iff
+//   groups:      inputs spike (stream_key32/weyl_u32: a 32-bit key per input,
+//                one hash per input and step, spike iff u32 < floor(rate dt 2^32));
+//                neuron S <- A.S + C; spike iff
 //                S[th_var] > vt; traces A_pre <- dp*A_pre, A_post <- dm*A_post
 //   connections: ge += w[i] for spiking inputs i in ascending order
 //                (connection.py:240-241); pre updater A_pre[i] += Ap,
@@ -54,21 +56,20 @@ __global__ __launch_bounds__(64) void k_stdp_batch(const StdpBatchDev d, int64_t
   T *wc = static_cast<T *>(d.wc) + (int64_t)inst * nin;
   T *ap = static_cast<T *>(d.apre) + (int64_t)inst * nin;
   T rw[SB_NJ], rc[SB_NJ], ra[SB_NJ];
-  uint64_t key[SB_NJ];
-  const uint64_t G = 0x9E3779B97F4A7C15ull;
+  uint32_t key[SB_NJ];
 #pragma unroll
   for (int j = 0; j < SB_NJ; ++j) {
     const int i = j * 64 + lane;
     rw[j] = (T)0;
     rc[j] = (T)0;
     ra[j] = (T)0;
-    key[j] = 0;
+    key[j] = 0u;
     if (i < nin) {
       rw[j] = w[i];
       ra[j] = ap[i];
       if (KAHAN) rc[j] = wc[i];
       // stream = global input index (same as a PoissonGroup of ninst*nin)
-      key[j] = mix64(d.seed * G + (uint64_t)((int64_t)inst * nin + i));
+      key[j] = stream_key32(d.seed, (uint64_t)((int64_t)inst * nin + i));
     }
   }
   T s[M];
@@ -84,13 +85,13 @@ __global__ __launch_bounds__(64) void k_stdp_batch(const StdpBatchDev d, int64_t
   const T wmin = (T)d.wmin, wmax = (T)d.wmax, vt = (T)d.vt, vr = (T)d.vr;
 
   for (int st = 0; st < nsteps; ++st) {
-    const uint64_t n = (uint64_t)(n0 + st);
+    const uint32_t n = (uint32_t)(n0 + st);
     // ---- groups: inputs, neuron state update + threshold, trace decay
     bool spk[SB_NJ];
 #pragma unroll
     for (int j = 0; j < SB_NJ; ++j) {
       const int i = j * 64 + lane;
-      spk[j] = (i < nin) && (uint64_t)(uint32_t)(mix64(key[j] + n * G) >> 32) < d.thr;
+      spk[j] = (i < nin) && (uint64_t)weyl_u32(key[j], n) < d.thr;
     }
     {
       T tmp[M];

@@ -67,6 +67,22 @@ def counter_uniform_u32(seed, stream, counter):
     return (out >> np.uint64(32)).astype(np.uint32)
 
 
+def weyl_uniform_u32(seed, stream, counter):
+    """32 random bits for inputs drawn every step (the batched STDP engine):
+    key = high 32 bits of mix(seed*G + stream); out = MurmurHash3 fmix32 of
+    key + counter*0x9E3779B9 (mod 2^32)."""
+    G = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        key = (_mix64(np.uint64(seed) * G + np.asarray(stream, dtype=np.uint64)) >> np.uint64(32)).astype(np.uint32)
+        h = key + np.uint32(np.uint64(counter) & np.uint64(0xFFFFFFFF)) * np.uint32(0x9E3779B9)
+        h = h ^ (h >> np.uint32(16))
+        h = h * np.uint32(0x85EBCA6B)
+        h = h ^ (h >> np.uint32(13))
+        h = h * np.uint32(0xC2B2AE35)
+        h = h ^ (h >> np.uint32(16))
+    return h.astype(np.uint32)
+
+
 def poisson_threshold_u32(rate, dt):
     """Spike iff u32 < floor(rate*dt*2^32) (clamped to 2^32-1)."""
     p = np.asarray(rate, dtype=np.float64) * dt
@@ -292,7 +308,7 @@ class SpikeSource:
     def __init__(self, N, step_lists=None, poisson=None):
         self.N = int(N)
         self.step_lists = step_lists
-        self.poisson = poisson          # (rates[N], seed, dt)
+        self.poisson = poisson          # (rates[N], seed, dt[, "weyl"])
         self.history = []
         self.S = np.zeros((0, self.N))
         self.m = 0
@@ -310,9 +326,9 @@ class SpikeSource:
 
     def update(self, t, n):
         if self.poisson is not None:
-            rates, seed, dt = self.poisson
-            u = counter_uniform_u32(seed, np.arange(self.N, dtype=np.uint64),
-                                    np.uint64(n))
+            rates, seed, dt = self.poisson[:3]
+            draw = weyl_uniform_u32 if self.poisson[3:] == ("weyl",) else counter_uniform_u32
+            u = draw(seed, np.arange(self.N, dtype=np.uint64), np.uint64(n))
             thr = poisson_threshold_u32(rates, dt)
             spikes = (u.astype(np.uint64) < thr).nonzero()[0]
         elif self.step_lists is not None and n < len(self.step_lists):

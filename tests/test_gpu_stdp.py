@@ -43,7 +43,7 @@ def make_batch(precision, ninst, nin, seed, wseed=5):
 def oracle_batch(ninst, nin, seed, w0):
     A, C, dp, dm, Ap, Am = stdp_consts()
     net = O.Net(DT)
-    sid = net.add_group(O.SpikeSource(ninst * nin, poisson=(np.full(ninst * nin, RATE), seed, DT)),
+    sid = net.add_group(O.SpikeSource(ninst * nin, poisson=(np.full(ninst * nin, RATE), seed, DT, "weyl")),
                         record=False)
     S0 = np.zeros((2, ninst))
     S0[0] = vr

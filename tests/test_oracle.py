@@ -265,6 +265,32 @@ def test_counter_rng_splitmix64_vector():
     assert int(O._mix64(np.array([G], dtype=np.uint64))[0]) == 0xE220A8397B1DCDAF
 
 
+def test_weyl_input_stream():
+    """The batched-STDP input stream: Python-integer restatement, and the
+    Bernoulli rate it yields."""
+    def fm(h):
+        h ^= h >> 16
+        h = (h * 0x85EBCA6B) & 0xFFFFFFFF
+        h ^= h >> 13
+        h = (h * 0xC2B2AE35) & 0xFFFFFFFF
+        return h ^ (h >> 16)
+
+    def mix(z):
+        M = (1 << 64) - 1
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+    G = 0x9E3779B97F4A7C15
+    for seed, stream, n in ((0, 0, 0), (2024, 5, 17), (7, 123456, 999999), (1 << 40, 3, 1 << 31)):
+        key = mix((seed * G + stream) & ((1 << 64) - 1)) >> 32
+        want = fm((key + (n & 0xFFFFFFFF) * 0x9E3779B9) & 0xFFFFFFFF)
+        assert int(O.weyl_uniform_u32(seed, np.uint64(stream), n)) == want
+    thr = O.poisson_threshold_u32(15.0, DT)
+    u = np.stack([O.weyl_uniform_u32(3, np.arange(2000, dtype=np.uint64), k) for k in range(500)])
+    p = float(np.mean(u.astype(np.uint64) < thr))
+    assert abs(p - 15.0 * DT) < 4 * np.sqrt(15.0 * DT / u.size)
+
+
 def cobahh_oracle(steps):
     """The reference's COBAHH golden model (dev/tests/correctness/test_optimizations.py:48-128)."""
     z = np.load(os.path.join(HERE, "golden", "cobahh.npz"))

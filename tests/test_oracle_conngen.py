@@ -19,6 +19,13 @@ def test_numpy_hash_matches_python_integers():
         np.testing.assert_array_equal(got, np.array(want, dtype=np.uint32))
 
 
+def test_fmix32_murmur3_known_answers():
+    """MurmurHash3_x86_32 of the empty string is fmix32(seed): the published
+    test vectors for seeds 1 and 0xFFFFFFFF are 0x514E28B7 and 0x81F16F39."""
+    assert int(G.fmix32(np.uint32(1))) == 0x514E28B7
+    assert int(G.fmix32(np.uint32(0xFFFFFFFF))) == 0x81F16F39
+
+
 def test_fmix32_is_a_bijection_on_a_sample():
     assert int(G.fmix32(np.uint32(0))) == 0
     x = np.arange(1 << 20, dtype=np.uint32) * np.uint32(4099)
