@@ -506,7 +506,15 @@ class Connection(_Tracked):
 
     def __new__(cls, source=None, target=None, state=0, delay=0 * ms, *a, **kw):
         if cls is Connection and not isinstance(delay, float):
-            return super().__new__(DelayConnection)
+            # connection.py:176-181: a DelayConnection, given the arguments
+            # that follow `delay` by name (its own positional order differs)
+            obj = super().__new__(DelayConnection)
+            names = ("modulation", "structure", "weight", "sparseness", "max_delay")
+            kw2 = dict(kw)
+            kw2.update(zip(names, a))
+            DelayConnection.__init__(obj, source, target, state, delay=delay, **kw2)
+            obj._initialised = True
+            return obj
         return super().__new__(cls)
 
     def __init__(self, source, target, state=0, delay=0 * ms, modulation=None,
@@ -597,8 +605,14 @@ class Connection(_Tracked):
             raise ValueError("one-to-one needs equal sizes")
         self.connect(P, Q, sp.identity(len(P)) * float(weight))
 
-    def connect_from_sparse(self, W, column_access=True):
+    def connect_from_sparse(self, W, delay=None, column_access=True):
+        """connection.py:420-443: weights (and, for a DelayConnection, the
+        delay matrix of the same structure) from scipy sparse matrices."""
         self._W = sp.lil_matrix(W)
+        if delay is not None:
+            if not isinstance(self, DelayConnection):
+                raise TypeError("delays need a DelayConnection")
+            self.delayvec = sp.lil_matrix(delay)
 
     # ---- matrix access
     @property
@@ -623,87 +637,107 @@ class Connection(_Tracked):
         return self._compiled_W
 
 
-class _DelayMatrix:
-    """Dense-indexable store of per-synapse delays (seconds)."""
-
-    def __init__(self, conn, init):
-        self.conn = conn
-        self.M = sp.lil_matrix((len(conn.source), len(conn.target)))
-        self.default = float(init) if isinstance(init, (int, float)) else 0.0
-
-    def __setitem__(self, ij, v):
-        self.M[ij] = v
-
-    def __getitem__(self, ij):
-        return self.M[ij]
-
-    def values_for(self, rows, cols):
-        """Delay of each listed synapse; entries never set take the default."""
-        D = self.M.tocsr()
-        D.sort_indices()
-        out = np.full(len(rows), self.default)
-        for q, (i, j) in enumerate(zip(rows, cols)):
-            lo, hi = D.indptr[i], D.indptr[i + 1]
-            k = np.searchsorted(D.indices[lo:hi], j)
-            if k < hi - lo and D.indices[lo + k] == j:
-                out[q] = D.data[lo + k]
-        return out
+def _argcount(f):
+    try:
+        return len(inspect.signature(f).parameters)
+    except (TypeError, ValueError):
+        return f.__code__.co_argcount
 
 
 class DelayConnection(Connection):
     """brian/connections/delayconnection.py:17-337: per-synapse delays,
-    delivered through a ring of int(max_delay/dt)+1 slots."""
+    delivered through a ring of int(max_delay/dt)+1 slots.  ``delayvec``
+    (also ``delay``) holds the delays in seconds; as in the reference, delays
+    are set for the synapses that exist when they are given, and only the
+    entries at the weight matrix's nonzeros count when the network runs."""
 
     def __init__(self, source, target, state=0, modulation=None, structure="sparse",
                  weight=None, sparseness=None, delay=None, max_delay=5 * ms, **kwds):
+        if getattr(self, "_initialised", False):    # built by Connection.__new__
+            return
         self._track(1)
         self._max_delay = float(max_delay)
         self._delay_kind = 1
         self._delays = None
-        self._delay_spec = delay
+        self.delayvec = sp.lil_matrix((len(source), len(target)))
         self._init_common(source, target, state, modulation, weight, sparseness)
+        if delay is True:       # connection.py:176-178: heterogeneous, none set yet
+            delay = None
+        if delay is not None:
+            self.set_delays(delay=delay)
 
     def set_delays(self, source=None, target=None, delay=None):
-        self._delay_spec = delay
+        """delayconnection.py:257-313: a number, a (min, max) pair drawn
+        uniformly per synapse, a function of no arguments called per synapse,
+        a function (i, j) called per row with the row's target indices
+        (relative to the subgroups), or a matrix."""
+        if delay is None:
+            return
+        P = source if source is not None else self.source
+        Q = target if target is not None else self.target
+        i0, j0 = self._origin(P, Q)
+        i1, j1 = i0 + len(P), j0 + len(Q)
+        W = self._W.tocsr()
+        W.sort_indices()
+        if isinstance(delay, (float, int)) or (isinstance(delay, (tuple, list)) and len(delay) == 2) \
+                or callable(delay):
+            n_args = _argcount(delay) if callable(delay) else None
+            for i in range(i0, i1):
+                cols = W.indices[W.indptr[i]:W.indptr[i + 1]]
+                inds = cols[(cols >= j0) & (cols < j1)]
+                L = len(inds)
+                if isinstance(delay, (float, int)):
+                    vals = np.full(L, float(delay))
+                elif isinstance(delay, (tuple, list)):
+                    lo, hi = float(delay[0]), float(delay[1])
+                    vals = np.random.rand(L) * (hi - lo) + lo
+                elif n_args == 0:
+                    vals = np.array([float(delay()) for _ in range(L)])
+                elif n_args == 2:
+                    vals = np.broadcast_to(np.asarray(delay(i - i0, inds - j0), dtype=np.float64), (L,))
+                else:
+                    raise TypeError("delays must be a number, a pair or a function of 0 or 2 arguments")
+                if L:
+                    self.delayvec[i, inds] = vals
+        else:
+            self.delayvec[i0:i1, j0:j1] = delay
+
+    def connect(self, source=None, target=None, W=None, delay=None):
+        Connection.connect(self, source, target, W)
+        self.set_delays(source, target, delay)
+
+    def connect_random(self, source=None, target=None, p=1.0, weight=1.0, fixed=False, seed=None,
+                       sparseness=None, delay=None):
+        Connection.connect_random(self, source, target, p, weight, fixed, seed, sparseness)
+        self.set_delays(source, target, delay)
+
+    def connect_full(self, source=None, target=None, weight=1.0, delay=None):
+        Connection.connect_full(self, source, target, weight)
+        self.set_delays(source, target, delay)
+
+    def connect_one_to_one(self, source=None, target=None, weight=1.0, delay=None):
+        Connection.connect_one_to_one(self, source, target, weight)
+        self.set_delays(source, target, delay)
 
     @property
     def delay(self):
-        """Per-synapse delays in seconds, settable like ``C.delay[i, :] = d``
-        (the delayvec matrix of delayconnection.py:101-106)."""
-        if not isinstance(self._delay_spec, _DelayMatrix):
-            self._delay_spec = _DelayMatrix(self, self._delay_spec)
-        return self._delay_spec
+        """The delay matrix (delayconnection.py:208-211)."""
+        return self.delayvec
 
     @delay.setter
     def delay(self, value):
-        if value is None or getattr(self, "_delay_kind", 0) == 0:
+        if "delayvec" not in self.__dict__:       # Connection.__init__ not run on this path
             object.__setattr__(self, "_delay_value", value)
             return
-        self._delay_spec = value
+        self.delayvec[:, :] = value
 
     def _delay_steps(self, W):
-        """int(invdt * d) per synapse (delayconnection.py:123, :152)."""
+        """int(invdt * d) per synapse of the compressed matrix W, d read from
+        delayvec at W's nonzeros (delayconnection.py:123, :152, :213-255)."""
         invdt = 1 / self.target.clock._dt
-        spec = self._delay_spec
-        n = W.nnz
         rows = np.repeat(np.arange(W.shape[0]), np.diff(W.indptr))
-        if isinstance(spec, _DelayMatrix):
-            d = spec.values_for(rows, W.indices) if n else np.zeros(0)
-        elif spec is None:
-            d = np.zeros(n)
-        elif isinstance(spec, (int, float)):
-            d = np.full(n, float(spec))
-        elif isinstance(spec, tuple) and len(spec) == 2:
-            lo, hi = float(spec[0]), float(spec[1])
-            d = lo + np.random.rand(n) * (hi - lo)
-        elif callable(spec):
-            d = np.array([float(spec(i, j)) for i, j in zip(rows, W.indices)], dtype=np.float64)
-        else:
-            D = sp.csr_matrix(spec) if sp.issparse(spec) else None
-            if D is not None:
-                d = np.asarray(D[rows, W.indices]).ravel()
-            else:
-                d = np.asarray(spec, dtype=np.float64)[rows, W.indices]
+        D = self.delayvec.tocsr()
+        d = np.asarray(D[rows, W.indices]).ravel().astype(np.float64) if W.nnz else np.zeros(0)
         self._delays = d
         return np.array(invdt * d, dtype=np.int64)
 
