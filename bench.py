@@ -6,8 +6,10 @@ benchmark 2 (examples/misc/CUBA.py of the reference) scaled to N=2^20 LIF
 neurons, 80/20 excitatory/inhibitory, fixed in-degree 80 with presynaptic ids
 uniform from a seed, we=1.62 mV onto ge, wi=-9 mV onto gi, homogeneous 1 ms
 delay (10 steps), dt=0.1 ms.  A *step* is one simulation time step of the
-whole network.  Inputs (connectivity, initial state) are generated on the
-host once and are resident in HBM before the timed region.
+whole network.  The connectivity (80 distinct presynaptic neurons per
+target) is sampled on the device, ``--host-connectivity`` builds it with
+numpy instead; the initial state comes from the host.  Everything is resident
+in HBM before the timed region.
 
 Reported: neuron-updates/s of the FP32 windowed path (``value``), synaptic
 events/s, the plain FP64 GPU path (one step per launch, check mode) for the
@@ -232,15 +234,39 @@ def cpu_baseline(d, budget_s=15.0):
             "steps": steps, "seconds": el, "host_cores": host_cores()}
 
 
+def device_cuba_csr(h, ce, ci):
+    """The device-built connectivity read back into the host description
+    the CPU oracle takes."""
+    from brian_amd import workloads as W
+    Ne = int(0.8 * N)
+    erp, ecol, _ = h.get_connection(ce, Ne)
+    irp, icol, _ = h.get_connection(ci, N - Ne)
+    return {"N": N, "Ne": Ne, "Ni": N - Ne, "S0": W.cuba_initial_state(N, 2), "Vt": W.Vt, "Vr": W.Vr,
+            "exc": (erp, ecol, np.full(len(ecol), W.we)), "inh": (irp, icol, np.full(len(icol), W.wi))}
+
+
 def bench_cuba1m(args):
-    d = build_workload()
     K = args.steps or 10000
     Wm = 200 if args.warmup is None else args.warmup
+    if args.host_connectivity:
+        d = build_workload()
+
+        def build(prec, window, tuning=None):
+            return make_net(d, prec, window, npt=args.npt, tuning=tuning)[:2] + (None, None)
+    else:
+        from brian_amd.batch import build_cuba_fixed_indegree
+        d = None
+
+        def build(prec, window, tuning=None):
+            return build_cuba_fixed_indegree(N, INDEG, 2, DELAY, prec, window, args.npt, tuning=tuning)
 
     # ---- FP32 windowed path (the product fast path), launch shape autotuned
-    tuning, tune_report = autotuned(lambda lay: make_net(d, 32, args.window, npt=args.npt, tuning=lay)[0],
-                                    args.no_autotune, args.tuning)
-    h32, g32 = make_net(d, 32, args.window, npt=args.npt, tuning=tuning)
+    tuning, tune_report = autotuned(lambda lay: build(32, args.window, lay)[0], args.no_autotune, args.tuning)
+    t0 = time.perf_counter()
+    h32, g32, ce, ci = build(32, args.window, tuning)
+    build_s = time.perf_counter() - t0
+    if d is None and not args.no_cpu:
+        d = device_cuba_csr(h32, ce, ci)
     el32 = timed_run(h32, K, Wm)
     r32 = h32.get_spikes(g32)
     n_sp32 = int((r32[0] >= Wm).sum())
@@ -258,7 +284,7 @@ def bench_cuba1m(args):
     # ---- plain FP64 GPU path: check mode, one step per launch
     fp64 = None
     if not args.no_fp64:
-        h64, g64 = make_net(d, 64, 1, npt=args.npt)
+        h64, g64 = build(64, 1)[:2]
         el64 = timed_run(h64, K, Wm)
         r64 = h64.get_spikes(g64)
         h64.close()
@@ -297,7 +323,10 @@ def bench_cuba1m(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "CUBA LIF network, 1,048,576 neurons (80/20 E/I), fixed in-degree 80, "
                                "1 ms homogeneous delay, dt 0.1 ms, FP32 windowed path",
-                   "neurons": N, "synapses": int(len(d["exc"][1]) + len(d["inh"][1])),
+                   "neurons": N, "synapses": N * INDEG,
+                   "connectivity": "host (numpy PCG64)" if args.host_connectivity else
+                                   "device (Floyd sampling of 80 distinct sources per target)",
+                   "network_build_s": build_s,
                    "dt_ms": 0.1, "simulated_s": K * 1e-4, "window_steps": window},
         "synaptic_events_per_s": n_sp32 * INDEG / el32,
         "spikes": n_sp32,
@@ -733,6 +762,8 @@ def main():
     ap.add_argument("--npt", type=int, default=0)
     ap.add_argument("--window", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--host-connectivity", action="store_true",
+                    help="cuba1m: build the fixed in-degree connectivity with numpy on the host")
     ap.add_argument("--tuning", type=json.loads, default=None,
                     help='fixed launch shape as JSON, e.g. \'{"window": 11, "neurons_per_thread": 4}\'')
     ap.add_argument("--probe-copy", type=int, default=0,

@@ -60,6 +60,8 @@ _SIGS = {
                                      dbl, dbl, u64, C.c_int, C.c_int, dbl, dbl, C.c_int,
                                      C.POINTER(C.c_int)],
     "bm_net_connection_nnz": [P, C.c_int, Pi64],
+    "bm_net_connect_fixed_indegree": [P, C.c_int, i64, i64, i64, i64, C.c_int, i64, i64, C.c_int, C.c_int,
+                                      dbl, u64, C.c_int, C.POINTER(C.c_int)],
     "bm_net_get_connection": [P, C.c_int, Pi64, Pi32, Pi32],
     "bm_net_record": [P, C.c_int, i64],
     "bm_net_prepare": [P, C.c_int, C.c_int, C.c_int],
@@ -283,6 +285,17 @@ class NetHandle:
         call("bm_net_connect_random_blocks", self.h, src, tgt, var, len(a[0]), *[i64ptr(x) for x in a],
              float(p), float(weight), C.c_uint64(seed), kind, int(delay_steps),
              float(dmin or 0.0), float(dmax or 0.0), int(max_delay_steps), C.byref(cid))
+        return cid.value
+
+    def connect_fixed_indegree(self, src, src_off, src_n, tgt, tgt_off, tgt_n, var, k, weight, seed,
+                               delay_steps=0, keep=None):
+        """Device-built fixed in-degree connection (see brianmi.h); ``keep`` =
+        (offset, count) of the sampled source rows this connection keeps."""
+        keep_off, keep_n = keep if keep is not None else (0, src_n)
+        cid = C.c_int()
+        call("bm_net_connect_fixed_indegree", self.h, src, int(src_off), int(src_n), int(keep_off),
+             int(keep_n), tgt, int(tgt_off), int(tgt_n), var, int(k), float(weight), C.c_uint64(seed),
+             int(delay_steps), C.byref(cid))
         return cid.value
 
     def get_connection(self, cid, src_n, rows_only=False):

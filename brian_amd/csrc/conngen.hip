@@ -161,4 +161,108 @@ int64_t generate_block_csr(const GenSpec &s, DevBuf &row_ptr, DevBuf &col, DevBu
   return nnz;
 }
 
+// ------------------------------------------------------------ fixed in-degree
+// random_matrix_fixed_column (construction.py:149-177): every target column
+// j gets k distinct sources drawn uniformly from [0, n).  Floyd's algorithm
+// per column: for t = 0..k-1 with m = n-k+t, draw r uniform in [0, m]
+// (r = (pair_u32(seed, j, t) * (m+1)) >> 32); take r, or m if r was already
+// taken.  One thread per column; the column's set lives in pre[j*k ..].
+__global__ __launch_bounds__(256) void k_fixed_sample(int64_t ncols, int k, uint32_t n, uint64_t seed,
+                                                      int32_t *pre, int64_t *counts) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= ncols) return;
+  int32_t *S = pre + j * k;
+  for (int t = 0; t < k; ++t) {
+    const uint32_t m = n - (uint32_t)k + (uint32_t)t;
+    const uint32_t u = pair_u32(seed, (uint32_t)j, (uint32_t)t);
+    uint32_t r = (uint32_t)(((uint64_t)u * ((uint64_t)m + 1ull)) >> 32);
+    bool seen = false;
+    for (int q = 0; q < t; ++q) seen = seen || (S[q] == (int32_t)r);
+    if (seen) r = m;
+    S[t] = (int32_t)r;
+    atomicAdd(reinterpret_cast<unsigned long long *>(counts) + r, 1ull);
+  }
+}
+
+// Scatter (source, column) pairs into the source rows; the order inside a
+// row is fixed afterwards by a segmented sort.
+__global__ __launch_bounds__(256) void k_fixed_scatter(int64_t ncols, int k, const int32_t *pre,
+                                                       int64_t *cursor, int32_t *col) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ncols * (int64_t)k) return;
+  const int64_t j = e / k;
+  const int32_t src = pre[e];
+  const unsigned long long pos = atomicAdd(reinterpret_cast<unsigned long long *>(cursor) + src, 1ull);
+  col[pos] = (int32_t)j;
+}
+
+int64_t generate_fixed_indegree_all(const GenSpec &s, DevBuf &row_ptr, DevBuf &col, hipStream_t st);
+
+__global__ __launch_bounds__(256) void k_rebase_rows(const int64_t *in, int64_t n, int64_t *out) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r <= n) out[r] = in[r] - in[0];
+}
+
+// Source-row CSR (columns ascending within each row) of a fixed in-degree
+// connection: nrows sampled sources, ncols targets, k sources per target;
+// rows [keep_off, keep_off + keep_n) of the sampled range are returned (two
+// connections with the same seed split one sampling by source range).
+// Returns nnz of the kept rows.
+int64_t generate_fixed_indegree_csr(const GenSpec &s, DevBuf &row_ptr, DevBuf &col, hipStream_t st) {
+  DevBuf full_rp, full_col;
+  const int64_t nnz_all = generate_fixed_indegree_all(s, full_rp, full_col, st);
+  (void)nnz_all;
+  int64_t lo = 0, hi = 0;
+  BM_HIP(hipMemcpyAsync(&lo, full_rp.as<int64_t>() + s.keep_off, 8, hipMemcpyDeviceToHost, st));
+  BM_HIP(hipMemcpyAsync(&hi, full_rp.as<int64_t>() + s.keep_off + s.keep_n, 8, hipMemcpyDeviceToHost, st));
+  BM_HIP(hipStreamSynchronize(st));
+  row_ptr.alloc((size_t)(s.keep_n + 1) * 8);
+  hipLaunchKernelGGL(k_rebase_rows, dim3((unsigned)((s.keep_n + 1 + 255) / 256)), dim3(256), 0, st,
+                     full_rp.as<int64_t>() + s.keep_off, s.keep_n, row_ptr.as<int64_t>());
+  BM_HIP(hipGetLastError());
+  col.alloc((size_t)std::max<int64_t>(hi - lo, 1) * 4);
+  if (hi > lo)
+    BM_HIP(hipMemcpyAsync(col.p, full_col.as<int32_t>() + lo, (size_t)(hi - lo) * 4, hipMemcpyDeviceToDevice, st));
+  BM_HIP(hipStreamSynchronize(st));
+  return hi - lo;
+}
+
+int64_t generate_fixed_indegree_all(const GenSpec &s, DevBuf &row_ptr, DevBuf &col, hipStream_t st) {
+  const int64_t nnz = s.ncols * (int64_t)s.k;
+  BM_REQUIRE(nnz < (int64_t)INT32_MAX, "fixed in-degree connection too large for one sort");
+  DevBuf pre, counts, cursor, tmp_col, temp;
+  pre.alloc((size_t)std::max<int64_t>(nnz, 1) * 4);
+  counts.alloc((size_t)(s.nrows + 1) * 8);
+  BM_HIP(hipMemsetAsync(counts.p, 0, (size_t)(s.nrows + 1) * 8, st));
+  if (s.ncols > 0)
+    hipLaunchKernelGGL(k_fixed_sample, dim3((unsigned)((s.ncols + 255) / 256)), dim3(256), 0, st, s.ncols, s.k,
+                       (uint32_t)s.nrows, s.seed, pre.as<int32_t>(), counts.as<int64_t>());
+  BM_HIP(hipGetLastError());
+  row_ptr.alloc((size_t)(s.nrows + 1) * 8);
+  size_t tb = 0;
+  BM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, counts.as<int64_t>(), row_ptr.as<int64_t>(),
+                                          (int)(s.nrows + 1), st));
+  temp.alloc(tb);
+  BM_HIP(hipcub::DeviceScan::ExclusiveSum(temp.p, tb, counts.as<int64_t>(), row_ptr.as<int64_t>(),
+                                          (int)(s.nrows + 1), st));
+  cursor.alloc((size_t)(s.nrows + 1) * 8);
+  BM_HIP(hipMemcpyAsync(cursor.p, row_ptr.p, (size_t)(s.nrows + 1) * 8, hipMemcpyDeviceToDevice, st));
+  tmp_col.alloc((size_t)std::max<int64_t>(nnz, 1) * 4);
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_fixed_scatter, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, s.ncols, s.k,
+                       pre.as<int32_t>(), cursor.as<int64_t>(), tmp_col.as<int32_t>());
+  BM_HIP(hipGetLastError());
+  col.alloc((size_t)std::max<int64_t>(nnz, 1) * 4);
+  const int64_t *rp = row_ptr.as<int64_t>();
+  size_t tb2 = 0;
+  BM_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb2, tmp_col.as<int32_t>(), col.as<int32_t>(),
+                                                    (int)nnz, (int)s.nrows, rp, rp + 1, 0, 32, st));
+  DevBuf temp2;
+  temp2.alloc(tb2);
+  BM_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(temp2.p, tb2, tmp_col.as<int32_t>(), col.as<int32_t>(),
+                                                    (int)nnz, (int)s.nrows, rp, rp + 1, 0, 32, st));
+  BM_HIP(hipStreamSynchronize(st));
+  return nnz;
+}
+
 }  // namespace bm

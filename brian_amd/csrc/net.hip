@@ -282,6 +282,14 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
   // ---- per-connection storage and count/float mode
   for (size_t ci = 0; ci < net->conns.size(); ++ci) {
     HostConn &c = *net->conns[ci];
+    if (c.generated && c.gen.kind == 1) {
+      // fixed in-degree k: counts per slot and channel are at most k
+      c.max_indeg = c.gen.k;
+      c.count_mode = c.stdp < 0;
+      BM_REQUIRE(c.count_mode, "STDP on device-generated connections is not supported");
+      c.nnz = generate_fixed_indegree_csr(c.gen, c.row_ptr_d, c.col_d, net->stream);
+      continue;
+    }
     if (c.generated) {
       // counts per slot and channel are bounded by the largest source block
       int64_t mx = 0;
@@ -1033,6 +1041,46 @@ int bm_net_connect_random_blocks(bm_net *net, int src_gid, int tgt_gid, int tgt_
       c->max_delay = max_delay_steps;
       c->min_delay = smin;
     }
+    net->conns.push_back(std::move(c));
+    *cid = (int)net->conns.size() - 1;
+  });
+}
+
+int bm_net_connect_fixed_indegree(bm_net *net, int src_gid, int64_t src_off, int64_t src_n, int64_t keep_off,
+                                  int64_t keep_n, int tgt_gid, int64_t tgt_off, int64_t tgt_n, int tgt_var, int k,
+                                  double weight, uint64_t seed, int delay_steps, int *cid) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &sg = grp(net, src_gid);
+    HostGroup &tg = grp(net, tgt_gid);
+    BM_REQUIRE(tg.kind == GK_NEURON && tgt_var >= 0 && tgt_var < tg.M, "target variable out of range");
+    BM_REQUIRE(src_off >= 0 && src_n > 0 && src_off + src_n <= sg.N, "source range out of bounds");
+    BM_REQUIRE(keep_off >= 0 && keep_n >= 0 && keep_off + keep_n <= src_n, "kept rows outside the sampled range");
+    BM_REQUIRE(tgt_off >= 0 && tgt_n >= 0 && tgt_off + tgt_n <= tg.N, "target range out of bounds");
+    BM_REQUIRE(src_n < (int64_t(1) << 31) && tgt_n < (int64_t(1) << 31), "ranges must fit 31 bits");
+    BM_REQUIRE(k >= 0 && k <= src_n && k <= 65535, "k must be in [0, min(src_n, 65535)]");
+    BM_REQUIRE(delay_steps >= 0 && delay_steps < 65536, "delay out of range");
+    auto c = std::make_unique<HostConn>();
+    c->generated = true;
+    c->gen.kind = 1;
+    c->gen.k = k;
+    c->gen.nrows = src_n;
+    c->gen.ncols = tgt_n;
+    c->gen.keep_off = keep_off;
+    c->gen.keep_n = keep_n;
+    c->gen.seed = seed;
+    c->src = src_gid;
+    c->tgt = tgt_gid;
+    c->var = tgt_var;
+    c->src_off = src_off + keep_off;
+    c->src_n = keep_n;
+    c->tgt_off = tgt_off;
+    c->tgt_n = tgt_n;
+    c->w.assign(1, weight);
+    c->wval = weight;
+    c->delay_kind = 0;
+    c->hom_delay = delay_steps;
+    c->max_delay = c->min_delay = delay_steps;
     net->conns.push_back(std::move(c));
     *cid = (int)net->conns.size() - 1;
   });

@@ -28,12 +28,17 @@ void launch_probe_copy(const void *src, void *dst, int64_t bytes, hipStream_t st
 
 // block-diagonal random connectivity built on the device (conngen.hip)
 struct GenSpec {
+  int kind = 0;                       // 0 block-diagonal Bernoulli, 1 fixed in-degree
   std::vector<int64_t> src_off, src_n, tgt_off, tgt_n;   // blocks sorted by src_off
   int64_t src_lo = 0, tgt_lo = 0, nrows = 0;
+  int64_t ncols = 0;                  // fixed in-degree: target columns
+  int k = 0;                          // fixed in-degree: sources per target
+  int64_t keep_off = 0, keep_n = 0;   // fixed in-degree: sampled rows kept
   double p = 0, dmin = 0, dmax = 0, invdt = 0;
   uint64_t seed = 0;
   int with_delay = 0, dly_bytes = 1;
 };
 int64_t generate_block_csr(const GenSpec &s, DevBuf &row_ptr, DevBuf &col, DevBuf &dly, hipStream_t st);
+int64_t generate_fixed_indegree_csr(const GenSpec &s, DevBuf &row_ptr, DevBuf &col, hipStream_t st);
 
 }  // namespace bm

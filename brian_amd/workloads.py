@@ -76,6 +76,15 @@ def fixed_indegree_sources(rng, N, k):
         pre[dup] = fresh
 
 
+def cuba_initial_state(N, seed):
+    """[v, ge, gi] with v uniform in [Vr, Vt) from PCG64(seed) (the first draw
+    of the generator, as in cuba())."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    S0 = np.zeros((3, N))
+    S0[0] = Vr + rng.random(N) * (Vt - Vr)
+    return S0
+
+
 def cuba(N=4000, seed=1, connectivity="bernoulli", p=0.02, indegree=80, delay_steps=0):
     """CUBA network description: dict with v0 and two CSR blocks.
 

@@ -140,6 +140,19 @@ int bm_net_connect_random_blocks(bm_net *net, int src_gid, int tgt_gid, int tgt_
                                  const int64_t *src_off, const int64_t *src_n, const int64_t *tgt_off,
                                  const int64_t *tgt_n, double p, double weight, uint64_t seed, int delay_kind,
                                  int hom_delay_steps, double dmin, double dmax, int max_delay_steps, int *cid);
+/* Fixed in-degree connectivity built on the device (random_matrix_fixed_column,
+ * brian/connections/construction.py:149-177, i.e. connect_random(fixed=True)):
+ * every target j of [tgt_off, +tgt_n) receives k distinct sources of
+ * [src_off, +src_n), sampled by Floyd's algorithm from the pair hash
+ * (r = (pair_u32(seed, j, t) * (m + 1)) >> 32 for m = src_n - k + t, taking m
+ * when r is already chosen); all weights `weight`, homogeneous delay.  The
+ * connection keeps the sampled rows [keep_off, keep_off + keep_n) (relative
+ * to src_off): two calls with the same seed and sampled range split one
+ * sampling by source, e.g. excitatory and inhibitory rows onto ge and gi.
+ * The source-row CSR (columns ascending) is built at bm_net_prepare. */
+int bm_net_connect_fixed_indegree(bm_net *net, int src_gid, int64_t src_off, int64_t src_n, int64_t keep_off,
+                                  int64_t keep_n, int tgt_gid, int64_t tgt_off, int64_t tgt_n, int tgt_var, int k,
+                                  double weight, uint64_t seed, int delay_steps, int *cid);
 int bm_net_connection_nnz(bm_net *net, int cid, int64_t *nnz);
 /* row_ptr [src_n+1] over the connection's source range, col [nnz] relative
  * to its target range (nullable), delay_steps [nnz] (nullable). */

@@ -102,3 +102,29 @@ def block_csr(blocks, p, seed, dmin=None, dmax=None, dt=1e-4):
     col = np.concatenate(cols) if cols else np.zeros(0, np.int32)
     dl = (np.concatenate(dls) if dls else np.zeros(0, np.int32)) if dmin is not None else None
     return row_ptr, col, dl
+
+
+def fixed_indegree_pre(nsrc, ntgt, k, seed):
+    """Sources of every target column (Floyd's algorithm, as the device):
+    [ntgt][k] int64, in draw order."""
+    j = np.arange(ntgt, dtype=np.uint32)
+    pre = np.zeros((ntgt, k), dtype=np.int64)
+    for t in range(k):
+        m = nsrc - k + t
+        u = pair_u32(seed, j, np.uint32(t)).astype(np.uint64)
+        r = ((u * np.uint64(m + 1)) >> np.uint64(32)).astype(np.int64)
+        seen = (pre[:, :t] == r[:, None]).any(axis=1) if t else np.zeros(ntgt, bool)
+        pre[:, t] = np.where(seen, m, r)
+    return pre
+
+
+def fixed_indegree_csr(nsrc, ntgt, k, seed):
+    """Source-row CSR (columns ascending) of the fixed in-degree connection."""
+    pre = fixed_indegree_pre(nsrc, ntgt, k, seed)
+    src = pre.ravel()
+    tgt = np.repeat(np.arange(ntgt, dtype=np.int64), k)
+    order = np.lexsort((tgt, src))
+    counts = np.bincount(src, minlength=nsrc)
+    row_ptr = np.zeros(nsrc + 1, dtype=np.int64)
+    np.cumsum(counts, out=row_ptr[1:])
+    return row_ptr, tgt[order].astype(np.int32)

@@ -116,3 +116,44 @@ def test_gap_rows_are_empty():
     rp, col, _ = G.block_csr([(0, 10, 0, 10), (30, 10, 30, 10)], 0.5, seed=1)
     assert np.all(np.diff(rp)[10:30] == 0)
     assert len(rp) == 41
+
+
+def test_fixed_indegree_distinct_and_uniform():
+    """random_matrix_fixed_column semantics: k distinct sources per target,
+    every source equally likely; every k-subset equally likely (brute force
+    over a tiny source range)."""
+    nsrc, ntgt, k = 300, 4000, 20
+    pre = G.fixed_indegree_pre(nsrc, ntgt, k, seed=77)
+    s = np.sort(pre, axis=1)
+    assert np.all(np.diff(s, axis=1) > 0)
+    assert pre.min() >= 0 and pre.max() < nsrc
+    counts = np.bincount(pre.ravel(), minlength=nsrc)
+    assert stats.chisquare(counts).pvalue > 1e-3
+    # subsets of size 2 out of 5 sources: 10 subsets, uniform
+    pre2 = np.sort(G.fixed_indegree_pre(5, 20000, 2, seed=5), axis=1)
+    code = pre2[:, 0] * 5 + pre2[:, 1]
+    freq = np.bincount(code, minlength=25)
+    nonzero = freq[freq > 0]
+    assert len(nonzero) == 10
+    assert stats.chisquare(nonzero).pvalue > 1e-3
+
+
+def test_fixed_indegree_csr_structure():
+    rp, col = G.fixed_indegree_csr(100, 250, 7, seed=3)
+    assert rp[-1] == 250 * 7
+    for i in range(100):
+        c = col[rp[i]:rp[i + 1]]
+        assert np.all(np.diff(c) > 0)
+    # every target appears exactly k times
+    np.testing.assert_array_equal(np.bincount(col, minlength=250), 7)
+
+
+def test_fixed_indegree_floyd_scalar():
+    """Floyd's steps for one column recomputed with Python integers."""
+    nsrc, k, seed, j = 50, 6, 11, 17
+    chosen = []
+    for t in range(k):
+        m = nsrc - k + t
+        r = (G.pair_u32_scalar(seed, j, t) * (m + 1)) >> 32
+        chosen.append(m if r in chosen else r)
+    np.testing.assert_array_equal(G.fixed_indegree_pre(nsrc, 40, k, seed)[j], chosen)
