@@ -39,6 +39,7 @@ struct HostGroup {
   uint64_t seed = 0;
   std::vector<int32_t> gen_steps, gen_idx;
   int64_t raster_cap = 0;
+  int64_t raster_region = 0;         // entries per append region on the device
   bool need_hist = false;
   int H = 0;
   // device
@@ -532,12 +533,14 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     }
     d.nout = nout;
     if (g.raster_cap > 0) {
-      g.raster_d.alloc((size_t)g.raster_cap * 8);
-      g.raster_count_d.alloc(8);
-      BM_HIP(hipMemsetAsync(g.raster_count_d.p, 0, 8, net->stream));
+      // regions of 1.5x the even share, so an uneven spread over regions fits
+      g.raster_region = (g.raster_cap + RSHARD - 1) / RSHARD * 3 / 2 + 1024;
+      g.raster_d.alloc((size_t)g.raster_region * RSHARD * 8);
+      g.raster_count_d.alloc((size_t)RSHARD * RSTRIDE * 8);
+      BM_HIP(hipMemsetAsync(g.raster_count_d.p, 0, (size_t)RSHARD * RSTRIDE * 8, net->stream));
       d.raster = g.raster_d.as<unsigned long long>();
       d.raster_count = g.raster_count_d.as<unsigned long long>();
-      d.raster_cap = g.raster_cap;
+      d.raster_cap = g.raster_region;
     }
     if (g.need_hist) {
       g.hist_d.alloc((size_t)g.H * g.N * 4);
@@ -1369,14 +1372,24 @@ int bm_net_set_state(bm_net *net, int gid, int var, const double *in) {
   });
 }
 
+namespace {
+// counters of the raster's append regions
+std::vector<unsigned long long> raster_counts(bm_net *net, HostGroup &g) {
+  std::vector<unsigned long long> all((size_t)RSHARD * RSTRIDE), c(RSHARD);
+  BM_HIP(hipMemcpyAsync(all.data(), g.raster_count_d.p, all.size() * 8, hipMemcpyDeviceToHost, net->stream));
+  BM_HIP(hipStreamSynchronize(net->stream));
+  for (int r = 0; r < RSHARD; ++r) c[r] = all[(size_t)r * RSTRIDE];
+  return c;
+}
+}  // namespace
+
 int bm_net_spike_count(bm_net *net, int gid, int64_t *count) {
   return guard([&] {
     HostGroup &g = grp(net, gid);
     BM_REQUIRE(g.raster_cap > 0 && net->prepared, "group is not recorded or network not prepared");
-    unsigned long long c = 0;
-    BM_HIP(hipMemcpyAsync(&c, g.raster_count_d.p, 8, hipMemcpyDeviceToHost, net->stream));
-    BM_HIP(hipStreamSynchronize(net->stream));
-    *count = (int64_t)c;
+    int64_t tot = 0;
+    for (unsigned long long c : raster_counts(net, g)) tot += (int64_t)c;
+    *count = tot;
   });
 }
 
@@ -1384,23 +1397,31 @@ int bm_net_get_spikes(bm_net *net, int gid, int32_t *steps, int32_t *idx, int64_
   return guard([&] {
     HostGroup &g = grp(net, gid);
     BM_REQUIRE(g.raster_cap > 0 && net->prepared, "group is not recorded or network not prepared");
-    unsigned long long c = 0;
-    BM_HIP(hipMemcpyAsync(&c, g.raster_count_d.p, 8, hipMemcpyDeviceToHost, net->stream));
-    BM_HIP(hipStreamSynchronize(net->stream));
-    if ((int64_t)c > g.raster_cap)
-      throw Error(BM_EOVERFLOW, "spike raster overflow: " + std::to_string(c) + " spikes > capacity " +
-                                    std::to_string(g.raster_cap));
-    BM_REQUIRE((int64_t)c <= cap, "output buffer too small");
+    const std::vector<unsigned long long> cs = raster_counts(net, g);
+    int64_t c = 0;
+    for (int r = 0; r < RSHARD; ++r) {
+      if ((int64_t)cs[r] > g.raster_region)
+        throw Error(BM_EOVERFLOW, "spike raster overflow: region " + std::to_string(r) + " received " +
+                                      std::to_string(cs[r]) + " spikes > " + std::to_string(g.raster_region) +
+                                      " (record with a larger capacity)");
+      c += (int64_t)cs[r];
+    }
+    BM_REQUIRE(c <= cap, "output buffer too small");
     std::vector<uint64_t> tmp(c);
-    if (c)
-      BM_HIP(hipMemcpyAsync(tmp.data(), g.raster_d.p, c * 8, hipMemcpyDeviceToHost, net->stream));
+    int64_t at = 0;
+    for (int r = 0; r < RSHARD; ++r) {
+      if (cs[r])
+        BM_HIP(hipMemcpyAsync(tmp.data() + at, g.raster_d.as<unsigned long long>() + (int64_t)r * g.raster_region,
+                              cs[r] * 8, hipMemcpyDeviceToHost, net->stream));
+      at += (int64_t)cs[r];
+    }
     BM_HIP(hipStreamSynchronize(net->stream));
     std::sort(tmp.begin(), tmp.end());
     for (size_t q = 0; q < tmp.size(); ++q) {
       steps[q] = (int32_t)(tmp[q] >> 32);
       idx[q] = (int32_t)(tmp[q] & 0xFFFFFFFFu);
     }
-    *n_out = (int64_t)c;
+    *n_out = c;
   });
 }
 
@@ -1408,7 +1429,7 @@ int bm_net_clear_spikes(bm_net *net, int gid) {
   return guard([&] {
     HostGroup &g = grp(net, gid);
     BM_REQUIRE(g.raster_cap > 0 && net->prepared, "group is not recorded or network not prepared");
-    BM_HIP(hipMemsetAsync(g.raster_count_d.p, 0, 8, net->stream));
+    BM_HIP(hipMemsetAsync(g.raster_count_d.p, 0, (size_t)RSHARD * RSTRIDE * 8, net->stream));
     BM_HIP(hipStreamSynchronize(net->stream));
   });
 }

@@ -190,12 +190,18 @@ __device__ inline void zero_slice(unsigned char *slot, int i, int sw) {
   else reinterpret_cast<uint32_t *>(slot)[i] = 0u;
 }
 
-// Applies `cnt` deliveries of weight w to x in the reference order:
-// Connection adds each weight to the target (connection.py:240-241);
-// DelayConnection sums into its ring slot, then adds once (delayconnection.py:112-119).
+// Applies `cnt` deliveries of weight w to x.  FP64 (check mode) repeats the
+// reference's additions in its order: Connection adds each weight to the
+// target (connection.py:240-241); DelayConnection sums into its ring slot,
+// then adds once (delayconnection.py:112-119).  FP32 (fast path) adds
+// cnt * w in one step: no data-dependent loop, the same value up to FP32
+// rounding of the repeated sum.
 template <typename T>
 __device__ inline T apply_count(T x, uint32_t cnt, T w, int is_delay) {
 #pragma clang fp contract(off)
+  if constexpr (sizeof(T) == 4) {
+    return cnt ? x + (T)cnt * w : x;
+  }
   if (is_delay) {
     T acc = (T)0;
     for (uint32_t q = 0; q < cnt; ++q) acc = acc + w;
@@ -674,15 +680,17 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
   if (nsp > 0) {
     // ---------------- raster append (SpikeMonitor, brian/monitor.py:156-166)
     if (g.raster != nullptr) {
-      if (tid == 0) sh_base = atomicAdd(g.raster_count, (unsigned long long)nsp);
+      const int shard = blockIdx.x & (RSHARD - 1);
+      if (tid == 0) sh_base = atomicAdd(g.raster_count + shard * RSTRIDE, (unsigned long long)nsp);
       __syncthreads();
+      unsigned long long *region = g.raster + (int64_t)shard * g.raster_cap;
       for (int p = tid; p < nsp; p += BLOCK) {
         const uint32_t e = sh_list[p];
         const unsigned long long pos = sh_base + p;
         if (pos < (unsigned long long)g.raster_cap) {
           const uint64_t step = (uint64_t)(n0 + (e >> 12));
           const uint64_t idx = (uint64_t)(blk0 + (e & 0xFFFu));
-          g.raster[pos] = (step << 32) | idx;
+          region[pos] = (step << 32) | idx;
         }
       }
     }
@@ -744,10 +752,11 @@ __global__ __launch_bounds__(BLOCK) void k_generator(const GroupDev g, const int
       int pos = 0;
       if (act) pos = atomicAdd(&sh_cnt, 1);
       __syncthreads();
-      if (threadIdx.x == 0) sh_b2 = atomicAdd(g.raster_count, (unsigned long long)sh_cnt);
+      const int shard = blockIdx.x & (RSHARD - 1);
+      if (threadIdx.x == 0) sh_b2 = atomicAdd(g.raster_count + shard * RSTRIDE, (unsigned long long)sh_cnt);
       __syncthreads();
       if (act && sh_b2 + pos < (unsigned long long)g.raster_cap)
-        g.raster[sh_b2 + pos] = ((uint64_t)n << 32) | (uint64_t)i;
+        g.raster[(int64_t)shard * g.raster_cap + sh_b2 + pos] = ((uint64_t)n << 32) | (uint64_t)i;
       __syncthreads();
     }
     if (!act) continue;

@@ -12,6 +12,11 @@ constexpr int MAXCHAN = 8;
 constexpr int MAXOUT = 8;
 constexpr int MAXRESET = 4;
 constexpr int MAXMON = 4;
+// spike raster: RSHARD append regions with their own counters (one 64-byte
+// line apart) so the workgroups of a launch do not all contend for one
+// atomic; a workgroup appends to region blockIdx % RSHARD
+constexpr int RSHARD = 32;
+constexpr int RSTRIDE = 8;     // counters are RSTRIDE uint64 apart
 constexpr int BLOCK = 256;
 constexpr int WMAX = 16;
 constexpr int32_t NEVER = INT32_MIN;
@@ -102,8 +107,8 @@ struct GroupDev {
   int nout, H;
   OutDev outs[MAXOUT];
   unsigned long long *raster;
-  unsigned long long *raster_count;
-  int64_t raster_cap;
+  unsigned long long *raster_count;   // [RSHARD * RSTRIDE], region r's counter at r * RSTRIDE
+  int64_t raster_cap;                 // entries per region
   int32_t *hist;              // [H][N] spike lists (STDP), null if unused
   unsigned int *hist_count;   // [H]
   const uint64_t *pthr;       // Poisson thresholds [N]
