@@ -242,3 +242,29 @@ def test_cobahh_golden_exp_euler():
     print(f"COBAHH device vs golden: {len(s)} vs {len(gs)} spikes, first divergence step {first}")
     assert len(s) == len(gs) == 156843
     assert first is None
+
+
+@pytest.mark.parametrize("method,kind,order", [("Euler", "euler", 1), ("RK", "rk2", 2)])
+def test_nonlinear_schemes_match_oracle(method, kind, order):
+    """Euler (equations.py:655-670) and RK2 midpoint (:714-739) on the device
+    bytecode interpreter against the oracle's schemes, FP64, on
+    dv/dt = -v^2/s + w/s, dw/dt = -w*v/s (nonlinear, coupled)."""
+    from brian_amd import NeuronGroup, StateMonitor, Network, second
+    from oracle import brian1 as O
+    v0 = np.array([0.5, 1.0, 2.0, 3.0])
+    w0 = np.array([0.1, 0.2, 0.3, 0.4])
+    G = NeuronGroup(4, "dv/dt = (-v*v + w)/second : 1\ndw/dt = -w*v/second : 1", method=method,
+                    order=order)
+    G.v = v0
+    G.w = w0
+    M = StateMonitor(G, "v", record=True)
+    net = Network(G, M)
+    net.run(2000 * 1e-4)
+    rhs = lambda S: [-S[0] * S[0] + S[1], -S[1] * S[0]]
+    og = O.Group(4, np.array([v0, w0]), (kind, rhs), None, None, dt=1e-4)
+    ref = []
+    for k in range(2000):
+        og.update(k * 1e-4, k)
+        ref.append(og.S[0].copy())
+    got = M.values.T
+    np.testing.assert_allclose(got, np.array(ref), rtol=1e-12, atol=1e-15)

@@ -364,3 +364,81 @@ def _cobahh_exact(steps):
     assert len(s) == int(m.sum()) and len(s) > 1000
     np.testing.assert_array_equal(s, gs[m])
     np.testing.assert_array_equal(i, gi[m])
+
+
+def _stdp_pair(pre_step, post_step, taup=20e-3, taum=20e-3, Ap=0.01, Am=-0.0105, wmax=1.0, w0=0.5,
+               nsteps=None):
+    """One pre (SpikeSource) -> one post neuron forced to fire at post_step."""
+    nsteps = nsteps or max(pre_step, post_step) + 5
+    net = O.Net(DT)
+    src = O.SpikeSource(1, step_lists=[np.array([0]) if k == pre_step else np.zeros(0, np.int64)
+                                       for k in range(nsteps)])
+    sid = net.add_group(src, record=False)
+    drive = np.zeros((nsteps, 1))
+    drive[post_step] = 1.0
+    grp = O.Group(1, np.zeros((1, 1)), ("linear", np.eye(1), np.zeros(1)), ("const", 0, 0.5),
+                  [(0, "=", 0.0, False)], dt=DT, inputs=[(0, drive)])
+    gid = net.add_group(grp)
+    c = net.add_connection((sid, 0, 1), (gid, 0, 1), 0, np.array([0, 1]), np.array([0]), np.array([w0]))
+    net.add_stdp(c, taup=taup, taum=taum, Ap=Ap, Am=Am, wmin=0.0, wmax=wmax)
+    net.run(nsteps)
+    return float(c.data[0])
+
+
+def test_stdp_pair_closed_forms():
+    """ExponentialSTDP (stdp.py:452-539) on a single pairing: pre then post by
+    k steps potentiates by Ap*wmax*exp(-k dt/taup); post then pre depresses
+    by Am*wmax*exp(-k dt/taum); weights stay in [wmin, wmax]."""
+    Ap, Am, wmax, w0 = 0.01, -0.0105, 1.0, 0.5
+    for k in (1, 7, 100, 400):
+        w = _stdp_pair(10, 10 + k, Ap=Ap, Am=Am, wmax=wmax, w0=w0)
+        assert abs((w - w0) - Ap * wmax * np.exp(-k * DT / 20e-3)) < 1e-12
+        w = _stdp_pair(10 + k, 10, Ap=Ap, Am=Am, wmax=wmax, w0=w0)
+        assert abs((w - w0) - Am * wmax * np.exp(-k * DT / 20e-3)) < 1e-12
+    # asymmetric time constants select the right trace
+    w = _stdp_pair(10, 60, taup=5e-3, taum=40e-3, Ap=Ap, Am=Am, w0=w0)
+    assert abs((w - w0) - Ap * np.exp(-50 * DT / 5e-3)) < 1e-12
+    w = _stdp_pair(60, 10, taup=5e-3, taum=40e-3, Ap=Ap, Am=Am, w0=w0)
+    assert abs((w - w0) - Am * np.exp(-50 * DT / 40e-3)) < 1e-12
+    # clipping
+    assert _stdp_pair(10, 11, Ap=10.0, w0=0.9) == 1.0
+    assert _stdp_pair(11, 10, Am=-10.0, w0=0.1) == 0.0
+
+
+def _integrate(kind, rhs, v0, T, dt):
+    n = int(round(T / dt))
+    g = O.Group(len(v0), np.array([v0]), (kind, rhs), None, None, dt=dt)
+    for k in range(n):
+        g.update(k * dt, k)
+    return g.S[0]
+
+
+def test_integration_scheme_orders():
+    """Euler (equations.py:655-670) converges at order 1, RK2 midpoint
+    (:714-739) at order 2 on dv/dt = -v^2 (v(t) = v0/(1 + v0 t)); exponential
+    Euler (:741-789) is exact on a linear equation and order 1 on a
+    conditionally linear system (dx/dt = -y x, dy/dt = -y/tau:
+    x(t) = x0 exp(-y0 tau (1 - exp(-t/tau))))."""
+    v0 = np.array([0.5, 1.0, 2.0])
+    T = 1.0
+    sq = lambda S: [-S[0] * S[0]]
+    exact = v0 / (1 + v0 * T)
+    for kind, order in (("euler", 1), ("rk2", 2)):
+        e1 = np.abs(_integrate(kind, sq, v0, T, 1e-2) - exact).max()
+        e2 = np.abs(_integrate(kind, sq, v0, T, 5e-3) - exact).max()
+        assert abs(np.log2(e1 / e2) - order) < 0.15, (kind, e1, e2)
+    lin = lambda S: [(0.3 - S[0]) / 0.02]
+    got = _integrate("expeuler", lin, v0, 0.05, 1e-3)
+    np.testing.assert_allclose(got, 0.3 + (v0 - 0.3) * np.exp(-0.05 / 0.02), rtol=1e-12)
+    tau = 0.5
+    cond = lambda S: [-S[1] * S[0], -S[1] / tau]
+
+    def run2(dt):
+        g = O.Group(3, np.array([v0, [2.0, 2.0, 2.0]]), ("expeuler", cond), None, None, dt=dt)
+        for k in range(int(round(T / dt))):
+            g.update(k * dt, k)
+        return g.S[0]
+    exact2 = v0 * np.exp(-2.0 * tau * (1 - np.exp(-T / tau)))
+    e1 = np.abs(run2(1e-2) - exact2).max()
+    e2 = np.abs(run2(5e-3) - exact2).max()
+    assert abs(np.log2(e1 / e2) - 1) < 0.15, (e1, e2)
