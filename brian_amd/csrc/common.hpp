@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+
+#include "devcommon.hpp"
 #include <stdexcept>
 #include <string>
 
@@ -75,35 +77,5 @@ struct DevBuf {
   }
   template <typename T> T *as() const { return static_cast<T *>(p); }
 };
-
-// splitmix64 finaliser; the Poisson sources use two rounds keyed by
-// (seed, stream) and the step counter (same integer arithmetic on host tests).
-__host__ __device__ inline uint64_t mix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-__host__ __device__ inline uint32_t counter_u32(uint64_t seed, uint64_t stream, uint64_t counter) {
-  const uint64_t G = 0x9E3779B97F4A7C15ull;
-  uint64_t key = mix64(seed * G + stream);
-  return (uint32_t)(mix64(key + counter * G) >> 32);
-}
-
-// Cheaper per-step stream for inputs drawn every step (batched STDP): a
-// 32-bit key per stream, then MurmurHash3's finaliser of a Weyl sequence
-// key + counter * 0x9E3779B9 (a bijection of the counter for each key).
-__host__ __device__ inline uint32_t stream_key32(uint64_t seed, uint64_t stream) {
-  const uint64_t G = 0x9E3779B97F4A7C15ull;
-  return (uint32_t)(mix64(seed * G + stream) >> 32);
-}
-__host__ __device__ inline uint32_t weyl_u32(uint32_t key, uint32_t counter) {
-  uint32_t h = key + counter * 0x9E3779B9u;
-  h ^= h >> 16;
-  h *= 0x85EBCA6Bu;
-  h ^= h >> 13;
-  h *= 0xC2B2AE35u;
-  h ^= h >> 16;
-  return h;
-}
 
 }  // namespace bm

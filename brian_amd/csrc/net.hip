@@ -2,17 +2,31 @@
 #include "../../include/brianmi.h"
 #include "net_launch.hpp"
 
+#include <hip/hiprtc.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
+#include <sstream>
+#include <string>
 #include <vector>
+
+#include "jit_src.inc"   // kJitHeaders / kJitHeaderNames: the kernel headers as text (generated at build)
 
 namespace bm {
 
 static thread_local std::string g_last_error;
 void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+// a module compiled at run time and the kernel in it
+struct JitKernel {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+};
 
 struct HostGroup {
   int64_t N = 0;
@@ -52,6 +66,8 @@ struct HostGroup {
   int64_t gen_nsteps = 0;
   int64_t host_raster_read = 0;
   GroupDev dev{};
+  JitKernel jit;          // specialised lean sweep for jit_K neurons per thread
+  int jit_K = 0;
   struct Mon {
     int var = 0, timestep = 1, mode = 0;
     int64_t cap = 0;
@@ -102,6 +118,7 @@ struct bm_net {
   bool prepared = false;
   int W = 1, K = 4;
   int Wcap = 1;           // largest window the rings are sized for
+  bool specialize = true; // lean sweep compiled at run time for each group's shape
   bool use_graph = true;
   std::vector<std::pair<int, int>> mon_ids;   // (group, monitor within group)
   // tiling knobs (bm_net_set_tuning)
@@ -199,6 +216,96 @@ void check_launch_shapes(bm_net *net) {
   }
 }
 
+// ---- lean sweep specialised at run time (hipRTC)
+// The lean kernel's ring layout, channel table, reset ops and threshold are
+// fixed per group; compiled with those as constants the kernel carries no
+// per-step table lookups.  Modules are cached per source for the process.
+std::mutex g_jit_mu;
+std::map<std::string, JitKernel> g_jit;
+
+std::string jit_list(const std::vector<int> &v) {
+  std::ostringstream o;
+  for (size_t q = 0; q < 4; ++q) o << (q ? "," : "") << (q < v.size() ? v[q] : 0);
+  return o.str();
+}
+
+JitKernel jit_sweep(bm_net *net, const HostGroup &g, int K) {
+  std::vector<int> sh, var, dly, rv, ro, rvv;
+  const int nring = (int)g.ring_desc.size();
+  const int bits = nring ? g.ring_desc[0].bits : 8, sw = nring ? g.ring_desc[0].sw : 1;
+  for (const ChanDev &c : g.chans) {
+    sh.push_back(c.ch * bits);
+    var.push_back(c.var);
+    dly.push_back(c.is_delay);
+  }
+  for (size_t r = 0; r < g.rs_var.size(); ++r) {
+    rv.push_back(g.rs_var[r]);
+    ro.push_back(g.rs_op[r]);
+    rvv.push_back(g.rs_vvar[r]);
+  }
+  BM_REQUIRE(g.chans.size() <= 4 && g.rs_var.size() <= 4, "lean kernel shape out of range");
+  std::ostringstream d;
+  d << "#define BM_JIT 1\n#define BM_JIT_SW " << sw << "\n#define BM_JIT_BITS " << bits << "\n#define BM_JIT_NCH "
+    << g.chans.size() << "\n#define BM_JIT_NRESET " << g.rs_var.size() << "\n#define BM_JIT_NRING " << nring
+    << "\n#define BM_JIT_CH_SHIFT " << jit_list(sh) << "\n#define BM_JIT_CH_VAR " << jit_list(var)
+    << "\n#define BM_JIT_CH_DELAY " << jit_list(dly) << "\n#define BM_JIT_RS_VAR " << jit_list(rv)
+    << "\n#define BM_JIT_RS_OP " << jit_list(ro) << "\n#define BM_JIT_RS_VVAR " << jit_list(rvv)
+    << "\n#define BM_JIT_TH_KIND " << g.th_kind << "\n#define BM_JIT_TH_VAR " << g.th_var
+    << "\n#define BM_JIT_TH_VTVAR " << g.th_vtvar << "\n#include \"net_kernels.hpp\"\n";
+  const std::string name = std::string("bm::k_sweep<") + (net->precision == 64 ? "double" : "float") + ", " +
+                           std::to_string(g.M) + ", " + std::to_string(K) + ", 0>";
+  const std::string src = d.str() + "// " + name + "\n";
+  std::lock_guard<std::mutex> lk(g_jit_mu);
+  auto it = g_jit.find(src);
+  if (it != g_jit.end()) return it->second;
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "sweep_jit.hip", kJitNumHeaders, kJitHeaders, kJitHeaderNames) !=
+      HIPRTC_SUCCESS)
+    throw Error(BM_EHIP, "hiprtcCreateProgram failed for the sweep kernel");
+  hiprtcAddNameExpression(prog, name.c_str());
+  // packed FP32 instructions stay disabled, as for the rest of the library
+  const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                        "-packed-fp32-ops"};
+  hiprtcResult rc = hiprtcCompileProgram(prog, 7, opts);
+  if (rc != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    hiprtcGetProgramLogSize(prog, &n);
+    std::string log(n, '\0');
+    if (n) hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    throw Error(BM_EHIP, "specialised sweep kernel does not compile:\n" + log.substr(0, 4000));
+  }
+  const char *lowered = nullptr;
+  hiprtcGetLoweredName(prog, name.c_str(), &lowered);
+  const std::string low = lowered ? lowered : "";
+  size_t sz = 0;
+  hiprtcGetCodeSize(prog, &sz);
+  std::vector<char> code(sz);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  BM_REQUIRE(!low.empty(), "specialised sweep kernel: no lowered name");
+  JitKernel k;
+  BM_HIP(hipModuleLoadData(&k.mod, code.data()));
+  BM_HIP(hipModuleGetFunction(&k.fn, k.mod, low.c_str()));
+  g_jit[src] = k;
+  return k;
+}
+
+// Compiles (or takes from the cache) the specialised lean kernel of every
+// group for the current neurons-per-thread; called before any launch or
+// graph capture.
+void ensure_jit(bm_net *net) {
+  if (!net->specialize) return;
+  for (auto &gp : net->groups) {
+    HostGroup &g = *gp;
+    if (g.kind == GK_GENERATOR || !fast_eligible(g)) continue;
+    const int K = group_K(net, g);
+    if (g.jit_K == K && g.jit.fn) continue;
+    g.jit = jit_sweep(net, g, K);
+    g.jit_K = K;
+  }
+}
+
 void launch_group(bm_net *net, HostGroup &g, int64_t rel0, int W, int skip_first_tail,
                   int tail_only) {
   const int64_t *dbase = net->d_base.as<int64_t>();
@@ -214,6 +321,14 @@ void launch_group(bm_net *net, HostGroup &g, int64_t rel0, int W, int skip_first
   if (blocks == 0) return;
   const size_t shmem = sweep_shmem(net, g, K, W);
   const bool fast = fast_eligible(g);
+  if (fast && net->specialize) {
+    BM_REQUIRE(g.jit_K == K && g.jit.fn, "specialised sweep kernel not prepared for this shape");
+    void *args[] = {(void *)&g.dev, (void *)&dbase, (void *)&rel0, (void *)&W, (void *)&skip_first_tail,
+                    (void *)&tail_only};
+    BM_HIP(hipModuleLaunchKernel(g.jit.fn, (unsigned)blocks, 1, 1, BLOCK, 1, 1, (unsigned)shmem, net->stream,
+                                 args, nullptr));
+    return;
+  }
   SweepArgs a{dbase, rel0, W, skip_first_tail, tail_only};
   if (net->precision == 64) launch_sweep<double>(g.dev, fast, g.M, K, (unsigned)blocks, shmem, net->stream, a);
   else launch_sweep<float>(g.dev, fast, g.M, K, (unsigned)blocks, shmem, net->stream, a);
@@ -629,6 +744,7 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     d.post_N = tg.N;
   }
   check_launch_shapes(net);
+  ensure_jit(net);
   net->d_base.alloc(8);
   int64_t zero = 0;
   BM_HIP(hipMemcpyAsync(net->d_base.p, &zero, 8, hipMemcpyHostToDevice, net->stream));
@@ -662,6 +778,7 @@ void run(bm_net *net, int64_t nsteps) {
   BM_REQUIRE(net->prepared, "bm_net_prepare must be called before bm_net_run");
   BM_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
   if (nsteps == 0) return;
+  ensure_jit(net);
   const int W = net->W;
   int64_t done = 0;
   set_base(net, net->step);
@@ -1293,6 +1410,9 @@ int bm_net_set_tuning(bm_net *net, int key, int value) {
       case BM_TUNE_USE_GRAPH:
         net->use_graph = value != 0;
         break;
+      case BM_TUNE_SPECIALIZE:
+        net->specialize = value != 0;
+        break;
       default:
         throw Error(BM_EINVAL, "unknown tuning key");
     }
@@ -1317,6 +1437,7 @@ int bm_net_get_tuning(bm_net *net, int key, int *value) {
       case BM_TUNE_RING_PACK: *value = net->ring_pack; break;
       case BM_TUNE_USE_GRAPH: *value = net->use_graph ? 1 : 0; break;
       case BM_TUNE_WINDOW_MAX: *value = net->prepared ? net->Wcap : 0; break;
+      case BM_TUNE_SPECIALIZE: *value = net->specialize ? 1 : 0; break;
       default: throw Error(BM_EINVAL, "unknown tuning key");
     }
   });

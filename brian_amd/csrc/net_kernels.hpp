@@ -348,6 +348,16 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
   constexpr bool GEN = (F & F_GEN) != 0;
   constexpr int PF = GEN ? 1 : 8;                 // ring prefetch depth (steps)
   constexpr int MM = M > 0 ? M : 1;
+#ifdef BM_JIT
+  // lean kernel specialised at run time (hipRTC) for one group: the ring
+  // layout, the channel table, the reset ops and the threshold become
+  // compile-time constants (values such as weights stay arguments)
+  constexpr int J_SW = BM_JIT_SW, J_BITS = BM_JIT_BITS, J_NCH = BM_JIT_NCH, J_NRESET = BM_JIT_NRESET;
+  constexpr int J_CH_SHIFT[4] = {BM_JIT_CH_SHIFT}, J_CH_VAR[4] = {BM_JIT_CH_VAR}, J_CH_DELAY[4] = {BM_JIT_CH_DELAY};
+  constexpr int J_RS_VAR[4] = {BM_JIT_RS_VAR}, J_RS_OP[4] = {BM_JIT_RS_OP}, J_RS_VVAR[4] = {BM_JIT_RS_VVAR};
+  constexpr int J_TH_KIND = BM_JIT_TH_KIND, J_TH_VAR = BM_JIT_TH_VAR, J_TH_VTVAR = BM_JIT_TH_VTVAR;
+  constexpr int J_NRING = BM_JIT_NRING;
+#endif
   extern __shared__ __align__(16) unsigned char smem[];
   uint16_t *sh_list = reinterpret_cast<uint16_t *>(smem);   // (w << 12) | local index
   __shared__ int sh_n;
@@ -404,7 +414,11 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
     uint32_t cw[PF][K];
     if constexpr (!GEN) {
       // lean path: one packed count ring; prefetch PF slots, then zero them
+#ifdef BM_JIT
+      if (J_NRING == 1) {
+#else
       if (g.nring == 1) {
+#endif
         const RingDev &rd = g.rings[0];
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
@@ -415,7 +429,11 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
           const bool on = w < nsteps && !(w == 0 && skip_first_tail);
 #pragma unroll
           for (int k = 0; k < K; ++k)
+#ifdef BM_JIT
+            cw[u][k] = (on && valid[k]) ? load_slice(slot, blk0 + k * BLOCK + tid, J_SW) : 0u;
+#else
             cw[u][k] = (on && valid[k]) ? load_slice(slot, blk0 + k * BLOCK + tid, rd.sw) : 0u;
+#endif
         }
       }
     }
@@ -437,7 +455,11 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
           const bool on = w < nsteps && !(w == 0 && skip_first_tail);
 #pragma unroll
           for (int k = 0; k < K; ++k)
+#ifdef BM_JIT
+            if (on && valid[k]) zero_slice(slot, blk0 + k * BLOCK + tid, J_SW);
+#else
             if (on && valid[k]) zero_slice(slot, blk0 + k * BLOCK + tid, rd.sw);
+#endif
         }
       }
     }
@@ -452,6 +474,20 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
       if (!(w == 0 && skip_first_tail)) {
         const int64_t kst = n - 1;
         if constexpr (!GEN) {
+#ifdef BM_JIT
+          if (J_NRING == 1) {
+            constexpr uint32_t mask = (1u << J_BITS) - 1u;
+#pragma unroll
+            for (int c = 0; c < J_NCH; ++c) {
+              const T wv = (T)g.chans[c].w;
+#pragma unroll
+              for (int k = 0; k < K; ++k) {
+                const uint32_t cnt = (cw[u][k] >> J_CH_SHIFT[c]) & mask;
+                s[J_CH_VAR[c]][k] = apply_count<T>(s[J_CH_VAR[c]][k], cnt, wv, J_CH_DELAY[c]);
+              }
+            }
+          }
+#else
           if (g.nring == 1) {
             const RingDev &rd = g.rings[0];
             const uint32_t mask = (1u << rd.bits) - 1u;
@@ -467,6 +503,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
               }
             }
           }
+#endif
         } else {
           uint32_t words[MAXRING][K];
           T fval[MAXRING][K];
@@ -543,6 +580,16 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
               }
             }
           } else if (last[k] == kst) {
+#ifdef BM_JIT
+            if constexpr (!GEN) {
+#pragma unroll
+              for (int r = 0; r < J_NRESET; ++r) {
+                const T v = J_RS_VVAR[r] >= 0 ? s[J_RS_VVAR[r] >= 0 ? J_RS_VVAR[r] : 0][k] : (T)g.rs_val[r];
+                s[J_RS_VAR[r]][k] = J_RS_OP[r] == 0 ? v : s[J_RS_VAR[r]][k] + v;
+              }
+              continue;
+            }
+#endif
             for (int r = 0; r < g.nreset; ++r) {
               T v = (T)g.rs_val[r];
               if (g.rs_vvar[r] >= 0) {
@@ -638,12 +685,17 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
         if (GEN && g.kind == GK_POISSON) {
           spk[k] = (uint64_t)counter_u32(g.seed, (uint64_t)i, (uint64_t)n) < g.pthr[i];
         } else if (g.th_kind != TH_NONE) {
+#ifdef BM_JIT
+          const T v = s[J_TH_VAR][k];
+          const T vt = J_TH_KIND == TH_VAR ? s[J_TH_VTVAR >= 0 ? J_TH_VTVAR : 0][k] : (T)g.vt;
+#else
           T v = (T)0, vt = (T)g.vt;
 #pragma unroll
           for (int j = 0; j < MM; ++j) {
             if (j == g.th_var) v = s[j][k];
             if (g.th_kind == TH_VAR && j == g.th_vtvar) vt = s[j][k];
           }
+#endif
           bool c = v > vt;
           if constexpr (GEN) {
             if (c && g.th_kind == TH_EMPIRICAL && last[k] != NEVER && n - last[k] <= g.th_period) c = false;

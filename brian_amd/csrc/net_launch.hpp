@@ -2,6 +2,7 @@
 // kernels_f32.hip / kernels_f64.hip (separate translation units so the
 // library builds in parallel).
 #pragma once
+#include "common.hpp"
 #include "net_types.hpp"
 
 #include <vector>

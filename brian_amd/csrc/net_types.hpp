@@ -2,7 +2,7 @@
 // the kernels: groups, rings, connections and STDP descriptors passed to the
 // kernels by value.
 #pragma once
-#include "common.hpp"
+#include "devcommon.hpp"
 
 namespace bm {
 
@@ -19,7 +19,7 @@ constexpr int RSHARD = 32;
 constexpr int RSTRIDE = 8;     // counters are RSTRIDE uint64 apart
 constexpr int BLOCK = 256;
 constexpr int WMAX = 16;
-constexpr int32_t NEVER = INT32_MIN;
+constexpr int32_t NEVER = -2147483647 - 1;   // INT32_MIN
 
 enum RingKind { RING_COUNT = 0, RING_FLOAT = 1 };
 enum UpdKind { UPD_NONE = 0, UPD_LINEAR = 1, UPD_PERNEURON = 2, UPD_PROGRAM = 3 };

@@ -205,7 +205,11 @@ int bm_net_prepare(bm_net *net, int window, int neurons_per_thread, int use_grap
  *   BM_TUNE_RING_PACK          1 pack count channels into one ring word,
  *                              0 one ring per channel (before prepare)
  *   BM_TUNE_USE_GRAPH          replay captured launch chunks (hipGraph)
- *   BM_TUNE_WINDOW_MAX         (get only) largest window the rings allow */
+ *   BM_TUNE_WINDOW_MAX         (get only) largest window the rings allow
+ *   BM_TUNE_SPECIALIZE         1 (default): the lean sweep kernel of each group
+ *                              is compiled at run time (hipRTC) with the group's
+ *                              ring layout, channel table, resets and threshold
+ *                              as constants; 0: the precompiled generic form */
 enum {
   BM_TUNE_WINDOW = 1,
   BM_TUNE_NEURONS_PER_THREAD = 2,
@@ -214,7 +218,8 @@ enum {
   BM_TUNE_RING_BITS = 5,
   BM_TUNE_RING_PACK = 6,
   BM_TUNE_USE_GRAPH = 7,
-  BM_TUNE_WINDOW_MAX = 8
+  BM_TUNE_WINDOW_MAX = 8,
+  BM_TUNE_SPECIALIZE = 9
 };
 int bm_net_set_tuning(bm_net *net, int key, int value);
 int bm_net_get_tuning(bm_net *net, int key, int *value);

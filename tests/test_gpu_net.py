@@ -198,3 +198,24 @@ def test_state_monitor_on_device_windowed():
     assert n == step
     np.testing.assert_allclose(sm, sum_v, rtol=1e-12)
     np.testing.assert_allclose(sq, sq_v, rtol=1e-12)
+
+
+def test_specialised_sweep_matches_generic():
+    """The lean sweep compiled at run time for the group's shape and the
+    precompiled generic form produce the same FP64 raster and state."""
+    d = W.cuba(4000, seed=23, delay_steps=10)
+    out = []
+    for spec in (1, 0):
+        h, g = device_cuba(d, precision=64)
+        h.set_tuning("specialize", spec)
+        assert h.get_tuning("specialize") == spec
+        h.run(1500)
+        out.append((h.get_spikes(g), [h.get_state(g, v, 4000) for v in range(3)]))
+        h.close()
+    (s1, i1), st1 = out[0]
+    (s0, i0), st0 = out[1]
+    assert len(s1) > 1000
+    np.testing.assert_array_equal(s1, s0)
+    np.testing.assert_array_equal(i1, i0)
+    for a, b in zip(st1, st0):
+        np.testing.assert_array_equal(a, b)
