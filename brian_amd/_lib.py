@@ -60,6 +60,7 @@ _SIGS = {
                                      dbl, dbl, u64, C.c_int, C.c_int, dbl, dbl, C.c_int,
                                      C.POINTER(C.c_int)],
     "bm_net_connection_nnz": [P, C.c_int, Pi64],
+    "bm_net_set_modulation": [P, C.c_int, C.c_int],
     "bm_net_connect_fixed_indegree": [P, C.c_int, i64, i64, i64, i64, C.c_int, i64, i64, C.c_int, C.c_int,
                                       dbl, u64, C.c_int, C.POINTER(C.c_int)],
     "bm_net_get_connection": [P, C.c_int, Pi64, Pi32, Pi32],
@@ -275,6 +276,9 @@ class NetHandle:
              len(indices), i64ptr(indptr), i32ptr(indices), dptr(w), 1 if sd is not None else 0,
              int(delay_steps), i32ptr(sd), int(max_delay_steps), C.byref(cid))
         return cid.value
+
+    def set_modulation(self, cid, src_var):
+        call("bm_net_set_modulation", self.h, cid, int(src_var))
 
     def connect_random_blocks(self, src, tgt, var, src_off, src_n, tgt_off, tgt_n, p, weight, seed,
                               delay_steps=0, dmin=None, dmax=None, max_delay_steps=0):

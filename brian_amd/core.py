@@ -523,14 +523,22 @@ class Connection(_Tracked):
         self._init_common(source, target, state, modulation, weight, sparseness)
         self.delay = int(float(delay) / source.clock.dt)      # connection.py:200
         self._delay_kind = 0
+        if self._mod is not None and self.delay > 0:
+            # connection.py:242-247 reads the modulating variable `delay`
+            # steps after the spike; the device captures it at the spike step
+            raise NotImplementedError("modulation on a Connection with a homogeneous delay "
+                                      "is not supported (use DelayConnection)")
 
     def _init_common(self, source, target, state, modulation, weight, sparseness):
         self.source = source
         self.target = target
         self.state = state
         self.nstate = target._var(state) if not isinstance(state, int) else state
+        # modulation (connection.py:190-193): index of the source variable
+        # that scales the weights, read at the spike step
+        self._mod = None
         if modulation is not None:
-            raise NotImplementedError("modulation is not supported by the device backend")
+            self._mod = source._var(modulation) if not isinstance(modulation, int) else modulation
         self._W = sp.lil_matrix((len(source), len(target)))
         self._compiled_W = None
         self._net = None
@@ -1103,6 +1111,8 @@ class Network:
             c.cid = h.add_connection(self.gid[id(src)], c.source._origin, len(c.source),
                                      self.gid[id(tgt)], c.target._origin, len(c.target), c.nstate,
                                      W.indptr, W.indices, W.data, **kw)
+            if c._mod is not None:
+                h.set_modulation(c.cid, c._mod)
             c._net = self
         for s in self.stdps:
             dp, dm = s.decays(dt)

@@ -55,6 +55,7 @@ struct HostGroup {
   int64_t raster_cap = 0;
   int64_t raster_region = 0;         // entries per append region on the device
   bool need_hist = false;
+  int mod_var = -1;          // variable captured per spike for modulated outgoing connections
   int H = 0;
   // device
   DevBuf S, last, Apn_d, Cpn_d, code_d, consts_d, refr_d, ti_d, pthr_d, gen_idx_d, gen_off_d,
@@ -91,6 +92,7 @@ struct HostConn {
   int ring = -1, ch = 0;
   int stdp = -1;
   int64_t max_indeg = 0;
+  int mod_var = -1;          // source variable scaling the weights (modulation), -1 none
   DevBuf row_ptr_d, col_d, w_d, dly_d;
   int dly_bytes = 0;
   bool generated = false;   // CSR built on the device at prepare time
@@ -186,7 +188,7 @@ void require_unprepared(bm_net *net) {
 // (packed count) ring, no timed input.
 bool fast_eligible(const HostGroup &g) {
   if (g.kind != GK_NEURON || g.upd != UPD_LINEAR || g.M < 1 || g.M > 4) return false;
-  if (!g.mons.empty()) return false;
+  if (!g.mons.empty() || g.mod_var >= 0) return false;
   if (g.refr != REFR_NONE || g.th_kind == TH_EMPIRICAL || g.ti_nt > 0) return false;
   if (g.ring_desc.size() > 1) return false;
   if (g.ring_desc.size() == 1 && g.ring_desc[0].kind != RING_COUNT) return false;
@@ -200,8 +202,13 @@ int group_K(bm_net *net, const HostGroup &g) {
   return net->K >= 4 ? 4 : 1;
 }
 
+// dynamic LDS of a sweep launch: spike list (u16, padded to 16 bytes), one
+// modulation value per listed spike when the group drives a modulated
+// connection, then the registers of a bytecode program
 size_t sweep_shmem(bm_net *net, const HostGroup &g, int K, int W) {
-  size_t shmem = sizeof(uint16_t) * BLOCK * K * std::max(W, 1) + 16;
+  const size_t cap = (size_t)BLOCK * K * std::max(W, 1);
+  size_t shmem = (sizeof(uint16_t) * cap + 15) / 16 * 16 + 16;
+  if (g.mod_var >= 0) shmem += cap * net->tsize();
   if (g.upd == UPD_PROGRAM) shmem += (size_t)g.prog_nregs * BLOCK * net->tsize();
   return shmem;
 }
@@ -424,7 +431,13 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     std::vector<int64_t> indeg(c.tgt_n, 0);
     for (int64_t q = 0; q < c.nnz; ++q) indeg[c.col[q]]++;
     c.max_indeg = c.tgt_n ? *std::max_element(indeg.begin(), indeg.end()) : 0;
-    c.count_mode = uniform && c.stdp < 0 && c.max_indeg <= 65535;
+    c.count_mode = uniform && c.stdp < 0 && c.max_indeg <= 65535 && c.mod_var < 0;
+    if (c.mod_var >= 0) {
+      HostGroup &sg = *net->groups[c.src];
+      BM_REQUIRE(sg.mod_var < 0 || sg.mod_var == c.mod_var,
+                 "modulated connections of one source group must use the same variable");
+      sg.mod_var = c.mod_var;
+    }
     c.wval = c.nnz ? c.w[0] : 0.0;
     upload_raw(c.row_ptr_d, c.row_ptr.data(), c.row_ptr.size(), net->stream);
     upload_raw(c.col_d, c.col.data(), c.col.size(), net->stream);
@@ -590,6 +603,7 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     d.D = g.D;
     d.wave_max = net->wave_max;
     d.row_split = net->row_split;
+    d.mod_var = g.mod_var;
     d.nmon = (int)g.mons.size();
     for (int m = 0; m < d.nmon; ++m) {
       HostGroup::Mon &hm = *g.mons[m];
@@ -645,6 +659,7 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       o.ch = c.ch;
       o.D = tg.D;
       o.tgt_off = c.tgt_off;
+      o.mod = c.mod_var >= 0 ? 1 : 0;
     }
     d.nout = nout;
     if (g.raster_cap > 0) {
@@ -1203,6 +1218,21 @@ int bm_net_connect_fixed_indegree(bm_net *net, int src_gid, int64_t src_off, int
     c->max_delay = c->min_delay = delay_steps;
     net->conns.push_back(std::move(c));
     *cid = (int)net->conns.size() - 1;
+  });
+}
+
+int bm_net_set_modulation(bm_net *net, int cid, int src_var) {
+  return guard([&] {
+    require_unprepared(net);
+    BM_REQUIRE(cid >= 0 && cid < (int)net->conns.size(), "bad connection id");
+    HostConn &c = *net->conns[cid];
+    HostGroup &sg = grp(net, c.src);
+    BM_REQUIRE(sg.kind == GK_NEURON && src_var >= 0 && src_var < sg.M, "modulation variable out of range");
+    BM_REQUIRE(!c.generated, "modulation of device-generated connections is not supported");
+    BM_REQUIRE(c.delay_kind == 1 || c.hom_delay == 0,
+               "a modulated Connection with a homogeneous delay reads the presynaptic value at delivery "
+               "time; use delay 0 or a DelayConnection");
+    c.mod_var = src_var;
   });
 }
 

@@ -243,8 +243,8 @@ __device__ inline int block_exclusive_scan(int v, int tid, int *sh_wsum, int *to
 // lanes stride over each row (two dependent round trips per spike, no block
 // barriers).
 template <typename T>
-__device__ inline void propagate_waves(const GroupDev &g, const uint16_t *sh_list, int nsp, int64_t n0,
-                                       int blk0, int tid) {
+__device__ inline void propagate_waves(const GroupDev &g, const uint16_t *sh_list, const T *sh_mval, int nsp,
+                                       int64_t n0, int blk0, int tid) {
   const int wave = tid >> 6, lane = tid & 63;
   const int rs = g.row_split;                       // power of two in [1, 64]
   const int per_wave = 64 / rs;
@@ -272,7 +272,9 @@ __device__ inline void propagate_waves(const GroupDev &g, const uint16_t *sh_lis
           const int64_t pos = tj * pbits + od.ch * od.bits;
           atomicAdd(reinterpret_cast<unsigned int *>(slot) + (pos >> 5), 1u << (int)(pos & 31));
         } else {
-          unsafeAtomicAdd(reinterpret_cast<T *>(slot) + tj, static_cast<const T *>(od.w)[q]);
+          T wq = static_cast<const T *>(od.w)[q];
+          if (od.mod) wq = wq * sh_mval[p];       // connection.py:242-247
+          unsafeAtomicAdd(reinterpret_cast<T *>(slot) + tj, wq);
         }
       }
     }
@@ -280,10 +282,10 @@ __device__ inline void propagate_waves(const GroupDev &g, const uint16_t *sh_lis
 }
 
 template <typename T>
-__device__ inline void propagate_block(const GroupDev &g, const uint16_t *sh_list, int nsp, int64_t n0,
-                                       int blk0, int tid) {
+__device__ inline void propagate_block(const GroupDev &g, const uint16_t *sh_list, const T *sh_mval, int nsp,
+                                       int64_t n0, int blk0, int tid) {
   if (nsp <= g.wave_max) {
-    propagate_waves<T>(g, sh_list, nsp, n0, blk0, tid);
+    propagate_waves<T>(g, sh_list, sh_mval, nsp, n0, blk0, tid);
     return;
   }
   __shared__ int64_t sh_lo[BLOCK];
@@ -333,7 +335,9 @@ __device__ inline void propagate_block(const GroupDev &g, const uint16_t *sh_lis
           const int64_t pos = tj * pbits + od.ch * od.bits;
           atomicAdd(reinterpret_cast<unsigned int *>(slot) + (pos >> 5), 1u << (int)(pos & 31));
         } else {
-          unsafeAtomicAdd(reinterpret_cast<T *>(slot) + tj, static_cast<const T *>(od.w)[syn]);
+          T wq = static_cast<const T *>(od.w)[syn];
+          if (od.mod) wq = wq * sh_mval[base + a];
+          unsafeAtomicAdd(reinterpret_cast<T *>(slot) + tj, wq);
         }
       }
       __syncthreads();
@@ -404,7 +408,11 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
       }
     }
   }
-  T *reg = reinterpret_cast<T *>(smem + sizeof(uint16_t) * BLOCK * K * (W > 0 ? W : 1) + 16);
+  // dynamic LDS: spike list (u16), then (GEN) the modulation value of each
+  // listed spike (T), then the program registers
+  const int list_cap = BLOCK * K * (W > 0 ? W : 1);
+  T *sh_mval = reinterpret_cast<T *>(smem + ((sizeof(uint16_t) * list_cap + 15) / 16) * 16);
+  T *reg = GEN && g.mod_var >= 0 ? sh_mval + list_cap : sh_mval;
   const int nsteps = tail_only ? 1 : W;
   // ring slot of the tail at window offset w is (n0 - 1 + w) mod D
   int slot0 = (int)((n0 - 1) % D);
@@ -722,6 +730,17 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
         if (spk[k]) {
           const unsigned long long below = mask & ((1ull << lane_id()) - 1ull);
           sh_list[base + __popcll(below)] = (uint16_t)((w << 12) | (k * BLOCK + tid));
+          if constexpr (GEN) {
+            // modulated connections read the source variable at the spike
+            // step, after the update and before the reset (connection.py:231-247)
+            if (g.mod_var >= 0) {
+              T mv = (T)0;
+#pragma unroll
+              for (int j = 0; j < MM; ++j)
+                if (j == g.mod_var) mv = s[j][k];
+              sh_mval[base + __popcll(below)] = mv;
+            }
+          }
         }
       }
     }
@@ -754,7 +773,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
       for (int p = tid; p < nsp; p += BLOCK)
         g.hist[hs * N + sh_hbase + p] = (int32_t)(blk0 + (sh_list[p] & 0xFFFu));
     }
-    propagate_block<T>(g, sh_list, nsp, n0, blk0, tid);
+    propagate_block<T>(g, sh_list, sh_mval, nsp, n0, blk0, tid);
   }
 
   // ---------------- write back

@@ -9,7 +9,7 @@ namespace bm {
 constexpr int MAXM = 8;
 constexpr int MAXRING = 4;
 constexpr int MAXCHAN = 8;
-constexpr int MAXOUT = 8;
+constexpr int MAXOUT = 16;       // GroupDev travels as a kernel argument (< 4 KiB)
 constexpr int MAXRESET = 4;
 constexpr int MAXMON = 4;
 // spike raster: RSHARD append regions with their own counters (one 64-byte
@@ -59,7 +59,8 @@ struct OutDev {         // one outgoing connection of this group
   int hom_delay;
   void *ring;
   int64_t slot_bytes;
-  int kind, bits, nch, ch, D, pad;
+  int kind, bits, nch, ch, D;
+  int mod;              // 1: weights scaled by the source's modulation value (FLOAT rings)
   int64_t tgt_off;
 };
 
@@ -126,9 +127,12 @@ struct GroupDev {
   // 64/row_split rows per wavefront); more spikes use the block-cooperative
   // scan over the concatenated rows.
   int wave_max, row_split;
-  int nmon, pad2;
+  int nmon;
+  int mod_var;          // source variable captured per spike for modulated connections, -1 none
   MonDev mons[MAXMON];
 };
+// passed by value to every sweep launch; kernel arguments are limited to 4 KiB
+static_assert(sizeof(GroupDev) <= 3584, "GroupDev must leave room in the 4 KiB kernel arguments");
 
 
 struct StdpDev {

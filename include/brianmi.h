@@ -127,6 +127,13 @@ int bm_net_add_connection(bm_net *net, int src_gid, int64_t src_off, int64_t src
                           const double *w, int delay_kind, int hom_delay_steps,
                           const int32_t *syn_delay, int max_delay_steps, int *cid);
 
+/* Modulation (connection.py:190-193, :242-247): the connection's weights are
+ * multiplied by the source neuron's variable src_var, read at the spike step
+ * after the state update and before the reset.  Plain connections only with
+ * delay 0 (the reference reads the value at delivery time); DelayConnections
+ * propagate at the spike step, as the reference does.  Before prepare. */
+int bm_net_set_modulation(bm_net *net, int cid, int src_var);
+
 /* Block-diagonal random connectivity built on the device (the role of
  * connect_random, brian/connections/construction.py:42-177, for many
  * independent networks): block b connects source rows [src_off[b],

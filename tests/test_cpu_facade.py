@@ -19,3 +19,19 @@ def test_build_objects_without_device():
         m3.mean
     net = Network(G, C, m1, m2, m3, S, R)
     assert net is not None
+
+
+def test_modulation_facade_checks():
+    """connection.py:190-193: modulation names a source variable; a plain
+    Connection with a homogeneous delay reads it `delay` steps late, which
+    the device does not reproduce, so it is refused."""
+    import brian_amd as b
+    b.reinit()
+    H = b.NeuronGroup(2, 'V:1\nmod:1', reset=0, threshold=1)
+    G = b.NeuronGroup(3, 'V:1', reset=0, threshold=1)
+    c = b.Connection(H, G, 'V', weight=2, modulation='mod')
+    assert c._mod == 1
+    d = b.Connection(H, G, 'V', weight=2, delay=True, modulation='mod')
+    assert d._mod == 1
+    with pytest.raises(NotImplementedError):
+        b.Connection(H, G, 'V', weight=2, delay=1 * b.ms, modulation='mod')

@@ -74,36 +74,6 @@ def test_epsp_peak():
     assert abs(M.times[i] - 1.69735 * msecond) < 0.1 * ms
 
 
-def test_propagation_structures():
-    """brian/tests/testfeatures/test_propagation.py:3-35 (the non-modulation
-    cases: sparse/dense/dynamic, with and without delays)."""
-    from brian_amd import NeuronGroup, Connection, SpikeMonitor, run, ms, defaultclock, arange
-    H = NeuronGroup(1, 'V:1\nmod:1', reset=0, threshold=1)
-    H.V = 2
-    H.mod = 1
-    G = [NeuronGroup(10, 'V:1', reset=0, threshold=1) for _ in range(6)]
-    M = [SpikeMonitor(g) for g in G]
-    C0 = Connection(H, G[0], 'V', weight=2)
-    C1 = Connection(H, G[1], 'V', weight=2, structure='dense')
-    C2 = Connection(H, G[2], 'V', weight=2, structure='dynamic')
-    C6 = Connection(H, G[3], 'V', weight=2, delay=True)
-    C7 = Connection(H, G[4], 'V', weight=2, delay=True, structure='dense')
-    C8 = Connection(H, G[5], 'V', weight=2, delay=True, structure='dynamic')
-    for c in [C6, C7, C8]:
-        c.delay[0, :] = arange(10) * defaultclock.dt + defaultclock.dt / 2
-    run(2 * ms)
-    for k, m in enumerate(M):
-        assert len(m.spikes), k
-        i, j = zip(*m.spikes)
-        i = np.array(i)
-        j = np.array((np.array(j) + defaultclock.dt / 2) / defaultclock.dt, dtype=int)
-        assert (i == np.arange(10)).all(), k
-        if k < 3:
-            assert (j == 1).all(), k
-        else:
-            assert (j == (1 + np.arange(10))).all(), (k, j)
-
-
 def test_refractoriness_clamp():
     """brian/tests/testinterface/test_reset.py:126-141."""
     from brian_amd import NeuronGroup, Network, Refractoriness, msecond
@@ -268,3 +238,99 @@ def test_nonlinear_schemes_match_oracle(method, kind, order):
         ref.append(og.S[0].copy())
     got = M.values.T
     np.testing.assert_allclose(got, np.array(ref), rtol=1e-12, atol=1e-15)
+
+
+def test_propagation_structures():
+    """brian/tests/testfeatures/test_propagation.py:3-37 in full: sparse,
+    dense and dynamic structures, with and without delays, with and without
+    modulation."""
+    from brian_amd import NeuronGroup, Connection, SpikeMonitor, run, ms, defaultclock, arange
+    H = NeuronGroup(1, 'V:1\nmod:1', reset=0, threshold=1)
+    H.V = 2
+    H.mod = 1
+    G = [NeuronGroup(10, 'V:1', reset=0, threshold=1) for _ in range(12)]
+    M = [SpikeMonitor(g) for g in G]
+    C = [Connection(H, G[0], 'V', weight=2), Connection(H, G[1], 'V', weight=2, structure='dense'),
+         Connection(H, G[2], 'V', weight=2, structure='dynamic'),
+         Connection(H, G[3], 'V', weight=2, modulation='mod'),
+         Connection(H, G[4], 'V', weight=2, structure='dense', modulation='mod'),
+         Connection(H, G[5], 'V', weight=2, structure='dynamic', modulation='mod'),
+         Connection(H, G[6], 'V', weight=2, delay=True),
+         Connection(H, G[7], 'V', weight=2, delay=True, structure='dense'),
+         Connection(H, G[8], 'V', weight=2, delay=True, structure='dynamic'),
+         Connection(H, G[9], 'V', weight=2, delay=True, modulation='mod'),
+         Connection(H, G[10], 'V', weight=2, delay=True, structure='dense', modulation='mod'),
+         Connection(H, G[11], 'V', weight=2, delay=True, structure='dynamic', modulation='mod')]
+    for c in C[6:]:
+        c.delay[0, :] = arange(10) * defaultclock.dt + defaultclock.dt / 2
+    run(2 * ms)
+    for k, m in enumerate(M):
+        assert len(m.spikes), k
+        i, j = zip(*m.spikes)
+        i = np.array(i)
+        j = np.array((np.array(j) + defaultclock.dt / 2) / defaultclock.dt, dtype=int)
+        assert (i == np.arange(10)).all(), k
+        if k < 6:
+            assert (j == 1).all(), k
+        else:
+            assert (j == 1 + np.arange(10)).all(), (k, j)
+
+
+def test_modulated_connection_matches_oracle():
+    """Weights scaled by the presynaptic variable at the spike step
+    (connection.py:190-193, :242-247; delayconnection.py:139-206): the 4000
+    neuron CUBA network with a fourth variable ``mod`` (relaxing to 1.5,
+    reset to 0.5 on a spike) modulating both projections, zero delay and
+    DelayConnection, FP64 raster bit-exact against the oracle."""
+    from brian_amd import workloads as W
+    from brian_amd._lib import NetHandle
+    from oracle import brian1 as O
+    from tests.netbuild import cuba_propagator
+    d = W.cuba(N=4000, seed=11)
+    N, Ne, Ni = d["N"], d["Ne"], d["Ni"]
+    A3, C3 = cuba_propagator()
+    # mod relaxes to 1.5 with a 20 ms time constant and is reset to 0.5 on a
+    # spike: the weights must see its value before the reset
+    e = np.exp(-1e-4 / 0.02)
+    A = np.eye(4)
+    A[:3, :3] = A3
+    A[3, 3] = e
+    C = np.concatenate([C3, [1.5 * (1.0 - e)]])
+    rng = np.random.default_rng(12)
+    S0 = np.vstack([d["S0"], 0.5 + rng.random(N)])
+    steps = 2000
+    for kind in (0, 1):
+        dl = {}
+        if kind == 1:
+            dl = {k: rng.integers(0, 8, len(d[k][1])).astype(np.int32) for k in ("exc", "inh")}
+        h = NetHandle(1e-4, 64)
+        g = h.add_group(S0)
+        h.set_linear(g, A, C)
+        h.set_threshold(g, 0, d["Vt"])
+        h.add_reset_op(g, 0, 0, d["Vr"])
+        h.add_reset_op(g, 3, 0, 0.5)
+        net = O.Net(1e-4)
+        og = O.Group(N, S0, ("linear", A, C), ("const", 0, d["Vt"]),
+                     [(0, "=", d["Vr"], False), (3, "=", 0.5, False)], dt=1e-4)
+        gid = net.add_group(og)
+        for (ip, ix, w), off, n, var, key in ((d["exc"], 0, Ne, 1, "exc"), (d["inh"], Ne, Ni, 2, "inh")):
+            if kind == 0:
+                cid = h.add_connection(g, off, n, g, 0, N, var, ip, ix, w, delay_steps=0)
+                net.add_connection((gid, off, n), (gid, 0, N), var, ip, ix, w, delay_steps=0,
+                                   modulation=3)
+            else:
+                cid = h.add_connection(g, off, n, g, 0, N, var, ip, ix, w, syn_delay=dl[key],
+                                       max_delay_steps=8)
+                net.add_connection((gid, off, n), (gid, 0, N), var, ip, ix, w,
+                                   syn_delay_steps=dl[key], max_delay_steps=8, modulation=3)
+            h.set_modulation(cid, 3)
+        h.record(g, 1 << 20)
+        h.prepare()
+        h.run(steps)
+        net.run(steps)
+        s, i = h.get_spikes(g)
+        rs, ri = net.raster(gid)
+        assert len(rs) > 1000, kind
+        np.testing.assert_array_equal(s, rs)
+        np.testing.assert_array_equal(i, ri)
+        h.close()
