@@ -61,6 +61,8 @@ _SIGS = {
                                      C.POINTER(C.c_int)],
     "bm_net_connection_nnz": [P, C.c_int, Pi64],
     "bm_net_set_modulation": [P, C.c_int, C.c_int],
+    "bm_net_add_stp": [P, C.c_int, Pd, Pd, Pd, C.c_int],
+    "bm_net_get_stp": [P, C.c_int, C.c_int, Pd],
     "bm_net_connect_fixed_indegree": [P, C.c_int, i64, i64, i64, i64, C.c_int, i64, i64, C.c_int, C.c_int,
                                       dbl, u64, C.c_int, C.POINTER(C.c_int)],
     "bm_net_get_connection": [P, C.c_int, Pi64, Pi32, Pi32],
@@ -279,6 +281,22 @@ class NetHandle:
 
     def set_modulation(self, cid, src_var):
         call("bm_net_set_modulation", self.h, cid, int(src_var))
+
+    def add_stp(self, cid, U, taud, tauf, src_n):
+        """Short-term plasticity on connection ``cid``; scalars or arrays
+        over its ``src_n`` source neurons."""
+        per = not (np.isscalar(U) and np.isscalar(taud) and np.isscalar(tauf))
+        arr = [f64(np.atleast_1d(np.asarray(v, dtype=np.float64))) for v in (U, taud, tauf)]
+        if per:
+            if any(len(a) not in (1, src_n) for a in arr):
+                raise ValueError("STP parameter arrays must have one value per source neuron")
+            arr = [f64(np.broadcast_to(a, (src_n,))) for a in arr]
+        call("bm_net_add_stp", self.h, cid, dptr(arr[0]), dptr(arr[1]), dptr(arr[2]), 1 if per else 0)
+
+    def get_stp(self, cid, which, n):
+        out = np.empty(n, np.float64)
+        call("bm_net_get_stp", self.h, cid, {"u": 0, "x": 1, "ux": 2, "lastt": 3}[which], dptr(out))
+        return out
 
     def connect_random_blocks(self, src, tgt, var, src_off, src_n, tgt_off, tgt_n, p, weight, seed,
                               delay_steps=0, dmin=None, dmax=None, max_delay_steps=0):

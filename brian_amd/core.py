@@ -27,7 +27,7 @@ __all__ = [
     "Threshold", "VariableThreshold", "EmpiricalThreshold", "NoThreshold", "StringThreshold",
     "Reset", "VariableReset", "StringReset", "Refractoriness", "NoReset",
     "NeuronGroup", "SpikeGeneratorGroup", "PoissonGroup", "TimedArray",
-    "Connection", "DelayConnection", "STDP", "ExponentialSTDP",
+    "Connection", "DelayConnection", "STDP", "ExponentialSTDP", "STP",
     "SpikeMonitor", "PopulationRateMonitor", "PopulationSpikeCounter", "SpikeCounter",
     "StateMonitor", "Network", "MagicNetwork", "run", "reinit", "stop", "network_operation",
     "NetworkOperation", "set_global_preferences", "get_global_preference", "forget",
@@ -537,6 +537,7 @@ class Connection(_Tracked):
         # modulation (connection.py:190-193): index of the source variable
         # that scales the weights, read at the spike step
         self._mod = None
+        self._stp = None
         if modulation is not None:
             self._mod = source._var(modulation) if not isinstance(modulation, int) else modulation
         self._W = sp.lil_matrix((len(source), len(target)))
@@ -806,6 +807,47 @@ class ExponentialSTDP(_Tracked):
         return net.handle.get_stdp_traces(self.sid, n_pre, n_post)
 
 
+class _STPVars:
+    """The STPGroup's variables u, x, ux (brian/stp.py:30-43), one per
+    source neuron of the connection, read from the device after a run."""
+
+    def __init__(self, stp):
+        self._stp = stp
+
+    def _get(self, name):
+        s = self._stp
+        C = s.C
+        n = len(C.source)
+        if C._net is None or C.cid is None:
+            U = np.broadcast_to(np.asarray(s.U, dtype=np.float64), (n,)).copy()
+            return {"u": U, "x": np.ones(n), "ux": U.copy()}[name]
+        return C._net.handle.get_stp(C.cid, name, n)
+
+    u = property(lambda self: self._get("u"))
+    x = property(lambda self: self._get("x"))
+    ux = property(lambda self: self._get("ux"))
+
+
+class STP(_Tracked):
+    """Short-term plasticity (brian/stp.py:124-170), Tsodyks-Markram: per
+    source neuron, dx/dt = (1-x)/taud, du/dt = (U-u)/tauf between spikes; on
+    a spike the weights are scaled by u*x, then x <- x(1-u), u <- u + U(1-u).
+    Scalar or per-neuron taud, tauf, U (STPUpdater / STPUpdater2).  ``vars``
+    holds u, x, ux."""
+
+    def __init__(self, C, taud, tauf, U):
+        self._track(1)
+        if isinstance(C, DelayConnection):
+            raise AttributeError("STP does not handle heterogeneous connections yet.")
+        if C._mod is not None or C._stp is not None:
+            raise NotImplementedError("STP on a modulated connection is not supported")
+        self.C = C
+        conv = lambda v: float(v) if np.isscalar(v) else np.asarray(v, dtype=np.float64)
+        self.taud, self.tauf, self.U = conv(taud), conv(tauf), conv(U)
+        C._stp = self
+        self.vars = _STPVars(self)
+
+
 # ------------------------------------------------------------------ monitors
 class SpikeMonitor(_Tracked):
     """brian/monitor.py:84-218: ``spikes`` list of (i, t), ``nspikes``,
@@ -1055,6 +1097,9 @@ class Network:
             elif isinstance(o, ExponentialSTDP):
                 if o not in self.stdps:
                     self.stdps.append(o)
+            elif isinstance(o, STP):
+                if o.C not in self.conns:
+                    self.conns.append(o.C)
             elif isinstance(o, Connection):
                 if o not in self.conns:
                     self.conns.append(o)
@@ -1113,6 +1158,9 @@ class Network:
                                      W.indptr, W.indices, W.data, **kw)
             if c._mod is not None:
                 h.set_modulation(c.cid, c._mod)
+            if c._stp is not None:
+                st = c._stp
+                h.add_stp(c.cid, st.U, st.taud, st.tauf, len(c.source))
             c._net = self
         for s in self.stdps:
             dp, dm = s.decays(dt)

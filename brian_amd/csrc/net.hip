@@ -56,6 +56,7 @@ struct HostGroup {
   int64_t raster_region = 0;         // entries per append region on the device
   bool need_hist = false;
   int mod_var = -1;          // variable captured per spike for modulated outgoing connections
+  int stp_conn = -1;         // outgoing connection with short-term plasticity, -1 none
   int H = 0;
   // device
   DevBuf S, last, Apn_d, Cpn_d, code_d, consts_d, refr_d, ti_d, pthr_d, gen_idx_d, gen_off_d,
@@ -93,6 +94,10 @@ struct HostConn {
   int stdp = -1;
   int64_t max_indeg = 0;
   int mod_var = -1;          // source variable scaling the weights (modulation), -1 none
+  bool stp = false;          // weights scaled by the source neuron's u*x (short-term plasticity)
+  std::vector<double> stp_init;   // [4][src_n] u, x, ux, last update time
+  std::vector<double> stp_par;    // [3][src_n] U, -1/taud, -1/tauf
+  DevBuf stp_state_d, stp_par_d;
   DevBuf row_ptr_d, col_d, w_d, dly_d;
   int dly_bytes = 0;
   bool generated = false;   // CSR built on the device at prepare time
@@ -188,17 +193,23 @@ void require_unprepared(bm_net *net) {
 // (packed count) ring, no timed input.
 bool fast_eligible(const HostGroup &g) {
   if (g.kind != GK_NEURON || g.upd != UPD_LINEAR || g.M < 1 || g.M > 4) return false;
-  if (!g.mons.empty() || g.mod_var >= 0) return false;
+  if (!g.mons.empty() || g.mod_var >= 0 || g.stp_conn >= 0) return false;
   if (g.refr != REFR_NONE || g.th_kind == TH_EMPIRICAL || g.ti_nt > 0) return false;
   if (g.ring_desc.size() > 1) return false;
   if (g.ring_desc.size() == 1 && g.ring_desc[0].kind != RING_COUNT) return false;
   return true;
 }
 
+size_t sweep_shmem(bm_net *net, const HostGroup &g, int K, int W);
+
 int group_K(bm_net *net, const HostGroup &g) {
   if (fast_eligible(g)) return net->K;
   if (g.upd == UPD_PROGRAM) return 1;
   if (g.upd == UPD_PERNEURON && g.M > 3) return 1;
+  // a captured value per listed spike: four neurons per thread only while
+  // the largest window's list still fits the LDS
+  if ((g.mod_var >= 0 || g.stp_conn >= 0) && sweep_shmem(net, g, 4, std::max(net->W, net->Wcap)) > 65536)
+    return 1;
   return net->K >= 4 ? 4 : 1;
 }
 
@@ -208,7 +219,7 @@ int group_K(bm_net *net, const HostGroup &g) {
 size_t sweep_shmem(bm_net *net, const HostGroup &g, int K, int W) {
   const size_t cap = (size_t)BLOCK * K * std::max(W, 1);
   size_t shmem = (sizeof(uint16_t) * cap + 15) / 16 * 16 + 16;
-  if (g.mod_var >= 0) shmem += cap * net->tsize();
+  if (g.mod_var >= 0 || g.stp_conn >= 0) shmem += cap * net->tsize();
   if (g.upd == UPD_PROGRAM) shmem += (size_t)g.prog_nregs * BLOCK * net->tsize();
   return shmem;
 }
@@ -431,12 +442,23 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     std::vector<int64_t> indeg(c.tgt_n, 0);
     for (int64_t q = 0; q < c.nnz; ++q) indeg[c.col[q]]++;
     c.max_indeg = c.tgt_n ? *std::max_element(indeg.begin(), indeg.end()) : 0;
-    c.count_mode = uniform && c.stdp < 0 && c.max_indeg <= 65535 && c.mod_var < 0;
+    c.count_mode = uniform && c.stdp < 0 && c.max_indeg <= 65535 && c.mod_var < 0 && !c.stp;
     if (c.mod_var >= 0) {
       HostGroup &sg = *net->groups[c.src];
       BM_REQUIRE(sg.mod_var < 0 || sg.mod_var == c.mod_var,
                  "modulated connections of one source group must use the same variable");
+      BM_REQUIRE(sg.stp_conn < 0, "a group with short-term plasticity cannot also modulate by a variable");
       sg.mod_var = c.mod_var;
+    }
+    if (c.stp) {
+      HostGroup &sg = *net->groups[c.src];
+      BM_REQUIRE(sg.stp_conn < 0 && sg.mod_var < 0,
+                 "one connection with short-term plasticity (and no other modulation) per source group");
+      sg.stp_conn = (int)ci;
+      c.stp_state_d.alloc(c.stp_init.size() * sizeof(double));
+      BM_HIP(hipMemcpyAsync(c.stp_state_d.p, c.stp_init.data(), c.stp_init.size() * sizeof(double),
+                            hipMemcpyHostToDevice, net->stream));
+      upload_raw(c.stp_par_d, c.stp_par.data(), c.stp_par.size(), net->stream);
     }
     c.wval = c.nnz ? c.w[0] : 0.0;
     upload_raw(c.row_ptr_d, c.row_ptr.data(), c.row_ptr.size(), net->stream);
@@ -604,6 +626,16 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     d.wave_max = net->wave_max;
     d.row_split = net->row_split;
     d.mod_var = g.mod_var;
+    d.mcap = g.mod_var >= 0 || g.stp_conn >= 0;
+    d.stp = g.stp_conn >= 0;
+    if (d.stp) {
+      const HostConn &sc = *net->conns[g.stp_conn];
+      d.stp_off = sc.src_off;
+      d.stp_n = sc.src_n;
+      d.stp_state = sc.stp_state_d.as<double>();
+      d.stp_par = sc.stp_par_d.as<double>();
+      d.stp_delay = sc.hom_delay;
+    }
     d.nmon = (int)g.mons.size();
     for (int m = 0; m < d.nmon; ++m) {
       HostGroup::Mon &hm = *g.mons[m];
@@ -659,7 +691,7 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       o.ch = c.ch;
       o.D = tg.D;
       o.tgt_off = c.tgt_off;
-      o.mod = c.mod_var >= 0 ? 1 : 0;
+      o.mod = c.mod_var >= 0 || c.stp ? 1 : 0;
     }
     d.nout = nout;
     if (g.raster_cap > 0) {
@@ -1236,6 +1268,51 @@ int bm_net_set_modulation(bm_net *net, int cid, int src_var) {
   });
 }
 
+int bm_net_add_stp(bm_net *net, int cid, const double *U, const double *taud, const double *tauf,
+                   int per_neuron) {
+  return guard([&] {
+    require_unprepared(net);
+    BM_REQUIRE(cid >= 0 && cid < (int)net->conns.size(), "bad connection id");
+    HostConn &c = *net->conns[cid];
+    BM_REQUIRE(grp(net, c.src).kind == GK_NEURON, "short-term plasticity needs a neuron source group");
+    BM_REQUIRE(c.delay_kind == 0, "STP does not handle heterogeneous connections yet.");
+    BM_REQUIRE(!c.generated, "short-term plasticity on device-generated connections is not supported");
+    BM_REQUIRE(c.mod_var < 0 && !c.stp && c.stdp < 0,
+               "the connection already has modulation, short-term plasticity or STDP");
+    BM_REQUIRE(U && taud && tauf, "null parameter array");
+    const int64_t n = c.src_n;
+    c.stp_par.assign(3 * n, 0.0);
+    c.stp_init.assign(4 * n, 0.0);
+    for (int64_t r = 0; r < n; ++r) {
+      const int64_t q = per_neuron ? r : 0;
+      BM_REQUIRE(taud[q] > 0 && tauf[q] > 0, "STP time constants must be positive");
+      c.stp_par[r] = U[q];
+      c.stp_par[n + r] = -1.0 / taud[q];      // stp.py:53-54
+      c.stp_par[2 * n + r] = -1.0 / tauf[q];
+      c.stp_init[r] = U[q];                   // P.u = U, P.x = 1, P.ux = U, lastt = 0 (stp.py:157-159, :57)
+      c.stp_init[n + r] = 1.0;
+      c.stp_init[2 * n + r] = U[q];
+    }
+    c.stp = true;
+  });
+}
+
+int bm_net_get_stp(bm_net *net, int cid, int which, double *out) {
+  return guard([&] {
+    BM_REQUIRE(cid >= 0 && cid < (int)net->conns.size(), "bad connection id");
+    HostConn &c = *net->conns[cid];
+    BM_REQUIRE(c.stp, "connection has no short-term plasticity");
+    BM_REQUIRE(which >= 0 && which < 4, "which: 0 u, 1 x, 2 ux, 3 last update time");
+    const int64_t n = c.src_n;
+    if (!net->prepared) {
+      std::copy(c.stp_init.begin() + which * n, c.stp_init.begin() + (which + 1) * n, out);
+      return;
+    }
+    BM_HIP(hipStreamSynchronize(net->stream));
+    BM_HIP(hipMemcpy(out, c.stp_state_d.as<double>() + which * n, n * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
 int bm_net_connection_nnz(bm_net *net, int cid, int64_t *nnz) {
   return guard([&] {
     BM_REQUIRE(cid >= 0 && cid < (int)net->conns.size(), "bad connection id");
@@ -1279,6 +1356,7 @@ int bm_net_add_stdp(bm_net *net, int cid, double decay_pre, double decay_post, d
     BM_REQUIRE(c.delay_kind == 0, "STDP on DelayConnection is not supported");
     BM_REQUIRE(!c.generated, "STDP on device-generated connections is not supported");
     BM_REQUIRE(c.stdp < 0, "connection already has STDP");
+    BM_REQUIRE(!c.stp, "STDP on a connection with short-term plasticity is not supported");
     BM_REQUIRE(update == 0 || update == 1, "update must be 0 (additive) or 1 (multiplicative)");
     BM_REQUIRE(interactions >= 0 && interactions <= 3, "interactions out of range");
     auto s = std::make_unique<HostStdp>();

@@ -211,6 +211,31 @@ __device__ inline T apply_count(T x, uint32_t cnt, T w, int is_delay) {
   return x;
 }
 
+// Tsodyks-Markram short-term plasticity on a spike of row r at step n
+// (brian/stp.py:48-70 for scalar parameters, :78-90 per neuron; the same
+// arithmetic either way): u and x relax since the last update, the weights
+// are scaled by u*x, then x <- x(1-u), u <- u + U(1-u).  FP64 whatever the
+// network precision.
+__device__ inline double stp_spike(const GroupDev &g, int64_t r, int64_t n) {
+#pragma clang fp contract(off)
+  double *u = g.stp_state, *x = u + g.stp_n, *ux = x + g.stp_n, *lt = ux + g.stp_n;
+  const double *U = g.stp_par, *mtd = U + g.stp_n, *mtf = mtd + g.stp_n;
+  const double t = (double)(n + g.stp_delay) * g.dt + 0.0;   // Clock._t at the update
+  const double interval = t - lt[r];
+  const double Ur = U[r];
+  double uu = Ur + (u[r] - Ur) * exp(interval * mtf[r]);
+  const double tmp = 1.0 - uu;
+  double xx = 1.0 + (x[r] - 1.0) * exp(interval * mtd[r]);
+  const double v = uu * xx;
+  xx = xx * tmp;
+  uu = uu + Ur * tmp;
+  u[r] = uu;
+  x[r] = xx;
+  ux[r] = v;
+  lt[r] = t;
+  return v;
+}
+
 // Block-cooperative CSR propagation of the block's spike list into the
 // target rings.  Up to BLOCK spikes at a time: every thread loads the row
 // bounds of one spike (one memory round trip for all rows), a block scan
@@ -412,7 +437,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
   // listed spike (T), then the program registers
   const int list_cap = BLOCK * K * (W > 0 ? W : 1);
   T *sh_mval = reinterpret_cast<T *>(smem + ((sizeof(uint16_t) * list_cap + 15) / 16) * 16);
-  T *reg = GEN && g.mod_var >= 0 ? sh_mval + list_cap : sh_mval;
+  T *reg = GEN && g.mcap ? sh_mval + list_cap : sh_mval;
   const int nsteps = tail_only ? 1 : W;
   // ring slot of the tail at window offset w is (n0 - 1 + w) mod D
   int slot0 = (int)((n0 - 1) % D);
@@ -733,11 +758,16 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
           if constexpr (GEN) {
             // modulated connections read the source variable at the spike
             // step, after the update and before the reset (connection.py:231-247)
-            if (g.mod_var >= 0) {
+            if (g.mcap) {
               T mv = (T)0;
+              if (g.stp) {
+                const int64_t r = (int64_t)(blk0 + k * BLOCK + tid) - g.stp_off;
+                if (r >= 0 && r < g.stp_n) mv = (T)stp_spike(g, r, n);
+              } else {
 #pragma unroll
-              for (int j = 0; j < MM; ++j)
-                if (j == g.mod_var) mv = s[j][k];
+                for (int j = 0; j < MM; ++j)
+                  if (j == g.mod_var) mv = s[j][k];
+              }
               sh_mval[base + __popcll(below)] = mv;
             }
           }

@@ -130,6 +130,19 @@ struct GroupDev {
   int nmon;
   int mod_var;          // source variable captured per spike for modulated connections, -1 none
   MonDev mons[MAXMON];
+  // a value is captured per spike into LDS for modulated outgoing
+  // connections: the source variable mod_var or the short-term plasticity
+  // product u*x of the neuron (stp)
+  int mcap, stp;
+  // short-term plasticity (brian/stp.py:48-70, :124-170) of the neurons
+  // [stp_off, stp_off + stp_n): state [4][stp_n] u, x, ux, last update time
+  // (s), parameters [3][stp_n] U, -1/taud, -1/tauf; stp_delay = the
+  // connection's homogeneous delay (the updater sees a spike that many
+  // steps later)
+  int64_t stp_off, stp_n;
+  double *stp_state;
+  const double *stp_par;
+  int stp_delay, pad2;
 };
 // passed by value to every sweep launch; kernel arguments are limited to 4 KiB
 static_assert(sizeof(GroupDev) <= 3584, "GroupDev must leave room in the 4 KiB kernel arguments");

@@ -134,6 +134,24 @@ int bm_net_add_connection(bm_net *net, int src_gid, int64_t src_off, int64_t src
  * propagate at the spike step, as the reference does.  Before prepare. */
 int bm_net_set_modulation(bm_net *net, int cid, int src_var);
 
+/* Short-term plasticity (brian/stp.py:124-170, replaces STP(C, taud, tauf,
+ * U) and its STPUpdater / STPUpdater2, :39-90): per source neuron of the
+ * connection, u and x relax to U and 1 with time constants tauf and taud
+ * (seconds) between spikes; on a spike the weights are scaled by u*x, then
+ * x <- x(1-u), u <- u + U(1-u).  per_neuron 0: U, taud, tauf are scalars
+ * ([0] read); 1: arrays of src_n.  Plain connections only (homogeneous
+ * delay d: the update happens at spike step + d, as the reference's
+ * delayed updater does).  Initial u = U, x = 1, ux = U.  Before prepare. */
+int bm_net_add_stp(bm_net *net, int cid, const double *U, const double *taud, const double *tauf,
+                   int per_neuron);
+
+/* STP state of the connection's source neurons (the STPGroup's variables,
+ * stp.py:30-43): which 0 u, 1 x, 2 ux, 3 time of the last update; out[src_n].
+ * With a delay d the state already holds the updates of spikes of the last
+ * d steps (applied at the spike step with the time t = (step + d) * dt the
+ * reference's delayed updater uses when it sees them). */
+int bm_net_get_stp(bm_net *net, int cid, int which, double *out);
+
 /* Block-diagonal random connectivity built on the device (the role of
  * connect_random, brian/connections/construction.py:42-177, for many
  * independent networks): block b connects source rows [src_off[b],

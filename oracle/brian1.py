@@ -25,7 +25,9 @@ propagate, STDP pre then post updaters, monitors) -> after_connections
 Pinning (see tests/test_oracle_*.py): closed-form propagators, the tutorial-1c
 840-spike count, the exponential-current and EPSP known answers, delay arrival
 steps and truncation, reset/refractory clamp timing, threshold semantics,
-STDP sign and bounds, brute-force dense-matrix stepping on tiny networks.
+STDP sign and bounds, brute-force dense-matrix stepping on tiny networks,
+short-term plasticity against the Tsodyks-Markram closed forms (first and
+second spike, periodic steady state).
 """
 from __future__ import annotations
 
@@ -33,7 +35,7 @@ import numpy as np
 import scipy.linalg
 
 __all__ = [
-    "linear_propagator", "numeric_propagator", "Group", "SpikeSource", "Net",
+    "linear_propagator", "numeric_propagator", "Group", "SpikeSource", "Net", "STPState",
     "counter_uniform_u32", "poisson_threshold_u32",
 ]
 
@@ -353,6 +355,35 @@ def _sub(spikes, off, n):
     return s - off
 
 
+class STPState:
+    """Tsodyks-Markram short-term plasticity of a connection's source neurons
+    (stp.py:48-90 for STPUpdater / STPUpdater2; STP at :124-170 swaps the
+    connection's source for this state's group and moves the connection's
+    delay onto the updater).  ``update`` runs when the updater sees spikes,
+    i.e. ``delay`` steps after they happened, with the clock time then."""
+
+    def __init__(self, n, U, taud, tauf):
+        b = lambda v: np.broadcast_to(np.asarray(v, dtype=np.float64), (n,)).copy()
+        self.U = b(U)
+        self.mtd = -1.0 / b(taud)
+        self.mtf = -1.0 / b(tauf)
+        self.u = self.U.copy()          # P.u = U, P.x = 1, P.ux = U (stp.py:157-159)
+        self.x = np.ones(n)
+        self.ux = self.U.copy()
+        self.lastt = np.zeros(n)
+
+    def update(self, sp, t):
+        dtl = t - self.lastt[sp]
+        U = self.U[sp]
+        u = U + (self.u[sp] - U) * np.exp(dtl * self.mtf[sp])
+        x = 1.0 + (self.x[sp] - 1.0) * np.exp(dtl * self.mtd[sp])
+        rest = 1.0 - u
+        self.ux[sp] = u * x
+        self.x[sp] = x * rest
+        self.u[sp] = u + U * rest
+        self.lastt[sp] = t
+
+
 class Connection:
     """Connection / DelayConnection on a CSR matrix (connection.py:224-296,
     delayconnection.py:85-206).  Rows are source neurons (within the source
@@ -361,7 +392,7 @@ class Connection:
 
     def __init__(self, net, src, tgt, var, indptr, indices, data,
                  delay_steps=0, syn_delay_steps=None, max_delay_steps=None,
-                 modulation=None):
+                 modulation=None, stp=None):
         self.src, self.tgt, self.var = src, tgt, var
         self.indptr = np.asarray(indptr, dtype=np.int64)
         self.indices = np.asarray(indices, dtype=np.int64)
@@ -369,6 +400,9 @@ class Connection:
         self.delay = int(delay_steps)
         self.modulation = modulation
         self.is_delay = syn_delay_steps is not None
+        # stp = (U, taud, tauf): scalars or arrays over the source range
+        self.stp = STPState(src[2], *stp) if stp is not None else None
+        assert not (self.stp is not None and (self.is_delay or modulation is not None))
         if self.is_delay:
             self.dsteps = np.asarray(syn_delay_steps, dtype=np.int64)
             self.md = int(max_delay_steps) + 1       # delayconnection.py:94
@@ -398,6 +432,13 @@ class Connection:
             sv_pre = net.groups[g].S[self.modulation][off:off + n]
             lens = self.indptr[spikes + 1] - self.indptr[spikes]
             w = w * np.repeat(sv_pre[spikes], lens)
+        elif self.stp is not None:
+            # the updater (a SpikeMonitor with the connection's delay) runs
+            # first, then the connection propagates its spikes at modulation
+            # ux (stp.py:160-167)
+            self.stp.update(spikes, net.n * net.dt + 0.0)
+            lens = self.indptr[spikes + 1] - self.indptr[spikes]
+            w = w * np.repeat(self.stp.ux[spikes], lens)
         tg, toff, tn = self.tgt
         if self.is_delay:
             slot = (self.cdi + self.dsteps[p]) % self.md

@@ -35,3 +35,19 @@ def test_modulation_facade_checks():
     assert d._mod == 1
     with pytest.raises(NotImplementedError):
         b.Connection(H, G, 'V', weight=2, delay=1 * b.ms, modulation='mod')
+
+
+def test_stp_facade_checks():
+    """stp.py:148-149: STP refuses a DelayConnection; before a run its
+    variables hold u = U, x = 1, ux = U (stp.py:157-159)."""
+    import brian_amd as b
+    b.reinit()
+    H = b.NeuronGroup(4, 'V:1', reset=0, threshold=1)
+    G = b.NeuronGroup(3, 'V:1', reset=0, threshold=1)
+    c = b.Connection(H, G, 'V', weight=2)
+    s = b.STP(c, taud=200 * b.ms, tauf=20 * b.ms, U=0.2)
+    np.testing.assert_array_equal(s.vars.u, [0.2] * 4)
+    np.testing.assert_array_equal(s.vars.x, [1.0] * 4)
+    d = b.Connection(H, G, 'V', weight=2, delay=True)
+    with pytest.raises(AttributeError):
+        b.STP(d, taud=200 * b.ms, tauf=20 * b.ms, U=0.2)

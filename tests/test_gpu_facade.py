@@ -334,3 +334,89 @@ def test_modulated_connection_matches_oracle():
         np.testing.assert_array_equal(s, rs)
         np.testing.assert_array_equal(i, ri)
         h.close()
+
+
+@pytest.mark.parametrize("delay,per_neuron", [(0, False), (10, False), (3, True)])
+def test_stp_cuba_matches_oracle(delay, per_neuron):
+    """examples/plasticity/short_term_plasticity2.py: the 4000-neuron CUBA
+    network with STP(Ce, taud=200 ms, tauf=20 ms, U=0.2) on the excitatory
+    projection, FP64 against the oracle: raster and the u, x, ux state of
+    every excitatory neuron (stp.py:48-90).  A homogeneous delay moves the
+    update to the step the delayed updater sees the spike."""
+    from brian_amd import workloads as W
+    from brian_amd._lib import NetHandle
+    from oracle import brian1 as O
+    from tests.netbuild import cuba_propagator
+    d = W.cuba(N=4000, seed=21)
+    N, Ne, Ni = d["N"], d["Ne"], d["Ni"]
+    A, C = cuba_propagator()
+    rng = np.random.default_rng(22)
+    if per_neuron:
+        U = 0.1 + 0.3 * rng.random(Ne)
+        taud = 0.1 + 0.2 * rng.random(Ne)
+        tauf = 0.01 + 0.03 * rng.random(Ne)
+    else:
+        U, taud, tauf = 0.2, 0.2, 0.02
+    steps = 3000
+    h = NetHandle(1e-4, 64)
+    g = h.add_group(d["S0"])
+    h.set_linear(g, A, C)
+    h.set_threshold(g, 0, d["Vt"])
+    h.add_reset_op(g, 0, 0, d["Vr"])
+    net = O.Net(1e-4)
+    og = O.Group(N, d["S0"], ("linear", A, C), ("const", 0, d["Vt"]), [(0, "=", d["Vr"], False)], dt=1e-4)
+    gid = net.add_group(og)
+    ip, ix, w = d["exc"]
+    ce = h.add_connection(g, 0, Ne, g, 0, N, 1, ip, ix, w, delay_steps=delay)
+    h.add_stp(ce, U, taud, tauf, Ne)
+    oc = net.add_connection((gid, 0, Ne), (gid, 0, N), 1, ip, ix, w, delay_steps=delay,
+                            stp=(U, taud, tauf))
+    ip, ix, w = d["inh"]
+    h.add_connection(g, Ne, Ni, g, 0, N, 2, ip, ix, w, delay_steps=delay)
+    net.add_connection((gid, Ne, Ni), (gid, 0, N), 2, ip, ix, w, delay_steps=delay)
+    h.record(g, 1 << 20)
+    h.prepare()
+    h.run(steps)
+    net.run(steps)
+    s, i = h.get_spikes(g)
+    rs, ri = net.raster(gid)
+    assert len(rs) > 1000
+    np.testing.assert_array_equal(s, rs)
+    np.testing.assert_array_equal(i, ri)
+    # the device applies an update at the spike step (with the time the
+    # delayed updater would use); the reference applies it `delay` steps
+    # later, so its state catches up after `delay` more steps
+    net.run(delay)
+    for name in ("u", "x", "ux"):
+        np.testing.assert_allclose(h.get_stp(ce, name, Ne), getattr(oc.stp, name), rtol=1e-13, atol=1e-15)
+    np.testing.assert_array_equal(h.get_stp(ce, "lastt", Ne), oc.stp.lastt)
+    h.close()
+
+
+def test_stp_example_steady_state():
+    """examples/plasticity/short_term_plasticity.py: regular inputs at 5-30 Hz
+    through one-to-one facilitating synapses (taud=1 ms, tauf=100 ms,
+    U=0.1); after 3 s (at least 14 spikes per input) each synapse's u*x
+    equals the Tsodyks-Markram steady state for its inter-spike interval."""
+    from brian_amd import (NeuronGroup, Connection, SpikeMonitor, STP, run, ms, Hz, pA, Mohm,
+                           linspace, defaultclock)
+    tau_e, taum, A_SE, Rm, N = 3 * ms, 10 * ms, 250 * pA, 100 * Mohm, 10
+    inp = NeuronGroup(N, model="dx/dt=rate : 1\nrate : Hz", threshold=1., reset=0)
+    inp.rate = linspace(5 * Hz, 30 * Hz, N)
+    neuron = NeuronGroup(N, model="dv/dt=(Rm*i-v)/taum:volt\ndi/dt=-i/tau_e:amp")
+    C = Connection(inp, neuron, "i")
+    C.connect_one_to_one(weight=A_SE)
+    stp = STP(C, taud=1 * ms, tauf=100 * ms, U=.1)
+    M = SpikeMonitor(inp)
+    run(3000 * ms)
+    ux = stp.vars.ux
+    U = 0.1
+    for j in range(N):
+        t = np.array(M[j])
+        isi = np.diff(t)
+        assert len(t) >= 14 and np.ptp(isi) < 1.5 * float(defaultclock.dt), (j, isi)
+        T = isi[-1]
+        Ef, Ed = np.exp(-T / 0.1), np.exp(-T / 1e-3)
+        us = U / (1 - (1 - U) * Ef)
+        xs = (1 - Ed) / (1 - (1 - us) * Ed)
+        assert abs(ux[j] - us * xs) < 1e-6 * us * xs, (j, ux[j], us * xs)

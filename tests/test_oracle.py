@@ -442,3 +442,50 @@ def test_integration_scheme_orders():
     e1 = np.abs(run2(1e-2) - exact2).max()
     e2 = np.abs(run2(5e-3) - exact2).max()
     assert abs(np.log2(e1 / e2) - 1) < 0.15, (e1, e2)
+
+
+def _stp_run(steps_by_neuron, nsteps, U, taud, tauf, delay=0):
+    """Spike sources -> one-to-one Connection (weight 1) with STP onto a
+    parameter-only target: the target accumulates u*x at every spike."""
+    n = len(steps_by_neuron)
+    lists = [[] for _ in range(nsteps)]
+    for i, st in enumerate(steps_by_neuron):
+        for k in st:
+            lists[k].append(i)
+    src = O.SpikeSource(n, step_lists=[np.array(l, dtype=np.int64) for l in lists])
+    tgt = O.Group(n, np.zeros((1, n)), None, None, None)
+    net = O.Net(DT)
+    gs = net.add_group(src)
+    gt = net.add_group(tgt)
+    c = net.add_connection((gs, 0, n), (gt, 0, n), 0, np.arange(n + 1), np.arange(n), np.ones(n),
+                           delay_steps=delay, stp=(U, taud, tauf))
+    net.run(nsteps)
+    return c.stp, tgt.S[0]
+
+
+def test_stp_closed_forms():
+    """Tsodyks-Markram (stp.py:1-18, :48-70): the first spike transmits U;
+    a second spike k steps later transmits (U + U(1-U)Ef)(1 - U Ed) with
+    Ef = exp(-k dt/tauf), Ed = exp(-k dt/taud); periodic spikes converge to
+    u* = U/(1-(1-U)Ef), x* = (1-Ed)/(1-(1-u*)Ed).  Swapping the order of the
+    x and u updates, or relaxing with the updated u, fails these."""
+    U, taud, tauf = 0.2, 0.2, 0.02
+    for k in (1, 37, 500):
+        st, acc = _stp_run([[3, 3 + k]], 3 + k + 2, U, taud, tauf)
+        Ef, Ed = np.exp(-k * DT / tauf), np.exp(-k * DT / taud)
+        second = (U + U * (1 - U) * Ef) * (1 - U * Ed)
+        assert abs(acc[0] - (U + second)) < 1e-15
+        assert abs(st.ux[0] - second) < 1e-15
+    # per-neuron parameters (STPUpdater2): three neurons, periods 50/120/300 steps
+    Us, tds, tfs = np.array([0.1, 0.5, 0.8]), np.array([0.1, 0.3, 0.05]), np.array([0.2, 0.01, 0.5])
+    periods = [50, 120, 300]
+    nsteps = 60 * 300
+    st, _ = _stp_run([list(range(0, nsteps, p)) for p in periods], nsteps, Us, tds, tfs)
+    for i, p in enumerate(periods):
+        Ef, Ed = np.exp(-p * DT / tfs[i]), np.exp(-p * DT / tds[i])
+        us = Us[i] / (1 - (1 - Us[i]) * Ef)
+        xs = (1 - Ed) / (1 - (1 - us) * Ed)
+        assert abs(st.ux[i] - us * xs) < 1e-12, (i, st.ux[i], us * xs)
+    # homogeneous delay: the updater runs `delay` steps after the spike
+    st, acc = _stp_run([[4]], 20, U, taud, tauf, delay=7)
+    assert st.lastt[0] == (4 + 7) * DT + 0.0 and acc[0] == U
