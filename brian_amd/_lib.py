@@ -62,6 +62,7 @@ _SIGS = {
     "bm_net_connection_nnz": [P, C.c_int, Pi64],
     "bm_net_set_modulation": [P, C.c_int, C.c_int],
     "bm_net_add_stp": [P, C.c_int, Pd, Pd, Pd, C.c_int],
+    "bm_net_set_stdp_delays": [P, C.c_int, Pd],
     "bm_net_get_stp": [P, C.c_int, C.c_int, Pd],
     "bm_net_connect_fixed_indegree": [P, C.c_int, i64, i64, i64, i64, C.c_int, i64, i64, C.c_int, C.c_int,
                                       dbl, u64, C.c_int, C.POINTER(C.c_int)],
@@ -334,10 +335,16 @@ class NetHandle:
         call("bm_net_get_connection", self.h, cid, i64ptr(rp), i32ptr(col), i32ptr(dl))
         return rp, col[:n.value], dl[:n.value]
 
-    def add_stdp(self, cid, decay_pre, decay_post, Ap, Am, wmin, wmax, update=0, interactions=0):
+    def add_stdp(self, cid, decay_pre, decay_post, Ap, Am, wmin, wmax, update=0, interactions=0,
+                 syn_delay_s=None):
+        """ExponentialSTDP on connection ``cid``; ``syn_delay_s`` (seconds,
+        CSR order) when the connection is a DelayConnection."""
         sid = C.c_int()
         call("bm_net_add_stdp", self.h, cid, decay_pre, decay_post, Ap, Am, wmin, wmax, update,
              interactions, C.byref(sid))
+        if syn_delay_s is not None:
+            d = f64(syn_delay_s)
+            call("bm_net_set_stdp_delays", self.h, sid.value, dptr(d))
         return sid.value
 
     def add_state_monitor(self, gid, var, indices=None, timestep=1, capacity=0):

@@ -774,8 +774,8 @@ class ExponentialSTDP(_Tracked):
         self._track(1)
         if wmax is None:
             raise AttributeError("You must specify the maximum synaptic weight")
-        if isinstance(C, DelayConnection):
-            raise NotImplementedError("STDP on DelayConnection is not supported")
+        if isinstance(C, DelayConnection) and (delay_pre is not None or delay_post is not None):
+            raise ValueError("Must use delay_pre=delay_post=None for the moment.")   # stdp.py:339-340
         if delay_pre is not None or delay_post is not None:
             raise NotImplementedError("explicit delay_pre/delay_post are not supported")
         wmax = float(wmax)
@@ -1164,7 +1164,9 @@ class Network:
             c._net = self
         for s in self.stdps:
             dp, dm = s.decays(dt)
-            s.sid = h.add_stdp(s.C.cid, dp, dm, s.Ap, s.Am, s.wmin, s.wmax, s.update, s.interactions)
+            dly = s.C._delays if isinstance(s.C, DelayConnection) else None
+            s.sid = h.add_stdp(s.C.cid, dp, dm, s.Ap, s.Am, s.wmin, s.wmax, s.update, s.interactions,
+                               syn_delay_s=dly)
             s._net = self
         self._capacity = {}
         for g in self.groups:

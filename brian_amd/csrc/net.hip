@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -108,7 +110,11 @@ struct HostStdp {
   int conn = 0;
   double decay_pre = 0, decay_post = 0, Ap = 0, Am = 0, wmin = 0, wmax = 0;
   int update = 0, interactions = 0;
-  DevBuf col_ptr_d, col_perm_d, col_row_d, a_pre_d, a_post_d;
+  // DelayConnection: per-synapse history offsets (CSR order), the delay of
+  // the weight part of pre updates, and the history depth
+  std::vector<int32_t> kpre, kpost;
+  int wdelay = 0, HP = 0, kbytes = 1;
+  DevBuf col_ptr_d, col_perm_d, col_row_d, a_pre_d, a_post_d, h_pre_d, h_post_d, kpre_d, kpost_d;
   StdpDev dev{};
 };
 
@@ -391,6 +397,28 @@ void build_stdp(bm_net *net, HostStdp &s) {
   std::vector<double> z1(c.src_n, 0.0), z2(c.tgt_n, 0.0);
   upload_T(net, s.a_pre_d, z1.data(), z1.size());
   upload_T(net, s.a_post_d, z2.data(), z2.size());
+  if (c.delay_kind == 1) {
+    BM_REQUIRE(!s.kpre.empty() || c.nnz == 0,
+               "STDP on a DelayConnection needs the synaptic delays in seconds (bm_net_set_stdp_delays)");
+    int kmax = 0;
+    for (int64_t q = 0; q < c.nnz; ++q) kmax = std::max(kmax, std::max(s.kpre[q], s.kpost[q]));
+    s.HP = kmax + 2;
+    const size_t hb = (size_t)s.HP * net->tsize();
+    s.h_pre_d.alloc(hb * std::max<int64_t>(c.src_n, 1));
+    s.h_post_d.alloc(hb * std::max<int64_t>(c.tgt_n, 1));
+    BM_HIP(hipMemsetAsync(s.h_pre_d.p, 0, s.h_pre_d.bytes, net->stream));
+    BM_HIP(hipMemsetAsync(s.h_post_d.p, 0, s.h_post_d.bytes, net->stream));
+    s.kbytes = kmax <= 255 ? 1 : 2;
+    if (s.kbytes == 1) {
+      std::vector<uint8_t> a(s.kpre.begin(), s.kpre.end()), b(s.kpost.begin(), s.kpost.end());
+      upload_raw(s.kpre_d, a.data(), a.size(), net->stream);
+      upload_raw(s.kpost_d, b.data(), b.size(), net->stream);
+    } else {
+      std::vector<uint16_t> a(s.kpre.begin(), s.kpre.end()), b(s.kpost.begin(), s.kpost.end());
+      upload_raw(s.kpre_d, a.data(), a.size(), net->stream);
+      upload_raw(s.kpost_d, b.data(), b.size(), net->stream);
+    }
+  }
 }
 
 void prepare(bm_net *net, int window, int K, int use_graph) {
@@ -562,7 +590,7 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     HostConn &c = *net->conns[sp->conn];
     net->groups[c.src]->need_hist = true;
     net->groups[c.tgt]->need_hist = true;
-    net->groups[c.src]->H = std::max(net->groups[c.src]->H, c.hom_delay + 2);
+    net->groups[c.src]->H = std::max(net->groups[c.src]->H, std::max(c.hom_delay, sp->wdelay) + 2);
     net->groups[c.tgt]->H = std::max(net->groups[c.tgt]->H, 2);
     build_stdp(net, *sp);
   }
@@ -671,6 +699,7 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     for (size_t ci = 0; ci < net->conns.size(); ++ci) {
       HostConn &c = *net->conns[ci];
       if (c.src != (int)gi) continue;
+      if (c.stdp >= 0 && c.delay_kind == 0 && c.hom_delay > 0) continue;   // propagated by k_stdp_pre
       BM_REQUIRE(nout < MAXOUT, "too many outgoing connections for one group");
       HostGroup &tg = *net->groups[c.tgt];
       OutDev &o = d.outs[nout++];
@@ -779,6 +808,27 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     d.update = s.update;
     d.interactions = s.interactions;
     d.pre_delay = c.hom_delay;
+    // a Connection with a homogeneous delay reads its weights when it
+    // propagates, `delay` steps after the spike (connection.py:296), just
+    // before the pre updater of the same spikes changes them: the STDP pre
+    // kernel delivers those rows into the target ring slot of the step
+    d.prop = c.delay_kind == 0 && c.hom_delay > 0 ? 1 : 0;
+    if (d.prop) {
+      HostGroup &tgp = *net->groups[c.tgt];
+      const RingDev &rd = tgp.ring_desc[c.ring];
+      BM_REQUIRE(rd.kind == RING_FLOAT, "STDP connections deliver through a float ring");
+      d.ring = rd.base;
+      d.ring_slot_bytes = rd.slot_bytes;
+      d.ring_D = tgp.D;
+    }
+    d.dmode = c.delay_kind == 1 ? 1 : 0;
+    d.wdelay = s.wdelay;
+    d.HP = s.HP;
+    d.kbytes = s.kbytes;
+    d.h_pre = s.h_pre_d.p;
+    d.h_post = s.h_post_d.p;
+    d.kpre = s.kpre_d.p;
+    d.kpost = s.kpost_d.p;
     HostGroup &sg = *net->groups[c.src];
     HostGroup &tg = *net->groups[c.tgt];
     d.pre_hist = sg.hist_d.as<int32_t>();
@@ -1313,6 +1363,42 @@ int bm_net_get_stp(bm_net *net, int cid, int which, double *out) {
   });
 }
 
+int bm_net_set_stdp_delays(bm_net *net, int sid, const double *syn_delay_s) {
+  return guard([&] {
+    require_unprepared(net);
+    BM_REQUIRE(sid >= 0 && sid < (int)net->stdps.size(), "bad STDP id");
+    HostStdp &s = *net->stdps[sid];
+    HostConn &c = *net->conns[s.conn];
+    BM_REQUIRE(c.delay_kind == 1, "synaptic delays apply to STDP on a DelayConnection");
+    BM_REQUIRE(syn_delay_s || c.nnz == 0, "null delay array");
+    // the reference's expressions, evaluated in double as it does:
+    // max_delay = C._max_delay * dt with C._max_delay = int(max_delay/dt)+1
+    // (stdp.py:337, delayconnection.py:94); the pre weight updater is a
+    // SpikeMonitor with that delay, int(max_delay/dt) bins (connection.py:200);
+    // the delay expressions are 'max_delay-d' with max_delay printed by
+    // Python 2 str() (12 significant digits) and 'd' (stdp.py:73-75,
+    // :334-335); the history index is int(d/dt) through 1.0/dt
+    // (monitor.py:1041, :1092)
+    const double dt = net->dt;
+    const double max_delay = (double)(c.max_delay + 1) * dt;
+    s.wdelay = (int)(max_delay / dt);
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.12g", max_delay);
+    const double lit = std::strtod(buf, nullptr);
+    const double invdt = 1.0 / dt;
+    s.kpre.assign(c.nnz, 0);
+    s.kpost.assign(c.nnz, 0);
+    for (int64_t q = 0; q < c.nnz; ++q) {
+      const double d = syn_delay_s[q];
+      BM_REQUIRE(d >= 0, "synaptic delays must be non-negative");
+      s.kpre[q] = (int32_t)(invdt * (lit - d));
+      s.kpost[q] = (int32_t)(invdt * d);
+      BM_REQUIRE(s.kpre[q] >= 0 && s.kpost[q] >= 0 && s.kpre[q] < 65534 && s.kpost[q] < 65534,
+                 "delay offsets out of range");
+    }
+  });
+}
+
 int bm_net_connection_nnz(bm_net *net, int cid, int64_t *nnz) {
   return guard([&] {
     BM_REQUIRE(cid >= 0 && cid < (int)net->conns.size(), "bad connection id");
@@ -1353,7 +1439,6 @@ int bm_net_add_stdp(bm_net *net, int cid, double decay_pre, double decay_post, d
     require_unprepared(net);
     BM_REQUIRE(cid >= 0 && cid < (int)net->conns.size(), "bad connection id");
     HostConn &c = *net->conns[cid];
-    BM_REQUIRE(c.delay_kind == 0, "STDP on DelayConnection is not supported");
     BM_REQUIRE(!c.generated, "STDP on device-generated connections is not supported");
     BM_REQUIRE(c.stdp < 0, "connection already has STDP");
     BM_REQUIRE(!c.stp, "STDP on a connection with short-term plasticity is not supported");

@@ -895,17 +895,45 @@ __global__ __launch_bounds__(BLOCK) void k_generator(const GroupDev g, const int
 // Trace decay: the 1x1 LinearStateUpdater of the STDP trace groups
 // (brian/stdp.py:282-287): A = decay*A + 0.
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_stdp_decay(const StdpDev sd) {
+__global__ __launch_bounds__(BLOCK) void k_stdp_decay(const StdpDev sd, const int64_t *d_base, int64_t rel0) {
 #pragma clang fp contract(off)
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  // DelayConnection: the traces at the end of the previous step go to the
+  // recent-state history first (RecentStateMonitor at 'end', monitor.py:1044-1061)
+  int64_t hrow = -1;
+  if (sd.HP > 0) {
+    hrow = (*d_base + rel0 - 1) % sd.HP;
+    if (hrow < 0) hrow += sd.HP;
+  }
   if (i < sd.src_n) {
     T *a = static_cast<T *>(sd.a_pre);
+    if (hrow >= 0) static_cast<T *>(sd.h_pre)[hrow * sd.src_n + i] = a[i];
     a[i] = (T)sd.decay_pre * a[i] + (T)0;
   } else if (i < sd.src_n + sd.tgt_n) {
     T *a = static_cast<T *>(sd.a_post);
     const int64_t j = i - sd.src_n;
+    if (hrow >= 0) static_cast<T *>(sd.h_post)[hrow * sd.tgt_n + j] = a[j];
     a[j] = (T)sd.decay_post * a[j] + (T)0;
   }
+}
+
+// History row holding the traces at the end of step (m - 1 - k).
+__device__ inline int64_t stdp_hist_row(const StdpDev &sd, int64_t m, int64_t q, const void *k) {
+  const int kk = sd.kbytes == 1 ? static_cast<const uint8_t *>(k)[q] : static_cast<const uint16_t *>(k)[q];
+  int64_t r = (m - 1 - kk) % sd.HP;
+  return r < 0 ? r + sd.HP : r;
+}
+
+// One weight update of ExponentialSTDP (stdp.py:507-523) with the other
+// side's trace a, then the clip; `other` is the sign-carrying amplitude that
+// selects the multiplicative form.
+template <typename T>
+__device__ inline T stdp_weight(const StdpDev &sd, T x, T a, double other) {
+#pragma clang fp contract(off)
+  if (sd.update == 0) x = x + a;
+  else if (other < 0) x = x * ((T)1 + a);
+  else x = x + ((T)sd.wmax - x) * a;
+  return x < (T)sd.wmin ? (T)sd.wmin : (x > (T)sd.wmax ? (T)sd.wmax : x);
 }
 
 // Presynaptic updater (brian/stdp.py:371-393): for each pre spike i,
@@ -913,7 +941,41 @@ __global__ __launch_bounds__(BLOCK) void k_stdp_decay(const StdpDev sd) {
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_stdp_pre(const StdpDev sd, const int64_t *d_base, int64_t rel0) {
 #pragma clang fp contract(off)
-  const int64_t n = *d_base + rel0 - sd.pre_delay;
+  const int64_t m = *d_base + rel0;
+  if (sd.dmode == 1) {
+    const int wave = (blockIdx.x * BLOCK + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    const int nwaves = gridDim.x * (BLOCK / 64);
+    T *w = static_cast<T *>(sd.w);
+    T *apre = static_cast<T *>(sd.a_pre);
+    const T *hpost = static_cast<const T *>(sd.h_post);
+    // trace lines at the spike (immediate updater, delay 0)
+    {
+      const int64_t hs = m % sd.pre_H;
+      const unsigned int cnt = sd.pre_hist_count[hs];
+      for (unsigned int p = wave * 64 + lane; p < cnt; p += nwaves * 64) {
+        const int64_t i = (int64_t)sd.pre_hist[hs * sd.pre_N + p] - sd.src_off;
+        if (i < 0 || i >= sd.src_n) continue;
+        if (sd.interactions == 0 || sd.interactions == 3) apre[i] = apre[i] + (T)sd.Ap;
+        else apre[i] = (T)sd.Ap;
+      }
+    }
+    // weight lines of the spikes of step m - wdelay (DelayedSTDPUpdater, stdp.py:77-87)
+    const int64_t n = m - sd.wdelay;
+    if (n < 0) return;
+    const int64_t hs = n % sd.pre_H;
+    const unsigned int cnt = sd.pre_hist_count[hs];
+    for (unsigned int p = wave; p < cnt; p += nwaves) {
+      const int64_t i = (int64_t)sd.pre_hist[hs * sd.pre_N + p] - sd.src_off;
+      if (i < 0 || i >= sd.src_n) continue;
+      const int64_t lo = sd.row_ptr[i], hi = sd.row_ptr[i + 1];
+      for (int64_t q = lo + lane; q < hi; q += 64) {
+        const int64_t r = stdp_hist_row(sd, m, q, sd.kpre);
+        w[q] = stdp_weight<T>(sd, w[q], hpost[r * sd.tgt_n + sd.col[q]], sd.Am);
+      }
+    }
+    return;
+  }
+  const int64_t n = m - sd.pre_delay;
   if (n < 0) return;
   const int64_t hs = n % sd.pre_H;
   const unsigned int cnt = sd.pre_hist_count[hs];
@@ -930,8 +992,12 @@ __global__ __launch_bounds__(BLOCK) void k_stdp_pre(const StdpDev sd, const int6
       else apre[i] = (T)sd.Ap;
     }
     const int64_t lo = sd.row_ptr[i], hi = sd.row_ptr[i + 1];
+    T *ring = sd.prop ? reinterpret_cast<T *>(static_cast<unsigned char *>(sd.ring) +
+                                              (m % sd.ring_D) * sd.ring_slot_bytes)
+                      : nullptr;
     for (int64_t q = lo + lane; q < hi; q += 64) {
       T x = w[q];
+      if (sd.prop) unsafeAtomicAdd(ring + sd.tgt_off + sd.col[q], x);   // connection.py:240-241
       const T ap = apost[sd.col[q]];
       if (sd.update == 0) x = x + ap;
       else if (sd.Am < 0) x = x * ((T)1 + ap);
@@ -960,10 +1026,21 @@ __global__ __launch_bounds__(BLOCK) void k_stdp_post(const StdpDev sd, const int
     const int64_t j = (int64_t)sd.post_hist[hs * sd.post_N + p] - sd.tgt_off;
     if (j < 0 || j >= sd.tgt_n) continue;
     if (lane == 0) {
-      if (sd.interactions == 0 || sd.interactions == 2) apost[j] = apost[j] + (T)sd.Am;
+      // 'nearest_pre' writes A_post=+Am, an assignment (stdp.py:500-502)
+      if (sd.interactions == 0) apost[j] = apost[j] + (T)sd.Am;
       else apost[j] = (T)sd.Am;
     }
     const int64_t lo = sd.col_ptr[j], hi = sd.col_ptr[j + 1];
+    if (sd.dmode == 1) {
+      // pre trace as recorded kpost steps back (stdp.py:77-87)
+      const T *hpre = static_cast<const T *>(sd.h_pre);
+      for (int64_t q = lo + lane; q < hi; q += 64) {
+        const int64_t pw = sd.col_perm[q];
+        const int64_t r = stdp_hist_row(sd, n, pw, sd.kpost);
+        w[pw] = stdp_weight<T>(sd, w[pw], hpre[r * sd.src_n + sd.col_row[q]], sd.Ap);
+      }
+      continue;
+    }
     for (int64_t q = lo + lane; q < hi; q += 64) {
       const int64_t pw = sd.col_perm[q];
       T x = w[pw];

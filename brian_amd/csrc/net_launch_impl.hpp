@@ -67,7 +67,7 @@ template <typename T>
 void launch_stdp(const StdpDev &sd, hipStream_t st, const int64_t *d_base, int64_t rel0) {
   const int64_t tot = sd.src_n + sd.tgt_n;
   const unsigned b = (unsigned)((tot + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL(k_stdp_decay<T>, dim3(b), dim3(BLOCK), 0, st, sd);
+  hipLaunchKernelGGL(k_stdp_decay<T>, dim3(b), dim3(BLOCK), 0, st, sd, d_base, rel0);
   hipLaunchKernelGGL(k_stdp_pre<T>, dim3(128), dim3(BLOCK), 0, st, sd, d_base, rel0);
   hipLaunchKernelGGL(k_stdp_post<T>, dim3(128), dim3(BLOCK), 0, st, sd, d_base, rel0);
 }

@@ -166,6 +166,22 @@ struct StdpDev {
   const int32_t *post_hist;
   const unsigned int *post_hist_count;
   int64_t pre_N, post_N;      // group sizes (history row stride)
+  // DelayConnection (brian/stdp.py:296-368): dmode 1.  Trace lines run at
+  // the spike; the weight lines of a pre spike run wdelay steps later and
+  // read A_post as it stood at the end of step (now - 1 - kpre[q]), those of
+  // a post spike run at once and read A_pre at the end of step
+  // (now - 1 - kpost[q]).  h_pre / h_post hold the traces at the end of the
+  // last HP steps ([HP][src_n], [HP][tgt_n]); k arrays are per synapse in
+  // CSR order, u8 or u16 (kbytes).
+  int dmode, wdelay, HP, kbytes;
+  void *h_pre, *h_post;
+  const void *kpre, *kpost;
+  // homogeneous delay > 0: rows of the spikes of step now - pre_delay are
+  // delivered here (float ring of the target group, slot now mod ring_D)
+  // with the weights as they stand before this step's pre update
+  int prop, ring_D;
+  void *ring;
+  int64_t ring_slot_bytes;
 };
 
 }  // namespace bm

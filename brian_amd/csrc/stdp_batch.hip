@@ -153,7 +153,8 @@ __global__ __launch_bounds__(64) void k_stdp_batch(const StdpBatchDev d, int64_t
     }
     // ---- post updater
     if (post) {
-      apost = (d.interactions == 0 || d.interactions == 2) ? apost + Am : Am;
+      // 'nearest_pre' writes A_post=+Am, an assignment (stdp.py:500-502)
+      apost = d.interactions == 0 ? apost + Am : Am;
 #pragma unroll
       for (int j = 0; j < SB_NJ; ++j) {
         const int i = j * 64 + lane;

@@ -183,13 +183,26 @@ int bm_net_connection_nnz(bm_net *net, int cid, int64_t *nnz);
  * to its target range (nullable), delay_steps [nnz] (nullable). */
 int bm_net_get_connection(bm_net *net, int cid, int64_t *row_ptr, int32_t *col, int32_t *delay_steps);
 
-/* ExponentialSTDP on a delay_kind 0 connection (brian/stdp.py:452-539):
+/* ExponentialSTDP (brian/stdp.py:452-539):
  * decay_pre/decay_post are the 1x1 propagators exp(-dt/tau) (FP64, host);
  * Ap, Am already scaled as in stdp.py:509-513; update 0 additive,
- * 1 multiplicative; interactions 0 all, 1 nearest, 2 nearest_pre, 3 nearest_post. */
+ * 1 multiplicative; interactions 0 all, 1 nearest, 2 nearest_pre (whose
+ * post line is the reference's `A_post=+Am`, an assignment), 3 nearest_post.
+ * On a DelayConnection (delay_kind 1) also call bm_net_set_stdp_delays. */
 int bm_net_add_stdp(bm_net *net, int cid, double decay_pre, double decay_post,
                     double Ap, double Am, double wmin, double wmax, int update,
                     int interactions, int *sid);
+
+/* STDP on a DelayConnection (brian/stdp.py:296-368 with DelayedSTDPUpdater
+ * :56-87 and RecentStateMonitor, monitor.py:992-1096): syn_delay_s[nnz] are
+ * the synaptic delays in seconds in the connection's CSR order (the
+ * delayvec values).  Trace increments happen at the spike; the weight
+ * update of a pre spike happens int(max_delay/dt) steps later with the post
+ * trace as it stood at the end of step (now - 1 - int((max_delay - d)/dt)),
+ * that of a post spike at once with the pre trace at the end of step
+ * (now - 1 - int(d/dt)), max_delay = (max_delay_steps + 1) * dt, each offset
+ * evaluated in double as the reference does.  Before prepare. */
+int bm_net_set_stdp_delays(bm_net *net, int sid, const double *syn_delay_s);
 
 /* SpikeMonitor on a group: record (step, index) into a device raster of
  * capacity entries (brian/monitor.py:156-166). */

@@ -457,18 +457,28 @@ class Connection:
 
 
 class STDP:
-    """ExponentialSTDP on a (non-delay) Connection (stdp.py:181-424, :452-539).
+    """ExponentialSTDP (stdp.py:181-424, :452-539) on a Connection or, with
+    ``delays_s``, on a DelayConnection.
 
     Traces A_pre (per source) and A_post (per target) decay with the 1x1 exact
     propagator in the groups slot; on pre spikes (delay_pre = connection delay)
     ``A_pre[i] += Ap; w[i,:] += A_post; clip``; on post spikes (delay 0)
     ``A_post[j] += Am; w[:,j] += A_pre; clip`` (stdp.py:371-393).  Pre before
     post within a step.  update: 'additive' | 'multiplicative';
-    interactions: 'all' | 'nearest' | 'nearest_pre' | 'nearest_post'.
+    interactions: 'all' | 'nearest' | 'nearest_pre' | 'nearest_post'
+    (stdp.py:495-506; 'nearest_pre' writes ``A_post=+Am``, an assignment).
+
+    DelayConnection (stdp.py:296-368, DelayedSTDPUpdater :56-87,
+    RecentStateMonitor monitor.py:992-1096): the trace lines run at the
+    spike ('immediate'); the weight lines of a pre spike run max_delay
+    later and read A_post as recorded at the end of a past step, those of a
+    post spike run at once and read A_pre the same way.  The offsets are
+    computed from the per-synapse delays in seconds exactly as the
+    reference's expressions evaluate them in double precision.
     """
 
     def __init__(self, net, conn, taup, taum, Ap, Am, wmin, wmax,
-                 interactions="all", update="additive"):
+                 interactions="all", update="additive", delays_s=None):
         self.conn = conn
         dt = net.dt
         self.decay_pre, _ = linear_propagator(lambda x: -x / taup, 1, dt)
@@ -490,6 +500,45 @@ class STDP:
         self.col_ptr = np.concatenate(([0], np.cumsum(np.bincount(conn.indices, minlength=conn.tgt[2]))))
         rows = np.repeat(np.arange(conn.src[2]), np.diff(conn.indptr))
         self.col_rows = rows[order]
+        self.delayed = conn.is_delay
+        if self.delayed:
+            d = np.asarray(delays_s, dtype=np.float64)
+            max_delay = conn.md * dt                       # C._max_delay * dt (stdp.py:337)
+            self.P = int(max_delay / dt)                   # SpikeMonitor delay in bins (connection.py:200)
+            lit = float("%.12g" % max_delay)               # Python 2 str(float) in the lambda (stdp.py:73-75)
+            invdt = 1.0 / dt                               # RecentStateMonitor._invtargetdt (monitor.py:1041)
+            self.kpre = np.array(invdt * (lit - d), dtype=int)    # delay_expr 'max_delay-d' (stdp.py:334)
+            self.kpost = np.array(invdt * d, dtype=int)           # delay_expr 'd' (stdp.py:335)
+            duration = (conn.md + 1) * dt                  # stdp.py:359-360
+            self.nd = int(duration / (1 * dt)) + 1         # monitor.py:1028
+            self.h_pre = np.zeros((self.nd, conn.src[2]))
+            self.h_post = np.zeros((self.nd, conn.tgt[2]))
+            self.cur = 0
+
+    def end(self):
+        """RecentStateMonitor.__call__ at 'end' (monitor.py:1044-1061)."""
+        if self.delayed:
+            self.h_pre[self.cur] = self.A_pre
+            self.h_post[self.cur] = self.A_post
+            self.cur = (self.cur + 1) % self.nd
+
+    def _apply(self, data, p, a, sign_other):
+        if self.update == "additive":
+            data[p] += a
+        elif sign_other < 0:
+            data[p] *= (1 + a)
+        else:
+            data[p] += (self.wmax - data[p]) * a
+        data[p] = np.clip(data[p], self.wmin, self.wmax)
+
+    def _pre_delayed(self, net):
+        c = self.conn
+        g, off, n = c.src
+        spikes = _sub(net.groups[g].get_spikes(self.P), off, n)
+        for i in spikes:
+            lo, hi = c.indptr[i], c.indptr[i + 1]
+            rows = (self.cur - 1 - self.kpre[lo:hi]) % self.nd    # get_past_values_sequence
+            self._apply(c.data, np.arange(lo, hi), self.h_post[rows, c.indices[lo:hi]], self.Am)
 
     def decay(self):
         # LinearStateUpdater on 1x1: S = A*S; S += C with C = 0
@@ -499,12 +548,14 @@ class STDP:
     def pre(self, net):
         c = self.conn
         g, off, n = c.src
-        spikes = _sub(net.groups[g].get_spikes(c.delay), off, n)
+        spikes = _sub(net.groups[g].get_spikes(0 if self.delayed else c.delay), off, n)
         for i in spikes:
             if self.interactions in ("all", "nearest_post"):
                 self.A_pre[i] += self.Ap
             else:
                 self.A_pre[i] = self.Ap
+            if self.delayed:
+                continue
             lo, hi = c.indptr[i], c.indptr[i + 1]
             cols = c.indices[lo:hi]
             if self.update == "additive":
@@ -515,13 +566,24 @@ class STDP:
                 else:
                     c.data[lo:hi] += (self.wmax - c.data[lo:hi]) * self.A_post[cols]
             c.data[lo:hi] = np.clip(c.data[lo:hi], self.wmin, self.wmax)
+        if self.delayed:
+            self._pre_delayed(net)
 
     def post(self, net):
         c = self.conn
         g, off, n = c.tgt
         spikes = _sub(net.groups[g].get_spikes(0), off, n)
+        if self.delayed:
+            for j in spikes:
+                self.A_post[j] = self.A_post[j] + self.Am if self.interactions == "all" else self.Am
+            for j in spikes:
+                lo, hi = self.col_ptr[j], self.col_ptr[j + 1]
+                p = self.col_perm[lo:hi]
+                rows = (self.cur - 1 - self.kpost[p]) % self.nd
+                self._apply(c.data, p, self.h_pre[rows, self.col_rows[lo:hi]], self.Ap)
+            return
         for j in spikes:
-            if self.interactions in ("all", "nearest_pre"):
+            if self.interactions == "all":
                 self.A_post[j] += self.Am
             else:
                 self.A_post[j] = self.Am
@@ -586,6 +648,8 @@ class Net:
                 c.flush(self)
         for g in self.groups:
             g.do_reset(t)
+        for s in self.stdps:
+            s.end()
         self.n += 1
 
     def run(self, nsteps):

@@ -120,6 +120,35 @@ def test_stdp_potentiation_sign():
     assert con.W[0, 0] > 1
 
 
+def test_stdp_delayconnection_pairing():
+    """ExponentialSTDP on a DelayConnection through the Brian API
+    (stdp.py:296-368): a pre spike at step n with delay d and a post spike
+    at step p change the weight by Ap*wmax*exp(-(p-1-q-n) dt/taup) with
+    q = int(d/dt) (the trace lookups of the delayed updaters, as in the
+    oracle's closed forms).  d = 0.3 ms is 2.9999999999999996 steps in
+    double, so q = 2, as the reference truncates it (delayconnection.py:152,
+    monitor.py:1092).  The post neuron is driven by a separate teacher."""
+    from brian_amd import (NeuronGroup, SpikeGeneratorGroup, Connection, DelayConnection,
+                           ExponentialSTDP, SpikeMonitor, Network, ms)
+    pre = SpikeGeneratorGroup(1, [(0, 1 * ms)])
+    teach = SpikeGeneratorGroup(1, [(0, 2 * ms)])
+    G = NeuronGroup(1, "v:1\nx:1", threshold=1, reset=0)
+    drive = Connection(teach, G, "v", weight=2)
+    con = DelayConnection(pre, G, "x", max_delay=5 * ms)
+    con.connect_full(pre, G, weight=0.5, delay=0.3 * ms)
+    stdp = ExponentialSTDP(con, 20 * ms, 20 * ms, 0.01, -0.0105, wmax=1.0)
+    Mpre, Mpost = SpikeMonitor(pre), SpikeMonitor(G)
+    net = Network(pre, teach, G, drive, con, stdp, Mpre, Mpost)
+    net.run(10 * ms)
+    dt = 1e-4
+    n = int(round(Mpre.spikes[0][1] / dt))
+    p = int(round(Mpost.spikes[0][1] / dt))
+    assert p - n > 4
+    q = 2
+    want = 0.01 * np.exp(-(p - 1 - q - n) * dt / 20e-3)
+    assert abs((con.W[0, 0] - 0.5) - want) < 1e-12, (n, p, con.W[0, 0])
+
+
 def test_facade_cuba_matches_oracle():
     """examples/misc/CUBA.py through the Brian API (fixed host connectivity)
     against the oracle: bit-exact raster in FP64 check mode."""

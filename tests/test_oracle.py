@@ -489,3 +489,75 @@ def test_stp_closed_forms():
     # homogeneous delay: the updater runs `delay` steps after the spike
     st, acc = _stp_run([[4]], 20, U, taud, tauf, delay=7)
     assert st.lastt[0] == (4 + 7) * DT + 0.0 and acc[0] == U
+
+
+def _stdp_trains(pre_steps, post_steps, interactions="all", q=None, Ap=0.01, Am=-0.0105, w0=0.5,
+                 tau=20e-3):
+    """Pre spikes (SpikeSource) -> one post neuron forced to fire at
+    post_steps; ExponentialSTDP with wmax = 1 on a Connection (q None) or a
+    DelayConnection with delay q*0.1 ms and max_delay 5 ms."""
+    nsteps = max(pre_steps + post_steps) + 60
+    net = O.Net(DT)
+    src = O.SpikeSource(1, step_lists=[np.array([0]) if k in pre_steps else np.zeros(0, np.int64)
+                                       for k in range(nsteps)])
+    sid = net.add_group(src, record=False)
+    drive = np.zeros((nsteps, 1))
+    drive[post_steps] = 1.0
+    grp = O.Group(1, np.zeros((1, 1)), ("linear", np.eye(1), np.zeros(1)), ("const", 0, 0.5),
+                  [(0, "=", 0.0, False)], dt=DT, inputs=[(0, drive)])
+    gid = net.add_group(grp)
+    kw, skw = {}, {}
+    if q is not None:
+        d = q * 1e-4
+        kw = {"syn_delay_steps": np.array([int(10000.0 * d)]), "max_delay_steps": int(5e-3 / DT)}
+        skw = {"delays_s": np.array([d])}
+    c = net.add_connection((sid, 0, 1), (gid, 0, 1), 0, np.array([0, 1]), np.array([0]), np.array([w0]),
+                           **kw)
+    net.add_stdp(c, taup=tau, taum=tau, Ap=Ap, Am=Am, wmin=0.0, wmax=1.0, interactions=interactions, **skw)
+    net.run(nsteps)
+    return float(c.data[0]) - w0
+
+
+def test_stdp_interactions_closed_forms():
+    """stdp.py:495-506: 'all' sums every pairing; 'nearest' keeps the last
+    spike's trace only; 'nearest_post' sums pre traces, resets post ones;
+    'nearest_pre' writes ``A_post=+Am`` (an assignment), so it resets both."""
+    e = lambda k: np.exp(-k * DT / 20e-3)
+    Ap, Am = 0.01, -0.0105
+    # two post spikes, then a pre spike: depression from A_post
+    for inter, both in (("all", True), ("nearest", False), ("nearest_pre", False), ("nearest_post", False)):
+        dw = _stdp_trains([200], [100, 150], inter)
+        want = Am * (e(100) + e(50)) if both else Am * e(50)
+        assert abs(dw - want) < 1e-13, (inter, dw, want)
+    # two pre spikes, then a post spike: potentiation from A_pre
+    for inter, both in (("all", True), ("nearest", False), ("nearest_pre", False), ("nearest_post", True)):
+        dw = _stdp_trains([100, 150], [200], inter)
+        want = Ap * (e(100) + e(50)) if both else Ap * e(50)
+        # the two depressions of the pre spikes against no earlier post: none
+        assert abs(dw - want) < 1e-13, (inter, dw, want)
+
+
+def test_stdp_delayconnection_closed_forms():
+    """STDP on a DelayConnection (stdp.py:296-368): with delay q steps a pre
+    spike at n pairs with the post trace recorded at the end of step n+q-1
+    (applied max_delay + 0.1 ms = 51 steps later), a post spike at p with the
+    pre trace recorded at the end of step p-1-q.  So pre at n, post at p > n+q
+    potentiates by Ap exp(-(p-1-q-n) dt/taup); post at p < n+q depresses by
+    Am exp(-(n+q-1-p) dt/taum); a post spike at the arrival step n+q pairs
+    with neither, except where the double expression int(1e4*(0.0051-d))
+    truncates one step short (q = 42, 45, 48), which reads the end of step
+    n+q and depresses by Am."""
+    e = lambda k: np.exp(-k * DT / 20e-3)
+    Ap, Am = 0.01, -0.0105
+    n = 100
+    for q in (1, 3, 10, 25, 50):
+        for gap in (1, 5, 40):
+            dw = _stdp_trains([n], [n + q + gap], q=q)
+            assert abs(dw - Ap * e(gap - 1)) < 1e-13, (q, gap, dw)
+            if n + q - gap >= 0:
+                dw = _stdp_trains([n], [n + q - gap], q=q)
+                assert abs(dw - Am * e(gap - 1)) < 1e-13, (q, -gap, dw)
+        assert _stdp_trains([n], [n + q], q=q) == 0.0
+    for q in (42, 45, 48):
+        assert abs(_stdp_trains([n], [n + q], q=q) - Am) < 1e-15
+    assert _stdp_trains([n], [n + 44], q=44) == 0.0
