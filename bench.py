@@ -599,21 +599,23 @@ def bench_fit(args):
     from brian_amd import learned as L
     from brian_amd import workloads as W
     from brian_amd.equations import Equations
-    from brian_amd.modelfitting import FitEngine, modelfitting, spike_steps
+    from brian_amd.modelfitting import FitEngine, OUInput, modelfitting, spike_steps
     DT = 1e-4
     ntarget, pop = args.targets, args.popsize
     steps = args.steps or 100000
     dur = steps * DT
     t_setup = time.perf_counter()
-    I = W.ou_current(ntarget, steps, seed=41)
+    # OU input (mean 0.5 nA, sd 0.2 nA, tau 5 ms) generated on the device
+    I_dev = OUInput(ntarget, steps, 0.5e-9, 0.2e-9, 5e-3, seed=41)
     hidden = W.fit_hidden_params(ntarget, seed=42)
     eqs = Equations(W.FIT_EQS, tau_t=W.FIT_TAU_T)
     b = W.FIT_BOUNDS
     refr = spike_steps(hidden["refractory"], DT)
     # target trains: the hidden neurons run through the fitting kernel in
     # record mode (FP64)
-    eng = FitEngine(eqs, W.FIT_THRESHOLD, W.FIT_RESET, I, [np.zeros(0, np.int32)] * ntarget,
+    eng = FitEngine(eqs, W.FIT_THRESHOLD, W.FIT_RESET, I_dev, [np.zeros(0, np.int32)] * ntarget,
                     np.arange(ntarget), DT, 40, precision=64, record=int(ntarget * dur * 200) + 1024)
+    I = eng.input()                          # host copy for the learned proposal's features
     eng.set_candidates({k: v for k, v in hidden.items() if k != "refractory"},
                        init_state={"V": 0.0, "Vt": 0.0}, refractory_steps=refr)
     eng.run()

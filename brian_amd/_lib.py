@@ -114,6 +114,8 @@ _SIGS = {
     "bm_fit_sync": [P],
     "bm_fit_results": [P, Pi32, Pi32],
     "bm_fit_get_state": [P, Pd],
+    "bm_fit_set_ou_input": [P, dbl, dbl, dbl, u64],
+    "bm_fit_get_input": [P, i64, i64, Pd],
     "bm_stdp_batch_run": [P, i64, C.c_int],
     "bm_stdp_batch_sync": [P],
     "bm_stdp_batch_get": [P, C.c_int, Pd],

@@ -26,10 +26,14 @@ __host__ __device__ inline uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
 }
+__host__ __device__ inline uint64_t counter_key(uint64_t seed, uint64_t stream) {
+  return mix64(seed * 0x9E3779B97F4A7C15ull + stream);
+}
+__host__ __device__ inline uint64_t counter_u64(uint64_t key, uint64_t counter) {
+  return mix64(key + counter * 0x9E3779B97F4A7C15ull);
+}
 __host__ __device__ inline uint32_t counter_u32(uint64_t seed, uint64_t stream, uint64_t counter) {
-  const uint64_t G = 0x9E3779B97F4A7C15ull;
-  uint64_t key = mix64(seed * G + stream);
-  return (uint32_t)(mix64(key + counter * G) >> 32);
+  return (uint32_t)(counter_u64(counter_key(seed, stream), counter) >> 32);
 }
 
 // Cheaper per-step stream for inputs drawn every step (batched STDP): a

@@ -15,6 +15,8 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -209,6 +211,77 @@ struct bm_fit {
 };
 
 namespace {
+// ---- Ornstein-Uhlenbeck input generated on the device
+// Standard normal for (target key, step n): Box-Muller (Box & Muller 1958)
+// on two 53-bit uniforms drawn from counters 2n and 2n+1 of the target's
+// counter stream.
+__device__ inline double ou_normal(uint64_t key, int64_t n) {
+#pragma clang fp contract(off)
+  const uint64_t h1 = counter_u64(key, (uint64_t)(2 * n)), h2 = counter_u64(key, (uint64_t)(2 * n + 1));
+  const double u1 = ((double)(h1 >> 11) + 0.5) * 0x1.0p-53;   // (0, 1)
+  const double u2 = (double)(h2 >> 11) * 0x1.0p-53;           // [0, 1)
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// The exact OU update y[n] = a y[n-1] + b z[n] (y = I - mu, y[0] = sigma
+// z[0]) is affine in y, so time splits into chunks of L steps: pass 1 runs
+// every chunk from y = 0 and keeps its end value, pass 2 chains the chunk
+// ends per target (y_end = a^len y_end_prev + local_end), pass 3 reruns
+// every chunk from its carried-in value and writes I.  One thread per
+// (target, chunk) in passes 1 and 3.
+struct OuArgs {
+  int ntarget, nchunks, L;
+  int64_t nsteps;
+  double mu, a, b, sigma, aL, alast;
+  uint64_t seed;
+};
+
+__global__ void k_ou_local(double *E, const OuArgs o) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)o.ntarget * o.nchunks) return;
+  const int j = (int)(t / o.nchunks), c = (int)(t % o.nchunks);
+  const uint64_t key = counter_key(o.seed, (uint64_t)j);
+  const int64_t s0 = (int64_t)c * o.L, s1 = s0 + o.L < o.nsteps ? s0 + o.L : o.nsteps;
+  double y = 0.0;
+  for (int64_t k = s0; k < s1; ++k) {
+    const double z = ou_normal(key, k);
+    y = k == 0 ? o.sigma * z : o.a * y + o.b * z;
+  }
+  E[t] = y;
+}
+
+__global__ void k_ou_carry(double *E, const OuArgs o) {
+#pragma clang fp contract(off)
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= o.ntarget) return;
+  double y = 0.0;
+  for (int c = 0; c < o.nchunks; ++c) {
+    const int64_t q = (int64_t)j * o.nchunks + c;
+    const double local = E[q];
+    E[q] = y;                                   // value just before the chunk
+    y = (c + 1 == o.nchunks ? o.alast : o.aL) * y + local;
+  }
+}
+
+template <typename T>
+__global__ void k_ou_write(T *I, const double *E, const OuArgs o) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)o.ntarget * o.nchunks) return;
+  const int j = (int)(t / o.nchunks), c = (int)(t % o.nchunks);
+  const uint64_t key = counter_key(o.seed, (uint64_t)j);
+  const int64_t s0 = (int64_t)c * o.L, s1 = s0 + o.L < o.nsteps ? s0 + o.L : o.nsteps;
+  const double y0 = E[t];
+  double v = 0.0, ap = 1.0;
+  for (int64_t k = s0; k < s1; ++k) {
+    const double z = ou_normal(key, k);
+    v = k == 0 ? o.sigma * z : o.a * v + o.b * z;
+    ap = ap * o.a;
+    I[(int64_t)j * o.nsteps + k] = (T)(o.mu + (ap * y0 + v));
+  }
+}
+
 void upload_scalar(bm_fit *h, DevBuf &b, const double *src, size_t n) {
   b.alloc(n * h->tsize());
   if (!n) return;
@@ -293,7 +366,12 @@ int bm_fit_create(int precision, double dt, int ncand, int ntarget, int nvars, i
     replace_all(src, "%FEAT_STORE%", h->features ? kFeatStore : "");
     replace_all(src, "%REC_SPIKE%", h->record ? kRecSpike : "");
     h->mod = compile_fit(src);
-    upload_scalar(h.get(), h->I, I, (size_t)ntarget * nsteps_input);
+    if (I) {
+      upload_scalar(h.get(), h->I, I, (size_t)ntarget * nsteps_input);
+    } else {     // filled later by bm_fit_set_ou_input
+      h->I.alloc((size_t)ntarget * nsteps_input * h->tsize());
+      BM_HIP(hipMemsetAsync(h->I.p, 0, h->I.bytes, h->stream));
+    }
     for (int c = 0; c < ncand; ++c)
       BM_REQUIRE(cand_target[c] >= 0 && cand_target[c] < ntarget, "candidate target out of range");
     h->cand_tgt.assign(cand_target, cand_target + ncand);
@@ -420,6 +498,56 @@ int bm_fit_features(bm_fit *h, float *out) {
     BM_REQUIRE(h->features, "the fitting handle was created without feature mode");
     BM_HIP(hipMemcpyAsync(out, h->feat.p, (size_t)h->ncand * 66 * 4, hipMemcpyDeviceToHost, h->stream));
     BM_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int bm_fit_set_ou_input(bm_fit *h, double mu, double sigma, double tau, uint64_t seed) {
+  return guard([&] {
+    BM_REQUIRE(sigma >= 0 && tau > 0, "OU input needs sigma >= 0 and tau > 0");
+    OuArgs o{};
+    o.ntarget = h->ntarget;
+    o.nsteps = h->nsteps_input;
+    o.L = 128;
+    const int64_t nch = (o.nsteps + o.L - 1) / o.L;
+    BM_REQUIRE(nch < (1 << 30), "input too long");
+    o.nchunks = (int)nch;
+    o.mu = mu;
+    o.sigma = sigma;
+    o.a = std::exp(-h->dt / tau);
+    o.b = sigma * std::sqrt(1.0 - o.a * o.a);
+    o.aL = std::pow(o.a, (double)o.L);
+    o.alast = std::pow(o.a, (double)(o.nsteps - (nch - 1) * o.L));
+    o.seed = seed;
+    DevBuf E;
+    E.alloc((size_t)o.ntarget * o.nchunks * sizeof(double));
+    const int64_t nt = (int64_t)o.ntarget * o.nchunks;
+    const unsigned blocks = (unsigned)((nt + 255) / 256);
+    hipLaunchKernelGGL(k_ou_local, dim3(blocks), dim3(256), 0, h->stream, E.as<double>(), o);
+    hipLaunchKernelGGL(k_ou_carry, dim3((o.ntarget + 255) / 256), dim3(256), 0, h->stream, E.as<double>(), o);
+    if (h->precision == 64)
+      hipLaunchKernelGGL(k_ou_write<double>, dim3(blocks), dim3(256), 0, h->stream, h->I.as<double>(),
+                         E.as<double>(), o);
+    else
+      hipLaunchKernelGGL(k_ou_write<float>, dim3(blocks), dim3(256), 0, h->stream, h->I.as<float>(),
+                         E.as<double>(), o);
+    BM_HIP(hipGetLastError());
+    BM_HIP(hipStreamSynchronize(h->stream));     // E is released on return
+  });
+}
+
+int bm_fit_get_input(bm_fit *h, int64_t t0, int64_t t1, double *out) {
+  return guard([&] {
+    BM_REQUIRE(t0 >= 0 && t1 >= t0 && t1 <= h->nsteps_input, "time range outside the input");
+    const int64_t w = t1 - t0;
+    if (w == 0) return;
+    const size_t ts = h->tsize();
+    std::vector<unsigned char> buf((size_t)h->ntarget * w * ts);
+    BM_HIP(hipMemcpy2DAsync(buf.data(), w * ts, h->I.as<unsigned char>() + t0 * ts, h->nsteps_input * ts, w * ts,
+                            h->ntarget, hipMemcpyDeviceToHost, h->stream));
+    BM_HIP(hipStreamSynchronize(h->stream));
+    const size_t n = (size_t)h->ntarget * w;
+    if (h->precision == 64) std::memcpy(out, buf.data(), n * 8);
+    else for (size_t i = 0; i < n; ++i) out[i] = reinterpret_cast<const float *>(buf.data())[i];
   });
 }
 

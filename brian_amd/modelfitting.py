@@ -178,11 +178,24 @@ def spike_steps(times, dt):
     return np.array(np.rint(np.asarray(times, dtype=np.float64) / dt), dtype=np.int32)
 
 
+class OUInput:
+    """Ornstein-Uhlenbeck input current generated on the device for every
+    target (bm_fit_set_ou_input): exact update with mean ``mu``, standard
+    deviation ``sigma``, time constant ``tau``, ``nsteps`` steps per target,
+    counter-RNG ``seed``.  Pass it as ``input`` / ``I`` where an array of
+    shape [ntarget][nsteps] is accepted."""
+
+    def __init__(self, ntarget, nsteps, mu=0.5e-9, sigma=0.2e-9, tau=5e-3, seed=0):
+        self.shape = (int(ntarget), int(nsteps))
+        self.mu, self.sigma, self.tau, self.seed = float(mu), float(sigma), float(tau), int(seed)
+
+
 class FitEngine:
     """Owning wrapper of a ``bm_fit`` handle.
 
     targets: list of sorted spike-step arrays (one per target); I: input
-    current [ntarget][nsteps]; cand_target: target index of each candidate.
+    current [ntarget][nsteps] or an OUInput generated on the device;
+    cand_target: target index of each candidate.
     """
 
     def __init__(self, eqs, threshold, reset, I, targets, cand_target, dt, delta_steps,
@@ -191,8 +204,13 @@ class FitEngine:
         self._lib = _lib.load()
         src = fit_kernel_source(eqs, threshold, reset, input_var, method, precision)
         self.state_names, self.param_names = src["state"], src["params"]
-        I = np.ascontiguousarray(np.atleast_2d(np.asarray(I, dtype=np.float64)))
-        self.ntarget, self.nsteps = I.shape
+        ou = I if isinstance(I, OUInput) else None
+        if ou is not None:
+            self.ntarget, self.nsteps = ou.shape
+            I = None
+        else:
+            I = np.ascontiguousarray(np.atleast_2d(np.asarray(I, dtype=np.float64)))
+            self.ntarget, self.nsteps = I.shape
         self.cand_target = np.ascontiguousarray(cand_target, dtype=np.int32)
         self.ncand = len(self.cand_target)
         lens = np.array([len(t) for t in targets], dtype=np.int32)
@@ -205,14 +223,24 @@ class FitEngine:
                   len(self.state_names), len(self.param_names), src["load"].encode(),
                   src["update"].encode(), src["threshold"].encode(), src["reset1"].encode(),
                   src["reset2"].encode(), src["store"].encode(), input_var.encode(),
-                  self.nsteps, _lib.dptr(I), _lib.i32ptr(self.cand_target), _lib.i32ptr(lens),
+                  self.nsteps, None if I is None else _lib.dptr(I), _lib.i32ptr(self.cand_target),
+                  _lib.i32ptr(lens),
                   _lib.i32ptr(flat), int(delta_steps), int(onset_steps),
                   (1 if features else 0) | (2 if record else 0), C.byref(h))
         self.h = h
+        if ou is not None:
+            _lib.call("bm_fit_set_ou_input", h, ou.mu, ou.sigma, ou.tau, ou.seed)
         if record:
             _lib.call("bm_fit_set_record", h, int(record))
         self.dt = dt
         self.src = src
+
+    def input(self, t0=0, t1=None):
+        """Input rows [ntarget][t1 - t0] as the engine holds them."""
+        t1 = self.nsteps if t1 is None else t1
+        out = np.empty((self.ntarget, t1 - t0), dtype=np.float64)
+        _lib.call("bm_fit_get_input", self.h, int(t0), int(t1), _lib.dptr(out))
+        return out
 
     def close(self):
         if self.h:
@@ -360,7 +388,7 @@ def modelfitting(model=None, reset=None, threshold=None, refractory=0 * ms, data
                  evaluate_initial=None, **params):
     """Fit ``model`` to ``data`` (list of (target index, spike time) or one
     array of spike times); ``input`` is one current trace shared by all
-    targets or one trace per target.  ``params`` give [min, max] bounds (or
+    targets, one trace per target, or an OUInput generated on the device.  ``params`` give [min, max] bounds (or
     [min, init_min, init_max, max]); ``refractory`` may be a bound pair.
     ``initial_params`` optionally gives the starting particles as a dict of
     [ntargets*popsize] arrays (the learned proposal uses this).
@@ -374,10 +402,16 @@ def modelfitting(model=None, reset=None, threshold=None, refractory=0 * ms, data
     if data.ndim == 1:
         data = np.stack([np.zeros(len(data)), data], axis=1)
     ntarget = int(data[:, 0].max()) + 1 if len(data) else 1
-    I = np.atleast_2d(np.asarray(input, dtype=np.float64))
-    if I.shape[0] == 1 and ntarget > 1:
-        I = np.repeat(I, ntarget, axis=0)
-    nsteps = I.shape[1]
+    if isinstance(input, OUInput):
+        I = input                       # generated on the device, one row per target
+        if I.shape[0] != ntarget:
+            raise ValueError("OUInput must have one row per target")
+        nsteps = I.shape[1]
+    else:
+        I = np.atleast_2d(np.asarray(input, dtype=np.float64))
+        if I.shape[0] == 1 and ntarget > 1:
+            I = np.repeat(I, ntarget, axis=0)
+        nsteps = I.shape[1]
     duration = nsteps * dt
     targets = [np.sort(spike_steps(data[data[:, 0] == j, 1], dt)) for j in range(ntarget)]
     tlen = np.array([len(t) for t in targets], dtype=np.float64)

@@ -340,6 +340,17 @@ int bm_fit_run(bm_fit *h, int64_t t0, int64_t t1);
 int bm_fit_sync(bm_fit *h);
 int bm_fit_results(bm_fit *h, int32_t *ncoinc, int32_t *nspikes);
 int bm_fit_get_state(bm_fit *h, double *out);
+/* Ornstein-Uhlenbeck input generated on the device into every target row
+ * (bm_fit_create may then pass I = NULL): I[j][0] = mu + sigma z(j,0),
+ * I[j][n] = mu + a (I[j][n-1] - mu) + sigma sqrt(1 - a^2) z(j,n), a =
+ * exp(-dt/tau) (the exact update), z(j,n) standard normal by Box-Muller on
+ * two 53-bit uniforms from counters 2n, 2n+1 of target j's counter stream
+ * (key = splitmix64(seed * 0x9E3779B97F4A7C15 + j)).  Computed in FP64,
+ * stored in the engine's precision.  The role of the current arrays the
+ * reference's fitting examples load (examples/modelfitting/current.txt). */
+int bm_fit_set_ou_input(bm_fit *h, double mu, double sigma, double tau, uint64_t seed);
+/* Input rows for steps [t0, t1): out[ntarget][t1 - t0]. */
+int bm_fit_get_input(bm_fit *h, int64_t t0, int64_t t1, double *out);
 /* Feature mode only: [ncand][66] raw spike-train features (32 log-binned ISI
  * counts, 32 spike-triggered input sums at 1 ms lags, triggered-spike count,
  * unused) for the learned proposal; no counterpart in the reference. */

@@ -128,3 +128,39 @@ def test_fit_kernel_record_mode_trains_exact():
     _, ns = eng.results()
     np.testing.assert_array_equal(ns, [len(t) for t in ref])
     eng.close()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_device_ou_input_matches_oracle(precision):
+    """OU input generated on the device (chunked in time) against the
+    sequential FP64 oracle; FP64 within 1e-12 sigma, FP32 within its
+    rounding.  Then the hidden neurons driven by the device input (record
+    mode) spike exactly where the oracle's neurons do on the read-back
+    input."""
+    from brian_amd.modelfitting import OUInput
+    from oracle import inputs as OI
+    n, steps = 8, 30000                      # not a multiple of the 128-step chunks
+    mu, sigma, tau = 0.5e-9, 0.2e-9, 5e-3
+    empty = [np.zeros(0, np.int32)] * n
+    eng = FitEngine(fit_eqs(), W.FIT_THRESHOLD, W.FIT_RESET, OUInput(n, steps, mu, sigma, tau, seed=5), empty,
+                    np.arange(n), DT, 40, precision=precision, record=1 << 16)
+    got = eng.input()
+    ref = OI.ou_process(n, steps, DT, mu, sigma, tau, seed=5)
+    if precision == 64:
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * sigma)
+    else:
+        np.testing.assert_allclose(got, ref, rtol=2e-7, atol=1e-7 * sigma)
+        eng.close()
+        return
+    np.testing.assert_array_equal(eng.input(1000, 1300), got[:, 1000:1300])
+    hidden = W.fit_hidden_params(n, seed=6)
+    refr = spike_steps(hidden["refractory"], DT)
+    eng.set_candidates({k: v for k, v in hidden.items() if k != "refractory"},
+                       init_state={"V": 0.0, "Vt": 0.0}, refractory_steps=refr)
+    eng.run(0, steps)
+    trains = eng.trains()
+    want = oracle_trains(hidden, refr, got, steps)
+    assert sum(len(t) for t in want) > 50
+    for j in range(n):
+        np.testing.assert_array_equal(trains[j], want[j])
+    eng.close()

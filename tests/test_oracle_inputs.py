@@ -1,0 +1,62 @@
+"""Pins of the OU input oracle (oracle/inputs.py)."""
+import numpy as np
+import scipy.stats
+
+from oracle import inputs as OI
+
+M64 = (1 << 64) - 1
+
+
+def _mix64_int(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def test_counter_normal_integer_stream():
+    """The uniforms come from the same 64-bit words a Python-integer
+    restatement gives; splitmix64 of 0 + G is the published first output
+    0xE220A8397B1DCDAF."""
+    G = 0x9E3779B97F4A7C15
+    assert _mix64_int(G) == 0xE220A8397B1DCDAF
+    seed, j, n = 123456789, 7, 4242
+    key = _mix64_int((seed * G + j) & M64)
+    h1 = _mix64_int((key + 2 * n * G) & M64)
+    h2 = _mix64_int((key + (2 * n + 1) * G) & M64)
+    u1 = ((h1 >> 11) + 0.5) * 2.0 ** -53
+    u2 = (h2 >> 11) * 2.0 ** -53
+    z = np.sqrt(-2.0 * np.log(u1)) * np.cos(6.283185307179586 * u2)
+    assert OI.counter_normal(seed, [j], [n])[0, 0] == z
+
+
+def test_counter_normal_distribution():
+    """Standard normal: mean, variance, KS distance over 2^20 draws; streams
+    and steps are uncorrelated."""
+    z = OI.counter_normal(3, np.arange(64), np.arange(1 << 14))
+    x = z.ravel()
+    assert abs(x.mean()) < 5 / np.sqrt(x.size)
+    assert abs(x.var() - 1.0) < 5 * np.sqrt(2.0 / x.size)
+    assert scipy.stats.kstest(x, "norm").pvalue > 1e-3
+    assert abs(np.corrcoef(z[:, :-1].ravel(), z[:, 1:].ravel())[0, 1]) < 5 / np.sqrt(x.size)
+    assert abs(np.corrcoef(z[:-1].ravel(), z[1:].ravel())[0, 1]) < 5 / np.sqrt(x.size)
+
+
+def test_ou_process_statistics_and_update():
+    """Stationary OU: mean mu, variance sigma^2, lag-k autocorrelation
+    exp(-k dt/tau); consecutive values give back the normals exactly up to
+    rounding (a wrong decay or noise scale fails this)."""
+    dt, mu, sigma, tau = 1e-4, 0.5e-9, 0.2e-9, 5e-3
+    I = OI.ou_process(32, 20000, dt, mu, sigma, tau, seed=11)
+    a = np.exp(-dt / tau)
+    y = (I - mu) / sigma
+    neff = y.size * (1 - a) / (1 + a)           # effective samples of a correlated series
+    assert abs(y.mean()) < 5 / np.sqrt(neff)
+    assert abs(y.var() - 1.0) < 0.05
+    for k in (1, 10, 50):
+        r = np.mean(y[:, :-k] * y[:, k:]) / y.var()
+        assert abs(r - a ** k) < 0.05, (k, r, a ** k)
+    z = OI.counter_normal(11, np.arange(32), np.arange(20000))
+    b = np.sqrt(1 - a * a)
+    zr = (y[:, 1:] - a * y[:, :-1]) / b
+    np.testing.assert_allclose(zr, z[:, 1:], atol=1e-9)
+    np.testing.assert_allclose(y[:, 0], z[:, 0], rtol=0, atol=1e-12)
