@@ -496,8 +496,20 @@ def _full_csr(W, nsrc, ntgt):
     return W
 
 
+def _dense_csr(W, nsrc, ntgt):
+    """Every (i, j) as a synapse, zeros included: a dense connection matrix
+    propagates and (with STDP) updates whole rows and columns
+    (connectionmatrix.py:141-193)."""
+    data = np.asarray(sp.csr_matrix(W, shape=(nsrc, ntgt)).toarray(), dtype=np.float64).ravel()
+    indptr = np.arange(nsrc + 1, dtype=np.int64) * ntgt
+    indices = np.tile(np.arange(ntgt, dtype=np.int32), nsrc)
+    return sp.csr_matrix((data, indices, indptr), shape=(nsrc, ntgt))
+
+
 class Connection(_Tracked):
-    """brian/connections/connection.py:15-462 (sparse structure).
+    """brian/connections/connection.py:15-462 with the 'sparse', 'dense'
+    (every pair is a synapse, zeros included) and 'dynamic' structures
+    (connectionmatrix.py:141-193, :196-663); the device stores each as CSR.
 
     ``Connection(source, target, state=0, delay=0*ms, modulation=None,
     structure='sparse', weight=None, sparseness=None, max_delay=5*ms)``.
@@ -520,7 +532,7 @@ class Connection(_Tracked):
     def __init__(self, source, target, state=0, delay=0 * ms, modulation=None,
                  structure="sparse", weight=None, sparseness=None, max_delay=5 * ms, **kwds):
         self._track(1)
-        self._init_common(source, target, state, modulation, weight, sparseness)
+        self._init_common(source, target, state, modulation, weight, sparseness, structure)
         self.delay = int(float(delay) / source.clock.dt)      # connection.py:200
         self._delay_kind = 0
         if self._mod is not None and self.delay > 0:
@@ -529,7 +541,12 @@ class Connection(_Tracked):
             raise NotImplementedError("modulation on a Connection with a homogeneous delay "
                                       "is not supported (use DelayConnection)")
 
-    def _init_common(self, source, target, state, modulation, weight, sparseness):
+    def _init_common(self, source, target, state, modulation, weight, sparseness, structure="sparse"):
+        if structure not in ("sparse", "dense", "dynamic"):
+            raise NotImplementedError("structure must be 'sparse', 'dense' or 'dynamic'")
+        # 'dynamic' stores like 'sparse' here (synapses are fixed once the
+        # network runs); 'dense' makes every pair a synapse
+        self._structure = structure
         self.source = source
         self.target = target
         self.state = state
@@ -642,8 +659,14 @@ class Connection(_Tracked):
 
     def compress(self):
         if self._compiled_W is None:
-            self._compiled_W = _full_csr(self._W, len(self.source), len(self.target))
+            self._compiled_W = self._pattern_csr()
         return self._compiled_W
+
+    def _pattern_csr(self):
+        """The synapses of the connection with their weights, CSR, columns ascending."""
+        if self._structure == "dense":
+            return _dense_csr(self._W, len(self.source), len(self.target))
+        return _full_csr(self._W, len(self.source), len(self.target))
 
 
 def _argcount(f):
@@ -669,7 +692,7 @@ class DelayConnection(Connection):
         self._delay_kind = 1
         self._delays = None
         self.delayvec = sp.lil_matrix((len(source), len(target)))
-        self._init_common(source, target, state, modulation, weight, sparseness)
+        self._init_common(source, target, state, modulation, weight, sparseness, structure)
         if delay is True:       # connection.py:176-178: heterogeneous, none set yet
             delay = None
         if delay is not None:
@@ -686,8 +709,7 @@ class DelayConnection(Connection):
         Q = target if target is not None else self.target
         i0, j0 = self._origin(P, Q)
         i1, j1 = i0 + len(P), j0 + len(Q)
-        W = self._W.tocsr()
-        W.sort_indices()
+        W = self._pattern_csr()
         if isinstance(delay, (float, int)) or (isinstance(delay, (tuple, list)) and len(delay) == 2) \
                 or callable(delay):
             n_args = _argcount(delay) if callable(delay) else None

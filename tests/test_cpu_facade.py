@@ -51,3 +51,31 @@ def test_stp_facade_checks():
     d = b.Connection(H, G, 'V', weight=2, delay=True)
     with pytest.raises(AttributeError):
         b.STP(d, taud=200 * b.ms, tauf=20 * b.ms, U=0.2)
+
+
+def test_dense_structure_keeps_every_pair():
+    """connectionmatrix.py:141-193: a dense matrix holds every (i, j), zeros
+    included, so propagation and STDP see whole rows and columns; a sparse
+    one only its nonzeros.  A dense DelayConnection has a delay per pair."""
+    import brian_amd as b
+    b.reinit()
+    H = b.NeuronGroup(3, 'V:1', reset=0, threshold=1)
+    G = b.NeuronGroup(4, 'V:1', reset=0, threshold=1)
+    W = np.zeros((3, 4))
+    W[0, 1] = 2.0
+    W[2, 3] = 1.5
+    cd = b.Connection(H, G, 'V', structure='dense')
+    cd.connect(H, G, W)
+    cs = b.Connection(H, G, 'V')
+    cs.connect(H, G, W)
+    Md, Ms = cd.compress(), cs.compress()
+    assert Md.nnz == 12 and Ms.nnz == 2
+    np.testing.assert_array_equal(Md.toarray(), W)
+    np.testing.assert_array_equal(np.diff(Md.indptr), [4, 4, 4])
+    dd = b.DelayConnection(H, G, 'V', structure='dense')
+    dd.connect(H, G, W)
+    dd.set_delays(delay=1 * b.ms)
+    steps = dd._delay_steps(dd.compress())
+    assert len(steps) == 12 and (steps == 10).all()
+    with pytest.raises(NotImplementedError):
+        b.Connection(H, G, 'V', structure='banded')

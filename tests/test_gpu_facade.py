@@ -149,6 +149,36 @@ def test_stdp_delayconnection_pairing():
     assert abs((con.W[0, 0] - 0.5) - want) < 1e-12, (n, p, con.W[0, 0])
 
 
+@pytest.mark.parametrize("structure", ["dense", "sparse"])
+def test_stdp_dense_structure_updates_zero_weights(structure):
+    """A dense connection matrix makes every pair a synapse
+    (connectionmatrix.py:141-193), so STDP potentiates a zero weight too;
+    in a sparse one the zero is no synapse and stays zero.  Pre spikes at
+    step n, post spike at step p: dw = Ap exp(-(p-n) dt/taup)."""
+    from brian_amd import (NeuronGroup, SpikeGeneratorGroup, Connection, ExponentialSTDP, SpikeMonitor,
+                           Network, ms)
+    pre = SpikeGeneratorGroup(2, [(0, 1 * ms), (1, 1 * ms)])
+    teach = SpikeGeneratorGroup(1, [(0, 2 * ms)])
+    G = NeuronGroup(1, "v:1\nx:1", threshold=1, reset=0)
+    drive = Connection(teach, G, "v", weight=2)
+    con = Connection(pre, G, "x", structure=structure)
+    con.connect(pre, G, np.array([[0.5], [0.0]]))
+    stdp = ExponentialSTDP(con, 20 * ms, 20 * ms, 0.01, -0.0105, wmax=1.0)
+    Mpre, Mpost = SpikeMonitor(pre), SpikeMonitor(G)
+    net = Network(pre, teach, G, drive, con, stdp, Mpre, Mpost)
+    net.run(10 * ms)
+    dt = 1e-4
+    n = int(round(Mpre.spikes[0][1] / dt))
+    p = int(round(Mpost.spikes[0][1] / dt))
+    dw = 0.01 * np.exp(-(p - n) * dt / 20e-3)
+    W = con.W.toarray()
+    assert abs(W[0, 0] - (0.5 + dw)) < 1e-12
+    if structure == "dense":
+        assert abs(W[1, 0] - dw) < 1e-12
+    else:
+        assert W[1, 0] == 0.0 and con.W.nnz == 1
+
+
 def test_facade_cuba_matches_oracle():
     """examples/misc/CUBA.py through the Brian API (fixed host connectivity)
     against the oracle: bit-exact raster in FP64 check mode."""
