@@ -27,7 +27,8 @@ __all__ = [
     "Threshold", "VariableThreshold", "EmpiricalThreshold", "NoThreshold", "StringThreshold",
     "Reset", "VariableReset", "StringReset", "Refractoriness", "NoReset",
     "NeuronGroup", "SpikeGeneratorGroup", "PoissonGroup", "TimedArray",
-    "Connection", "DelayConnection", "STDP", "ExponentialSTDP", "STP",
+    "Connection", "DelayConnection", "STDP", "ExponentialSTDP", "STP", "ConnectionMatrix",
+    "SparseConnectionVector", "random_matrix", "random_matrix_fixed_column",
     "SpikeMonitor", "PopulationRateMonitor", "PopulationSpikeCounter", "SpikeCounter",
     "StateMonitor", "Network", "MagicNetwork", "run", "reinit", "stop", "network_operation",
     "NetworkOperation", "set_global_preferences", "get_global_preference", "forget",
@@ -263,6 +264,7 @@ class NeuronGroup(_GroupBase):
         if init is not None:
             for j, v in enumerate(init):
                 self._S[j, :] = v
+        self._S0 = self._S.copy()      # restored by reinit (neurongroup.py:523-528)
         self._method = method
         self._order = order
         self._implicit = implicit or method == "exponential_Euler"
@@ -496,6 +498,163 @@ def _full_csr(W, nsrc, ntgt):
     return W
 
 
+def _nargs(f):
+    n = _argcount(f)
+    if n not in (0, 2):
+        raise AttributeError("Bad number of arguments in value function (should be 0 or 2)")
+    return n
+
+
+def _row_values(value, i, cols):
+    """value(), value(i, j) per synapse, or value(i, cols) on a whole row when
+    the function takes vectors (construction.py:58-66)."""
+    if not callable(value):
+        return np.full(len(cols), float(value))
+    if _nargs(value) == 0:
+        return np.array([value() for _ in cols], dtype=np.float64)
+    try:
+        v = np.asarray(value(i, cols), dtype=np.float64)
+        if v.size == len(cols):
+            return v.reshape(-1)
+    except Exception:
+        pass
+    return np.array([value(i, j) for j in cols], dtype=np.float64)
+
+
+def random_matrix(n, m, p, value=1.0, rng=None):
+    """construction.py:42-139: lil (n, m) with each entry present with
+    probability p (a number, or a function of no arguments or of (i, j),
+    vectorised over a row when possible), valued ``value`` (a number or a
+    function of no arguments or of (i, j))."""
+    rng = rng if rng is not None else np.random.default_rng()
+    W = sp.lil_matrix((n, m))
+    cols_all = np.arange(m)
+    for i in range(n):
+        if callable(p):
+            if _nargs(p) == 0:
+                cols = np.array([j for j in range(m) if rng.random() < p()], dtype=np.int64)
+            else:
+                try:
+                    pr = np.asarray(p(i, cols_all), dtype=np.float64)
+                    ok = pr.size == m
+                except Exception:
+                    ok = False
+                if ok:
+                    cols = np.nonzero(rng.random(m) < pr.reshape(-1))[0]
+                else:
+                    cols = np.array([j for j in range(m) if rng.random() < p(i, j)], dtype=np.int64)
+        else:
+            k = rng.binomial(m, p)
+            cols = np.sort(rng.choice(m, k, replace=False))
+        W.rows[i] = [int(j) for j in cols]
+        W.data[i] = [float(x) for x in _row_values(value, i, cols)]
+    return W
+
+
+def random_matrix_fixed_column(n, m, p, value=1.0, rng=None):
+    """construction.py:141-170: lil (n, m) with int(p*n) distinct rows per
+    column, valued ``value`` (a number or a function of no arguments or of
+    (i, j))."""
+    rng = rng if rng is not None else np.random.default_rng()
+    k = int(p * n)
+    rows = [[] for _ in range(n)]
+    for j in range(m):
+        for i in rng.choice(n, k, replace=False):
+            rows[int(i)].append(j)
+    W = sp.lil_matrix((n, m))
+    for i in range(n):
+        cols = np.array(sorted(rows[i]), dtype=np.int64)
+        W.rows[i] = [int(j) for j in cols]
+        W.data[i] = [float(x) for x in _row_values(value, i, cols)]
+    return W
+
+
+class SparseConnectionVector(np.ndarray):
+    """Values of one row or column of a sparse connection matrix, with the
+    indices they sit at in ``ind`` and the full length in ``n``
+    (connectionvector.py:38-131)."""
+
+    def __new__(cls, n, ind, values):
+        obj = np.asarray(values, dtype=np.float64).view(cls)
+        obj.ind = np.asarray(ind, dtype=np.int64)
+        obj.n = int(n)
+        return obj
+
+    def __array_finalize__(self, obj):
+        self.ind = getattr(obj, "ind", None)
+        self.n = getattr(obj, "n", None)
+
+    def todense(self):
+        out = np.zeros(self.n)
+        out[self.ind] = np.asarray(self)
+        return out
+
+
+class ConnectionMatrix:
+    """Read access to a connection's weights in the reference's form
+    (connectionmatrix.py:141-455): ``W[i, j]`` a value, ``W[i, :]`` /
+    ``W[:, j]`` the row / column (the full vector for a dense structure,
+    the values at the synapses as a SparseConnectionVector otherwise),
+    ``todense()``, ``nnz`` (synapses), ``tocsr()``."""
+
+    def __init__(self, csr, dense=False):
+        self._M = csr
+        self.dense = dense
+        self.shape = csr.shape
+
+    @property
+    def nnz(self):
+        return self._M.nnz
+
+    def tocsr(self):
+        return self._M.copy()
+
+    def todense(self):
+        return np.asarray(self._M.toarray())
+
+    toarray = todense
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.todense()
+        return a if dtype is None else a.astype(dtype)
+
+    def nonzero(self):
+        return self._M.nonzero()
+
+    def get_row(self, i):
+        lo, hi = self._M.indptr[i], self._M.indptr[i + 1]
+        if self.dense:
+            return np.asarray(self._M[i].toarray()).ravel()
+        return SparseConnectionVector(self.shape[1], self._M.indices[lo:hi], self._M.data[lo:hi])
+
+    def get_col(self, j):
+        C = self._M.tocsc()
+        C.sort_indices()
+        lo, hi = C.indptr[j], C.indptr[j + 1]
+        if self.dense:
+            return np.asarray(C[:, j].toarray()).ravel()
+        return SparseConnectionVector(self.shape[0], C.indices[lo:hi], C.data[lo:hi])
+
+    def __getitem__(self, key):
+        if isinstance(key, tuple) and len(key) == 2:
+            i, j = key
+            full = slice(None)
+            if isinstance(j, slice) and j == full and np.isscalar(i):
+                return self.get_row(int(i))
+            if isinstance(i, slice) and i == full and np.isscalar(j):
+                return self.get_col(int(j))
+            if np.isscalar(i) and np.isscalar(j):
+                return float(self._M[int(i), int(j)])
+        return self._M[key]
+
+    def __eq__(self, other):
+        return self.todense() == np.asarray(other)
+
+    def __repr__(self):
+        kind = "dense" if self.dense else "sparse"
+        return f"ConnectionMatrix({kind}, shape={self.shape}, nnz={self.nnz})"
+
+
 def _dense_csr(W, nsrc, ntgt):
     """Every (i, j) as a synapse, zeros included: a dense connection matrix
     propagates and (with STDP) updates whole rows and columns
@@ -593,36 +752,20 @@ class Connection(_Tracked):
 
     def connect_random(self, source=None, target=None, p=0.1, weight=1.0, fixed=False, seed=None,
                        sparseness=None):
-        """Bernoulli(p) connectivity drawn on the host with numpy (the
-        reference's random_matrix stream is not reproduced)."""
+        """connection.py:319-361: Bernoulli(p) per pair (random_matrix), or
+        with ``fixed`` int(p*len(P)) distinct sources per target
+        (random_matrix_fixed_column); ``p`` and ``weight`` may be functions
+        of no arguments or of (i, j).  Drawn on the host with numpy (the
+        reference's Python/numpy random streams are not reproduced)."""
         P = source if source is not None else self.source
         Q = target if target is not None else self.target
         if sparseness is not None:
             p = sparseness
-        rng = np.random.default_rng(seed) if seed is not None else np.random
-        i0, j0 = self._origin(P, Q)
-        rows, cols = [], []
-        for a in range(0, len(P), 4096):
-            b = min(len(P), a + 4096)
-            if fixed:
-                k = int(p * len(Q))
-                for i in range(a, b):
-                    c = np.sort(rng.permutation(len(Q))[:k]) if hasattr(rng, "permutation") else None
-                    rows.append(np.full(k, i))
-                    cols.append(c)
-                continue
-            mask = rng.random((b - a, len(Q))) < p
-            r, c = np.nonzero(mask)
-            rows.append(r + a)
-            cols.append(c)
-        r = np.concatenate(rows) if rows else np.zeros(0, int)
-        c = np.concatenate(cols) if cols else np.zeros(0, int)
-        if callable(weight):
-            w = np.array([weight(i, j) for i, j in zip(r, c)], dtype=np.float64)
-        else:
-            w = np.full(len(r), float(weight))
-        M = sp.coo_matrix((w, (r + i0, c + j0)), shape=self._W.shape).tolil()
-        self._W = (self._W + M).tolil() if self._W.nnz else M
+        rng = np.random.default_rng(seed) if seed is not None else None
+        f = random_matrix_fixed_column if fixed else random_matrix
+        if not callable(weight):
+            weight = float(weight)
+        self.connect(P, Q, f(len(P), len(Q), p, value=weight, rng=rng))
 
     def connect_one_to_one(self, source=None, target=None, weight=1.0):
         P = source if source is not None else self.source
@@ -643,11 +786,14 @@ class Connection(_Tracked):
     # ---- matrix access
     @property
     def W(self):
+        """The weights as a ConnectionMatrix (connectionmatrix.py): after a
+        run they are read back from the device (STDP may have changed them)."""
         if self._net is not None and self._net.handle is not None and self.cid is not None:
             M = self._compiled_W.copy()
             M.data = self._net.handle.get_weights(self.cid, M.nnz)
-            return M
-        return self._W
+        else:
+            M = self._compiled_W if self._compiled_W is not None else self._pattern_csr()
+        return ConnectionMatrix(M, self._structure == "dense")
 
     def __getitem__(self, ij):
         return self.W[ij]
@@ -656,6 +802,7 @@ class Connection(_Tracked):
         if self._net is not None and self._net.handle is not None:
             raise RuntimeError("the connection matrix is frozen once the network runs")
         self._W[ij] = v
+        self._compiled_W = None
 
     def compress(self):
         if self._compiled_W is None:
@@ -883,9 +1030,11 @@ class SpikeMonitor(_Tracked):
         self._net = None
         self._base_count = 0
 
-    def _raster(self):
+    def _raster(self, all_spikes=False):
         net = self._net
-        if net is None or net.handle is None:
+        if net is None or net.handle is None or (self.function is not None and not all_spikes):
+            # with a function, the monitor's propagate is that function and
+            # nothing is stored (monitor.py:160-163)
             return np.zeros(0, np.int64), np.zeros(0, np.int64)
         owner = self.source._owner
         steps, idx = net.raster(owner)
@@ -1047,9 +1196,10 @@ class StateMonitor(_Tracked):
 
     @property
     def var(self):
+        """Unbiased: (sum x^2 - N mean^2) / (N - 1) (monitor.py:776-781)."""
         n, sm, sq = self._moments()
         m = sm / n
-        return sq / n - m * m
+        return (sq - n * m ** 2) / (n - 1)
 
     @property
     def std(self):
@@ -1068,18 +1218,26 @@ class NetworkOperation(_Tracked):
         self._track(1)
         self.function = function
         self.when = when
+        self.clock = clock or defaultclock
+        self._has_arg = _argcount(function) == 1
 
     def __call__(self):
-        return self.function()
+        # a one-argument function is passed the clock (network.py:714-721)
+        return self.function(self.clock) if self._has_arg else self.function()
 
 
 def network_operation(*args, **kw):
-    def deco(f):
-        op = NetworkOperation(f, **kw)
+    """@network_operation, @network_operation(clock), @network_operation(when=...)
+    (network.py:724-765)."""
+    if len(args) == 1 and callable(args[0]) and not isinstance(args[0], Clock) and not kw:
+        op = NetworkOperation(args[0])
         op._track(1)
         return op
-    if len(args) == 1 and callable(args[0]) and not kw:
-        op = NetworkOperation(args[0])
+    if args and isinstance(args[0], Clock):
+        kw.setdefault("clock", args[0])
+
+    def deco(f):
+        op = NetworkOperation(f, **kw)
         op._track(1)
         return op
     return deco
@@ -1254,15 +1412,27 @@ class Network:
             self._compile()
         state_mons = [m for m in self.monitors if isinstance(m, StateMonitor) and not m._record_false]
         _StopFlag.stop = False
+        fmons = [m for m in self.monitors if isinstance(m, SpikeMonitor) and m.function is not None]
+        first = self.steps_run
         if self.ops:
+            # one step per launch; operations run with the clock at the
+            # step's time (network.py:365-380): 'start' and 'before_groups'
+            # before the step, the others after it
+            early = [op for op in self.ops if op.when in ("start", "before_groups")]
+            late = [op for op in self.ops if op not in early]
             for _ in range(nsteps):
+                for op in early:
+                    op()
                 self.handle.run(1)
                 self.steps_run += 1
-                defaultclock._n += 1
                 for m in state_mons:
                     m._drain()
-                for op in self.ops:
+                if fmons:
+                    self._rasters = {}
+                    self._call_spike_functions(fmons, self.steps_run - 1, self.steps_run)
+                for op in late:
                     op()
+                defaultclock._n += 1
                 if _StopFlag.stop:
                     break
         else:
@@ -1283,7 +1453,24 @@ class Network:
             self.handle.sync()
             self.steps_run += nsteps
             defaultclock._n += nsteps
+            self._rasters = {}
+            if fmons:
+                self._call_spike_functions(fmons, first, self.steps_run)
         self._rasters = {}
+
+    def _call_spike_functions(self, fmons, s0, s1):
+        """SpikeMonitor(source, function=f): f(spikes) once per step, in step
+        order, with the step's ascending spike indices (monitor.py:160-163);
+        here after the steps have run on the device."""
+        for m in fmons:
+            steps, idx = m._raster(all_spikes=True)
+            sel = (steps >= s0) & (steps < s1)
+            steps, idx = steps[sel], idx[sel]
+            order = np.lexsort((idx, steps))
+            steps, idx = steps[order], idx[order]
+            bounds = np.searchsorted(steps, np.arange(s0, s1 + 1))
+            for k in range(s1 - s0):
+                m.function(idx[bounds[k]:bounds[k + 1]])
 
     def raster(self, owner):
         key = id(owner)
@@ -1296,7 +1483,36 @@ class Network:
         self._rasters.pop(id(owner), None)
 
     def reinit(self):
-        raise NotImplementedError("reinit: build a new Network")
+        """network.py:588-600: clock back to 0, groups to their initial
+        state, monitors and STDP traces cleared, spike generators restarted;
+        weights keep what plasticity made of them.  The device network is
+        released and built again at the next run."""
+        if self.handle is not None:
+            for c in self.conns:
+                if c.cid is not None and c._compiled_W is not None:
+                    M = c.W.tocsr()
+                    c._W = M.tolil()
+                c._compiled_W = None
+                c.cid = None
+                c._net = None
+            for st in self.stdps:
+                st.sid = None
+                st._net = None
+            for g in self.groups:
+                if hasattr(g, "_S0"):
+                    g._S = g._S0.copy()
+                g._net = None
+            self.handle.close()
+            self.handle = None
+        for m in self.monitors:
+            if isinstance(m, StateMonitor):
+                m.reinit()
+            m._net = None
+        self.gid = {}
+        self._rasters = {}
+        self.steps_run = 0
+        self._tuned = False
+        defaultclock.reinit()
 
     def stop(self):
         _StopFlag.stop = True

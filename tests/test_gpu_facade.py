@@ -179,6 +179,40 @@ def test_stdp_dense_structure_updates_zero_weights(structure):
         assert W[1, 0] == 0.0 and con.W.nnz == 1
 
 
+def test_connection_construction_two_runs():
+    """brian/tests/testinterface/test_connection.py:89-195: connect_full,
+    connect_one_to_one, element assignment; spikes of a generator reach
+    the right variables, and a second run continues from the first."""
+    from brian_amd import (SpikeGeneratorGroup, NeuronGroup, Connection, Network, msecond,
+                           reinit_default_clock)
+    reinit_default_clock()
+    eqs = """
+    da/dt = 0.*hertz : 1.
+    db/dt = 0.*hertz : 1.
+    """
+    G1 = SpikeGeneratorGroup(2, [(0, 1 * msecond), (1, 3 * msecond)])
+    G2 = NeuronGroup(2, model=eqs, threshold=10., reset=0.)
+    C = Connection(G1, G2)
+    C.connect_full(G1, G2, weight=2.)
+    assert all(C[i, j] == 2. for i in range(2) for j in range(2))
+    C = Connection(G1, G2)
+    C.connect_one_to_one(G1, G2)
+    assert all(C[i, j] == (1. if i == j else 0.) for i in range(2) for j in range(2))
+    Ca = Connection(G1, G2, 'a')
+    Cb = Connection(G1, G2, 'b')
+    Ca[0, 0] = 1.
+    Ca[0, 1] = 1.
+    Ca[1, 0] = 1.
+    Cb[0, 1] = 1.
+    Cb[1, 0] = 1.
+    Cb[1, 1] = 1.
+    net = Network(G1, G2, Ca, Cb)
+    net.run(2 * msecond)
+    assert G2.a[0] == 1. and G2.a[1] == 1. and G2.b[0] == 0. and G2.b[1] == 1.
+    net.run(2 * msecond)
+    assert G2.a[0] == 2. and G2.a[1] == 1. and G2.b[0] == 1. and G2.b[1] == 2.
+
+
 def test_facade_cuba_matches_oracle():
     """examples/misc/CUBA.py through the Brian API (fixed host connectivity)
     against the oracle: bit-exact raster in FP64 check mode."""

@@ -1,0 +1,103 @@
+"""brian/tests/testinterface/test_monitor.py (test_spikemonitor) and
+test_network.py restated through ``from brian_amd import *`` on the GPU."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _fresh():
+    import brian_amd as b
+    b.reinit()
+    b.set_global_preferences(precision=64, window=0)
+    yield
+
+
+def test_spikemonitor_interface():
+    """test_monitor.py:111-171: spikes of a generator, spiketimes, reinit,
+    the function form, and Network.reinit clearing a monitor that shares
+    its source with a connection."""
+    from brian_amd import (SpikeGeneratorGroup, SpikeMonitor, NeuronGroup, Connection, Network, ms,
+                           defaultclock, reinit_default_clock)
+    reinit_default_clock()
+    spikes = [(0, 3 * ms), (1, 4 * ms), (0, 7 * ms)]
+    G = SpikeGeneratorGroup(2, spikes, clock=defaultclock)
+    M = SpikeMonitor(G)
+    net = Network(G, M)
+    net.run(10 * ms)
+    assert M.nspikes == 3
+    for (mi, mt), (i, t) in zip(M.spikes, spikes):
+        assert mi == i and abs(mt - t) < 1e-12
+    assert len(M[0]) == len(M.spiketimes[0]) == 2 and len(M[1]) == len(M.spiketimes[1]) == 1
+    assert (M.spiketimes[0] == M[0]).all() and (M.spiketimes[1] == M[1]).all()
+    M.reinit()
+    assert M.nspikes == 0 and len(M.spikes) == 0
+    assert len(M[0]) == 0 and len(M[1]) == 0
+
+    f_spikes = []
+    calls = []
+
+    def f(sp):
+        calls.append(len(sp))
+        if len(sp):
+            f_spikes.extend(sp)
+    G = SpikeGeneratorGroup(2, spikes, clock=defaultclock)
+    M = SpikeMonitor(G, function=f)
+    net = Network(G, M)
+    reinit_default_clock()
+    net.run(10 * ms)
+    assert f_spikes == [0, 1, 0]
+    assert len(calls) == 100 and M.nspikes == 0
+
+    G = SpikeGeneratorGroup(2, spikes, clock=defaultclock)
+    G2 = NeuronGroup(1, model='dv/dt = -v / (5*ms) : 1')
+    C = Connection(G, G2, 'v', weight=0)
+    M = SpikeMonitor(G)
+    net = Network(G, G2, C, M)
+    reinit_default_clock()
+    net.run(10 * ms)
+    assert M.nspikes == 3
+    net.reinit()
+    assert M.nspikes == 0 and len(M.spikes) == 0
+    assert len(M[0]) == 0 and len(M[1]) == 0
+    # after reinit the network runs again from t = 0
+    net.run(10 * ms)
+    assert [i for i, _ in M.spikes] == [0, 1, 0]
+
+
+def test_statemonitor_with_start_operation():
+    """test_monitor.py:173-210: a network operation at 'start' sets V from
+    t >= 1 ms; StateMonitors record 0 first and the set values last;
+    record=False gives mean, var (unbiased estimator of the recorded
+    traces) and std; unrecorded neurons raise IndexError."""
+    from brian_amd import NeuronGroup, StateMonitor, Network, network_operation, ms, reinit_default_clock
+    reinit_default_clock()
+    G = NeuronGroup(3, model='dV/dt = 0*Hz : 1.', reset=0., threshold=10.)
+
+    @network_operation(when='start')
+    def f(clock):
+        if clock.t >= 1 * ms:
+            G.V = [1., 2., 3.]
+    M1 = StateMonitor(G, 'V')
+    M2 = StateMonitor(G, 'V', record=0)
+    M3 = StateMonitor(G, 'V', record=[0, 1])
+    M4 = StateMonitor(G, 'V', record=True)
+    net = Network(G, f, M1, M2, M3, M4)
+    net.run(2 * ms)
+    assert abs(M2[0][0]) < 1e-15 and abs(M2[0][-1] - 1.) < 1e-15
+    assert abs(M3[1][0]) < 1e-15 and abs(M3[1][-1] - 2.) < 1e-15
+    assert abs(M4[2][0]) < 1e-15 and abs(M4[2][-1] - 3.) < 1e-15
+    for M, i in ((M1, 0), (M2, 1), (M3, 2), (M4, 3)):
+        with pytest.raises(IndexError):
+            M[i]
+    for M in (M3, M4):
+        assert np.max(np.abs(M.times - M2.times)) == 0.0
+    assert M2.times[0] == 0.0
+    d = np.diff(M2.times)
+    assert abs(d.max() - d.min()) < 1e-15 and abs(d.max() - 1e-4) < 1e-15
+    v = np.array([np.var(M4[0]), np.var(M4[1]), np.var(M4[2])]) * len(M4[0]) / (len(M4[0]) - 1)
+    m = np.array([0.5, 1.0, 1.5])
+    assert np.max(np.abs(M1.mean - m)) < 1e-12
+    assert np.max(np.abs(M1.var - v)) < 1e-12
+    assert np.max(np.abs(M1.std - v ** 0.5)) < 1e-12
