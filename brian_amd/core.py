@@ -31,7 +31,7 @@ __all__ = [
     "SparseConnectionVector", "random_matrix", "random_matrix_fixed_column",
     "SpikeMonitor", "PopulationRateMonitor", "PopulationSpikeCounter", "SpikeCounter",
     "StateMonitor", "Network", "MagicNetwork", "run", "reinit", "stop", "network_operation",
-    "NetworkOperation", "set_global_preferences", "get_global_preference", "forget",
+    "NetworkOperation", "set_global_preferences", "get_global_preference", "forget", "recall",
 ]
 
 _PREFS = {"precision": 64, "window": 0, "neurons_per_thread": 0, "use_graph": True,
@@ -74,9 +74,24 @@ class _Tracked:
         _Tracked._instances.append(weakref.ref(self))
 
 
+def _flatten(objs):
+    for o in objs:
+        if isinstance(o, (list, tuple, set)):
+            yield from _flatten(o)
+        else:
+            yield o
+
+
 def forget(*objs):
-    ids = set(id(o) for o in objs)
-    _Tracked._instances = [r for r in _Tracked._instances if r() is not None and id(r()) not in ids]
+    """magic.py:246-263: the objects are no longer found by run() / MagicNetwork."""
+    for o in _flatten(objs):
+        o._forgotten = True
+
+
+def recall(*objs):
+    """magic.py:265-279: undo forget()."""
+    for o in _flatten(objs):
+        o._forgotten = False
 
 
 # ------------------------------------------------------------------ clock
@@ -1155,6 +1170,24 @@ class StateMonitor(_Tracked):
         self._tblocks = []
         self._net = None
         self.mid = None
+        self._host = False          # sampled on the host after 'end' operations
+        self._hsum = self._hsq = None
+        self._hn = 0
+
+    def _sample_host(self, step):
+        """One sample after the step's 'end' operations (monitor.py:744-770)."""
+        P = self.P
+        full = P._get_state(self._vidx, P._origin, P.N)
+        if self._record_false:
+            if self._hsum is None:
+                self._hsum, self._hsq = np.zeros(P.N), np.zeros(P.N)
+            self._hsum += full
+            self._hsq += full * full
+            self._hn += 1
+            return
+        if step % self.timestep == 0:
+            self._blocks.append(np.asarray(full)[self.record][None, :])
+            self._tblocks.append(np.array([(self._net._n0 + step) * defaultclock.dt]))
 
     def _drain(self):
         net = self._net
@@ -1182,6 +1215,10 @@ class StateMonitor(_Tracked):
 
     def _moments(self):
         net = self._net
+        if self._host:
+            if not self._hn:
+                raise RuntimeError("the monitor has not run")
+            return self._hn, self._hsum, self._hsq
         if net is None or self.mid is None:
             raise RuntimeError("the monitor has not run")
         owner = self.P._owner
@@ -1207,6 +1244,8 @@ class StateMonitor(_Tracked):
 
     def reinit(self):
         self._blocks, self._tblocks = [], []
+        self._hsum = self._hsq = None
+        self._hn = 0
 
 
 # ------------------------------------------------------------------ network
@@ -1243,6 +1282,47 @@ def network_operation(*args, **kw):
     return deco
 
 
+class _Reporter:
+    """Progress text of Network.run(report=...) (network.py:541-586 with
+    utils/progressreporting.py): 'text', 'print' or 'stdout' write to
+    standard output, 'stderr' to standard error, a file-like object is
+    written to; a line at the start, every report_period of wall time
+    (checked between device launches) and at the end."""
+
+    def __init__(self, report, period, total):
+        import sys
+        import time
+        self._time = time.perf_counter
+        self.out = None
+        if report in ("text", "print", "stdout"):
+            self.out = sys.stdout
+        elif report == "stderr":
+            self.out = sys.stderr
+        elif report is not None and hasattr(report, "write"):
+            self.out = report
+        self.period = period if self.out is not None else float("inf")
+        self.total = total
+        self.t0 = self._time()
+        self.last = self.t0
+        if self.out is not None:
+            self.out.write(f"0% complete, 0s elapsed\n")
+
+    def update(self, done):
+        if self.out is None:
+            return
+        now = self._time()
+        if now - self.last >= self.period:
+            self.last = now
+            frac = done / self.total if self.total else 1.0
+            el = now - self.t0
+            rem = el / frac - el if frac > 0 else 0.0
+            self.out.write(f"{int(100 * frac)}% complete, {el:.1f}s elapsed, approximately {rem:.1f}s remaining\n")
+
+    def finish(self):
+        if self.out is not None:
+            self.out.write(f"100% complete, {self._time() - self.t0:.1f}s elapsed\n")
+
+
 class _StopFlag:
     stop = False
 
@@ -1258,6 +1338,7 @@ class Network:
 
     def __init__(self, *args, **kw):
         self.groups, self.conns, self.stdps, self.monitors, self.ops = [], [], [], [], []
+        self._seq = []          # operations and state monitors in the order added
         self.handle = None
         self.gid = {}
         self.steps_run = 0
@@ -1286,16 +1367,46 @@ class Network:
             elif isinstance(o, (SpikeMonitor, StateMonitor)):
                 if o not in self.monitors:
                     self.monitors.append(o)
+                    self._seq.append(o)
             elif isinstance(o, NetworkOperation):
-                self.ops.append(o)
+                if o not in self.ops:
+                    self.ops.append(o)
+                    self._seq.append(o)
             elif isinstance(o, Network):
-                self.add(o.groups, o.conns, o.stdps, o.monitors, o.ops)
+                self.add(o.groups, o.conns, o.stdps, o._seq)
+            else:
+                raise TypeError("Only the following types of objects can be added to a network: "
+                                "NeuronGroup, Connection or NetworkOperation")
         return self
+
+    def __call__(self, obj):
+        """network.py:211-216: add ``obj`` and return it."""
+        self.add(obj)
+        return obj
+
+    def __len__(self):
+        """Total number of neurons (network.py:630-635)."""
+        return sum(len(g) for g in self.groups)
+
+    def same_clocks(self):
+        """network.py:451-465: whether every group runs on one clock."""
+        clocks = set(id(g.clock) for g in self.groups)
+        return len(clocks) <= 1
 
     # ---- compilation
     def _compile(self):
         prec = self.precision or _PREFS["precision"]
         dt = defaultclock.dt
+        # objects that ran in another network continue from where they are
+        # there: state and (plastic) weights come back from that device
+        for g in self.groups:
+            old = getattr(g, "_net", None)
+            if old is not None and old is not self and old.handle is not None and hasattr(g, "_S0"):
+                g._S = np.array([g._get_state(j, 0, g.N) for j in range(len(g.var_names))]).reshape(g._S.shape)
+        for c in self.conns:
+            if c._net is not None and c._net is not self and c._net.handle is not None and c.cid is not None:
+                c._W = c.W.tocsr().tolil()
+                c._compiled_W = None
         for g in self.groups:
             if abs(g.clock.dt - dt) > 1e-15:
                 raise NotImplementedError("all objects must share one clock")
@@ -1355,8 +1466,18 @@ class Network:
                 self._capacity[id(g)] = cap
                 h.record(self.gid[id(g)], cap)
         self._n0 = defaultclock._n
+        # a StateMonitor added after an operation of the 'end' slots samples
+        # on the host after that operation, in the order they were added
+        # (both run at 'end', network.py:365-380)
+        late_seen = False
+        for o in self._seq:
+            if isinstance(o, NetworkOperation) and o.when not in ("start", "before_groups"):
+                late_seen = True
+            elif isinstance(o, StateMonitor):
+                o._host = late_seen
         for m in self.monitors:
-            if not isinstance(m, StateMonitor):
+            if not isinstance(m, StateMonitor) or m._host:
+                m._net = self if isinstance(m, StateMonitor) else m._net
                 continue
             gid = self.gid[id(m.P._owner)]
             m._net = self
@@ -1407,10 +1528,12 @@ class Network:
     def run(self, duration, report=None, report_period=10 * second, threads=1):
         dt = defaultclock.dt
         nsteps = int(np.ceil(float(duration) / dt))    # clock.py:203-206
+        rep = _Reporter(report, float(report_period), nsteps * dt)
         if self.handle is None:
             self._planned_steps = max(nsteps, 1)
             self._compile()
-        state_mons = [m for m in self.monitors if isinstance(m, StateMonitor) and not m._record_false]
+        state_mons = [m for m in self.monitors
+                      if isinstance(m, StateMonitor) and not m._record_false and not m._host]
         _StopFlag.stop = False
         fmons = [m for m in self.monitors if isinstance(m, SpikeMonitor) and m.function is not None]
         first = self.steps_run
@@ -1419,7 +1542,8 @@ class Network:
             # step's time (network.py:365-380): 'start' and 'before_groups'
             # before the step, the others after it
             early = [op for op in self.ops if op.when in ("start", "before_groups")]
-            late = [op for op in self.ops if op not in early]
+            late = [o for o in self._seq if (isinstance(o, NetworkOperation) and o not in early)
+                    or (isinstance(o, StateMonitor) and o._host)]
             for _ in range(nsteps):
                 for op in early:
                     op()
@@ -1430,9 +1554,13 @@ class Network:
                 if fmons:
                     self._rasters = {}
                     self._call_spike_functions(fmons, self.steps_run - 1, self.steps_run)
-                for op in late:
-                    op()
+                for o in late:
+                    if isinstance(o, StateMonitor):
+                        o._sample_host(self.steps_run - 1)
+                    else:
+                        o()
                 defaultclock._n += 1
+                rep.update((self.steps_run - first) * dt)
                 if _StopFlag.stop:
                     break
         else:
@@ -1444,12 +1572,15 @@ class Network:
                 self._tuned = True
             # state monitor buffers hold _cap samples: drain at least that often
             chunk = min([(m._cap - 1) * m.timestep for m in state_mons] + [nsteps - done]) or 1
+            if rep.period < float("inf"):
+                chunk = max(1, min(chunk, int(rep.period / dt) or 1))
             while done < nsteps:
                 k = min(chunk, nsteps - done)
                 self.handle.run(k)
                 done += k
                 for m in state_mons:
                     m._drain()
+                rep.update(done * dt)
             self.handle.sync()
             self.steps_run += nsteps
             defaultclock._n += nsteps
@@ -1457,6 +1588,7 @@ class Network:
             if fmons:
                 self._call_spike_functions(fmons, first, self.steps_run)
         self._rasters = {}
+        rep.finish()
 
     def _call_spike_functions(self, fmons, s0, s1):
         """SpikeMonitor(source, function=f): f(spikes) once per step, in step
@@ -1482,11 +1614,20 @@ class Network:
         self.handle.clear_spikes(self.gid[id(owner)])
         self._rasters.pop(id(owner), None)
 
-    def reinit(self):
-        """network.py:588-600: clock back to 0, groups to their initial
-        state, monitors and STDP traces cleared, spike generators restarted;
+    def reinit(self, states=True):
+        """network.py:291-310: clock back to 0, groups to their initial
+        state (unless ``states`` is False: they keep their current values),
+        monitors and STDP traces cleared, spike generators restarted;
         weights keep what plasticity made of them.  The device network is
         released and built again at the next run."""
+        if not states and self.handle is not None:
+            for g in self.groups:
+                if hasattr(g, "_S0"):
+                    g._S = np.array([g._get_state(j, 0, g.N) for j in range(len(g.var_names))]).reshape(
+                        g._S.shape)
+            keep = {id(g): g._S.copy() for g in self.groups if hasattr(g, "_S0")}
+        else:
+            keep = {}
         if self.handle is not None:
             for c in self.conns:
                 if c.cid is not None and c._compiled_W is not None:
@@ -1500,7 +1641,7 @@ class Network:
                 st._net = None
             for g in self.groups:
                 if hasattr(g, "_S0"):
-                    g._S = g._S0.copy()
+                    g._S = keep.get(id(g), g._S0).copy()
                 g._net = None
             self.handle.close()
             self.handle = None
@@ -1512,6 +1653,10 @@ class Network:
         self._rasters = {}
         self.steps_run = 0
         self._tuned = False
+        if states and not keep:
+            for g in self.groups:
+                if hasattr(g, "_S0"):
+                    g._S = g._S0.copy()
         defaultclock.reinit()
 
     def stop(self):
@@ -1526,7 +1671,7 @@ class MagicNetwork(Network):
         frame = inspect.currentframe()
         try:
             f = frame
-            for _ in range(level + 1):
+            for _ in range(level):          # level 1: the caller's frame
                 f = f.f_back
             fid = id(f)
         finally:
@@ -1534,8 +1679,10 @@ class MagicNetwork(Network):
         objs = []
         for r in _Tracked._instances:
             o = r()
-            if o is not None and getattr(o, "_frame_id", None) == fid:
+            if o is not None and getattr(o, "_frame_id", None) == fid and not getattr(o, "_forgotten", False):
                 objs.append(o)
+        if verbose:
+            print(f"MagicNetwork: {len(objs)} objects")
         Network.__init__(self, *objs)
 
 
@@ -1549,7 +1696,8 @@ def run(duration, threads=1, report=None, report_period=10 * second):
     fid = id(frame)
     del frame
     objs = [r() for r in _Tracked._instances]
-    objs = [o for o in objs if o is not None and getattr(o, "_frame_id", None) == fid]
+    objs = [o for o in objs if o is not None and getattr(o, "_frame_id", None) == fid
+            and not getattr(o, "_forgotten", False)]
     key = (fid, tuple(sorted(id(o) for o in objs)))
     net = _magic_cache.get(key)
     if net is None:
@@ -1561,6 +1709,10 @@ def run(duration, threads=1, report=None, report_period=10 * second):
 
 
 def reinit(states=True):
-    """Forget the magic network; the next run() rebuilds it."""
+    """network.py:941-957: reinitialise the objects run() found (states
+    too unless ``states`` is False) and the clock; the next run() builds
+    the network again."""
+    for net in list(_magic_cache.values()):
+        net.reinit(states)
     _magic_cache.clear()
     reinit_default_clock()

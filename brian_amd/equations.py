@@ -21,6 +21,8 @@ import inspect
 import math
 import re
 
+import warnings
+
 import numpy as np
 import scipy.linalg
 
@@ -210,7 +212,11 @@ class Equations:
             for j in range(n):
                 M[j, i] = fi[j] + AB[j, 0]
         try:
-            with np.errstate(all="ignore"):
+            # a singular M (a parameter among the variables) falls back to the
+            # numerical solution, as LinAlgError does in the reference; scipy
+            # may report it as NaN from its diagonal fast path instead
+            with np.errstate(all="ignore"), warnings.catch_warnings():
+                warnings.simplefilter("ignore", scipy.linalg.LinAlgWarning)
                 B = scipy.linalg.solve(M, AB)
             if not np.all(np.isfinite(B)):
                 raise np.linalg.LinAlgError("singular matrix")

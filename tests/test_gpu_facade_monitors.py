@@ -101,3 +101,87 @@ def test_statemonitor_with_start_operation():
     assert np.max(np.abs(M1.mean - m)) < 1e-12
     assert np.max(np.abs(M1.var - v)) < 1e-12
     assert np.max(np.abs(M1.std - v ** 0.5)) < 1e-12
+
+
+def test_network_generation_and_reporting(capsys):
+    """test_network.py:11-104: len(net) counts neurons, adding twice is a
+    no-op, other objects raise TypeError, net(G) returns G, MagicNetwork
+    with forget/recall; run(report=...) writes progress to the stream asked."""
+    import io
+    from brian_amd import NeuronGroup, Network, MagicNetwork, forget, recall, msecond, defaultclock
+    G = NeuronGroup(42, model="v : 1")
+    net = Network(G)
+    assert len(net) == 42
+    net = Network()
+    net.add(G)
+    net.add(G)
+    assert len(net) == 42
+    with pytest.raises(TypeError):
+        net.add(1 * msecond)
+    net = Network()
+    assert net(G) is G and len(net) == 42
+    assert len(MagicNetwork()) == 42
+    forget(G)
+    assert len(MagicNetwork()) == 0
+    recall(G)
+    assert len(MagicNetwork()) == 42
+    forget([G])
+    assert len(MagicNetwork()) == 0
+    recall([G])
+    assert len(MagicNetwork()) == 42
+    net = Network(G)
+    net.run(defaultclock.dt, report="text")
+    assert len(capsys.readouterr().out) > 0
+    net.run(defaultclock.dt, report="stderr")
+    cap = capsys.readouterr()
+    assert len(cap.out) == 0 and len(cap.err) > 0
+    buf = io.StringIO()
+    net.run(defaultclock.dt, report=buf)
+    assert len(buf.getvalue()) > 0
+
+
+def test_network_reinit_states():
+    """test_network.py:106-126: reinit(False) resets only the clock;
+    reinit(True) also the state variables (to their initial values)."""
+    from brian_amd import NeuronGroup, Network, defaultclock, ms, reinit_default_clock
+    reinit_default_clock()
+    G = NeuronGroup(1, model="dv/dt = -v / (1 * ms) : 1\np : 1")
+    G.v = 0.1
+    G.p = 1
+    net = Network(G)
+    net.run(defaultclock.dt)
+    assert G.v > 0 and G.p == 1
+    net.reinit(False)
+    assert defaultclock.t == 0
+    assert G.v > 0 and G.p == 1
+    net.run(defaultclock.dt)
+    net.reinit(True)
+    assert defaultclock.t == 0
+    assert G.v == 0 and G.p == 0
+
+
+def test_network_operation_order():
+    """test_network.py:172-198: an 'end' operation that holds v at 1 runs
+    before a StateMonitor added after it, which then records 1.0; with an
+    operation that does nothing, v decays."""
+    from brian_amd import NeuronGroup, StateMonitor, Network, network_operation, defaultclock, \
+        reinit_default_clock
+    reinit_default_clock()
+    G = NeuronGroup(1, model="dv/dt = -v / (1 * ms) : 1")
+    G.v = 1
+
+    @network_operation(when="end")
+    def keep_voltage(clock):
+        G.v = 1
+    mon = StateMonitor(G, "v", record=True)
+    net = Network(G, keep_voltage, mon)
+    net.run(5 * defaultclock.dt)
+    assert len(mon[0]) == 5 and (mon[0] == 1.0).all()
+    mon.reinit()
+
+    @network_operation()
+    def do_nothing(clock):
+        pass
+    net = Network(G, do_nothing, mon)
+    net.run(5 * defaultclock.dt)
+    assert len(mon[0]) == 5 and (mon[0] < 1.0).all()
