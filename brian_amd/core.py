@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import inspect
 import re
+import warnings
 import weakref
 
 import numpy as np
@@ -26,7 +27,7 @@ __all__ = [
     "Clock", "defaultclock", "reinit_default_clock", "get_default_clock",
     "Threshold", "VariableThreshold", "EmpiricalThreshold", "NoThreshold", "StringThreshold",
     "Reset", "VariableReset", "StringReset", "Refractoriness", "NoReset",
-    "NeuronGroup", "SpikeGeneratorGroup", "PoissonGroup", "TimedArray",
+    "NeuronGroup", "SpikeGeneratorGroup", "PulsePacket", "PoissonGroup", "TimedArray",
     "Connection", "DelayConnection", "STDP", "ExponentialSTDP", "STP", "ConnectionMatrix",
     "SparseConnectionVector", "random_matrix", "random_matrix_fixed_column",
     "SpikeMonitor", "PopulationRateMonitor", "PopulationSpikeCounter", "SpikeCounter",
@@ -450,7 +451,16 @@ class _SubGroup(NeuronGroup):
 
 class SpikeGeneratorGroup(_GroupBase):
     """brian/directcontrol.py:63-215 with FastSpikeGeneratorThreshold
-    (:219-268): spikes (i, t) are emitted at step ceil(t/dt)."""
+    (:219-268): a spike (i, t) is emitted at step ceil(t/dt).
+
+    ``spiketimes``: a list of (i, t) with i an index or a list of indices,
+    a tuple (indices, times), an array of rows [i, t] (times in seconds), a
+    callable returning one of these or an iterator (called again when the
+    network is reinitialised), or an iterator of (i, t) sorted in time
+    (drawn as far as each run needs).  ``period`` repeats the spikes: at
+    step n the spikes of bin int(round((t - int(t/period) period)/dt)),
+    t = n dt, are emitted.  Spikes of one neuron falling into one step are
+    emitted once."""
 
     def __init__(self, N, spiketimes, clock=None, period=None, sort=True, gather=False, level=0):
         self._track(level + 1)
@@ -459,21 +469,140 @@ class SpikeGeneratorGroup(_GroupBase):
         self._origin = 0
         self._next_sub = 0
         self.clock = clock or defaultclock
-        if period is not None:
-            raise NotImplementedError("periodic spike generators are not supported")
-        if isinstance(spiketimes, tuple) and len(spiketimes) == 2 and not np.isscalar(spiketimes[0]):
-            I, T = np.asarray(spiketimes[0]), np.asarray(spiketimes[1], dtype=np.float64)
-        else:
-            st = list(spiketimes)
-            I = np.array([int(i) for i, _ in st], dtype=np.int64)
-            T = np.array([float(t) for _, t in st], dtype=np.float64)
-        steps = np.array(np.ceil(T / self.clock.dt), dtype=np.int64)
-        order = np.lexsort((I, steps))
-        self._steps = steps[order]
-        self._idx = I[order]
+        self.period = None if period is None else float(period)
         self._net = None
         self.var_index = {}
         self.var_names = []
+        self.spiketimes = spiketimes
+
+    # ---- spike containers
+    @property
+    def spiketimes(self):
+        return self._spec
+
+    @spiketimes.setter
+    def spiketimes(self, value):
+        self._spec = value
+        self._start_stream()
+
+    def _start_stream(self):
+        """(Re)start reading the container; finite ones are read at once."""
+        src = self._spec
+        if callable(src) and not hasattr(src, "__next__"):
+            src = src()
+        self._pending = None
+        if isinstance(src, np.ndarray):
+            I, T = src[:, 0], src[:, 1]
+            self._iter = None
+        elif isinstance(src, tuple) and len(src) == 2 and not np.isscalar(src[0]):
+            I, T = np.asarray(src[0]), np.asarray(src[1], dtype=np.float64)
+            self._iter = None
+        elif isinstance(src, (list, tuple)):
+            I, T = self._flatten(src)
+            self._iter = None
+        else:                                    # an iterator: read on demand
+            self._iter = iter(src)
+            I, T = np.zeros(0), np.zeros(0)
+        self._bins = np.array(np.ceil(np.asarray(T, dtype=np.float64) / self.clock.dt), dtype=np.int64)
+        self._bin_idx = np.asarray(I, dtype=np.int64)
+
+    @staticmethod
+    def _flatten(items):
+        I, T = [], []
+        for i, t in items:
+            if np.isscalar(i):
+                I.append(int(i))
+                T.append(float(t))
+            else:
+                for k in np.asarray(i).ravel():
+                    I.append(int(k))
+                    T.append(float(t))
+        return np.array(I, dtype=np.int64), np.array(T, dtype=np.float64)
+
+    def _read_until(self, bin_limit):
+        """Draw iterator items whose bin is below ``bin_limit``."""
+        if self._iter is None:
+            return
+        I, B = [], []
+        while True:
+            item = self._pending
+            if item is None:
+                try:
+                    item = next(self._iter)
+                except StopIteration:
+                    self._iter = None
+                    break
+            i, t = item
+            b = int(np.ceil(float(t) / self.clock.dt))
+            if b >= bin_limit:
+                self._pending = item
+                break
+            self._pending = None
+            for k in (np.atleast_1d(np.asarray(i)).ravel()):
+                I.append(int(k))
+                B.append(b)
+        if I:
+            self._bins = np.concatenate([self._bins, np.array(B, dtype=np.int64)])
+            self._bin_idx = np.concatenate([self._bin_idx, np.array(I, dtype=np.int64)])
+
+    @property
+    def _open(self):
+        """More spikes may follow beyond what has been read."""
+        return self._iter is not None or self.period is not None
+
+    def _table(self, n0, n1):
+        """(steps - n0, indices) of the spikes emitted at clock steps
+        [n0, n1), sorted by (step, index), one spike per neuron and step."""
+        dt = self.clock.dt
+        if self.period is None:
+            self._read_until(n1)
+            steps, idx = self._bins, self._bin_idx
+        else:
+            nb = int(np.ceil(self.period / dt)) + 1
+            self._read_until(nb)
+            n = np.arange(n0, n1, dtype=np.int64)
+            t = n * dt + 0.0
+            cp = np.floor(t / self.period)
+            b = np.array(np.round((t - cp * self.period) / dt), dtype=np.int64)
+            order = np.argsort(self._bins, kind="stable")
+            bins, bidx = self._bins[order], self._bin_idx[order]
+            lo = np.searchsorted(bins, b, side="left")
+            hi = np.searchsorted(bins, b, side="right")
+            cnt = hi - lo
+            steps = np.repeat(n, cnt)
+            pos = np.repeat(lo - np.concatenate(([0], np.cumsum(cnt)[:-1])), cnt) + np.arange(cnt.sum())
+            idx = bidx[pos] if len(pos) else np.zeros(0, np.int64)
+        keep = (steps >= n0) & (steps < n1)
+        steps, idx = steps[keep] - n0, idx[keep]
+        if len(idx) and (idx.min() < 0 or idx.max() >= self.N):
+            raise ValueError("spike generator index out of range")
+        key = np.unique(steps * np.int64(self.N) + idx)
+        if len(key) < len(steps):
+            warnings.warn("several spikes of one neuron in one time step: the extra ones are discarded")
+        return (key // self.N).astype(np.int32), (key % self.N).astype(np.int32)
+
+    def reinit(self):
+        self._start_stream()
+
+
+class PulsePacket(SpikeGeneratorGroup):
+    """directcontrol.py:355-406: ``n`` neurons fire once each, at times drawn
+    from a Gaussian of mean ``t`` and standard deviation ``sigma`` (numpy
+    draws; the reference uses Python's random module)."""
+
+    def __init__(self, t, n, sigma, clock=None):
+        self._t, self._sigma = float(t), float(sigma)
+        SpikeGeneratorGroup.__init__(self, int(n), self._draw(int(n)), clock=clock, level=1)
+
+    def _draw(self, n):
+        times = np.sort(np.random.normal(self._t, self._sigma, n)) if self._sigma > 0 else np.full(n, self._t)
+        neurons = np.random.permutation(n)
+        return list(zip(neurons.tolist(), times.tolist()))
+
+    def generate(self, t, n, sigma):
+        self._t, self._sigma = float(t), float(sigma)
+        self.N = int(n)
+        self.spiketimes = self._draw(self.N)
 
 
 class PoissonGroup(_GroupBase):
@@ -1187,7 +1316,7 @@ class StateMonitor(_Tracked):
             return
         if step % self.timestep == 0:
             self._blocks.append(np.asarray(full)[self.record][None, :])
-            self._tblocks.append(np.array([(self._net._n0 + step) * defaultclock.dt]))
+            self._tblocks.append(np.array([(self._net._n0 + step) * self._net.clock.dt]))
 
     def _drain(self):
         net = self._net
@@ -1197,7 +1326,7 @@ class StateMonitor(_Tracked):
         if len(vals):
             self._blocks.append(vals)
             steps = first + np.arange(len(vals)) * self.timestep
-            self._tblocks.append((net._n0 + steps) * defaultclock.dt)
+            self._tblocks.append((net._n0 + steps) * net.clock.dt)
 
     @property
     def values(self):
@@ -1393,10 +1522,21 @@ class Network:
         clocks = set(id(g.clock) for g in self.groups)
         return len(clocks) <= 1
 
+    def _the_clock(self):
+        """The one clock of the network's groups (network.py:451-465)."""
+        owners = list(self.groups)
+        owners += [P for c in self.conns for P in (c.source._owner, c.target._owner)]
+        owners += [m.source._owner if isinstance(m, SpikeMonitor) else m.P._owner for m in self.monitors]
+        clocks = {id(g.clock): g.clock for g in owners}
+        if len(clocks) > 1:
+            raise NotImplementedError("all objects must share one clock")
+        return next(iter(clocks.values())) if clocks else defaultclock
+
     # ---- compilation
     def _compile(self):
         prec = self.precision or _PREFS["precision"]
-        dt = defaultclock.dt
+        self.clock = self._the_clock()
+        dt = self.clock.dt
         # objects that ran in another network continue from where they are
         # there: state and (plastic) weights come back from that device
         for g in self.groups:
@@ -1407,9 +1547,6 @@ class Network:
             if c._net is not None and c._net is not self and c._net.handle is not None and c.cid is not None:
                 c._W = c.W.tocsr().tolil()
                 c._compiled_W = None
-        for g in self.groups:
-            if abs(g.clock.dt - dt) > 1e-15:
-                raise NotImplementedError("all objects must share one clock")
         for c in self.conns:
             for P in (c.source._owner, c.target._owner):
                 if P not in self.groups:
@@ -1420,10 +1557,15 @@ class Network:
                 self.groups.append(src)
         h = _lib.NetHandle(dt, prec)
         self.handle = h
+        self._n0 = self.clock._n
+        self._gen_horizon = self._n0 + self._planned_steps
         state_mon = any(isinstance(m, StateMonitor) for m in self.monitors)
         for g in self.groups:
             if isinstance(g, SpikeGeneratorGroup):
-                gid = h.add_generator(g.N, g._steps, g._idx)
+                # finite sources go to the device whole; open ones (iterators,
+                # periods) up to the run's end, then extended run by run
+                n1 = self._gen_horizon if g._open else max(self._n0, int(g._bins.max()) + 1 if len(g._bins) else 0)
+                gid = h.add_generator(g.N, *g._table(self._n0, n1))
             elif isinstance(g, PoissonGroup):
                 gid = h.add_poisson(g.N, g.rates, g.seed)
             else:
@@ -1465,7 +1607,6 @@ class Network:
                 cap = max(1 << 20, int(g.N * self._planned_steps * dt * _PREFS["max_rate_hz"]))
                 self._capacity[id(g)] = cap
                 h.record(self.gid[id(g)], cap)
-        self._n0 = defaultclock._n
         # a StateMonitor added after an operation of the 'end' slots samples
         # on the host after that operation, in the order they were added
         # (both run at 'end', network.py:365-380)
@@ -1526,12 +1667,15 @@ class Network:
 
     # ---- running
     def run(self, duration, report=None, report_period=10 * second, threads=1):
-        dt = defaultclock.dt
+        clk = self._the_clock() if self.handle is None else self.clock
+        dt = clk.dt
         nsteps = int(np.ceil(float(duration) / dt))    # clock.py:203-206
         rep = _Reporter(report, float(report_period), nsteps * dt)
         if self.handle is None:
             self._planned_steps = max(nsteps, 1)
             self._compile()
+        else:
+            self._extend_generators(clk._n + nsteps)
         state_mons = [m for m in self.monitors
                       if isinstance(m, StateMonitor) and not m._record_false and not m._host]
         _StopFlag.stop = False
@@ -1559,7 +1703,7 @@ class Network:
                         o._sample_host(self.steps_run - 1)
                     else:
                         o()
-                defaultclock._n += 1
+                clk._n += 1
                 rep.update((self.steps_run - first) * dt)
                 if _StopFlag.stop:
                     break
@@ -1583,7 +1727,7 @@ class Network:
                 rep.update(done * dt)
             self.handle.sync()
             self.steps_run += nsteps
-            defaultclock._n += nsteps
+            clk._n += nsteps
             self._rasters = {}
             if fmons:
                 self._call_spike_functions(fmons, first, self.steps_run)
@@ -1603,6 +1747,16 @@ class Network:
             bounds = np.searchsorted(steps, np.arange(s0, s1 + 1))
             for k in range(s1 - s0):
                 m.function(idx[bounds[k]:bounds[k + 1]])
+
+    def _extend_generators(self, n1):
+        """Spike generators whose spikes continue (iterators, periods) get
+        their device table extended to clock step n1."""
+        if n1 <= self._gen_horizon:
+            return
+        for g in self.groups:
+            if isinstance(g, SpikeGeneratorGroup) and g._open:
+                self.handle.set_generator_spikes(self.gid[id(g)], *g._table(self._n0, n1))
+        self._gen_horizon = n1
 
     def raster(self, owner):
         key = id(owner)
@@ -1657,6 +1811,10 @@ class Network:
             for g in self.groups:
                 if hasattr(g, "_S0"):
                     g._S = g._S0.copy()
+        for g in self.groups:
+            if isinstance(g, SpikeGeneratorGroup):
+                g.reinit()
+        getattr(self, "clock", defaultclock).reinit()
         defaultclock.reinit()
 
     def stop(self):

@@ -185,6 +185,31 @@ void upload_raw(DevBuf &buf, const R *src, size_t n, hipStream_t st) {
   }
 }
 
+// Spike table of a generator group on the device: per-step offsets into
+// the (step, index)-sorted indices (FastSpikeGeneratorThreshold's offsets,
+// directcontrol.py:219-250).
+void upload_generator(bm_net *net, HostGroup &g) {
+  const int64_t nst = g.gen_steps.empty() ? 0 : (int64_t)g.gen_steps.back() + 1;
+  std::vector<int64_t> off(nst + 1, 0);
+  for (int32_t st : g.gen_steps) off[st + 1]++;
+  for (int64_t k = 0; k < nst; ++k) off[k + 1] += off[k];
+  upload_raw(g.gen_off_d, off.data(), off.size(), net->stream);
+  upload_raw(g.gen_idx_d, g.gen_idx.data(), g.gen_idx.size(), net->stream);
+  g.dev.gen_idx = g.gen_idx_d.as<int32_t>();
+  g.dev.gen_off = g.gen_off_d.as<int64_t>();
+  g.dev.gen_nsteps = nst;
+  g.gen_nsteps = nst;
+}
+
+void check_generator_spikes(int64_t N, int64_t nspikes, const int32_t *steps, const int32_t *idx) {
+  for (int64_t q = 0; q < nspikes; ++q) {
+    BM_REQUIRE(idx[q] >= 0 && idx[q] < N, "generator index out of range");
+    BM_REQUIRE(steps[q] >= 0, "generator step must be >= 0");
+    if (q) BM_REQUIRE(steps[q] > steps[q - 1] || (steps[q] == steps[q - 1] && idx[q] > idx[q - 1]),
+                      "generator spikes must be sorted by (step, index) without duplicates");
+  }
+}
+
 HostGroup &grp(bm_net *net, int gid) {
   BM_REQUIRE(gid >= 0 && gid < (int)net->groups.size(), "bad group id");
   return *net->groups[gid];
@@ -763,18 +788,7 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     } else {
       d.ti_var = -1;
     }
-    if (g.kind == GK_GENERATOR) {
-      int64_t nst = g.gen_steps.empty() ? 0 : (int64_t)g.gen_steps.back() + 1;
-      std::vector<int64_t> off(nst + 1, 0);
-      for (int32_t st : g.gen_steps) off[st + 1]++;
-      for (int64_t k = 0; k < nst; ++k) off[k + 1] += off[k];
-      upload_raw(g.gen_off_d, off.data(), off.size(), net->stream);
-      upload_raw(g.gen_idx_d, g.gen_idx.data(), g.gen_idx.size(), net->stream);
-      d.gen_idx = g.gen_idx_d.as<int32_t>();
-      d.gen_off = g.gen_off_d.as<int64_t>();
-      d.gen_nsteps = nst;
-      g.gen_nsteps = nst;
-    }
+    if (g.kind == GK_GENERATOR) upload_generator(net, g);
   }
   net->nhist = (int)hptrs.size();
   if (net->nhist) {
@@ -1121,16 +1135,30 @@ int bm_net_add_generator(bm_net *net, int64_t N, int64_t nspikes, const int32_t 
     auto g = std::make_unique<HostGroup>();
     g->N = N;
     g->kind = GK_GENERATOR;
+    check_generator_spikes(N, nspikes, steps, idx);
     g->gen_steps.assign(steps, steps + nspikes);
     g->gen_idx.assign(idx, idx + nspikes);
-    for (int64_t q = 0; q < nspikes; ++q) {
-      BM_REQUIRE(idx[q] >= 0 && idx[q] < N, "generator index out of range");
-      BM_REQUIRE(steps[q] >= 0, "generator step must be >= 0");
-      if (q) BM_REQUIRE(steps[q] > steps[q - 1] || (steps[q] == steps[q - 1] && idx[q] > idx[q - 1]),
-                        "generator spikes must be sorted by (step, index) without duplicates");
-    }
     net->groups.push_back(std::move(g));
     *gid = (int)net->groups.size() - 1;
+  });
+}
+
+int bm_net_set_generator_spikes(bm_net *net, int gid, int64_t nspikes, const int32_t *steps, const int32_t *idx) {
+  return guard([&] {
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.kind == GK_GENERATOR, "not a spike generator group");
+    check_generator_spikes(g.N, nspikes, steps, idx);
+    g.gen_steps.assign(steps, steps + nspikes);
+    g.gen_idx.assign(idx, idx + nspikes);
+    if (!net->prepared) return;
+    // launches in flight read the old table; captured graphs hold its pointers
+    BM_HIP(hipStreamSynchronize(net->stream));
+    upload_generator(net, g);
+    if (net->graph_exec) {
+      hipGraphExecDestroy(net->graph_exec);
+      net->graph_exec = nullptr;
+      net->graph_windows = 0;
+    }
   });
 }
 

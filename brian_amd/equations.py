@@ -20,7 +20,6 @@ import ast
 import inspect
 import math
 import re
-
 import warnings
 
 import numpy as np

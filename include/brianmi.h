@@ -109,6 +109,11 @@ int bm_net_set_timed_input(bm_net *net, int gid, int var, int64_t nt, int per_ne
  * nspikes (step, index) pairs sorted by (step, index). */
 int bm_net_add_generator(bm_net *net, int64_t N, int64_t nspikes, const int32_t *steps,
                          const int32_t *idx, int *gid);
+/* Replace a generator group's spike table (same sorting rules), before or
+ * after prepare: steps are counted from the network's first step.  Used for
+ * spike sources read as the run proceeds (iterators, periodic sources,
+ * brian/directcontrol.py:63-215, :219-268). */
+int bm_net_set_generator_spikes(bm_net *net, int gid, int64_t nspikes, const int32_t *steps, const int32_t *idx);
 
 /* Poisson source with a counter-based RNG: neuron i spikes at step n iff
  * u32(seed, i, n) < floor(rate_i*dt*2^32) (splitmix64 construction). */

@@ -185,3 +185,74 @@ def test_network_operation_order():
     net = Network(G, do_nothing, mon)
     net.run(5 * defaultclock.dt)
     assert len(mon[0]) == 5 and (mon[0] < 1.0).all()
+
+
+def test_directcontrol_interface():
+    """brian/tests/testinterface/test_directcontrol.py:51-245 end to end:
+    each spike container through a SpikeMonitor; a 2 ms clock; reinit of a
+    callable generator; spiketimes set after a reinit; periods extended over
+    two runs; PulsePacket."""
+    from brian_amd import (SpikeGeneratorGroup, PulsePacket, SpikeMonitor, Network, Clock, msecond, ms,
+                           array, reinit_default_clock)
+
+    def mininet(grouptype, *args, **kwds):
+        reinit_default_clock()
+        G = grouptype(*args, **kwds)
+        M = SpikeMonitor(G, True)
+        net = Network(G, M)
+        net.run(5 * msecond)
+        reinit_default_clock()
+        return M.spikes
+
+    def test1(spikes):
+        assert len(spikes) == 5
+        i, t = zip(*spikes)
+        assert i == (0, 1, 0, 1, 0)
+        for k, tk in enumerate(t):
+            assert abs(k * 1e-3 - tk) < 1e-12
+    st = [(0, 0 * msecond), (1, 1 * msecond), (0, 2 * msecond), (1, 3 * msecond), (0, 4 * msecond)]
+    test1(mininet(SpikeGeneratorGroup, 2, st))
+    test1(mininet(SpikeGeneratorGroup, 2, st[::-1]))
+    test1(mininet(SpikeGeneratorGroup, 2, st, clock=Clock(dt=0.1 * msecond)))
+    sp = mininet(SpikeGeneratorGroup, 2, st, clock=Clock(dt=2 * msecond))
+    assert [round(t * 1e3, 9) for _, t in sp] == [0, 2, 2, 4, 4]
+    test1(mininet(SpikeGeneratorGroup, 2, lambda: st))
+    test1(mininet(SpikeGeneratorGroup, 2, array([[0, 0.0], [1, 0.001], [0, 0.002], [1, 0.003], [0, 0.004]])))
+
+    def sg():
+        for x in st:
+            yield x
+    test1(mininet(SpikeGeneratorGroup, 2, sg()))
+    # reinit restarts a callable generator
+    reinit_default_clock()
+    G = SpikeGeneratorGroup(2, sg)
+    M = SpikeMonitor(G, True)
+    net = Network(G, M)
+    net.run(5 * msecond)
+    test1(M.spikes)
+    net.reinit()
+    net.run(5 * msecond)
+    test1(M.spikes)
+    # spiketimes set after the network ran empty and was reinitialised
+    reinit_default_clock()
+    G = SpikeGeneratorGroup(2, [])
+    M = SpikeMonitor(G, True)
+    net = Network(G, M)
+    net.run(5 * msecond)
+    assert len(M.spikes) == 0
+    net.reinit()
+    G.spiketimes = st
+    net.run(5 * msecond)
+    test1(M.spikes)
+    # periods: one run, then a second run reads the next periods
+    test1(mininet(SpikeGeneratorGroup, 2, [(0, 0 * msecond), (1, 1 * msecond)], period=2 * ms))
+    reinit_default_clock()
+    G = SpikeGeneratorGroup(2, [(0, 0 * msecond), (1, 1 * msecond)], period=2 * ms)
+    M = SpikeMonitor(G, True)
+    net = Network(G, M)
+    net.run(3 * msecond)
+    net.run(3 * msecond)
+    assert [i for i, _ in M.spikes] == [0, 1, 0, 1, 0, 1]
+    # pulse packet with no spread
+    sp = mininet(PulsePacket, 2.5 * msecond, 10, 0 * msecond)
+    assert len(sp) == 10 and all(abs(t - 2.5e-3) < 1e-12 for _, t in sp)
