@@ -52,6 +52,8 @@ _SIGS = {
     "bm_net_set_timed_input": [P, C.c_int, C.c_int, i64, C.c_int, Pd],
     "bm_net_add_generator": [P, i64, i64, Pi32, Pi32, C.POINTER(C.c_int)],
     "bm_net_set_generator_spikes": [P, C.c_int, i64, Pi32, Pi32],
+    "bm_net_add_poisson_input": [P, C.c_int, C.c_int, C.c_int, dbl, dbl, u64, C.c_int, C.c_int, dbl,
+                                 C.POINTER(C.c_int)],
     "bm_net_add_poisson": [P, i64, Pd, u64, C.POINTER(C.c_int)],
     "bm_net_add_connection": [P, C.c_int, i64, i64, C.c_int, i64, i64, C.c_int, i64, Pi64, Pi32,
                               Pd, C.c_int, C.c_int, Pi32, C.c_int, C.POINTER(C.c_int)],
@@ -270,6 +272,12 @@ class NetHandle:
     def set_generator_spikes(self, gid, steps, idx):
         steps, idx = as_i32(steps), as_i32(idx)
         call("bm_net_set_generator_spikes", self.h, gid, len(steps), i32ptr(steps), i32ptr(idx))
+
+    def add_poisson_input(self, gid, var, n, p, weight, seed, frozen=False, copies=0, reliability=-1.0):
+        pid = C.c_int()
+        call("bm_net_add_poisson_input", self.h, gid, var, int(n), float(p), float(weight), int(seed),
+             1 if frozen else 0, int(copies), float(reliability), C.byref(pid))
+        return pid.value
 
     def add_poisson(self, N, rates, seed):
         r = f64(np.broadcast_to(np.asarray(rates, dtype=np.float64), (N,)))

@@ -27,7 +27,7 @@ __all__ = [
     "Clock", "defaultclock", "reinit_default_clock", "get_default_clock",
     "Threshold", "VariableThreshold", "EmpiricalThreshold", "NoThreshold", "StringThreshold",
     "Reset", "VariableReset", "StringReset", "Refractoriness", "NoReset",
-    "NeuronGroup", "SpikeGeneratorGroup", "PulsePacket", "PoissonGroup", "TimedArray",
+    "NeuronGroup", "SpikeGeneratorGroup", "PulsePacket", "PoissonGroup", "PoissonInput", "TimedArray",
     "Connection", "DelayConnection", "STDP", "ExponentialSTDP", "STP", "ConnectionMatrix",
     "SparseConnectionVector", "random_matrix", "random_matrix_fixed_column",
     "SpikeMonitor", "PopulationRateMonitor", "PopulationSpikeCounter", "SpikeCounter",
@@ -622,6 +622,36 @@ class PoissonGroup(_GroupBase):
         self._net = None
         self.var_index = {}
         self.var_names = []
+
+
+class PoissonInput(_Tracked):
+    """brian/directcontrol.py:474-619: N independent Poisson inputs of rate
+    ``rate`` and weight ``weight`` onto variable ``state`` of every neuron
+    of ``target`` (a whole group), drawn on the device each step as one
+    Binomial(N, rate*dt) per neuron from the counter RNG (``freeze``: one
+    draw shared by all neurons).  ``reliability`` scales the weight by a
+    shared Binomial(copies, reliability) each step.  ``jitter`` and
+    ``record`` are not supported."""
+
+    def __init__(self, target, N=None, rate=None, weight=None, state=None, jitter=None, reliability=None,
+                 copies=1, record=False, freeze=False, seed=None):
+        self._track(1)
+        if jitter is not None:
+            raise NotImplementedError("PoissonInput jitter is not supported")
+        if record:
+            raise NotImplementedError("PoissonInput record=True is not supported")
+        if target._owner is not target:
+            raise NotImplementedError("PoissonInput onto a subgroup is not supported")
+        self.target = target
+        self.n = int(N)
+        self.rate = float(rate)
+        self.w = float(weight)
+        self.state = state
+        self.index = target._var(state if state is not None else 0)
+        self.reliability = reliability
+        self.copies = int(copies)
+        self.frozen = bool(freeze)
+        self.seed = int(np.random.randint(0, 2 ** 31)) if seed is None else int(seed)
 
 
 class TimedArray:
@@ -1467,6 +1497,7 @@ class Network:
 
     def __init__(self, *args, **kw):
         self.groups, self.conns, self.stdps, self.monitors, self.ops = [], [], [], [], []
+        self.pinputs = []
         self._seq = []          # operations and state monitors in the order added
         self.handle = None
         self.gid = {}
@@ -1490,6 +1521,9 @@ class Network:
             elif isinstance(o, STP):
                 if o.C not in self.conns:
                     self.conns.append(o.C)
+            elif isinstance(o, PoissonInput):
+                if o not in self.pinputs:
+                    self.pinputs.append(o)
             elif isinstance(o, Connection):
                 if o not in self.conns:
                     self.conns.append(o)
@@ -1502,7 +1536,7 @@ class Network:
                     self.ops.append(o)
                     self._seq.append(o)
             elif isinstance(o, Network):
-                self.add(o.groups, o.conns, o.stdps, o._seq)
+                self.add(o.groups, o.conns, o.stdps, o.pinputs, o._seq)
             else:
                 raise TypeError("Only the following types of objects can be added to a network: "
                                 "NeuronGroup, Connection or NetworkOperation")
@@ -1551,6 +1585,9 @@ class Network:
             for P in (c.source._owner, c.target._owner):
                 if P not in self.groups:
                     self.groups.append(P)
+        for pin in self.pinputs:
+            if pin.target not in self.groups:
+                self.groups.append(pin.target)
         for m in self.monitors:
             src = m.source._owner if isinstance(m, SpikeMonitor) else m.P._owner
             if src not in self.groups:
@@ -1573,6 +1610,10 @@ class Network:
                 self._compile_neurons(h, gid, g, dt)
             self.gid[id(g)] = gid
             g._net = self
+        for pin in self.pinputs:
+            copies = pin.copies if pin.reliability is not None else 0
+            h.add_poisson_input(self.gid[id(pin.target)], pin.index, pin.n, pin.rate * dt, pin.w, pin.seed,
+                                pin.frozen, copies, -1.0 if pin.reliability is None else pin.reliability)
         recorded = set()
         for m in self.monitors:
             if isinstance(m, SpikeMonitor):

@@ -59,6 +59,7 @@ struct HostGroup {
   bool need_hist = false;
   int mod_var = -1;          // variable captured per spike for modulated outgoing connections
   int stp_conn = -1;         // outgoing connection with short-term plasticity, -1 none
+  std::vector<PinDev> pins;  // PoissonInputs onto this group
   int H = 0;
   // device
   DevBuf S, last, Apn_d, Cpn_d, code_d, consts_d, refr_d, ti_d, pthr_d, gen_idx_d, gen_off_d,
@@ -224,7 +225,7 @@ void require_unprepared(bm_net *net) {
 // (packed count) ring, no timed input.
 bool fast_eligible(const HostGroup &g) {
   if (g.kind != GK_NEURON || g.upd != UPD_LINEAR || g.M < 1 || g.M > 4) return false;
-  if (!g.mons.empty() || g.mod_var >= 0 || g.stp_conn >= 0) return false;
+  if (!g.mons.empty() || g.mod_var >= 0 || g.stp_conn >= 0 || !g.pins.empty()) return false;
   if (g.refr != REFR_NONE || g.th_kind == TH_EMPIRICAL || g.ti_nt > 0) return false;
   if (g.ring_desc.size() > 1) return false;
   if (g.ring_desc.size() == 1 && g.ring_desc[0].kind != RING_COUNT) return false;
@@ -689,6 +690,8 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       d.stp_par = sc.stp_par_d.as<double>();
       d.stp_delay = sc.hom_delay;
     }
+    d.npin = (int)g.pins.size();
+    for (int q = 0; q < d.npin; ++q) d.pins[q] = g.pins[q];
     d.nmon = (int)g.mons.size();
     for (int m = 0; m < d.nmon; ++m) {
       HostGroup::Mon &hm = *g.mons[m];
@@ -1159,6 +1162,35 @@ int bm_net_set_generator_spikes(bm_net *net, int gid, int64_t nspikes, const int
       net->graph_exec = nullptr;
       net->graph_windows = 0;
     }
+  });
+}
+
+int bm_net_add_poisson_input(bm_net *net, int gid, int var, int n, double p, double weight, uint64_t seed,
+                             int frozen, int copies, double reliability, int *pid) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.kind == GK_NEURON, "PoissonInput targets a neuron group");
+    BM_REQUIRE(var >= 0 && var < g.M, "variable out of range");
+    BM_REQUIRE(g.pins.size() < (size_t)MAXPIN, "too many PoissonInputs on one group");
+    BM_REQUIRE(n >= 0 && p >= 0 && p <= 1, "PoissonInput needs N >= 0 and 0 <= rate*dt <= 1");
+    // the inverse-CDF sampler starts from (1-p)^n
+    BM_REQUIRE(p >= 1 || -(double)n * std::log1p(-p) < 600.0,
+               "PoissonInput with more than ~600 expected events per neuron and step");
+    BM_REQUIRE(copies >= 0 && (copies == 0 || (reliability >= 0 && reliability <= 1)),
+               "reliability must lie in [0, 1]");
+    PinDev d{};
+    d.var = var;
+    d.n = n;
+    d.frozen = frozen ? 1 : 0;
+    d.copies = copies;
+    d.p = p;
+    d.w = weight;
+    d.alpha = reliability;
+    d.seed = seed;
+    d.key_shared = counter_key(seed, (uint64_t)g.N);     // a stream no neuron uses
+    g.pins.push_back(d);
+    *pid = (int)g.pins.size() - 1;
   });
 }
 

@@ -211,6 +211,26 @@ __device__ inline T apply_count(T x, uint32_t cnt, T w, int is_delay) {
   return x;
 }
 
+// Binomial(n, p) by inversion of its CDF with one 53-bit uniform: the
+// pmf runs up from (1-p)^n by P(k+1) = P(k) (n-k)/(k+1) p/(1-p).  The host
+// keeps n p small enough that (1-p)^n does not underflow.
+__device__ inline int binomial_inv(uint64_t bits, int n, double p) {
+#pragma clang fp contract(off)
+  if (p <= 0.0 || n <= 0) return 0;
+  if (p >= 1.0) return n;
+  const double u = (double)(bits >> 11) * 0x1.0p-53;
+  const double q = 1.0 - p, r = p / q;
+  double pk = pow(q, (double)n);
+  double cdf = pk;
+  int k = 0;
+  while (u >= cdf && k < n) {
+    pk = pk * (double)(n - k) / (double)(k + 1) * r;
+    ++k;
+    cdf = cdf + pk;
+  }
+  return k;
+}
+
 // Tsodyks-Markram short-term plasticity on a spike of row r at step n
 // (brian/stp.py:48-70 for scalar parameters, :78-90 per neuron; the same
 // arithmetic either way): u and x relax since the last update, the weights
@@ -562,6 +582,29 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
                 fval[r][k] = *fp;
                 *fp = (T)0;
               }
+            }
+          }
+          // PoissonInputs of step kst (connections slot), before the
+          // incoming connections: x += weff * Binomial(n, rate*dt)
+          for (int q = 0; q < g.npin; ++q) {
+            const PinDev &pin = g.pins[q];
+            double weff = pin.w;
+            if (pin.copies > 0)
+              weff = pin.w * (double)binomial_inv(counter_u64(pin.key_shared, 2 * (uint64_t)kst + 1), pin.copies,
+                                                  pin.alpha);
+            const int kf = pin.frozen ? binomial_inv(counter_u64(pin.key_shared, 2 * (uint64_t)kst), pin.n, pin.p) : 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              if (!valid[k]) continue;
+              const int i = blk0 + k * BLOCK + tid;
+              const int kk = pin.frozen ? kf
+                                        : binomial_inv(counter_u64(counter_key(pin.seed, (uint64_t)i), (uint64_t)kst),
+                                                       pin.n, pin.p);
+              if (kk == 0) continue;
+              const T add = (T)(weff * (double)kk);
+#pragma unroll
+              for (int j = 0; j < MM; ++j)
+                if (j == pin.var) s[j][k] = s[j][k] + add;
             }
           }
           for (int c = 0; c < g.nchan; ++c) {

@@ -9,6 +9,7 @@ namespace bm {
 constexpr int MAXM = 8;
 constexpr int MAXRING = 4;
 constexpr int MAXCHAN = 8;
+constexpr int MAXPIN = 2;         // PoissonInputs per target group
 constexpr int MAXOUT = 16;       // GroupDev travels as a kernel argument (< 4 KiB)
 constexpr int MAXRESET = 4;
 constexpr int MAXMON = 4;
@@ -83,6 +84,17 @@ struct MonDev {
   const int64_t *base;       // device: sample index held in buf row 0
 };
 
+// PoissonInput (brian/directcontrol.py:474-619): each step the target
+// variable gains weff * Binomial(n, p), p = rate * dt, drawn per neuron
+// (one draw for all when frozen) from the counter RNG; with reliability,
+// weff = w * Binomial(copies, alpha) drawn once per step.
+struct PinDev {
+  int var, n, frozen, copies;     // copies 0: no reliability draw
+  double p, w, alpha;
+  uint64_t seed;                  // per-neuron keys: counter_key(seed, i)
+  uint64_t key_shared;            // frozen and reliability draws
+};
+
 struct GroupDev {
   int64_t N;
   int M, kind, upd, pad0;
@@ -142,7 +154,8 @@ struct GroupDev {
   int64_t stp_off, stp_n;
   double *stp_state;
   const double *stp_par;
-  int stp_delay, pad2;
+  int stp_delay, npin;
+  PinDev pins[MAXPIN];
 };
 // passed by value to every sweep launch; kernel arguments are limited to 4 KiB
 static_assert(sizeof(GroupDev) <= 3584, "GroupDev must leave room in the 4 KiB kernel arguments");

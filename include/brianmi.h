@@ -119,6 +119,18 @@ int bm_net_set_generator_spikes(bm_net *net, int gid, int64_t nspikes, const int
  * u32(seed, i, n) < floor(rate_i*dt*2^32) (splitmix64 construction). */
 int bm_net_add_poisson(bm_net *net, int64_t N, const double *rates, uint64_t seed, int *gid);
 
+/* PoissonInput (brian/directcontrol.py:474-619) onto variable var of group
+ * gid: every step, in the connections slot and before the group's incoming
+ * connections, each neuron i gains weff * Binomial(n, p) with p = rate * dt
+ * drawn by inverting the binomial CDF with the uniform
+ * (counter_u64(counter_key(seed, i), step) >> 11) * 2^-53 (frozen: one draw
+ * per step for all neurons, stream N of the same seed).  copies > 0 adds
+ * the reference's reliability: weff = weight * Binomial(copies,
+ * reliability), drawn once per step; otherwise weff = weight.  The expected
+ * count n p must stay below ~600 per neuron and step.  Before prepare. */
+int bm_net_add_poisson_input(bm_net *net, int gid, int var, int n, double p, double weight, uint64_t seed,
+                             int frozen, int copies, double reliability, int *pid);
+
 /* Connection / DelayConnection (brian/connections/connection.py:224-296,
  * delayconnection.py:85-206) as CSR over the source range [src_off, src_off+src_n)
  * into target range [tgt_off, tgt_off+tgt_n), target variable tgt_var.

@@ -85,6 +85,67 @@ def weyl_uniform_u32(seed, stream, counter):
     return h.astype(np.uint32)
 
 
+def counter_u64(seed, stream, counter):
+    """64 random bits: mix(mix(seed*G + stream) + counter*G), uint64 arrays."""
+    G = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        key = _mix64(np.uint64(seed) * G + np.asarray(stream, dtype=np.uint64))
+        return _mix64(key + np.asarray(counter, dtype=np.uint64) * G)
+
+
+def binomial_cdf_table(n, p):
+    """CDF of Binomial(n, p) from P(0) = (1-p)^n by the pmf recurrence
+    P(k+1) = P(k) (n-k)/(k+1) p/(1-p), accumulated left to right."""
+    q = 1.0 - p
+    r = p / q
+    pk = q ** n
+    cdf = [pk]
+    for k in range(n):
+        pk = pk * (n - k) / (k + 1) * r
+        cdf.append(cdf[-1] + pk)
+    return np.array(cdf)
+
+
+def binomial_inv(bits, n, p):
+    """Binomial(n, p) by CDF inversion of the 53-bit uniforms in ``bits``:
+    the first k with u < CDF(k), n at most."""
+    bits = np.asarray(bits, dtype=np.uint64)
+    if p <= 0.0 or n <= 0:
+        return np.zeros(bits.shape, np.int64)
+    if p >= 1.0:
+        return np.full(bits.shape, n, np.int64)
+    u = (bits >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    cdf = binomial_cdf_table(n, p)
+    return np.minimum(np.searchsorted(cdf, u, side="right"), n).astype(np.int64)
+
+
+class PoissonInput:
+    """PoissonInput (directcontrol.py:474-619) with the backend's counter
+    RNG: at step n, neuron i of the target range gains
+    weff * Binomial(N, rate*dt), the uniform drawn from stream i (one draw
+    from stream len(group) when frozen); reliability draws
+    weff = w * Binomial(copies, alpha) once per step from that stream."""
+
+    def __init__(self, gid, var, n, p, w, seed, frozen=False, copies=0, alpha=0.0):
+        self.gid, self.var, self.n, self.p, self.w = gid, var, int(n), float(p), float(w)
+        self.seed, self.frozen, self.copies, self.alpha = int(seed), frozen, int(copies), float(alpha)
+
+    def apply(self, net):
+        g = net.groups[self.gid]
+        N = g.N
+        n = np.uint64(net.n)
+        weff = self.w
+        if self.copies > 0:
+            weff = self.w * float(binomial_inv(counter_u64(self.seed, N, 2 * n + np.uint64(1)), self.copies,
+                                               self.alpha))
+        if self.frozen:
+            k = np.full(N, binomial_inv(counter_u64(self.seed, N, 2 * n), self.n, self.p))
+        else:
+            k = binomial_inv(counter_u64(self.seed, np.arange(N), n), self.n, self.p)
+        nz = k != 0
+        g.S[self.var][nz] = g.S[self.var][nz] + weff * k[nz].astype(np.float64)
+
+
 def poisson_threshold_u32(rate, dt):
     """Spike iff u32 < floor(rate*dt*2^32) (clamped to 2^32-1)."""
     p = np.asarray(rate, dtype=np.float64) * dt
@@ -608,6 +669,7 @@ class Net:
         self.groups = []
         self.conns = []
         self.stdps = []
+        self.pinputs = []
         self.n = 0
         self.record = {}     # group id -> list of (step, idx array)
 
@@ -622,6 +684,11 @@ class Net:
         c = Connection(self, *a, **k)
         self.conns.append(c)
         return c
+
+    def add_poisson_input(self, *a, **k):
+        pin = PoissonInput(*a, **k)
+        self.pinputs.append(pin)
+        return pin
 
     def add_stdp(self, conn, **k):
         s = STDP(self, conn, **k)
@@ -638,6 +705,8 @@ class Net:
                 self.record[gid].append((n, sp.copy()))
         for s in self.stdps:
             s.decay()
+        for pin in self.pinputs:
+            pin.apply(self)
         for c in self.conns:
             c.propagate(self)
         for s in self.stdps:

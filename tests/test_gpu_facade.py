@@ -513,3 +513,59 @@ def test_stp_example_steady_state():
         us = U / (1 - (1 - U) * Ef)
         xs = (1 - Ed) / (1 - (1 - us) * Ed)
         assert abs(ux[j] - us * xs) < 1e-6 * us * xs, (j, ux[j], us * xs)
+
+
+@pytest.mark.parametrize("frozen,reliability", [(False, None), (True, None), (False, 0.6)])
+def test_poisson_input_matches_oracle(frozen, reliability):
+    """PoissonInput (directcontrol.py:474-619) onto the 4000-neuron CUBA
+    network's ge (N = 50 inputs at 20 Hz, weight 0.5 mV), FP64: device
+    draws and the oracle's sampler give the same raster and state."""
+    from brian_amd import workloads as W
+    from brian_amd._lib import NetHandle
+    from oracle import brian1 as O
+    from tests.netbuild import cuba_propagator
+    d = W.cuba(N=4000, seed=41)
+    N, Ne, Ni = d["N"], d["Ne"], d["Ni"]
+    A, C = cuba_propagator()
+    steps, n_in, rate, w = 2000, 50, 20.0, 0.5e-3
+    copies = 3 if reliability is not None else 0
+    alpha = reliability if reliability is not None else -1.0
+    h = NetHandle(1e-4, 64)
+    g = h.add_group(d["S0"])
+    h.set_linear(g, A, C)
+    h.set_threshold(g, 0, d["Vt"])
+    h.add_reset_op(g, 0, 0, d["Vr"])
+    net = O.Net(1e-4)
+    gid = net.add_group(O.Group(N, d["S0"], ("linear", A, C), ("const", 0, d["Vt"]),
+                                [(0, "=", d["Vr"], False)], dt=1e-4))
+    for (ip, ix, ww), off, n, var in ((d["exc"], 0, Ne, 1), (d["inh"], Ne, Ni, 2)):
+        h.add_connection(g, off, n, g, 0, N, var, ip, ix, ww)
+        net.add_connection((gid, off, n), (gid, 0, N), var, ip, ix, ww)
+    h.add_poisson_input(g, 1, n_in, rate * 1e-4, w, 77, frozen, copies, alpha)
+    net.add_poisson_input(gid, 1, n_in, rate * 1e-4, w, 77, frozen=frozen, copies=copies,
+                          alpha=max(alpha, 0.0))
+    h.record(g, 1 << 20)
+    h.prepare()
+    h.run(steps)
+    net.run(steps)
+    s, i = h.get_spikes(g)
+    rs, ri = net.raster(gid)
+    assert len(rs) > 1000
+    np.testing.assert_array_equal(s, rs)
+    np.testing.assert_array_equal(i, ri)
+    for var in range(3):
+        np.testing.assert_allclose(h.get_state(g, var, N), net.groups[gid].S[var], rtol=1e-12, atol=1e-15)
+    h.close()
+
+
+def test_poisson_input_interface():
+    """brian/tests/testinterface/test_directcontrol.py:247-255: a neuron
+    driven only by a PoissonInput spikes."""
+    from brian_amd import NeuronGroup, PoissonInput, SpikeCounter, Network, Equations, Hz, ms
+    eqs = Equations("dv/dt=(1-v)/(1*second) : 1")
+    group = NeuronGroup(N=1, model=eqs, reset=0, threshold=1)
+    inp = PoissonInput(group, N=10, rate=50 * Hz, weight=.11, state='v')
+    m = SpikeCounter(group)
+    net = Network(group, inp, m)
+    net.run(500 * ms)
+    assert m.nspikes >= 1

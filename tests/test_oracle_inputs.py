@@ -60,3 +60,27 @@ def test_ou_process_statistics_and_update():
     zr = (y[:, 1:] - a * y[:, :-1]) / b
     np.testing.assert_allclose(zr, z[:, 1:], atol=1e-9)
     np.testing.assert_allclose(y[:, 0], z[:, 0], rtol=0, atol=1e-12)
+
+
+def test_binomial_sampler_pins():
+    """The inverse-CDF binomial sampler (PoissonInput): its CDF table
+    equals scipy's Binomial CDF, a brute-force enumeration of 2^3 outcomes
+    for n = 3, the draws follow the pmf (chi-square), and p = 0 / p = 1
+    give 0 / n."""
+    from itertools import product
+    from oracle import brian1 as O
+    for n, p in ((10, 0.005), (1000, 0.002), (50, 0.3), (3, 0.25)):
+        np.testing.assert_allclose(O.binomial_cdf_table(n, p), scipy.stats.binom.cdf(np.arange(n + 1), n, p),
+                                   rtol=1e-12, atol=1e-15)
+    pmf = np.zeros(4)
+    for outcome in product((0, 1), repeat=3):
+        pmf[sum(outcome)] += np.prod([0.25 if o else 0.75 for o in outcome])
+    np.testing.assert_allclose(O.binomial_cdf_table(3, 0.25), np.cumsum(pmf), rtol=1e-15)
+    bits = O.counter_u64(17, np.arange(64), np.arange(4096)[:, None]).ravel()
+    k = O.binomial_inv(bits, 20, 0.1)
+    obs = np.bincount(k, minlength=21)[:21]
+    exp = scipy.stats.binom.pmf(np.arange(21), 20, 0.1) * len(k)
+    m = exp > 5
+    chi2 = ((obs[m] - exp[m]) ** 2 / exp[m]).sum() + (obs[~m].sum() - exp[~m].sum()) ** 2 / exp[~m].sum()
+    assert scipy.stats.chi2.sf(chi2, m.sum()) > 1e-3
+    assert (O.binomial_inv(bits[:10], 5, 0.0) == 0).all() and (O.binomial_inv(bits[:10], 5, 1.0) == 5).all()
