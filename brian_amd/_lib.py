@@ -54,6 +54,8 @@ _SIGS = {
     "bm_net_set_generator_spikes": [P, C.c_int, i64, Pi32, Pi32],
     "bm_net_add_poisson_input": [P, C.c_int, C.c_int, C.c_int, dbl, dbl, u64, C.c_int, C.c_int, dbl,
                                  C.POINTER(C.c_int)],
+    "bm_net_add_coincidence_counter": [P, C.c_int, i64, i64, i64, Pd, Pi32, Pd, dbl, dbl, C.c_int],
+    "bm_net_read_coincidence_counter": [P, C.c_int, Pi32, Pi32, Pi32],
     "bm_net_add_poisson": [P, i64, Pd, u64, C.POINTER(C.c_int)],
     "bm_net_add_connection": [P, C.c_int, i64, i64, C.c_int, i64, i64, C.c_int, i64, Pi64, Pi32,
                               Pd, C.c_int, C.c_int, Pi32, C.c_int, C.POINTER(C.c_int)],
@@ -278,6 +280,17 @@ class NetHandle:
         call("bm_net_add_poisson_input", self.h, gid, var, int(n), float(p), float(weight), int(seed),
              1 if frozen else 0, int(copies), float(reliability), C.byref(pid))
         return pid.value
+
+    def add_coincidence_counter(self, gid, off, n, data, offsets, spikedelays, delta, onset, exclusive=True):
+        data, offsets = f64(data), as_i32(offsets)
+        sd = f64(spikedelays) if spikedelays is not None else None
+        call("bm_net_add_coincidence_counter", self.h, gid, int(off), int(n), len(data), dptr(data),
+             i32ptr(offsets), None if sd is None else dptr(sd), float(delta), float(onset), 1 if exclusive else 0)
+
+    def read_coincidence_counter(self, gid, n):
+        out = [np.zeros(n, np.int32) for _ in range(3)]
+        call("bm_net_read_coincidence_counter", self.h, gid, *[i32ptr(o) for o in out])
+        return out
 
     def add_poisson(self, N, rates, seed):
         r = f64(np.broadcast_to(np.asarray(rates, dtype=np.float64), (N,)))

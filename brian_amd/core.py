@@ -30,7 +30,7 @@ __all__ = [
     "NeuronGroup", "SpikeGeneratorGroup", "PulsePacket", "PoissonGroup", "PoissonInput", "TimedArray",
     "Connection", "DelayConnection", "STDP", "ExponentialSTDP", "STP", "ConnectionMatrix",
     "SparseConnectionVector", "random_matrix", "random_matrix_fixed_column",
-    "SpikeMonitor", "PopulationRateMonitor", "PopulationSpikeCounter", "SpikeCounter",
+    "SpikeMonitor", "PopulationRateMonitor", "PopulationSpikeCounter", "SpikeCounter", "CoincidenceCounter",
     "StateMonitor", "Network", "MagicNetwork", "run", "reinit", "stop", "network_operation",
     "NetworkOperation", "set_global_preferences", "get_global_preference", "forget", "recall",
 ]
@@ -1407,6 +1407,44 @@ class StateMonitor(_Tracked):
         self._hn = 0
 
 
+class CoincidenceCounter(_Tracked):
+    """brian/monitor.py:1230-1376: coincidences of each neuron's spikes with
+    a target train, counted online on the device at every spike.  ``data``:
+    concatenated target trains (seconds), each padded with a spike long
+    before 0 and one after the run; ``spiketimes_offset[i]``: first entry of
+    neuron i's train (default 0); ``spikedelays`` shift the model spikes;
+    ``coincidence_count_algorithm`` 'exclusive' or 'inclusive'.  Attributes
+    ``coincidences``, ``model_length``, ``target_length``."""
+
+    def __init__(self, source, data, spiketimes_offset=None, spikedelays=None,
+                 coincidence_count_algorithm="exclusive", onset=None, delta=4 * ms):
+        self._track(1)
+        if coincidence_count_algorithm not in ("exclusive", "inclusive"):
+            raise ValueError("coincidence_count_algorithm is 'exclusive' or 'inclusive'")
+        self.source = source
+        self.N = len(source)
+        self.data = np.array([float(x) for x in data], dtype=np.float64)
+        self.spiketimes_offset = (np.zeros(self.N, np.int32) if spiketimes_offset is None
+                                  else np.asarray(spiketimes_offset, dtype=np.int32))
+        self.spikedelays = None if spikedelays is None else np.asarray(spikedelays, dtype=np.float64)
+        self.coincidence_count_algorithm = coincidence_count_algorithm
+        self.onset = 0.0 if onset is None else float(onset)
+        self.delta_s = float(delta)
+        self._net = None
+
+    def _read(self):
+        net = self._net
+        if net is None or net.handle is None:
+            z = np.zeros(self.N, np.int64)
+            return z, z.copy(), z.copy()
+        c, m, t = net.handle.read_coincidence_counter(net.gid[id(self.source._owner)], self.N)
+        return c.astype(np.int64), m.astype(np.int64), t.astype(np.int64)
+
+    coincidences = property(lambda self: self._read()[0])
+    model_length = property(lambda self: self._read()[1])
+    target_length = property(lambda self: self._read()[2])
+
+
 # ------------------------------------------------------------------ network
 class NetworkOperation(_Tracked):
     """A host callback run after every step (brian/network.py:694-721);
@@ -1498,6 +1536,7 @@ class Network:
     def __init__(self, *args, **kw):
         self.groups, self.conns, self.stdps, self.monitors, self.ops = [], [], [], [], []
         self.pinputs = []
+        self.coincs = []
         self._seq = []          # operations and state monitors in the order added
         self.handle = None
         self.gid = {}
@@ -1524,6 +1563,9 @@ class Network:
             elif isinstance(o, PoissonInput):
                 if o not in self.pinputs:
                     self.pinputs.append(o)
+            elif isinstance(o, CoincidenceCounter):
+                if o not in self.coincs:
+                    self.coincs.append(o)
             elif isinstance(o, Connection):
                 if o not in self.conns:
                     self.conns.append(o)
@@ -1536,7 +1578,7 @@ class Network:
                     self.ops.append(o)
                     self._seq.append(o)
             elif isinstance(o, Network):
-                self.add(o.groups, o.conns, o.stdps, o.pinputs, o._seq)
+                self.add(o.groups, o.conns, o.stdps, o.pinputs, o.coincs, o._seq)
             else:
                 raise TypeError("Only the following types of objects can be added to a network: "
                                 "NeuronGroup, Connection or NetworkOperation")
@@ -1588,6 +1630,9 @@ class Network:
         for pin in self.pinputs:
             if pin.target not in self.groups:
                 self.groups.append(pin.target)
+        for cc in self.coincs:
+            if cc.source._owner not in self.groups:
+                self.groups.append(cc.source._owner)
         for m in self.monitors:
             src = m.source._owner if isinstance(m, SpikeMonitor) else m.P._owner
             if src not in self.groups:
@@ -1614,6 +1659,11 @@ class Network:
             copies = pin.copies if pin.reliability is not None else 0
             h.add_poisson_input(self.gid[id(pin.target)], pin.index, pin.n, pin.rate * dt, pin.w, pin.seed,
                                 pin.frozen, copies, -1.0 if pin.reliability is None else pin.reliability)
+        for cc in self.coincs:
+            h.add_coincidence_counter(self.gid[id(cc.source._owner)], cc.source._origin, cc.N, cc.data,
+                                      cc.spiketimes_offset, cc.spikedelays, cc.delta_s, cc.onset,
+                                      cc.coincidence_count_algorithm == "exclusive")
+            cc._net = self
         recorded = set()
         for m in self.monitors:
             if isinstance(m, SpikeMonitor):
@@ -1834,6 +1884,8 @@ class Network:
             for st in self.stdps:
                 st.sid = None
                 st._net = None
+            for cc in self.coincs:
+                cc._net = None
             for g in self.groups:
                 if hasattr(g, "_S0"):
                     g._S = keep.get(id(g), g._S0).copy()

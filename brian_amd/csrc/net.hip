@@ -60,6 +60,16 @@ struct HostGroup {
   int mod_var = -1;          // variable captured per spike for modulated outgoing connections
   int stp_conn = -1;         // outgoing connection with short-term plasticity, -1 none
   std::vector<PinDev> pins;  // PoissonInputs onto this group
+  struct Coin {              // CoincidenceCounter of neurons [off, off + n)
+    int64_t off = 0, n = 0, delta = 0;
+    double onset = 0;
+    int exclusive = 1;
+    std::vector<int64_t> tsteps;
+    std::vector<int32_t> offsets;
+    std::vector<double> sdelay;
+    DevBuf tsteps_d, sdelay_d, sidx_d, last_d, next_d, lallow_d, nallow_d, coinc_d, model_d, target_d, dev_d;
+  };
+  std::unique_ptr<Coin> coin;
   int H = 0;
   // device
   DevBuf S, last, Apn_d, Cpn_d, code_d, consts_d, refr_d, ti_d, pthr_d, gen_idx_d, gen_off_d,
@@ -225,7 +235,7 @@ void require_unprepared(bm_net *net) {
 // (packed count) ring, no timed input.
 bool fast_eligible(const HostGroup &g) {
   if (g.kind != GK_NEURON || g.upd != UPD_LINEAR || g.M < 1 || g.M > 4) return false;
-  if (!g.mons.empty() || g.mod_var >= 0 || g.stp_conn >= 0 || !g.pins.empty()) return false;
+  if (!g.mons.empty() || g.mod_var >= 0 || g.stp_conn >= 0 || !g.pins.empty() || g.coin) return false;
   if (g.refr != REFR_NONE || g.th_kind == TH_EMPIRICAL || g.ti_nt > 0) return false;
   if (g.ring_desc.size() > 1) return false;
   if (g.ring_desc.size() == 1 && g.ring_desc[0].kind != RING_COUNT) return false;
@@ -689,6 +699,46 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       d.stp_state = sc.stp_state_d.as<double>();
       d.stp_par = sc.stp_par_d.as<double>();
       d.stp_delay = sc.hom_delay;
+    }
+    if (g.coin) {
+      HostGroup::Coin &c = *g.coin;
+      const int64_t n = c.n;
+      upload_raw(c.tsteps_d, c.tsteps.data(), c.tsteps.size(), net->stream);
+      if (!c.sdelay.empty()) upload_raw(c.sdelay_d, c.sdelay.data(), c.sdelay.size(), net->stream);
+      upload_raw(c.sidx_d, c.offsets.data(), c.offsets.size(), net->stream);
+      std::vector<int64_t> last(n), next(n);
+      for (int64_t r = 0; r < n; ++r) {          // monitor.py:1322-1323
+        last[r] = c.tsteps[c.offsets[r]];
+        next[r] = c.tsteps[c.offsets[r] + 1];
+      }
+      upload_raw(c.last_d, last.data(), n, net->stream);
+      upload_raw(c.next_d, next.data(), n, net->stream);
+      std::vector<uint8_t> ones(n, 1);
+      upload_raw(c.lallow_d, ones.data(), n, net->stream);
+      upload_raw(c.nallow_d, ones.data(), n, net->stream);
+      std::vector<int32_t> zeros(n, 0);
+      upload_raw(c.coinc_d, zeros.data(), n, net->stream);
+      upload_raw(c.model_d, zeros.data(), n, net->stream);
+      upload_raw(c.target_d, zeros.data(), n, net->stream);
+      CoinDev cd{};
+      cd.off = c.off;
+      cd.n = n;
+      cd.ndata = (int64_t)c.tsteps.size();
+      cd.tsteps = c.tsteps_d.as<int64_t>();
+      cd.sdelay = c.sdelay.empty() ? nullptr : c.sdelay_d.as<double>();
+      cd.sidx = c.sidx_d.as<int32_t>();
+      cd.last = c.last_d.as<int64_t>();
+      cd.next = c.next_d.as<int64_t>();
+      cd.lallow = c.lallow_d.as<uint8_t>();
+      cd.nallow = c.nallow_d.as<uint8_t>();
+      cd.coinc = c.coinc_d.as<int32_t>();
+      cd.model_len = c.model_d.as<int32_t>();
+      cd.target_len = c.target_d.as<int32_t>();
+      cd.delta = c.delta;
+      cd.onset = c.onset;
+      cd.exclusive = c.exclusive;
+      upload_raw(c.dev_d, &cd, 1, net->stream);
+      d.coin = c.dev_d.as<CoinDev>();
     }
     d.npin = (int)g.pins.size();
     for (int q = 0; q < d.npin; ++q) d.pins[q] = g.pins[q];
@@ -1191,6 +1241,52 @@ int bm_net_add_poisson_input(bm_net *net, int gid, int var, int n, double p, dou
     d.key_shared = counter_key(seed, (uint64_t)g.N);     // a stream no neuron uses
     g.pins.push_back(d);
     *pid = (int)g.pins.size() - 1;
+  });
+}
+
+int bm_net_add_coincidence_counter(bm_net *net, int gid, int64_t off, int64_t n, int64_t ndata, const double *data,
+                                   const int32_t *offsets, const double *spikedelays, double delta, double onset,
+                                   int exclusive) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.kind == GK_NEURON, "a CoincidenceCounter monitors a neuron group");
+    BM_REQUIRE(!g.coin, "one CoincidenceCounter per group");
+    BM_REQUIRE(off >= 0 && n > 0 && off + n <= g.N, "neuron range outside the group");
+    BM_REQUIRE(ndata >= 2 && data && offsets, "target trains need the padding spikes");
+    auto c = std::make_unique<HostGroup::Coin>();
+    c->off = off;
+    c->n = n;
+    const double dt = net->dt;
+    c->tsteps.resize(ndata);
+    for (int64_t q = 0; q < ndata; ++q) c->tsteps[q] = (int64_t)std::nearbyint(data[q] / dt);   // rint(data/dt)
+    c->offsets.assign(offsets, offsets + n);
+    for (int64_t r = 0; r < n; ++r)
+      BM_REQUIRE(offsets[r] >= 0 && offsets[r] + 1 < ndata, "train offset out of range");
+    if (spikedelays) c->sdelay.assign(spikedelays, spikedelays + n);
+    c->delta = (int64_t)std::nearbyint(delta / dt);                                            // monitor.py:1312
+    c->onset = onset;
+    c->exclusive = exclusive ? 1 : 0;
+    g.coin = std::move(c);
+  });
+}
+
+int bm_net_read_coincidence_counter(bm_net *net, int gid, int32_t *coincidences, int32_t *model_length,
+                                    int32_t *target_length) {
+  return guard([&] {
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.coin, "group has no CoincidenceCounter");
+    const size_t n = (size_t)g.coin->n;
+    if (!net->prepared) {
+      std::fill(coincidences, coincidences + n, 0);
+      std::fill(model_length, model_length + n, 0);
+      std::fill(target_length, target_length + n, 0);
+      return;
+    }
+    BM_HIP(hipStreamSynchronize(net->stream));
+    BM_HIP(hipMemcpy(coincidences, g.coin->coinc_d.p, n * 4, hipMemcpyDeviceToHost));
+    BM_HIP(hipMemcpy(model_length, g.coin->model_d.p, n * 4, hipMemcpyDeviceToHost));
+    BM_HIP(hipMemcpy(target_length, g.coin->target_d.p, n * 4, hipMemcpyDeviceToHost));
   });
 }
 

@@ -231,6 +231,45 @@ __device__ inline int binomial_inv(uint64_t bits, int n, double p) {
   return k;
 }
 
+// CoincidenceCounter.propagate (brian/monitor.py:1336-1376) for one spike
+// of neuron i at step n: pass the target spikes before it, then count a
+// coincidence with the last or next target spike within delta ('exclusive':
+// each target spike counts once).
+__device__ inline void coin_spike(const CoinDev &c, int64_t i, int64_t n, double dt) {
+#pragma clang fp contract(off)
+  const int64_t r = i - c.off;
+  if (r < 0 || r >= c.n) return;
+  const double t = (double)n * dt + 0.0;
+  const bool counting = t >= c.onset;
+  if (counting) c.model_len[r] = c.model_len[r] + 1;
+  const int64_t T = (int64_t)rint((t + (c.sdelay ? c.sdelay[r] : 0.0)) / dt);
+  int32_t idx = c.sidx[r];
+  int64_t last = c.last[r], next = c.next[r];
+  bool la = c.lallow[r] != 0, na = c.nallow[r] != 0;
+  while (T > next && idx + 2 < c.ndata) {
+    c.target_len[r] = c.target_len[r] + 1;
+    ++idx;
+    last = next;
+    next = c.tsteps[idx + 1];
+    if (c.exclusive) {
+      la = na;
+      na = true;
+    }
+  }
+  const bool near_last = last + c.delta >= T, near_next = next - c.delta <= T;
+  if (counting && ((near_last && la) || (near_next && na))) c.coinc[r] = c.coinc[r] + 1;
+  if (c.exclusive) {
+    const bool both = (near_last && la) && (near_next && na);
+    la = la && !near_last;
+    na = (na && !near_next) || both;
+  }
+  c.sidx[r] = idx;
+  c.last[r] = last;
+  c.next[r] = next;
+  c.lallow[r] = la ? 1 : 0;
+  c.nallow[r] = na ? 1 : 0;
+}
+
 // Tsodyks-Markram short-term plasticity on a spike of row r at step n
 // (brian/stp.py:48-70 for scalar parameters, :78-90 per neuron; the same
 // arithmetic either way): u and x relax since the last update, the weights
@@ -784,7 +823,12 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
           }
           spk[k] = c;
         }
-        if (spk[k]) last[k] = (int32_t)n;
+        if (spk[k]) {
+          last[k] = (int32_t)n;
+          if constexpr (GEN) {
+            if (g.coin) coin_spike(*g.coin, blk0 + k * BLOCK + tid, n, g.dt);
+          }
+        }
       }
       // wavefront ballot + prefix compaction into the block's LDS list
 #pragma unroll

@@ -95,6 +95,23 @@ struct PinDev {
   uint64_t key_shared;            // frozen and reliability draws
 };
 
+// CoincidenceCounter (brian/monitor.py:1230-1376) of the neurons
+// [off, off + n) of a group, updated online at each of their spikes: the
+// target trains are concatenated steps rint(t/dt) with padding spikes
+// (offsets[r] = first entry of neuron r's train), per-neuron state in HBM.
+struct CoinDev {
+  int64_t off, n, ndata;
+  const int64_t *tsteps;      // [ndata]
+  const double *sdelay;       // [n] spike delays (s) or null
+  int32_t *sidx;              // [n] index of the last passed target spike
+  int64_t *last, *next;       // [n] steps of the target spikes around now
+  uint8_t *lallow, *nallow;   // [n] exclusive algorithm flags
+  int32_t *coinc, *model_len, *target_len;
+  int64_t delta;              // window half-width in steps
+  double onset;               // counting starts at t >= onset (s)
+  int exclusive, pad;
+};
+
 struct GroupDev {
   int64_t N;
   int M, kind, upd, pad0;
@@ -156,6 +173,7 @@ struct GroupDev {
   const double *stp_par;
   int stp_delay, npin;
   PinDev pins[MAXPIN];
+  const CoinDev *coin;        // device copy, null if the group has no counter
 };
 // passed by value to every sweep launch; kernel arguments are limited to 4 KiB
 static_assert(sizeof(GroupDev) <= 3584, "GroupDev must leave room in the 4 KiB kernel arguments");

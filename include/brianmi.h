@@ -119,6 +119,20 @@ int bm_net_set_generator_spikes(bm_net *net, int gid, int64_t nspikes, const int
  * u32(seed, i, n) < floor(rate_i*dt*2^32) (splitmix64 construction). */
 int bm_net_add_poisson(bm_net *net, int64_t N, const double *rates, uint64_t seed, int *gid);
 
+/* CoincidenceCounter (brian/monitor.py:1230-1376) on neurons [off, off+n)
+ * of group gid, counted online at each spike on the device: data[ndata]
+ * are the concatenated target trains in seconds, each padded with a spike
+ * long before 0 and one after the run; offsets[n] the first entry of each
+ * neuron's train; spikedelays[n] (nullable) shift the model spikes; delta
+ * and onset in seconds; exclusive 1 ('exclusive') or 0 ('inclusive').
+ * Times become steps rint(t/dt).  Before prepare. */
+int bm_net_add_coincidence_counter(bm_net *net, int gid, int64_t off, int64_t n, int64_t ndata, const double *data,
+                                   const int32_t *offsets, const double *spikedelays, double delta, double onset,
+                                   int exclusive);
+/* coincidences, model_length, target_length [n] so far. */
+int bm_net_read_coincidence_counter(bm_net *net, int gid, int32_t *coincidences, int32_t *model_length,
+                                    int32_t *target_length);
+
 /* PoissonInput (brian/directcontrol.py:474-619) onto variable var of group
  * gid: every step, in the connections slot and before the group's incoming
  * connections, each neuron i gains weff * Binomial(n, p) with p = rate * dt

@@ -10,7 +10,9 @@ brian/tools/statistics.py:236-240.
 
 Pinned by tests/test_oracle_fitting.py: hand-computed coincidence counts,
 gamma = 1 for identical trains and ~0 for independent Poisson trains, and a
-pure-Python single-candidate loop.
+pure-Python single-candidate loop.  The event-driven CoincidenceCounter of
+brian/monitor.py (``coincidence_counter``) is pinned against that loop and
+hand cases in tests/test_oracle_inputs.py.
 """
 from __future__ import annotations
 
@@ -107,3 +109,38 @@ def exclusive_coincidences_single(model_spikes, target, delta):
             last, nxt = nxt, padded[k + 1]
             la, na = na, True
     return nc
+
+
+def coincidence_counter(spikes, data_steps, offsets, delta, dt, onset=0.0, sdelay=None, exclusive=True):
+    """CoincidenceCounter.propagate (brian/monitor.py:1336-1376), event by
+    event for each neuron r: its model spikes ``spikes[r]`` (sorted steps),
+    target trains ``data_steps`` (rint(t/dt), padded at both ends) from
+    ``offsets[r]``.  Returns (coincidences, model_length, target_length)."""
+    n = len(spikes)
+    coinc = np.zeros(n, np.int64)
+    model = np.zeros(n, np.int64)
+    target = np.zeros(n, np.int64)
+    for r in range(n):
+        idx = int(offsets[r])
+        last, nxt = int(data_steps[idx]), int(data_steps[idx + 1])
+        la = na = True
+        for step in spikes[r]:
+            t = int(step) * dt + 0.0
+            if t >= onset:
+                model[r] += 1
+            T = int(np.rint((t + (0.0 if sdelay is None else sdelay[r])) / dt))
+            while T > nxt:
+                target[r] += 1
+                idx += 1
+                last, nxt = nxt, int(data_steps[idx + 1])
+                if exclusive:
+                    la, na = na, True
+            near_last = last + delta >= T
+            near_next = nxt - delta <= T
+            if t >= onset and ((near_last and la) or (near_next and na)):
+                coinc[r] += 1
+            if exclusive:
+                both = (near_last and la) and (near_next and na)
+                la = la and not near_last
+                na = (na and not near_next) or both
+    return coinc, model, target

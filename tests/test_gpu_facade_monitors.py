@@ -256,3 +256,57 @@ def test_directcontrol_interface():
     # pulse packet with no spread
     sp = mininet(PulsePacket, 2.5 * msecond, 10, 0 * msecond)
     assert len(sp) == 10 and all(abs(t - 2.5e-3) < 1e-12 for _, t in sp)
+
+
+def test_coincidence_counter_online():
+    """brian/tests/testinterface/test_monitor.py:272-330: target spikes
+    from one IF neuron under a noisy current; ten neurons with spread time
+    constants; the device's online CoincidenceCounter equals the oracle's
+    restatement of CoincidenceCounter.propagate on the same model spikes,
+    and the offline coincidence count of gamma_factor(normalize=False)
+    within one per neuron."""
+    from brian_amd import (NeuronGroup, SpikeMonitor, CoincidenceCounter, Network, TimedArray, ms, second,
+                           defaultclock, reinit_default_clock, gamma_factor, ones)
+    from oracle import fitting as F
+    reinit_default_clock()
+    eqs = """
+    dV/dt = (-V+R*I)/tau : 1
+    I : 1
+    R : 1
+    tau : second
+    """
+    duration = 500 * ms
+    dt = defaultclock.dt
+    rng = np.random.default_rng(5)
+    inp = 1.2 + .2 * rng.standard_normal(int(duration / dt))
+    delta = 4 * ms
+    n = 10
+    group = NeuronGroup(N=1, model=eqs, reset=0, threshold=1, method='Euler', refractory=3 * delta)
+    group.I = TimedArray(inp, start=0 * second, dt=dt)
+    group.R = 1.0
+    group.tau = 20 * ms
+    M = SpikeMonitor(group)
+    net = Network(group, M)
+    net.run(duration)
+    train0 = [t for i, t in M.spikes]
+    assert len(train0) >= 8
+    reinit_default_clock()
+    group = NeuronGroup(n, eqs, reset=0, threshold=1, method='Euler')
+    group.I = TimedArray(inp, start=0 * second, dt=dt)
+    group.R = 1.0 * ones(n)
+    group.tau = 20 * ms * (1 + .1 * (2 * rng.random(n) - 1))
+    data = [-1 * second] + train0 + [duration + 1 * second]
+    cc = CoincidenceCounter(source=group, data=data, delta=delta)
+    sm = SpikeMonitor(group)
+    net = Network(group, cc, sm)
+    net.run(duration)
+    online = cc.coincidences
+    spikes = [np.array(np.rint(np.asarray(sm[i]) / dt), dtype=np.int64) for i in range(n)]
+    c, m, t = F.coincidence_counter(spikes, np.array(np.rint(np.asarray(data) / dt), dtype=np.int64),
+                                    np.zeros(n, np.int64), int(np.rint(delta / dt)), dt)
+    np.testing.assert_array_equal(online, c)
+    np.testing.assert_array_equal(cc.model_length, m)
+    np.testing.assert_array_equal(cc.target_length, t)
+    assert online.sum() > 2 * n
+    offline = np.array([gamma_factor(sm[i], train0, delta=delta, normalize=False, dt=dt) for i in range(n)])
+    assert np.abs(online - offline).max() <= 1, (online, offline)

@@ -84,3 +84,32 @@ def test_binomial_sampler_pins():
     chi2 = ((obs[m] - exp[m]) ** 2 / exp[m]).sum() + (obs[~m].sum() - exp[~m].sum()) ** 2 / exp[~m].sum()
     assert scipy.stats.chi2.sf(chi2, m.sum()) > 1e-3
     assert (O.binomial_inv(bits[:10], 5, 0.0) == 0).all() and (O.binomial_inv(bits[:10], 5, 1.0) == 5).all()
+
+
+def test_coincidence_counter_restatements_agree():
+    """The event-driven CoincidenceCounter (monitor.py:1336-1376) and the
+    per-step GPU runsim counter (gpu_modelfitting.py:118-156) are two
+    formulations of one exclusive algorithm: equal counts on random trains;
+    hand cases: a model spike within delta of one target spike counts once,
+    one between two close targets counts once, onset excludes early spikes,
+    target_length counts the target spikes passed."""
+    from oracle import fitting as F
+    rng = np.random.default_rng(3)
+    BIG = F.BIG
+    for _ in range(40):
+        tgt = np.unique(rng.integers(50, 5000, rng.integers(5, 60)))
+        mod = np.unique(rng.integers(50, 5000, rng.integers(5, 60)))
+        data = np.concatenate(([-BIG], tgt, [BIG]))
+        c, m, tl = F.coincidence_counter([mod], data, [0], 40, 1e-4)
+        assert c[0] == F.exclusive_coincidences_single(mod, tgt, 40)
+        assert m[0] == len(mod)
+        assert tl[0] == np.sum(tgt < mod.max())
+    data = np.array([-BIG, 100, 130, 1000, BIG])
+    c, m, tl = F.coincidence_counter([np.array([120])], data, [0], 40, 1e-4)
+    assert c[0] == 1
+    c, m, _ = F.coincidence_counter([np.array([95, 105])], data, [0], 40, 1e-4)
+    assert c[0] == 2                                # two targets, two model spikes
+    c, m, _ = F.coincidence_counter([np.array([95, 105])], np.array([-BIG, 100, 5000, BIG]), [0], 40, 1e-4)
+    assert c[0] == 1                                # one target spike counts once
+    c, m, _ = F.coincidence_counter([np.array([100, 1000])], data, [0], 40, 1e-4, onset=0.05)
+    assert c[0] == 1 and m[0] == 1
