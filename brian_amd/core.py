@@ -31,6 +31,7 @@ __all__ = [
     "Connection", "DelayConnection", "STDP", "ExponentialSTDP", "STP", "ConnectionMatrix",
     "SparseConnectionVector", "random_matrix", "random_matrix_fixed_column",
     "SpikeMonitor", "PopulationRateMonitor", "PopulationSpikeCounter", "SpikeCounter", "CoincidenceCounter",
+    "FileSpikeMonitor", "AERSpikeMonitor",
     "StateMonitor", "Network", "MagicNetwork", "run", "reinit", "stop", "network_operation",
     "NetworkOperation", "set_global_preferences", "get_global_preference", "forget", "recall",
 ]
@@ -459,8 +460,8 @@ class SpikeGeneratorGroup(_GroupBase):
     network is reinitialised), or an iterator of (i, t) sorted in time
     (drawn as far as each run needs).  ``period`` repeats the spikes: at
     step n the spikes of bin int(round((t - int(t/period) period)/dt)),
-    t = n dt, are emitted.  Spikes of one neuron falling into one step are
-    emitted once."""
+    t = n dt, are emitted.  Several spikes of one neuron in one step are all
+    emitted, as the reference's fast threshold does (:252-262)."""
 
     def __init__(self, N, spiketimes, clock=None, period=None, sort=True, gather=False, level=0):
         self._track(level + 1)
@@ -576,10 +577,8 @@ class SpikeGeneratorGroup(_GroupBase):
         steps, idx = steps[keep] - n0, idx[keep]
         if len(idx) and (idx.min() < 0 or idx.max() >= self.N):
             raise ValueError("spike generator index out of range")
-        key = np.unique(steps * np.int64(self.N) + idx)
-        if len(key) < len(steps):
-            warnings.warn("several spikes of one neuron in one time step: the extra ones are discarded")
-        return (key // self.N).astype(np.int32), (key % self.N).astype(np.int32)
+        order = np.lexsort((idx, steps))
+        return steps[order].astype(np.int32), idx[order].astype(np.int32)
 
     def reinit(self):
         self._start_stream()
@@ -1248,6 +1247,65 @@ class SpikeMonitor(_Tracked):
             self._net.clear_raster(self.source._owner)
 
 
+class FileSpikeMonitor(SpikeMonitor):
+    """monitor.py:500-538: also writes the spikes to ``filename`` as
+    ``i, t`` lines (t in seconds as Python 2's str(float) printed it); the
+    lines of a run are written when it ends."""
+
+    def __init__(self, source, filename, record=False, delay=0):
+        SpikeMonitor.__init__(self, source, record, delay)
+        self._track(1)
+        self.filename = filename
+        self.f = open(filename, "w")
+
+    def _write(self, steps, idx):
+        from .spikeio import py2_float_str
+        dt = self.source.clock.dt
+        for i, s in zip(idx.tolist(), steps.tolist()):
+            self.f.write(f"{i}, {py2_float_str(s * dt + 0.0)}\n")
+        self.f.flush()
+
+    def reinit(self):
+        SpikeMonitor.reinit(self)
+        self.close()
+        self.f = open(self.filename, "w")
+
+    def close(self):
+        if not self.f.closed:
+            self.f.close()
+
+    close_file = close
+
+
+class AERSpikeMonitor(FileSpikeMonitor):
+    """monitor.py:546-588: the spikes in AER-DAT 2.0 (big-endian int32
+    address, int32 timestamp in ticks of dt, timestamp ceil(t/dt)); read
+    back with brian_amd.spikeio.load_aer."""
+
+    def __init__(self, source, filename, record=False, delay=0):
+        SpikeMonitor.__init__(self, source, record, delay)
+        self._track(1)
+        from .spikeio import aer_header
+        self.filename = filename
+        self.f = open(filename, "wb")
+        self.f.write(aer_header(source.clock.dt).encode())
+
+    def _write(self, steps, idx):
+        from .spikeio import aer_events
+        dt = self.source.clock.dt
+        t = steps * dt + 0.0
+        ticks = np.array(np.ceil(t / dt), dtype=np.int64)          # monitor.py:585
+        self.f.write(aer_events(idx, ticks))
+        self.f.flush()
+
+    def reinit(self):
+        SpikeMonitor.reinit(self)
+        from .spikeio import aer_header
+        self.close()
+        self.f = open(self.filename, "wb")
+        self.f.write(aer_header(self.source.clock.dt).encode())
+
+
 class PopulationSpikeCounter(SpikeMonitor):
     """brian/monitor.py:267-295."""
 
@@ -1823,6 +1881,12 @@ class Network:
             if fmons:
                 self._call_spike_functions(fmons, first, self.steps_run)
         self._rasters = {}
+        writers = [m for m in self.monitors if isinstance(m, FileSpikeMonitor)]
+        for m in writers:
+            steps, idx = m._raster(all_spikes=True)
+            sel = (steps >= first) & (steps < self.steps_run)
+            order = np.lexsort((idx[sel], steps[sel]))
+            m._write((steps[sel][order] + self._n0), idx[sel][order])
         rep.finish()
 
     def _call_spike_functions(self, fmons, s0, s1):

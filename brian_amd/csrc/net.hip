@@ -213,11 +213,16 @@ void upload_generator(bm_net *net, HostGroup &g) {
 }
 
 void check_generator_spikes(int64_t N, int64_t nspikes, const int32_t *steps, const int32_t *idx) {
+  int64_t run = 0;    // spikes in the current step: the step's spike list holds at most N
   for (int64_t q = 0; q < nspikes; ++q) {
+    run = (q && steps[q] == steps[q - 1]) ? run + 1 : 1;
+    BM_REQUIRE(run <= N, "more spikes in one step than the generator has neurons");
     BM_REQUIRE(idx[q] >= 0 && idx[q] < N, "generator index out of range");
     BM_REQUIRE(steps[q] >= 0, "generator step must be >= 0");
-    if (q) BM_REQUIRE(steps[q] > steps[q - 1] || (steps[q] == steps[q - 1] && idx[q] > idx[q - 1]),
-                      "generator spikes must be sorted by (step, index) without duplicates");
+    // repeated (step, index) pairs are emitted as often as they occur, as
+    // FastSpikeGeneratorThreshold does (directcontrol.py:252-262)
+    if (q) BM_REQUIRE(steps[q] > steps[q - 1] || (steps[q] == steps[q - 1] && idx[q] >= idx[q - 1]),
+                      "generator spikes must be sorted by (step, index)");
   }
 }
 

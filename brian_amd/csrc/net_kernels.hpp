@@ -951,7 +951,7 @@ __global__ __launch_bounds__(BLOCK) void k_generator(const GroupDev g, const int
     if (g.hist != nullptr) {
       const int64_t hs = n % g.H;
       const unsigned int hp = atomicAdd(&g.hist_count[hs], 1u);
-      g.hist[hs * g.N + hp] = (int32_t)i;
+      if (hp < (unsigned int)g.N) g.hist[hs * g.N + hp] = (int32_t)i;   // the host caps spikes per step at N
     }
     for (int o = 0; o < g.nout; ++o) {
       const OutDev &od = g.outs[o];

@@ -85,12 +85,12 @@ def test_infinite_generator_read_on_demand():
     assert (s + 45).tolist() == [50, 60, 70] and i.tolist() == [2, 0, 1]
 
 
-def test_duplicate_spikes_are_discarded():
-    """Two spikes of one neuron in one step: the extra one is dropped with a warning."""
+def test_repeated_spikes_are_kept():
+    """directcontrol.py:252-262: FastSpikeGeneratorThreshold returns every
+    spike binned into a step, repeats included."""
     import brian_amd as b
     from brian_amd import SpikeGeneratorGroup, ms
     b.reinit()
     G = SpikeGeneratorGroup(2, [(0, 0.95 * ms), (0, 1 * ms), (1, 1 * ms)])
-    with pytest.warns(UserWarning):
-        s, i = G._table(0, 20)
-    assert list(zip(i.tolist(), s.tolist())) == [(0, 10), (1, 10)]
+    s, i = G._table(0, 20)
+    assert list(zip(i.tolist(), s.tolist())) == [(0, 10), (0, 10), (1, 10)]

@@ -310,3 +310,49 @@ def test_coincidence_counter_online():
     assert online.sum() > 2 * n
     offline = np.array([gamma_factor(sm[i], train0, delta=delta, normalize=False, dt=dt) for i in range(n)])
     assert np.abs(online - offline).max() <= 1, (online, offline)
+
+
+def _spikeio_save(path, dt):
+    import brian_amd as b
+    from brian_amd import PoissonGroup, AERSpikeMonitor, FileSpikeMonitor, SpikeMonitor, run, Hz, ms
+    b.reinit()
+    b.defaultclock.dt = dt
+    g = PoissonGroup(1000, 200 * Hz)
+    Maer = AERSpikeMonitor(g, str(path / "dummy.aedat"))
+    Mtxt = FileSpikeMonitor(g, str(path / "dummy.txt"))
+    M = SpikeMonitor(g)
+    run(100 * ms)
+    Maer.close()
+    Mtxt.close()
+    return M.nspikes, M.spikes[0]
+
+
+def _spikeio_reload(addr, timestamps, dt):
+    import brian_amd as b
+    from brian_amd import SpikeGeneratorGroup, SpikeMonitor, run, ms
+    b.reinit()
+    b.defaultclock.dt = dt
+    group = SpikeGeneratorGroup(1000, (addr, timestamps))
+    newM = SpikeMonitor(group, record=True)
+    run(100 * ms)
+    return newM.nspikes
+
+
+def test_spike_io(tmp_path):
+    """brian/tests/testfeatures/test_io_spikes.py: Poisson spikes saved by
+    an AERSpikeMonitor reload with load_aer as many spikes as a
+    SpikeMonitor saw, and replayed by a SpikeGeneratorGroup give the same
+    count; a FileSpikeMonitor writes one 'i, t' line per spike."""
+    import brian_amd as b
+    from brian_amd import load_aer, ms
+    for dt in (1 * ms, 0.1 * ms):
+        n, first = _spikeio_save(tmp_path, dt)
+        addr, timestamps = load_aer(str(tmp_path / "dummy.aedat"))
+        assert len(addr) == n > 1000
+        lines = open(tmp_path / "dummy.txt").read().splitlines()
+        assert len(lines) == n
+        i0, t0 = lines[0].split(", ")
+        assert int(i0) == first[0] and abs(float(t0) - first[1]) < 1e-12
+        assert _spikeio_reload(addr, timestamps, dt) == len(addr)
+    b.reinit()
+    b.defaultclock.dt = 0.1 * ms
