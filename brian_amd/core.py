@@ -31,7 +31,7 @@ __all__ = [
     "Connection", "DelayConnection", "STDP", "ExponentialSTDP", "STP", "ConnectionMatrix",
     "SparseConnectionVector", "random_matrix", "random_matrix_fixed_column",
     "SpikeMonitor", "PopulationRateMonitor", "PopulationSpikeCounter", "SpikeCounter", "CoincidenceCounter",
-    "FileSpikeMonitor", "AERSpikeMonitor",
+    "FileSpikeMonitor", "AERSpikeMonitor", "ISIHistogramMonitor", "MultiStateMonitor", "RecentStateMonitor",
     "StateMonitor", "Network", "MagicNetwork", "run", "reinit", "stop", "network_operation",
     "NetworkOperation", "set_global_preferences", "get_global_preference", "forget", "recall",
 ]
@@ -1362,6 +1362,98 @@ class PopulationRateMonitor(SpikeMonitor):
 
     times_ = times
 
+    def smooth_rate(self, width=None, filter="gaussian"):
+        """monitor.py:640-680: the rates convolved with a normalised gaussian
+        (+-2 widths) or flat window of ``width``; a flat window without a
+        width gets its size from Shimazaki and Shinomoto's cost (2K - var)/b^2
+        over 2..499 bins."""
+        rate = np.asarray(self.rate_, dtype=np.float64)
+        if width is None:
+            if filter != "flat":
+                raise AttributeError("Automatic width selection is not implemented yet!")
+            counts = rate / self._factor
+            best, width_dt = np.inf, 1
+            for nbins in range(2, 500):
+                binsize = len(counts) // nbins
+                if binsize < 1:
+                    break
+                x = np.resize(counts, (len(counts) // binsize, binsize)).sum(1)
+                value = (2 * np.mean(x) - np.var(x)) / binsize ** 2
+                if value < best:
+                    best, width_dt = value, binsize
+        else:
+            width_dt = int(float(width) / (self._bin * self.source.clock.dt))
+        if filter == "gaussian":
+            window = np.exp(-np.arange(-2 * width_dt, 2 * width_dt + 1) ** 2 * 1. / (2 * width_dt ** 2))
+        elif filter == "flat":
+            window = np.ones(width_dt)
+        else:
+            raise KeyError(filter)
+        return np.convolve(rate, window * 1. / np.sum(window), mode="same")
+
+
+class ISIHistogramMonitor(SpikeMonitor):
+    """monitor.py:428-480: histogram of inter-spike intervals over ``bins``
+    (lower bounds; the last bin is open).  As in the reference, a neuron's
+    first spike measures its interval from t = -1000 s and lands in the last
+    bin."""
+
+    def __init__(self, source, bins, delay=0):
+        SpikeMonitor.__init__(self, source)
+        self._track(1)
+        self.bins = np.array([float(b) for b in bins])
+
+    @property
+    def count(self):
+        s, i = self._raster(all_spikes=True)
+        out = np.zeros(len(self.bins))
+        if not len(s):
+            return out
+        dt = self.source.clock.dt
+        order = np.lexsort((s, i))
+        s, i = s[order], i[order]
+        t = s * dt + 0.0
+        prev = np.concatenate(([1000.0], t[:-1]))
+        first = np.concatenate(([True], i[1:] != i[:-1]))
+        prev[first] = 1000.0
+        isi = t - prev
+        h, _ = np.histogram(isi, self.bins)
+        out[:-1] += h
+        out[-1] += len(isi) - h.sum()
+        return out
+
+
+class MultiStateMonitor(_Tracked):
+    """monitor.py:1144-1215: one StateMonitor per variable of ``G`` (all of
+    them by default); ``M[name]`` is a monitor, ``M[name, i]`` its trace of
+    neuron i."""
+
+    def __init__(self, G, vars=None, clock=None, **kwds):
+        self._track(1)
+        if vars is None:
+            vars = [n for n in G.var_names]
+        self.vars = list(vars)
+        self.monitors = {v: StateMonitor(G, v, clock=clock, **kwds) for v in self.vars}
+
+    def __getitem__(self, key):
+        if isinstance(key, tuple):
+            name, i = key
+            return self.monitors[name][i]
+        return self.monitors[key]
+
+    def items(self):
+        return self.monitors.items()
+
+    iteritems = items
+
+    @property
+    def times(self):
+        return next(iter(self.monitors.values())).times
+
+    def reinit(self):
+        for m in self.monitors.values():
+            m.reinit()
+
 
 class StateMonitor(_Tracked):
     """brian/monitor.py:683-990.  Recorded on the device inside the sweep, at
@@ -1463,6 +1555,29 @@ class StateMonitor(_Tracked):
         self._blocks, self._tblocks = [], []
         self._hsum = self._hsq = None
         self._hn = 0
+
+
+class RecentStateMonitor(StateMonitor):
+    """monitor.py:992-1110: a StateMonitor (record=True by default) whose
+    values and times keep the last int(duration/(timestep dt)) + 1 samples."""
+
+    def __init__(self, P, varname, duration=5 * ms, clock=None, record=True, timestep=1, when="end"):
+        StateMonitor.__init__(self, P, varname, clock=clock, record=record, timestep=timestep, when=when)
+        self._track(1)
+        self.duration = float(duration)
+        self.num_duration = int(self.duration / (timestep * P.clock.dt)) + 1
+
+    @property
+    def values(self):
+        return StateMonitor.values.fget(self)[:, -self.num_duration:]
+
+    @property
+    def times(self):
+        return StateMonitor.times.fget(self)[-self.num_duration:]
+
+    def __getitem__(self, i):
+        k = int(np.nonzero(self.record == i)[0][0])
+        return self.values[k]
 
 
 class CoincidenceCounter(_Tracked):
@@ -1624,6 +1739,8 @@ class Network:
             elif isinstance(o, CoincidenceCounter):
                 if o not in self.coincs:
                     self.coincs.append(o)
+            elif isinstance(o, MultiStateMonitor):
+                self.add(list(o.monitors.values()))
             elif isinstance(o, Connection):
                 if o not in self.conns:
                     self.conns.append(o)

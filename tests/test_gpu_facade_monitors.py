@@ -356,3 +356,41 @@ def test_spike_io(tmp_path):
         assert _spikeio_reload(addr, timestamps, dt) == len(addr)
     b.reinit()
     b.defaultclock.dt = 0.1 * ms
+
+
+def test_more_monitors():
+    """monitor.py:428-480 ISIHistogramMonitor on known trains (a neuron's
+    first spike counts in the last bin, as in the reference); :640-680
+    smooth_rate keeps a constant rate constant inside the run; :1144-1215
+    MultiStateMonitor equals separate StateMonitors; :992-1110
+    RecentStateMonitor keeps the last duration of samples."""
+    from brian_amd import (SpikeGeneratorGroup, ISIHistogramMonitor, PopulationRateMonitor, NeuronGroup,
+                           MultiStateMonitor, StateMonitor, RecentStateMonitor, Network, ms,
+                           reinit_default_clock)
+    reinit_default_clock()
+    st = [(0, 1 * ms), (0, 4 * ms), (0, 16 * ms), (1, 2 * ms), (1, 9 * ms)]
+    G = SpikeGeneratorGroup(2, st)
+    H = ISIHistogramMonitor(G, [0 * ms, 5 * ms, 10 * ms])
+    R = PopulationRateMonitor(G, bin=1 * ms)
+    net = Network(G, H, R)
+    net.run(20 * ms)
+    # ISIs 3 ms, 12 ms, 7 ms, plus two first spikes (last bin)
+    np.testing.assert_array_equal(H.count, [1, 1, 3])
+    assert len(R.rate) == 20
+    sm = R.smooth_rate(2 * ms, filter="flat")
+    assert abs(sm.sum() - R.rate.sum()) < 1e-9 * R.rate.sum()
+    reinit_default_clock()
+    P = NeuronGroup(3, "dv/dt = (2 - v)/(10*ms) : 1\ndw/dt = -w/(5*ms) : 1")
+    P.v = [0.0, 1.0, 2.0]
+    P.w = [1.0, 2.0, 3.0]
+    MM = MultiStateMonitor(P, record=True)
+    Mv = StateMonitor(P, "v", record=True)
+    Rw = RecentStateMonitor(P, "w", duration=1 * ms)
+    net = Network(P, MM, Mv, Rw)
+    net.run(5 * ms)
+    np.testing.assert_array_equal(MM["v"].values, Mv.values)
+    np.testing.assert_array_equal(MM["v", 1], Mv[1])
+    assert MM["w"].values.shape == (3, 50)
+    assert Rw.values.shape == (3, 11)
+    np.testing.assert_array_equal(Rw[2], MM["w"][2][-11:])
+    np.testing.assert_allclose(Rw.times, MM.times[-11:])
