@@ -107,12 +107,18 @@ def tune(h, trial_steps=256, reps=2, sweeps=2, knobs=KNOB_ORDER, log=None, max_s
     return best, used
 
 
-def tune_layout(build, layouts=({"ring_bits": 0, "ring_pack": 1}, {"ring_bits": 16, "ring_pack": 1},
-                                {"ring_bits": 0, "ring_pack": 0}), trial_steps=256, reps=2):
-    """Offline search over ring layouts.  ``build(layout)`` returns a fresh
-    prepared handle with the layout applied before preparation; each is run
-    for a few trial segments and closed.  Returns (best layout, seconds per
-    step of each layout)."""
+LAYOUTS = ({"ring_bits": 0, "ring_pack": 1}, {"ring_bits": 16, "ring_pack": 1},
+           {"ring_bits": 0, "ring_pack": 0},
+           {"ring_bits": 0, "ring_pack": 1, "defer": 1, "neurons_per_thread": 1},
+           {"ring_bits": 0, "ring_pack": 1, "defer": 1, "neurons_per_thread": 2})
+
+
+def tune_layout(build, layouts=LAYOUTS, trial_steps=256, reps=2):
+    """Offline search over preparation-time choices: ring layouts and
+    deferred propagation (which fixes neurons per thread).  ``build(layout)``
+    returns a fresh prepared handle with the layout applied before
+    preparation; each is run for a few trial segments and closed.  Returns
+    (best layout, seconds per step of each layout)."""
     res = []
     for lay in layouts:
         try:

@@ -58,7 +58,7 @@ def build_cuba_fixed_indegree(N, k=80, seed=2, delay_steps=10, precision=32, win
 
 def _prepare(h, tuning, window, npt, use_graph):
     tuning = tuning or {}
-    for key in ("ring_bits", "ring_pack"):
+    for key in ("ring_bits", "ring_pack", "defer"):
         if key in tuning:
             h.set_tuning(key, tuning[key])
     h.prepare(window=tuning.get("window", window), neurons_per_thread=tuning.get("neurons_per_thread", npt),

@@ -70,6 +70,8 @@ struct HostGroup {
     DevBuf tsteps_d, sdelay_d, sidx_d, last_d, next_d, lallow_d, nallow_d, coinc_d, model_d, target_d, dev_d;
   };
   std::unique_ptr<Coin> coin;
+  bool defer = false;        // deferred propagation (specialised lean kernel)
+  DevBuf dlist_d, dcount_d, dbase_d, dsel_d;
   int H = 0;
   // device
   DevBuf S, last, Apn_d, Cpn_d, code_d, consts_d, refr_d, ti_d, pthr_d, gen_idx_d, gen_off_d,
@@ -148,6 +150,8 @@ struct bm_net {
   // tiling knobs (bm_net_set_tuning)
   int wave_max = BLOCK / 64, row_split = 64;
   int ring_bits = 0, ring_pack = 1;
+  int defer_req = 0;      // BM_TUNE_DEFER before prepare
+  int defer_W = 0;        // largest window with deferred propagation, 0 = off
   int64_t step = 0;       // next step to run
   hipStream_t stream = nullptr;
   DevBuf d_base;          // int64 device step base
@@ -316,7 +320,8 @@ JitKernel jit_sweep(bm_net *net, const HostGroup &g, int K) {
     << "\n#define BM_JIT_CH_DELAY " << jit_list(dly) << "\n#define BM_JIT_RS_VAR " << jit_list(rv)
     << "\n#define BM_JIT_RS_OP " << jit_list(ro) << "\n#define BM_JIT_RS_VVAR " << jit_list(rvv)
     << "\n#define BM_JIT_TH_KIND " << g.th_kind << "\n#define BM_JIT_TH_VAR " << g.th_var
-    << "\n#define BM_JIT_TH_VTVAR " << g.th_vtvar << "\n#include \"net_kernels.hpp\"\n";
+    << "\n#define BM_JIT_TH_VTVAR " << g.th_vtvar << "\n#define BM_JIT_DEFER " << (g.defer ? 1 : 0)
+    << "\n#include \"net_kernels.hpp\"\n";
   const std::string name = std::string("bm::k_sweep<") + (net->precision == 64 ? "double" : "float") + ", " +
                            std::to_string(g.M) + ", " + std::to_string(K) + ", 0>";
   const std::string src = d.str() + "// " + name + "\n";
@@ -474,6 +479,14 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
   if (!net->stdps.empty()) Wcap = 1;
   int W = window > 0 ? window : Wcap;
   if (!net->stdps.empty()) W = 1;
+  // deferred propagation delivers a window's spikes during the next launch:
+  // they must land in slots no launch consumes before then, W <= (dmin+1)/2
+  net->defer_W = 0;
+  if (net->defer_req && net->specialize && net->stdps.empty() && !net->conns.empty() && (dmin + 1) / 2 >= 1) {
+    net->defer_W = std::min(WMAX, (dmin + 1) / 2);
+    BM_REQUIRE(window <= net->defer_W, "window exceeds (minimum delay + 1) / 2 with deferred propagation");
+    W = std::min(W, net->defer_W);
+  }
   BM_REQUIRE(W >= 1 && W <= WMAX, "window out of range [1,16]");
   BM_REQUIRE(net->conns.empty() || W <= dmin + 1,
              "window exceeds (minimum synaptic delay + 1) steps");
@@ -744,6 +757,24 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       cd.exclusive = c.exclusive;
       upload_raw(c.dev_d, &cd, 1, net->stream);
       d.coin = c.dev_d.as<CoinDev>();
+    }
+    g.defer = net->defer_W > 0 && fast_eligible(g) && g.kind == GK_NEURON;
+    if (g.defer) {
+      const int K = group_K(net, g);
+      const int64_t nb = (g.N + (int64_t)BLOCK * K - 1) / ((int64_t)BLOCK * K);
+      const int64_t cap = (int64_t)BLOCK * K * net->defer_W;
+      g.dlist_d.alloc((size_t)nb * 2 * cap * sizeof(uint16_t));
+      g.dcount_d.alloc((size_t)nb * 2 * sizeof(int32_t));
+      g.dbase_d.alloc((size_t)nb * 2 * sizeof(int64_t));
+      g.dsel_d.alloc((size_t)nb * sizeof(int32_t));
+      BM_HIP(hipMemsetAsync(g.dcount_d.p, 0, g.dcount_d.bytes, net->stream));
+      BM_HIP(hipMemsetAsync(g.dbase_d.p, 0, g.dbase_d.bytes, net->stream));
+      BM_HIP(hipMemsetAsync(g.dsel_d.p, 0, g.dsel_d.bytes, net->stream));
+      d.dlist = g.dlist_d.as<uint16_t>();
+      d.dcount = g.dcount_d.as<int32_t>();
+      d.dbase = g.dbase_d.as<int64_t>();
+      d.dsel = g.dsel_d.as<int32_t>();
+      d.dcap = cap;
     }
     d.npin = (int)g.pins.size();
     for (int q = 0; q < d.npin; ++q) d.pins[q] = g.pins[q];
@@ -1713,8 +1744,10 @@ int bm_net_set_tuning(bm_net *net, int key, int value) {
     switch (key) {
       case BM_TUNE_WINDOW: {
         BM_REQUIRE(net->prepared, "the window is set by bm_net_prepare before preparation");
-        const int W = value == 0 ? net->Wcap : value;
-        BM_REQUIRE(W >= 1 && W <= net->Wcap, "window out of range [1, minimum delay + 1]");
+        const int wmax = net->defer_W > 0 ? net->defer_W : net->Wcap;
+        const int W = value == 0 ? wmax : value;
+        BM_REQUIRE(W >= 1 && W <= wmax, "window out of range [1, minimum delay + 1] ((minimum delay + 1) / 2 "
+                                        "with deferred propagation)");
         BM_REQUIRE(net->stdps.empty() || W == 1, "networks with STDP run one step per launch");
         const int old = net->W;
         net->W = W;
@@ -1728,6 +1761,9 @@ int bm_net_set_tuning(bm_net *net, int key, int value) {
       }
       case BM_TUNE_NEURONS_PER_THREAD: {
         BM_REQUIRE(value == 1 || value == 2 || value == 4, "neurons_per_thread must be 1, 2 or 4");
+        // deferred lists are kept per workgroup of the prepared launch shape
+        BM_REQUIRE(net->defer_W == 0 || value == net->K,
+                   "neurons_per_thread is fixed at prepare with deferred propagation");
         const int old = net->K;
         net->K = value;
         if (net->prepared) {
@@ -1765,7 +1801,12 @@ int bm_net_set_tuning(bm_net *net, int key, int value) {
         net->use_graph = value != 0;
         break;
       case BM_TUNE_SPECIALIZE:
+        BM_REQUIRE(net->defer_W == 0 || value != 0, "deferred propagation needs the specialised kernels");
         net->specialize = value != 0;
+        break;
+      case BM_TUNE_DEFER:
+        require_unprepared(net);
+        net->defer_req = value != 0;
         break;
       default:
         throw Error(BM_EINVAL, "unknown tuning key");
@@ -1790,8 +1831,9 @@ int bm_net_get_tuning(bm_net *net, int key, int *value) {
       case BM_TUNE_RING_BITS: *value = net->ring_bits; break;
       case BM_TUNE_RING_PACK: *value = net->ring_pack; break;
       case BM_TUNE_USE_GRAPH: *value = net->use_graph ? 1 : 0; break;
-      case BM_TUNE_WINDOW_MAX: *value = net->prepared ? net->Wcap : 0; break;
+      case BM_TUNE_WINDOW_MAX: *value = net->prepared ? (net->defer_W > 0 ? net->defer_W : net->Wcap) : 0; break;
       case BM_TUNE_SPECIALIZE: *value = net->specialize ? 1 : 0; break;
+      case BM_TUNE_DEFER: *value = net->prepared ? (net->defer_W > 0 ? 1 : 0) : net->defer_req; break;
       default: throw Error(BM_EINVAL, "unknown tuning key");
     }
   });

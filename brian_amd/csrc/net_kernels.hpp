@@ -445,6 +445,9 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
   constexpr int J_RS_VAR[4] = {BM_JIT_RS_VAR}, J_RS_OP[4] = {BM_JIT_RS_OP}, J_RS_VVAR[4] = {BM_JIT_RS_VVAR};
   constexpr int J_TH_KIND = BM_JIT_TH_KIND, J_TH_VAR = BM_JIT_TH_VAR, J_TH_VTVAR = BM_JIT_TH_VTVAR;
   constexpr int J_NRING = BM_JIT_NRING;
+  constexpr bool J_DEFER = BM_JIT_DEFER != 0;
+#else
+  constexpr bool J_DEFER = false;
 #endif
   extern __shared__ __align__(16) unsigned char smem[];
   uint16_t *sh_list = reinterpret_cast<uint16_t *>(smem);   // (w << 12) | local index
@@ -497,6 +500,20 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
   const int list_cap = BLOCK * K * (W > 0 ? W : 1);
   T *sh_mval = reinterpret_cast<T *>(smem + ((sizeof(uint16_t) * list_cap + 15) / 16) * 16);
   T *reg = GEN && g.mcap ? sh_mval + list_cap : sh_mval;
+  // deferred propagation: the spike list this workgroup stored in the last
+  // launch (its deliveries fall in slots this launch does not consume)
+  int dsel = 0, dprev = 0;
+  int64_t dprev_base = 0;
+  const uint16_t *dprev_list = nullptr;
+  if constexpr (J_DEFER) {
+    if (!tail_only) {
+      dsel = g.dsel[blockIdx.x];
+      dprev = g.dcount[2 * (int64_t)blockIdx.x + dsel];
+      dprev_base = g.dbase[2 * (int64_t)blockIdx.x + dsel];
+      dprev_list = g.dlist + (2 * (int64_t)blockIdx.x + dsel) * g.dcap;
+      if ((blockIdx.x & 1) == 0 && dprev > 0) propagate_block<T>(g, dprev_list, sh_mval, dprev, dprev_base, blk0, tid);
+    }
+  }
   const int nsteps = tail_only ? 1 : W;
   // ring slot of the tail at window offset w is (n0 - 1 + w) mod D
   int slot0 = (int)((n0 - 1) % D);
@@ -890,7 +907,21 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
       for (int p = tid; p < nsp; p += BLOCK)
         g.hist[hs * N + sh_hbase + p] = (int32_t)(blk0 + (sh_list[p] & 0xFFFu));
     }
-    propagate_block<T>(g, sh_list, sh_mval, nsp, n0, blk0, tid);
+    if constexpr (!J_DEFER) propagate_block<T>(g, sh_list, sh_mval, nsp, n0, blk0, tid);
+  }
+  if constexpr (J_DEFER) {
+    if (!tail_only) {
+      // keep this window's list for the next launch, deliver the last one
+      const int wsel = 1 - dsel;
+      uint16_t *dst = g.dlist + (2 * (int64_t)blockIdx.x + wsel) * g.dcap;
+      for (int p = tid; p < nsp; p += BLOCK) dst[p] = sh_list[p];
+      if (tid == 0) {
+        g.dcount[2 * (int64_t)blockIdx.x + wsel] = nsp;
+        g.dbase[2 * (int64_t)blockIdx.x + wsel] = n0;
+        g.dsel[blockIdx.x] = wsel;
+      }
+      if ((blockIdx.x & 1) == 1 && dprev > 0) propagate_block<T>(g, dprev_list, sh_mval, dprev, dprev_base, blk0, tid);
+    }
   }
 
   // ---------------- write back

@@ -174,6 +174,17 @@ struct GroupDev {
   int stp_delay, npin;
   PinDev pins[MAXPIN];
   const CoinDev *coin;        // device copy, null if the group has no counter
+  // deferred propagation (lean kernel): each workgroup stores its window's
+  // spike list and delivers it during the next launch, half of the
+  // workgroups before their sweep and half after, so that delivery atomics
+  // overlap the sweeps of other workgroups.  Per workgroup two lists of dcap
+  // entries, their counts and first steps, and the index of the list the
+  // last launch wrote.
+  uint16_t *dlist;
+  int32_t *dcount;
+  int64_t *dbase;
+  int32_t *dsel;
+  int64_t dcap;
 };
 // passed by value to every sweep launch; kernel arguments are limited to 4 KiB
 static_assert(sizeof(GroupDev) <= 3584, "GroupDev must leave room in the 4 KiB kernel arguments");

@@ -278,7 +278,13 @@ int bm_net_prepare(bm_net *net, int window, int neurons_per_thread, int use_grap
  *   BM_TUNE_SPECIALIZE         1 (default): the lean sweep kernel of each group
  *                              is compiled at run time (hipRTC) with the group's
  *                              ring layout, channel table, resets and threshold
- *                              as constants; 0: the precompiled generic form */
+ *                              as constants; 0: the precompiled generic form
+ *   BM_TUNE_DEFER              (before prepare) 1: specialised lean groups
+ *                              deliver each window's spikes during the next
+ *                              launch, half of the workgroups before and half
+ *                              after their sweep, so delivery atomics overlap
+ *                              sweeps; limits the window to (minimum delay +
+ *                              1) / 2 and fixes neurons_per_thread */
 enum {
   BM_TUNE_WINDOW = 1,
   BM_TUNE_NEURONS_PER_THREAD = 2,
@@ -288,7 +294,8 @@ enum {
   BM_TUNE_RING_PACK = 6,
   BM_TUNE_USE_GRAPH = 7,
   BM_TUNE_WINDOW_MAX = 8,
-  BM_TUNE_SPECIALIZE = 9
+  BM_TUNE_SPECIALIZE = 9,
+  BM_TUNE_DEFER = 10
 };
 int bm_net_set_tuning(bm_net *net, int key, int value);
 int bm_net_get_tuning(bm_net *net, int key, int *value);

@@ -219,3 +219,48 @@ def test_specialised_sweep_matches_generic():
     np.testing.assert_array_equal(i1, i0)
     for a, b in zip(st1, st0):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_deferred_propagation_matches_immediate(precision):
+    """Deferred propagation (each window's spikes delivered during the next
+    launch) gives the same raster and state as immediate delivery: count
+    deliveries commute, and they land in slots no launch consumes earlier.
+    Runs of uneven lengths and a window change between runs exercise the
+    lists kept across launches and runs."""
+    from brian_amd._lib import NetHandle
+    from tests.netbuild import cuba_propagator
+    d = W.cuba(4000, seed=29, delay_steps=10)
+    A, C = cuba_propagator()
+    out = []
+    for defer in (0, 1):
+        h = NetHandle(1e-4, precision)
+        g = h.add_group(d["S0"])
+        h.set_linear(g, A, C)
+        h.set_threshold(g, 0, d["Vt"])
+        h.add_reset_op(g, 0, 0, d["Vr"])
+        N, Ne, Ni = d["N"], d["Ne"], d["Ni"]
+        h.add_connection(g, 0, Ne, g, 0, N, 1, *d["exc"], delay_steps=10)
+        h.add_connection(g, Ne, Ni, g, 0, N, 2, *d["inh"], delay_steps=10)
+        h.record(g, 1 << 20)
+        h.set_tuning("defer", defer)
+        h.prepare(neurons_per_thread=1)
+        assert h.get_tuning("defer") == defer
+        if defer:
+            assert h.get_tuning("window_max") == 5 and h.get_tuning("window") == 5
+            with pytest.raises(Exception):
+                h.set_tuning("neurons_per_thread", 2)
+        h.run(777)
+        h.set_tuning("window", 3)
+        h.run(1234)
+        h.set_tuning("window", 0)
+        h.run(989)
+        out.append((h.get_spikes(g), [h.get_state(g, v, N) for v in range(3)]))
+        h.close()
+    (s0, i0), st0 = out[0]
+    (s1, i1), st1 = out[1]
+    assert len(s0) > 1000
+    np.testing.assert_array_equal(s1, s0)
+    np.testing.assert_array_equal(i1, i0)
+    for a, b in zip(st1, st0):
+        np.testing.assert_array_equal(a, b)
