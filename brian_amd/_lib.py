@@ -68,6 +68,7 @@ _SIGS = {
     "bm_net_set_modulation": [P, C.c_int, C.c_int],
     "bm_net_add_stp": [P, C.c_int, Pd, Pd, Pd, C.c_int],
     "bm_net_set_stdp_delays": [P, C.c_int, Pd],
+    "bm_net_set_stdp_rule": [P, C.c_int, C.c_int, dbl, dbl, C.c_int, dbl, dbl],
     "bm_net_get_stp": [P, C.c_int, C.c_int, Pd],
     "bm_net_connect_fixed_indegree": [P, C.c_int, i64, i64, i64, i64, C.c_int, i64, i64, C.c_int, C.c_int,
                                       dbl, u64, C.c_int, C.POINTER(C.c_int)],
@@ -374,6 +375,11 @@ class NetHandle:
             d = f64(syn_delay_s)
             call("bm_net_set_stdp_delays", self.h, sid.value, dptr(d))
         return sid.value
+
+    def set_stdp_rule(self, sid, pre, post):
+        """pre/post: (mode, off, scale) weight rules (bm_net_set_stdp_rule)."""
+        call("bm_net_set_stdp_rule", self.h, sid, int(pre[0]), float(pre[1]), float(pre[2]), int(post[0]),
+             float(post[1]), float(post[2]))
 
     def add_state_monitor(self, gid, var, indices=None, timestep=1, capacity=0):
         """Device StateMonitor; indices None = record=False (sum / sum of squares)."""

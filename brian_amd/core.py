@@ -1098,18 +1098,148 @@ class DelayConnection(Connection):
 
 
 # ------------------------------------------------------------------ STDP
+_NUM = r"[^()]+|\([^()]*\)"
+
+
+def _stdp_statements(code):
+    """Statements of STDP pre/post code, split on newlines and ';'."""
+    return [t.strip() for line in code.split("\n") for t in line.split(";") if t.strip()]
+
+
 class STDP(_Tracked):
-    """Generic STDP is restricted to the exponential rule; see ExponentialSTDP."""
+    """brian/stdp.py:181-424 for the pair-based rules the device engine
+    runs: ``eqs`` holds two traces that decay exactly (dX/dt = -X/tau); the
+    pre code changes one of them (``X += c`` or ``X = c``) and the weights,
+    the post code the other; a weight line uses the other side's trace Y as
+    ``w += Y``, ``w += Y*s``, ``w += s*Y``, ``w += (Y - a)*s``,
+    ``w += s*(Y - a)``, ``w *= (1 + Y)`` or ``w += (wmax - w)*Y``; weights are
+    clipped to [wmin, wmax].  Constants come from the caller's namespace
+    (stdp.py:270-280 freezes them the same way).  Other rules raise
+    NotImplementedError."""
 
-    def __init__(self, *a, **k):
-        raise NotImplementedError("only ExponentialSTDP is supported by the device backend")
+    def __init__(self, C, eqs, pre, post, wmin=0, wmax=np.inf, level=0, clock=None, delay_pre=None,
+                 delay_post=None):
+        self._track(1)
+        if isinstance(C, DelayConnection) and (delay_pre is not None or delay_post is not None):
+            raise ValueError("Must use delay_pre=delay_post=None for the moment.")   # stdp.py:339-340
+        if delay_pre is not None or delay_post is not None:
+            raise NotImplementedError("explicit delay_pre/delay_post are not supported")
+        ns = dict(_caller_ns(level + 1))
+        if isinstance(eqs, str):
+            eqs = Equations(eqs, level=level + 1)
+        traces = [n for n in eqs._order if n in eqs._diffeq]
+        if len(traces) != 2 or eqs._params or eqs._static:
+            raise NotImplementedError("STDP with two decaying traces only")
+        ns.update(eqs._namespace)
+        self._traces = traces
+        self.C = C
+        self.wmin, self.wmax = float(wmin), float(wmax)
+        self._ns = ns
+        pre_s, post_s = _stdp_statements(pre), _stdp_statements(post)
+        xp, op_p, cp = self._trace_line(pre_s, traces)
+        xm, op_m, cm = self._trace_line(post_s, traces)
+        if xp == xm:
+            raise Exception(str([xp]) + " are both presynaptic and postsynaptic!")
+        self._eqs, self._xpre, self._xpost = eqs, xp, xm
+        self.Ap, self.Am = cp, cm
+        self.update = 0
+        # interactions code: 0 both add, 1 both set, 3 pre adds / post sets, 4 pre sets / post adds
+        self.interactions = {("+=", "+="): 0, ("=", "="): 1, ("+=", "="): 3, ("=", "+="): 4}[(op_p, op_m)]
+        self.rules = (self._weight_rule(pre_s, xm), self._weight_rule(post_s, xp))
+        self.sid = None
+        self._net = None
+
+    def _eval(self, expr):
+        """A constant of the code, from the caller's namespace."""
+        if any(re.search(r"\b" + re.escape(v) + r"\b", expr) for v in self._traces + ["w"]):
+            raise NotImplementedError(f"unsupported STDP expression {expr!r}")
+        try:
+            return float(eval(expr, {"__builtins__": {}}, self._ns))
+        except Exception as e:
+            raise NotImplementedError(f"cannot evaluate {expr!r} in the STDP code: {e}")
+
+    def _trace_line(self, stmts, traces):
+        found = [t for t in stmts if re.match(r"^(\w+)\s*(\+=|=)", t) and re.match(r"^(\w+)", t).group(1) in traces]
+        if len(found) != 1:
+            raise NotImplementedError("each of pre and post must change one trace once")
+        m = re.match(r"^(\w+)\s*(\+=|=)\s*(.+)$", found[0])
+        return m.group(1), m.group(2), self._eval(m.group(3))
+
+    def _weight_rule(self, stmts, Y):
+        """(mode, off, scale) of the code's w line (mode 0 with scale 0 when
+        there is none: the clip alone)."""
+        lines = [t for t in stmts if re.match(r"^w\s*(\+=|\*=|=)", t)]
+        if not lines:
+            return (0, 0.0, 0.0)
+        if len(lines) > 1:
+            raise NotImplementedError("one weight line per pre/post code")
+        m = re.match(r"^w\s*(\+=|\*=)\s*(.+?)\s*$", lines[0])
+        if not m:
+            raise NotImplementedError(f"unsupported weight line {lines[0]!r}")
+        op, rhs = m.group(1), m.group(2)
+        y = re.escape(Y)
+        if op == "*=":
+            if re.fullmatch(r"\(\s*1\.?0*\s*\+\s*" + y + r"\s*\)", rhs):
+                return (1, 0.0, 1.0)
+            raise NotImplementedError(f"unsupported weight line {lines[0]!r}")
+        mm = re.fullmatch(r"\(\s*(\w+)\s*-\s*w\s*\)\s*\*\s*" + y, rhs)
+        if mm:
+            if self._eval(mm.group(1)) != self.wmax:
+                raise NotImplementedError("soft bound other than wmax")
+            return (2, 0.0, 1.0)
+        if re.fullmatch(y, rhs):
+            return (0, 0.0, 1.0)
+        for pat in (y + r"\s*\*\s*(" + _NUM + r")", r"(" + _NUM + r")\s*\*\s*" + y):
+            mm = re.fullmatch(pat, rhs)
+            if mm and not re.search(r"\b" + y + r"\b|\bw\b", mm.group(1)):
+                return (0, 0.0, self._eval(mm.group(1)))
+        sh = r"\(\s*" + y + r"\s*([+-])\s*(" + _NUM + r")\s*\)"
+        mm = re.fullmatch(sh + r"\s*\*\s*(" + _NUM + r")", rhs)           # (Y - a) * s
+        if mm:
+            off = self._eval(mm.group(2))
+            return (0, off if mm.group(1) == "+" else -off, self._eval(mm.group(3)))
+        mm = re.fullmatch(r"(" + _NUM + r")\s*\*\s*" + sh, rhs)           # s * (Y - a)
+        if mm:
+            off = self._eval(mm.group(3))
+            return (0, off if mm.group(2) == "+" else -off, self._eval(mm.group(1)))
+        mm = re.fullmatch(sh, rhs)                                           # (Y - a)
+        if mm:
+            off = self._eval(mm.group(2))
+            return (0, off if mm.group(1) == "+" else -off, 1.0)
+        raise NotImplementedError(f"unsupported weight line {lines[0]!r}")
+
+    def decays(self, dt):
+        out = []
+        for x in (self._xpre, self._xpost):
+            e = Equations(f"d{x}/dt = {self._eqs._diffeq[x]} : 1", **self._eqs._namespace)
+            A, C = e.linear_propagator(dt)
+            if A.shape != (1, 1) or C[0] != 0.0:
+                raise NotImplementedError("STDP traces must decay to zero")
+            out.append(A[0, 0])
+        return out[0], out[1]
+
+    @property
+    def traces(self):
+        net = self._net
+        n_pre, n_post = len(self.C.source), len(self.C.target)
+        return net.handle.get_stdp_traces(self.sid, n_pre, n_post)
+
+    def __getattr__(self, name):
+        # stdp.A_pre / stdp.A_post (stdp.py:426-449)
+        if name.startswith("_") or "_xpre" not in self.__dict__:
+            raise AttributeError(name)
+        if name == self._xpre:
+            return self.traces[0]
+        if name == self._xpost:
+            return self.traces[1]
+        raise AttributeError(name)
 
 
-class ExponentialSTDP(_Tracked):
+class ExponentialSTDP(STDP):
     """brian/stdp.py:452-539 (with the trace groups and updaters of
     :181-424): traces decay with the exact 1x1 propagator, pre spikes
     (delay = connection delay) update rows, post spikes update columns,
-    weights clipped to [wmin, wmax]."""
+    weights clipped to [wmin, wmax]; additive, multiplicative or mixed."""
 
     def __init__(self, C, taup, taum, Ap, Am, interactions="all", wmin=0, wmax=None,
                  update="additive", delay_pre=None, delay_post=None, clock=None):
@@ -1123,17 +1253,34 @@ class ExponentialSTDP(_Tracked):
         wmax = float(wmax)
         self.C = C
         self.taup, self.taum = float(taup), float(taum)
+        if interactions not in ("all", "nearest", "nearest_pre", "nearest_post"):
+            raise AttributeError("Unknown interaction type " + interactions)
         self.interactions = {"all": 0, "nearest": 1, "nearest_pre": 2, "nearest_post": 3}[interactions]
+        self.rules = None
         if update == "additive":
             Ap *= wmax
             Am *= wmax
             self.update = 0
         elif update == "multiplicative":
             self.update = 1
+        elif update == "mixed":
+            # stdp.py:515-530: the depressing side multiplicative, the other additive
+            self.update = 0
+            if Am < 0 and Ap > 0:
+                Ap *= wmax
+                self.rules = ((1, 0.0, 1.0), (0, 0.0, 1.0))
+            elif Am > 0 and Ap < 0:
+                Am *= wmax
+                self.rules = ((0, 0.0, 1.0), (1, 0.0, 1.0))
+            elif Am > 0:
+                raise AttributeError("There is no depression in STDP rule")
+            else:
+                raise AttributeError("There is no potentiation in STDP rule")
         else:
-            raise NotImplementedError("mixed STDP updates are not supported")
+            raise AttributeError("Unknown update type " + update)
         self.Ap, self.Am = float(Ap), float(Am)
         self.wmin, self.wmax = float(wmin), wmax
+        self._xpre, self._xpost = "A_pre", "A_post"
         self.sid = None
         self._net = None
 
@@ -1141,12 +1288,6 @@ class ExponentialSTDP(_Tracked):
         ep = Equations("dA/dt = -A/taup : 1", taup=self.taup)
         em = Equations("dA/dt = -A/taum : 1", taum=self.taum)
         return ep.linear_propagator(dt)[0][0, 0], em.linear_propagator(dt)[0][0, 0]
-
-    @property
-    def traces(self):
-        net = self._net
-        n_pre, n_post = len(self.C.source), len(self.C.target)
-        return net.handle.get_stdp_traces(self.sid, n_pre, n_post)
 
 
 class _STPVars:
@@ -1727,7 +1868,7 @@ class Network:
                 g = o._owner
                 if g not in self.groups:
                     self.groups.append(g)
-            elif isinstance(o, ExponentialSTDP):
+            elif isinstance(o, STDP):
                 if o not in self.stdps:
                     self.stdps.append(o)
             elif isinstance(o, STP):
@@ -1866,6 +2007,8 @@ class Network:
             dly = s.C._delays if isinstance(s.C, DelayConnection) else None
             s.sid = h.add_stdp(s.C.cid, dp, dm, s.Ap, s.Am, s.wmin, s.wmax, s.update, s.interactions,
                                syn_delay_s=dly)
+            if s.rules is not None:
+                h.set_stdp_rule(s.sid, *s.rules)
             s._net = self
         self._capacity = {}
         for g in self.groups:

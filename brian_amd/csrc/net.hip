@@ -123,6 +123,8 @@ struct HostStdp {
   int conn = 0;
   double decay_pre = 0, decay_post = 0, Ap = 0, Am = 0, wmin = 0, wmax = 0;
   int update = 0, interactions = 0;
+  int pre_mode = 0, post_mode = 0;                 // weight rules (StdpDev)
+  double pre_off = 0, pre_scale = 1, post_off = 0, post_scale = 1;
   // DelayConnection: per-synapse history offsets (CSR order), the delay of
   // the weight part of pre updates, and the history depth
   std::vector<int32_t> kpre, kpost;
@@ -909,6 +911,12 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     d.wmin = s.wmin;
     d.wmax = s.wmax;
     d.update = s.update;
+    d.pre_mode = s.pre_mode;
+    d.post_mode = s.post_mode;
+    d.pre_off = s.pre_off;
+    d.pre_scale = s.pre_scale;
+    d.post_off = s.post_off;
+    d.post_scale = s.post_scale;
     d.interactions = s.interactions;
     d.pre_delay = c.hom_delay;
     // a Connection with a homogeneous delay reads its weights when it
@@ -1555,6 +1563,22 @@ int bm_net_get_stp(bm_net *net, int cid, int which, double *out) {
   });
 }
 
+int bm_net_set_stdp_rule(bm_net *net, int sid, int pre_mode, double pre_off, double pre_scale, int post_mode,
+                         double post_off, double post_scale) {
+  return guard([&] {
+    require_unprepared(net);
+    BM_REQUIRE(sid >= 0 && sid < (int)net->stdps.size(), "bad STDP id");
+    BM_REQUIRE(pre_mode >= 0 && pre_mode <= 2 && post_mode >= 0 && post_mode <= 2, "weight rule mode out of range");
+    HostStdp &s = *net->stdps[sid];
+    s.pre_mode = pre_mode;
+    s.pre_off = pre_off;
+    s.pre_scale = pre_scale;
+    s.post_mode = post_mode;
+    s.post_off = post_off;
+    s.post_scale = post_scale;
+  });
+}
+
 int bm_net_set_stdp_delays(bm_net *net, int sid, const double *syn_delay_s) {
   return guard([&] {
     require_unprepared(net);
@@ -1635,7 +1659,7 @@ int bm_net_add_stdp(bm_net *net, int cid, double decay_pre, double decay_post, d
     BM_REQUIRE(c.stdp < 0, "connection already has STDP");
     BM_REQUIRE(!c.stp, "STDP on a connection with short-term plasticity is not supported");
     BM_REQUIRE(update == 0 || update == 1, "update must be 0 (additive) or 1 (multiplicative)");
-    BM_REQUIRE(interactions >= 0 && interactions <= 3, "interactions out of range");
+    BM_REQUIRE(interactions >= 0 && interactions <= 4, "interactions out of range");
     auto s = std::make_unique<HostStdp>();
     s->conn = cid;
     s->decay_pre = decay_pre;
@@ -1646,6 +1670,11 @@ int bm_net_add_stdp(bm_net *net, int cid, double decay_pre, double decay_post, d
     s->wmax = wmax;
     s->update = update;
     s->interactions = interactions;
+    // ExponentialSTDP's weight lines (stdp.py:507-523): additive w += A;
+    // multiplicative w *= (1 + A) when the other side depresses, else
+    // w += (wmax - w) * A
+    s->pre_mode = update == 0 ? 0 : (Am < 0 ? 1 : 2);
+    s->post_mode = update == 0 ? 0 : (Ap < 0 ? 1 : 2);
     net->stdps.push_back(std::move(s));
     c.stdp = (int)net->stdps.size() - 1;
     *sid = c.stdp;

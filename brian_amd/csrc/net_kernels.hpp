@@ -1042,14 +1042,14 @@ __device__ inline int64_t stdp_hist_row(const StdpDev &sd, int64_t m, int64_t q,
   return r < 0 ? r + sd.HP : r;
 }
 
-// One weight update of ExponentialSTDP (stdp.py:507-523) with the other
-// side's trace a, then the clip; `other` is the sign-carrying amplitude that
-// selects the multiplicative form.
+// One weight update (stdp.py:507-523 for ExponentialSTDP, the affine
+// 'w+=(A-a)*s' lines of a generic STDP) with the other side's trace a, then
+// the clip to [wmin, wmax].
 template <typename T>
-__device__ inline T stdp_weight(const StdpDev &sd, T x, T a, double other) {
+__device__ inline T stdp_weight(const StdpDev &sd, T x, T a, int mode, double off, double scale) {
 #pragma clang fp contract(off)
-  if (sd.update == 0) x = x + a;
-  else if (other < 0) x = x * ((T)1 + a);
+  if (mode == 0) x = x + (a + (T)off) * (T)scale;
+  else if (mode == 1) x = x * ((T)1 + a);
   else x = x + ((T)sd.wmax - x) * a;
   return x < (T)sd.wmin ? (T)sd.wmin : (x > (T)sd.wmax ? (T)sd.wmax : x);
 }
@@ -1088,7 +1088,7 @@ __global__ __launch_bounds__(BLOCK) void k_stdp_pre(const StdpDev sd, const int6
       const int64_t lo = sd.row_ptr[i], hi = sd.row_ptr[i + 1];
       for (int64_t q = lo + lane; q < hi; q += 64) {
         const int64_t r = stdp_hist_row(sd, m, q, sd.kpre);
-        w[q] = stdp_weight<T>(sd, w[q], hpost[r * sd.tgt_n + sd.col[q]], sd.Am);
+        w[q] = stdp_weight<T>(sd, w[q], hpost[r * sd.tgt_n + sd.col[q]], sd.pre_mode, sd.pre_off, sd.pre_scale);
       }
     }
     return;
@@ -1116,12 +1116,7 @@ __global__ __launch_bounds__(BLOCK) void k_stdp_pre(const StdpDev sd, const int6
     for (int64_t q = lo + lane; q < hi; q += 64) {
       T x = w[q];
       if (sd.prop) unsafeAtomicAdd(ring + sd.tgt_off + sd.col[q], x);   // connection.py:240-241
-      const T ap = apost[sd.col[q]];
-      if (sd.update == 0) x = x + ap;
-      else if (sd.Am < 0) x = x * ((T)1 + ap);
-      else x = x + ((T)sd.wmax - x) * ap;
-      x = x < (T)sd.wmin ? (T)sd.wmin : (x > (T)sd.wmax ? (T)sd.wmax : x);
-      w[q] = x;
+      w[q] = stdp_weight<T>(sd, x, apost[sd.col[q]], sd.pre_mode, sd.pre_off, sd.pre_scale);
     }
   }
 }
@@ -1145,7 +1140,7 @@ __global__ __launch_bounds__(BLOCK) void k_stdp_post(const StdpDev sd, const int
     if (j < 0 || j >= sd.tgt_n) continue;
     if (lane == 0) {
       // 'nearest_pre' writes A_post=+Am, an assignment (stdp.py:500-502)
-      if (sd.interactions == 0) apost[j] = apost[j] + (T)sd.Am;
+      if (sd.interactions == 0 || sd.interactions == 4) apost[j] = apost[j] + (T)sd.Am;
       else apost[j] = (T)sd.Am;
     }
     const int64_t lo = sd.col_ptr[j], hi = sd.col_ptr[j + 1];
@@ -1155,19 +1150,14 @@ __global__ __launch_bounds__(BLOCK) void k_stdp_post(const StdpDev sd, const int
       for (int64_t q = lo + lane; q < hi; q += 64) {
         const int64_t pw = sd.col_perm[q];
         const int64_t r = stdp_hist_row(sd, n, pw, sd.kpost);
-        w[pw] = stdp_weight<T>(sd, w[pw], hpre[r * sd.src_n + sd.col_row[q]], sd.Ap);
+        w[pw] = stdp_weight<T>(sd, w[pw], hpre[r * sd.src_n + sd.col_row[q]], sd.post_mode, sd.post_off,
+                               sd.post_scale);
       }
       continue;
     }
     for (int64_t q = lo + lane; q < hi; q += 64) {
       const int64_t pw = sd.col_perm[q];
-      T x = w[pw];
-      const T a = apre[sd.col_row[q]];
-      if (sd.update == 0) x = x + a;
-      else if (sd.Ap < 0) x = x * ((T)1 + a);
-      else x = x + ((T)sd.wmax - x) * a;
-      x = x < (T)sd.wmin ? (T)sd.wmin : (x > (T)sd.wmax ? (T)sd.wmax : x);
-      w[pw] = x;
+      w[pw] = stdp_weight<T>(sd, w[pw], apre[sd.col_row[q]], sd.post_mode, sd.post_off, sd.post_scale);
     }
   }
 }

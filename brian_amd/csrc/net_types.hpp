@@ -215,6 +215,10 @@ struct StdpDev {
   // (now - 1 - kpost[q]).  h_pre / h_post hold the traces at the end of the
   // last HP steps ([HP][src_n], [HP][tgt_n]); k arrays are per synapse in
   // CSR order, u8 or u16 (kbytes).
+  // weight rule of each side (pre spike reads A_post, post spike A_pre):
+  // mode 0 w += (a + off) * scale, 1 w *= (1 + a), 2 w += (wmax - w) * a
+  int pre_mode, post_mode;
+  double pre_off, pre_scale, post_off, post_scale;
   int dmode, wdelay, HP, kbytes;
   void *h_pre, *h_post;
   const void *kpre, *kpost;

@@ -218,11 +218,21 @@ int bm_net_get_connection(bm_net *net, int cid, int64_t *row_ptr, int32_t *col, 
  * decay_pre/decay_post are the 1x1 propagators exp(-dt/tau) (FP64, host);
  * Ap, Am already scaled as in stdp.py:509-513; update 0 additive,
  * 1 multiplicative; interactions 0 all, 1 nearest, 2 nearest_pre (whose
- * post line is the reference's `A_post=+Am`, an assignment), 3 nearest_post.
+ * post line is the reference's `A_post=+Am`, an assignment), 3 nearest_post,
+ * 4 pre trace set and post trace added (a generic STDP's `A_pre=c`,
+ * `A_post+=c`).
  * On a DelayConnection (delay_kind 1) also call bm_net_set_stdp_delays. */
 int bm_net_add_stdp(bm_net *net, int cid, double decay_pre, double decay_post,
                     double Ap, double Am, double wmin, double wmax, int update,
                     int interactions, int *sid);
+
+/* Weight rules of an STDP (the w lines of a generic STDP's pre and post
+ * code, brian/stdp.py:181-424): on a pre spike with the post trace a (and
+ * on a post spike with the pre trace), mode 0: w = w + (a + off) * scale,
+ * 1: w = w * (1 + a), 2: w = w + (wmax - w) * a; then the clip to [wmin,
+ * wmax].  bm_net_add_stdp sets ExponentialSTDP's rules.  Before prepare. */
+int bm_net_set_stdp_rule(bm_net *net, int sid, int pre_mode, double pre_off, double pre_scale, int post_mode,
+                         double post_off, double post_scale);
 
 /* STDP on a DelayConnection (brian/stdp.py:296-368 with DelayedSTDPUpdater
  * :56-87 and RecentStateMonitor, monitor.py:992-1096): syn_delay_s[nnz] are

@@ -539,7 +539,11 @@ class STDP:
     """
 
     def __init__(self, net, conn, taup, taum, Ap, Am, wmin, wmax,
-                 interactions="all", update="additive", delays_s=None):
+                 interactions="all", update="additive", delays_s=None, rules=None):
+        """rules: ((mode, off, scale) on pre spikes, (mode, off, scale) on
+        post spikes) for a generic STDP whose w lines are w += (A + off) *
+        scale (mode 0), w *= (1 + A) (1) or w += (wmax - w) * A (2); the
+        amplitudes are then used as given (stdp.py:181-424)."""
         self.conn = conn
         dt = net.dt
         self.decay_pre, _ = linear_propagator(lambda x: -x / taup, 1, dt)
@@ -548,11 +552,21 @@ class STDP:
         self.decay_post = self.decay_post[0, 0]
         wmax = float(wmax)
         self.update = update
-        if update == "additive":
-            Ap = Ap * wmax
-            Am = Am * wmax
+        if rules is None:
+            if update == "additive":
+                Ap = Ap * wmax
+                Am = Am * wmax
+                rules = ((0, 0.0, 1.0), (0, 0.0, 1.0))
+            else:
+                rules = ((1 if Am < 0 else 2, 0.0, 1.0), (1 if Ap < 0 else 2, 0.0, 1.0))
+        self.pre_rule, self.post_rule = rules
         self.Ap, self.Am, self.wmin, self.wmax = Ap, Am, float(wmin), wmax
         self.interactions = interactions
+        # trace lines (stdp.py:495-506): '+=' adds, '=' sets; a generic STDP
+        # gives them as a pair
+        self.trace_ops = interactions if isinstance(interactions, tuple) else {
+            "all": ("+=", "+="), "nearest": ("=", "="), "nearest_pre": ("=", "="),
+            "nearest_post": ("+=", "=")}[interactions]
         self.A_pre = np.zeros(conn.src[2])
         self.A_post = np.zeros(conn.tgt[2])
         # column access: stable permutation into data (connectionmatrix.py:317-333)
@@ -583,13 +597,14 @@ class STDP:
             self.h_post[self.cur] = self.A_post
             self.cur = (self.cur + 1) % self.nd
 
-    def _apply(self, data, p, a, sign_other):
-        if self.update == "additive":
-            data[p] += a
-        elif sign_other < 0:
-            data[p] *= (1 + a)
+    def _apply(self, data, p, a, rule):
+        mode, off, scale = rule
+        if mode == 0:
+            data[p] = data[p] + (a + off) * scale
+        elif mode == 1:
+            data[p] = data[p] * (1 + a)
         else:
-            data[p] += (self.wmax - data[p]) * a
+            data[p] = data[p] + (self.wmax - data[p]) * a
         data[p] = np.clip(data[p], self.wmin, self.wmax)
 
     def _pre_delayed(self, net):
@@ -599,7 +614,7 @@ class STDP:
         for i in spikes:
             lo, hi = c.indptr[i], c.indptr[i + 1]
             rows = (self.cur - 1 - self.kpre[lo:hi]) % self.nd    # get_past_values_sequence
-            self._apply(c.data, np.arange(lo, hi), self.h_post[rows, c.indices[lo:hi]], self.Am)
+            self._apply(c.data, np.arange(lo, hi), self.h_post[rows, c.indices[lo:hi]], self.pre_rule)
 
     def decay(self):
         # LinearStateUpdater on 1x1: S = A*S; S += C with C = 0
@@ -611,22 +626,14 @@ class STDP:
         g, off, n = c.src
         spikes = _sub(net.groups[g].get_spikes(0 if self.delayed else c.delay), off, n)
         for i in spikes:
-            if self.interactions in ("all", "nearest_post"):
+            if self.trace_ops[0] == "+=":
                 self.A_pre[i] += self.Ap
             else:
                 self.A_pre[i] = self.Ap
             if self.delayed:
                 continue
             lo, hi = c.indptr[i], c.indptr[i + 1]
-            cols = c.indices[lo:hi]
-            if self.update == "additive":
-                c.data[lo:hi] += self.A_post[cols]
-            else:
-                if self.Am < 0:
-                    c.data[lo:hi] *= (1 + self.A_post[cols])
-                else:
-                    c.data[lo:hi] += (self.wmax - c.data[lo:hi]) * self.A_post[cols]
-            c.data[lo:hi] = np.clip(c.data[lo:hi], self.wmin, self.wmax)
+            self._apply(c.data, np.arange(lo, hi), self.A_post[c.indices[lo:hi]], self.pre_rule)
         if self.delayed:
             self._pre_delayed(net)
 
@@ -636,29 +643,20 @@ class STDP:
         spikes = _sub(net.groups[g].get_spikes(0), off, n)
         if self.delayed:
             for j in spikes:
-                self.A_post[j] = self.A_post[j] + self.Am if self.interactions == "all" else self.Am
+                self.A_post[j] = self.A_post[j] + self.Am if self.trace_ops[1] == "+=" else self.Am
             for j in spikes:
                 lo, hi = self.col_ptr[j], self.col_ptr[j + 1]
                 p = self.col_perm[lo:hi]
                 rows = (self.cur - 1 - self.kpost[p]) % self.nd
-                self._apply(c.data, p, self.h_pre[rows, self.col_rows[lo:hi]], self.Ap)
+                self._apply(c.data, p, self.h_pre[rows, self.col_rows[lo:hi]], self.post_rule)
             return
         for j in spikes:
-            if self.interactions == "all":
+            if self.trace_ops[1] == "+=":
                 self.A_post[j] += self.Am
             else:
                 self.A_post[j] = self.Am
             lo, hi = self.col_ptr[j], self.col_ptr[j + 1]
-            p = self.col_perm[lo:hi]
-            rows = self.col_rows[lo:hi]
-            if self.update == "additive":
-                c.data[p] += self.A_pre[rows]
-            else:
-                if self.Ap < 0:
-                    c.data[p] *= (1 + self.A_pre[rows])
-                else:
-                    c.data[p] += (self.wmax - c.data[p]) * self.A_pre[rows]
-            c.data[p] = np.clip(c.data[p], self.wmin, self.wmax)
+            self._apply(c.data, self.col_perm[lo:hi], self.A_pre[self.col_rows[lo:hi]], self.post_rule)
 
 
 class Net:

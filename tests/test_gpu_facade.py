@@ -569,3 +569,49 @@ def test_poisson_input_interface():
     net = Network(group, inp, m)
     net.run(500 * ms)
     assert m.nspikes >= 1
+
+
+def test_generic_stdp_forms():
+    """examples/plasticity/STDP1.py's generic STDP (pre='A_pre+=dA_pre;
+    w+=A_post', post='A_post+=dA_post;w+=A_pre') equals ExponentialSTDP on
+    the same network bit for bit; the Vogels et al. (2011) rule
+    pre='A_pre+=1.; w+=(A_post-alpha)*eta;' runs and depresses a synapse
+    whose post neuron never fires by alpha*eta per pre spike."""
+    from brian_amd import (NeuronGroup, SpikeGeneratorGroup, Connection, STDP, ExponentialSTDP, Network,
+                           ms, reinit_default_clock)
+    out = []
+    for generic in (True, False):
+        reinit_default_clock()
+        pre = SpikeGeneratorGroup(2, [(0, 1 * ms), (1, 3 * ms), (0, 9 * ms)])
+        teach = SpikeGeneratorGroup(1, [(0, 2 * ms), (0, 6 * ms)])
+        G = NeuronGroup(1, "v:1\nx:1", threshold=1, reset=0)
+        drive = Connection(teach, G, "v", weight=2)
+        con = Connection(pre, G, "x", weight=0.5)
+        taupre = taupost = 20 * ms
+        gmax = 1.0
+        dA_pre, dA_post = 0.01 * gmax, -0.0105 * gmax
+        eqs_stdp = """
+        dA_pre/dt  = -A_pre/taupre  : 1
+        dA_post/dt = -A_post/taupost : 1
+        """
+        if generic:
+            stdp = STDP(con, eqs=eqs_stdp, pre="A_pre+=dA_pre;w+=A_post", post="A_post+=dA_post;w+=A_pre",
+                        wmax=gmax)
+        else:
+            stdp = ExponentialSTDP(con, taupre, taupost, 0.01, -0.0105, wmax=gmax)
+        net = Network(pre, teach, G, drive, con, stdp)
+        net.run(12 * ms)
+        out.append(con.W.todense().copy())
+    np.testing.assert_array_equal(out[0], out[1])
+    assert abs(out[0][0, 0] - 0.5) > 1e-4
+    reinit_default_clock()
+    pre = SpikeGeneratorGroup(1, [(0, 1 * ms), (0, 5 * ms)])
+    G = NeuronGroup(1, "v:1", threshold=1, reset=0)
+    con = Connection(pre, G, "v", weight=0.5)
+    alpha, eta, tau_stdp = 0.12, 0.01, 20 * ms
+    eqs_inhib = "dA_pre/dt=-A_pre/tau_stdp : 1\ndA_post/dt=-A_post/tau_stdp : 1"
+    stdp = STDP(con, eqs=eqs_inhib, pre="A_pre+=1.; w+=(A_post-alpha)*eta;", post="A_post+=1.; w+=A_pre*eta;",
+                wmax=10.0)
+    net = Network(pre, G, con, stdp)
+    net.run(10 * ms)
+    assert abs(con.W[0, 0] - (0.5 - 2 * alpha * eta)) < 1e-15

@@ -13,7 +13,10 @@ from tests.netbuild import cuba_propagator
 
 pytestmark = pytest.mark.gpu
 DT = 1e-4
-INTER = {"all": 0, "nearest": 1, "nearest_pre": 2, "nearest_post": 3}
+INTER = {"all": 0, "nearest": 1, "nearest_pre": 2, "nearest_post": 3, "pre_set_post_add": 4}
+
+
+RULES = {"vogels": ((0, -0.12, 0.01), (0, 0.0, 0.01)), "mixed": ((1, 0.0, 1.0), (0, 0.0, 1.0))}
 
 
 @pytest.mark.parametrize("kind,inter,update", [
@@ -23,6 +26,9 @@ INTER = {"all": 0, "nearest": 1, "nearest_pre": 2, "nearest_post": 3}
     ("delay", "all", "additive"),
     ("delay", "nearest", "additive"),
     ("delay", "nearest_pre", "multiplicative"),
+    ("plain0", "pre_set_post_add", "vogels"),
+    ("delay", "all", "vogels"),
+    ("plain3", "all", "mixed"),
 ])
 def test_stdp_network_fp64_matches_oracle(kind, inter, update):
     d = W.cuba(N=4000, seed=31)
@@ -56,11 +62,20 @@ def test_stdp_network_fp64_matches_oracle(kind, inter, update):
     ipi, ixi, wi = d["inh"]
     h.add_connection(g, Ne, Ni, g, 0, N, 2, ipi, ixi, wi, delay_steps=0 if kind == "delay" else hd)
     net.add_connection((gid, Ne, Ni), (gid, 0, N), 2, ipi, ixi, wi, delay_steps=0 if kind == "delay" else hd)
-    st = net.add_stdp(oc, taup=taup, taum=taum, Ap=Ap, Am=Am, wmin=0.0, wmax=wmax, interactions=inter,
-                      update=update, delays_s=dl_s)
-    scale = wmax if update == "additive" else 1.0
-    sid = h.add_stdp(ce, st.decay_pre, st.decay_post, Ap * scale, Am * scale, 0.0, wmax,
-                     0 if update == "additive" else 1, INTER[inter], syn_delay_s=dl_s)
+    if update in RULES:
+        # generic rules: raw amplitudes (trace increments of the pre/post code)
+        Ap, Am = (1.0, 1.0) if update == "vogels" else (0.01 * wmax, -0.0105)
+        ops = ("=", "+=") if inter == "pre_set_post_add" else inter
+        st = net.add_stdp(oc, taup=taup, taum=taum, Ap=Ap, Am=Am, wmin=0.0, wmax=wmax, interactions=ops,
+                          delays_s=dl_s, rules=RULES[update])
+        sid = h.add_stdp(ce, st.decay_pre, st.decay_post, Ap, Am, 0.0, wmax, 0, INTER[inter], syn_delay_s=dl_s)
+        h.set_stdp_rule(sid, *RULES[update])
+    else:
+        st = net.add_stdp(oc, taup=taup, taum=taum, Ap=Ap, Am=Am, wmin=0.0, wmax=wmax, interactions=inter,
+                          update=update, delays_s=dl_s)
+        scale = wmax if update == "additive" else 1.0
+        sid = h.add_stdp(ce, st.decay_pre, st.decay_post, Ap * scale, Am * scale, 0.0, wmax,
+                         0 if update == "additive" else 1, INTER[inter], syn_delay_s=dl_s)
     h.record(g, 1 << 20)
     h.prepare()
     print(f"case {kind} {inter} {update}", flush=True)

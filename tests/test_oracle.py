@@ -561,3 +561,39 @@ def test_stdp_delayconnection_closed_forms():
     for q in (42, 45, 48):
         assert abs(_stdp_trains([n], [n + q], q=q) - Am) < 1e-15
     assert _stdp_trains([n], [n + 44], q=44) == 0.0
+
+
+def test_generic_stdp_rule_closed_forms():
+    """A generic STDP (stdp.py:181-424) with the Vogels et al. rule
+    pre='A_pre+=1; w+=(A_post-alpha)*eta', post='A_post+=1; w+=A_pre*eta':
+    a lone pre spike moves w by -alpha*eta; post k steps after pre by
+    eta*exp(-k dt/tau) (plus the pre spike's -alpha*eta); pre k steps after
+    post by eta*(exp(-k dt/tau) - alpha).  Setting instead of adding the
+    pre trace (interactions ('=', '+=')) keeps only the last pre spike."""
+    alpha, eta, tau = 0.12, 0.01, 20e-3
+    rules = ((0, -alpha, eta), (0, 0.0, eta))
+
+    def run(pre_steps, post_steps, ops=("+=", "+=")):
+        nsteps = max(pre_steps + post_steps) + 5
+        net = O.Net(DT)
+        src = O.SpikeSource(1, step_lists=[np.array([0]) if k in pre_steps else np.zeros(0, np.int64)
+                                           for k in range(nsteps)])
+        sid = net.add_group(src, record=False)
+        drive = np.zeros((nsteps, 1))
+        drive[post_steps] = 1.0
+        grp = O.Group(1, np.zeros((1, 1)), ("linear", np.eye(1), np.zeros(1)), ("const", 0, 0.5),
+                      [(0, "=", 0.0, False)], dt=DT, inputs=[(0, drive)])
+        gid = net.add_group(grp)
+        c = net.add_connection((sid, 0, 1), (gid, 0, 1), 0, np.array([0, 1]), np.array([0]), np.array([0.5]))
+        net.add_stdp(c, taup=tau, taum=tau, Ap=1.0, Am=1.0, wmin=0.0, wmax=10.0, interactions=ops, rules=rules)
+        net.run(nsteps)
+        return float(c.data[0]) - 0.5
+    e = lambda k: np.exp(-k * DT / tau)
+    assert abs(run([10], []) - (-alpha * eta)) < 1e-15
+    for k in (1, 30, 200):
+        assert abs(run([10], [10 + k]) - (-alpha * eta + eta * e(k))) < 1e-14
+        assert abs(run([10 + k], [10]) - eta * (e(k) - alpha)) < 1e-14
+    both = run([10, 20], [40])
+    assert abs(both - (-2 * alpha * eta + eta * (e(30) + e(20)))) < 1e-14
+    last = run([10, 20], [40], ops=("=", "+="))
+    assert abs(last - (-2 * alpha * eta + eta * e(20))) < 1e-14
