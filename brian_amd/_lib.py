@@ -56,6 +56,7 @@ _SIGS = {
                                  C.POINTER(C.c_int)],
     "bm_net_add_coincidence_counter": [P, C.c_int, i64, i64, i64, Pd, Pi32, Pd, dbl, dbl, C.c_int],
     "bm_net_read_coincidence_counter": [P, C.c_int, Pi32, Pi32, Pi32],
+    "bm_net_set_poisson_rates": [P, C.c_int, i64, i64, C.c_int, Pd],
     "bm_net_add_poisson": [P, i64, Pd, u64, C.POINTER(C.c_int)],
     "bm_net_add_connection": [P, C.c_int, i64, i64, C.c_int, i64, i64, C.c_int, i64, Pi64, Pi32,
                               Pd, C.c_int, C.c_int, Pi32, C.c_int, C.POINTER(C.c_int)],
@@ -292,6 +293,11 @@ class NetHandle:
         out = [np.zeros(n, np.int32) for _ in range(3)]
         call("bm_net_read_coincidence_counter", self.h, gid, *[i32ptr(o) for o in out])
         return out
+
+    def set_poisson_rates(self, gid, first_step, rates, per_neuron):
+        r = f64(rates)
+        n = r.shape[0] if r.ndim else 0
+        call("bm_net_set_poisson_rates", self.h, gid, int(first_step), n, 1 if per_neuron else 0, dptr(r))
 
     def add_poisson(self, N, rates, seed):
         r = f64(np.broadcast_to(np.asarray(rates, dtype=np.float64), (N,)))

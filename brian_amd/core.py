@@ -607,7 +607,9 @@ class PulsePacket(SpikeGeneratorGroup):
 class PoissonGroup(_GroupBase):
     """Poisson sources (brian/directcontrol.py:406-447) drawn with the
     counter-based RNG of the backend: neuron i spikes at step n iff
-    u32(seed, i, n) < floor(rate_i*dt*2^32)."""
+    u32(seed, i, n) < floor(rate_i*dt*2^32).  ``rates`` may be a function
+    of t returning a rate or an array of N rates: it is evaluated at every
+    step of each run (t = n dt) and the table goes to the device."""
 
     def __init__(self, N, rates=0, clock=None, seed=None, level=0):
         self._track(level + 1)
@@ -616,6 +618,9 @@ class PoissonGroup(_GroupBase):
         self._origin = 0
         self._next_sub = 0
         self.clock = clock or defaultclock
+        self._rate_fn = rates if callable(rates) else None
+        if self._rate_fn is not None:
+            rates = 0.0
         self.rates = np.broadcast_to(np.asarray(rates, dtype=np.float64), (self.N,)).copy()
         self.seed = int(np.random.randint(0, 2 ** 31)) if seed is None else int(seed)
         self._net = None
@@ -2013,7 +2018,14 @@ class Network:
         self._capacity = {}
         for g in self.groups:
             if id(g) in recorded:
-                cap = max(1 << 20, int(g.N * self._planned_steps * dt * _PREFS["max_rate_hz"]))
+                rmax = _PREFS["max_rate_hz"]
+                if isinstance(g, PoissonGroup):     # sources fire at their own rates
+                    if g._rate_fn is not None:
+                        rmax = max([rmax] + [float(np.max(g._rate_fn(n * dt + 0.0)))
+                                             for n in range(self.clock._n, self.clock._n + self._planned_steps)])
+                    else:
+                        rmax = max(rmax, float(g.rates.max(initial=0.0)))
+                cap = max(1 << 20, int(g.N * self._planned_steps * dt * rmax))
                 self._capacity[id(g)] = cap
                 h.record(self.gid[id(g)], cap)
         # a StateMonitor added after an operation of the 'end' slots samples
@@ -2085,6 +2097,7 @@ class Network:
             self._compile()
         else:
             self._extend_generators(clk._n + nsteps)
+        self._poisson_tables(clk, nsteps)
         state_mons = [m for m in self.monitors
                       if isinstance(m, StateMonitor) and not m._record_false and not m._host]
         _StopFlag.stop = False
@@ -2162,6 +2175,18 @@ class Network:
             bounds = np.searchsorted(steps, np.arange(s0, s1 + 1))
             for k in range(s1 - s0):
                 m.function(idx[bounds[k]:bounds[k + 1]])
+
+    def _poisson_tables(self, clk, nsteps):
+        """Rates of time-varying Poisson groups over this run's steps."""
+        for g in self.groups:
+            if isinstance(g, PoissonGroup) and g._rate_fn is not None:
+                dt = clk.dt
+                vals = [np.asarray(g._rate_fn(n * dt + 0.0), dtype=np.float64)
+                        for n in range(clk._n, clk._n + nsteps)]
+                per_neuron = any(v.ndim > 0 and v.size > 1 for v in vals)
+                tab = (np.array([np.broadcast_to(v, (g.N,)) for v in vals]) if per_neuron
+                       else np.array([float(v) for v in vals]))
+                self.handle.set_poisson_rates(self.gid[id(g)], clk._n - self._n0, tab, per_neuron)
 
     def _extend_generators(self, n1):
         """Spike generators whose spikes continue (iterators, periods) get

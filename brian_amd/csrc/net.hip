@@ -71,6 +71,9 @@ struct HostGroup {
   };
   std::unique_ptr<Coin> coin;
   bool defer = false;        // deferred propagation (specialised lean kernel)
+  DevBuf ptab_d;             // time-varying Poisson thresholds
+  int64_t ptab_n0 = 0, ptab_len = 0;
+  int ptab_per_neuron = 0;
   DevBuf dlist_d, dcount_d, dbase_d, dsel_d;
   int H = 0;
   // device
@@ -840,9 +843,22 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     }
     d.nout = nout;
     if (g.raster_cap > 0) {
-      // regions of 1.5x the even share, so an uneven spread over regions fits
-      g.raster_region = (g.raster_cap + RSHARD - 1) / RSHARD * 3 / 2 + 1024;
-      g.raster_d.alloc((size_t)g.raster_region * RSHARD * 8);
+      // A workgroup appends to region blockIdx % RSHARD: a group swept by B
+      // workgroups fills min(B, RSHARD) regions, the busiest taking the
+      // spikes of ceil(B / RSHARD) workgroups (the generator kernel's grid
+      // is 64 workgroups).  Neurons per thread may be retuned after prepare,
+      // so the regions are sized for the busiest share over K = 1, 2, 4 and
+      // counted for K = 1 (the most workgroups); each holds 1.5x that share,
+      // so an uneven spread over workgroups fits.
+      int64_t R = RSHARD, region = 0;
+      for (int K = 1; K <= 4; K *= 2) {
+        const int64_t B = g.kind == GK_GENERATOR ? 64 : (g.N + (int64_t)BLOCK * K - 1) / ((int64_t)BLOCK * K);
+        const int64_t per = (B + RSHARD - 1) / RSHARD;
+        if (K == 1) R = std::min<int64_t>(B, RSHARD);
+        region = std::max(region, (g.raster_cap * per + B - 1) / B * 3 / 2 + 1024);
+      }
+      g.raster_region = region;
+      g.raster_d.alloc((size_t)(g.raster_region * R) * 8);
       g.raster_count_d.alloc((size_t)RSHARD * RSTRIDE * 8);
       BM_HIP(hipMemsetAsync(g.raster_count_d.p, 0, (size_t)RSHARD * RSTRIDE * 8, net->stream));
       d.raster = g.raster_d.as<unsigned long long>();
@@ -869,6 +885,10 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       upload_raw(g.pthr_d, thr.data(), thr.size(), net->stream);
       d.pthr = g.pthr_d.as<uint64_t>();
       d.seed = g.seed;
+      d.ptab = g.ptab_d.as<uint64_t>();
+      d.ptab_n0 = g.ptab_n0;
+      d.ptab_len = g.ptab_len;
+      d.ptab_per_neuron = g.ptab_per_neuron;
     }
     if (g.ti_nt > 0) {
       upload_T(net, g.ti_d, g.ti_vals.data(), g.ti_vals.size());
@@ -1331,6 +1351,36 @@ int bm_net_read_coincidence_counter(bm_net *net, int gid, int32_t *coincidences,
     BM_HIP(hipMemcpy(coincidences, g.coin->coinc_d.p, n * 4, hipMemcpyDeviceToHost));
     BM_HIP(hipMemcpy(model_length, g.coin->model_d.p, n * 4, hipMemcpyDeviceToHost));
     BM_HIP(hipMemcpy(target_length, g.coin->target_d.p, n * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int bm_net_set_poisson_rates(bm_net *net, int gid, int64_t first_step, int64_t nsteps, int per_neuron,
+                             const double *rates) {
+  return guard([&] {
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.kind == GK_POISSON, "not a Poisson group");
+    BM_REQUIRE(first_step >= 0 && nsteps >= 0 && (nsteps == 0 || rates), "bad rate table");
+    const int64_t width = per_neuron ? g.N : 1;
+    BM_REQUIRE(nsteps * width <= (int64_t(1) << 28), "rate table too large");
+    std::vector<uint64_t> thr((size_t)(nsteps * width));
+    for (size_t q = 0; q < thr.size(); ++q) {     // the threshold of poisson_threshold (rate * dt * 2^32)
+      double t = std::floor(rates[q] * net->dt * 4294967296.0);
+      thr[q] = (uint64_t)std::min(std::max(t, 0.0), 4294967295.0);
+    }
+    if (net->prepared) BM_HIP(hipStreamSynchronize(net->stream));   // launches in flight read the old table
+    upload_raw(g.ptab_d, thr.data(), thr.size(), net->stream);
+    g.ptab_n0 = first_step;
+    g.ptab_len = nsteps;
+    g.ptab_per_neuron = per_neuron ? 1 : 0;
+    g.dev.ptab = g.ptab_d.as<uint64_t>();
+    g.dev.ptab_n0 = first_step;
+    g.dev.ptab_len = nsteps;
+    g.dev.ptab_per_neuron = g.ptab_per_neuron;
+    if (net->graph_exec) {
+      hipGraphExecDestroy(net->graph_exec);
+      net->graph_exec = nullptr;
+      net->graph_windows = 0;
+    }
   });
 }
 

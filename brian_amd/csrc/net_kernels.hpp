@@ -815,7 +815,10 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
         if (!valid[k]) continue;
         const int i = blk0 + k * BLOCK + tid;
         if (GEN && g.kind == GK_POISSON) {
-          spk[k] = (uint64_t)counter_u32(g.seed, (uint64_t)i, (uint64_t)n) < g.pthr[i];
+          uint64_t thr = g.pthr[i];
+          const int64_t r = n - g.ptab_n0;
+          if (g.ptab && r >= 0 && r < g.ptab_len) thr = g.ptab[g.ptab_per_neuron ? r * N + i : r];
+          spk[k] = (uint64_t)counter_u32(g.seed, (uint64_t)i, (uint64_t)n) < thr;
         } else if (g.th_kind != TH_NONE) {
 #ifdef BM_JIT
           const T v = s[J_TH_VAR][k];

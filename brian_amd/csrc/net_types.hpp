@@ -185,6 +185,13 @@ struct GroupDev {
   int64_t *dbase;
   int32_t *dsel;
   int64_t dcap;
+  // Poisson group with rates varying in time (PoissonGroup(N, rates(t)),
+  // brian/directcontrol.py:406-447): thresholds of steps [ptab_n0,
+  // ptab_n0 + ptab_len), one per step or one per step and neuron; steps
+  // outside the table use pthr
+  const uint64_t *ptab;
+  int64_t ptab_n0, ptab_len;
+  int ptab_per_neuron, pad4;
 };
 // passed by value to every sweep launch; kernel arguments are limited to 4 KiB
 static_assert(sizeof(GroupDev) <= 3584, "GroupDev must leave room in the 4 KiB kernel arguments");

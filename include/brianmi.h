@@ -119,6 +119,14 @@ int bm_net_set_generator_spikes(bm_net *net, int gid, int64_t nspikes, const int
  * u32(seed, i, n) < floor(rate_i*dt*2^32) (splitmix64 construction). */
 int bm_net_add_poisson(bm_net *net, int64_t N, const double *rates, uint64_t seed, int *gid);
 
+/* Rates of a Poisson group for steps [first_step, first_step + nsteps)
+ * (counted from the network's first step), one per step (per_neuron 0) or
+ * [nsteps][N]: the role of PoissonGroup(N, rates=f(t)) evaluated each step
+ * (brian/directcontrol.py:406-447).  Steps outside the table use the rates
+ * of bm_net_add_poisson.  Before or after prepare (replaces the table). */
+int bm_net_set_poisson_rates(bm_net *net, int gid, int64_t first_step, int64_t nsteps, int per_neuron,
+                             const double *rates);
+
 /* CoincidenceCounter (brian/monitor.py:1230-1376) on neurons [off, off+n)
  * of group gid, counted online at each spike on the device: data[ndata]
  * are the concatenated target trains in seconds, each padded with a spike

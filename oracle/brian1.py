@@ -371,7 +371,7 @@ class SpikeSource:
     def __init__(self, N, step_lists=None, poisson=None):
         self.N = int(N)
         self.step_lists = step_lists
-        self.poisson = poisson          # (rates[N], seed, dt[, "weyl"])
+        self.poisson = poisson          # (rates[N] or rates(t), seed, dt[, "weyl"])
         self.history = []
         self.S = np.zeros((0, self.N))
         self.m = 0
@@ -392,6 +392,8 @@ class SpikeSource:
             rates, seed, dt = self.poisson[:3]
             draw = weyl_uniform_u32 if self.poisson[3:] == ("weyl",) else counter_uniform_u32
             u = draw(seed, np.arange(self.N, dtype=np.uint64), np.uint64(n))
+            if callable(rates):             # rates(t) of a PoissonGroup (directcontrol.py:436-445)
+                rates = np.broadcast_to(np.asarray(rates(n * dt + 0.0), dtype=np.float64), (self.N,))
             thr = poisson_threshold_u32(rates, dt)
             spikes = (u.astype(np.uint64) < thr).nonzero()[0]
         elif self.step_lists is not None and n < len(self.step_lists):

@@ -615,3 +615,37 @@ def test_generic_stdp_forms():
     net = Network(pre, G, con, stdp)
     net.run(10 * ms)
     assert abs(con.W[0, 0] - (0.5 - 2 * alpha * eta)) < 1e-15
+
+
+def test_poisson_group_rates_in_time():
+    """PoissonGroup(N, rates=f) with f a function of t (directcontrol.py:
+    436-445): the rate schedule of examples' ``lambda t: (t < 200*ms and
+    2000*Hz) or 0*Hz`` over two runs, and a per-neuron rate vector that
+    varies with t, give the oracle's counter-RNG raster exactly."""
+    from brian_amd import PoissonGroup, SpikeMonitor, Network, Hz, ms, reinit_default_clock, defaultclock
+    from oracle import brian1 as O
+    dt = float(defaultclock.dt)
+    cases = (
+        (200, lambda t: (t < 200 * ms and 2000 * Hz) or 0 * Hz),
+        (300, lambda t: np.linspace(0, 500, 300) * (1 + np.sin(2 * np.pi * 8 * t))),
+    )
+    for N, fn in cases:
+        reinit_default_clock()
+        P = PoissonGroup(N, rates=fn, seed=1234)
+        M = SpikeMonitor(P)
+        net = Network(P, M)
+        net.run(150 * ms)
+        net.run(150 * ms)
+        src = O.SpikeSource(N, poisson=(fn, 1234, dt))
+        steps, idx = [], []
+        for n in range(3000):
+            sp = src.update(n * dt, n)
+            steps += [n] * len(sp)
+            idx += list(sp)
+        got = np.array([(round(t / dt), i) for i, t in M.spikes], dtype=np.int64).reshape(-1, 2)
+        want = np.array(list(zip(steps, idx)), dtype=np.int64).reshape(-1, 2)
+        assert len(want) > 1000
+        order = np.lexsort((got[:, 1], got[:, 0]))
+        np.testing.assert_array_equal(got[order], want)
+        if N == 200:
+            assert want[:, 0].max() < 2000 and want[:, 0].max() > 1900

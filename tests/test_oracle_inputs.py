@@ -113,3 +113,21 @@ def test_coincidence_counter_restatements_agree():
     assert c[0] == 1                                # one target spike counts once
     c, m, _ = F.coincidence_counter([np.array([100, 1000])], data, [0], 40, 1e-4, onset=0.05)
     assert c[0] == 1 and m[0] == 1
+
+
+def test_poisson_rate_function():
+    """A rate function of t draws the same spikes as the constant rate it
+    returns, and none while it returns 0 (directcontrol.py:436-445)."""
+    from oracle import brian1 as O
+    dt, N = 1e-4, 50
+    a = O.SpikeSource(N, poisson=(np.full(N, 300.0), 9, dt))
+    b = O.SpikeSource(N, poisson=(lambda t: 300.0, 9, dt))
+    c = O.SpikeSource(N, poisson=(lambda t: 300.0 if t >= 0.01 else 0.0, 9, dt))
+    for n in range(300):
+        sa, sb, sc = a.update(n * dt, n), b.update(n * dt, n), c.update(n * dt, n)
+        np.testing.assert_array_equal(sa, sb)
+        if n < 100:
+            assert len(sc) == 0
+        else:
+            np.testing.assert_array_equal(sa, sc)
+    assert sum(len(x) for x in a.history) > 200
