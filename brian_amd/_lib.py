@@ -83,6 +83,8 @@ _SIGS = {
     "bm_net_set_state": [P, C.c_int, C.c_int, Pd],
     "bm_net_spike_count": [P, C.c_int, Pi64],
     "bm_net_get_spikes": [P, C.c_int, Pi32, Pi32, i64, Pi64],
+    "bm_net_record_spike_states": [P, C.c_int, C.c_int, Pi32],
+    "bm_net_get_spike_states": [P, C.c_int, Pi32, Pi32, Pd, i64, Pi64],
     "bm_net_clear_spikes": [P, C.c_int],
     "bm_net_get_weights": [P, C.c_int, Pd],
     "bm_net_get_stdp_traces": [P, C.c_int, Pd, Pd],
@@ -447,6 +449,25 @@ class NetHandle:
         got = C.c_int64()
         call("bm_net_get_spikes", self.h, gid, i32ptr(steps), i32ptr(idx), n, C.byref(got))
         return steps[:got.value].astype(np.int64), idx[:got.value].astype(np.int64)
+
+    def record_spike_states(self, gid, vars):
+        v = as_i32(vars)
+        call("bm_net_record_spike_states", self.h, gid, len(v), i32ptr(v))
+        self._ssm = getattr(self, "_ssm", {})
+        self._ssm[gid] = len(v)
+
+    def get_spike_states(self, gid):
+        """Raster sorted by (step, index) and the recorded values, one row
+        per entry."""
+        n = self.spike_count(gid)
+        nv = self._ssm[gid]
+        steps = np.empty(n, dtype=np.int32)
+        idx = np.empty(n, dtype=np.int32)
+        vals = np.empty((n, nv), dtype=np.float64)
+        got = C.c_int64()
+        call("bm_net_get_spike_states", self.h, gid, i32ptr(steps), i32ptr(idx), dptr(vals), n, C.byref(got))
+        k = got.value
+        return steps[:k].astype(np.int64), idx[:k].astype(np.int64), vals[:k]
 
     def clear_spikes(self, gid):
         call("bm_net_clear_spikes", self.h, gid)

@@ -31,7 +31,7 @@ __all__ = [
     "Connection", "DelayConnection", "STDP", "ExponentialSTDP", "STP", "ConnectionMatrix",
     "SparseConnectionVector", "random_matrix", "random_matrix_fixed_column",
     "SpikeMonitor", "PopulationRateMonitor", "PopulationSpikeCounter", "SpikeCounter", "CoincidenceCounter",
-    "FileSpikeMonitor", "AERSpikeMonitor", "ISIHistogramMonitor", "MultiStateMonitor", "RecentStateMonitor",
+    "FileSpikeMonitor", "AERSpikeMonitor", "ISIHistogramMonitor", "StateSpikeMonitor", "MultiStateMonitor", "RecentStateMonitor",
     "StateMonitor", "Network", "MagicNetwork", "run", "reinit", "stop", "network_operation",
     "NetworkOperation", "set_global_preferences", "get_global_preference", "forget", "recall",
 ]
@@ -1538,6 +1538,50 @@ class PopulationRateMonitor(SpikeMonitor):
         return np.convolve(rate, window * 1. / np.sum(window), mode="same")
 
 
+class StateSpikeMonitor(SpikeMonitor):
+    """brian/monitor.py:337-397: spikes with the values of state variables
+    at the spike, read after the threshold and before the reset.  ``spikes``
+    holds tuples (i, t, v0, v1, ...); ``times(i)`` and ``values(var, i)``.
+    Recorded on the device beside the raster (at most 4 variables per
+    group)."""
+
+    def __init__(self, source, var):
+        SpikeMonitor.__init__(self, source)
+        if isinstance(var, (str, int)):
+            var = (var,)
+        self._varnames = tuple(var)
+        owner = source._owner
+        self._var_ids = [v if isinstance(v, int) else owner.var_index[v] for v in self._varnames]
+        self._varindex = dict((v, k + 2) for k, v in enumerate(self._varnames))
+
+    def _vals(self):
+        net = self._net
+        if net is None or net.handle is None:
+            return np.zeros((0, len(self._var_ids)))
+        owner = self.source._owner
+        steps, idx = net.raster(owner)
+        o, n = self.source._origin, self.source.N
+        m = (idx >= o) & (idx < o + n)
+        return np.stack([net.raster_values(owner, v)[m] for v in self._var_ids], axis=1)
+
+    @property
+    def spikes(self):
+        s, i = self._raster()
+        dt = self.source.clock.dt
+        vals = self._vals()
+        return [(a, b) + tuple(r) for a, b, r in zip(i.tolist(), (s * dt + 0.0).tolist(), vals.tolist())]
+
+    def times(self, i=None):
+        s, idx = self._raster()
+        t = s * self.source.clock.dt + 0.0
+        return t if i is None else t[idx == i]
+
+    def values(self, var, i=None):
+        s, idx = self._raster()
+        v = self._vals()[:, self._varindex[var] - 2]
+        return v if i is None else v[idx == i]
+
+
 class ISIHistogramMonitor(SpikeMonitor):
     """monitor.py:428-480: histogram of inter-spike intervals over ``bins``
     (lower bounds; the last bin is open).  As in the reference, a neuron's
@@ -1861,6 +1905,7 @@ class Network:
         self.gid = {}
         self.steps_run = 0
         self._rasters = {}
+        self._raster_vals = {}
         self.precision = kw.pop("precision", None)
         for a in args:
             self.add(a)
@@ -2016,6 +2061,7 @@ class Network:
                 h.set_stdp_rule(s.sid, *s.rules)
             s._net = self
         self._capacity = {}
+        self._svars = {}
         for g in self.groups:
             if id(g) in recorded:
                 rmax = _PREFS["max_rate_hz"]
@@ -2028,6 +2074,15 @@ class Network:
                 cap = max(1 << 20, int(g.N * self._planned_steps * dt * rmax))
                 self._capacity[id(g)] = cap
                 h.record(self.gid[id(g)], cap)
+                svars = []
+                for m in self.monitors:
+                    if isinstance(m, StateSpikeMonitor) and m.source._owner is g:
+                        svars += [v for v in m._var_ids if v not in svars]
+                if svars:
+                    if len(svars) > 4:
+                        raise NotImplementedError("at most 4 variables per group are recorded at spike times")
+                    h.record_spike_states(self.gid[id(g)], svars)
+                    self._svars[id(g)] = svars
         # a StateMonitor added after an operation of the 'end' slots samples
         # on the host after that operation, in the order they were added
         # (both run at 'end', network.py:365-380)
@@ -2201,8 +2256,18 @@ class Network:
     def raster(self, owner):
         key = id(owner)
         if key not in self._rasters:
-            self._rasters[key] = self.handle.get_spikes(self.gid[key])
+            if key in getattr(self, "_svars", {}):
+                s, i, v = self.handle.get_spike_states(self.gid[key])
+                self._rasters[key] = (s, i)
+                self._raster_vals[key] = v
+            else:
+                self._rasters[key] = self.handle.get_spikes(self.gid[key])
         return self._rasters[key]
+
+    def raster_values(self, owner, var):
+        """Values of variable number ``var`` at the raster's entries."""
+        self.raster(owner)
+        return self._raster_vals[id(owner)][:, self._svars[id(owner)].index(var)]
 
     def clear_raster(self, owner):
         self.handle.clear_spikes(self.gid[id(owner)])

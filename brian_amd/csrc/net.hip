@@ -74,6 +74,8 @@ struct HostGroup {
   DevBuf ptab_d;             // time-varying Poisson thresholds
   int64_t ptab_n0 = 0, ptab_len = 0;
   int ptab_per_neuron = 0;
+  std::vector<int> ssm_vars;         // variables recorded at spike times
+  DevBuf ssm_d;
   DevBuf dlist_d, dcount_d, dbase_d, dsel_d;
   int H = 0;
   // device
@@ -250,6 +252,7 @@ void require_unprepared(bm_net *net) {
 bool fast_eligible(const HostGroup &g) {
   if (g.kind != GK_NEURON || g.upd != UPD_LINEAR || g.M < 1 || g.M > 4) return false;
   if (!g.mons.empty() || g.mod_var >= 0 || g.stp_conn >= 0 || !g.pins.empty() || g.coin) return false;
+  if (!g.ssm_vars.empty()) return false;
   if (g.refr != REFR_NONE || g.th_kind == TH_EMPIRICAL || g.ti_nt > 0) return false;
   if (g.ring_desc.size() > 1) return false;
   if (g.ring_desc.size() == 1 && g.ring_desc[0].kind != RING_COUNT) return false;
@@ -859,6 +862,13 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       }
       g.raster_region = region;
       g.raster_d.alloc((size_t)(g.raster_region * R) * 8);
+      d.ssm_n = (int)g.ssm_vars.size();
+      d.ssm_vals = nullptr;
+      if (d.ssm_n > 0) {
+        g.ssm_d.alloc((size_t)(g.raster_region * R) * d.ssm_n * 8);
+        d.ssm_vals = g.ssm_d.as<double>();
+        for (int j = 0; j < 4; ++j) d.ssm_var[j] = (int8_t)(j < d.ssm_n ? g.ssm_vars[j] : -1);
+      }
       g.raster_count_d.alloc((size_t)RSHARD * RSTRIDE * 8);
       BM_HIP(hipMemsetAsync(g.raster_count_d.p, 0, (size_t)RSHARD * RSTRIDE * 8, net->stream));
       d.raster = g.raster_d.as<unsigned long long>();
@@ -1989,35 +1999,87 @@ int bm_net_spike_count(bm_net *net, int gid, int64_t *count) {
   });
 }
 
+namespace {
+// The raster's regions gathered on the host (entries step << 32 | index),
+// with the values recorded at spike times when vals is given.
+std::vector<uint64_t> fetch_raster(bm_net *net, HostGroup &g, int64_t cap, std::vector<double> *vals) {
+  const std::vector<unsigned long long> cs = raster_counts(net, g);
+  int64_t c = 0;
+  for (int r = 0; r < RSHARD; ++r) {
+    if ((int64_t)cs[r] > g.raster_region)
+      throw Error(BM_EOVERFLOW, "spike raster overflow: region " + std::to_string(r) + " received " +
+                                    std::to_string(cs[r]) + " spikes > " + std::to_string(g.raster_region) +
+                                    " (record with a larger capacity)");
+    c += (int64_t)cs[r];
+  }
+  BM_REQUIRE(c <= cap, "output buffer too small");
+  std::vector<uint64_t> tmp(c);
+  const int64_t nv = vals ? (int64_t)g.ssm_vars.size() : 0;
+  if (vals) vals->resize((size_t)(c * nv));
+  int64_t at = 0;
+  for (int r = 0; r < RSHARD; ++r) {
+    if (cs[r]) {
+      BM_HIP(hipMemcpyAsync(tmp.data() + at, g.raster_d.as<unsigned long long>() + (int64_t)r * g.raster_region,
+                            cs[r] * 8, hipMemcpyDeviceToHost, net->stream));
+      if (nv)
+        BM_HIP(hipMemcpyAsync(vals->data() + at * nv, g.ssm_d.as<double>() + (int64_t)r * g.raster_region * nv,
+                              cs[r] * nv * 8, hipMemcpyDeviceToHost, net->stream));
+    }
+    at += (int64_t)cs[r];
+  }
+  BM_HIP(hipStreamSynchronize(net->stream));
+  return tmp;
+}
+}  // namespace
+
 int bm_net_get_spikes(bm_net *net, int gid, int32_t *steps, int32_t *idx, int64_t cap, int64_t *n_out) {
   return guard([&] {
     HostGroup &g = grp(net, gid);
     BM_REQUIRE(g.raster_cap > 0 && net->prepared, "group is not recorded or network not prepared");
-    const std::vector<unsigned long long> cs = raster_counts(net, g);
-    int64_t c = 0;
-    for (int r = 0; r < RSHARD; ++r) {
-      if ((int64_t)cs[r] > g.raster_region)
-        throw Error(BM_EOVERFLOW, "spike raster overflow: region " + std::to_string(r) + " received " +
-                                      std::to_string(cs[r]) + " spikes > " + std::to_string(g.raster_region) +
-                                      " (record with a larger capacity)");
-      c += (int64_t)cs[r];
-    }
-    BM_REQUIRE(c <= cap, "output buffer too small");
-    std::vector<uint64_t> tmp(c);
-    int64_t at = 0;
-    for (int r = 0; r < RSHARD; ++r) {
-      if (cs[r])
-        BM_HIP(hipMemcpyAsync(tmp.data() + at, g.raster_d.as<unsigned long long>() + (int64_t)r * g.raster_region,
-                              cs[r] * 8, hipMemcpyDeviceToHost, net->stream));
-      at += (int64_t)cs[r];
-    }
-    BM_HIP(hipStreamSynchronize(net->stream));
+    std::vector<uint64_t> tmp = fetch_raster(net, g, cap, nullptr);
     std::sort(tmp.begin(), tmp.end());
     for (size_t q = 0; q < tmp.size(); ++q) {
       steps[q] = (int32_t)(tmp[q] >> 32);
       idx[q] = (int32_t)(tmp[q] & 0xFFFFFFFFu);
     }
-    *n_out = c;
+    *n_out = (int64_t)tmp.size();
+  });
+}
+
+int bm_net_record_spike_states(bm_net *net, int gid, int nvars, const int *vars) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.kind == GK_NEURON, "only neuron groups have state variables");
+    BM_REQUIRE(g.raster_cap > 0, "record the group's spikes first");
+    BM_REQUIRE(nvars >= 1 && nvars <= 4 && vars, "1 to 4 variables");
+    g.ssm_vars.clear();
+    for (int j = 0; j < nvars; ++j) {
+      BM_REQUIRE(vars[j] >= 0 && vars[j] < g.M, "variable out of range");
+      g.ssm_vars.push_back(vars[j]);
+    }
+  });
+}
+
+int bm_net_get_spike_states(bm_net *net, int gid, int32_t *steps, int32_t *idx, double *vals, int64_t cap,
+                            int64_t *n_out) {
+  return guard([&] {
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.raster_cap > 0 && net->prepared, "group is not recorded or network not prepared");
+    BM_REQUIRE(!g.ssm_vars.empty(), "no variables are recorded at spike times for this group");
+    std::vector<double> v;
+    const std::vector<uint64_t> tmp = fetch_raster(net, g, cap, &v);
+    const size_t nv = g.ssm_vars.size();
+    std::vector<int64_t> order(tmp.size());
+    for (size_t q = 0; q < order.size(); ++q) order[q] = (int64_t)q;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return tmp[a] < tmp[b]; });
+    for (size_t q = 0; q < order.size(); ++q) {
+      const uint64_t e = tmp[order[q]];
+      steps[q] = (int32_t)(e >> 32);
+      idx[q] = (int32_t)(e & 0xFFFFFFFFu);
+      for (size_t j = 0; j < nv; ++j) vals[q * nv + j] = v[order[q] * nv + j];
+    }
+    *n_out = (int64_t)tmp.size();
   });
 }
 

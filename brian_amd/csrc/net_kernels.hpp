@@ -879,6 +879,33 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
             }
           }
         }
+        if constexpr (GEN) {
+          // state at spike times: the raster entry and its values are
+          // written here, at the spike step, before the reset
+          // (StateSpikeMonitor.propagate, brian/monitor.py:388-396)
+          if (g.ssm_n > 0 && g.raster != nullptr) {
+            const int shard = blockIdx.x & (RSHARD - 1);
+            unsigned long long rb = 0;
+            if (lane_id() == leader)
+              rb = atomicAdd(g.raster_count + shard * RSTRIDE, (unsigned long long)__popcll(mask));
+            rb = __shfl(rb, leader);
+            if (spk[k]) {
+              const unsigned long long below = mask & ((1ull << lane_id()) - 1ull);
+              const unsigned long long pos = rb + __popcll(below);
+              if (pos < (unsigned long long)g.raster_cap) {
+                const int64_t at = (int64_t)shard * g.raster_cap + (int64_t)pos;
+                g.raster[at] = ((uint64_t)n << 32) | (uint64_t)(blk0 + k * BLOCK + tid);
+                for (int q = 0; q < g.ssm_n; ++q) {
+                  double v = 0.0;
+#pragma unroll
+                  for (int j = 0; j < MM; ++j)
+                    if (j == g.ssm_var[q]) v = (double)s[j][k];
+                  g.ssm_vals[at * g.ssm_n + q] = v;
+                }
+              }
+            }
+          }
+        }
       }
     }
   }
@@ -887,7 +914,9 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
   const int nsp = sh_n;
   if (nsp > 0) {
     // ---------------- raster append (SpikeMonitor, brian/monitor.py:156-166)
-    if (g.raster != nullptr) {
+    bool appended = false;      // entries already written at the spike step
+    if constexpr (GEN) appended = g.ssm_n > 0;
+    if (g.raster != nullptr && !appended) {
       const int shard = blockIdx.x & (RSHARD - 1);
       if (tid == 0) sh_base = atomicAdd(g.raster_count + shard * RSTRIDE, (unsigned long long)nsp);
       __syncthreads();

@@ -192,6 +192,11 @@ struct GroupDev {
   const uint64_t *ptab;
   int64_t ptab_n0, ptab_len;
   int ptab_per_neuron, pad4;
+  // state at spike times (StateSpikeMonitor, brian/monitor.py:337-397):
+  // ssm_n values per raster entry, at the entry's position, in FP64
+  double *ssm_vals;
+  int ssm_n;
+  int8_t ssm_var[4];
 };
 // passed by value to every sweep launch; kernel arguments are limited to 4 KiB
 static_assert(sizeof(GroupDev) <= 3584, "GroupDev must leave room in the 4 KiB kernel arguments");

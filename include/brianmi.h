@@ -257,6 +257,13 @@ int bm_net_set_stdp_delays(bm_net *net, int sid, const double *syn_delay_s);
  * capacity entries (brian/monitor.py:156-166). */
 int bm_net_record(bm_net *net, int gid, int64_t capacity);
 
+/* StateSpikeMonitor (brian/monitor.py:337-397): with the raster, the values
+ * of state variables vars[0..nvars) (1 to 4) of each spiking neuron at its
+ * spike step, after the threshold and before the reset, as the monitor's
+ * propagate reads them.  Neuron groups recorded with bm_net_record; before
+ * prepare.  The group then runs the general sweep. */
+int bm_net_record_spike_states(bm_net *net, int gid, int nvars, const int *vars);
+
 /* StateMonitor (brian/monitor.py:683-990, when='end'): recorded inside the
  * sweep after the resets of each step, so windows of several steps keep
  * running.  indices != NULL: variable `var` of neurons indices[0..nrec) at
@@ -329,6 +336,10 @@ int bm_net_spike_count(bm_net *net, int gid, int64_t *count);
 /* Raster sorted by (step, index); n_out = number of entries written. */
 int bm_net_get_spikes(bm_net *net, int gid, int32_t *steps, int32_t *idx, int64_t cap,
                       int64_t *n_out);
+/* The raster with the recorded states: vals[q * nvars + j] is variable
+ * vars[j] of entry q (bm_net_record_spike_states). */
+int bm_net_get_spike_states(bm_net *net, int gid, int32_t *steps, int32_t *idx, double *vals, int64_t cap,
+                            int64_t *n_out);
 int bm_net_clear_spikes(bm_net *net, int gid);
 int bm_net_get_weights(bm_net *net, int cid, double *w);
 int bm_net_get_stdp_traces(bm_net *net, int sid, double *a_pre, double *a_post);

@@ -672,6 +672,8 @@ class Net:
         self.pinputs = []
         self.n = 0
         self.record = {}     # group id -> list of (step, idx array)
+        self.state_vars = {}     # group id -> variables read at spike times
+        self.state_record = {}   # group id -> list of (nspikes, nvars) arrays
 
     def add_group(self, g, record=True):
         self.groups.append(g)
@@ -679,6 +681,20 @@ class Net:
         if record:
             self.record[gid] = []
         return gid
+
+    def record_states(self, gid, vars):
+        """StateSpikeMonitor(source, vars) (monitor.py:379-396): the values of
+        ``vars`` of the spiking neurons, read after the group's update and
+        threshold, before any connection of the step and before the reset."""
+        self.state_vars[gid] = list(vars)
+        self.state_record[gid] = []
+
+    def spike_states(self, gid):
+        """Values recorded at the raster's entries, in raster order."""
+        rec = self.state_record[gid]
+        if not rec:
+            return np.zeros((0, len(self.state_vars[gid])))
+        return np.concatenate(rec, axis=0)
 
     def add_connection(self, *a, **k):
         c = Connection(self, *a, **k)
@@ -703,6 +719,9 @@ class Net:
             sp = g.update(t, n)
             if gid in self.record and len(sp):
                 self.record[gid].append((n, sp.copy()))
+                if gid in self.state_vars:
+                    self.state_record[gid].append(
+                        np.stack([g.S[v][sp] for v in self.state_vars[gid]], axis=1).astype(np.float64))
         for s in self.stdps:
             s.decay()
         for pin in self.pinputs:

@@ -394,3 +394,83 @@ def test_more_monitors():
     assert Rw.values.shape == (3, 11)
     np.testing.assert_array_equal(Rw[2], MM["w"][2][-11:])
     np.testing.assert_allclose(Rw.times, MM.times[-11:])
+
+
+def test_state_spike_monitor_matches_oracle():
+    """StateSpikeMonitor (monitor.py:337-397) on the 4000-neuron CUBA
+    network, FP64: the raster and the values of v, ge and gi at each spike
+    (after the threshold, before the reset) equal the oracle's bit for bit."""
+    from brian_amd import workloads as W
+    from brian_amd._lib import NetHandle
+    from oracle import brian1 as O
+    from tests.netbuild import cuba_propagator
+    d = W.cuba(N=4000, seed=43)
+    N, Ne, Ni = d["N"], d["Ne"], d["Ni"]
+    A, C = cuba_propagator()
+    h = NetHandle(1e-4, 64)
+    g = h.add_group(d["S0"])
+    h.set_linear(g, A, C)
+    h.set_threshold(g, 0, d["Vt"])
+    h.add_reset_op(g, 0, 0, d["Vr"])
+    net = O.Net(1e-4)
+    gid = net.add_group(O.Group(N, d["S0"], ("linear", A, C), ("const", 0, d["Vt"]),
+                                [(0, "=", d["Vr"], False)], dt=1e-4))
+    for (ip, ix, ww), off, n, var in ((d["exc"], 0, Ne, 1), (d["inh"], Ne, Ni, 2)):
+        h.add_connection(g, off, n, g, 0, N, var, ip, ix, ww)
+        net.add_connection((gid, off, n), (gid, 0, N), var, ip, ix, ww)
+    h.record(g, 1 << 20)
+    h.record_spike_states(g, [0, 1, 2])
+    net.record_states(gid, [0, 1, 2])
+    h.prepare()
+    h.run(1500)
+    net.run(1500)
+    s, i, v = h.get_spike_states(g)
+    rs, ri = net.raster(gid)
+    assert len(rs) > 1000
+    np.testing.assert_array_equal(s, rs)
+    np.testing.assert_array_equal(i, ri)
+    np.testing.assert_array_equal(v, net.spike_states(gid))
+    assert np.all(v[:, 0] > d["Vt"])
+    s2, i2 = h.get_spikes(g)
+    np.testing.assert_array_equal(s2, rs)
+    h.close()
+
+
+def test_state_spike_monitor_interface():
+    """examples/frompapers/Touboul_Brette_2008.py shortened: AdEx neurons
+    with a per-neuron reset value Vr; a StateSpikeMonitor(neuron, ("Vr", "w"))
+    added for the second run records Vr and w at spike times.  Its raster
+    equals a SpikeMonitor's, Vr is the neuron's own, vm at the spike is past
+    the cut and w is the adaptation current read before its reset."""
+    from brian_amd import (NeuronGroup, StateSpikeMonitor, SpikeMonitor, StateMonitor, run, ms, mV, nS, pF, nA,
+                           second, defaultclock, linspace)
+    defaultclock.dt = 0.01 * ms
+    C, gL, EL, VT, DeltaT = 281 * pF, 30 * nS, -70.6 * mV, -50.4 * mV, 2 * mV
+    tauw, a, b, I = 40 * ms, 4 * nS, 0.08 * nA, .8 * nA
+    Vcut = VT + 5 * DeltaT
+    N = 50
+    eqs = """
+    dvm/dt=(gL*(EL-vm)+gL*DeltaT*exp((vm-VT)/DeltaT)+I-w)/C : volt
+    dw/dt=(a*(vm-EL)-w)/tauw : amp
+    Vr:volt
+    """
+    neuron = NeuronGroup(N, model=eqs, threshold=Vcut, reset="vm=Vr;w+=b")
+    neuron.vm = EL
+    neuron.w = a * (neuron.vm - EL)
+    neuron.Vr = linspace(-48.3 * mV, -47.7 * mV, N)
+    run(100 * ms)
+    M = StateSpikeMonitor(neuron, ("Vr", "w"))
+    S = SpikeMonitor(neuron)
+    V = StateSpikeMonitor(neuron, "vm")
+    run(100 * ms)
+    Vr, w = M.values("Vr"), M.values("w")
+    assert M.nspikes == S.nspikes > 50
+    assert [sp[:2] for sp in M.spikes] == S.spikes
+    idx = np.array([sp[0] for sp in M.spikes])
+    np.testing.assert_array_equal(Vr, np.asarray(neuron.Vr)[idx])
+    assert np.all(V.values("vm") > Vcut)
+    assert np.all(w > 0)
+    for i in (0, N - 1):
+        np.testing.assert_array_equal(M.times(i), S[i])
+        np.testing.assert_array_equal(M.values("w", i), w[idx == i])
+    defaultclock.dt = 0.1 * ms

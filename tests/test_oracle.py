@@ -597,3 +597,38 @@ def test_generic_stdp_rule_closed_forms():
     assert abs(both - (-2 * alpha * eta + eta * (e(30) + e(20)))) < 1e-14
     last = run([10, 20], [40], ops=("=", "+="))
     assert abs(last - (-2 * alpha * eta + eta * e(20))) < 1e-14
+
+
+def test_spike_states_closed_form():
+    """Net.record_states (StateSpikeMonitor, monitor.py:379-396): for
+    dv/dt = (I - v)/tau with exact propagator, threshold v > 0.8 I and reset
+    to 0, neuron i starting at v0_i first spikes at the least k with
+    I - (I - v0_i) a^k > 0.8 I (a = exp(-dt/tau)) and then every K steps
+    (K the least k with I (1 - a^k) > 0.8 I); the value recorded at every
+    spike is the pre-reset I - (I - v0) a^k, and the parameter u_i comes
+    back as the neuron's own."""
+    dt, tau, Iin = 1e-4, 0.01, 1.0
+    a = np.exp(-dt / tau)
+    v0 = np.array([0.0, 0.3, 0.6, 0.75])
+    N = len(v0)
+    u = np.arange(N) + 10.0
+    A = np.array([[a, 0.0], [0.0, 1.0]])
+    Cv = np.array([Iin * (1 - a), 0.0])
+    net = O.Net(dt)
+    gid = net.add_group(O.Group(N, np.stack([v0, u]), ("linear", A, Cv), ("const", 0, 0.8 * Iin),
+                                [(0, "=", 0.0, False)], dt=dt))
+    net.record_states(gid, [1, 0])
+    net.run(2000)
+    steps, idx = net.raster(gid)
+    vals = net.spike_states(gid)
+    K = int(np.ceil(np.log(0.2) / np.log(a)))
+    assert len(steps) > 4 * 10
+    for i in range(N):
+        k1 = int(np.ceil(np.log(0.2 * Iin / (Iin - v0[i])) / np.log(a)))
+        want_steps = np.arange(k1 - 1, 2000, K)      # the state after update n is v after n + 1 updates
+        got = steps[idx == i]
+        np.testing.assert_array_equal(got, want_steps[:len(got)])
+        assert len(got) == len(want_steps)
+        np.testing.assert_array_equal(vals[idx == i, 0], u[i])
+        np.testing.assert_allclose(vals[idx == i, 1][0], Iin - (Iin - v0[i]) * a ** k1, rtol=1e-12)
+        np.testing.assert_allclose(vals[idx == i, 1][1:], Iin * (1 - a ** K), rtol=1e-12)
