@@ -872,11 +872,6 @@ class Connection(_Tracked):
         self._init_common(source, target, state, modulation, weight, sparseness, structure)
         self.delay = int(float(delay) / source.clock.dt)      # connection.py:200
         self._delay_kind = 0
-        if self._mod is not None and self.delay > 0:
-            # connection.py:242-247 reads the modulating variable `delay`
-            # steps after the spike; the device captures it at the spike step
-            raise NotImplementedError("modulation on a Connection with a homogeneous delay "
-                                      "is not supported (use DelayConnection)")
 
     def _init_common(self, source, target, state, modulation, weight, sparseness, structure="sparse"):
         if structure not in ("sparse", "dense", "dynamic"):

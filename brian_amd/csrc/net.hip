@@ -164,6 +164,7 @@ struct bm_net {
   DevBuf d_base;          // int64 device step base
   DevBuf hist_ptrs_d, hist_H_d;
   int nhist = 0;
+  std::vector<LateModDev> lates;   // modulated Connections with a homogeneous delay
   // graph cache
   hipGraphExec_t graph_exec = nullptr;
   int graph_windows = 0;
@@ -249,6 +250,18 @@ void require_unprepared(bm_net *net) {
 // The lean kernel covers: shared linear propagator, 1..4 variables, constant
 // or variable threshold, reset ops without refractoriness, at most one
 // (packed count) ring, no timed input.
+// a modulated Connection with a homogeneous delay reads the modulating
+// variable when it delivers (connection.py:242-247, :296)
+bool late_mod(const HostConn &c) { return c.mod_var >= 0 && c.delay_kind == 0 && c.hom_delay > 0; }
+
+// STDP and delayed modulation run kernels between steps: one step per launch
+bool one_step(const bm_net *net) {
+  if (!net->stdps.empty()) return true;
+  for (auto &cp : net->conns)
+    if (late_mod(*cp)) return true;
+  return false;
+}
+
 bool fast_eligible(const HostGroup &g) {
   if (g.kind != GK_NEURON || g.upd != UPD_LINEAR || g.M < 1 || g.M > 4) return false;
   if (!g.mons.empty() || g.mod_var >= 0 || g.stp_conn >= 0 || !g.pins.empty() || g.coin) return false;
@@ -418,6 +431,10 @@ void launch_stdp_step(bm_net *net, int64_t rel0) {
     if (net->precision == 64) launch_stdp<double>(sp->dev, net->stream, dbase, rel0);
     else launch_stdp<float>(sp->dev, net->stream, dbase, rel0);
   }
+  for (const LateModDev &lm : net->lates) {
+    if (net->precision == 64) launch_late_mod<double>(lm, net->stream, dbase, rel0);
+    else launch_late_mod<float>(lm, net->stream, dbase, rel0);
+  }
   if (net->nhist > 0)
     launch_hist_clear(net->hist_ptrs_d.as<unsigned int *>(), net->hist_H_d.as<int>(), net->nhist, dbase,
                       rel0 + 1, net->stream);
@@ -426,7 +443,7 @@ void launch_stdp_step(bm_net *net, int64_t rel0) {
 // One window of W steps starting at base+rel0 for every group (+ STDP when W==1).
 void launch_window(bm_net *net, int64_t rel0, int W, int skip_first_tail) {
   for (auto &gp : net->groups) launch_group(net, *gp, rel0, W, skip_first_tail, 0);
-  if (!net->stdps.empty()) launch_stdp_step(net, rel0);
+  if (one_step(net)) launch_stdp_step(net, rel0);
 }
 
 void check_launch() { BM_HIP(hipGetLastError()); }
@@ -484,13 +501,13 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
   int dmin = 1 << 30;
   for (auto &cp : net->conns) dmin = std::min(dmin, cp->min_delay);
   int Wcap = net->conns.empty() ? WMAX : std::min(WMAX, dmin + 1);
-  if (!net->stdps.empty()) Wcap = 1;
+  if (one_step(net)) Wcap = 1;
   int W = window > 0 ? window : Wcap;
-  if (!net->stdps.empty()) W = 1;
+  if (one_step(net)) W = 1;
   // deferred propagation delivers a window's spikes during the next launch:
   // they must land in slots no launch consumes before then, W <= (dmin+1)/2
   net->defer_W = 0;
-  if (net->defer_req && net->specialize && net->stdps.empty() && !net->conns.empty() && (dmin + 1) / 2 >= 1) {
+  if (net->defer_req && net->specialize && !one_step(net) && !net->conns.empty() && (dmin + 1) / 2 >= 1) {
     net->defer_W = std::min(WMAX, (dmin + 1) / 2);
     BM_REQUIRE(window <= net->defer_W, "window exceeds (minimum delay + 1) / 2 with deferred propagation");
     W = std::min(W, net->defer_W);
@@ -646,6 +663,14 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       g.chans.push_back(cd);
     }
     BM_REQUIRE(g.chans.size() <= (size_t)MAXCHAN, "too many incoming connections for one group");
+  }
+  // ---- spike histories of delayed modulated connections
+  for (auto &cp : net->conns) {
+    if (!late_mod(*cp)) continue;
+    BM_REQUIRE(cp->stdp < 0, "STDP on a modulated Connection with a homogeneous delay is not supported");
+    HostGroup &sg = *net->groups[cp->src];
+    sg.need_hist = true;
+    sg.H = std::max(sg.H, cp->hom_delay + 2);
   }
   // ---- STDP structures and histories
   for (auto &sp : net->stdps) {
@@ -822,6 +847,7 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       HostConn &c = *net->conns[ci];
       if (c.src != (int)gi) continue;
       if (c.stdp >= 0 && c.delay_kind == 0 && c.hom_delay > 0) continue;   // propagated by k_stdp_pre
+      if (late_mod(c)) continue;                                            // propagated by k_late_mod
       BM_REQUIRE(nout < MAXOUT, "too many outgoing connections for one group");
       HostGroup &tg = *net->groups[c.tgt];
       OutDev &o = d.outs[nout++];
@@ -980,6 +1006,33 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     d.post_hist_count = tg.hist_count_d.as<unsigned int>();
     d.post_H = tg.H;
     d.post_N = tg.N;
+  }
+  net->lates.clear();
+  for (auto &cp : net->conns) {
+    HostConn &c = *cp;
+    if (!late_mod(c)) continue;
+    HostGroup &sg = *net->groups[c.src];
+    HostGroup &tg = *net->groups[c.tgt];
+    const RingDev &rd = tg.ring_desc[c.ring];
+    BM_REQUIRE(rd.kind == RING_FLOAT, "modulated connections deliver through a float ring");
+    LateModDev lm{};
+    lm.row_ptr = c.row_ptr_d.as<int64_t>();
+    lm.col = c.col_d.as<int32_t>();
+    lm.w = c.w_d.p;
+    lm.S = sg.S.p;
+    lm.hist = sg.hist_d.as<int32_t>();
+    lm.hist_count = sg.hist_count_d.as<unsigned int>();
+    lm.ring = rd.base;
+    lm.src_off = c.src_off;
+    lm.src_n = c.src_n;
+    lm.src_N = sg.N;
+    lm.tgt_off = c.tgt_off;
+    lm.ring_slot_bytes = rd.slot_bytes;
+    lm.H = sg.H;
+    lm.ring_D = tg.D;
+    lm.delay = c.hom_delay;
+    lm.mod_var = c.mod_var;
+    net->lates.push_back(lm);
   }
   check_launch_shapes(net);
   ensure_jit(net);
@@ -1571,9 +1624,6 @@ int bm_net_set_modulation(bm_net *net, int cid, int src_var) {
     HostGroup &sg = grp(net, c.src);
     BM_REQUIRE(sg.kind == GK_NEURON && src_var >= 0 && src_var < sg.M, "modulation variable out of range");
     BM_REQUIRE(!c.generated, "modulation of device-generated connections is not supported");
-    BM_REQUIRE(c.delay_kind == 1 || c.hom_delay == 0,
-               "a modulated Connection with a homogeneous delay reads the presynaptic value at delivery "
-               "time; use delay 0 or a DelayConnection");
     c.mod_var = src_var;
   });
 }
@@ -1837,7 +1887,8 @@ int bm_net_set_tuning(bm_net *net, int key, int value) {
         const int W = value == 0 ? wmax : value;
         BM_REQUIRE(W >= 1 && W <= wmax, "window out of range [1, minimum delay + 1] ((minimum delay + 1) / 2 "
                                         "with deferred propagation)");
-        BM_REQUIRE(net->stdps.empty() || W == 1, "networks with STDP run one step per launch");
+        BM_REQUIRE(!one_step(net) || W == 1,
+                   "networks with STDP or delayed modulated connections run one step per launch");
         const int old = net->W;
         net->W = W;
         try {

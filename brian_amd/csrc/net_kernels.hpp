@@ -1088,6 +1088,35 @@ __device__ inline T stdp_weight(const StdpDev &sd, T x, T a, int mode, double of
 
 // Presynaptic updater (brian/stdp.py:371-393): for each pre spike i,
 // A_pre[i] += Ap; w[i,:] += A_post[cols]; clip.  One wavefront per spike.
+// Connection.propagate of a modulated Connection with a homogeneous delay
+// d (connection.py:224-247, :296): at step m the spikes of step m - d add
+// w * mod into the target's ring slot of step m, mod being the source
+// variable as it stands at step m, after the update and before the reset.
+// Networks with such connections run one step per launch, so the state the
+// sweep wrote back is that value.  One wavefront per spike.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_late_mod(const LateModDev lm, const int64_t *d_base, int64_t rel0) {
+#pragma clang fp contract(off)
+  const int64_t m = *d_base + rel0;
+  const int64_t n = m - lm.delay;
+  if (n < 0) return;
+  const int64_t hs = n % lm.H;
+  const unsigned int cnt = lm.hist_count[hs];
+  const int wave = (blockIdx.x * BLOCK + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  const int nwaves = gridDim.x * (BLOCK / 64);
+  const T *S = static_cast<const T *>(lm.S);
+  const T *w = static_cast<const T *>(lm.w);
+  T *ring = reinterpret_cast<T *>(static_cast<unsigned char *>(lm.ring) + (m % lm.ring_D) * lm.ring_slot_bytes);
+  for (unsigned int p = wave; p < cnt; p += nwaves) {
+    const int64_t gi = lm.hist[hs * lm.src_N + p];
+    const int64_t i = gi - lm.src_off;
+    if (i < 0 || i >= lm.src_n) continue;
+    const T mv = S[(int64_t)lm.mod_var * lm.src_N + gi];
+    const int64_t lo = lm.row_ptr[i], hi = lm.row_ptr[i + 1];
+    for (int64_t q = lo + lane; q < hi; q += 64) unsafeAtomicAdd(ring + lm.tgt_off + lm.col[q], w[q] * mv);
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_stdp_pre(const StdpDev sd, const int64_t *d_base, int64_t rel0) {
 #pragma clang fp contract(off)

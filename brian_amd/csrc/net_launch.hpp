@@ -22,6 +22,8 @@ template <typename T>
 void launch_generator(const GroupDev &g, hipStream_t st, const int64_t *d_base, int64_t rel0, int W);
 template <typename T>
 void launch_stdp(const StdpDev &sd, hipStream_t st, const int64_t *d_base, int64_t rel0);
+template <typename T>
+void launch_late_mod(const LateModDev &lm, hipStream_t st, const int64_t *d_base, int64_t rel0);
 void launch_hist_clear(unsigned int *const *counts, const int *H, int ngroups, const int64_t *d_base,
                        int64_t rel_next, hipStream_t st);
 void launch_advance(int64_t *d_base, int64_t by, hipStream_t st);

@@ -72,10 +72,16 @@ void launch_stdp(const StdpDev &sd, hipStream_t st, const int64_t *d_base, int64
   hipLaunchKernelGGL(k_stdp_post<T>, dim3(128), dim3(BLOCK), 0, st, sd, d_base, rel0);
 }
 
+template <typename T>
+void launch_late_mod(const LateModDev &lm, hipStream_t st, const int64_t *d_base, int64_t rel0) {
+  hipLaunchKernelGGL(k_late_mod<T>, dim3(64), dim3(BLOCK), 0, st, lm, d_base, rel0);
+}
+
 #define BM_INSTANTIATE_LAUNCH(T)                                                                  \
   template void launch_sweep<T>(const GroupDev &, bool, int, int, unsigned, size_t, hipStream_t,  \
                                 const SweepArgs &);                                               \
   template void launch_generator<T>(const GroupDev &, hipStream_t, const int64_t *, int64_t, int); \
-  template void launch_stdp<T>(const StdpDev &, hipStream_t, const int64_t *, int64_t);
+  template void launch_stdp<T>(const StdpDev &, hipStream_t, const int64_t *, int64_t);            \
+  template void launch_late_mod<T>(const LateModDev &, hipStream_t, const int64_t *, int64_t);
 
 }  // namespace bm

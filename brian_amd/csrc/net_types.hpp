@@ -202,6 +202,20 @@ struct GroupDev {
 static_assert(sizeof(GroupDev) <= 3584, "GroupDev must leave room in the 4 KiB kernel arguments");
 
 
+// A modulated Connection with a homogeneous delay, propagated one step at
+// a time outside the sweep (the modulating value is read at delivery).
+struct LateModDev {
+  const int64_t *row_ptr;
+  const int32_t *col;
+  const void *w;               // T[nnz]
+  const void *S;               // source group's state, T[M * src_N]
+  const int32_t *hist;         // source spike history [H * src_N]
+  const unsigned int *hist_count;
+  void *ring;
+  int64_t src_off, src_n, src_N, tgt_off, ring_slot_bytes;
+  int H, ring_D, delay, mod_var;
+};
+
 struct StdpDev {
   int64_t src_off, src_n, tgt_off, tgt_n;
   const int64_t *row_ptr;

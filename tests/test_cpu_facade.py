@@ -22,9 +22,9 @@ def test_build_objects_without_device():
 
 
 def test_modulation_facade_checks():
-    """connection.py:190-193: modulation names a source variable; a plain
-    Connection with a homogeneous delay reads it `delay` steps late, which
-    the device does not reproduce, so it is refused."""
+    """connection.py:190-193: modulation names a source variable, on a
+    Connection with or without a homogeneous delay and on a
+    DelayConnection."""
     import brian_amd as b
     b.reinit()
     H = b.NeuronGroup(2, 'V:1\nmod:1', reset=0, threshold=1)
@@ -33,8 +33,8 @@ def test_modulation_facade_checks():
     assert c._mod == 1
     d = b.Connection(H, G, 'V', weight=2, delay=True, modulation='mod')
     assert d._mod == 1
-    with pytest.raises(NotImplementedError):
-        b.Connection(H, G, 'V', weight=2, delay=1 * b.ms, modulation='mod')
+    e = b.Connection(H, G, 'V', weight=2, delay=1 * b.ms, modulation='mod')
+    assert e._mod == 1 and e.delay == 10
 
 
 def test_stp_facade_checks():

@@ -373,8 +373,10 @@ def test_modulated_connection_matches_oracle():
     """Weights scaled by the presynaptic variable at the spike step
     (connection.py:190-193, :242-247; delayconnection.py:139-206): the 4000
     neuron CUBA network with a fourth variable ``mod`` (relaxing to 1.5,
-    reset to 0.5 on a spike) modulating both projections, zero delay and
-    DelayConnection, FP64 raster bit-exact against the oracle."""
+    reset to 0.5 on a spike) modulating both projections, zero delay,
+    DelayConnection, and a 1 ms homogeneous delay (the modulating value read
+    at delivery, connection.py:296), FP64 raster bit-exact against the
+    oracle."""
     from brian_amd import workloads as W
     from brian_amd._lib import NetHandle
     from oracle import brian1 as O
@@ -392,7 +394,7 @@ def test_modulated_connection_matches_oracle():
     rng = np.random.default_rng(12)
     S0 = np.vstack([d["S0"], 0.5 + rng.random(N)])
     steps = 2000
-    for kind in (0, 1):
+    for kind in (0, 1, 2):
         dl = {}
         if kind == 1:
             dl = {k: rng.integers(0, 8, len(d[k][1])).astype(np.int32) for k in ("exc", "inh")}
@@ -407,9 +409,10 @@ def test_modulated_connection_matches_oracle():
                      [(0, "=", d["Vr"], False), (3, "=", 0.5, False)], dt=1e-4)
         gid = net.add_group(og)
         for (ip, ix, w), off, n, var, key in ((d["exc"], 0, Ne, 1, "exc"), (d["inh"], Ne, Ni, 2, "inh")):
-            if kind == 0:
-                cid = h.add_connection(g, off, n, g, 0, N, var, ip, ix, w, delay_steps=0)
-                net.add_connection((gid, off, n), (gid, 0, N), var, ip, ix, w, delay_steps=0,
+            if kind in (0, 2):
+                ds = 0 if kind == 0 else 10
+                cid = h.add_connection(g, off, n, g, 0, N, var, ip, ix, w, delay_steps=ds)
+                net.add_connection((gid, off, n), (gid, 0, N), var, ip, ix, w, delay_steps=ds,
                                    modulation=3)
             else:
                 cid = h.add_connection(g, off, n, g, 0, N, var, ip, ix, w, syn_delay=dl[key],
