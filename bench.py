@@ -13,7 +13,8 @@ in HBM before the timed region.
 
 Reported: neuron-updates/s of the FP32 windowed path (``value``), synaptic
 events/s, the plain FP64 GPU path (one step per launch, check mode) for the
-same steps, FP32-vs-FP64 population-rate / ISI-CV error and first raster
+same steps and the FP64 path with the FP32 path's window and launch shape
+(its raster compared with the one-step raster), FP32-vs-FP64 population-rate / ISI-CV error and first raster
 divergence step, the roofline of the fused sweep kernel (HIP-event timing on
 the library's stream) and the numpy FP64 oracle timed on the host as the CPU
 baseline, next to the reference's own C++ CUBA program when it is built.
@@ -293,6 +294,14 @@ def bench_cuba1m(args):
         el64 = timed_run(h64, K, Wm)
         r64 = h64.get_spikes(g64)
         h64.close()
+        # the FP64 path with the FP32 path's window and launch shape: the
+        # same arithmetic, so its raster equals the one-step raster exactly
+        h64w, g64w = build(64, args.window, tuning)[:2]
+        el64w = timed_run(h64w, K, Wm)
+        r64w = h64w.get_spikes(g64w)
+        h64w.close()
+        same = len(r64w[0]) == len(r64[0]) and bool(np.array_equal(r64w[0], r64[0])
+                                                   and np.array_equal(r64w[1], r64[1]))
         m32, m64 = r32[0] >= Wm, r64[0] >= Wm
         rate32, cv32 = rate_stats(r32[0][m32], r32[1][m32], N, K)
         rate64, cv64 = rate_stats(r64[0][m64], r64[1][m64], N, K)
@@ -304,6 +313,8 @@ def bench_cuba1m(args):
             "rate_rel_err": abs(rate32 - rate64) / rate64,
             "isi_cv_abs_err": abs(cv32 - cv64),
             "first_raster_divergence_step": first_divergence(r32, r64),
+            "windowed": {"neuron_updates_per_s": N * K / el64w, "ms_per_step": el64w * 1e3 / K,
+                         "raster_equal_to_one_step": same},
         }
 
     # ---- roofline of the fused sweep/propagation kernel (FP32 path)
@@ -340,6 +351,7 @@ def bench_cuba1m(args):
         "spikes": n_sp32,
         "fp64_plain": fp64,
         "fp32_over_fp64_plain": (value / fp64["neuron_updates_per_s"]) if fp64 else None,
+        "fp32_over_fp64_windowed": (value / fp64["windowed"]["neuron_updates_per_s"]) if fp64 else None,
         "autotune": tune_report,
         "roofline": roof,
         "cpu_baseline": cpu,
