@@ -84,6 +84,7 @@ _SIGS = {
     "bm_net_spike_count": [P, C.c_int, Pi64],
     "bm_net_get_spikes": [P, C.c_int, Pi32, Pi32, i64, Pi64],
     "bm_net_record_spike_states": [P, C.c_int, C.c_int, Pi32],
+    "bm_pairwise_sq_distance": [Pd, i64, i64, C.c_double, Pd],
     "bm_net_get_spike_states": [P, C.c_int, Pi32, Pi32, Pd, i64, Pi64],
     "bm_net_clear_spikes": [P, C.c_int],
     "bm_net_get_weights": [P, C.c_int, Pd],
@@ -200,6 +201,16 @@ def as_i32(a):
 
 def as_i64(a):
     return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def pairwise_sq_distance(X, scale):
+    """D[i, j] = scale * sum_t (X[i, t] - X[j, t])^2 computed on the device
+    (the van Rossum distance matrix, brian/monitor.py:1427-1486)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    n, T = X.shape
+    D = np.empty((n, n), dtype=np.float64)
+    call("bm_pairwise_sq_distance", dptr(X), n, T, float(scale), dptr(D))
+    return D
 
 
 class NetHandle:

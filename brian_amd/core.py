@@ -31,7 +31,7 @@ __all__ = [
     "Connection", "DelayConnection", "STDP", "ExponentialSTDP", "STP", "ConnectionMatrix",
     "SparseConnectionVector", "random_matrix", "random_matrix_fixed_column",
     "SpikeMonitor", "PopulationRateMonitor", "PopulationSpikeCounter", "SpikeCounter", "CoincidenceCounter",
-    "FileSpikeMonitor", "AERSpikeMonitor", "ISIHistogramMonitor", "StateSpikeMonitor", "MultiStateMonitor", "RecentStateMonitor",
+    "FileSpikeMonitor", "AERSpikeMonitor", "ISIHistogramMonitor", "StateSpikeMonitor", "VanRossumMetric", "MultiStateMonitor", "RecentStateMonitor",
     "StateMonitor", "Network", "MagicNetwork", "run", "reinit", "stop", "network_operation",
     "NetworkOperation", "set_global_preferences", "get_global_preference", "forget", "recall",
 ]
@@ -1742,6 +1742,33 @@ class StateMonitor(_Tracked):
         self._hn = 0
 
 
+class VanRossumMetric(StateMonitor):
+    """brian/monitor.py:1378-1486: the van Rossum distance between the spike
+    trains of every pair of neurons of ``source``.  Each train is filtered
+    by dv/dt = -v/tau in a group of its own (fed one to one with weight 1)
+    whose v is recorded every step; ``distance`` is the symmetric matrix
+    dt/tau * sum_t (v_i - v_j)^2, computed on the device
+    (``bm_pairwise_sq_distance``)."""
+
+    def __init__(self, source, tau=2 * 0.001):
+        self.dt = source.clock.dt
+        self.source = source
+        self.nbr_neurons = len(source)
+        self.tau = float(tau)
+        kernel = NeuronGroup(self.nbr_neurons, model=Equations("dv/dt=(-v)/tau: volt", tau=self.tau),
+                             clock=source.clock, level=1)
+        C = Connection(source, kernel, "v")
+        C.connect_one_to_one(source, kernel)
+        StateMonitor.__init__(self, kernel, "v", record=True)
+        self.contained_objects = [kernel, C]
+
+    def get_distance(self):
+        from . import _lib
+        return _lib.pairwise_sq_distance(self.values, self.dt / self.tau)
+
+    distance = property(fget=get_distance)
+
+
 class RecentStateMonitor(StateMonitor):
     """monitor.py:992-1110: a StateMonitor (record=True by default) whose
     values and times keep the last int(duration/(timestep dt)) + 1 samples."""
@@ -1907,6 +1934,8 @@ class Network:
 
     def add(self, *objs):
         for o in objs:
+            if getattr(o, "contained_objects", None):     # base.py:8-58
+                self.add(o.contained_objects)
             if isinstance(o, (list, tuple, set)):
                 self.add(*o)
             elif isinstance(o, (NeuronGroup, SpikeGeneratorGroup, PoissonGroup)):

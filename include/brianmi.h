@@ -448,6 +448,12 @@ int bm_mlp_get_params(bm_mlp *h, float *out);
 /* C[M][N] = op(A) op(B) with BF16-rounded operands (layout check of the MFMA tile) */
 int bm_mlp_gemm_test(int M, int N, int K, int ta, int tb, const float *A, const float *B, float *C);
 
+/* VanRossumMetric.get_distance (brian/monitor.py:1427-1486): X [n][T] holds
+ * n sampled traces (row-major); D [n][n] receives
+ * scale * sum_t (X[i][t] - X[j][t])^2 (scale = dt / tau), symmetric with a
+ * zero diagonal.  Host buffers; computed on the device. */
+int bm_pairwise_sq_distance(const double *X, int64_t n, int64_t T, double scale, double *D);
+
 /* Streaming copy of nbytes (reps launches) used to calibrate hardware byte
  * counters against known traffic; no counterpart in the reference. */
 int bm_probe_copy(int64_t nbytes, int reps, double *avg_ms);

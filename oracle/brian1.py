@@ -36,7 +36,7 @@ import scipy.linalg
 
 __all__ = [
     "linear_propagator", "numeric_propagator", "Group", "SpikeSource", "Net", "STPState",
-    "counter_uniform_u32", "poisson_threshold_u32",
+    "counter_uniform_u32", "poisson_threshold_u32", "van_rossum_distance",
 ]
 
 # --------------------------------------------------------------------------
@@ -144,6 +144,19 @@ class PoissonInput:
             k = binomial_inv(counter_u64(self.seed, np.arange(N), n), self.n, self.p)
         nz = k != 0
         g.S[self.var][nz] = g.S[self.var][nz] + weff * k[nz].astype(np.float64)
+
+
+def van_rossum_distance(values, dt, tau):
+    """VanRossumMetric.get_distance, numpy branch (monitor.py:1465-1484):
+    values[i] is neuron i's filtered train sampled every step; row i of the
+    matrix is dt/tau * sum((V_i - V_j)^2) over samples, returned as
+    tril(D) + tril(D).T."""
+    values = np.asarray(values, dtype=np.float64)
+    n = values.shape[0]
+    D = np.zeros((n, n))
+    for i in range(n):
+        D[i, :] = (dt / tau) * np.sum((values[i][None, :] - values) ** 2, axis=1)
+    return np.tril(D, k=0) + np.tril(D, k=0).T
 
 
 def poisson_threshold_u32(rate, dt):

@@ -474,3 +474,44 @@ def test_state_spike_monitor_interface():
         np.testing.assert_array_equal(M.times(i), S[i])
         np.testing.assert_array_equal(M.values("w", i), w[idx == i])
     defaultclock.dt = 0.1 * ms
+
+
+@pytest.mark.parametrize("n,T", [(1, 10), (5, 0), (64, 32), (131, 1001), (300, 4000)])
+def test_pairwise_distance_kernel(n, T):
+    """bm_pairwise_sq_distance against the oracle's van Rossum matrix on
+    random traces, ragged sizes included (tiles of 64 traces, 32 samples)."""
+    from brian_amd import _lib
+    from oracle import brian1 as O
+    rng = np.random.default_rng(n * 7 + T)
+    X = rng.standard_normal((n, T))
+    D = _lib.pairwise_sq_distance(X, 0.025)
+    W = O.van_rossum_distance(X, 1e-4, 4e-3)
+    np.testing.assert_allclose(D, W, rtol=1e-12, atol=1e-300)
+    np.testing.assert_array_equal(D, D.T)
+    np.testing.assert_array_equal(np.diag(D), 0.0)
+
+
+def test_van_rossum_metric():
+    """VanRossumMetric(source, tau) (monitor.py:1378-1486) on three trains
+    (one spike at 10 ms, one at 35 ms, none) over 300 ms: the filtered
+    traces are a^(t - s) from the spike step on, and the distances equal
+    the geometric-series closed forms."""
+    from brian_amd import SpikeGeneratorGroup, VanRossumMetric, run, ms, reinit_default_clock
+    reinit_default_clock()
+    dt, tau, T = 1e-4, 4e-3, 3000
+    a = np.exp(-dt / tau)
+    G = SpikeGeneratorGroup(3, [(0, 10 * ms), (1, 35 * ms)])
+    vr = VanRossumMetric(G, tau=4 * ms)
+    run(300 * ms)
+    assert vr.values.shape == (3, T)
+    s1, s2 = 100, 350
+    t = np.arange(T)
+    np.testing.assert_allclose(vr.values[0], np.where(t >= s1, a ** np.maximum(t - s1, 0.0), 0.0),
+                               rtol=1e-12, atol=1e-300)
+    D = vr.distance
+    k = dt / tau
+    g = s2 - s1
+    np.testing.assert_allclose(D[0, 2], k * (1 - a ** (2 * (T - s1))) / (1 - a * a), rtol=1e-10)
+    np.testing.assert_allclose(D[1, 2], k * (1 - a ** (2 * (T - s2))) / (1 - a * a), rtol=1e-10)
+    np.testing.assert_allclose(D[0, 1], k * ((1 - a ** (2 * g)) + (1 - a ** g) ** 2 * (1 - a ** (2 * (T - s2))))
+                               / (1 - a * a), rtol=1e-10)

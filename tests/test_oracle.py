@@ -632,3 +632,39 @@ def test_spike_states_closed_form():
         np.testing.assert_array_equal(vals[idx == i, 0], u[i])
         np.testing.assert_allclose(vals[idx == i, 1][0], Iin - (Iin - v0[i]) * a ** k1, rtol=1e-12)
         np.testing.assert_allclose(vals[idx == i, 1][1:], Iin * (1 - a ** K), rtol=1e-12)
+
+
+def _vr_trace(s, T, a):
+    """A train with one spike at step s filtered by dv/dt = -v/tau (exact
+    step factor a), sampled after each step: a^(t - s) from step s on."""
+    t = np.arange(T)
+    return np.where(t >= s, a ** np.maximum(t - s, 0).astype(np.float64), 0.0) if s is not None else np.zeros(T)
+
+
+def test_van_rossum_closed_forms():
+    """van_rossum_distance (monitor.py:1465-1484) against geometric series:
+    one spike at s against an empty train gives dt/tau (1 - a^(2(T-s))) /
+    (1 - a^2) (-> 1/2 as dt/tau -> 0, van Rossum's normalisation); spikes
+    at s1 < s2 give dt/tau [(1 - a^(2 g)) + (1 - a^g)^2 (1 - a^(2(T-s2)))] /
+    (1 - a^2), g = s2 - s1; symmetric, zero diagonal; and equal to a
+    brute-force double loop on random traces."""
+    dt, tau, T = 1e-4, 4e-3, 3000
+    a = np.exp(-dt / tau)
+    s1, s2 = 100, 350
+    V = np.stack([_vr_trace(s1, T, a), _vr_trace(s2, T, a), _vr_trace(None, T, a)])
+    D = O.van_rossum_distance(V, dt, tau)
+    k = dt / tau
+    d_empty1 = k * (1 - a ** (2 * (T - s1))) / (1 - a * a)
+    d_empty2 = k * (1 - a ** (2 * (T - s2))) / (1 - a * a)
+    g = s2 - s1
+    d12 = k * ((1 - a ** (2 * g)) + (1 - a ** g) ** 2 * (1 - a ** (2 * (T - s2)))) / (1 - a * a)
+    np.testing.assert_allclose(D[0, 2], d_empty1, rtol=1e-10)
+    np.testing.assert_allclose(D[1, 2], d_empty2, rtol=1e-10)
+    np.testing.assert_allclose(D[0, 1], d12, rtol=1e-10)
+    assert abs(d_empty1 - 0.5) < 0.02
+    np.testing.assert_array_equal(D, D.T)
+    np.testing.assert_array_equal(np.diag(D), 0.0)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((7, 40))
+    B = np.array([[0.25 * sum((X[i, t] - X[j, t]) ** 2 for t in range(40)) for j in range(7)] for i in range(7)])
+    np.testing.assert_allclose(O.van_rossum_distance(X, 1.0, 4.0), B, rtol=1e-13)
