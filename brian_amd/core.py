@@ -117,6 +117,8 @@ class Clock:
     def t(self):
         return self._n * self._dt + 0.0
 
+    _t = t
+
     @t.setter
     def t(self, v):
         self._n = int(float(v) / self._dt)
@@ -296,6 +298,9 @@ class NeuronGroup(_GroupBase):
     # ---- construction helpers
     def _var(self, v):
         if isinstance(v, str):
+            eqs = self._owner.__dict__.get("_eqs")
+            if eqs is not None and v in eqs._alias and v not in self.var_index:
+                v = eqs._alias[v]
             return self.var_index[v]
         return int(v)
 
@@ -409,6 +414,14 @@ class NeuronGroup(_GroupBase):
             return self._get_state(vi[name], self._origin, self.N)
         if name.endswith("_") and name[:-1] in vi:
             return self._get_state(vi[name[:-1]], self._origin, self.N)
+        if name == "rate" and isinstance(self._owner, PoissonGroup):
+            return self._owner.rates[self._origin:self._origin + self.N].copy()
+        eqs = self._owner.__dict__.get("_eqs")
+        if eqs is not None and name in eqs._static:
+            # aliases and static equations are computed from the state
+            state = {v: self._get_state(j, self._origin, self.N) for v, j in vi.items()}
+            val = eqs.static_value(name, state)
+            return np.broadcast_to(np.asarray(val, dtype=np.float64), (self.N,)).copy()
         raise AttributeError(name)
 
     def __setattr__(self, name, value):
@@ -418,14 +431,22 @@ class NeuronGroup(_GroupBase):
             if key in vi and isinstance(value, TimedArray):
                 if self._owner is not self:
                     raise NotImplementedError("TimedArray on a subgroup is not supported")
-                arr = value.arr
-                if arr.ndim == 2 and arr.shape[1] != self.N:
+                if value.ndim == 2 and value.shape[1] != self.N:
                     raise ValueError("TimedArray columns must match the group size")
-                object.__setattr__(self, "_timed", (vi[key], arr))
+                object.__setattr__(self, "_timed", (vi[key], value))
                 return
             if key in vi:
                 self._set_state(vi[key], self._origin, self.N, value)
                 return
+            if name == "rate" and isinstance(self._owner, PoissonGroup):
+                self._owner._set_rate(self._origin, self.N, value)
+                return
+            eqs = self._owner.__dict__.get("_eqs")
+            if eqs is not None and name in eqs._alias:
+                setattr(self, eqs._alias[name], value)
+                return
+            if eqs is not None and name in eqs._static:
+                raise ValueError(f"{name} is defined by a static equation and cannot be assigned")
         object.__setattr__(self, name, value)
 
     def rest(self):
@@ -627,6 +648,21 @@ class PoissonGroup(_GroupBase):
         self.var_index = {}
         self.var_names = []
 
+    # the rate state variable (directcontrol.py:430-435), also on subgroups
+    @property
+    def rate(self):
+        return self.rates.copy()
+
+    @rate.setter
+    def rate(self, value):
+        self._set_rate(0, self.N, value)
+
+    def _set_rate(self, off, n, value):
+        self.rates[off:off + n] = value
+        net = self._net
+        if net is not None and net.handle is not None:
+            net.handle.set_poisson_base_rates(net.gid[id(self)], self.rates)
+
 
 class PoissonInput(_Tracked):
     """brian/directcontrol.py:474-619: N independent Poisson inputs of rate
@@ -658,14 +694,121 @@ class PoissonInput(_Tracked):
         self.seed = int(np.random.randint(0, 2 ** 31)) if seed is None else int(seed)
 
 
-class TimedArray:
-    """Values per time step (brian/timedarray.py:15-234); assign as
-    ``G.I = TimedArray(values)`` or ``set_group_var_by_array``."""
+class TimedArray(np.ndarray):
+    """Values with a time each (brian/timedarray.py:15-234): the first index
+    is time; ``times`` or a clock (``clock=``, or ``start``/``dt``, else the
+    default clock) give the times.  ``ta(t)`` returns the value of the
+    sample at rint((t - t0)/dt), clipped to the array (clock-based arrays
+    only); slicing keeps the times.  Assigned to a group variable
+    (``G.I = ta``, ``set_group_var_by_array``) it is written into the
+    variable at the start of every step, on the device."""
+
+    def __new__(cls, arr, times=None, clock=None, start=None, dt=None):
+        return np.asarray(arr, dtype=np.float64).view(cls)
 
     def __init__(self, arr, times=None, clock=None, start=None, dt=None):
-        self.arr = np.asarray(arr, dtype=np.float64)
-        if times is not None or (start not in (None, 0)):
-            raise NotImplementedError("TimedArray with explicit times/start is not supported")
+        if start is not None or dt is not None:
+            if clock is not None:
+                raise ValueError("Specify start and dt or clock, but not both.")
+            clock = Clock(dt=dt if dt is not None else defaultclock.dt, t=start if start is not None else 0.0)
+        if times is not None and clock is not None:
+            raise ValueError("Specify times or clock but not both.")
+        self.guessed_clock = times is None and clock is None
+        if self.guessed_clock:
+            clock = defaultclock
+        self.clock = clock
+        if clock is not None:
+            self._t_init = int(clock._t / clock._dt) * clock._dt
+            self._dt = clock._dt
+            times = clock._t + np.arange(len(self)) * clock._dt
+        else:
+            self._t_init = self._dt = None
+        self.times = np.asarray(times, dtype=np.float64)
+
+    def __array_finalize__(self, orig):
+        for a in ("times", "clock", "_t_init", "_dt", "guessed_clock"):
+            if hasattr(orig, a):
+                setattr(self, a, getattr(orig, a))
+
+    def _with_times(self, x, times, keep_clock):
+        out = np.asarray(x).view(TimedArray)
+        times = np.atleast_1d(np.asarray(times, dtype=np.float64))
+        out.guessed_clock = False
+        if keep_clock and self.clock is not None and len(times) > 1:
+            out.clock = Clock(dt=times[1] - times[0], t=times[0])
+            out._t_init = int(out.clock._t / out.clock._dt) * out.clock._dt
+            out._dt = out.clock._dt
+        else:
+            out.clock, out._t_init, out._dt = None, None, None
+        out.times = times
+        return out
+
+    def __getitem__(self, item):
+        x = np.asarray(self)[item]
+        if isinstance(item, slice):
+            return self._with_times(x, self.times[item], True)
+        if isinstance(item, (int, np.integer)):
+            return self._with_times(x, self.times[item:item + 1], False)
+        if isinstance(item, tuple):
+            return self._with_times(x, self.times[item[0]], isinstance(item[0], slice))
+        return self._with_times(x, self.times[item], False)
+
+    def __call__(self, t):
+        if self.clock is None:
+            raise ValueError("Can only call timed arrays if they are based on a clock.")
+        if isinstance(t, (list, tuple)):
+            t = np.array(t)
+        if isinstance(t, np.ndarray):
+            if self.ndim > 2:
+                raise ValueError("Calling TimedArray with array valued t only supported for 1D or 2D TimedArray.")
+            if self.ndim == 2 and len(t) != self.shape[1]:
+                raise ValueError("Calling TimedArray with array valued t on 2D TimedArray requires "
+                                 "len(t)=arr.shape[1]")
+            k = np.array(np.rint((t - self._t_init) / self._dt), dtype=int)
+            k[k < 0] = 0
+            k[k >= len(self.times)] = len(self.times) - 1
+            a = np.asarray(self)
+            return a[k] if self.ndim == 1 else a[k, np.arange(len(k))]
+        k = int(np.rint((float(t) - self._t_init) / self._dt))
+        k = min(max(k, 0), len(self.times) - 1)
+        return np.asarray(self)[k]
+
+    @property
+    def arr(self):
+        return np.asarray(self)
+
+    def step_table(self, dt):
+        """Value of the TimedArraySetter (timedarray.py:283-303) at the start
+        of every step n = 0, 1, ... of a clock of step dt, up to the step
+        from which it no longer changes (later steps repeat the last row)."""
+        if self.clock is not None:
+            if not self.guessed_clock and (abs(self._dt - dt) > 1e-12 * dt or self.clock._t != 0.0):
+                raise NotImplementedError("a TimedArray on its own clock (other dt or start) is not "
+                                          "supported; resample it on the network's clock")
+            n_end = int(np.ceil((self._t_init + (len(self.times) - 0.5) * self._dt) / dt)) + 2
+            t = np.arange(max(n_end, 1)) * dt + 0.0
+            k = np.array(np.rint((t - self._t_init) / self._dt), dtype=int)
+        else:
+            # irregular times: the setter steps to the last sample whose
+            # time has been reached
+            n_end = int(np.ceil(float(self.times[-1]) / dt)) + 2 if len(self.times) else 1
+            t = np.arange(max(n_end, 1)) * dt + 0.0
+            k = np.searchsorted(self.times, t, side="right") - 1
+        k = np.clip(k, 0, len(self.times) - 1)
+        return np.asarray(self)[k]
+
+
+def set_group_var_by_array(group, var, arr, times=None, clock=None, start=None, dt=None):
+    """timedarray.py:305-317: write ``arr`` (a TimedArray, or values with
+    times/clock/start/dt) into variable ``var`` of ``group`` at the start of
+    every step."""
+    if not isinstance(arr, TimedArray) or times is not None or clock is not None or start is not None \
+            or dt is not None:
+        arr = TimedArray(np.asarray(arr), times=times, clock=clock, start=start, dt=dt)
+    setattr(group, var, arr)
+
+
+TimedArraySetter = set_group_var_by_array
 
 
 # ------------------------------------------------------------------ connections
@@ -2162,8 +2305,9 @@ class Network:
                 h.set_refractory(gid, int(np.max(val) / dt) + 1, np.max(val) - 0.5 * dt, per_neuron=val)
         ti = getattr(g, "_timed", None)
         if ti is not None:
-            var, arr = ti
-            h.set_timed_input(gid, var, arr, per_neuron=arr.ndim == 2)
+            var, ta = ti
+            tab = ta.step_table(dt)
+            h.set_timed_input(gid, var, tab, per_neuron=tab.ndim == 2)
 
     # ---- running
     def run(self, duration, report=None, report_period=10 * second, threads=1):

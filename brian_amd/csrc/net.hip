@@ -250,6 +250,14 @@ void require_unprepared(bm_net *net) {
 // The lean kernel covers: shared linear propagator, 1..4 variables, constant
 // or variable threshold, reset ops without refractoriness, at most one
 // (packed count) ring, no timed input.
+// A Poisson source spikes iff u32 < floor(p * 2^32), p = rate * dt, the
+// threshold clamped to [0, 2^32] so that p >= 1 spikes at every step as
+// rand() < p does (brian/directcontrol.py:436-447).
+uint64_t poisson_thr(double p) {
+  const double t = std::floor(p * 4294967296.0);
+  return (uint64_t)std::min(std::max(t, 0.0), 4294967296.0);
+}
+
 // a modulated Connection with a homogeneous delay reads the modulating
 // variable when it delivers (connection.py:242-247, :296)
 bool late_mod(const HostConn &c) { return c.mod_var >= 0 && c.delay_kind == 0 && c.hom_delay > 0; }
@@ -913,11 +921,7 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     }
     if (g.kind == GK_POISSON) {
       std::vector<uint64_t> thr(g.N);
-      for (int64_t i = 0; i < g.N; ++i) {
-        double t = std::floor(g.rates[i] * net->dt * 4294967296.0);
-        t = std::min(std::max(t, 0.0), 4294967295.0);
-        thr[i] = (uint64_t)t;
-      }
+      for (int64_t i = 0; i < g.N; ++i) thr[i] = poisson_thr(g.rates[i] * net->dt);
       upload_raw(g.pthr_d, thr.data(), thr.size(), net->stream);
       d.pthr = g.pthr_d.as<uint64_t>();
       d.seed = g.seed;
@@ -1426,10 +1430,7 @@ int bm_net_set_poisson_rates(bm_net *net, int gid, int64_t first_step, int64_t n
     const int64_t width = per_neuron ? g.N : 1;
     BM_REQUIRE(nsteps * width <= (int64_t(1) << 28), "rate table too large");
     std::vector<uint64_t> thr((size_t)(nsteps * width));
-    for (size_t q = 0; q < thr.size(); ++q) {     // the threshold of poisson_threshold (rate * dt * 2^32)
-      double t = std::floor(rates[q] * net->dt * 4294967296.0);
-      thr[q] = (uint64_t)std::min(std::max(t, 0.0), 4294967295.0);
-    }
+    for (size_t q = 0; q < thr.size(); ++q) thr[q] = poisson_thr(rates[q] * net->dt);
     if (net->prepared) BM_HIP(hipStreamSynchronize(net->stream));   // launches in flight read the old table
     upload_raw(g.ptab_d, thr.data(), thr.size(), net->stream);
     g.ptab_n0 = first_step;
@@ -1444,6 +1445,21 @@ int bm_net_set_poisson_rates(bm_net *net, int gid, int64_t first_step, int64_t n
       net->graph_exec = nullptr;
       net->graph_windows = 0;
     }
+  });
+}
+
+int bm_net_set_poisson_base_rates(bm_net *net, int gid, const double *rates) {
+  return guard([&] {
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.kind == GK_POISSON && rates, "not a Poisson group");
+    g.rates.assign(rates, rates + g.N);
+    if (!net->prepared) return;
+    std::vector<uint64_t> thr(g.N);
+    for (int64_t i = 0; i < g.N; ++i) thr[i] = poisson_thr(g.rates[i] * net->dt);
+    BM_HIP(hipStreamSynchronize(net->stream));      // launches in flight read the old thresholds
+    BM_REQUIRE(g.pthr_d.bytes == thr.size() * 8, "threshold buffer size");
+    BM_HIP(hipMemcpyAsync(g.pthr_d.p, thr.data(), thr.size() * 8, hipMemcpyHostToDevice, net->stream));
+    BM_HIP(hipStreamSynchronize(net->stream));
   });
 }
 

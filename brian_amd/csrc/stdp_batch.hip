@@ -291,8 +291,9 @@ int bm_stdp_batch_create(int precision, double dt, int ninst, int nin, int nvars
     d.wmax = wmax;
     d.dt = dt;
     d.seed = seed;
+    // 2^32 for rate * dt >= 1: every draw spikes, as rand() < 1 does
     double t = std::floor(rate * dt * 4294967296.0);
-    t = std::min(std::max(t, 0.0), 4294967295.0);
+    t = std::min(std::max(t, 0.0), 4294967296.0);
     d.thr = (uint64_t)t;
     const size_t nw = (size_t)ninst * nin;
     if (precision == 64) {

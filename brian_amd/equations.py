@@ -40,6 +40,10 @@ _LINE = re.compile(r"^\s*(?:d(?P<dvar>\w+)\s*/\s*dt\s*=\s*(?P<dexpr>.+?)|(?P<eva
                    r"|(?P<pvar>\w+))\s*:\s*(?P<unit>.+?)\s*$")
 
 
+# `x = y` with no unit: an alias of another variable (brian/equations.py)
+_ALIAS = re.compile(r"^\s*(?P<name>\w+)\s*=\s*(?P<target>\w+)\s*$")
+
+
 def _caller_namespace(level):
     frame = inspect.currentframe()
     try:
@@ -63,6 +67,7 @@ class Equations:
     def __init__(self, expr="", level=0, **kwds):
         self._diffeq = {}          # name -> expression string (ordered)
         self._static = {}          # name -> expression string
+        self._alias = {}           # name -> aliased variable (also in _static)
         self._params = []          # parameter names
         self._order = []           # declaration order of variables
         self._units = {}
@@ -73,6 +78,11 @@ class Equations:
         for raw in expr.split("\n"):
             line = raw.split("#", 1)[0].strip()
             if not line:
+                continue
+            a = _ALIAS.match(line)
+            if a:
+                self._alias[a.group("name")] = a.group("target")
+                self._static[a.group("name")] = a.group("target")
                 continue
             m = _LINE.match(line)
             if not m:
@@ -94,7 +104,7 @@ class Equations:
         out = Equations()
         out._namespace = dict(self._namespace)
         out._namespace.update(other._namespace)
-        for attr in ("_diffeq", "_static", "_units"):
+        for attr in ("_diffeq", "_static", "_alias", "_units"):
             d = dict(getattr(self, attr))
             d.update(getattr(other, attr))
             setattr(out, attr, d)
@@ -122,6 +132,23 @@ class Equations:
                 return cand
         volt = [n for n in names if n in self._diffeq and self._units.get(n) in ("volt", "V")]
         return volt[0] if len(volt) == 1 else None
+
+    def static_value(self, name, state):
+        """Value of a static equation or alias from the state arrays
+        ``state`` (name -> array), other statics substituted."""
+        expr = self._static[name]
+        for _ in range(len(self._static) + 1):
+            changed = False
+            for s, sexpr in self._static.items():
+                new = re.sub(r"\b" + s + r"\b", "(" + sexpr + ")", expr)
+                if new != expr:
+                    expr, changed = new, True
+            if not changed:
+                break
+        ns = dict(self._namespace)
+        ns.update(_FUNCS)
+        ns.update(state)
+        return eval(expr, {"__builtins__": {}}, ns)
 
     def substituted(self, name):
         """Diffeq right-hand side with static equations substituted as

@@ -119,6 +119,12 @@ int bm_net_set_generator_spikes(bm_net *net, int gid, int64_t nspikes, const int
  * u32(seed, i, n) < floor(rate_i*dt*2^32) (splitmix64 construction). */
 int bm_net_add_poisson(bm_net *net, int64_t N, const double *rates, uint64_t seed, int *gid);
 
+/* Poisson thresholds are clamped to [0, 2^32]: rate * dt >= 1 spikes at
+ * every step.  New constant rates (Hz, [N]) of a Poisson group, before or
+ * between runs (the rate state of brian/directcontrol.py:430-435; a table
+ * set with bm_net_set_poisson_rates still takes precedence over its steps). */
+int bm_net_set_poisson_base_rates(bm_net *net, int gid, const double *rates);
+
 /* Rates of a Poisson group for steps [first_step, first_step + nsteps)
  * (counted from the network's first step), one per step (per_neuron 0) or
  * [nsteps][N]: the role of PoissonGroup(N, rates=f(t)) evaluated each step

@@ -160,10 +160,12 @@ def van_rossum_distance(values, dt, tau):
 
 
 def poisson_threshold_u32(rate, dt):
-    """Spike iff u32 < floor(rate*dt*2^32) (clamped to 2^32-1)."""
+    """Spike iff u32 < floor(rate*dt*2^32), clamped to [0, 2^32]: a rate of
+    1/dt or more spikes at every step, as rand() < rate*dt does
+    (directcontrol.py:436-447)."""
     p = np.asarray(rate, dtype=np.float64) * dt
     t = np.floor(p * 4294967296.0)
-    return np.clip(t, 0, 4294967295.0).astype(np.uint64)
+    return np.clip(t, 0, 4294967296.0).astype(np.uint64)
 
 
 # --------------------------------------------------------------------------

@@ -79,3 +79,113 @@ def test_dense_structure_keeps_every_pair():
     assert len(steps) == 12 and (steps == 10).all()
     with pytest.raises(NotImplementedError):
         b.Connection(H, G, 'V', structure='banded')
+
+
+def test_variable_setting():
+    """brian/tests/testinterface/test_neurongroup.py:96-122: state variables
+    and parameters are assigned directly or through aliases; a static
+    equation reads from the state and refuses assignment."""
+    import brian_amd as b
+    b.reinit()
+    eqs = '''
+    dv/dt = -v / tau : 1
+    vsquared = v ** 2 : 1
+    v_alias = v
+    tau : second
+    tau_alias = tau
+    '''
+    G = b.NeuronGroup(1, model=eqs)
+    G.v = 0.25
+    assert G.v == 0.25 and G.v_alias == 0.25
+    G.v_alias = 0.5
+    assert G.v == 0.5 and G.v_alias == 0.5
+    G.tau = 5 * b.ms
+    assert G.tau == 5 * b.ms and G.tau_alias == 5 * b.ms
+    G.tau_alias = 50 * b.ms
+    assert G.tau == 50 * b.ms and G.tau_alias == 50 * b.ms
+    assert G.vsquared == 0.25
+    with pytest.raises(ValueError):
+        G.vsquared = 2
+
+
+def test_poissongroup_rate_state():
+    """test_neurongroup.py:5-18: the rate of a PoissonGroup is set through
+    subgroups and read back from the group and the subgroups."""
+    import brian_amd as b
+    b.reinit()
+    G = b.PoissonGroup(10, rates=10 * b.Hz)
+    G1 = G[:5]
+    G2 = G[5:]
+    G1.rate = 5 * b.Hz
+    G2.rate = 20 * b.Hz
+    assert abs(G.rate[0] - 5.0) < 1e-12
+    assert abs(G1.rate[0] - 5.0) < 1e-12
+    assert abs(G2.rate[0] - 20.0) < 1e-12
+
+
+def test_timedarray_construction_and_access():
+    """brian/tests/testinterface/test_timedarray.py:4-87: clocks, start/dt
+    and explicit times; values at and between the sample times; slicing
+    keeps the times; calls with arrays of t, 1-D and 2-D."""
+    import brian_amd as b
+    b.reinit_default_clock()
+    ar = np.array([0, 2, 4, 6])
+
+    def assert_values(ta, ar, start, dt):
+        for index, val in enumerate(ar):
+            assert ta(start + dt * index) == val
+        for index, val in enumerate(ar[:-1]):
+            assert ta(start + dt * (index + 0.4)) == val
+
+    ta = b.TimedArray(ar)
+    assert ta.clock == b.defaultclock
+    assert_values(ta, ar, 0.0, b.defaultclock.dt)
+    clock = b.Clock(dt=0.25 * b.ms)
+    ta = b.TimedArray(ar, clock=clock)
+    assert ta.clock == clock
+    assert_values(ta, ar, 0.0, clock.dt)
+    ta = b.TimedArray(ar, dt=0.5 * b.ms)
+    assert_values(ta, ar, 0.0, 0.5 * b.ms)
+    ta = b.TimedArray(ar, start=1 * b.ms, dt=0.5 * b.ms)
+    assert_values(ta, ar, 1 * b.ms, 0.5 * b.ms)
+    with pytest.raises(ValueError):
+        b.TimedArray(ar, start=1 * b.ms, dt=0.5 * b.ms, clock=b.Clock())
+    times = np.arange(len(ar)) * 0.25 * b.ms
+    b.TimedArray(ar, times=times)
+    with pytest.raises(ValueError):
+        b.TimedArray(ar, times=times, clock=b.Clock())
+
+    b.reinit_default_clock()
+    ta = b.TimedArray(ar)
+    assert ta[1] == np.array(ar[1])
+    assert (ta[:] == ar[:]).all() and (ta.times == ta[:].times).all()
+    assert (ta[:2] == ar[:2]).all()
+    assert (ta[:2].times == ta.times[:2]).all()
+    index_array = np.array([0, 2])
+    assert (ta[index_array] == ar[index_array]).all()
+    assert (ta[index_array].times == ta.times[index_array]).all()
+    assert (ta(np.arange(len(ar)) * b.defaultclock.dt) == ar).all()
+    assert ta(1 * b.second) == ar[-1]
+    ar2 = np.array([[0, 1], [2, 3], [4, 5], [6, 7]])
+    ta = b.TimedArray(ar2)
+    assert len(ta(0.0)) == 2
+    assert (ta(1 * b.second) == ar2[-1, :]).all()
+    assert (ta(np.array([0.0, 2 * b.defaultclock.dt])) == np.array([ar2[0, 0], ar2[2, 1]])).all()
+
+
+def test_timedarray_step_tables():
+    """The setter's value at the start of each step (timedarray.py:283-303):
+    clock-based arrays on the network clock index by rint, irregular times
+    hold the last sample reached; the table ends once it stops changing."""
+    import brian_amd as b
+    b.reinit_default_clock()
+    dt = b.defaultclock.dt
+    ta = b.TimedArray(np.arange(5) * 3.0)
+    tab = ta.step_table(dt)
+    np.testing.assert_array_equal(tab[:5], np.arange(5) * 3.0)
+    assert (tab[5:] == 12.0).all()
+    ti = b.TimedArray(np.array([1.0, 5.0]), times=np.array([0.0, 3 * b.ms]))
+    tab = ti.step_table(dt)
+    assert (tab[:30] == 1.0).all() and (tab[30:] == 5.0).all() and len(tab) >= 31
+    with pytest.raises(NotImplementedError):
+        b.TimedArray(np.arange(4.0), dt=0.5 * b.ms).step_table(dt)

@@ -652,3 +652,73 @@ def test_poisson_group_rates_in_time():
         np.testing.assert_array_equal(got[order], want)
         if N == 200:
             assert want[:, 0].max() < 2000 and want[:, 0].max() > 1900
+
+
+def test_poissongroup_deterministic_rates():
+    """test_neurongroup.py:20-38: rates 0 and 1/dt are the only
+    deterministic ones (never / every step), as constants and as a rate
+    function; a rate changed between runs takes effect in the next run."""
+    from brian_amd import PoissonGroup, SpikeCounter, run, ms, Hz, defaultclock, reinit_default_clock
+    reinit_default_clock()
+    G = PoissonGroup(2, rates=np.array([0.0, 1.0 / defaultclock.dt]))
+    counter = SpikeCounter(G)
+    runtime = 10 * ms
+    run(runtime)
+    assert counter.count[0] == 0
+    assert counter.count[1] == round(runtime / defaultclock.dt)
+
+    def rate_fun(t):
+        return np.array([0.0, 1.0 / defaultclock.dt])
+
+    reinit_default_clock()
+    G = PoissonGroup(2, rates=rate_fun)
+    counter = SpikeCounter(G)
+    run(runtime)
+    assert counter.count[0] == 0
+    assert counter.count[1] == round(runtime / defaultclock.dt)
+
+    reinit_default_clock()
+    G = PoissonGroup(2, rates=0 * Hz)
+    counter = SpikeCounter(G)
+    from brian_amd import Network
+    net = Network(G, counter)
+    net.run(runtime)
+    assert counter.count.sum() == 0
+    G[1:].rate = 1.0 / defaultclock.dt
+    net.run(runtime)
+    assert counter.count[0] == 0 and counter.count[1] == round(runtime / defaultclock.dt)
+
+
+def test_timedarray_neurongroup_integration():
+    """test_timedarray.py:90-132: a TimedArray assigned to a parameter is
+    seen by a StateMonitor at every step: on the default clock, with
+    explicit regular times, and with irregular times (value held from each
+    sample time on)."""
+    from brian_amd import NeuronGroup, StateMonitor, TimedArray, run, ms, defaultclock, reinit_default_clock
+    reinit_default_clock()
+    duration = 5 * ms
+    ar = np.arange(0, int(duration / defaultclock.dt)) * 3
+    ta = TimedArray(ar)
+    G = NeuronGroup(1, model='p : 1')
+    G.p = ta
+    mon = StateMonitor(G, 'p', record=0)
+    run(duration)
+    assert (mon[0] == ar).all()
+
+    reinit_default_clock()
+    ta = TimedArray(ar, times=np.arange(int(duration / defaultclock.dt)) * defaultclock.dt)
+    G = NeuronGroup(1, model='p : 1')
+    G.p = ta
+    mon = StateMonitor(G, 'p', record=0)
+    run(duration)
+    assert (mon[0] == ar).all()
+
+    reinit_default_clock()
+    ar = np.array([1, 5])
+    ta = TimedArray(ar, times=np.array([0 * ms, 3 * ms]))
+    G = NeuronGroup(1, model='p : 1')
+    G.p = ta
+    mon = StateMonitor(G, 'p', record=0)
+    run(duration)
+    assert (mon[0][mon.times < 3 * ms] == 1).all()
+    assert (mon[0][mon.times >= 3 * ms] == 5).all()
