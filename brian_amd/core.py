@@ -11,6 +11,7 @@ demand.
 """
 from __future__ import annotations
 
+import gc
 import inspect
 import re
 import warnings
@@ -24,7 +25,8 @@ from .equations import Equations
 from .units import ms, mV, second
 
 __all__ = [
-    "Clock", "defaultclock", "reinit_default_clock", "get_default_clock",
+    "Clock", "EventClock", "defaultclock", "reinit_default_clock", "get_default_clock", "define_default_clock",
+    "guess_clock",
     "Threshold", "VariableThreshold", "EmpiricalThreshold", "NoThreshold", "StringThreshold",
     "Reset", "VariableReset", "StringReset", "Refractoriness", "NoReset",
     "NeuronGroup", "SpikeGeneratorGroup", "PulsePacket", "PoissonGroup", "PoissonInput", "TimedArray",
@@ -98,12 +100,30 @@ def recall(*objs):
 
 # ------------------------------------------------------------------ clock
 class Clock:
-    """Integer-tick clock, t = n*dt (brian/clock.py:52-224)."""
+    """Integer-tick clock, t = n*dt (brian/clock.py:52-224).  A clock
+    remembers the frame that created it, so that ``guess_clock`` can pick
+    the clock defined in the calling scope (clock.py:337-357); clocks made
+    inside this module are never guessed."""
+    _all = []
 
     def __init__(self, dt=0.1 * ms, t=0 * ms, order=0, makedefaultclock=False):
         self._dt = float(dt)
         self._n = int(float(t) / self._dt)
         self.order = order
+        frame = inspect.currentframe()
+        try:
+            f = frame.f_back
+            while f is not None and f.f_code.co_name == "__init__" and isinstance(f.f_locals.get("self"), Clock):
+                f = f.f_back          # a subclass initialiser
+            self._frame_id = None if f is None or f.f_code.co_filename == __file__ else id(f)
+        finally:
+            del frame
+        Clock._all.append(weakref.ref(self))
+        if makedefaultclock:
+            _PREFS["defaultclock"] = self
+
+    def tick(self):
+        self._n += 1
 
     @property
     def dt(self):
@@ -127,15 +147,63 @@ class Clock:
         self._n = int(float(t) / self._dt)
 
 
+class EventClock(Clock):
+    """clock.py:360-370: a clock that is never guessed."""
+
+
 defaultclock = Clock()
+_PREFS["defaultclock"] = defaultclock
 
 
 def get_default_clock():
-    return defaultclock
+    """clock.py:406-410: the clock of the 'defaultclock' preference."""
+    return _PREFS["defaultclock"]
+
+
+def define_default_clock(**kwds):
+    """clock.py:385-397: a new default clock from Clock's arguments."""
+    kwds["makedefaultclock"] = True
+    return Clock(**kwds)
 
 
 def reinit_default_clock(t=0 * ms):
-    defaultclock.reinit(t)
+    get_default_clock().reinit(t)
+
+
+def guess_clock(clock=None):
+    """clock.py:337-357: the clock passed, else the clock created in the
+    innermost calling scope that created any (TypeError if that scope
+    created several), else the default clock."""
+    if clock:
+        return clock
+
+    def scan(collect):
+        if collect:
+            # clocks held only by dead networks (group <-> network cycles)
+            # are not in scope any more
+            gc.collect()
+        Clock._all[:] = [r for r in Clock._all if r() is not None]
+        live = [c for c in (r() for r in Clock._all)
+                if c is not None and c._frame_id is not None and not isinstance(c, EventClock)]
+        frame = inspect.currentframe().f_back.f_back
+        try:
+            f = frame
+            while f is not None:
+                if f.f_code.co_filename != __file__:
+                    found = [c for c in live if c._frame_id == id(f)]
+                    if found:
+                        return found
+                f = f.f_back
+        finally:
+            del frame
+        return []
+
+    found = scan(False)
+    if found:
+        found = scan(True)
+    if len(found) > 1:
+        raise TypeError("Clock is ambiguous. Please specify it explicitly.")
+    return found[0] if found else get_default_clock()
 
 
 # ------------------------------------------------------------------ threshold / reset
@@ -161,7 +229,7 @@ class EmpiricalThreshold:
     def __init__(self, threshold=1 * mV, refractory=1 * ms, state=0, clock=None):
         self.threshold = threshold
         self.state = state
-        clock = clock or defaultclock
+        clock = guess_clock(clock)
         self.refractory = int(refractory / clock.dt)
         if abs(self.refractory * clock.dt - refractory) <= 1e-8 * abs(refractory) and self.refractory > 0:
             self.refractory -= 1
@@ -271,7 +339,7 @@ class NeuronGroup(_GroupBase):
         self._owner = self
         self._origin = 0
         self._next_sub = 0
-        self.clock = clock or defaultclock
+        self.clock = guess_clock(clock)
         if isinstance(model, str):
             model = Equations(model, level=level + 1)
         self._eqs = model
@@ -490,7 +558,7 @@ class SpikeGeneratorGroup(_GroupBase):
         self._owner = self
         self._origin = 0
         self._next_sub = 0
-        self.clock = clock or defaultclock
+        self.clock = guess_clock(clock)
         self.period = None if period is None else float(period)
         self._net = None
         self.var_index = {}
@@ -638,7 +706,7 @@ class PoissonGroup(_GroupBase):
         self._owner = self
         self._origin = 0
         self._next_sub = 0
-        self.clock = clock or defaultclock
+        self.clock = guess_clock(clock)
         self._rate_fn = rates if callable(rates) else None
         if self._rate_fn is not None:
             rates = 0.0
@@ -710,12 +778,13 @@ class TimedArray(np.ndarray):
         if start is not None or dt is not None:
             if clock is not None:
                 raise ValueError("Specify start and dt or clock, but not both.")
-            clock = Clock(dt=dt if dt is not None else defaultclock.dt, t=start if start is not None else 0.0)
+            clock = Clock(dt=dt if dt is not None else get_default_clock().dt,
+                          t=start if start is not None else 0.0)
         if times is not None and clock is not None:
             raise ValueError("Specify times or clock but not both.")
         self.guessed_clock = times is None and clock is None
         if self.guessed_clock:
-            clock = defaultclock
+            clock = guess_clock()
         self.clock = clock
         if clock is not None:
             self._t_init = int(clock._t / clock._dt) * clock._dt
@@ -1982,7 +2051,7 @@ class NetworkOperation(_Tracked):
         self._track(1)
         self.function = function
         self.when = when
-        self.clock = clock or defaultclock
+        self.clock = guess_clock(clock)
         self._has_arg = _argcount(function) == 1
 
     def __call__(self):
@@ -2139,7 +2208,7 @@ class Network:
         clocks = {id(g.clock): g.clock for g in owners}
         if len(clocks) > 1:
             raise NotImplementedError("all objects must share one clock")
-        return next(iter(clocks.values())) if clocks else defaultclock
+        return next(iter(clocks.values())) if clocks else get_default_clock()
 
     # ---- compilation
     def _compile(self):
@@ -2489,8 +2558,8 @@ class Network:
         for g in self.groups:
             if isinstance(g, SpikeGeneratorGroup):
                 g.reinit()
-        getattr(self, "clock", defaultclock).reinit()
-        defaultclock.reinit()
+        getattr(self, "clock", get_default_clock()).reinit()
+        get_default_clock().reinit()
 
     def stop(self):
         _StopFlag.stop = True

@@ -123,8 +123,8 @@ def test_poissongroup_rate_state():
     assert abs(G2.rate[0] - 20.0) < 1e-12
 
 
-def test_timedarray_construction_and_access():
-    """brian/tests/testinterface/test_timedarray.py:4-87: clocks, start/dt
+def test_timedarray_construction():
+    """brian/tests/testinterface/test_timedarray.py:4-47: clocks, start/dt
     and explicit times; values at and between the sample times; slicing
     keeps the times; calls with arrays of t, 1-D and 2-D."""
     import brian_amd as b
@@ -155,7 +155,13 @@ def test_timedarray_construction_and_access():
     with pytest.raises(ValueError):
         b.TimedArray(ar, times=times, clock=b.Clock())
 
+
+def test_timedarray_access():
+    """test_timedarray.py:50-87 (its own function: a clock created in the
+    calling scope would be the guessed clock)."""
+    import brian_amd as b
     b.reinit_default_clock()
+    ar = np.array([0, 2, 4, 6])
     ta = b.TimedArray(ar)
     assert ta[1] == np.array(ar[1])
     assert (ta[:] == ar[:]).all() and (ta.times == ta[:].times).all()
@@ -189,3 +195,69 @@ def test_timedarray_step_tables():
     assert (tab[:30] == 1.0).all() and (tab[30:] == 5.0).all() and len(tab) >= 31
     with pytest.raises(NotImplementedError):
         b.TimedArray(np.arange(4.0), dt=0.5 * b.ms).step_table(dt)
+
+
+def test_clock_interface():
+    """brian/tests/testinterface/test_clock.py:4-132: construction, the
+    default clock and its redefinition, tick and reinit, and guess_clock's
+    choice (the clock passed, the innermost scope's clock, the default;
+    TypeError when a scope made two)."""
+    from brian_amd import (Clock, defaultclock, guess_clock, get_global_preference, define_default_clock,
+                           get_default_clock, reinit_default_clock, set_global_preferences, msecond, second)
+    reinit_default_clock()
+    try:
+        assert defaultclock.t < 0.001 * msecond
+        c = guess_clock()
+        assert c.t < 0.001 * msecond
+        c = Clock()
+        c = Clock(t=1 * second)
+        assert c.t > 0.9 * second
+        c = Clock(dt=10 * msecond)
+        assert c.dt > 9 * msecond
+        c = Clock(t=2 * second, dt=1 * msecond)
+        assert c.t > 1.9 * second
+        assert 0.5 * msecond < c.dt < 2 * msecond
+        assert get_global_preference('defaultclock').dt < 9 * msecond
+        c = Clock(dt=10 * msecond, makedefaultclock=True)
+        assert get_global_preference('defaultclock').dt > 9 * msecond
+        define_default_clock(dt=3 * msecond)
+        assert 2.9 * msecond < get_global_preference('defaultclock').dt < 3.1 * msecond
+        assert 2.9 * msecond < get_default_clock().dt < 3.1 * msecond
+        c = Clock(10 * msecond, 1 * second)
+        assert c.t > 0.9 * second
+        assert c.dt > 9 * msecond
+        c = Clock(t=1 * second)
+        c.reinit()
+        assert c.t < 0.001 * second
+        c = Clock(t=1 * second, dt=1 * msecond)
+        for i in range(10):
+            c.tick()
+        assert c.t > 9 * msecond
+        for i in range(10):
+            get_default_clock().tick()
+        reinit_default_clock()
+        assert get_default_clock().t < 0.0001 * second
+        assert 0.6 * second < guess_clock(Clock(t=0.7 * second)).t < 0.8 * second
+        c = Clock()
+        assert guess_clock() is c
+        del c
+        import gc
+        gc.collect()
+        assert guess_clock() is get_default_clock()
+        c = Clock()
+        d = Clock()
+        with pytest.raises(TypeError):
+            guess_clock()
+        del d
+        gc.collect()
+        c = Clock()
+
+        def f():
+            d = Clock()
+            assert guess_clock() is d
+            del d
+            gc.collect()
+            return guess_clock()
+        assert f() is c
+    finally:
+        set_global_preferences(defaultclock=defaultclock)
