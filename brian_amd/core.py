@@ -26,7 +26,7 @@ from .units import ms, mV, second
 
 __all__ = [
     "Clock", "EventClock", "defaultclock", "reinit_default_clock", "get_default_clock", "define_default_clock",
-    "guess_clock",
+    "guess_clock", "linked_var", "LinkedVar",
     "Threshold", "VariableThreshold", "EmpiricalThreshold", "NoThreshold", "StringThreshold",
     "Reset", "VariableReset", "StringReset", "Refractoriness", "NoReset",
     "NeuronGroup", "SpikeGeneratorGroup", "PulsePacket", "PoissonGroup", "PoissonInput", "TimedArray",
@@ -496,6 +496,12 @@ class NeuronGroup(_GroupBase):
         vi = self.__dict__.get("var_index")
         if vi is not None and not name.startswith("_"):
             key = name[:-1] if name.endswith("_") and name[:-1] in vi else name
+            if key in vi and isinstance(value, LinkedVar):
+                self.link_var(key, value.source, value.var, value.func, value.when, value.clock)
+                return
+            eqs0 = self._owner.__dict__.get("_eqs")
+            if isinstance(value, LinkedVar) and eqs0 is not None and name in eqs0._static:
+                raise ValueError("Cannot set a static variable (equation) with a linked variable.")
             if key in vi and isinstance(value, TimedArray):
                 if self._owner is not self:
                     raise NotImplementedError("TimedArray on a subgroup is not supported")
@@ -516,6 +522,19 @@ class NeuronGroup(_GroupBase):
             if eqs is not None and name in eqs._static:
                 raise ValueError(f"{name} is defined by a static equation and cannot be assigned")
         object.__setattr__(self, name, value)
+
+    def link_var(self, var, source, sourcevar, func=None, when="start", clock=None):
+        """neurongroup.py:702-729: a network operation copying
+        source.sourcevar (through func) into this group's ``var``."""
+        func = func if func is not None else (lambda x: x)
+        name = sourcevar if isinstance(sourcevar, str) else source.var_names[int(sourcevar)]
+        target = self
+
+        def update_link_var():
+            setattr(target, var, func(np.asarray(getattr(source, name))))
+        op = NetworkOperation(update_link_var, clock=clock or self.clock, when=when)
+        self._owner.__dict__.setdefault("contained_objects", []).append(op)
+        return op
 
     def rest(self):
         A, C = self._eqs.linear_propagator(self.clock.dt)
@@ -2074,6 +2093,25 @@ def network_operation(*args, **kw):
         op._track(1)
         return op
     return deco
+
+
+class LinkedVar:
+    """neurongroup.py:92-98: the value assigned to link a variable."""
+
+    def __init__(self, source, var=0, func=None, when="start", clock=None):
+        self.source = source
+        self.var = var
+        self.func = func
+        self.when = when
+        self.clock = clock
+
+
+def linked_var(source, var=0, func=None, when="start", clock=None):
+    """neurongroup.py:100-140: ``G.V = linked_var(H, 'W')`` copies H.W
+    (through ``func`` if given) into G.V at ``when`` of every step.  The copy
+    is a network operation, so such networks run one step per launch; a
+    'middle' copy runs after the whole step."""
+    return LinkedVar(source, var, func, when, clock)
 
 
 class _Reporter:

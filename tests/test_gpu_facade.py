@@ -722,3 +722,27 @@ def test_timedarray_neurongroup_integration():
     run(duration)
     assert (mon[0][mon.times < 3 * ms] == 1).all()
     assert (mon[0][mon.times >= 3 * ms] == 5).all()
+
+
+def test_linked_var():
+    """test_neurongroup.py:41-93: a linked variable is copied at the start
+    of each step (one step behind the source), after the update with
+    when='middle' (equal to it), and through a function."""
+    import gc
+    from brian_amd import NeuronGroup, StateMonitor, linked_var, run, ms, reinit_default_clock, reinit
+    for when, func, check in (("start", None, lambda a, b: np.sum(np.abs(a[:-1] - b[1:])) == 0),
+                              ("middle", None, lambda a, b: np.sum(np.abs(a - b)) == 0),
+                              ("middle", lambda x: x * 2, lambda a, b: np.sum(np.abs(2 * a - b)) == 0)):
+        reinit()
+        reinit_default_clock()
+        G1 = G2 = mon1 = mon2 = None
+        gc.collect()                  # the previous case's objects leave the scope
+        G1 = NeuronGroup(1, model='dv/dt = -v/(10*ms) : 1')
+        G2 = NeuronGroup(1, model='v: 1')
+        G2.v = linked_var(G1, 'v', when=when, func=func)
+        G1.v = 1
+        mon1 = StateMonitor(G1, 'v', record=True)
+        mon2 = StateMonitor(G2, 'v', record=True)
+        run(50 * ms)
+        assert len(mon1[0]) == 500
+        assert check(mon1[0], mon2[0]), when
