@@ -26,7 +26,7 @@ from .units import ms, mV, second
 
 __all__ = [
     "Clock", "EventClock", "defaultclock", "reinit_default_clock", "get_default_clock", "define_default_clock",
-    "guess_clock", "linked_var", "LinkedVar",
+    "guess_clock", "linked_var", "LinkedVar", "LazyStateUpdater",
     "Threshold", "VariableThreshold", "EmpiricalThreshold", "NoThreshold", "StringThreshold",
     "Reset", "VariableReset", "StringReset", "Refractoriness", "NoReset",
     "NeuronGroup", "SpikeGeneratorGroup", "PulsePacket", "PoissonGroup", "PoissonInput", "TimedArray",
@@ -282,6 +282,14 @@ class NoReset:
     pass
 
 
+class LazyStateUpdater:
+    """stateupdater.py:680-705: ``numstates`` variables with no dynamics."""
+
+    def __init__(self, numstates=1, clock=None):
+        self.numstates = int(numstates)
+        self.clock = clock
+
+
 def _caller_ns(level):
     frame = inspect.currentframe()
     try:
@@ -342,6 +350,9 @@ class NeuronGroup(_GroupBase):
         self.clock = guess_clock(clock)
         if isinstance(model, str):
             model = Equations(model, level=level + 1)
+        elif isinstance(model, LazyStateUpdater):
+            # state variables that never change (stateupdater.py:680-705)
+            model = Equations("\n".join(f"lazy{k} : 1" for k in range(model.numstates)))
         self._eqs = model
         self.var_names = model.variables() if model is not None else []
         self.var_index = {n: j for j, n in enumerate(self.var_names)}

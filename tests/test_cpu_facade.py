@@ -261,3 +261,20 @@ def test_clock_interface():
         assert f() is c
     finally:
         set_global_preferences(defaultclock=defaultclock)
+
+
+def test_network_clocks():
+    """brian/tests/testinterface/test_network.py:148-170: same_clocks for
+    groups on the default clock, on one clock, and on two clocks."""
+    import brian_amd as b
+    G1 = b.NeuronGroup(42, model=b.LazyStateUpdater())
+    G2 = b.NeuronGroup(42, model=b.LazyStateUpdater())
+    assert b.Network(G1, G2).same_clocks()
+    clock = b.Clock(dt=0.333 * b.ms)
+    G1 = b.NeuronGroup(42, model=b.LazyStateUpdater(), clock=clock)
+    G2 = b.NeuronGroup(42, model=b.LazyStateUpdater(), clock=clock)
+    assert b.Network(G1, G2).same_clocks()
+    clock2 = b.Clock(dt=0.5 * b.ms)
+    G1 = b.NeuronGroup(42, model=b.LazyStateUpdater(), clock=clock)
+    G2 = b.NeuronGroup(42, model=b.LazyStateUpdater(), clock=clock2)
+    assert not b.Network(G1, G2).same_clocks()

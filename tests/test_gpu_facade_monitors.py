@@ -520,3 +520,27 @@ def test_van_rossum_metric():
     np.testing.assert_allclose(D[1, 2], k * (1 - a ** (2 * (T - s2))) / (1 - a * a), rtol=1e-10)
     np.testing.assert_allclose(D[0, 1], k * ((1 - a ** (2 * g)) + (1 - a ** g) ** 2 * (1 - a ** (2 * (T - s2))))
                                / (1 - a * a), rtol=1e-10)
+
+
+def test_counter():
+    """test_monitor.py:239-269: SpikeCounter, PopulationSpikeCounter and
+    SpikeMonitor count the same spikes, per neuron too, and reinit clears
+    all three."""
+    from brian_amd import (SpikeGeneratorGroup, SpikeMonitor, SpikeCounter, PopulationSpikeCounter, Network, ms,
+                           defaultclock, reinit_default_clock)
+    reinit_default_clock()
+    spikes = [(0, 3 * ms), (1, 4 * ms), (0, 7 * ms)]
+    G = SpikeGeneratorGroup(2, spikes, clock=defaultclock)
+    M = SpikeMonitor(G)
+    C = SpikeCounter(G)
+    P = PopulationSpikeCounter(G)
+    net = Network(G, M, C, P)
+    net.run(10 * ms)
+    assert M.nspikes == C.nspikes == P.nspikes == 3
+    assert len(M[0]) == C[0] == C.count[0] == 2
+    assert len(M[1]) == C[1] == C.count[1] == 1
+    net.reinit()
+    assert M.nspikes == C.nspikes == P.nspikes == 0
+    assert len(M[0]) == C[0] == C.count[0] == 0
+    assert len(M[1]) == C[1] == C.count[1] == 0
+    reinit_default_clock()
