@@ -26,7 +26,9 @@ from .units import ms, mV, second
 
 __all__ = [
     "Clock", "EventClock", "defaultclock", "reinit_default_clock", "get_default_clock", "define_default_clock",
-    "guess_clock", "linked_var", "LinkedVar", "LazyStateUpdater",
+    "guess_clock", "linked_var", "LinkedVar", "LazyStateUpdater", "FunThreshold", "SimpleFunThreshold",
+    "PoissonThreshold", "HomogeneousPoissonThreshold", "FunReset", "CustomRefractoriness",
+    "SimpleCustomRefractoriness",
     "Threshold", "VariableThreshold", "EmpiricalThreshold", "NoThreshold", "StringThreshold",
     "Reset", "VariableReset", "StringReset", "Refractoriness", "NoReset",
     "NeuronGroup", "SpikeGeneratorGroup", "PulsePacket", "PoissonGroup", "PoissonInput", "TimedArray",
@@ -282,6 +284,56 @@ class NoReset:
     pass
 
 
+class _HostRule:
+    """Thresholds and resets given as Python functions or random draws of
+    the reference run per step on the host there; the device sweep has no
+    host call inside a step, so a group built with one raises
+    NotImplementedError."""
+
+
+class FunThreshold(_HostRule):
+    """threshold.py:266-299: spikes = thresholdfun(*state variables)."""
+
+    def __init__(self, thresholdfun, state=0):
+        self.thresholdfun = thresholdfun
+        self.state = state
+
+
+class SimpleFunThreshold(FunThreshold):
+    """threshold.py:301-330: spikes = thresholdfun(state variable)."""
+
+
+class PoissonThreshold(_HostRule):
+    """threshold.py:461-480: spike with probability state * dt per step."""
+
+    def __init__(self, state=0):
+        self.state = state
+
+
+class HomogeneousPoissonThreshold(PoissonThreshold):
+    """threshold.py:482-500: one rate for the whole group."""
+
+
+class FunReset(_HostRule):
+    """reset.py:260-286: resetfun(group, spikes)."""
+
+    def __init__(self, resetfun):
+        self.resetfun = resetfun
+
+
+class CustomRefractoriness(_HostRule):
+    """reset.py:407-460: resetfun(group, spikes) and refracfunc during the period."""
+
+    def __init__(self, resetfun, period=5 * ms, refracfunc=None):
+        self.resetfun = resetfun
+        self.period = period
+        self.refracfunc = refracfunc if refracfunc is not None else resetfun
+
+
+class SimpleCustomRefractoriness(CustomRefractoriness):
+    """reset.py:338-405: one function for the reset and the refractory period."""
+
+
 class LazyStateUpdater:
     """stateupdater.py:680-705: ``numstates`` variables with no dynamics."""
 
@@ -386,6 +438,9 @@ class NeuronGroup(_GroupBase):
     def _make_threshold(self, th, level):
         if th is None or isinstance(th, NoThreshold):
             return None
+        if isinstance(th, _HostRule):
+            raise NotImplementedError(f"{type(th).__name__} runs a host function or draw per step; "
+                                      "use a threshold on a state variable")
         if isinstance(th, (int, float, np.floating)):
             return ("const", 0, float(th))
         if isinstance(th, Threshold):
@@ -413,6 +468,9 @@ class NeuronGroup(_GroupBase):
         refr = None
         if isinstance(reset, NoReset) or reset is None:
             return [], None
+        if isinstance(reset, _HostRule):
+            raise NotImplementedError(f"{type(reset).__name__} runs a host function per step; "
+                                      "use Reset, VariableReset, StringReset or Refractoriness")
         if isinstance(reset, Refractoriness):
             var = self._var(reset.state)
             ops = [(var, 0, float(reset.resetvalue), -1)]

@@ -278,3 +278,53 @@ def test_network_clocks():
     G1 = b.NeuronGroup(42, model=b.LazyStateUpdater(), clock=clock)
     G2 = b.NeuronGroup(42, model=b.LazyStateUpdater(), clock=clock2)
     assert not b.Network(G1, G2).same_clocks()
+
+
+def test_str_repr():
+    """brian/tests/testinterface/test_str_repr.py:3-82: every main class
+    constructs and prints; host-function thresholds and resets construct
+    but a group refuses them."""
+    import brian_amd as b
+    from brian_amd import ms, mV, Hz, mvolt
+
+    def assert_str_repr(obj):
+        assert len(str(obj)) and len(repr(obj))
+
+    b.reinit()
+
+    def build():
+        clock = b.Clock(dt=0.2 * ms)
+        assert_str_repr(clock)
+        assert_str_repr(b.defaultclock)
+    build()
+    eqs = b.Equations('dv/dt=-v/(5 * ms) : volt')
+    group = b.NeuronGroup(100, model=eqs, reset=0 * mV, threshold=10 * mV)
+    for o in (eqs, group, group.subgroup(50), b.PoissonGroup(1, rates=100 * Hz), b.SpikeGeneratorGroup(1, [])):
+        assert_str_repr(o)
+    thresholds = [b.Threshold(1), b.StringThreshold('v > 1'), b.NoThreshold(), b.VariableThreshold('v', 'w'),
+                  b.EmpiricalThreshold(), b.FunThreshold(lambda v: v > 1), b.SimpleFunThreshold(lambda v: v > 1),
+                  b.PoissonThreshold(), b.HomogeneousPoissonThreshold()]
+    resets = [b.Reset(0), b.StringReset('v = 0'), b.VariableReset(), b.FunReset(lambda G, spikes: None),
+              b.NoReset(), b.Refractoriness(), b.SimpleCustomRefractoriness(lambda G, spikes: None),
+              b.CustomRefractoriness(lambda G, spikes: None)]
+    for o in thresholds + resets:
+        assert_str_repr(o)
+    with pytest.raises(NotImplementedError):
+        b.NeuronGroup(2, model='v : 1', threshold=b.PoissonThreshold())
+    with pytest.raises(NotImplementedError):
+        b.NeuronGroup(2, model='v : 1', threshold=1, reset=b.FunReset(lambda G, spikes: None))
+    con = b.Connection(group, group, 'v')
+    con.connect_full(group, group, 1 * mvolt)
+    con_delay = b.Connection(group, group, 'v', delay=True)
+    con_delay.connect_full(group, group, weight=1 * mvolt, delay=5 * ms)
+    monitors = [b.SpikeMonitor(group), b.SpikeMonitor(group, delay=1 * ms),
+                b.SpikeMonitor(group, delay=1 * ms, function=lambda spikes: None),
+                b.StateMonitor(group, 'v'), b.SpikeCounter(group), b.StateSpikeMonitor(group, 'v'),
+                b.RecentStateMonitor(group, 'v'), b.MultiStateMonitor(group),
+                b.PopulationRateMonitor(group), b.PopulationSpikeCounter(group),
+                b.PopulationSpikeCounter(group, delay=1 * ms),
+                b.ISIHistogramMonitor(group, bins=[0 * ms, 10 * ms, 20 * ms]),
+                b.ISIHistogramMonitor(group, bins=[0 * ms, 10 * ms, 20 * ms], delay=1 * ms),
+                b.CoincidenceCounter(group, [1 * ms, 2 * ms]), b.VanRossumMetric(group)]
+    for o in [con, con_delay] + monitors + [b.MagicNetwork(), b.Network(group, con)]:
+        assert_str_repr(o)
