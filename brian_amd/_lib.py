@@ -85,6 +85,7 @@ _SIGS = {
     "bm_net_get_spikes": [P, C.c_int, Pi32, Pi32, i64, Pi64],
     "bm_net_record_spike_states": [P, C.c_int, C.c_int, Pi32],
     "bm_net_set_poisson_base_rates": [P, C.c_int, Pd],
+    "bm_net_set_poisson_threshold": [P, C.c_int, C.c_int, C.c_uint64],
     "bm_pairwise_sq_distance": [Pd, i64, i64, C.c_double, Pd],
     "bm_net_get_spike_states": [P, C.c_int, Pi32, Pi32, Pd, i64, Pi64],
     "bm_net_clear_spikes": [P, C.c_int],
@@ -307,6 +308,9 @@ class NetHandle:
         out = [np.zeros(n, np.int32) for _ in range(3)]
         call("bm_net_read_coincidence_counter", self.h, gid, *[i32ptr(o) for o in out])
         return out
+
+    def set_poisson_threshold(self, gid, var, seed):
+        call("bm_net_set_poisson_threshold", self.h, gid, int(var), int(seed))
 
     def set_poisson_base_rates(self, gid, rates):
         r = f64(rates)

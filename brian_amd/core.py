@@ -303,11 +303,13 @@ class SimpleFunThreshold(FunThreshold):
     """threshold.py:301-330: spikes = thresholdfun(state variable)."""
 
 
-class PoissonThreshold(_HostRule):
-    """threshold.py:461-480: spike with probability state * dt per step."""
+class PoissonThreshold:
+    """threshold.py:461-480: spike with probability state * dt per step,
+    drawn on the device with the counter RNG of Poisson groups."""
 
-    def __init__(self, state=0):
+    def __init__(self, state=0, seed=None):
         self.state = state
+        self.seed = int(np.random.randint(0, 2 ** 31)) if seed is None else int(seed)
 
 
 class HomogeneousPoissonThreshold(PoissonThreshold):
@@ -438,6 +440,8 @@ class NeuronGroup(_GroupBase):
     def _make_threshold(self, th, level):
         if th is None or isinstance(th, NoThreshold):
             return None
+        if isinstance(th, PoissonThreshold):
+            return ("poisson", self._var(th.state), th.seed)
         if isinstance(th, _HostRule):
             raise NotImplementedError(f"{type(th).__name__} runs a host function or draw per step; "
                                       "use a threshold on a state variable")
@@ -2469,6 +2473,8 @@ class Network:
                 h.set_threshold(gid, th[1], th[2])
             elif th[0] == "var":
                 h.set_threshold(gid, th[1], 0.0, th[2])
+            elif th[0] == "poisson":
+                h.set_poisson_threshold(gid, th[1], th[2])
             else:
                 h.set_empirical_threshold(gid, th[1], th[2], th[3])
         for var, op, val, vvar in g._reset:

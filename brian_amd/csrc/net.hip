@@ -274,7 +274,7 @@ bool fast_eligible(const HostGroup &g) {
   if (g.kind != GK_NEURON || g.upd != UPD_LINEAR || g.M < 1 || g.M > 4) return false;
   if (!g.mons.empty() || g.mod_var >= 0 || g.stp_conn >= 0 || !g.pins.empty() || g.coin) return false;
   if (!g.ssm_vars.empty()) return false;
-  if (g.refr != REFR_NONE || g.th_kind == TH_EMPIRICAL || g.ti_nt > 0) return false;
+  if (g.refr != REFR_NONE || g.th_kind == TH_EMPIRICAL || g.th_kind == TH_POISSON || g.ti_nt > 0) return false;
   if (g.ring_desc.size() > 1) return false;
   if (g.ring_desc.size() == 1 && g.ring_desc[0].kind != RING_COUNT) return false;
   return true;
@@ -726,6 +726,7 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       d.prog.scheme = g.prog_scheme;
     }
     d.th_kind = g.th_kind;
+    d.seed = g.seed;
     d.th_var = g.th_var;
     d.th_vtvar = g.th_vtvar;
     d.th_period = g.th_period;
@@ -1294,6 +1295,18 @@ int bm_net_set_empirical_threshold(bm_net *net, int gid, int var, double vt, int
     g.th_var = var;
     g.vt = vt;
     g.th_period = period;
+  });
+}
+
+int bm_net_set_poisson_threshold(bm_net *net, int gid, int var, uint64_t seed) {
+  return guard([&] {
+    require_unprepared(net);
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.kind == GK_NEURON, "only neuron groups have a threshold");
+    BM_REQUIRE(var >= 0 && var < g.M, "threshold variable out of range");
+    g.th_kind = TH_POISSON;
+    g.th_var = var;
+    g.seed = seed;
   });
 }
 

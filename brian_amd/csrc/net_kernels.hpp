@@ -833,6 +833,13 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const GroupDev g, const int64_t
 #endif
           bool c = v > vt;
           if constexpr (GEN) {
+            if (g.th_kind == TH_POISSON) {
+              // PoissonThreshold (brian/threshold.py:461-480): spike with
+              // probability v*dt, u32 < floor(v dt 2^32) clamped to [0, 2^32]
+              double tp = floor((double)v * g.dt * 4294967296.0);
+              tp = tp < 0.0 ? 0.0 : (tp > 4294967296.0 ? 4294967296.0 : tp);
+              c = (double)counter_u32(g.seed, (uint64_t)i, (uint64_t)n) < tp;
+            }
             if (c && g.th_kind == TH_EMPIRICAL && last[k] != NEVER && n - last[k] <= g.th_period) c = false;
             if (c && g.use_next_allowed && last[k] != NEVER) {
               const double t = (double)n * g.dt + 0.0;

@@ -25,7 +25,7 @@ constexpr int32_t NEVER = -2147483647 - 1;   // INT32_MIN
 enum RingKind { RING_COUNT = 0, RING_FLOAT = 1 };
 enum UpdKind { UPD_NONE = 0, UPD_LINEAR = 1, UPD_PERNEURON = 2, UPD_PROGRAM = 3 };
 enum GroupKind { GK_NEURON = 0, GK_POISSON = 1, GK_GENERATOR = 2 };
-enum ThKind { TH_NONE = 0, TH_CONST = 1, TH_VAR = 2, TH_EMPIRICAL = 3 };
+enum ThKind { TH_NONE = 0, TH_CONST = 1, TH_VAR = 2, TH_EMPIRICAL = 3, TH_POISSON = 4 };
 enum RefrKind { REFR_NONE = 0, REFR_CONST = 1, REFR_ARRAY = 2 };
 enum Op {
   OP_MOV = 0, OP_CONST, OP_ADD, OP_SUB, OP_MUL, OP_DIV, OP_NEG, OP_EXP, OP_LOG,

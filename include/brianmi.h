@@ -92,6 +92,10 @@ int bm_net_add_reset_op(bm_net *net, int gid, int var, int op, double value, int
 /* EmpiricalThreshold (brian/threshold.py:409-457): spike iff S[var] > vt and
  * the neuron did not spike in the last `period` steps (period = int(refr/dt),
  * minus one when refr is a whole number of steps). */
+/* PoissonThreshold (brian/threshold.py:461-480): neuron i spikes at step n
+ * with probability S[var][i] * dt: u32(seed, i, n) < floor(S[var] dt 2^32),
+ * clamped to [0, 2^32] (the counter RNG of Poisson groups).  Before prepare. */
+int bm_net_set_poisson_threshold(bm_net *net, int gid, int var, uint64_t seed);
 int bm_net_set_empirical_threshold(bm_net *net, int gid, int var, double vt, int period);
 
 /* Refractoriness (brian/neurongroup.py:368-429, :506-515, brian/reset.py:311-332):

@@ -340,8 +340,13 @@ class Group:
             th = None
         elif th is not None:                 # threshold.py:160-191, :373-396
             v = self.S[th[1]]
-            vt = th[2] if th[0] == "const" else self.S[th[2]]
-            spikes = (v > vt).nonzero()[0]
+            if th[0] == "poisson":           # threshold.py:461-480: rand(N) < v*dt
+                u = counter_uniform_u32(th[2], np.arange(self.N, dtype=np.uint64), np.uint64(n))
+                p = np.clip(np.floor(v * self.dt * 4294967296.0), 0.0, 4294967296.0)
+                spikes = (u.astype(np.float64) < p).nonzero()[0]
+            else:
+                vt = th[2] if th[0] == "const" else self.S[th[2]]
+                spikes = (v > vt).nonzero()[0]
             if self.use_next_allowed:        # neurongroup.py:506-515
                 spikes = spikes[self.next_allowed[spikes] <= t]
                 if self.variable_refr:

@@ -310,7 +310,7 @@ def test_str_repr():
     for o in thresholds + resets:
         assert_str_repr(o)
     with pytest.raises(NotImplementedError):
-        b.NeuronGroup(2, model='v : 1', threshold=b.PoissonThreshold())
+        b.NeuronGroup(2, model='v : 1', threshold=b.FunThreshold(lambda v: v > 1))
     with pytest.raises(NotImplementedError):
         b.NeuronGroup(2, model='v : 1', threshold=1, reset=b.FunReset(lambda G, spikes: None))
     con = b.Connection(group, group, 'v')

@@ -746,3 +746,35 @@ def test_linked_var():
         run(50 * ms)
         assert len(mon1[0]) == 500
         assert check(mon1[0], mon2[0]), when
+
+
+def test_poisson_threshold_matches_oracle():
+    """PoissonThreshold (threshold.py:461-480) on a group whose rate
+    variable relaxes from 0 to 300 Hz and whose reset counts the spikes:
+    raster and state equal the oracle's counter-RNG draws exactly (FP64)."""
+    from brian_amd._lib import NetHandle
+    from oracle import brian1 as O
+    N, steps, dt = 500, 3000, 1e-4
+    a = np.exp(-dt / 0.05)
+    A = np.array([[a, 0.0], [0.0, 1.0]])
+    C = np.array([300.0 * (1 - a), 0.0])
+    S0 = np.vstack([np.zeros(N), np.zeros(N)])
+    h = NetHandle(dt, 64)
+    g = h.add_group(S0)
+    h.set_linear(g, A, C)
+    h.set_poisson_threshold(g, 0, 99)
+    h.add_reset_op(g, 1, 1, 1.0)           # count += 1 at each spike
+    h.record(g, 1 << 20)
+    net = O.Net(dt)
+    gid = net.add_group(O.Group(N, S0, ("linear", A, C), ("poisson", 0, 99), [(1, "+=", 1.0, False)],
+                                dt=dt))
+    h.prepare()
+    h.run(steps)
+    net.run(steps)
+    s, i = h.get_spikes(g)
+    rs, ri = net.raster(gid)
+    assert len(rs) > 1000
+    np.testing.assert_array_equal(s, rs)
+    np.testing.assert_array_equal(i, ri)
+    np.testing.assert_array_equal(h.get_state(g, 1, N), net.groups[gid].S[1])
+    h.close()

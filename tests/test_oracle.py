@@ -668,3 +668,20 @@ def test_van_rossum_closed_forms():
     X = rng.standard_normal((7, 40))
     B = np.array([[0.25 * sum((X[i, t] - X[j, t]) ** 2 for t in range(40)) for j in range(7)] for i in range(7)])
     np.testing.assert_allclose(O.van_rossum_distance(X, 1.0, 4.0), B, rtol=1e-13)
+
+
+def test_poisson_threshold_deterministic_rates():
+    """The 'poisson' threshold (threshold.py:461-480): a rate variable of 0
+    never fires, 1/dt fires every step, and a constant rate r draws the same
+    spikes as a Poisson source of rate r with the same seed."""
+    dt, N = 1e-4, 3
+    S0 = np.array([[0.0, 1.0 / dt, 250.0]])
+    net = O.Net(dt)
+    gid = net.add_group(O.Group(N, S0, ("linear", np.eye(1), np.zeros(1)), ("poisson", 0, 7), dt=dt))
+    src = O.SpikeSource(N, poisson=(S0[0], 7, dt))
+    for n in range(400):
+        net.step()
+        src.update(n * dt, n)
+    steps, idx = net.raster(gid)
+    assert (idx == 0).sum() == 0 and (idx == 1).sum() == 400
+    np.testing.assert_array_equal(steps[idx == 2], [k for k, sp in enumerate(src.history) if 2 in sp])
