@@ -313,7 +313,10 @@ class PoissonThreshold:
 
 
 class HomogeneousPoissonThreshold(PoissonThreshold):
-    """threshold.py:482-500: one rate for the whole group."""
+    """threshold.py:482-500: a group whose neurons share one rate.  The
+    reference draws a Poisson count of spikes from the first neuron's rate
+    and samples the neurons; here every neuron draws with its own (equal)
+    rate, the same process up to the count's cap at N."""
 
 
 class FunReset(_HostRule):
