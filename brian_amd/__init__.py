@@ -14,8 +14,6 @@ from .units import *  # noqa: F401,F403
 from .units import unit_namespace  # noqa: F401
 from .equations import Equations  # noqa: F401
 from .core import *  # noqa: F401,F403
-from .statistics import (CV, firing_rate, gamma_factor, get_gamma_factor, NaN, correlogram,  # noqa: F401
-                         autocorrelogram, CCF, ACF, CCVF, ACVF, total_correlation, sort_spikes,
-                         group_correlations, vector_strength)
+from .statistics import CV, firing_rate, gamma_factor, get_gamma_factor, NaN  # noqa: F401
 from .spikeio import load_aer, load_raster, save_raster  # noqa: F401
 from numpy.random import rand, randn, seed  # noqa: F401

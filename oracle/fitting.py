@@ -9,8 +9,9 @@ vectorised over candidates, and the gamma factor of
 brian/tools/statistics.py:236-240.
 
 Pinned by tests/test_oracle_fitting.py: hand-computed coincidence counts,
-gamma = 1 for identical trains and ~0 for independent Poisson trains, and a
-pure-Python single-candidate loop.  The event-driven CoincidenceCounter of
+gamma = 1 for identical trains, E[gamma] = -0.0034 for independent 10 Hz
+Poisson trains at delta = 40 steps (Nt (1 - exp(-2 delta r)) expected
+coincidences), and a pure-Python single-candidate loop.  The event-driven CoincidenceCounter of
 brian/monitor.py (``coincidence_counter``) is pinned against that loop and
 hand cases in tests/test_oracle_inputs.py.
 """

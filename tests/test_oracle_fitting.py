@@ -45,3 +45,22 @@ def test_gamma_closed_forms():
     assert abs(g - 1.0) < 1e-12
     # a train with no coincidence at all is below zero
     assert F.gamma_factor_counts(0, len(t), len(t), r, 40) < 0
+
+
+def test_gamma_independent_poisson_near_zero():
+    """Independent Poisson trains on the step grid (10 Hz, dt = 0.1 ms,
+    delta = 40 steps): targets with a model spike within delta number
+    Nt (1 - exp(-2 delta r)), so E[gamma] = -0.0034.  A rate in Hz where
+    the steps want a per-step rate, or delta in place of 2 delta, moves
+    gamma by more than 0.04."""
+    rng = np.random.default_rng(3)
+    g = []
+    for _ in range(8):
+        model = np.unique(np.cumsum(np.rint(rng.exponential(1000.0, 3000))).astype(np.int64))
+        target = np.unique(np.cumsum(np.rint(rng.exponential(1000.0, 3000))).astype(np.int64))
+        pos = np.searchsorted(model, target)
+        left = model[np.clip(pos - 1, 0, len(model) - 1)]
+        right = model[np.clip(pos, 0, len(model) - 1)]
+        nc = np.count_nonzero(np.minimum(np.abs(target - left), np.abs(target - right)) <= 40)
+        g.append(F.gamma_factor_counts(nc, len(model), len(target), F.firing_rate_steps(target), 40))
+    assert abs(np.mean(g) + 0.0034) < 0.01, g
