@@ -46,13 +46,30 @@ HBM_PEAK_GBS = 8000.0
 N = 1 << 20
 INDEG = 80
 DELAY = 10
-# algorithmic bytes (streamed-state model): per neuron-step 2*m*s state +
-# 2*k*s ring consume (m=3, k=2) ; per spike 12 B (row_ptr pair + id) ; per
-# synaptic event 4 B col + s w + 2 s ring read-modify-write (+ 1 B per-synapse
-# delay for DelayConnection)
-BYTES_NEURON_STEP = {32: 40, 64: 80}
-BYTES_SPIKE = 12
-BYTES_EVENT = {32: 16, 64: 28}
+ACC_STEPS = 10000          # the accuracy leg always covers 1 s of simulated time
+# Algorithmic bytes of one launch of the windowed sweep kernel (W steps, the
+# state held in registers for the whole window), per neuron:
+#   state read + write once per launch        2 * m * s   (m = 3 variables)
+#   last spike step read + write once         8
+#   packed count ring: read + zero per step   2 * sw * W  (sw bytes per neuron)
+# per spike: row pointer pair 16 B + raster entry 8 B; per synaptic event:
+# column index 4 B + the 32-bit count word read and written by the atomic 8 B
+# (+ 1 B per-synapse delay for a DelayConnection).
+STATE_VARS = 3
+BYTES_LAST = 8
+BYTES_SPIKE = 24
+BYTES_EVENT = 12
+
+
+def sweep_bytes_per_neuron(prec, window, ring_sw):
+    return 2 * STATE_VARS * (prec // 8) + BYTES_LAST + 2 * ring_sw * window
+
+
+def ring_sw(tuning, nch=2):
+    """Bytes per neuron of the packed count ring the tuning selects: 8-bit
+    counters (16 with ring_bits 16) for the nch incoming channels."""
+    bits = 16 if (tuning or {}).get("ring_bits") == 16 else 8
+    return nch * bits // 8
 METRIC = "neuron-updates/s + synaptic events/s @1M-LIF CUBA, %HBM peak, FP32 vs FP64"
 
 
@@ -251,6 +268,27 @@ def device_cuba_csr(h, ce, ci):
             "exc": (erp, ecol, np.full(len(ecol), W.we)), "inh": (irp, icol, np.full(len(icol), W.wi))}
 
 
+def accuracy_leg(build, tuning):
+    """FP32 fast path (the tuned launch shape) against the FP64 check mode
+    over 1 s of simulated time from step 0, whatever the timed step count:
+    population rate, mean per-neuron ISI CV, first raster divergence step."""
+    rasters = {}
+    for prec, window, tun in ((32, 0, tuning), (64, 1, None)):
+        h, g = build(prec, window, tun)[:2]
+        h.run(ACC_STEPS)
+        h.sync()
+        rasters[prec] = h.get_spikes(g)
+        h.close()
+    (s32, i32), (s64, i64) = rasters[32], rasters[64]
+    rate32, cv32 = rate_stats(s32, i32, N, ACC_STEPS)
+    rate64, cv64 = rate_stats(s64, i64, N, ACC_STEPS)
+    return {"steps": ACC_STEPS, "simulated_s": ACC_STEPS * 1e-4,
+            "rate_hz_fp64": rate64, "rate_hz_fp32": rate32, "rate_rel_err": abs(rate32 - rate64) / rate64,
+            "isi_cv_fp64": cv64, "isi_cv_fp32": cv32, "isi_cv_abs_err": abs(cv32 - cv64),
+            "first_raster_divergence_step": first_divergence((s32, i32), (s64, i64)),
+            "spikes_fp32": int(len(s32)), "spikes_fp64": int(len(s64))}
+
+
 def bench_cuba1m(args):
     K = args.steps or 10000
     Wm = 200 if args.warmup is None else args.warmup
@@ -302,34 +340,30 @@ def bench_cuba1m(args):
         h64w.close()
         same = len(r64w[0]) == len(r64[0]) and bool(np.array_equal(r64w[0], r64[0])
                                                    and np.array_equal(r64w[1], r64[1]))
-        m32, m64 = r32[0] >= Wm, r64[0] >= Wm
-        rate32, cv32 = rate_stats(r32[0][m32], r32[1][m32], N, K)
-        rate64, cv64 = rate_stats(r64[0][m64], r64[1][m64], N, K)
+        m64 = r64[0] >= Wm
         fp64 = {
             "neuron_updates_per_s": N * K / el64, "ms_per_step": el64 * 1e3 / K,
             "synaptic_events_per_s": int(m64.sum()) * INDEG / el64,
-            "rate_hz": rate64, "isi_cv": cv64,
-            "fp32_rate_hz": rate32, "fp32_isi_cv": cv32,
-            "rate_rel_err": abs(rate32 - rate64) / rate64,
-            "isi_cv_abs_err": abs(cv32 - cv64),
-            "first_raster_divergence_step": first_divergence(r32, r64),
             "windowed": {"neuron_updates_per_s": N * K / el64w, "ms_per_step": el64w * 1e3 / K,
                          "raster_equal_to_one_step": same},
         }
+    accuracy = None if args.no_fp64 else accuracy_leg(build, tuning)
 
     # ---- roofline of the fused sweep/propagation kernel (FP32 path)
     avg_launch_s = (prof_ms / 1e3) / max(launches, 1)
     spikes_per_launch = prof_spikes / max(launches, 1)
-    alg_bytes = (N * window * BYTES_NEURON_STEP[32] + spikes_per_launch * BYTES_SPIKE
-                 + spikes_per_launch * INDEG * BYTES_EVENT[32])
+    per_neuron = sweep_bytes_per_neuron(32, window, ring_sw(tuning))
+    alg_bytes = (N * per_neuron + spikes_per_launch * BYTES_SPIKE
+                 + spikes_per_launch * INDEG * BYTES_EVENT)
     roof = roofline(alg_bytes, avg_launch_s, hbm_traffic("sweep_cuba1m_pmc"),
                     kernel="k_sweep<float,3,K,0> (fused state update + ring consume + threshold + reset "
                            "+ propagation)",
                     steps_per_launch=window, avg_launch_us=avg_launch_s * 1e6,
                     alg_bytes_per_launch=alg_bytes, launches_timed=launches,
-                    note="algorithmic bytes follow the streamed-state model (state read and written "
-                         "every step); the fused kernel keeps the state in registers for a whole "
-                         "window, so frac can exceed 1; dram_frac is the measured DRAM traffic")
+                    bytes_per_neuron_per_launch=per_neuron, spikes_per_launch=spikes_per_launch,
+                    note="algorithmic bytes of a windowed launch: state and last spike step once per "
+                         "launch, packed ring slot read and zeroed per step, 24 B per spike, 12 B per "
+                         "synaptic event")
 
     cpu = None if args.no_cpu else cpu_baseline(d, args.cpu_budget)
     cpp = None if args.no_cpu else cpp_reference()
@@ -350,6 +384,7 @@ def bench_cuba1m(args):
         "synaptic_events_per_s": n_sp32 * INDEG / el32,
         "spikes": n_sp32,
         "fp64_plain": fp64,
+        "accuracy_1s": accuracy,
         "fp32_over_fp64_plain": (value / fp64["neuron_updates_per_s"]) if fp64 else None,
         "fp32_over_fp64_windowed": (value / fp64["windowed"]["neuron_updates_per_s"]) if fp64 else None,
         "autotune": tune_report,
@@ -420,8 +455,10 @@ def bench_uneven(args):
     r32 = res[32]
     launches, prof_ms, window, psp, pev = r32["prof"]
     avg_launch_s = prof_ms / 1e3 / max(launches, 1)
-    alg_bytes = (d["N"] * window * BYTES_NEURON_STEP[32] + psp / launches * BYTES_SPIKE
-                 + pev / launches * (BYTES_EVENT[32] + 1))
+    # 16-bit counters (source blocks of up to 20,000 neurons), two channels;
+    # per-synapse delay byte on every event
+    alg_bytes = (d["N"] * sweep_bytes_per_neuron(32, window, 4) + psp / launches * BYTES_SPIKE
+                 + pev / launches * (BYTES_EVENT + 1))
     roof = roofline(alg_bytes, avg_launch_s, hbm_traffic("sweep_uneven_pmc"),
                     kernel="k_sweep<float,3,K,0> on the 21.4M-neuron block-diagonal batch",
                     steps_per_launch=window, avg_launch_us=avg_launch_s * 1e6,

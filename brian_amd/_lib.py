@@ -25,7 +25,7 @@ Pi64 = C.POINTER(C.c_int64)
 
 # bm_net_set_tuning keys (brianmi.h)
 TUNE = {"window": 1, "neurons_per_thread": 2, "wave_max": 3, "row_split": 4, "ring_bits": 5,
-        "ring_pack": 6, "use_graph": 7, "window_max": 8, "specialize": 9, "defer": 10}
+        "ring_pack": 6, "use_graph": 7, "window_max": 8, "specialize": 9, "defer": 10, "lean": 11}
 
 
 class BrianMIError(RuntimeError):
@@ -226,6 +226,7 @@ class NetHandle:
         self.dt = dt
         self.precision = precision
         self._group_nvars = {}
+        self._group_n = {}
 
     def close(self):
         if self.h:
@@ -245,6 +246,7 @@ class NetHandle:
         gid = C.c_int()
         call("bm_net_add_group", self.h, n, m, dptr(S0), C.byref(gid))
         self._group_nvars[gid.value] = m
+        self._group_n[gid.value] = n
         return gid.value
 
     def set_linear(self, gid, A, Cv):
@@ -252,7 +254,11 @@ class NetHandle:
         call("bm_net_set_linear", self.h, gid, dptr(A), dptr(Cv))
 
     def set_linear_per_neuron(self, gid, A, Cv):
+        """A[m][m][N], C[m][N] (FP64, neuron index fastest)."""
         A, Cv = f64(A), f64(Cv)
+        m, n = self._nvars(gid), self._group_n[gid]
+        if A.size != m * m * n or Cv.size != m * n:
+            raise ValueError(f"per-neuron propagator needs A of {m}x{m}x{n} and C of {m}x{n} values")
         call("bm_net_set_linear_per_neuron", self.h, gid, dptr(A), dptr(Cv))
 
     def set_program(self, gid, scheme, nregs, code, consts):

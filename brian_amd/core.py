@@ -26,7 +26,8 @@ from .units import ms, mV, second
 
 __all__ = [
     "Clock", "EventClock", "defaultclock", "reinit_default_clock", "get_default_clock", "define_default_clock",
-    "guess_clock", "linked_var", "LinkedVar", "LazyStateUpdater", "FunThreshold", "SimpleFunThreshold",
+    "guess_clock", "linked_var", "LinkedVar", "LazyStateUpdater", "MultiLinearStateUpdater",
+    "get_multilinear_state_updater", "MultiLinearNeuronGroup", "FunThreshold", "SimpleFunThreshold",
     "PoissonThreshold", "HomogeneousPoissonThreshold", "FunReset", "CustomRefractoriness",
     "SimpleCustomRefractoriness",
     "Threshold", "VariableThreshold", "EmpiricalThreshold", "NoThreshold", "StringThreshold",
@@ -347,6 +348,38 @@ class LazyStateUpdater:
         self.clock = clock
 
 
+class MultiLinearStateUpdater:
+    """experimental/multilinearstateupdater.py:112-171: a linear update with
+    one propagator per neuron, S[:,i] = dot(A[:,:,i], S[:,i]) + B[:,i].  On
+    the device each neuron's A and B are loaded once per launch into
+    registers; the rows are evaluated as the numpy path of ``__call__``
+    does, the left fold A[j,0]*S0 + A[j,1]*S1 + ... then + B[j] (:149-158)."""
+
+    def __init__(self, A, B=None):
+        self.A = np.ascontiguousarray(A, dtype=np.float64)
+        if self.A.ndim != 3 or self.A.shape[0] != self.A.shape[1]:
+            raise ValueError("A must have shape (m, m, N)")
+        m, _, n = self.A.shape
+        self.B = (np.zeros((m, n)) if B is None
+                  else np.ascontiguousarray(np.broadcast_to(np.asarray(B, dtype=np.float64), (m, n))))
+
+    def __len__(self):
+        return self.A.shape[0]
+
+    def __repr__(self):
+        return "MultiLinearStateUpdater"
+    __str__ = __repr__
+
+
+def get_multilinear_state_updater(eqs, subs, level=0, clock=None):
+    """experimental/multilinearstateupdater.py:79-109: ``subs`` maps
+    parameter names to arrays of one value per neuron; A and B are computed
+    per neuron on the host in FP64."""
+    e = eqs if isinstance(eqs, Equations) else Equations(eqs, level=level + 1)
+    A, B = e.multilinear_propagator(guess_clock(clock).dt, subs)
+    return MultiLinearStateUpdater(A, B)
+
+
 def _caller_ns(level):
     frame = inspect.currentframe()
     try:
@@ -410,6 +443,13 @@ class NeuronGroup(_GroupBase):
         elif isinstance(model, LazyStateUpdater):
             # state variables that never change (stateupdater.py:680-705)
             model = Equations("\n".join(f"lazy{k} : 1" for k in range(model.numstates)))
+        elif isinstance(model, MultiLinearStateUpdater):
+            # a state updater given as the model: unnamed variables
+            # (neurongroup.py:318-331), here one per row of A
+            if model.A.shape[2] != self.N:
+                raise ValueError("MultiLinearStateUpdater size does not match the group")
+            self._state_updater = model
+            model = Equations("\n".join(f"x{k} : 1" for k in range(len(model))))
         self._eqs = model
         self.var_names = model.variables() if model is not None else []
         self.var_index = {n: j for j, n in enumerate(self.var_names)}
@@ -632,6 +672,42 @@ class _SubGroup(NeuronGroup):
 
     def _get_state(self, var, off, n):
         return NeuronGroup._get_state(self, var, off, n)
+
+
+class MultiLinearNeuronGroup(NeuronGroup):
+    """experimental/multilinearstateupdater.py:174-238: one set of linear
+    equations whose parameters ``subs`` (a list of names) take one value per
+    neuron.  The values are read by name from the caller's namespace, the
+    parameter lines are removed from the equations, the group has one neuron
+    per value and a ``MultiLinearStateUpdater`` with each neuron's exact
+    propagator."""
+
+    def __init__(self, eqs, subs, clock=None, level=0, **kwds):
+        subs_set = set(subs)
+        lines = []
+        units = {}
+        for line in re.sub(r"\\\s*?\n", " ", eqs).splitlines():
+            line = re.sub("#.*", "", line)
+            m = re.match(r"\s*(\w+)\s*:\s*(.*)", line)
+            if m is not None and m.group(1) in subs_set:
+                units[m.group(1)] = m.group(2)
+                continue
+            lines.append(line)
+        if not units:
+            raise ValueError("none of subs is a parameter of the equations")
+        ns = _caller_ns(level + 1)
+        values = {}
+        for k in units:
+            if k not in ns:
+                raise NameError(f"substituted parameter {k!r} is not defined in the caller's namespace")
+            values[k] = np.asarray(ns[k], dtype=np.float64).ravel()
+        n = len(next(iter(values.values())))
+        text = "\n".join(lines)
+        eq = Equations(text, level=level + 1)
+        clk = guess_clock(clock)
+        su = get_multilinear_state_updater(eq, values, clock=clk)
+        NeuronGroup.__init__(self, n, eq, clock=clk, level=level + 1, **kwds)
+        object.__setattr__(self, "_state_updater", su)
 
 
 class SpikeGeneratorGroup(_GroupBase):
@@ -2462,8 +2538,11 @@ class Network:
     def _compile_neurons(self, h, gid, g, dt):
         eqs = g._eqs
         implicit = g._implicit
-        linear = (g._method in (None, "linear")) and not implicit and eqs.is_linear()
-        if linear:
+        su = g.__dict__.get("_state_updater")
+        linear = (su is None and (g._method in (None, "linear")) and not implicit and eqs.is_linear())
+        if isinstance(su, MultiLinearStateUpdater):
+            h.set_linear_per_neuron(gid, su.A, su.B)
+        elif linear:
             A, C = eqs.linear_propagator(dt)
             h.set_linear(gid, A, C)
         elif len(g.var_names):

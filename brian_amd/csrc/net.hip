@@ -80,7 +80,10 @@ struct HostGroup {
   int H = 0;
   // device
   DevBuf S, last, Apn_d, Cpn_d, code_d, consts_d, refr_d, ti_d, pthr_d, gen_idx_d, gen_off_d,
-      raster_d, raster_count_d, hist_d, hist_count_d;
+      raster_d, raster_count_d, hist_d, hist_count_d, hist_mult_d;
+  // generator groups: the most times one neuron may be listed in one step
+  // (set at prepare from the count rings of its outgoing connections)
+  int64_t gen_mult_cap = INT64_MAX;
   std::vector<DevBuf> rings;
   std::vector<RingDev> ring_desc;
   std::vector<ChanDev> chans;
@@ -90,6 +93,7 @@ struct HostGroup {
   GroupDev dev{};
   JitKernel jit;          // specialised lean sweep for jit_K neurons per thread
   int jit_K = 0;
+  bool jit_lean = false;  // jit holds k_lean (else the table-driven k_sweep)
   struct Mon {
     int var = 0, timestep = 1, mode = 0;
     int64_t cap = 0;
@@ -113,6 +117,9 @@ struct HostConn {
   int ring = -1, ch = 0;
   int stdp = -1;
   int64_t max_indeg = 0;
+  // largest count one target can receive in one ring slot: in-degree times
+  // the most times a source neuron is listed in one step (generators)
+  int64_t max_count = 0;
   int mod_var = -1;          // source variable scaling the weights (modulation), -1 none
   bool stp = false;          // weights scaled by the source neuron's u*x (short-term plasticity)
   std::vector<double> stp_init;   // [4][src_n] u, x, ux, last update time
@@ -135,6 +142,7 @@ struct HostStdp {
   std::vector<int32_t> kpre, kpost;
   int wdelay = 0, HP = 0, kbytes = 1;
   DevBuf col_ptr_d, col_perm_d, col_row_d, a_pre_d, a_post_d, h_pre_d, h_post_d, kpre_d, kpost_d;
+  DevBuf wc_d;            // FP32: Kahan compensation of the weights
   StdpDev dev{};
 };
 
@@ -152,6 +160,7 @@ struct bm_net {
   int W = 1, K = 4;
   int Wcap = 1;           // largest window the rings are sized for
   bool specialize = true; // lean sweep compiled at run time for each group's shape
+  bool lean_kernel = false; // specialised groups use k_lean (the model as code) rather than k_sweep tables
   bool use_graph = true;
   std::vector<std::pair<int, int>> mon_ids;   // (group, monitor within group)
   // tiling knobs (bm_net_set_tuning)
@@ -236,6 +245,17 @@ void check_generator_spikes(int64_t N, int64_t nspikes, const int32_t *steps, co
     if (q) BM_REQUIRE(steps[q] > steps[q - 1] || (steps[q] == steps[q - 1] && idx[q] >= idx[q - 1]),
                       "generator spikes must be sorted by (step, index)");
   }
+}
+
+// The most times one neuron is listed in one step of a sorted generator
+// table (at least 1).
+int64_t generator_multiplicity(const HostGroup &g) {
+  int64_t best = 1, run = 0;
+  for (size_t q = 0; q < g.gen_idx.size(); ++q) {
+    run = (q && g.gen_steps[q] == g.gen_steps[q - 1] && g.gen_idx[q] == g.gen_idx[q - 1]) ? run + 1 : 1;
+    best = std::max(best, run);
+  }
+  return best;
 }
 
 HostGroup &grp(bm_net *net, int gid) {
@@ -327,6 +347,78 @@ std::string jit_list(const std::vector<int> &v) {
   return o.str();
 }
 
+// A double as a C++ hexadecimal literal (exact), cast to float for FP32.
+std::string lit(double v, bool f32) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%a", v);
+  return f32 ? std::string("(float)(") + buf + ")" : std::string("(") + buf + ")";
+}
+
+// The group's model as compile-time code for k_lean (net_lean.hpp): the
+// channel deliveries, reset ops, linear update and threshold of this group
+// with every constant written out.  FP64 keeps the reference's arithmetic
+// (count loops, left-fold rows with no contraction); FP32 uses one fused
+// multiply-add per delivery and per non-zero entry of A.
+std::string lean_prelude(bm_net *net, const HostGroup &g) {
+  const bool f32 = net->precision == 32;
+  const int M = g.M;
+  const int nring = (int)g.ring_desc.size();
+  const int bits = nring ? g.ring_desc[0].bits : 8, sw = nring ? g.ring_desc[0].sw : 0;
+  std::ostringstream d;
+  d << "#include \"net_kernels.hpp\"\nnamespace bm {\n"
+    << "typedef " << (f32 ? "float" : "double") << " lean_t;\n#define LEAN_M " << M << "\n#define LEAN_SW " << sw
+    << "\n";
+  // deliveries in the channel order (Connections, then DelayConnection flushes)
+  d << "__device__ inline void lean_deliver(lean_t (&x)[LEAN_M], uint32_t word) {\n";
+  const uint32_t mask = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+  for (const ChanDev &c : g.chans) {
+    d << "  { const uint32_t c = (word >> " << c.ch * bits << ") & " << mask << "u;\n";
+    if (f32) d << "    x[" << c.var << "] = __builtin_fmaf((float)c, " << lit(c.w, true) << ", x[" << c.var << "]); }\n";
+    else d << "    x[" << c.var << "] = apply_count<double>(x[" << c.var << "], c, " << lit(c.w, false) << ", "
+           << c.is_delay << "); }\n";
+  }
+  d << "}\n";
+  d << "__device__ inline void lean_reset(lean_t (&x)[LEAN_M]) {\n";
+  for (size_t r = 0; r < g.rs_var.size(); ++r) {
+    const std::string v = g.rs_vvar[r] >= 0 ? "x[" + std::to_string(g.rs_vvar[r]) + "]" : lit(g.rs_val[r], f32);
+    if (g.rs_op[r] == 0) d << "  x[" << g.rs_var[r] << "] = " << v << ";\n";
+    else d << "  x[" << g.rs_var[r] << "] = x[" << g.rs_var[r] << "] + " << v << ";\n";
+  }
+  d << "}\n";
+  d << "__device__ inline void lean_update(lean_t (&x)[LEAN_M]) {\n";
+  if (!f32) {
+    // S <- A.S + C as the left fold A[j,0]*S0 + A[j,1]*S1 + ..., then + C[j]
+    // (brian/stateupdater.py:413-416 in that order), no contraction
+    d << "#pragma clang fp contract(off)\n";
+    for (int j = 0; j < M; ++j) {
+      d << "  const lean_t t" << j << " = ";
+      for (int q = 0; q < M; ++q) d << (q ? " + " : "") << lit(g.A[j * M + q], false) << " * x[" << q << "]";
+      d << ";\n";
+    }
+    for (int j = 0; j < M; ++j) d << "  x[" << j << "] = t" << j << " + " << lit(g.C[j], false) << ";\n";
+  } else {
+    for (int j = 0; j < M; ++j) {
+      std::string e;
+      if (g.C[j] != 0.0) e = lit(g.C[j], true);
+      for (int q = 0; q < M; ++q) {
+        const double a = g.A[j * M + q];
+        if (a == 0.0) continue;
+        const std::string term = "x[" + std::to_string(q) + "]";
+        if (e.empty()) e = (a == 1.0) ? term : lit(a, true) + " * " + term;
+        else e = "__builtin_fmaf(" + lit(a, true) + ", " + term + ", " + e + ")";
+      }
+      d << "  const lean_t t" << j << " = " << (e.empty() ? "0.0f" : e) << ";\n";
+    }
+    for (int j = 0; j < M; ++j) d << "  x[" << j << "] = t" << j << ";\n";
+  }
+  d << "}\n";
+  d << "__device__ inline bool lean_spike(const lean_t (&x)[LEAN_M]) { return x[" << g.th_var << "] > ";
+  if (g.th_kind == TH_VAR) d << "x[" << g.th_vtvar << "]";
+  else d << lit(g.vt, f32);
+  d << "; }\n}\n#include \"net_lean.hpp\"\n";
+  return d.str();
+}
+
 JitKernel jit_sweep(bm_net *net, const HostGroup &g, int K) {
   std::vector<int> sh, var, dly, rv, ro, rvv;
   const int nring = (int)g.ring_desc.size();
@@ -351,9 +443,13 @@ JitKernel jit_sweep(bm_net *net, const HostGroup &g, int K) {
     << "\n#define BM_JIT_TH_KIND " << g.th_kind << "\n#define BM_JIT_TH_VAR " << g.th_var
     << "\n#define BM_JIT_TH_VTVAR " << g.th_vtvar << "\n#define BM_JIT_DEFER " << (g.defer ? 1 : 0)
     << "\n#include \"net_kernels.hpp\"\n";
-  const std::string name = std::string("bm::k_sweep<") + (net->precision == 64 ? "double" : "float") + ", " +
-                           std::to_string(g.M) + ", " + std::to_string(K) + ", 0>";
-  const std::string src = d.str() + "// " + name + "\n";
+  // deferred propagation keeps the table-driven specialisation; every other
+  // lean group gets its model compiled in (k_lean)
+  const bool lean = !g.defer && net->lean_kernel;
+  const std::string name = lean ? "bm::k_lean<" + std::to_string(K) + ">"
+                                : std::string("bm::k_sweep<") + (net->precision == 64 ? "double" : "float") + ", " +
+                                      std::to_string(g.M) + ", " + std::to_string(K) + ", 0>";
+  const std::string src = (lean ? lean_prelude(net, g) : d.str()) + "// " + name + "\n";
   std::lock_guard<std::mutex> lk(g_jit_mu);
   auto it = g_jit.find(src);
   if (it != g_jit.end()) return it->second;
@@ -399,9 +495,10 @@ void ensure_jit(bm_net *net) {
     HostGroup &g = *gp;
     if (g.kind == GK_GENERATOR || !fast_eligible(g)) continue;
     const int K = group_K(net, g);
-    if (g.jit_K == K && g.jit.fn) continue;
+    if (g.jit_K == K && g.jit.fn && g.jit_lean == net->lean_kernel) continue;
     g.jit = jit_sweep(net, g, K);
     g.jit_K = K;
+    g.jit_lean = net->lean_kernel;
   }
 }
 
@@ -476,6 +573,10 @@ void build_stdp(bm_net *net, HostStdp &s) {
   std::vector<double> z1(c.src_n, 0.0), z2(c.tgt_n, 0.0);
   upload_T(net, s.a_pre_d, z1.data(), z1.size());
   upload_T(net, s.a_post_d, z2.data(), z2.size());
+  if (net->precision == 32) {
+    std::vector<float> zc(c.nnz, 0.0f);
+    upload_raw(s.wc_d, zc.data(), zc.size(), net->stream);
+  }
   if (c.delay_kind == 1) {
     BM_REQUIRE(!s.kpre.empty() || c.nnz == 0,
                "STDP on a DelayConnection needs the synaptic delays in seconds (bm_net_set_stdp_delays)");
@@ -531,11 +632,15 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
   // ---- per-connection storage and count/float mode
   for (size_t ci = 0; ci < net->conns.size(); ++ci) {
     HostConn &c = *net->conns[ci];
+    const HostGroup &srcg = *net->groups[c.src];
+    const int64_t mult = srcg.kind == GK_GENERATOR ? generator_multiplicity(srcg) : 1;
     if (c.generated && c.gen.kind == 1) {
       // fixed in-degree k: counts per slot and channel are at most k
       c.max_indeg = c.gen.k;
+      c.max_count = c.max_indeg * mult;
       c.count_mode = c.stdp < 0;
       BM_REQUIRE(c.count_mode, "STDP on device-generated connections is not supported");
+      BM_REQUIRE(c.max_count <= 65535, "in-degree x generator repeats per step exceeds 65535");
       c.nnz = generate_fixed_indegree_csr(c.gen, c.row_ptr_d, c.col_d, net->stream);
       continue;
     }
@@ -544,7 +649,8 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       int64_t mx = 0;
       for (int64_t n : c.gen.src_n) mx = std::max(mx, n);
       c.max_indeg = mx;
-      c.count_mode = c.stdp < 0 && mx <= 65535;
+      c.max_count = mx * mult;
+      c.count_mode = c.stdp < 0 && c.max_count <= 65535;
       BM_REQUIRE(c.count_mode, "generated connections need a source block of at most 65535 neurons");
       c.gen.dly_bytes = c.max_delay <= 255 ? 1 : 2;
       c.dly_bytes = c.delay_kind == 1 ? c.gen.dly_bytes : 0;
@@ -557,7 +663,10 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     std::vector<int64_t> indeg(c.tgt_n, 0);
     for (int64_t q = 0; q < c.nnz; ++q) indeg[c.col[q]]++;
     c.max_indeg = c.tgt_n ? *std::max_element(indeg.begin(), indeg.end()) : 0;
-    c.count_mode = uniform && c.stdp < 0 && c.max_indeg <= 65535 && c.mod_var < 0 && !c.stp;
+    c.max_count = c.max_indeg * mult;
+    // counts of a slot must fit the ring's counters; a generator neuron
+    // listed k times in one step adds k per synapse
+    c.count_mode = uniform && c.stdp < 0 && c.max_count <= 65535 && c.mod_var < 0 && !c.stp;
     if (c.mod_var >= 0) {
       HostGroup &sg = *net->groups[c.src];
       BM_REQUIRE(sg.mod_var < 0 || sg.mod_var == c.mod_var,
@@ -611,10 +720,18 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     std::vector<int> cnt_chans, flt_chans;
     for (int ci : incoming) (net->conns[ci]->count_mode ? cnt_chans : flt_chans).push_back(ci);
     bool small = true;
-    for (int ci : cnt_chans) small = small && net->conns[ci]->max_indeg <= 255;
-    BM_REQUIRE(net->ring_bits != 8 || small, "8-bit ring counters need every in-degree <= 255");
+    for (int ci : cnt_chans) small = small && net->conns[ci]->max_count <= 255;
+    BM_REQUIRE(net->ring_bits != 8 || small,
+               "8-bit ring counters need every in-degree (times generator repeats per step) <= 255");
     if (net->ring_bits == 16) small = false;
     const int bits = small ? 8 : 16;
+    // generator sources: the repeats per step a later spike table may hold
+    for (int ci : cnt_chans) {
+      const HostConn &c = *net->conns[ci];
+      HostGroup &sg = *net->groups[c.src];
+      if (sg.kind == GK_GENERATOR)
+        sg.gen_mult_cap = std::min<int64_t>(sg.gen_mult_cap, ((1 << bits) - 1) / std::max<int64_t>(c.max_indeg, 1));
+    }
     const int per_ring = net->ring_pack ? (small ? 4 : 2) : 1;
     g.rings.clear();
     g.ring_desc.clear();
@@ -917,6 +1034,10 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       d.hist = g.hist_d.as<int32_t>();
       d.hist_count = g.hist_count_d.as<unsigned int>();
       d.H = g.H;
+      if (g.kind == GK_GENERATOR) {
+        g.hist_mult_d.alloc((size_t)g.H * g.N * 4);
+        d.hist_mult = g.hist_mult_d.as<int32_t>();
+      }
       hptrs.push_back(d.hist_count);
       hH.push_back(g.H);
     }
@@ -960,6 +1081,7 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     d.row_ptr = c.row_ptr_d.as<int64_t>();
     d.col = c.col_d.as<int32_t>();
     d.w = c.w_d.p;
+    d.wc = s.wc_d.p;
     d.col_ptr = s.col_ptr_d.as<int64_t>();
     d.col_perm = s.col_perm_d.as<int64_t>();
     d.col_row = s.col_row_d.as<int32_t>();
@@ -1005,6 +1127,7 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
     HostGroup &tg = *net->groups[c.tgt];
     d.pre_hist = sg.hist_d.as<int32_t>();
     d.pre_hist_count = sg.hist_count_d.as<unsigned int>();
+    d.pre_hist_mult = sg.hist_mult_d.p ? sg.hist_mult_d.as<int32_t>() : nullptr;
     d.pre_H = sg.H;
     d.pre_N = sg.N;
     d.post_hist = tg.hist_d.as<int32_t>();
@@ -1345,6 +1468,15 @@ int bm_net_set_generator_spikes(bm_net *net, int gid, int64_t nspikes, const int
     HostGroup &g = grp(net, gid);
     BM_REQUIRE(g.kind == GK_GENERATOR, "not a spike generator group");
     check_generator_spikes(g.N, nspikes, steps, idx);
+    if (net->prepared) {
+      HostGroup probe;
+      probe.gen_steps.assign(steps, steps + nspikes);
+      probe.gen_idx.assign(idx, idx + nspikes);
+      const int64_t mult = generator_multiplicity(probe);
+      BM_REQUIRE(mult <= g.gen_mult_cap,
+                 "a generator neuron is listed more times in one step than the count rings of its "
+                 "connections (sized at prepare from in-degree x repeats) can hold");
+    }
     g.gen_steps.assign(steps, steps + nspikes);
     g.gen_idx.assign(idx, idx + nspikes);
     if (!net->prepared) return;
@@ -1977,6 +2109,9 @@ int bm_net_set_tuning(bm_net *net, int key, int value) {
         require_unprepared(net);
         net->defer_req = value != 0;
         break;
+      case BM_TUNE_LEAN:
+        net->lean_kernel = value != 0;
+        break;
       default:
         throw Error(BM_EINVAL, "unknown tuning key");
     }
@@ -2003,6 +2138,7 @@ int bm_net_get_tuning(bm_net *net, int key, int *value) {
       case BM_TUNE_WINDOW_MAX: *value = net->prepared ? (net->defer_W > 0 ? net->defer_W : net->Wcap) : 0; break;
       case BM_TUNE_SPECIALIZE: *value = net->specialize ? 1 : 0; break;
       case BM_TUNE_DEFER: *value = net->prepared ? (net->defer_W > 0 ? 1 : 0) : net->defer_req; break;
+      case BM_TUNE_LEAN: *value = net->lean_kernel ? 1 : 0; break;
       default: throw Error(BM_EINVAL, "unknown tuning key");
     }
   });

@@ -1019,9 +1019,21 @@ __global__ __launch_bounds__(BLOCK) void k_generator(const GroupDev g, const int
     }
     if (!act) continue;
     if (g.hist != nullptr) {
-      const int64_t hs = n % g.H;
-      const unsigned int hp = atomicAdd(&g.hist_count[hs], 1u);
-      if (hp < (unsigned int)g.N) g.hist[hs * g.N + hp] = (int32_t)i;   // the host caps spikes per step at N
+      // a neuron listed k times in step n (the table is sorted by (step,
+      // index), so repeats are adjacent) gets one history entry of
+      // multiplicity k, written by the first of them: the STDP kernels then
+      // apply its k updates in order within one wavefront
+      if (e == g.gen_off[n] || g.gen_idx[e - 1] != (int32_t)i) {
+        const int64_t e_end = g.gen_off[n + 1];
+        int32_t k = 1;
+        while (e + k < e_end && g.gen_idx[e + k] == (int32_t)i) ++k;
+        const int64_t hs = n % g.H;
+        const unsigned int hp = atomicAdd(&g.hist_count[hs], 1u);
+        if (hp < (unsigned int)g.N) {      // distinct neurons per step <= N
+          g.hist[hs * g.N + hp] = (int32_t)i;
+          g.hist_mult[hs * g.N + hp] = k;
+        }
+      }
     }
     for (int o = 0; o < g.nout; ++o) {
       const OutDev &od = g.outs[o];
@@ -1093,6 +1105,46 @@ __device__ inline T stdp_weight(const StdpDev &sd, T x, T a, int mode, double of
   return x < (T)sd.wmin ? (T)sd.wmin : (x > (T)sd.wmax ? (T)sd.wmax : x);
 }
 
+// FP32 weights carry a Kahan compensation term c per synapse: every rule is
+// applied as an increment (a*scale, x*a or (wmax - x)*a) added with
+// compensated summation, so the rounding of many small updates does not
+// accumulate; a clipped weight drops its compensation.  FP64 (check mode)
+// uses stdp_weight, the reference's arithmetic.
+__device__ inline float stdp_weight_kahan(const StdpDev &sd, float x, float &c, float a, int mode, double off,
+                                          double scale) {
+#pragma clang fp contract(off)
+  float inc;
+  if (mode == 0) inc = (a + (float)off) * (float)scale;
+  else if (mode == 1) inc = x * a;
+  else inc = ((float)sd.wmax - x) * a;
+  const float y = inc - c;
+  const float t = x + y;
+  c = (t - x) - y;
+  x = t;
+  if (x < (float)sd.wmin) { x = (float)sd.wmin; c = 0.0f; }
+  else if (x > (float)sd.wmax) { x = (float)sd.wmax; c = 0.0f; }
+  return x;
+}
+
+// k updates of synapse q by one rule with the other side's trace a, in order.
+template <typename T>
+__device__ inline void stdp_apply(const StdpDev &sd, T *w, int64_t q, T a, int mode, double off, double scale,
+                                  int k) {
+  if constexpr (sizeof(T) == 4) {
+    if (sd.wc != nullptr) {
+      float *wc = static_cast<float *>(sd.wc);
+      float x = w[q], c = wc[q];
+      for (int r = 0; r < k; ++r) x = stdp_weight_kahan(sd, x, c, a, mode, off, scale);
+      w[q] = x;
+      wc[q] = c;
+      return;
+    }
+  }
+  T x = w[q];
+  for (int r = 0; r < k; ++r) x = stdp_weight<T>(sd, x, a, mode, off, scale);
+  w[q] = x;
+}
+
 // Presynaptic updater (brian/stdp.py:371-393): for each pre spike i,
 // A_pre[i] += Ap; w[i,:] += A_post[cols]; clip.  One wavefront per spike.
 // Connection.propagate of a modulated Connection with a homogeneous delay
@@ -1141,8 +1193,11 @@ __global__ __launch_bounds__(BLOCK) void k_stdp_pre(const StdpDev sd, const int6
       for (unsigned int p = wave * 64 + lane; p < cnt; p += nwaves * 64) {
         const int64_t i = (int64_t)sd.pre_hist[hs * sd.pre_N + p] - sd.src_off;
         if (i < 0 || i >= sd.src_n) continue;
-        if (sd.interactions == 0 || sd.interactions == 3) apre[i] = apre[i] + (T)sd.Ap;
-        else apre[i] = (T)sd.Ap;
+        const int mult = sd.pre_hist_mult ? sd.pre_hist_mult[hs * sd.pre_N + p] : 1;
+        for (int r = 0; r < mult; ++r) {
+          if (sd.interactions == 0 || sd.interactions == 3) apre[i] = apre[i] + (T)sd.Ap;
+          else apre[i] = (T)sd.Ap;
+        }
       }
     }
     // weight lines of the spikes of step m - wdelay (DelayedSTDPUpdater, stdp.py:77-87)
@@ -1153,10 +1208,11 @@ __global__ __launch_bounds__(BLOCK) void k_stdp_pre(const StdpDev sd, const int6
     for (unsigned int p = wave; p < cnt; p += nwaves) {
       const int64_t i = (int64_t)sd.pre_hist[hs * sd.pre_N + p] - sd.src_off;
       if (i < 0 || i >= sd.src_n) continue;
+      const int mult = sd.pre_hist_mult ? sd.pre_hist_mult[hs * sd.pre_N + p] : 1;
       const int64_t lo = sd.row_ptr[i], hi = sd.row_ptr[i + 1];
       for (int64_t q = lo + lane; q < hi; q += 64) {
         const int64_t r = stdp_hist_row(sd, m, q, sd.kpre);
-        w[q] = stdp_weight<T>(sd, w[q], hpost[r * sd.tgt_n + sd.col[q]], sd.pre_mode, sd.pre_off, sd.pre_scale);
+        stdp_apply<T>(sd, w, q, hpost[r * sd.tgt_n + sd.col[q]], sd.pre_mode, sd.pre_off, sd.pre_scale, mult);
       }
     }
     return;
@@ -1173,18 +1229,27 @@ __global__ __launch_bounds__(BLOCK) void k_stdp_pre(const StdpDev sd, const int6
   for (unsigned int p = wave; p < cnt; p += nwaves) {
     const int64_t i = (int64_t)sd.pre_hist[hs * sd.pre_N + p] - sd.src_off;
     if (i < 0 || i >= sd.src_n) continue;
+    // a generator neuron listed k times: the connection propagates every
+    // repeat with the weights before this step's updates, then the
+    // updater's loop over the spike list applies k updates in order
+    // ('for _i in spikes', stdp.py:371-393)
+    const int mult = sd.pre_hist_mult ? sd.pre_hist_mult[hs * sd.pre_N + p] : 1;
     if (lane == 0) {
-      if (sd.interactions == 0 || sd.interactions == 3) apre[i] = apre[i] + (T)sd.Ap;
-      else apre[i] = (T)sd.Ap;
+      for (int k = 0; k < mult; ++k) {
+        if (sd.interactions == 0 || sd.interactions == 3) apre[i] = apre[i] + (T)sd.Ap;
+        else apre[i] = (T)sd.Ap;
+      }
     }
     const int64_t lo = sd.row_ptr[i], hi = sd.row_ptr[i + 1];
     T *ring = sd.prop ? reinterpret_cast<T *>(static_cast<unsigned char *>(sd.ring) +
                                               (m % sd.ring_D) * sd.ring_slot_bytes)
                       : nullptr;
     for (int64_t q = lo + lane; q < hi; q += 64) {
-      T x = w[q];
-      if (sd.prop) unsafeAtomicAdd(ring + sd.tgt_off + sd.col[q], x);   // connection.py:240-241
-      w[q] = stdp_weight<T>(sd, x, apost[sd.col[q]], sd.pre_mode, sd.pre_off, sd.pre_scale);
+      if (sd.prop) {                                     // connection.py:240-241
+        const T x = w[q];
+        for (int k = 0; k < mult; ++k) unsafeAtomicAdd(ring + sd.tgt_off + sd.col[q], x);
+      }
+      stdp_apply<T>(sd, w, q, apost[sd.col[q]], sd.pre_mode, sd.pre_off, sd.pre_scale, mult);
     }
   }
 }
@@ -1218,14 +1283,13 @@ __global__ __launch_bounds__(BLOCK) void k_stdp_post(const StdpDev sd, const int
       for (int64_t q = lo + lane; q < hi; q += 64) {
         const int64_t pw = sd.col_perm[q];
         const int64_t r = stdp_hist_row(sd, n, pw, sd.kpost);
-        w[pw] = stdp_weight<T>(sd, w[pw], hpre[r * sd.src_n + sd.col_row[q]], sd.post_mode, sd.post_off,
-                               sd.post_scale);
+        stdp_apply<T>(sd, w, pw, hpre[r * sd.src_n + sd.col_row[q]], sd.post_mode, sd.post_off, sd.post_scale, 1);
       }
       continue;
     }
     for (int64_t q = lo + lane; q < hi; q += 64) {
       const int64_t pw = sd.col_perm[q];
-      w[pw] = stdp_weight<T>(sd, w[pw], apre[sd.col_row[q]], sd.post_mode, sd.post_off, sd.post_scale);
+      stdp_apply<T>(sd, w, pw, apre[sd.col_row[q]], sd.post_mode, sd.post_off, sd.post_scale, 1);
     }
   }
 }

@@ -1,0 +1,176 @@
+// Lean sweep kernel compiled at run time (hipRTC) for one group of the
+// common shape: shared linear propagator, at most one packed count ring,
+// reset ops without refractoriness, constant or variable threshold.
+//
+// The source that includes this header first defines the group's model as
+// compile-time code (generated on the host, net.hip:lean_prelude):
+//   lean_t                     the arithmetic type (float or double)
+//   LEAN_M, LEAN_SW            state variables; ring bytes per neuron (0: none)
+//   lean_deliver(x, word)      applies the channel counts of one ring slice
+//   lean_reset(x)              the reset ops (brian/reset.py:139-147)
+//   lean_update(x)             S <- A.S + C with A and C as literals
+//   lean_spike(x)              the threshold condition (brian/threshold.py:160-191)
+// so that a step costs only the arithmetic of the model: zero entries of A
+// vanish in FP32, and there are no table lookups.
+//
+// One launch advances W <= WMAX steps (kernel step n runs the tail of step
+// n-1 -- deliveries, then the reset of a neuron that spiked at n-1 -- and
+// the head of step n: update and threshold, as k_sweep does).  The ring
+// slices of the whole window are loaded at once and their slots zeroed, the
+// state stays in registers, and a spike is recorded only as the wavefront's
+// threshold ballot of its step, kept in scalar registers: the spike list,
+// the last spike step, the raster and the propagation are produced once per
+// window from those ballots.
+#pragma once
+#include "net_kernels.hpp"
+
+namespace bm {
+
+template <int K>
+__global__ __launch_bounds__(BLOCK) void k_lean(const GroupDev g, const int64_t *d_base, int64_t rel0, int W,
+                                                int skip_first_tail, int tail_only) {
+  using T = lean_t;
+  constexpr int M = LEAN_M;
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint16_t *sh_list = reinterpret_cast<uint16_t *>(smem);   // (w << 12) | local index
+  __shared__ int sh_n;
+  __shared__ unsigned long long sh_base;
+  __shared__ unsigned int sh_hbase;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int64_t n0 = *d_base + rel0;
+  const int N = (int)g.N;
+  const int blk0 = blockIdx.x * (BLOCK * K);
+  const int nsteps = tail_only ? 1 : W;
+
+  T s[K][M];
+  int32_t last[K];
+  bool valid[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = blk0 + k * BLOCK + tid;
+    valid[k] = i < N;
+    last[k] = valid[k] ? g.last[i] : NEVER;
+#pragma unroll
+    for (int j = 0; j < M; ++j) s[k][j] = valid[k] ? static_cast<const T *>(g.S)[(int64_t)j * N + i] : (T)0;
+  }
+
+  // ring slices of every step of the window, loaded at once, then zeroed
+  uint32_t cw[WMAX][K];
+  int slot0 = (int)((n0 - 1) % g.D);
+  if (slot0 < 0) slot0 += g.D;
+#pragma unroll
+  for (int u = 0; u < WMAX; ++u) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) cw[u][k] = 0u;
+    if constexpr (LEAN_SW > 0) {
+      if (u < nsteps && !(u == 0 && skip_first_tail)) {
+        int sl = slot0 + u;
+        if (sl >= g.D) sl -= g.D;
+        const unsigned char *slot =
+            static_cast<const unsigned char *>(g.rings[0].base) + (int64_t)sl * g.rings[0].slot_bytes;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (valid[k]) cw[u][k] = load_slice(slot, blk0 + k * BLOCK + tid, LEAN_SW);
+      }
+    }
+  }
+  if constexpr (LEAN_SW > 0) {
+#pragma unroll
+    for (int u = 0; u < WMAX; ++u) {
+      if (u < nsteps && !(u == 0 && skip_first_tail)) {
+        int sl = slot0 + u;
+        if (sl >= g.D) sl -= g.D;
+        unsigned char *slot = static_cast<unsigned char *>(g.rings[0].base) + (int64_t)sl * g.rings[0].slot_bytes;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (valid[k]) zero_slice(slot, blk0 + k * BLOCK + tid, LEAN_SW);
+      }
+    }
+  }
+  if (tid == 0) sh_n = 0;
+
+  // ---------------- the window's steps
+  bool prev[K];                       // spiked at the previous step: reset in this tail
+#pragma unroll
+  for (int k = 0; k < K; ++k) prev[k] = valid[k] && last[k] == (int32_t)(n0 - 1);
+  unsigned long long spk[WMAX][K];    // threshold ballot of each step (wavefront-uniform)
+#pragma unroll
+  for (int w = 0; w < WMAX; ++w) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) spk[w][k] = 0ull;
+    if (w >= nsteps) continue;
+    if (!(w == 0 && skip_first_tail)) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        lean_deliver(s[k], cw[w][k]);
+        if (prev[k]) lean_reset(s[k]);
+      }
+    }
+    if (tail_only) continue;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      lean_update(s[k]);
+      const bool sp = valid[k] && lean_spike(s[k]);
+      spk[w][k] = __ballot(sp);
+      prev[k] = sp;
+    }
+  }
+
+  // ---------------- spike list of the window (wavefront ballots -> LDS)
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < WMAX; ++w) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const unsigned long long mask = spk[w][k];
+      if (mask == 0ull) continue;
+      const bool mine = (mask >> lane) & 1ull;
+      if (mine) last[k] = (int32_t)(n0 + w);
+      int base = 0;
+      if (lane == __ffsll((long long)mask) - 1) base = atomicAdd(&sh_n, __popcll(mask));
+      base = __shfl(base, __ffsll((long long)mask) - 1);
+      if (mine)
+        sh_list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (uint16_t)((w << 12) | (k * BLOCK + tid));
+    }
+  }
+  __syncthreads();
+  const int nsp = sh_n;
+  if (nsp > 0) {
+    // raster append (SpikeMonitor, brian/monitor.py:156-166)
+    if (g.raster != nullptr) {
+      const int shard = blockIdx.x & (RSHARD - 1);
+      if (tid == 0) sh_base = atomicAdd(g.raster_count + shard * RSTRIDE, (unsigned long long)nsp);
+      __syncthreads();
+      unsigned long long *region = g.raster + (int64_t)shard * g.raster_cap;
+      for (int p = tid; p < nsp; p += BLOCK) {
+        const uint32_t e = sh_list[p];
+        const unsigned long long pos = sh_base + p;
+        if (pos < (unsigned long long)g.raster_cap)
+          region[pos] = ((uint64_t)(n0 + (e >> 12)) << 32) | (uint64_t)(blk0 + (e & 0xFFFu));
+      }
+    }
+    // spike history for STDP (networks with STDP run one step per launch)
+    if (g.hist != nullptr) {
+      const int64_t hs = n0 % g.H;
+      if (tid == 0) sh_hbase = atomicAdd(&g.hist_count[hs], (unsigned int)nsp);
+      __syncthreads();
+      for (int p = tid; p < nsp; p += BLOCK)
+        g.hist[hs * N + sh_hbase + p] = (int32_t)(blk0 + (sh_list[p] & 0xFFFu));
+    }
+    propagate_block<T>(g, sh_list, nullptr, nsp, n0, blk0, tid);
+  }
+
+  // ---------------- write back
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (!valid[k]) continue;
+    const int i = blk0 + k * BLOCK + tid;
+    g.last[i] = last[k];
+#pragma unroll
+    for (int j = 0; j < M; ++j) static_cast<T *>(g.S)[(int64_t)j * N + i] = s[k][j];
+  }
+}
+
+}  // namespace bm

@@ -141,6 +141,9 @@ struct GroupDev {
   int64_t raster_cap;                 // entries per region
   int32_t *hist;              // [H][N] spike lists (STDP), null if unused
   unsigned int *hist_count;   // [H]
+  // generator groups: a neuron listed k times in one step has one history
+  // entry with multiplicity k here ([H][N]); null for other groups (k = 1)
+  int32_t *hist_mult;
   const uint64_t *pthr;       // Poisson thresholds [N]
   uint64_t seed;
   int ti_var, ti_per_neuron;
@@ -221,6 +224,7 @@ struct StdpDev {
   const int64_t *row_ptr;
   const int32_t *col;
   void *w;                    // T[nnz]
+  void *wc;                   // FP32: Kahan compensation per synapse (T[nnz]); null in FP64
   const int64_t *col_ptr;     // [tgt_n+1]
   const int64_t *col_perm;    // [nnz] index into w, ascending source within column
   const int32_t *col_row;     // [nnz] source row of that synapse
@@ -229,6 +233,7 @@ struct StdpDev {
   int update, interactions, pre_delay, pad;
   const int32_t *pre_hist;    // source group's history [H][Nsrc]
   const unsigned int *pre_hist_count;
+  const int32_t *pre_hist_mult;   // multiplicity of each entry (generator sources), null: 1
   int pre_H;
   int post_H;
   const int32_t *post_hist;

@@ -237,26 +237,51 @@ class Equations:
             fi = vec(e)
             for j in range(n):
                 M[j, i] = fi[j] + AB[j, 0]
-        try:
-            # a singular M (a parameter among the variables) falls back to the
-            # numerical solution, as LinAlgError does in the reference; scipy
-            # may report it as NaN from its diagonal fast path instead
-            with np.errstate(all="ignore"), warnings.catch_warnings():
-                warnings.simplefilter("ignore", scipy.linalg.LinAlgWarning)
-                B = scipy.linalg.solve(M, AB)
-            if not np.all(np.isfinite(B)):
-                raise np.linalg.LinAlgError("singular matrix")
-            A = scipy.linalg.expm(M * dt)
-            C = -np.dot(A, B) + B
-            return np.array(A, order="C"), np.array(C[:, 0])
-        except (np.linalg.LinAlgError, scipy.linalg.LinAlgError):
-            deltat = dt / 100
-            E = np.eye(n) + deltat * M
-            Cm = np.eye(n)
-            D = np.zeros((n, 1))
-            for _ in range(100):
-                Cm, D = np.dot(E, Cm), np.dot(E, D) - AB * deltat
-            return np.array(Cm, order="C"), D[:, 0]
+        return _propagator(M, AB, dt)
+
+    def multilinear_propagator(self, dt, subs):
+        """Per-neuron (A[m,m,N], C[m,N]) with S[:,i] <- A[:,:,i].S[:,i] + C[:,i]
+        for a linear model whose parameters ``subs`` (name -> N values) differ
+        between neurons (experimental/multilinearstateupdater.py:20-76): the
+        right-hand side is evaluated once over all neurons with the arrays in
+        the namespace, giving AB[j,i] = -f_j(0) and M[j,k,i] = f_j(e_k) + AB[j,i]
+        (element by element the values each neuron's scalar evaluation
+        gives); then per neuron the same solve / expm / Euler fallback as
+        ``linear_propagator``.  Neurons with equal (M, AB) share one
+        computation."""
+        names = self.variables()
+        n = len(names)
+        arrs = {k: np.asarray(v, dtype=np.float64).ravel() for k, v in subs.items()}
+        if not arrs:
+            raise ValueError("subs names no parameter")
+        N = len(next(iter(arrs.values())))
+        if any(len(v) != N for v in arrs.values()):
+            raise ValueError("every substituted parameter needs one value per neuron")
+        e = Equations.__new__(Equations)
+        e.__dict__.update(self.__dict__)
+        e._namespace = dict(self._namespace)
+        e._namespace.update(arrs)
+        f = e.rhs_function()
+
+        def vec(x):
+            return np.array([v if v is not None else np.zeros(N) for v in f(x)])
+        AB = -vec(np.zeros((n, N)))                       # [n][N]
+        M = np.zeros((n, n, N))
+        for i in range(n):
+            x = np.zeros((n, N))
+            x[i] = 1.0
+            M[:, i, :] = vec(x) + AB
+        key = np.concatenate([M.reshape(n * n, N), AB], axis=0).T       # [N][n*n+n]
+        uniq, inv = np.unique(key, axis=0, return_inverse=True)
+        inv = np.asarray(inv).ravel()
+        A = np.empty((n, n, N))
+        C = np.empty((n, N))
+        for u in range(len(uniq)):
+            sel = inv == u
+            Au, Cu = _propagator(uniq[u, :n * n].reshape(n, n), uniq[u, n * n:].reshape(n, 1), dt)
+            A[:, :, sel] = Au[:, :, None]
+            C[:, sel] = Cu[:, None]
+        return A, C
 
     # ------------------------------------------------------------ bytecode
     def compile_program(self):
@@ -401,3 +426,30 @@ def _allocate(code, outs, m):
         out_code.append((op, pd, pa, pb))
     new_outs = [phys[r] if r >= 0 else -1 for r in outs]
     return out_code, new_outs, max(top, m)
+
+
+def _propagator(M, AB, dt):
+    """(A, C) of dX/dt = M X - AB over dt: B = solve(M, AB), A = expm(M dt),
+    C = -A.B + B (stateupdater.py:367-375); a singular M takes the product
+    of 100 Euler sub-steps (:284-314)."""
+    n = M.shape[0]
+    try:
+        # a singular M (a parameter among the variables) falls back to the
+        # numerical solution, as LinAlgError does in the reference; scipy
+        # may report it as NaN from its diagonal fast path instead
+        with np.errstate(all="ignore"), warnings.catch_warnings():
+            warnings.simplefilter("ignore", scipy.linalg.LinAlgWarning)
+            B = scipy.linalg.solve(M, AB)
+        if not np.all(np.isfinite(B)):
+            raise np.linalg.LinAlgError("singular matrix")
+        A = scipy.linalg.expm(M * dt)
+        C = -np.dot(A, B) + B
+        return np.array(A, order="C"), np.array(C[:, 0])
+    except (np.linalg.LinAlgError, scipy.linalg.LinAlgError):
+        deltat = dt / 100
+        E = np.eye(n) + deltat * M
+        Cm = np.eye(n)
+        D = np.zeros((n, 1))
+        for _ in range(100):
+            Cm, D = np.dot(E, Cm), np.dot(E, D) - AB * deltat
+        return np.array(Cm, order="C"), D[:, 0]

@@ -319,7 +319,12 @@ int bm_net_prepare(bm_net *net, int window, int neurons_per_thread, int use_grap
  *                              launch, half of the workgroups before and half
  *                              after their sweep, so delivery atomics overlap
  *                              sweeps; limits the window to (minimum delay +
- *                              1) / 2 and fixes neurons_per_thread */
+ *                              1) / 2 and fixes neurons_per_thread
+ *   BM_TUNE_LEAN               1 (default): a specialised group's kernel has
+ *                              its model compiled in (propagator, weights,
+ *                              reset and threshold values as literals; spikes
+ *                              kept as per-step wavefront ballots until the end
+ *                              of the window); 0: the table-driven form */
 enum {
   BM_TUNE_WINDOW = 1,
   BM_TUNE_NEURONS_PER_THREAD = 2,
@@ -330,7 +335,8 @@ enum {
   BM_TUNE_USE_GRAPH = 7,
   BM_TUNE_WINDOW_MAX = 8,
   BM_TUNE_SPECIALIZE = 9,
-  BM_TUNE_DEFER = 10
+  BM_TUNE_DEFER = 10,
+  BM_TUNE_LEAN = 11
 };
 int bm_net_set_tuning(bm_net *net, int key, int value);
 int bm_net_get_tuning(bm_net *net, int key, int *value);

@@ -14,12 +14,16 @@ def cuba_propagator(dt=1e-4):
 
 
 def device_cuba(d, precision=64, window=0, record=True, use_graph=True, npt=0,
-                delay_kind=0, syn_delays=None, max_delay=0, refractory=None):
-    """CUBA description ``d`` -> prepared NetHandle (group 0)."""
+                delay_kind=0, syn_delays=None, max_delay=0, refractory=None, per_neuron=None):
+    """CUBA description ``d`` -> prepared NetHandle (group 0); ``per_neuron``
+    = (A[3][3][N], C[3][N]) gives every neuron its own propagator."""
     A, C = cuba_propagator()
     h = NetHandle(1e-4, precision)
     g = h.add_group(d["S0"])
-    h.set_linear(g, A, C)
+    if per_neuron is not None:
+        h.set_linear_per_neuron(g, *per_neuron)
+    else:
+        h.set_linear(g, A, C)
     h.set_threshold(g, 0, d["Vt"])
     h.add_reset_op(g, 0, 0, d["Vr"])
     if refractory is not None:
@@ -37,10 +41,11 @@ def device_cuba(d, precision=64, window=0, record=True, use_graph=True, npt=0,
     return h, g
 
 
-def oracle_cuba(d, delay_kind=0, syn_delays=None, max_delay=0, refractory=None):
+def oracle_cuba(d, delay_kind=0, syn_delays=None, max_delay=0, refractory=None, per_neuron=None):
     A, C = cuba_propagator()
+    upd = ("linear", A, C) if per_neuron is None else ("multilinear",) + tuple(per_neuron)
     net = O.Net(1e-4)
-    grp = O.Group(d["N"], d["S0"], ("linear", A, C), ("const", 0, d["Vt"]),
+    grp = O.Group(d["N"], d["S0"], upd, ("const", 0, d["Vt"]),
                   [(0, "=", d["Vr"], False)], refractory=refractory, dt=1e-4)
     gid = net.add_group(grp)
     N, Ne, Ni = d["N"], d["Ne"], d["Ni"]

@@ -328,3 +328,52 @@ def test_str_repr():
                 b.CoincidenceCounter(group, [1 * ms, 2 * ms]), b.VanRossumMetric(group)]
     for o in [con, con_delay] + monitors + [b.MagicNetwork(), b.Network(group, con)]:
         assert_str_repr(o)
+
+
+def test_multilinear_propagator_per_neuron():
+    """Equations.multilinear_propagator (experimental/multilinearstateupdater.py:
+    20-76): each neuron's (A, C) equals the shared propagator of a model with
+    that neuron's parameter value, bit for bit, and the closed form A_vv =
+    exp(-dt/tau_i), A_v,ge = taue/(tau_i-taue)(exp(-dt/tau_i)-exp(-dt/taue))."""
+    import math
+    from brian_amd.equations import Equations
+    eqs = "dv/dt = (ge+gi-(v+49*mV))/taum : volt\ndge/dt = -ge/(5*ms) : volt\ndgi/dt = -gi/(10*ms) : volt"
+    taus = np.array([0.02, 0.011, 0.02, 0.037, 0.011])
+    A, C = Equations(eqs).multilinear_propagator(1e-4, {"taum": taus})
+    assert A.shape == (3, 3, 5) and C.shape == (3, 5)
+    for i, tau in enumerate(taus):
+        A1, C1 = Equations(eqs, taum=float(tau)).linear_propagator(1e-4)
+        assert np.array_equal(A[:, :, i], A1) and np.array_equal(C[:, i], C1)
+        em, ee = math.exp(-1e-4 / tau), math.exp(-1e-4 / 5e-3)
+        assert abs(A[0, 0, i] - em) < 1e-15
+        assert abs(A[0, 1, i] - 5e-3 / (tau - 5e-3) * (em - ee)) < 1e-15
+        assert A[1, 0, i] == 0.0 and A[2, 0, i] == 0.0
+
+
+def test_multilinear_neurongroup_construction():
+    """MultiLinearNeuronGroup(eqs, subs): the parameters named in ``subs``
+    are read from the caller's namespace, one neuron per value; the state
+    updater holds per-neuron matrices (the reference's own example at
+    multilinearstateupdater.py:240-246: v(t) = exp(k t))."""
+    import math
+    from brian_amd import MultiLinearNeuronGroup, MultiLinearStateUpdater, NeuronGroup, second
+    eqs = """
+    dv/dt = k*v/(1*second) : 1
+    dw/dt = k*w/(1*second) : 1
+    k : 1
+    """
+    k = np.array([-1.0, -2.0, -3.0])
+    G = MultiLinearNeuronGroup(eqs, ["k"])
+    assert len(G) == 3 and G.var_names == ["v", "w"]
+    su = G._state_updater
+    assert isinstance(su, MultiLinearStateUpdater) and len(su) == 2 and str(su) == "MultiLinearStateUpdater"
+    dt = G.clock.dt
+    np.testing.assert_allclose(su.A[0, 0], np.exp(k * dt), rtol=1e-14)
+    assert np.all(su.A[0, 1] == 0) and np.all(su.B == 0)
+    with pytest.raises(NameError):
+        MultiLinearNeuronGroup("dv/dt = q*v/second : 1\nq : 1", ["q"])
+    H = NeuronGroup(3, model=su)
+    assert H.var_names == ["x0", "x1"] and H._state_updater is su
+    with pytest.raises(ValueError):
+        NeuronGroup(4, model=su)
+    assert second == 1.0 and math.isfinite(dt)

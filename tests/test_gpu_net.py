@@ -264,3 +264,133 @@ def test_deferred_propagation_matches_immediate(precision):
     np.testing.assert_array_equal(i1, i0)
     for a, b in zip(st1, st0):
         np.testing.assert_array_equal(a, b)
+
+
+def _hetero_taum_propagator(N, seed):
+    from brian_amd.equations import Equations
+    rng = np.random.default_rng(seed)
+    taum = 10e-3 + 20e-3 * rng.random(N)
+    eqs = Equations(W.CUBA_EQS, El=W.El, taue=W.taue, taui=W.taui)
+    return eqs.multilinear_propagator(1e-4, {"taum": taum})
+
+
+@pytest.mark.parametrize("npt", [1, 4])
+def test_c1_per_neuron_propagator_fp64(npt):
+    """MultiLinearStateUpdater (experimental/multilinearstateupdater.py:
+    132-158) on the device: the 4000-neuron CUBA network with tau_m drawn per
+    neuron from [10, 30) ms, each neuron's A and C computed on the host in
+    FP64.  Raster bit-exact against the oracle's per-neuron update, state
+    within 1e-12 relative, over several run() calls."""
+    d = W.cuba(4000, seed=17)
+    A, C = _hetero_taum_propagator(4000, 17)
+    h, g = device_cuba(d, precision=64, npt=npt, per_neuron=(A, C))
+    net, gid = oracle_cuba(d, per_neuron=(A, C))
+    for chunk in (1, 999, 3000):
+        h.run(chunk)
+        net.run(chunk)
+        _state_close(h, g, net, gid, 4000)
+    s, i = h.get_spikes(g)
+    rs, ri = net.raster(gid)
+    assert len(rs) > 1000
+    assert first_divergence(s, i, rs, ri) is None
+
+
+def test_c1_per_neuron_propagator_fp32_rate():
+    """The same heterogeneous network on the FP32 path: population rate
+    within 5% of the FP64 oracle over 1 s (4000 chaotic neurons)."""
+    d = W.cuba(4000, seed=17)
+    A, C = _hetero_taum_propagator(4000, 17)
+    h, g = device_cuba(d, precision=32, per_neuron=(A, C))
+    net, gid = oracle_cuba(d, per_neuron=(A, C))
+    h.run(10000)
+    net.run(10000)
+    s, _ = h.get_spikes(g)
+    rs, _ = net.raster(gid)
+    assert len(rs) > 1000
+    assert abs(len(s) - len(rs)) / len(rs) < 0.05, (len(s), len(rs))
+
+
+def test_multilinear_neurongroup_facade_closed_form():
+    """The reference's own MultiLinearNeuronGroup example
+    (multilinearstateupdater.py:240-246): dv/dt = k v / second with
+    k = -1, -2, -3 from v = 1 follows exp(k t) on the device."""
+    import brian_amd as b
+    b.reinit()
+    b.reinit_default_clock()
+    eqs = """
+    dv/dt = k*v/(1*second) : 1
+    dw/dt = k*w/(1*second) : 1
+    k : 1
+    """
+    k = np.array([-1.0, -2.0, -3.0])
+    G = b.MultiLinearNeuronGroup(eqs, ["k"])
+    G.v = 1
+    G.w = 0
+    M = b.StateMonitor(G, "v", record=True)
+    net = b.Network(G, M)
+    net.run(100 * b.ms)
+    t = np.asarray(M.times)
+    assert len(t) == 1000
+    # a sample at t = n dt is the state after step n's update: v = exp(k (t + dt))
+    for i in range(3):
+        np.testing.assert_allclose(M[i], np.exp(k[i] * (t + 1e-4)), rtol=1e-12)
+    np.testing.assert_array_equal(np.asarray(G.w), 0.0)
+
+
+C2_N = 1 << 20
+
+
+def test_c2_fp64_prefix_matches_oracle():
+    """The 1,048,576-neuron CUBA network (fixed in-degree 80 drawn on the
+    device, 1 ms delay) in FP64 check mode against the oracle on the same
+    connectivity (read back through the C-ABI): raster bit-exact and state
+    within 1e-12 relative over the first 300 steps."""
+    from brian_amd.batch import build_cuba_fixed_indegree
+    h, g, ce, ci = build_cuba_fixed_indegree(C2_N, 80, 2, 10, precision=64, window=0)
+    Ne = int(0.8 * C2_N)
+    erp, ecol, _ = h.get_connection(ce, Ne)
+    irp, icol, _ = h.get_connection(ci, C2_N - Ne)
+    assert erp[-1] + irp[-1] == 80 * C2_N
+    d = {"N": C2_N, "Ne": Ne, "Ni": C2_N - Ne, "S0": W.cuba_initial_state(C2_N, 2), "Vt": W.Vt, "Vr": W.Vr,
+         "exc": (erp, ecol, np.full(len(ecol), W.we)), "inh": (irp, icol, np.full(len(icol), W.wi)),
+         "delay_steps": 10}
+    net, gid = oracle_cuba(d)
+    h.run(300)
+    net.run(300)
+    s, i = h.get_spikes(g)
+    rs, ri = net.raster(gid)
+    print(f"C2 FP64 prefix: {len(rs)} spikes in 300 steps", flush=True)
+    assert len(rs) > 10000
+    assert first_divergence(s, i, rs, ri) is None
+    _state_close(h, g, net, gid, C2_N)
+    h.close()
+
+
+def test_c2_fp32_vs_fp64_rate():
+    """The same network over 1 s (10,000 steps): FP32 fast path (autotuned
+    default launch shape) against the FP64 check mode.  Population rate
+    within 2%; the mean per-neuron ISI CV error and the first raster
+    divergence step are reported."""
+    from brian_amd import statistics as ST
+    from brian_amd.batch import build_cuba_fixed_indegree
+    res = {}
+    for prec in (32, 64):
+        h, g, _, _ = build_cuba_fixed_indegree(C2_N, 80, 2, 10, precision=prec, window=0 if prec == 32 else 1)
+        h.run(10000)
+        res[prec] = h.get_spikes(g)
+        h.close()
+    rate = {p: len(res[p][0]) / (C2_N * 1.0) for p in res}
+    cv = {}
+    for p in res:
+        c = ST.isi_cv_per_neuron(res[p][0], res[p][1], C2_N)
+        cv[p] = float(np.mean(c[np.isfinite(c)]))
+    s32, i32 = res[32]
+    s64, i64 = res[64]
+    div = first_divergence(s32, i32, s64, i64)
+    rel = abs(rate[32] - rate[64]) / rate[64]
+    print(f"C2 1 s: rate FP32 {rate[32]:.4f} Hz FP64 {rate[64]:.4f} Hz (rel err {rel:.2e}), "
+          f"ISI CV {cv[32]:.4f} vs {cv[64]:.4f} (abs err {abs(cv[32] - cv[64]):.2e}), "
+          f"first divergence step {div}", flush=True)
+    assert rate[64] > 1.0
+    assert rel < 0.02
+    assert abs(cv[32] - cv[64]) < 0.02

@@ -93,3 +93,57 @@ def test_stdp_network_fp64_matches_oracle(kind, inter, update):
     np.testing.assert_allclose(apre, st.A_pre, rtol=1e-12, atol=1e-18)
     np.testing.assert_allclose(apost, st.A_post, rtol=1e-12, atol=1e-18)
     h.close()
+
+
+def _fixed_train_stdp(prec, steps=20000, npre=200, npost=20, seed=8):
+    """Plastic Connection from 200 generator neurons (20 Hz) onto a
+    variable of 20 neurons that does not feed back: their spikes come from
+    a second generator (10 Hz), so both spike trains are the same at any
+    precision and the weights can be compared synapse by synapse."""
+    rng = np.random.default_rng(seed)
+    pre = rng.random((steps, npre)) < 20.0 * DT
+    post = rng.random((steps, npost)) < 10.0 * DT
+    ps, pi = np.nonzero(pre)
+    qs, qi = np.nonzero(post)
+    wmax = 0.01
+    indptr = np.arange(0, (npre + 1) * npost, npost, dtype=np.int64)
+    indices = np.tile(np.arange(npost, dtype=np.int32), npre)
+    w0 = wmax * rng.random(npre * npost)
+    A = np.array([[np.exp(-DT / 10e-3), 0.0], [0.0, 1.0]])
+    h = NetHandle(DT, prec)
+    gp = h.add_generator(npre, ps.astype(np.int32), pi.astype(np.int32))
+    gd = h.add_generator(npost, qs.astype(np.int32), qi.astype(np.int32))
+    t = h.add_group(np.zeros((2, npost)))
+    h.set_linear(t, A, np.zeros(2))
+    h.set_threshold(t, 0, 0.5)
+    h.add_reset_op(t, 0, 0, 0.0)
+    ce = h.add_connection(gp, 0, npre, t, 0, npost, 1, indptr, indices, w0)
+    h.add_connection(gd, 0, npost, t, 0, npost, 0, np.arange(npost + 1, dtype=np.int64),
+                     np.arange(npost, dtype=np.int32), np.ones(npost))
+    dec = np.exp(-DT / 20e-3)
+    h.add_stdp(ce, dec, dec, 0.01 * wmax, -0.0105 * wmax, 0.0, wmax, 0, 0)
+    h.record(t, 1 << 16)
+    h.prepare()
+    h.run(steps)
+    s, i = h.get_spikes(t)
+    wd = h.get_weights(ce, len(indices))
+    h.close()
+    return wd, (s, i), w0, wmax, (qs, qi)
+
+
+def test_stdp_network_fp32_kahan_against_fp64():
+    """Network STDP in FP32 (weights with Kahan compensation, traces FP32)
+    against the FP64 check mode on identical spike trains over 2 s: every
+    weight within 1e-5 wmax, the weight histogram unchanged."""
+    w64, r64, w0, wmax, (qs, qi) = _fixed_train_stdp(64)
+    w32, r32, _, _, _ = _fixed_train_stdp(32)
+    np.testing.assert_array_equal(r32[0], r64[0])
+    np.testing.assert_array_equal(r32[1], r64[1])
+    assert len(r64[0]) > 0.9 * len(qs)         # the post spikes come from the drive
+    assert np.abs(w64 - w0).max() > 0.05 * wmax  # plasticity happened
+    err = np.abs(w32 - w64).max() / wmax
+    bins = np.linspace(0, wmax, 21)
+    tv = 0.5 * np.abs(np.histogram(w32, bins)[0] - np.histogram(w64, bins)[0]).sum() / len(w64)
+    print(f"FP32-Kahan vs FP64 STDP weights: max err {err:.2e} wmax, histogram TV {tv:.2e}", flush=True)
+    assert err < 1e-5
+    assert tv < 0.01

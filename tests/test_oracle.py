@@ -685,3 +685,67 @@ def test_poisson_threshold_deterministic_rates():
     steps, idx = net.raster(gid)
     assert (idx == 0).sum() == 0 and (idx == 1).sum() == 400
     np.testing.assert_array_equal(steps[idx == 2], [k for k, sp in enumerate(src.history) if 2 in sp])
+
+
+def _cuba_exact(t, v0, ge0, gi0, taum, taue=5 * ms, taui=10 * ms, El=-49 * mV):
+    """Exact solution of the CUBA system at time t (per neuron arrays)."""
+    em = np.exp(-t / taum)
+    ee, ei = math.exp(-t / taue), math.exp(-t / taui)
+    v = (El + (v0 - El) * em + ge0 * taue / (taum - taue) * (em - ee)
+         + gi0 * taui / (taum - taui) * (em - ei))
+    return v, ge0 * ee, gi0 * ei
+
+
+def _cuba_per_neuron_AC(taum, taue=5 * ms, taui=10 * ms, El=-49 * mV):
+    """Per-neuron A[3][3][N], C[3][N] from the closed form (developer-
+    simulationprinciples.txt:112-114 with tau_m per neuron)."""
+    n = len(taum)
+    em = np.exp(-DT / taum)
+    ee, ei = math.exp(-DT / taue), math.exp(-DT / taui)
+    A = np.zeros((3, 3, n))
+    A[0, 0] = em
+    A[0, 1] = taue / (taum - taue) * (em - ee)
+    A[0, 2] = taui / (taum - taui) * (em - ei)
+    A[1, 1] = ee
+    A[2, 2] = ei
+    C = np.zeros((3, n))
+    C[0] = El * (1 - em)
+    return A, C
+
+
+def test_multilinear_step_follows_each_neurons_closed_form():
+    """MultiLinearStateUpdater (experimental/multilinearstateupdater.py:132-158)
+    with heterogeneous tau_m: 2000 steps of multilinear_step reproduce every
+    neuron's exact trajectory.  A transposed A (ge driven by v), a swapped
+    neuron index (the tau_m are all different) or a dropped C fails here."""
+    rng = np.random.default_rng(3)
+    n = 64
+    taum = np.linspace(7 * ms, 60 * ms, n)
+    A, C = _cuba_per_neuron_AC(taum)
+    S = np.vstack([-0.06 + 0.01 * rng.random(n), 1e-3 * rng.random(n), -2e-3 * rng.random(n)])
+    S0 = S.copy()
+    for _ in range(2000):
+        O.multilinear_step(A, C, S)
+    v, ge, gi = _cuba_exact(2000 * DT, S0[0], S0[1], S0[2], taum)
+    np.testing.assert_allclose(S[0], v, rtol=1e-11)
+    np.testing.assert_allclose(S[1], ge, rtol=1e-11)
+    np.testing.assert_allclose(S[2], gi, rtol=1e-11)
+    # each neuron relaxes with its own time constant
+    assert len(np.unique(np.round(S[0], 12))) == n
+
+
+def test_multilinear_step_equal_parameters_is_linear_step():
+    """All neurons sharing one propagator: the per-neuron update is the shared
+    linear update bit for bit (same left fold, then + C)."""
+    A, C = O.linear_propagator(lambda x: np.array([(x[1] + x[2] - (x[0] + 49 * mV)) / (20 * ms),
+                                                   -x[1] / (5 * ms), -x[2] / (10 * ms)]), 3, DT)
+    n = 100
+    rng = np.random.default_rng(5)
+    S1 = np.vstack([-0.06 + 0.01 * rng.random(n), 1e-3 * rng.random(n), -1e-3 * rng.random(n)])
+    S2 = S1.copy()
+    Apn = np.repeat(A[:, :, None], n, axis=2)
+    Cpn = np.repeat(C[:, None], n, axis=1)
+    for _ in range(50):
+        O.linear_step(A, C, S1)
+        O.multilinear_step(Apn, Cpn, S2)
+    assert np.array_equal(S1, S2)
