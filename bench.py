@@ -179,7 +179,7 @@ def build_workload(seed=2):
 
 def apply_tuning(h, tuning, stage):
     """Apply a tuning dict: stage 'layout' before prepare, 'launch' after."""
-    keys = ("ring_bits", "ring_pack", "defer") if stage == "layout" else ("wave_max", "row_split")
+    keys = ("ring_bits", "ring_pack", "defer") if stage == "layout" else ("wave_max", "row_split", "lean")
     for k in keys:
         if tuning and k in tuning:
             h.set_tuning(k, tuning[k])

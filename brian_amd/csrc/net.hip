@@ -160,7 +160,7 @@ struct bm_net {
   int W = 1, K = 4;
   int Wcap = 1;           // largest window the rings are sized for
   bool specialize = true; // lean sweep compiled at run time for each group's shape
-  bool lean_kernel = false; // specialised groups use k_lean (the model as code) rather than k_sweep tables
+  bool lean_kernel = true; // specialised groups use k_lean (the model as code) rather than k_sweep tables
   bool use_graph = true;
   std::vector<std::pair<int, int>> mon_ids;   // (group, monitor within group)
   // tiling knobs (bm_net_set_tuning)

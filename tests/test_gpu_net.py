@@ -394,3 +394,41 @@ def test_c2_fp32_vs_fp64_rate():
     assert rate[64] > 1.0
     assert rel < 0.02
     assert abs(cv[32] - cv[64]) < 0.02
+
+
+@pytest.mark.parametrize("npt", [1, 2, 4])
+def test_lean_kernel_fp64_bit_exact(npt):
+    """The model-as-code lean kernel (k_lean: propagator, weights, reset and
+    threshold compiled in; spikes kept as per-step wavefront ballots) in
+    FP64 check mode: raster bit-exact and state within 1e-12 against the
+    oracle with a 10-step delay, over several windows and run() calls, also
+    after switching to it between runs."""
+    d = W.cuba(4000, seed=21, delay_steps=10)
+    h, g = device_cuba(d, precision=64, npt=npt)
+    net, gid = oracle_cuba(d)
+    h.run(137)
+    net.run(137)
+    h.set_tuning("lean", 1)
+    for chunk, window in ((1, 11), (500, 11), (333, 7), (1200, 11)):
+        h.set_tuning("window", window)
+        h.run(chunk)
+        net.run(chunk)
+        _state_close(h, g, net, gid, 4000)
+    s, i = h.get_spikes(g)
+    rs, ri = net.raster(gid)
+    assert len(rs) > 1000
+    assert first_divergence(s, i, rs, ri) is None
+
+
+def test_lean_kernel_fp32_rate():
+    """The lean kernel in FP32 (one fused multiply-add per delivery and per
+    non-zero propagator entry): population rate within 5% of the oracle."""
+    d = W.cuba(4000, seed=1)
+    h, g = device_cuba(d, precision=32)
+    h.set_tuning("lean", 1)
+    net, gid = oracle_cuba(d)
+    h.run(10000)
+    net.run(10000)
+    s, _ = h.get_spikes(g)
+    rs, _ = net.raster(gid)
+    assert abs(len(s) - len(rs)) / len(rs) < 0.05, (len(s), len(rs))
