@@ -763,6 +763,47 @@ def bench_fit(args):
     return out
 
 
+# ---------------------------------------------------------------- mlp
+# algorithmic flops per training sample of the learned proposal's MLP
+# (64 -> 256 -> 256 -> 8): forward 2 (64*256 + 256*256 + 256*8), backward
+# dZ2 2*8*256, dZ1 2*256*256, dW3 2*256*8, dW2 2*256*256, dW1 2*64*256
+MLP_FLOPS_SAMPLE = 2 * (64 * 256 + 256 * 256 + 256 * 8) + 2 * (8 * 256 + 256 * 256 + 256 * 8 + 256 * 256 + 64 * 256)
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_mlp(args):
+    """Training throughput of the learned proposal's MLP on synthetic
+    (features, parameters) pairs of the fitting workload's shape: 1,000,000
+    pairs, batch 1024, Adam; the same kernels as the fitting benchmark."""
+    from brian_amd import learned as L
+    n = args.train_pairs
+    epochs = args.epochs
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((n, 64)).astype(np.float32)
+    Wt = rng.standard_normal((64, 4)) * 0.2
+    U = (1 / (1 + np.exp(-(X @ Wt)))).astype(np.float32)
+    net = L.MLP(64, 256, 4, seed=7)
+    net.train(X[:65536], U[:65536], epochs=1, batch=1024, lr=1e-3, seed=1)     # warm-up, graph capture
+    t0 = time.perf_counter()
+    loss = net.train(X, U, epochs=epochs, batch=1024, lr=1e-3, seed=2)
+    el = time.perf_counter() - t0
+    net.close()
+    samples = (n // 1024) * 1024 * epochs
+    tflops = samples * MLP_FLOPS_SAMPLE / el / 1e12
+    return {
+        "metric": METRIC, "value": samples / el, "unit": "training samples/s", "n_gpus": 1,
+        "steps": (n // 1024) * epochs, "warmup": 64, "ms_per_step": el * 1e3 / ((n // 1024) * epochs),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": f"learned-proposal MLP training: 64 -> 256 -> 256 -> 8, {n} pairs, batch 1024, "
+                               f"{epochs} epochs, Adam", "pairs": n, "epochs": epochs, "final_loss": loss},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / BF16_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_mlp_rows + k_mlp_wgrad + k_mlp_adam (one training step)",
+                     "flops_per_sample": MLP_FLOPS_SAMPLE},
+        "cpu_baseline": None,
+    }
+
+
 def cpu_baseline_fit(budget_s):
     """numpy oracle of the fitting loop (oracle/fitting.run_fit), 8 targets x
     256 candidates, as many steps as fit the budget."""
@@ -808,7 +849,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", choices=("cuba1m", "uneven", "stdp", "fit"), default="cuba1m")
+    ap.add_argument("--workload", choices=("cuba1m", "uneven", "stdp", "fit", "mlp"), default="cuba1m")
     ap.add_argument("--targets", type=int, default=2048)
     ap.add_argument("--popsize", type=int, default=10000)
     ap.add_argument("--plain-iters", type=int, default=8)
@@ -838,7 +879,8 @@ def main():
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
               "(the sharding design is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
-    fn = {"cuba1m": bench_cuba1m, "uneven": bench_uneven, "stdp": bench_stdp, "fit": bench_fit}[args.workload]
+    fn = {"cuba1m": bench_cuba1m, "uneven": bench_uneven, "stdp": bench_stdp, "fit": bench_fit,
+          "mlp": bench_mlp}[args.workload]
     print(json.dumps(fn(args)), flush=True)
 
 

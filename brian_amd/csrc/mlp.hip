@@ -5,9 +5,21 @@
 // brian/library/modelfitting/modelfitting.py:232-518); plain PSO remains
 // the fallback.
 //
-// Every product runs on the matrix cores: BF16 operands, FP32 accumulation,
-// v_mfma_f32_32x32x8_bf16 on 32x32 output tiles, one wavefront per tile.
-// Master weights and Adam moments are FP32; BF16 copies feed the GEMMs.
+// Every product runs on gfx950's matrix cores (v_mfma_f32_32x32x16_bf16:
+// BF16 operands, FP32 accumulation).  One training step is three kernels:
+//   k_mlp_rows   one workgroup per 32 batch rows runs the whole forward
+//                pass, the loss and the backward pass down to dZ1 with the
+//                activations in LDS; weights stream from L2 as BF16 with the
+//                reduction dimension contiguous.  It writes the operands of
+//                the weight gradients transposed (feature-major, batch
+//                contiguous) for the next kernel.
+//   k_mlp_wgrad  dW = P^T Q and the bias sums over slices of the batch, one
+//                wavefront per 32x32 output tile and slice, partial sums per
+//                slice (no atomics: the reduction order is fixed).
+//   k_mlp_adam   sums the slices, applies Adam to the FP32 master weights
+//                and refreshes the BF16 copies the other kernels read.
+// The step counter and the batch position live on the device, so a chunk
+// of steps is captured once in a hipGraph and replayed.
 #include "common.hpp"
 #include "../../include/brianmi.h"
 
@@ -18,8 +30,14 @@
 
 namespace bm {
 
-typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4_t;
-typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16_t;
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int MR = 32;          // batch rows per k_mlp_rows workgroup
+constexpr int NO = 32;          // padded output width (2 nu <= 32)
+constexpr int MAXNI = 256, MAXNH = 256;
+constexpr int WG_SLICES = 8;    // batch slices of the weight-gradient kernel
 
 __host__ __device__ inline uint16_t f2bf(float f) {
   // round to nearest even (finite inputs)
@@ -32,105 +50,319 @@ __host__ __device__ inline float bf2f(uint16_t b) {
   return __builtin_bit_cast(float, (uint32_t)b << 16);
 }
 
-// EPI_RELUB applies the ReLU to the BF16 copy only (FP32 keeps the
-// pre-activation for the backward mask).
-enum Epi { EPI_NONE = 0, EPI_BIAS = 1, EPI_RELU = 2, EPI_MASK = 4, EPI_RELUB = 8 };
+__device__ inline bf16x8_t ld8(const uint16_t *p) {
+  return __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t *>(p));
+}
 
-// C[M x N] = op(A) . op(B); A is [M x K] (or [K x M] when ta), B is [K x N]
-// (or [N x K] when tb), all BF16 row-major with leading dimensions lda/ldb.
-// Epilogue: + bias[n] (EPI_BIAS), ReLU (EPI_RELU), times (mask[m][n] > 0)
-// (EPI_MASK); writes FP32 C (if non-null) and a BF16 copy Cb (if non-null).
-// M, N multiples of 32; K multiple of 8.
-__global__ __launch_bounds__(64) void k_gemm_bf16(const uint16_t *__restrict__ A, int lda, int ta,
-                                                  const uint16_t *__restrict__ B, int ldb, int tb,
-                                                  float *C, uint16_t *Cb, int ldc, int M, int N, int K,
-                                                  const float *__restrict__ bias,
-                                                  const float *__restrict__ mask, int epi) {
-  const int lane = threadIdx.x;
-  const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
-  const int r = lane & 31, kg = (lane >> 5) * 4;
-  f32x16_t acc = {};
-  for (int k0 = 0; k0 < K; k0 += 8) {
-    bf16x4_t a, b;
+// acc += A[32 x K] . B[K x 32]: A row-major in LDS (row stride lda), B given
+// as Bt[n][k] (k contiguous, row stride ldb) starting at output column n0.
+// Lane layout of v_mfma_f32_32x32x16_bf16: lane l supplies row l % 32 of A
+// and column l % 32 of B at k = 8 (l / 32) .. +7 of each 16-wide K block.
+__device__ inline void tile_mma(f32x16_t &acc, const uint16_t *A, int lda, const uint16_t *Bt, int ldb, int n0,
+                                int K, int lane) {
+  const int r = lane & 31, kh = (lane >> 5) * 8;
+  const uint16_t *a = A + r * lda + kh;
+  const uint16_t *b = Bt + (size_t)(n0 + r) * ldb + kh;
+#pragma unroll 4
+  for (int k0 = 0; k0 < K; k0 += 16)
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(a + k0), ld8(b + k0), acc, 0, 0, 0);
+}
+// row and column of accumulator element q of a 32x32 tile
+__device__ inline int acc_row(int q, int lane) { return 8 * (q >> 2) + (lane >> 5) * 4 + (q & 3); }
+
+struct MlpDev {
+  int nin, ni, nh, nu, nout;          // ni: nin rounded up to 32
+  int64_t off[7];                     // master parameter offsets (W1 b1 W2 b2 W3 b3 end)
+  const float *p;                     // FP32 master parameters
+  const uint16_t *w1t, *w2t, *w3t;    // BF16 [nh][ni], [nh][nh], [NO][nh] (forward: k = input)
+  const uint16_t *w2, *w3;            // BF16 [nh][nh], [nh][NO] (backward: k = output)
+};
+
+// LDS row strides (BF16 elements): 8 elements of padding keep the 16-byte
+// operand reads aligned and spread rows over the banks
+__host__ __device__ inline int lds_stride(int n) { return n + 8; }
+
+// One workgroup: rows [r0, r0 + 32) of the batch.  X rows are gathered
+// through idx (FP32 [n][nin]).  TRAIN: loss per row, and the transposed
+// operands of the weight gradients, feature-major with the batch (ldb
+// columns) contiguous: XT [ni][B], A1T / A2T / dZ2T / dZ1T [nh][B], dOT [NO][B].
+// Forward only: the nout outputs per row into out [B][nout].
+template <bool TRAIN>
+__global__ __launch_bounds__(256) void k_mlp_rows(const MlpDev m, const float *__restrict__ X,
+                                                  const float *__restrict__ U, const int *__restrict__ idx,
+                                                  const int64_t *__restrict__ d_batch, int B, float *out,
+                                                  uint16_t *XT, uint16_t *A1T, uint16_t *A2T, uint16_t *dOT,
+                                                  uint16_t *dZ2T, uint16_t *dZ1T, float *loss,
+                                                  int64_t *d_step) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ni = m.ni, nh = m.nh;
+  const int sx = lds_stride(ni), sh = lds_stride(nh), so = lds_stride(NO);
+  uint16_t *sX = reinterpret_cast<uint16_t *>(smem);          // [32][sx]
+  uint16_t *sA1 = sX + MR * sx;                                // [32][sh]
+  uint16_t *sA2 = sA1 + MR * sh;                               // [32][sh], later dZ2
+  uint16_t *sZ1 = sA2 + MR * sh;                               // [32][sh] dZ1 (TRAIN)
+  uint16_t *sdO = sZ1 + MR * sh;                               // [32][so]
+  float *sO = reinterpret_cast<float *>(sdO + MR * so);        // [32][NO]
+  const int r0 = blockIdx.x * MR;
+  const int64_t boff = (TRAIN && d_batch) ? *d_batch * (int64_t)B : 0;
+  if (TRAIN && d_step && blockIdx.x == 0 && tid == 0) *d_step += 1;   // Adam step of this batch
+
+  // gather the rows (BF16), zero padding columns
+  for (int e = tid; e < MR * ni; e += 256) {
+    const int r = e / ni, k = e % ni;
+    const int64_t src = idx ? idx[boff + r0 + r] : (int64_t)(r0 + r);
+    sX[r * sx + k] = k < m.nin ? f2bf(X[src * m.nin + k]) : (uint16_t)0;
+  }
+  __syncthreads();
+  const float *b1 = m.p + m.off[1], *b2 = m.p + m.off[3], *b3 = m.p + m.off[5];
+  // layer 1: A1 = relu(X W1 + b1) (BF16; A1 > 0 is the backward mask)
+  for (int t = wave; t < nh / 32; t += 4) {
+    f32x16_t acc = {};
+    tile_mma(acc, sX, sx, m.w1t, ni, t * 32, ni, lane);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int k = k0 + kg + v;
-      const int m = m0 + r, n = n0 + r;
-      a[v] = (short)(ta ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k]);
-      b[v] = (short)(tb ? B[(size_t)n * ldb + k] : B[(size_t)k * ldb + n]);
+    for (int q = 0; q < 16; ++q) {
+      const int c = t * 32 + (lane & 31);
+      const float z = acc[q] + b1[c];
+      sA1[acc_row(q, lane) * sh + c] = f2bf(z > 0.f ? z : 0.f);
     }
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
   }
+  __syncthreads();
+  // layer 2
+  for (int t = wave; t < nh / 32; t += 4) {
+    f32x16_t acc = {};
+    tile_mma(acc, sA1, sh, m.w2t, nh, t * 32, nh, lane);
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int m = m0 + 8 * (q / 4) + (lane >> 5) * 4 + (q % 4);
-    const int n = n0 + r;
-    float x = acc[q];
-    if (epi & EPI_BIAS) x += bias[n];
-    if (epi & EPI_RELU) x = x > 0.f ? x : 0.f;
-    if (epi & EPI_MASK) x = mask[(size_t)m * ldc + n] > 0.f ? x : 0.f;
-    if (C) C[(size_t)m * ldc + n] = x;
-    if (Cb) Cb[(size_t)m * ldc + n] = f2bf((epi & EPI_RELUB) ? (x > 0.f ? x : 0.f) : x);
+    for (int q = 0; q < 16; ++q) {
+      const int c = t * 32 + (lane & 31);
+      const float z = acc[q] + b2[c];
+      sA2[acc_row(q, lane) * sh + c] = f2bf(z > 0.f ? z : 0.f);
+    }
+  }
+  __syncthreads();
+  // layer 3: O = A2 W3 + b3 (one tile)
+  if (wave == 0) {
+    f32x16_t acc = {};
+    tile_mma(acc, sA2, sh, m.w3t, nh, 0, nh, lane);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = lane & 31;
+      sO[acc_row(q, lane) * NO + c] = acc[q] + b3[c];
+    }
+  }
+  __syncthreads();
+  if constexpr (!TRAIN) {
+    for (int e = tid; e < MR * m.nout; e += 256) {
+      const int r = e / m.nout, k = e % m.nout;
+      if (r0 + r < B) out[(int64_t)(r0 + r) * m.nout + k] = sO[r * NO + k];
+    }
+    return;
+  } else {
+    // Gaussian NLL over nu outputs: O = [mu | s], sigma = exp(s) with s
+    // clamped to [-7, 3]; dO (BF16) scaled by 1/B
+    if (tid < MR) {
+      const int r = tid;
+      const int64_t src = idx ? idx[boff + r0 + r] : (int64_t)(r0 + r);
+      float L = 0.f;
+      const float invB = 1.f / (float)B;
+      for (int k = 0; k < NO; ++k) sdO[r * so + k] = 0;
+      for (int k = 0; k < m.nu; ++k) {
+        const float mu = sO[r * NO + k];
+        const float sr = sO[r * NO + m.nu + k];
+        const float s = fminf(fmaxf(sr, -7.f), 3.f);
+        const float iv = expf(-2.f * s);
+        const float e = U[src * m.nu + k] - mu;
+        L += 0.5f * e * e * iv + s;
+        sdO[r * so + k] = f2bf(-e * iv * invB);
+        const float ds = (sr > -7.f && sr < 3.f) ? (1.f - e * e * iv) : 0.f;
+        sdO[r * so + m.nu + k] = f2bf(ds * invB);
+      }
+      // the loss is indexed by the row's position in the batch
+      loss[r0 + r] = L;
+    }
+    // transposed operands of the weight gradients: thread = feature, 32
+    // consecutive batch columns (64 bytes) per store group
+    auto store_t = [&](const uint16_t *S, int stride, int nfeat, uint16_t *dst) {
+      for (int f = tid; f < nfeat; f += 256) {
+        u32x4_t *d = reinterpret_cast<u32x4_t *>(dst + (int64_t)f * B + r0);
+#pragma unroll
+        for (int c = 0; c < MR / 8; ++c) {
+          u32x4_t wv;
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            wv[q] = (uint32_t)S[(8 * c + 2 * q) * stride + f] | ((uint32_t)S[(8 * c + 2 * q + 1) * stride + f] << 16);
+          d[c] = wv;
+        }
+      }
+    };
+    store_t(sX, sx, ni, XT);
+    store_t(sA1, sh, nh, A1T);
+    store_t(sA2, sh, nh, A2T);
+    __syncthreads();
+    store_t(sdO, so, NO, dOT);
+    // dZ2 = (dO W3^T) * (A2 > 0), into the A2 buffer (each element's mask
+    // is read by the lane that overwrites it)
+    for (int t = wave; t < nh / 32; t += 4) {
+      f32x16_t acc = {};
+      tile_mma(acc, sdO, so, m.w3, NO, t * 32, NO, lane);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int at = acc_row(q, lane) * sh + t * 32 + (lane & 31);
+        sA2[at] = sA2[at] != 0 ? f2bf(acc[q]) : (uint16_t)0;
+      }
+    }
+    __syncthreads();
+    store_t(sA2, sh, nh, dZ2T);
+    // dZ1 = (dZ2 W2^T) * (A1 > 0)
+    for (int t = wave; t < nh / 32; t += 4) {
+      f32x16_t acc = {};
+      tile_mma(acc, sA2, sh, m.w2, nh, t * 32, nh, lane);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int at = acc_row(q, lane) * sh + t * 32 + (lane & 31);
+        sZ1[at] = sA1[at] != 0 ? f2bf(acc[q]) : (uint16_t)0;
+      }
+    }
+    __syncthreads();
+    store_t(sZ1, sh, nh, dZ1T);
   }
 }
 
-__global__ void k_cast_bf16(const float *__restrict__ x, uint16_t *y, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) y[i] = f2bf(x[i]);
-}
+// Weight-gradient tiles: task = (matrix, 32x32 tile, batch slice), one
+// wavefront each, writing partial sums part[slice][param].  Bias sums are
+// handled by the trailing workgroups: one thread per bias and slice.
+struct WgradArgs {
+  const uint16_t *XT, *A1T, *A2T, *dOT, *dZ2T, *dZ1T;
+  float *part;                 // [WG_SLICES][nparams]
+  int64_t nparams;
+  int B, ntiles1, ntiles2, ntiles3, nbias_threads;
+};
 
-// Gaussian NLL over nu outputs: O = [mu (nu) | s (nu) | padding], sigma = exp(s)
-// with s clamped to [-7, 3].  Writes dO (BF16, scaled by 1/B) and per-sample loss.
-__global__ void k_nll(const float *__restrict__ O, int ldo, const float *__restrict__ U, int nu, int B,
-                      uint16_t *dO, float *loss) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= B) return;
-  float L = 0.f;
-  const float invB = 1.f / (float)B;
-  for (int k = 0; k < ldo; ++k) dO[(size_t)i * ldo + k] = 0;
-  for (int k = 0; k < nu; ++k) {
-    const float mu = O[(size_t)i * ldo + k];
-    const float sr = O[(size_t)i * ldo + nu + k];
-    const float s = fminf(fmaxf(sr, -7.f), 3.f);
-    const float iv = expf(-2.f * s);
-    const float e = U[(size_t)i * nu + k] - mu;
-    L += 0.5f * e * e * iv + s;
-    dO[(size_t)i * ldo + k] = f2bf(-e * iv * invB);
-    const float ds = (sr > -7.f && sr < 3.f) ? (1.f - e * e * iv) : 0.f;
-    dO[(size_t)i * ldo + nu + k] = f2bf(ds * invB);
+__global__ __launch_bounds__(256) void k_mlp_wgrad(const MlpDev m, const WgradArgs a) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int ks = a.B / WG_SLICES;
+  const int ntask = (a.ntiles1 + a.ntiles2 + a.ntiles3) * WG_SLICES;
+  const int task = blockIdx.x * 4 + (tid >> 6);
+  if (task < ntask) {
+    const int slice = task % WG_SLICES;
+    int t = task / WG_SLICES;
+    const uint16_t *P, *Q;
+    int np_out, rows, cols;     // matrix [rows][cols] at offset
+    int64_t off;
+    if (t < a.ntiles1) {
+      P = a.XT; Q = a.dZ1T; rows = m.nin; cols = m.nh; off = m.off[0]; np_out = m.nh / 32;
+    } else if ((t -= a.ntiles1) < a.ntiles2) {
+      P = a.A1T; Q = a.dZ2T; rows = m.nh; cols = m.nh; off = m.off[2]; np_out = m.nh / 32;
+    } else {
+      t -= a.ntiles2;
+      P = a.A2T; Q = a.dOT; rows = m.nh; cols = NO; off = m.off[4]; np_out = 1;
+    }
+    const int ti = t / np_out, tj = t % np_out;
+    // C[i][j] = sum_b P[i][b] Q[j][b] over the slice: A = P rows (k = b), Bt = Q rows
+    f32x16_t acc = {};
+    const int r = lane & 31, kh = (lane >> 5) * 8;
+    const uint16_t *pa = P + (int64_t)(ti * 32 + r) * a.B + slice * ks + kh;
+    const uint16_t *pb = Q + (int64_t)(tj * 32 + r) * a.B + slice * ks + kh;
+#pragma unroll 4
+    for (int k0 = 0; k0 < ks; k0 += 16)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(pa + k0), ld8(pb + k0), acc, 0, 0, 0);
+    float *dst = a.part + (int64_t)slice * a.nparams + off;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int i = ti * 32 + acc_row(q, lane), j = tj * 32 + (lane & 31);
+      if (i < rows) dst[(int64_t)i * cols + j] = acc[q];
+    }
+    return;
   }
-  loss[i] = L;
-}
-
-// column sums of a BF16 [B x N] matrix into FP32 g[N]
-__global__ void k_colsum_bf16(const uint16_t *__restrict__ X, int B, int N, float *g) {
-  const int n = blockIdx.x * 256 + threadIdx.x;
-  if (n >= N) return;
+  // bias sums: thread -> (slice, bias index over b1 | b2 | b3)
+  const int bt = (blockIdx.x - (ntask + 3) / 4) * 256 + tid;
+  if (bt < 0 || bt >= a.nbias_threads) return;
+  const int nb = 2 * m.nh + NO;
+  const int slice = bt / nb;
+  int j = bt % nb;
+  const uint16_t *Q;
+  int64_t off;
+  if (j < m.nh) { Q = a.dZ1T; off = m.off[1]; }
+  else if ((j -= m.nh) < m.nh) { Q = a.dZ2T; off = m.off[3]; }
+  else { j -= m.nh; Q = a.dOT; off = m.off[5]; }
+  const uint16_t *q = Q + (int64_t)j * a.B + slice * ks;
   float s = 0.f;
-  for (int i = 0; i < B; ++i) s += bf2f(X[(size_t)i * N + n]);
-  g[n] = s;
+  for (int b = 0; b < ks; b += 8) {
+    const u32x4_t v = *reinterpret_cast<const u32x4_t *>(q + b);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s += bf2f((uint16_t)(v[c] & 0xFFFFu)) + bf2f((uint16_t)(v[c] >> 16));
+  }
+  a.part[(int64_t)slice * a.nparams + off + j] = s;
 }
 
-__global__ void k_adam(float *w, const float *__restrict__ g, float *m, float *v, int64_t n, float lr,
-                       float b1, float b2, float c1, float c2, float eps) {
+// Sums the slices (fixed order) into the gradient, then (adam) the Adam
+// step with bias correction for the step counted on the device and the
+// BF16 copies of the weights.  Without adam the gradient is written to g.
+struct AdamArgs {
+  float *p, *g, *m, *v;
+  const float *part;
+  int64_t n;
+  float lr, b1, b2, eps;
+  const int64_t *d_step;
+  int64_t *d_batch;            // advanced by one (next batch) when non-null
+  uint16_t *w1t, *w2t, *w3t, *w2, *w3;
+  int adam;
+};
+
+__global__ __launch_bounds__(256) void k_mlp_adam(const MlpDev md, const AdamArgs a) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.f - b1) * gi;
-  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  if (a.d_batch && i == 0) *a.d_batch += 1;
+  if (i >= a.n) return;
+  float gi = 0.f;
+#pragma unroll
+  for (int s = 0; s < WG_SLICES; ++s) gi += a.part[(int64_t)s * a.n + i];
+  if (!a.adam) {
+    a.g[i] = gi;
+    return;
+  }
+  const float t = (float)*a.d_step;
+  const float c1 = 1.f - powf(a.b1, t), c2 = 1.f - powf(a.b2, t);
+  const float mi = a.b1 * a.m[i] + (1.f - a.b1) * gi;
+  const float vi = a.b2 * a.v[i] + (1.f - a.b2) * gi * gi;
+  a.m[i] = mi;
+  a.v[i] = vi;
+  const float w = a.p[i] - a.lr * (mi / c1) / (sqrtf(vi / c2) + a.eps);
+  a.p[i] = w;
+  const uint16_t wb = f2bf(w);
+  // BF16 copies: forward layouts are [out][in], backward [in][out]
+  if (i < md.off[1]) {
+    const int64_t r = i / md.nh, c = i % md.nh;       // W1[r][c]
+    a.w1t[c * md.ni + r] = wb;
+  } else if (i >= md.off[2] && i < md.off[3]) {
+    const int64_t q = i - md.off[2], r = q / md.nh, c = q % md.nh;
+    a.w2t[c * md.nh + r] = wb;
+    a.w2[q] = wb;
+  } else if (i >= md.off[4] && i < md.off[5]) {
+    const int64_t q = i - md.off[4], r = q / NO, c = q % NO;
+    a.w3t[c * md.nh + r] = wb;
+    a.w3[q] = wb;
+  }
 }
 
-// gathers batch rows idx[0..B) of X (FP32 [n x nin]) into BF16 and of U
-__global__ void k_gather(const float *__restrict__ X, const float *__restrict__ U, const int *__restrict__ idx,
-                         int B, int nin, int nu, uint16_t *Xb, float *Ub) {
-  const int i = blockIdx.x;
-  if (i >= B) return;
-  const int64_t src = idx[i];
-  for (int k = threadIdx.x; k < nin; k += blockDim.x) Xb[(size_t)i * nin + k] = f2bf(X[src * nin + k]);
-  for (int k = threadIdx.x; k < nu; k += blockDim.x) Ub[(size_t)i * nu + k] = U[src * nu + k];
+// BF16 copies of the master weights in both layouts (W1T padding rows zero)
+__global__ void k_mlp_cast(const MlpDev md, uint16_t *w1t, uint16_t *w2t, uint16_t *w3t, uint16_t *w2,
+                           uint16_t *w3) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const float *p = md.p;
+  if (i < (int64_t)md.nh * md.ni) {
+    const int64_t c = i / md.ni, r = i % md.ni;       // w1t[c][r] = W1[r][c]
+    w1t[i] = r < md.nin ? f2bf(p[md.off[0] + r * md.nh + c]) : (uint16_t)0;
+  }
+  if (i < (int64_t)md.nh * md.nh) {
+    w2[i] = f2bf(p[md.off[2] + i]);
+    const int64_t c = i / md.nh, r = i % md.nh;       // w2t[c][r] = W2[r][c]
+    w2t[i] = f2bf(p[md.off[2] + r * md.nh + c]);
+  }
+  if (i < (int64_t)md.nh * NO) {
+    w3[i] = f2bf(p[md.off[4] + i]);
+    const int64_t c = i / md.nh, r = i % md.nh;       // w3t[c][r] = W3[r][c]
+    w3t[i] = f2bf(p[md.off[4] + r * NO + c]);
+  }
 }
 
 }  // namespace bm
@@ -138,115 +370,131 @@ __global__ void k_gather(const float *__restrict__ X, const float *__restrict__ 
 using namespace bm;
 
 struct bm_mlp {
-  int nin = 64, nh = 256, nout = 8, nu = 4, npad = 32;
-  // parameters (FP32 master) as one flat buffer: W1 b1 W2 b2 W3 b3 (W3/b3 padded to npad)
+  int nin = 64, ni = 64, nh = 256, nout = 8, nu = 4;
+  // parameters (FP32 master) as one flat buffer: W1 b1 W2 b2 W3 b3 (W3/b3 padded to NO)
   int64_t off[7] = {};
-  DevBuf p, g, m, v, wb;           // params, grads, adam m, adam v, bf16 weights (W1, W2, W3)
+  DevBuf p, g, m, v;
+  DevBuf w1t, w2t, w3t, w2, w3;     // BF16 copies
   DevBuf data_x, data_u;
   int64_t ndata = 0;
-  // activations for the current batch capacity
+  // per-batch operands for the batch capacity
   int cap = 0;
-  DevBuf Xb, Ub, Z1, A1b, Z2, A2b, O, dOb, dZ2b, dZ1b, loss, idx;
+  DevBuf xb, ub, outb, XT, A1T, A2T, dOT, dZ2T, dZ1T, loss, part, idx;
+  DevBuf d_step, d_batch;           // int64 counters
   int64_t step = 0;
   hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_batch = 0, graph_steps = 0;
+  float graph_lr = 0.f;
   ~bm_mlp() {
+    if (graph) (void)hipGraphExecDestroy(graph);
     if (stream) (void)hipStreamDestroy(stream);
   }
-  float *P() { return p.as<float>(); }
-  float *G() { return g.as<float>(); }
-  uint16_t *W1b() { return wb.as<uint16_t>(); }
-  uint16_t *W2b() { return wb.as<uint16_t>() + (size_t)nin * nh; }
-  uint16_t *W3b() { return wb.as<uint16_t>() + (size_t)nin * nh + (size_t)nh * nh; }
+  MlpDev dev() const {
+    MlpDev d{};
+    d.nin = nin;
+    d.ni = ni;
+    d.nh = nh;
+    d.nu = nu;
+    d.nout = nout;
+    for (int q = 0; q < 7; ++q) d.off[q] = off[q];
+    d.p = p.as<float>();
+    d.w1t = w1t.as<uint16_t>();
+    d.w2t = w2t.as<uint16_t>();
+    d.w3t = w3t.as<uint16_t>();
+    d.w2 = w2.as<uint16_t>();
+    d.w3 = w3.as<uint16_t>();
+    return d;
+  }
 };
 
 namespace {
 
 int64_t nparams(const bm_mlp *h) {
-  return (int64_t)h->nin * h->nh + h->nh + (int64_t)h->nh * h->nh + h->nh + (int64_t)h->nh * h->npad + h->npad;
+  return (int64_t)h->nin * h->nh + h->nh + (int64_t)h->nh * h->nh + h->nh + (int64_t)h->nh * NO + NO;
+}
+
+size_t rows_shmem(const bm_mlp *h) {
+  return (size_t)MR * (lds_stride(h->ni) + 3 * lds_stride(h->nh) + lds_stride(NO)) * 2 + (size_t)MR * NO * 4;
+}
+
+void drop_graph(bm_mlp *h) {
+  if (h->graph) (void)hipGraphExecDestroy(h->graph);
+  h->graph = nullptr;
+  h->graph_steps = 0;
 }
 
 void ensure_cap(bm_mlp *h, int B) {
   if (B <= h->cap) return;
-  const int c = (B + 31) / 32 * 32;
-  h->Xb.alloc((size_t)c * h->nin * 2);
-  h->Ub.alloc((size_t)c * h->nu * 4);
-  h->Z1.alloc((size_t)c * h->nh * 4);
-  h->A1b.alloc((size_t)c * h->nh * 2);
-  h->Z2.alloc((size_t)c * h->nh * 4);
-  h->A2b.alloc((size_t)c * h->nh * 2);
-  h->O.alloc((size_t)c * h->npad * 4);
-  h->dOb.alloc((size_t)c * h->npad * 2);
-  h->dZ2b.alloc((size_t)c * h->nh * 2);
-  h->dZ1b.alloc((size_t)c * h->nh * 2);
+  drop_graph(h);           // a captured training chunk holds the old buffers
+  const int c = (B + 255) / 256 * 256;
+  h->xb.alloc((size_t)c * h->nin * 4);
+  h->ub.alloc((size_t)c * h->nu * 4);
+  h->outb.alloc((size_t)c * h->nout * 4);
+  h->XT.alloc((size_t)c * h->ni * 2);
+  h->A1T.alloc((size_t)c * h->nh * 2);
+  h->A2T.alloc((size_t)c * h->nh * 2);
+  h->dOT.alloc((size_t)c * NO * 2);
+  h->dZ2T.alloc((size_t)c * h->nh * 2);
+  h->dZ1T.alloc((size_t)c * h->nh * 2);
   h->loss.alloc((size_t)c * 4);
-  h->idx.alloc((size_t)c * 4);
   h->cap = c;
 }
 
-void gemm(bm_mlp *h, const uint16_t *A, int lda, int ta, const uint16_t *B, int ldb, int tb, float *C,
-          uint16_t *Cb, int ldc, int M, int N, int K, const float *bias, const float *mask, int epi) {
-  BM_REQUIRE(M % 32 == 0 && N % 32 == 0 && K % 8 == 0, "GEMM shape must be 32x32x8-aligned");
-  hipLaunchKernelGGL(k_gemm_bf16, dim3(N / 32, M / 32), dim3(64), 0, h->stream, A, lda, ta, B, ldb, tb, C, Cb,
-                     ldc, M, N, K, bias, mask, epi);
+void cast_weights(bm_mlp *h) {
+  const int64_t n = std::max<int64_t>((int64_t)h->nh * h->ni, (int64_t)h->nh * h->nh);
+  hipLaunchKernelGGL(k_mlp_cast, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->dev(),
+                     h->w1t.as<uint16_t>(), h->w2t.as<uint16_t>(), h->w3t.as<uint16_t>(), h->w2.as<uint16_t>(),
+                     h->w3.as<uint16_t>());
 }
 
-void refresh_bf16(bm_mlp *h) {
-  const float *P = h->P();
-  auto cast = [&](const float *src, uint16_t *dst, int64_t n) {
-    hipLaunchKernelGGL(k_cast_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, src, dst, n);
-  };
-  cast(P + h->off[0], h->W1b(), (int64_t)h->nin * h->nh);
-  cast(P + h->off[2], h->W2b(), (int64_t)h->nh * h->nh);
-  cast(P + h->off[4], h->W3b(), (int64_t)h->nh * h->npad);
-}
-
-// forward on the batch already in Xb (B rows, B multiple of 32)
-void forward(bm_mlp *h, int B) {
-  const float *P = h->P();
-  // Z1 (FP32, pre-activation) and A1 = bf16(relu(Z1)) from one GEMM
-  gemm(h, h->Xb.as<uint16_t>(), h->nin, 0, h->W1b(), h->nh, 0, h->Z1.as<float>(), h->A1b.as<uint16_t>(), h->nh,
-       B, h->nh, h->nin, P + h->off[1], nullptr, EPI_BIAS | EPI_RELUB);
-  gemm(h, h->A1b.as<uint16_t>(), h->nh, 0, h->W2b(), h->nh, 0, h->Z2.as<float>(), h->A2b.as<uint16_t>(), h->nh,
-       B, h->nh, h->nh, P + h->off[3], nullptr, EPI_BIAS | EPI_RELUB);
-  gemm(h, h->A2b.as<uint16_t>(), h->nh, 0, h->W3b(), h->npad, 0, h->O.as<float>(), nullptr, h->npad, B, h->npad,
-       h->nh, P + h->off[5], nullptr, EPI_BIAS);
-}
-
-// loss + gradients into G for the batch in Xb/Ub
-void backward(bm_mlp *h, int B) {
-  float *G = h->G();
-  hipLaunchKernelGGL(k_nll, dim3((B + 255) / 256), dim3(256), 0, h->stream, h->O.as<float>(), h->npad,
-                     h->Ub.as<float>(), h->nu, B, h->dOb.as<uint16_t>(), h->loss.as<float>());
-  // layer 3: dW3 = A2^T dO ; db3 = colsum(dO) ; dZ2 = (dO W3^T) * (Z2 > 0)
-  gemm(h, h->A2b.as<uint16_t>(), h->nh, 1, h->dOb.as<uint16_t>(), h->npad, 0, G + h->off[4], nullptr, h->npad,
-       h->nh, h->npad, B, nullptr, nullptr, EPI_NONE);
-  hipLaunchKernelGGL(k_colsum_bf16, dim3((h->npad + 255) / 256), dim3(256), 0, h->stream, h->dOb.as<uint16_t>(),
-                     B, h->npad, G + h->off[5]);
-  gemm(h, h->dOb.as<uint16_t>(), h->npad, 0, h->W3b(), h->npad, 1, nullptr, h->dZ2b.as<uint16_t>(), h->nh, B,
-       h->nh, h->npad, nullptr, h->Z2.as<float>(), EPI_MASK);
-  // layer 2
-  gemm(h, h->A1b.as<uint16_t>(), h->nh, 1, h->dZ2b.as<uint16_t>(), h->nh, 0, G + h->off[2], nullptr, h->nh, h->nh,
-       h->nh, B, nullptr, nullptr, EPI_NONE);
-  hipLaunchKernelGGL(k_colsum_bf16, dim3((h->nh + 255) / 256), dim3(256), 0, h->stream, h->dZ2b.as<uint16_t>(),
-                     B, h->nh, G + h->off[3]);
-  gemm(h, h->dZ2b.as<uint16_t>(), h->nh, 0, h->W2b(), h->nh, 1, nullptr, h->dZ1b.as<uint16_t>(), h->nh, B, h->nh,
-       h->nh, nullptr, h->Z1.as<float>(), EPI_MASK);
-  // layer 1
-  gemm(h, h->Xb.as<uint16_t>(), h->nin, 1, h->dZ1b.as<uint16_t>(), h->nh, 0, G + h->off[0], nullptr, h->nh, h->nin,
-       h->nh, B, nullptr, nullptr, EPI_NONE);
-  hipLaunchKernelGGL(k_colsum_bf16, dim3((h->nh + 255) / 256), dim3(256), 0, h->stream, h->dZ1b.as<uint16_t>(),
-                     B, h->nh, G + h->off[1]);
-}
-
-void upload_batch(bm_mlp *h, const float *X, const float *U, int B, int Bp) {
-  std::vector<uint16_t> xb((size_t)Bp * h->nin, 0);
-  for (size_t i = 0; i < (size_t)B * h->nin; ++i) xb[i] = f2bf(X[i]);
-  BM_HIP(hipMemcpyAsync(h->Xb.p, xb.data(), xb.size() * 2, hipMemcpyHostToDevice, h->stream));
-  if (U) {
-    std::vector<float> ub((size_t)Bp * h->nu, 0.f);
-    std::copy(U, U + (size_t)B * h->nu, ub.begin());
-    BM_HIP(hipMemcpyAsync(h->Ub.p, ub.data(), ub.size() * 4, hipMemcpyHostToDevice, h->stream));
-  }
+// forward + loss + backward of one batch of B rows (B multiple of 256 for
+// the batch slices, rows gathered from X through idx or taken in order),
+// then either Adam (adam = 1, lr) or the summed gradient into g.
+void step(bm_mlp *h, const float *X, const float *U, const int *idx, const int64_t *d_batch, int B, int adam,
+          float lr) {
+  const MlpDev md = h->dev();
+  hipLaunchKernelGGL(k_mlp_rows<true>, dim3(B / MR), dim3(256), rows_shmem(h), h->stream, md, X, U, idx, d_batch,
+                     B, nullptr, h->XT.as<uint16_t>(), h->A1T.as<uint16_t>(), h->A2T.as<uint16_t>(),
+                     h->dOT.as<uint16_t>(), h->dZ2T.as<uint16_t>(), h->dZ1T.as<uint16_t>(), h->loss.as<float>(),
+                     adam ? h->d_step.as<int64_t>() : nullptr);
+  WgradArgs w{};
+  w.XT = h->XT.as<uint16_t>();
+  w.A1T = h->A1T.as<uint16_t>();
+  w.A2T = h->A2T.as<uint16_t>();
+  w.dOT = h->dOT.as<uint16_t>();
+  w.dZ2T = h->dZ2T.as<uint16_t>();
+  w.dZ1T = h->dZ1T.as<uint16_t>();
+  w.part = h->part.as<float>();
+  w.nparams = h->off[6];
+  w.B = B;
+  w.ntiles1 = (h->ni / 32) * (h->nh / 32);
+  w.ntiles2 = (h->nh / 32) * (h->nh / 32);
+  w.ntiles3 = h->nh / 32;
+  w.nbias_threads = WG_SLICES * (2 * h->nh + NO);
+  const int ntask = (w.ntiles1 + w.ntiles2 + w.ntiles3) * WG_SLICES;
+  const int blocks = (ntask + 3) / 4 + (w.nbias_threads + 255) / 256;
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3(blocks), dim3(256), 0, h->stream, md, w);
+  AdamArgs a{};
+  a.p = h->p.as<float>();
+  a.g = h->g.as<float>();
+  a.m = h->m.as<float>();
+  a.v = h->v.as<float>();
+  a.part = h->part.as<float>();
+  a.n = h->off[6];
+  a.lr = lr;
+  a.b1 = 0.9f;
+  a.b2 = 0.999f;
+  a.eps = 1e-8f;
+  a.d_step = h->d_step.as<int64_t>();
+  a.d_batch = adam && d_batch ? const_cast<int64_t *>(d_batch) : nullptr;
+  a.w1t = h->w1t.as<uint16_t>();
+  a.w2t = h->w2t.as<uint16_t>();
+  a.w3t = h->w3t.as<uint16_t>();
+  a.w2 = h->w2.as<uint16_t>();
+  a.w3 = h->w3.as<uint16_t>();
+  a.adam = adam;
+  hipLaunchKernelGGL(k_mlp_adam, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, h->stream, md, a);
 }
 
 }  // namespace
@@ -255,35 +503,46 @@ extern "C" {
 
 int bm_mlp_create(int nin, int nh, int nu, const float *params, bm_mlp **out) {
   return guard([&] {
-    BM_REQUIRE(nin % 8 == 0 && nin > 0, "nin must be a positive multiple of 8");
-    BM_REQUIRE(nh % 32 == 0 && nh > 0, "nh must be a positive multiple of 32");
-    BM_REQUIRE(nu > 0 && 2 * nu <= 32, "nu must be in [1, 16]");
+    BM_REQUIRE(nin > 0 && nin <= MAXNI, "nin must be in [1, 256]");
+    BM_REQUIRE(nh % 32 == 0 && nh > 0 && nh <= MAXNH, "nh must be a multiple of 32 in [32, 256]");
+    BM_REQUIRE(nu > 0 && 2 * nu <= NO, "nu must be in [1, 16]");
     auto h = std::make_unique<bm_mlp>();
     h->nin = nin;
+    h->ni = (nin + 31) / 32 * 32;
     h->nh = nh;
     h->nu = nu;
     h->nout = 2 * nu;
-    h->npad = 32;
     BM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     int64_t o = 0;
     h->off[0] = o; o += (int64_t)nin * nh;
     h->off[1] = o; o += nh;
     h->off[2] = o; o += (int64_t)nh * nh;
     h->off[3] = o; o += nh;
-    h->off[4] = o; o += (int64_t)nh * h->npad;
-    h->off[5] = o; o += h->npad;
+    h->off[4] = o; o += (int64_t)nh * NO;
+    h->off[5] = o; o += NO;
     h->off[6] = o;
+    BM_REQUIRE(rows_shmem(h.get()) <= 65536, "nin and nh too large for the row kernel's 64 KiB of LDS");
     const int64_t n = o;
     h->p.alloc(n * 4);
     h->g.alloc(n * 4);
     h->m.alloc(n * 4);
     h->v.alloc(n * 4);
-    h->wb.alloc(((size_t)nin * nh + (size_t)nh * nh + (size_t)nh * h->npad) * 2);
+    h->part.alloc((size_t)WG_SLICES * n * 4);
+    BM_HIP(hipMemsetAsync(h->part.p, 0, (size_t)WG_SLICES * n * 4, h->stream));
+    h->w1t.alloc((size_t)nh * h->ni * 2);
+    h->w2t.alloc((size_t)nh * nh * 2);
+    h->w3t.alloc((size_t)NO * nh * 2);
+    h->w2.alloc((size_t)nh * nh * 2);
+    h->w3.alloc((size_t)nh * NO * 2);
+    h->d_step.alloc(8);
+    h->d_batch.alloc(8);
     BM_HIP(hipMemcpyAsync(h->p.p, params, n * 4, hipMemcpyHostToDevice, h->stream));
     BM_HIP(hipMemsetAsync(h->m.p, 0, n * 4, h->stream));
     BM_HIP(hipMemsetAsync(h->v.p, 0, n * 4, h->stream));
     BM_HIP(hipMemsetAsync(h->g.p, 0, n * 4, h->stream));
-    refresh_bf16(h.get());
+    BM_HIP(hipMemsetAsync(h->d_step.p, 0, 8, h->stream));
+    BM_HIP(hipMemsetAsync(h->d_batch.p, 0, 8, h->stream));
+    cast_weights(h.get());
     BM_HIP(hipStreamSynchronize(h->stream));
     *out = h.release();
   });
@@ -309,25 +568,29 @@ int bm_mlp_destroy(bm_mlp *h) {
 
 int bm_mlp_forward(bm_mlp *h, const float *X, int B, float *out) {
   return guard([&] {
-    const int Bp = (B + 31) / 32 * 32;
+    BM_REQUIRE(B >= 0, "B must be >= 0");
+    if (B == 0) return;
+    const int Bp = (B + MR - 1) / MR * MR;
     ensure_cap(h, Bp);
-    upload_batch(h, X, nullptr, B, Bp);
-    forward(h, Bp);
-    std::vector<float> o((size_t)Bp * h->npad);
-    BM_HIP(hipMemcpyAsync(o.data(), h->O.p, o.size() * 4, hipMemcpyDeviceToHost, h->stream));
+    BM_HIP(hipMemsetAsync(h->xb.p, 0, (size_t)Bp * h->nin * 4, h->stream));
+    BM_HIP(hipMemcpyAsync(h->xb.p, X, (size_t)B * h->nin * 4, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_mlp_rows<false>, dim3(Bp / MR), dim3(256), rows_shmem(h), h->stream, h->dev(),
+                       h->xb.as<float>(), nullptr, nullptr, nullptr, B, h->outb.as<float>(), nullptr, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    BM_HIP(hipGetLastError());
+    BM_HIP(hipMemcpyAsync(out, h->outb.p, (size_t)B * h->nout * 4, hipMemcpyDeviceToHost, h->stream));
     BM_HIP(hipStreamSynchronize(h->stream));
-    for (int i = 0; i < B; ++i)
-      for (int k = 0; k < h->nout; ++k) out[(size_t)i * h->nout + k] = o[(size_t)i * h->npad + k];
   });
 }
 
 int bm_mlp_grad(bm_mlp *h, const float *X, const float *U, int B, double *loss, float *grads) {
   return guard([&] {
-    BM_REQUIRE(B % 32 == 0, "gradient batches must be a multiple of 32");
+    BM_REQUIRE(B > 0 && B % 256 == 0, "gradient batches must be a positive multiple of 256");
     ensure_cap(h, B);
-    upload_batch(h, X, U, B, B);
-    forward(h, B);
-    backward(h, B);
+    BM_HIP(hipMemcpyAsync(h->xb.p, X, (size_t)B * h->nin * 4, hipMemcpyHostToDevice, h->stream));
+    BM_HIP(hipMemcpyAsync(h->ub.p, U, (size_t)B * h->nu * 4, hipMemcpyHostToDevice, h->stream));
+    step(h, h->xb.as<float>(), h->ub.as<float>(), nullptr, nullptr, B, 0, 0.f);
+    BM_HIP(hipGetLastError());
     std::vector<float> l(B);
     BM_HIP(hipMemcpyAsync(l.data(), h->loss.p, (size_t)B * 4, hipMemcpyDeviceToHost, h->stream));
     BM_HIP(hipMemcpyAsync(grads, h->g.p, (size_t)h->off[6] * 4, hipMemcpyDeviceToHost, h->stream));
@@ -340,57 +603,68 @@ int bm_mlp_grad(bm_mlp *h, const float *X, const float *U, int B, double *loss, 
 
 int bm_mlp_set_data(bm_mlp *h, const float *X, const float *U, int64_t n) {
   return guard([&] {
+    BM_REQUIRE(n > 0, "empty data set");
     h->data_x.alloc((size_t)n * h->nin * 4);
     h->data_u.alloc((size_t)n * h->nu * 4);
+    h->idx.alloc((size_t)n * 4);
     BM_HIP(hipMemcpyAsync(h->data_x.p, X, (size_t)n * h->nin * 4, hipMemcpyHostToDevice, h->stream));
     BM_HIP(hipMemcpyAsync(h->data_u.p, U, (size_t)n * h->nu * 4, hipMemcpyHostToDevice, h->stream));
     BM_HIP(hipStreamSynchronize(h->stream));
     h->ndata = n;
+    drop_graph(h);
   });
 }
 
-// Adam on shuffled mini-batches of the resident data set.
+// Adam on shuffled mini-batches of the resident data set.  The batches of
+// an epoch read consecutive slices of a device permutation; the batch
+// position and step count advance on the device, so chunks of steps replay
+// as one captured graph.
 int bm_mlp_train(bm_mlp *h, int epochs, int batch, float lr, uint64_t seed, double *last_loss) {
   return guard([&] {
-    BM_REQUIRE(h->ndata >= batch && batch % 32 == 0 && batch > 0, "batch must be a multiple of 32 <= data size");
+    BM_REQUIRE(batch > 0 && batch % 256 == 0 && h->ndata >= batch,
+               "batch must be a positive multiple of 256, at most the data size");
     ensure_cap(h, batch);
     std::mt19937_64 rng(seed);
     std::vector<int> perm(h->ndata);
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const int64_t n = h->off[6];
-    double acc = 0;
-    int64_t nb_total = 0;
+    const int64_t nb = h->ndata / batch;
+    const int chunk = (int)std::min<int64_t>(nb, 64);
+    if (h->graph && (h->graph_batch != batch || h->graph_steps != chunk || h->graph_lr != lr)) drop_graph(h);
+    if (!h->graph) {
+      hipGraph_t gr;
+      BM_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < chunk; ++k)
+        step(h, h->data_x.as<float>(), h->data_u.as<float>(), h->idx.as<int>(), h->d_batch.as<int64_t>(), batch,
+             1, lr);
+      hipError_t e = hipStreamEndCapture(h->stream, &gr);
+      BM_HIP(e);
+      e = hipGraphInstantiate(&h->graph, gr, nullptr, nullptr, 0);
+      hipGraphDestroy(gr);
+      BM_HIP(e);
+      h->graph_batch = batch;
+      h->graph_steps = chunk;
+      h->graph_lr = lr;
+    }
+    const int64_t zero = 0;
     for (int e = 0; e < epochs; ++e) {
       for (int64_t i = 0; i < h->ndata; ++i) perm[i] = (int)i;
       std::shuffle(perm.begin(), perm.end(), rng);
-      const int64_t nb = h->ndata / batch;
-      acc = 0;
-      for (int64_t bi = 0; bi < nb; ++bi) {
-        BM_HIP(hipMemcpyAsync(h->idx.p, perm.data() + bi * batch, (size_t)batch * 4, hipMemcpyHostToDevice,
-                              h->stream));
-        hipLaunchKernelGGL(k_gather, dim3(batch), dim3(64), 0, h->stream, h->data_x.as<float>(),
-                           h->data_u.as<float>(), h->idx.as<int>(), batch, h->nin, h->nu, h->Xb.as<uint16_t>(),
-                           h->Ub.as<float>());
-        forward(h, batch);
-        backward(h, batch);
-        ++h->step;
-        const float c1 = 1.f - std::pow(b1, (float)h->step), c2 = 1.f - std::pow(b2, (float)h->step);
-        hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->P(), h->G(),
-                           h->m.as<float>(), h->v.as<float>(), n, lr, b1, b2, c1, c2, eps);
-        refresh_bf16(h);
-        if (bi == nb - 1 || bi % 64 == 0) {
-          std::vector<float> l(batch);
-          BM_HIP(hipMemcpyAsync(l.data(), h->loss.p, (size_t)batch * 4, hipMemcpyDeviceToHost, h->stream));
-          BM_HIP(hipStreamSynchronize(h->stream));
-          double s = 0;
-          for (float x : l) s += x;
-          acc = s / batch;
-        }
-        ++nb_total;
-      }
+      BM_HIP(hipMemcpyAsync(h->idx.p, perm.data(), (size_t)h->ndata * 4, hipMemcpyHostToDevice, h->stream));
+      BM_HIP(hipMemcpyAsync(h->d_batch.p, &zero, 8, hipMemcpyHostToDevice, h->stream));
+      int64_t done = 0;
+      for (; done + chunk <= nb; done += chunk) BM_HIP(hipGraphLaunch(h->graph, h->stream));
+      for (; done < nb; ++done)
+        step(h, h->data_x.as<float>(), h->data_u.as<float>(), h->idx.as<int>(), h->d_batch.as<int64_t>(), batch,
+             1, lr);
+      // the host copies above read perm and zero: finish before reuse
+      BM_HIP(hipStreamSynchronize(h->stream));
+      h->step += nb;
     }
+    std::vector<float> l(batch);
+    BM_HIP(hipMemcpyAsync(l.data(), h->loss.p, (size_t)batch * 4, hipMemcpyDeviceToHost, h->stream));
     BM_HIP(hipStreamSynchronize(h->stream));
-    *last_loss = acc;
+    double s = 0;
+    for (float x : l) s += x;
+    *last_loss = s / batch;
   });
 }
 
@@ -398,26 +672,6 @@ int bm_mlp_get_params(bm_mlp *h, float *out) {
   return guard([&] {
     BM_HIP(hipMemcpyAsync(out, h->p.p, (size_t)h->off[6] * 4, hipMemcpyDeviceToHost, h->stream));
     BM_HIP(hipStreamSynchronize(h->stream));
-  });
-}
-
-int bm_mlp_gemm_test(int M, int N, int K, int ta, int tb, const float *A, const float *B, float *C) {
-  return guard([&] {
-    bm_mlp h;
-    BM_HIP(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
-    std::vector<uint16_t> a((size_t)M * K), b((size_t)K * N);
-    for (size_t i = 0; i < a.size(); ++i) a[i] = f2bf(A[i]);
-    for (size_t i = 0; i < b.size(); ++i) b[i] = f2bf(B[i]);
-    DevBuf da, db, dc;
-    da.alloc(a.size() * 2);
-    db.alloc(b.size() * 2);
-    dc.alloc((size_t)M * N * 4);
-    BM_HIP(hipMemcpyAsync(da.p, a.data(), a.size() * 2, hipMemcpyHostToDevice, h.stream));
-    BM_HIP(hipMemcpyAsync(db.p, b.data(), b.size() * 2, hipMemcpyHostToDevice, h.stream));
-    gemm(&h, da.as<uint16_t>(), ta ? M : K, ta, db.as<uint16_t>(), tb ? K : N, tb, dc.as<float>(), nullptr, N, M,
-         N, K, nullptr, nullptr, EPI_NONE);
-    BM_HIP(hipMemcpyAsync(C, dc.p, (size_t)M * N * 4, hipMemcpyDeviceToHost, h.stream));
-    BM_HIP(hipStreamSynchronize(h.stream));
   });
 }
 

@@ -155,19 +155,6 @@ class MLP:
         return u.reshape(-1, self.nu).T
 
 
-def gemm_test(A, B, ta=False, tb=False):
-    """Device BF16 GEMM of op(A) op(B) (layout check)."""
-    A = np.ascontiguousarray(A, np.float32)
-    B = np.ascontiguousarray(B, np.float32)
-    M = A.shape[1] if ta else A.shape[0]
-    K = A.shape[0] if ta else A.shape[1]
-    N = B.shape[0] if tb else B.shape[1]
-    out = np.empty((M, N), np.float32)
-    _lib.call("bm_mlp_gemm_test", M, N, K, 1 if ta else 0, 1 if tb else 0, _lib.f32ptr(A),
-              _lib.f32ptr(B), _lib.f32ptr(out))
-    return out
-
-
 def simulate_training_set(engine_factory, I, bounds, per_input, rng, duration_s, dt):
     """Simulator-generated (features, unit parameters) pairs.
 

@@ -445,8 +445,9 @@ int bm_fit_get_spikes(bm_fit *h, int32_t *steps, int32_t *cand, int64_t cap, int
 
 /* ------------------------------------------------------- learned proposal
  * MLP nin -> nh -> nh -> 2*nu (ReLU), Gaussian NLL over nu normalised
- * parameters (outputs: nu means, nu log-sigmas).  BF16 MFMA GEMMs with FP32
- * accumulation; FP32 master weights and Adam.  No counterpart in the
+ * parameters (outputs: nu means, nu log-sigmas).  BF16 MFMA products with
+ * FP32 accumulation (gfx950 32x32x16); FP32 master weights and Adam; nin
+ * <= 128 with nh = 256 (the row kernel's activations fit 64 KiB of LDS).  No counterpart in the
  * reference (its fitting starts from uniform random particles); it seeds the
  * particle swarm of the fitting loop.  params: flat FP32 W1[nin][nh] b1[nh]
  * W2[nh][nh] b2[nh] W3[nh][32] b3[32] (output layer padded to 32). */
@@ -454,15 +455,16 @@ typedef struct bm_mlp bm_mlp;
 int bm_mlp_num_params(int nin, int nh, int nu, int64_t *n);
 int bm_mlp_create(int nin, int nh, int nu, const float *params, bm_mlp **out);
 int bm_mlp_destroy(bm_mlp *h);
-/* out [B][2*nu] */
+/* out [B][2*nu] (host buffers) */
 int bm_mlp_forward(bm_mlp *h, const float *X, int B, float *out);
-/* mean loss and its gradient (same flat layout as params); B multiple of 32 */
+/* mean loss and its gradient (same flat layout as params); B a positive
+ * multiple of 256 (host buffers) */
 int bm_mlp_grad(bm_mlp *h, const float *X, const float *U, int B, double *loss, float *grads);
 int bm_mlp_set_data(bm_mlp *h, const float *X, const float *U, int64_t n);
+/* Adam on shuffled mini-batches of the data set; batch a positive multiple
+ * of 256 */
 int bm_mlp_train(bm_mlp *h, int epochs, int batch, float lr, uint64_t seed, double *last_loss);
 int bm_mlp_get_params(bm_mlp *h, float *out);
-/* C[M][N] = op(A) op(B) with BF16-rounded operands (layout check of the MFMA tile) */
-int bm_mlp_gemm_test(int M, int N, int K, int ta, int tb, const float *A, const float *B, float *C);
 
 /* VanRossumMetric.get_distance (brian/monitor.py:1427-1486): X [n][T] holds
  * n sampled traces (row-major); D [n][n] receives

@@ -14,17 +14,25 @@ pytestmark = pytest.mark.gpu
 DT = 1e-4
 
 
-@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
-def test_mfma_gemm_layout(ta, tb):
-    rng = np.random.default_rng(1 + ta + 2 * tb)
-    M, N, K = 64, 96, 40
-    A = rng.standard_normal((K, M) if ta else (M, K))
-    B = rng.standard_normal((N, K) if tb else (K, N))
-    C = L.gemm_test(A, B, bool(ta), bool(tb))
-    Ar = OM.bf16(A).T if ta else OM.bf16(A)
-    Br = OM.bf16(B).T if tb else OM.bf16(B)
-    ref = Ar @ Br
-    assert np.max(np.abs(C - ref)) <= 1e-4 * (1 + np.max(np.abs(ref)))
+def test_mlp_forward_matches_oracle():
+    """The fused row kernel's forward pass (BF16 operands on the 32x32x16
+    MFMA, FP32 accumulation) against the FP64 oracle with the same BF16
+    roundings, on a ragged batch (not a multiple of the 32-row tile)."""
+    rng = np.random.default_rng(7)
+    nin, nh, nu, B = 64, 256, 4, 300
+    m = L.MLP(nin, nh, nu, seed=9)
+    p = m.params().astype(np.float64)
+    X = rng.standard_normal((B, nin)).astype(np.float32)
+    out = m.forward(X)
+    W1, b1, W2, b2, W3, b3 = OM.unpack(p, nin, nh)
+    A1 = OM.bf16(np.maximum(OM.bf16(X) @ OM.bf16(W1) + b1, 0))
+    A2 = OM.bf16(np.maximum(A1 @ OM.bf16(W2) + b2, 0))
+    ref = (A2 @ OM.bf16(W3) + b3)[:, :2 * nu]
+    assert out.shape == (B, 2 * nu)
+    err = np.max(np.abs(out - ref)) / (1 + np.max(np.abs(ref)))
+    print("forward max error vs oracle:", err)
+    assert err < 2e-3
+    m.close()
 
 
 def test_mlp_gradient_matches_oracle():
