@@ -119,6 +119,7 @@ _SIGS = {
     "bm_mlp_set_data": [P, C.POINTER(C.c_float), C.POINTER(C.c_float), i64],
     "bm_mlp_train": [P, C.c_int, C.c_int, C.c_float, u64, Pd],
     "bm_mlp_get_params": [P, C.POINTER(C.c_float)],
+    "bm_mlp_grad_noalloc": [P, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
     "bm_fit_set_candidates": [P, Pd, Pd, Pi32, Pi32],
     "bm_fit_run": [P, i64, i64],
     "bm_fit_sync": [P],

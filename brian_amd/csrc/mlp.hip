@@ -25,6 +25,7 @@
 
 #include <cmath>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -365,6 +366,30 @@ __global__ void k_mlp_cast(const MlpDev md, uint16_t *w1t, uint16_t *w2t, uint16
   }
 }
 
+// Mean of the per-row losses (one workgroup, fixed summation order).
+__global__ __launch_bounds__(256) void k_mlp_loss_mean(const float *__restrict__ loss, int B, float *out) {
+  __shared__ float sh[256];
+  float s = 0.f;
+  for (int i = threadIdx.x; i < B; i += 256) s += loss[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0] / (float)B;
+}
+
+// Scratch of bm_mlp_grad_noalloc: the per-step buffers of a batch of up to
+// NA_MAXB rows for nin <= NA_MAXNI and nh <= MAXNH, one copy per GPU for the
+// life of the process.
+constexpr int NA_MAXB = 1024, NA_MAXNI = 128;
+constexpr int64_t NA_MAXPARAMS =
+    (int64_t)NA_MAXNI * MAXNH + MAXNH + (int64_t)MAXNH * MAXNH + MAXNH + (int64_t)MAXNH * NO + NO;
+constexpr size_t NA_BF16 = (size_t)NA_MAXB * (NA_MAXNI + 4 * MAXNH + NO);          // BF16 elements
+constexpr size_t NA_BYTES = NA_BF16 * 2 + (size_t)NA_MAXB * 4 + (size_t)WG_SLICES * NA_MAXPARAMS * 4;
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
 }  // namespace bm
 
 using namespace bm;
@@ -451,21 +476,34 @@ void cast_weights(bm_mlp *h) {
 // forward + loss + backward of one batch of B rows (B multiple of 256 for
 // the batch slices, rows gathered from X through idx or taken in order),
 // then either Adam (adam = 1, lr) or the summed gradient into g.
-void step(bm_mlp *h, const float *X, const float *U, const int *idx, const int64_t *d_batch, int B, int adam,
-          float lr) {
+// Device buffers one step works in: the transposed operands of the weight
+// gradients, the per-row loss, the per-slice partial sums and the gradient
+// (written when the step does not apply Adam).
+struct StepBufs {
+  uint16_t *XT, *A1T, *A2T, *dOT, *dZ2T, *dZ1T;
+  float *loss, *part, *g;
+};
+
+StepBufs handle_bufs(bm_mlp *h) {
+  return {h->XT.as<uint16_t>(), h->A1T.as<uint16_t>(), h->A2T.as<uint16_t>(), h->dOT.as<uint16_t>(),
+          h->dZ2T.as<uint16_t>(), h->dZ1T.as<uint16_t>(), h->loss.as<float>(), h->part.as<float>(),
+          h->g.as<float>()};
+}
+
+void step(bm_mlp *h, const StepBufs &sb, const float *X, const float *U, const int *idx, const int64_t *d_batch,
+          int B, int adam, float lr) {
   const MlpDev md = h->dev();
   hipLaunchKernelGGL(k_mlp_rows<true>, dim3(B / MR), dim3(256), rows_shmem(h), h->stream, md, X, U, idx, d_batch,
-                     B, nullptr, h->XT.as<uint16_t>(), h->A1T.as<uint16_t>(), h->A2T.as<uint16_t>(),
-                     h->dOT.as<uint16_t>(), h->dZ2T.as<uint16_t>(), h->dZ1T.as<uint16_t>(), h->loss.as<float>(),
+                     B, nullptr, sb.XT, sb.A1T, sb.A2T, sb.dOT, sb.dZ2T, sb.dZ1T, sb.loss,
                      adam ? h->d_step.as<int64_t>() : nullptr);
   WgradArgs w{};
-  w.XT = h->XT.as<uint16_t>();
-  w.A1T = h->A1T.as<uint16_t>();
-  w.A2T = h->A2T.as<uint16_t>();
-  w.dOT = h->dOT.as<uint16_t>();
-  w.dZ2T = h->dZ2T.as<uint16_t>();
-  w.dZ1T = h->dZ1T.as<uint16_t>();
-  w.part = h->part.as<float>();
+  w.XT = sb.XT;
+  w.A1T = sb.A1T;
+  w.A2T = sb.A2T;
+  w.dOT = sb.dOT;
+  w.dZ2T = sb.dZ2T;
+  w.dZ1T = sb.dZ1T;
+  w.part = sb.part;
   w.nparams = h->off[6];
   w.B = B;
   w.ntiles1 = (h->ni / 32) * (h->nh / 32);
@@ -477,10 +515,10 @@ void step(bm_mlp *h, const float *X, const float *U, const int *idx, const int64
   hipLaunchKernelGGL(k_mlp_wgrad, dim3(blocks), dim3(256), 0, h->stream, md, w);
   AdamArgs a{};
   a.p = h->p.as<float>();
-  a.g = h->g.as<float>();
+  a.g = sb.g;
   a.m = h->m.as<float>();
   a.v = h->v.as<float>();
-  a.part = h->part.as<float>();
+  a.part = sb.part;
   a.n = h->off[6];
   a.lr = lr;
   a.b1 = 0.9f;
@@ -589,7 +627,7 @@ int bm_mlp_grad(bm_mlp *h, const float *X, const float *U, int B, double *loss, 
     ensure_cap(h, B);
     BM_HIP(hipMemcpyAsync(h->xb.p, X, (size_t)B * h->nin * 4, hipMemcpyHostToDevice, h->stream));
     BM_HIP(hipMemcpyAsync(h->ub.p, U, (size_t)B * h->nu * 4, hipMemcpyHostToDevice, h->stream));
-    step(h, h->xb.as<float>(), h->ub.as<float>(), nullptr, nullptr, B, 0, 0.f);
+    step(h, handle_bufs(h), h->xb.as<float>(), h->ub.as<float>(), nullptr, nullptr, B, 0, 0.f);
     BM_HIP(hipGetLastError());
     std::vector<float> l(B);
     BM_HIP(hipMemcpyAsync(l.data(), h->loss.p, (size_t)B * 4, hipMemcpyDeviceToHost, h->stream));
@@ -633,8 +671,8 @@ int bm_mlp_train(bm_mlp *h, int epochs, int batch, float lr, uint64_t seed, doub
       hipGraph_t gr;
       BM_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       for (int k = 0; k < chunk; ++k)
-        step(h, h->data_x.as<float>(), h->data_u.as<float>(), h->idx.as<int>(), h->d_batch.as<int64_t>(), batch,
-             1, lr);
+        step(h, handle_bufs(h), h->data_x.as<float>(), h->data_u.as<float>(), h->idx.as<int>(),
+             h->d_batch.as<int64_t>(), batch, 1, lr);
       hipError_t e = hipStreamEndCapture(h->stream, &gr);
       BM_HIP(e);
       e = hipGraphInstantiate(&h->graph, gr, nullptr, nullptr, 0);
@@ -653,8 +691,8 @@ int bm_mlp_train(bm_mlp *h, int epochs, int batch, float lr, uint64_t seed, doub
       int64_t done = 0;
       for (; done + chunk <= nb; done += chunk) BM_HIP(hipGraphLaunch(h->graph, h->stream));
       for (; done < nb; ++done)
-        step(h, h->data_x.as<float>(), h->data_u.as<float>(), h->idx.as<int>(), h->d_batch.as<int64_t>(), batch,
-             1, lr);
+        step(h, handle_bufs(h), h->data_x.as<float>(), h->data_u.as<float>(), h->idx.as<int>(),
+             h->d_batch.as<int64_t>(), batch, 1, lr);
       // the host copies above read perm and zero: finish before reuse
       BM_HIP(hipStreamSynchronize(h->stream));
       h->step += nb;
@@ -665,6 +703,55 @@ int bm_mlp_train(bm_mlp *h, int epochs, int batch, float lr, uint64_t seed, doub
     double s = 0;
     for (float x : l) s += x;
     *last_loss = s / batch;
+  });
+}
+
+namespace {
+// Device address of this GPU's copy of the scratch array, looked up once
+// per device (the array's name is only a host-side handle).
+unsigned char *scratch_ptr() {
+  static std::mutex mu;
+  static std::vector<unsigned char *> ptrs;
+  int dev = 0;
+  BM_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)ptrs.size() <= dev) ptrs.resize(dev + 1, nullptr);
+  if (!ptrs[dev]) {
+    void *p = nullptr;
+    BM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    BM_REQUIRE(p != nullptr, "no device address for the gradient scratch");
+    ptrs[dev] = static_cast<unsigned char *>(p);
+  }
+  return ptrs[dev];
+}
+}  // namespace
+
+int bm_mlp_grad_noalloc(bm_mlp *h, const float *X, const float *U, int B, float *loss, float *grads) {
+  return guard([&] {
+    BM_REQUIRE(h && X && U && loss && grads, "null argument");
+    BM_REQUIRE(B > 0 && B % 256 == 0 && B <= NA_MAXB, "B must be a multiple of 256 in [256, 1024]");
+    BM_REQUIRE(h->nin <= NA_MAXNI, "bm_mlp_grad_noalloc supports nin <= 128");
+    unsigned char *base = scratch_ptr();
+    // buffers at their size for this B, back to back in the scratch
+    uint16_t *bf = reinterpret_cast<uint16_t *>(base);
+    StepBufs sb;
+    sb.XT = bf;
+    sb.A1T = sb.XT + (size_t)h->ni * B;
+    sb.A2T = sb.A1T + (size_t)h->nh * B;
+    sb.dOT = sb.A2T + (size_t)h->nh * B;
+    sb.dZ2T = sb.dOT + (size_t)NO * B;
+    sb.dZ1T = sb.dZ2T + (size_t)h->nh * B;
+    unsigned char *fl = base + NA_BF16 * 2;
+    sb.loss = reinterpret_cast<float *>(fl);
+    sb.part = reinterpret_cast<float *>(fl + (size_t)NA_MAXB * 4);
+    sb.g = grads;
+    // every element the kernels read is written earlier in the same call
+    // (all batch columns of the operands, all slices of every parameter),
+    // so nothing is cleared
+    step(h, sb, X, U, nullptr, nullptr, B, 0, 0.f);
+    hipLaunchKernelGGL(k_mlp_loss_mean, dim3(1), dim3(256), 0, h->stream, sb.loss, B, loss);
+    BM_HIP(hipGetLastError());
+    BM_HIP(hipStreamSynchronize(h->stream));
   });
 }
 

@@ -465,6 +465,15 @@ int bm_mlp_set_data(bm_mlp *h, const float *X, const float *U, int64_t n);
  * of 256 */
 int bm_mlp_train(bm_mlp *h, int epochs, int batch, float lr, uint64_t seed, double *last_loss);
 int bm_mlp_get_params(bm_mlp *h, float *out);
+/* bm_mlp_grad without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  X [B][nin],
+ * U [B][nu], loss (one float: the mean loss) and grads (nparams floats) are
+ * DEVICE pointers; B a multiple of 256 up to 1024, nin <= 128.  The step's
+ * buffers live in a fixed scratch array of about 6 MB per GPU held until
+ * the process ends; the call returns when the results are written.  Calls
+ * that overlap on one GPU race on that scratch. */
+int bm_mlp_grad_noalloc(bm_mlp *h, const float *X, const float *U, int B, float *loss, float *grads);
 
 /* VanRossumMetric.get_distance (brian/monitor.py:1427-1486): X [n][T] holds
  * n sampled traces (row-major); D [n][n] receives
