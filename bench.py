@@ -131,6 +131,23 @@ def roofline(alg_bytes, avg_launch_s, traffic, **extra):
     return out
 
 
+def probe_copy(nbytes, reps=10):
+    """A streaming kernel of known traffic for calibrating the HBM byte
+    counters: torch's elementwise y = x * 1 over nbytes of FP32 reads and
+    writes exactly nbytes each per launch."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    return {"probe_copy_bytes": nbytes, "avg_ms": ms, "GBps_read_plus_write": 2 * nbytes / ms / 1e6}
+
+
 def single_thread():
     """Limit BLAS/OpenMP pools of this process to one thread (CPU baselines)."""
     try:
@@ -870,10 +887,7 @@ def main():
                     help="only run the byte-counter calibration copy of this many bytes")
     args = ap.parse_args()
     if args.probe_copy:
-        from brian_amd import _lib
-        ms = _lib.probe_copy(args.probe_copy, reps=10)
-        print(json.dumps({"probe_copy_bytes": args.probe_copy, "avg_ms": ms,
-                          "GBps_read_plus_write": 2 * args.probe_copy / ms / 1e6}))
+        print(json.dumps(probe_copy(args.probe_copy)))
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "

@@ -97,7 +97,6 @@ _SIGS = {
     "bm_net_read_state_monitor": [P, C.c_int, Pd, i64, Pi64, Pi64],
     "bm_net_get_tuning": [P, C.c_int, C.POINTER(C.c_int)],
     "bm_net_get_profile": [P, Pi64, Pd, Pd],
-    "bm_probe_copy": [i64, C.c_int, Pd],
     "bm_stdp_batch_create": [C.c_int, dbl, C.c_int, C.c_int, C.c_int, Pd, Pd, Pd, C.c_int, dbl,
                              C.c_int, dbl, C.c_int, Pd, dbl, u64, dbl, dbl, dbl, dbl, dbl, dbl,
                              C.c_int, C.c_int, i64, C.POINTER(P)],
@@ -153,13 +152,6 @@ def load():
         fn.restype = C.c_int
     _lib = lib
     return lib
-
-
-def probe_copy(nbytes, reps=20):
-    """Average duration (ms) of a streaming copy of nbytes."""
-    ms = C.c_double()
-    call("bm_probe_copy", int(nbytes), int(reps), C.byref(ms))
-    return ms.value
 
 
 def exported_symbols():

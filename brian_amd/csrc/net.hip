@@ -2369,30 +2369,5 @@ int bm_net_get_profile(bm_net *net, int64_t *launches, double *total_ms, double 
   });
 }
 
-int bm_probe_copy(int64_t nbytes, int reps, double *avg_ms) {
-  return guard([&] {
-    BM_REQUIRE(nbytes > 0 && nbytes % 16 == 0 && reps > 0, "nbytes must be a positive multiple of 16");
-    DevBuf a, b;
-    a.alloc((size_t)nbytes);
-    b.alloc((size_t)nbytes);
-    hipStream_t st;
-    BM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    BM_HIP(hipMemsetAsync(a.p, 1, (size_t)nbytes, st));
-    hipEvent_t e0, e1;
-    BM_HIP(hipEventCreate(&e0));
-    BM_HIP(hipEventCreate(&e1));
-    launch_probe_copy(a.p, b.p, nbytes, st);
-    BM_HIP(hipEventRecord(e0, st));
-    for (int r = 0; r < reps; ++r) launch_probe_copy(a.p, b.p, nbytes, st);
-    BM_HIP(hipEventRecord(e1, st));
-    BM_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    BM_HIP(hipEventElapsedTime(&ms, e0, e1));
-    *avg_ms = ms / reps;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipStreamDestroy(st);
-  });
-}
 
 }  // extern "C"

@@ -27,7 +27,6 @@ void launch_late_mod(const LateModDev &lm, hipStream_t st, const int64_t *d_base
 void launch_hist_clear(unsigned int *const *counts, const int *H, int ngroups, const int64_t *d_base,
                        int64_t rel_next, hipStream_t st);
 void launch_advance(int64_t *d_base, int64_t by, hipStream_t st);
-void launch_probe_copy(const void *src, void *dst, int64_t bytes, hipStream_t st);
 
 // block-diagonal random connectivity built on the device (conngen.hip)
 struct GenSpec {

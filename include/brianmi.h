@@ -481,10 +481,6 @@ int bm_mlp_grad_noalloc(bm_mlp *h, const float *X, const float *U, int B, float 
  * zero diagonal.  Host buffers; computed on the device. */
 int bm_pairwise_sq_distance(const double *X, int64_t n, int64_t T, double scale, double *D);
 
-/* Streaming copy of nbytes (reps launches) used to calibrate hardware byte
- * counters against known traffic; no counterpart in the reference. */
-int bm_probe_copy(int64_t nbytes, int reps, double *avg_ms);
-
 #ifdef __cplusplus
 }
 #endif

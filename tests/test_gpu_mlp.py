@@ -94,8 +94,17 @@ def test_device_features_match_host():
 
 
 def test_learned_proposal_beats_plain_pso():
-    """The trained network's proposal reaches the plain fit's median gamma
-    with <= 1/4 of the simulated candidate-seconds."""
+    """What the trained network gains over plain PSO (the method without
+    it), measured on the fitting histories:
+    - at the same fitting budget (three evaluations of the swarm each: the
+      learned proposal's starting particles are evaluated, then two PSO
+      iterations) the learned proposal's median gamma exceeds plain PSO's;
+    - it reaches the median gamma plain PSO has after 12 iterations with at
+      most 1/4 of the candidate-seconds plain PSO needed to get there.
+    The simulations that made the training set are reported beside it: at
+    this test's scale (32 targets) they are not amortised, at the fitting
+    benchmark's (2048 targets x 10,000 candidates) they are ~1 % of the
+    plain fit."""
     ntarget, steps, pop = 32, 10000, 64
     dur = steps * DT
     I, hidden, trains = _fit_setup(ntarget, steps, 21)
@@ -106,23 +115,32 @@ def test_learned_proposal_beats_plain_pso():
                   popsize=pop, delta=4e-3, initial_values={"V": 0.0, "Vt": 0.0}, R=b["R"], tau=b["tau"],
                   Vt0=b["Vt0"], refractory=b["refractory"])
     plain = modelfitting(maxiter=12, seed=1, **common)
+    plain3 = modelfitting(maxiter=3, seed=1, **common)
     rng = np.random.default_rng(3)
 
     def factory(cand_target):
         return FitEngine(eqs, W.FIT_THRESHOLD, W.FIT_RESET, I, trains, cand_target, DT, 40,
                          precision=32, features=True)
     bounds = {k: b[k] for k in ("R", "tau", "Vt0", "refractory")}
-    Ft, Ut, _ = L.simulate_training_set(factory, I, bounds, 8192, rng, dur, DT)
+    per_input = 8192
+    Ft, Ut, _ = L.simulate_training_set(factory, I, bounds, per_input, rng, dur, DT)
     net = L.MLP(64, 256, 4, seed=7)
     net.train(Ft, Ut, epochs=15, batch=1024, lr=1e-3, seed=2)
     Ftarget = L.normalise(np.stack([L.raw_features(trains[j], I[j]) for j in range(ntarget)]),
                           [len(t) for t in trains], I.mean(1), I.std(1), dur)
     u = net.propose(Ftarget, pop, rng)
+    net.close()
     init = {k: L.from_unit(u[i], bounds[k]) for i, k in enumerate(bounds)}
     learned = modelfitting(maxiter=2, seed=1, initial_params=init, **common)
-    ratio = learned["simulated_candidate_seconds"] / plain["simulated_candidate_seconds"]
-    print(f"median gamma plain {np.median(plain['fitness']):.3f} "
-          f"learned {np.median(learned['fitness']):.3f}; candidate-seconds ratio {ratio:.3f}")
-    assert ratio <= 0.25
-    assert np.median(learned["fitness"]) >= np.median(plain["fitness"])
-    net.close()
+    goal = plain["history"][-1][1]
+    plain_cs = next(cs for cs, g in plain["history"] if g >= goal)
+    hit = [cs for cs, g in learned["history"] if g >= goal]
+    train_cs = ntarget * per_input * dur
+    assert learned["simulated_candidate_seconds"] == plain3["simulated_candidate_seconds"]
+    g_plain3, g_learned = np.median(plain3["fitness"]), np.median(learned["fitness"])
+    print(f"median gamma after 3 evaluations: plain {g_plain3:.3f}, learned {g_learned:.3f}; "
+          f"plain after 12: {goal:.3f} at {plain_cs:.0f} candidate-s; learned reaches it at "
+          f"{hit[0] if hit else float('nan'):.0f}; training simulations {train_cs:.0f} candidate-s")
+    assert g_learned > g_plain3 + 0.05
+    assert hit, "the learned proposal does not reach plain PSO's 12-iteration median gamma"
+    assert hit[0] <= 0.25 * plain_cs
