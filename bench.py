@@ -422,7 +422,9 @@ def bench_uneven(args):
     def build32(lay):
         return build_uneven(d, 32, window=args.window, npt=args.npt, record=False, tuning=lay)[0]
     tuning, tune_report = autotuned(build32, args.no_autotune, args.tuning)
+    from brian_amd import autotune as AT
     res = {}
+    layout, ltimes = (0, 0), {}
     for prec in (32, 64):
         if prec == 64 and args.no_fp64:
             continue
@@ -571,14 +573,30 @@ def bench_stdp(args):
     d = W.stdp_batch()
     K = args.steps or 200000
     Wm = 0 if args.warmup is None else args.warmup
+    from brian_amd import autotune as AT
     res = {}
+    layout, ltimes = (0, 0), {}
     for prec in (32, 64):
         if prec == 64 and args.no_fp64:
             continue
         h = StdpBatchHandle(prec, 1e-4, d["ninst"], d["nin"], d["A"], d["C"], d["S0"], 0, d["vt"], 0,
                             d["vr"], 1, d["w0"], d["rate"], 2024, d["decay_pre"], d["decay_post"],
                             d["Ap"], d["Am"], 0.0, d["gmax"])
-        el = timed_run(h, K, Wm)
+        # the layout search's trial steps are simulated steps: they count as
+        # warm-up, so the timed region starts at the same step either way
+        if prec == 32 and args.tuning:
+            layout = (int(args.tuning["waves_per_instance"]), int(args.tuning["slots_per_lane"]))
+            h.set_layout(*layout)
+            used = 0
+        elif prec == 32 and not args.no_autotune:
+            layout, used, ltimes = AT.tune_stdp_batch(h, trial_steps=2000, reps=1)
+            Wm = max(Wm, used)
+        else:
+            h.set_layout(*layout)
+            used = 0
+        if prec == 32:
+            layout = h.layout()
+        el = timed_run(h, K, Wm - used)
         out = {"seconds": el, "w": h.weights(), "post_spikes": len(h.spikes()[0])}
         if prec == 32:
             h.set_profiling(True)
@@ -598,7 +616,8 @@ def bench_stdp(args):
     # HBM bytes are one load and one store of them (3 x 4 B) per synapse
     alg_bytes = syn * 3 * 4 * 2
     roof = roofline(alg_bytes, avg_launch_s, hbm_traffic("stdp_batch_pmc"),
-                    kernel="k_stdp_batch<float,true> (wavefront per instance, state in registers)",
+                    kernel=f"k_stdp_batch<float,2,true,{layout[0]},{layout[1]}> (workgroup per instance, "
+                           "weights and traces in registers)",
                     steps_per_launch=steps_per_launch, avg_launch_us=avg_launch_s * 1e6,
                     alg_bytes_per_launch=alg_bytes, launches_timed=launches,
                     note="register-resident weights: HBM is not the bound; the per-step input "
@@ -620,7 +639,9 @@ def bench_stdp(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "batched STDP: 1024 STDP1 instances x 1000 Poisson inputs at 15 Hz, "
                                "ExponentialSTDP, Kahan-compensated FP32 weights, dt 0.1 ms",
-                   "instances": ninst, "inputs": nin, "simulated_s": K * 1e-4},
+                   "instances": ninst, "inputs": nin, "simulated_s": K * 1e-4,
+                   "layout": {"waves_per_instance": layout[0], "slots_per_lane": layout[1]},
+                   "layout_us_per_step": {f"{k[0]}x{k[1]}": v / 2000 * 1e6 for k, v in ltimes.items()}},
         "neuron_updates_per_s": ninst * K / el,
         "fp64_check": check,
         "roofline": roof,

@@ -103,6 +103,8 @@ _SIGS = {
     "bm_stdp_batch_destroy": [P],
     "bm_stdp_batch_set_profiling": [P, C.c_int],
     "bm_stdp_batch_get_profile": [P, Pi64, Pd],
+    "bm_stdp_batch_set_layout": [P, C.c_int, C.c_int],
+    "bm_stdp_batch_get_layout": [P, Pi32, Pi32],
     "bm_fit_create": [C.c_int, dbl, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_char_p,
                       C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, i64, Pd, Pi32,
                       Pi32, Pi32, C.c_int, C.c_int, C.c_int, C.POINTER(P)],
@@ -550,6 +552,16 @@ class StdpBatchHandle:
 
     def sync(self):
         call("bm_stdp_batch_sync", self.h)
+
+    def set_layout(self, waves_per_instance=0, slots_per_lane=0):
+        """Wavefronts per instance and input slots per lane; (0, 0) is the
+        default.  Results do not depend on it."""
+        call("bm_stdp_batch_set_layout", self.h, int(waves_per_instance), int(slots_per_lane))
+
+    def layout(self):
+        nw, nj = C.c_int32(), C.c_int32()
+        call("bm_stdp_batch_get_layout", self.h, C.byref(nw), C.byref(nj))
+        return nw.value, nj.value
 
     def set_profiling(self, on):
         call("bm_stdp_batch_set_profiling", self.h, 1 if on else 0)

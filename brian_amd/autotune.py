@@ -131,3 +131,27 @@ def tune_layout(build, layouts=LAYOUTS, trial_steps=256, reps=2):
         res.append((lay, t / trial_steps))
     best = min(res, key=lambda x: x[1])[0]
     return best, res
+
+
+def stdp_layouts(nin):
+    """Launch layouts (wavefronts per instance, input slots per lane) the
+    batched STDP kernel is compiled for that hold nin inputs."""
+    return [(nw, nj) for nj in (1, 2, 4) for nw in (1, 2, 4, 8, 16)
+            if nw * nj <= 16 and nw * nj * 64 >= nin]
+
+
+def tune_stdp_batch(h, trial_steps=2000, reps=2, log=None):
+    """Time every layout of a batched-STDP handle on live trial segments and
+    keep the fastest.  The layout does not change results, so the trial
+    steps are ordinary simulated steps; returns (layout, steps consumed,
+    {layout: seconds})."""
+    times, used = {}, 0
+    for lay in stdp_layouts(h.nin):
+        h.set_layout(*lay)
+        times[lay] = _time_segment(h, trial_steps, reps)
+        used += trial_steps * reps
+        if log:
+            log(f"stdp layout {lay}: {times[lay] / trial_steps * 1e6:.3f} us/step")
+    best = min(times, key=times.get)
+    h.set_layout(*best)
+    return best, used, times

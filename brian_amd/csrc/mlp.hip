@@ -36,6 +36,8 @@ typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int MR = 32;          // batch rows per k_mlp_rows workgroup
+constexpr int RT = 512;         // threads of a k_mlp_rows workgroup (8 wavefronts)
+constexpr int RW = RT / 64;
 constexpr int NO = 32;          // padded output width (2 nu <= 32)
 constexpr int MAXNI = 256, MAXNH = 256;
 constexpr int WG_SLICES = 8;    // batch slices of the weight-gradient kernel
@@ -64,7 +66,7 @@ __device__ inline void tile_mma(f32x16_t &acc, const uint16_t *A, int lda, const
   const int r = lane & 31, kh = (lane >> 5) * 8;
   const uint16_t *a = A + r * lda + kh;
   const uint16_t *b = Bt + (size_t)(n0 + r) * ldb + kh;
-#pragma unroll 4
+#pragma unroll 16
   for (int k0 = 0; k0 < K; k0 += 16)
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(a + k0), ld8(b + k0), acc, 0, 0, 0);
 }
@@ -89,7 +91,7 @@ __host__ __device__ inline int lds_stride(int n) { return n + 8; }
 // columns) contiguous: XT [ni][B], A1T / A2T / dZ2T / dZ1T [nh][B], dOT [NO][B].
 // Forward only: the nout outputs per row into out [B][nout].
 template <bool TRAIN>
-__global__ __launch_bounds__(256) void k_mlp_rows(const MlpDev m, const float *__restrict__ X,
+__global__ __launch_bounds__(RT) void k_mlp_rows(const MlpDev m, const float *__restrict__ X,
                                                   const float *__restrict__ U, const int *__restrict__ idx,
                                                   const int64_t *__restrict__ d_batch, int B, float *out,
                                                   uint16_t *XT, uint16_t *A1T, uint16_t *A2T, uint16_t *dOT,
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(256) void k_mlp_rows(const MlpDev m, const float *_
   if (TRAIN && d_step && blockIdx.x == 0 && tid == 0) *d_step += 1;   // Adam step of this batch
 
   // gather the rows (BF16), zero padding columns
-  for (int e = tid; e < MR * ni; e += 256) {
+  for (int e = tid; e < MR * ni; e += RT) {
     const int r = e / ni, k = e % ni;
     const int64_t src = idx ? idx[boff + r0 + r] : (int64_t)(r0 + r);
     sX[r * sx + k] = k < m.nin ? f2bf(X[src * m.nin + k]) : (uint16_t)0;
@@ -118,7 +120,7 @@ __global__ __launch_bounds__(256) void k_mlp_rows(const MlpDev m, const float *_
   __syncthreads();
   const float *b1 = m.p + m.off[1], *b2 = m.p + m.off[3], *b3 = m.p + m.off[5];
   // layer 1: A1 = relu(X W1 + b1) (BF16; A1 > 0 is the backward mask)
-  for (int t = wave; t < nh / 32; t += 4) {
+  for (int t = wave; t < nh / 32; t += RW) {
     f32x16_t acc = {};
     tile_mma(acc, sX, sx, m.w1t, ni, t * 32, ni, lane);
 #pragma unroll
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(256) void k_mlp_rows(const MlpDev m, const float *_
   }
   __syncthreads();
   // layer 2
-  for (int t = wave; t < nh / 32; t += 4) {
+  for (int t = wave; t < nh / 32; t += RW) {
     f32x16_t acc = {};
     tile_mma(acc, sA1, sh, m.w2t, nh, t * 32, nh, lane);
 #pragma unroll
@@ -141,19 +143,24 @@ __global__ __launch_bounds__(256) void k_mlp_rows(const MlpDev m, const float *_
     }
   }
   __syncthreads();
-  // layer 3: O = A2 W3 + b3 (one tile)
-  if (wave == 0) {
+  // layer 3: O = A2 W3 + b3 (one tile): the reduction split over four
+  // wavefronts, partial tiles summed in a fixed order (sZ1 is free here)
+  float *part3 = reinterpret_cast<float *>(sZ1);               // [4][32][32]
+  if (wave < 4) {
+    const int kq = nh / 4;
     f32x16_t acc = {};
-    tile_mma(acc, sA2, sh, m.w3t, nh, 0, nh, lane);
+    tile_mma(acc, sA2 + wave * kq, sh, m.w3t + wave * kq, nh, 0, kq, lane);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int c = lane & 31;
-      sO[acc_row(q, lane) * NO + c] = acc[q] + b3[c];
-    }
+    for (int q = 0; q < 16; ++q) part3[wave * MR * NO + acc_row(q, lane) * NO + (lane & 31)] = acc[q];
+  }
+  __syncthreads();
+  for (int e = tid; e < MR * NO; e += RT) {
+    const int c = e % NO;
+    sO[e] = ((part3[e] + part3[MR * NO + e]) + (part3[2 * MR * NO + e] + part3[3 * MR * NO + e])) + b3[c];
   }
   __syncthreads();
   if constexpr (!TRAIN) {
-    for (int e = tid; e < MR * m.nout; e += 256) {
+    for (int e = tid; e < MR * m.nout; e += RT) {
       const int r = e / m.nout, k = e % m.nout;
       if (r0 + r < B) out[(int64_t)(r0 + r) * m.nout + k] = sO[r * NO + k];
     }
@@ -184,7 +191,7 @@ __global__ __launch_bounds__(256) void k_mlp_rows(const MlpDev m, const float *_
     // transposed operands of the weight gradients: thread = feature, 32
     // consecutive batch columns (64 bytes) per store group
     auto store_t = [&](const uint16_t *S, int stride, int nfeat, uint16_t *dst) {
-      for (int f = tid; f < nfeat; f += 256) {
+      for (int f = tid; f < nfeat; f += RT) {
         u32x4_t *d = reinterpret_cast<u32x4_t *>(dst + (int64_t)f * B + r0);
 #pragma unroll
         for (int c = 0; c < MR / 8; ++c) {
@@ -203,7 +210,7 @@ __global__ __launch_bounds__(256) void k_mlp_rows(const MlpDev m, const float *_
     store_t(sdO, so, NO, dOT);
     // dZ2 = (dO W3^T) * (A2 > 0), into the A2 buffer (each element's mask
     // is read by the lane that overwrites it)
-    for (int t = wave; t < nh / 32; t += 4) {
+    for (int t = wave; t < nh / 32; t += RW) {
       f32x16_t acc = {};
       tile_mma(acc, sdO, so, m.w3, NO, t * 32, NO, lane);
 #pragma unroll
@@ -215,7 +222,7 @@ __global__ __launch_bounds__(256) void k_mlp_rows(const MlpDev m, const float *_
     __syncthreads();
     store_t(sA2, sh, nh, dZ2T);
     // dZ1 = (dZ2 W2^T) * (A1 > 0)
-    for (int t = wave; t < nh / 32; t += 4) {
+    for (int t = wave; t < nh / 32; t += RW) {
       f32x16_t acc = {};
       tile_mma(acc, sA2, sh, m.w2, nh, t * 32, nh, lane);
 #pragma unroll
@@ -493,7 +500,7 @@ StepBufs handle_bufs(bm_mlp *h) {
 void step(bm_mlp *h, const StepBufs &sb, const float *X, const float *U, const int *idx, const int64_t *d_batch,
           int B, int adam, float lr) {
   const MlpDev md = h->dev();
-  hipLaunchKernelGGL(k_mlp_rows<true>, dim3(B / MR), dim3(256), rows_shmem(h), h->stream, md, X, U, idx, d_batch,
+  hipLaunchKernelGGL(k_mlp_rows<true>, dim3(B / MR), dim3(RT), rows_shmem(h), h->stream, md, X, U, idx, d_batch,
                      B, nullptr, sb.XT, sb.A1T, sb.A2T, sb.dOT, sb.dZ2T, sb.dZ1T, sb.loss,
                      adam ? h->d_step.as<int64_t>() : nullptr);
   WgradArgs w{};
@@ -542,7 +549,7 @@ extern "C" {
 int bm_mlp_create(int nin, int nh, int nu, const float *params, bm_mlp **out) {
   return guard([&] {
     BM_REQUIRE(nin > 0 && nin <= MAXNI, "nin must be in [1, 256]");
-    BM_REQUIRE(nh % 32 == 0 && nh > 0 && nh <= MAXNH, "nh must be a multiple of 32 in [32, 256]");
+    BM_REQUIRE(nh % 64 == 0 && nh > 0 && nh <= MAXNH, "nh must be a multiple of 64 in [64, 256]");
     BM_REQUIRE(nu > 0 && 2 * nu <= NO, "nu must be in [1, 16]");
     auto h = std::make_unique<bm_mlp>();
     h->nin = nin;
@@ -612,7 +619,7 @@ int bm_mlp_forward(bm_mlp *h, const float *X, int B, float *out) {
     ensure_cap(h, Bp);
     BM_HIP(hipMemsetAsync(h->xb.p, 0, (size_t)Bp * h->nin * 4, h->stream));
     BM_HIP(hipMemcpyAsync(h->xb.p, X, (size_t)B * h->nin * 4, hipMemcpyHostToDevice, h->stream));
-    hipLaunchKernelGGL(k_mlp_rows<false>, dim3(Bp / MR), dim3(256), rows_shmem(h), h->stream, h->dev(),
+    hipLaunchKernelGGL(k_mlp_rows<false>, dim3(Bp / MR), dim3(RT), rows_shmem(h), h->stream, h->dev(),
                        h->xb.as<float>(), nullptr, nullptr, nullptr, B, h->outb.as<float>(), nullptr, nullptr,
                        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     BM_HIP(hipGetLastError());

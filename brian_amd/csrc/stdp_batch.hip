@@ -13,9 +13,17 @@
 //                w[i] += A_post, clip (stdp.py:371-393); post updater on a
 //                neuron spike A_post += Am, w[:] += A_pre, clip
 //   resets:      S[reset_var] = vr for a spiking neuron
-// Layout: one wavefront per instance; input i of the instance lives in lane
-// i % 64, register slot i / 64 (up to 64*NJ inputs).  FP32 weights are
-// accumulated with Kahan compensation; FP64 (check mode) uses plain adds.
+// Layout: one workgroup of NW wavefronts per instance.  Wavefront v holds
+// the contiguous inputs [v*64*NJ, (v+1)*64*NJ): input i lives in lane i % 64,
+// register slot (i / 64) % NJ.  Every wavefront advances the instance's
+// neuron itself (the same operands in the same order give the same bits), so
+// the only exchange per step is the propagation list: each wavefront writes
+// the weights of its spiking inputs in ascending order to LDS, one barrier,
+// then every lane folds the lists of wavefronts 0..NW-1 from the left.  The
+// lists alternate between two LDS buffers by step parity, so one barrier per
+// step keeps a fast wavefront from overwriting a list still being read.
+// FP32 weights are accumulated with Kahan compensation; FP64 (check mode)
+// uses plain adds.
 #include "common.hpp"
 #include "../../include/brianmi.h"
 
@@ -27,7 +35,7 @@
 namespace bm {
 
 constexpr int SB_MAXM = 4;
-constexpr int SB_NJ = 16;     // input slots per lane: up to 1024 inputs
+constexpr int SB_MAXIN = 1024;   // inputs per instance: NW * NJ * 64 <= 1024
 
 struct StdpBatchDev {
   int ninst, nin, M, th_var, tgt_var, reset_var, update, interactions;
@@ -45,32 +53,38 @@ struct StdpBatchDev {
 template <typename T>
 __device__ inline T clipw(T x, T lo, T hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
-template <typename T, int M, bool KAHAN>
-__global__ __launch_bounds__(64) void k_stdp_batch(const StdpBatchDev d, int64_t n0, int nsteps) {
+template <typename T, int M, bool KAHAN, int NW, int NJ>
+__global__ __launch_bounds__(64 * NW) void k_stdp_batch(const StdpBatchDev d, int64_t n0, int nsteps) {
 #pragma clang fp contract(off)
   const int inst = blockIdx.x;
-  if (inst >= d.ninst) return;
-  const int lane = threadIdx.x;
+  if (inst >= d.ninst) return;  // uniform over the workgroup
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
   const int nin = d.nin;
+  const int first = wv * 64 * NJ;
   T *w = static_cast<T *>(d.w) + (int64_t)inst * nin;
   T *wc = static_cast<T *>(d.wc) + (int64_t)inst * nin;
   T *ap = static_cast<T *>(d.apre) + (int64_t)inst * nin;
-  T rw[SB_NJ], rc[SB_NJ], ra[SB_NJ];
-  uint32_t key[SB_NJ];
+  __shared__ T sh_list[NW > 1 ? 2 * NW * 64 * NJ : 1];
+  __shared__ int sh_cnt[NW > 1 ? 2 * NW : 1];
+  T rw[NJ], rc[NJ], ra[NJ];
+  uint32_t key[NJ];
+  bool valid[NJ];
+  // branch-free over the slots: a conditional block per slot makes the
+  // compiler copy whole register arrays at every merge
 #pragma unroll
-  for (int j = 0; j < SB_NJ; ++j) {
-    const int i = j * 64 + lane;
-    rw[j] = (T)0;
-    rc[j] = (T)0;
-    ra[j] = (T)0;
-    key[j] = 0u;
-    if (i < nin) {
-      rw[j] = w[i];
-      ra[j] = ap[i];
-      if (KAHAN) rc[j] = wc[i];
-      // stream = global input index (same as a PoissonGroup of ninst*nin)
-      key[j] = stream_key32(d.seed, (uint64_t)((int64_t)inst * nin + i));
-    }
+  for (int j = 0; j < NJ; ++j) {
+    const int i = first + j * 64 + lane;
+    valid[j] = i < nin;
+    const int ic = valid[j] ? i : 0;
+    const T wv0 = w[ic], av0 = ap[ic];
+    T cv0 = (T)0;
+    if constexpr (KAHAN) cv0 = wc[ic];
+    rw[j] = valid[j] ? wv0 : (T)0;
+    ra[j] = valid[j] ? av0 : (T)0;
+    rc[j] = valid[j] ? cv0 : (T)0;
+    // stream = global input index (same as a PoissonGroup of ninst*nin)
+    key[j] = stream_key32(d.seed, (uint64_t)((int64_t)inst * nin + i));
   }
   T s[M];
 #pragma unroll
@@ -87,12 +101,9 @@ __global__ __launch_bounds__(64) void k_stdp_batch(const StdpBatchDev d, int64_t
   for (int st = 0; st < nsteps; ++st) {
     const uint32_t n = (uint32_t)(n0 + st);
     // ---- groups: inputs, neuron state update + threshold, trace decay
-    bool spk[SB_NJ];
+    bool spk[NJ];
 #pragma unroll
-    for (int j = 0; j < SB_NJ; ++j) {
-      const int i = j * 64 + lane;
-      spk[j] = (i < nin) && (uint64_t)weyl_u32(key[j], n) < d.thr;
-    }
+    for (int j = 0; j < NJ; ++j) spk[j] = valid[j] && (uint64_t)weyl_u32(key[j], n) < d.thr;
     {
       T tmp[M];
 #pragma unroll
@@ -111,20 +122,42 @@ __global__ __launch_bounds__(64) void k_stdp_batch(const StdpBatchDev d, int64_t
       if (r == d.th_var) vth = s[r];
     const bool post = vth > vt;
 #pragma unroll
-    for (int j = 0; j < SB_NJ; ++j) ra[j] = dp * ra[j] + (T)0;
+    for (int j = 0; j < NJ; ++j) ra[j] = dp * ra[j] + (T)0;
     apost = dm * apost + (T)0;
     // ---- connections: propagation in ascending input order
     T x = s[0];
 #pragma unroll
     for (int r = 0; r < M; ++r)
       if (r == d.tgt_var) x = s[r];
+    if constexpr (NW == 1) {
 #pragma unroll
-    for (int j = 0; j < SB_NJ; ++j) {
-      unsigned long long mask = __ballot(spk[j]);
-      while (mask) {
-        const int l = __ffsll((long long)mask) - 1;
-        mask &= mask - 1ull;
-        x = x + __shfl(rw[j], l);
+      for (int j = 0; j < NJ; ++j) {
+        unsigned long long mask = __ballot(spk[j]);
+        while (mask) {
+          const int l = __ffsll((long long)mask) - 1;
+          mask &= mask - 1ull;
+          x = x + __shfl(rw[j], l);
+        }
+      }
+    } else {
+      const int b = st & 1;
+      T *list = sh_list + (b * NW + wv) * 64 * NJ;
+      int cnt = 0;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const unsigned long long mask = __ballot(spk[j]);
+        const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        if (spk[j]) list[cnt + below] = rw[j];
+        cnt += __popcll(mask);
+      }
+      if (lane == 0) sh_cnt[b * NW + wv] = cnt;
+      __syncthreads();
+#pragma unroll
+      for (int v = 0; v < NW; ++v) {
+        const int c = __builtin_amdgcn_readfirstlane(sh_cnt[b * NW + v]);
+        const T *lv = sh_list + (b * NW + v) * 64 * NJ;
+        for (int k = 0; k < c; ++k) x = x + lv[k];
       }
     }
 #pragma unroll
@@ -132,7 +165,7 @@ __global__ __launch_bounds__(64) void k_stdp_batch(const StdpBatchDev d, int64_t
       if (r == d.tgt_var) s[r] = x;
     // ---- pre updater
 #pragma unroll
-    for (int j = 0; j < SB_NJ; ++j) {
+    for (int j = 0; j < NJ; ++j) {
       if (!spk[j]) continue;
       ra[j] = (d.interactions == 0 || d.interactions == 3) ? ra[j] + Ap : Ap;
       if (d.update == 0 && KAHAN) {
@@ -156,58 +189,93 @@ __global__ __launch_bounds__(64) void k_stdp_batch(const StdpBatchDev d, int64_t
       // 'nearest_pre' writes A_post=+Am, an assignment (stdp.py:500-502)
       apost = d.interactions == 0 ? apost + Am : Am;
 #pragma unroll
-      for (int j = 0; j < SB_NJ; ++j) {
-        const int i = j * 64 + lane;
-        if (i >= nin) continue;
+      for (int j = 0; j < NJ; ++j) {
+        // slots beyond nin hold w = A_pre = 0 and are never stored back
+        T nw, nc = rc[j];
         if (d.update == 0 && KAHAN) {
           const T y = ra[j] - rc[j];
           const T t = rw[j] + y;
-          rc[j] = (t - rw[j]) - y;
-          rw[j] = t;
+          nc = (t - rw[j]) - y;
+          nw = t;
         } else if (d.update == 0) {
-          rw[j] = rw[j] + ra[j];
+          nw = rw[j] + ra[j];
         } else if (Ap < (T)0) {
-          rw[j] = rw[j] * ((T)1 + ra[j]);
+          nw = rw[j] * ((T)1 + ra[j]);
         } else {
-          rw[j] = rw[j] + (wmax - rw[j]) * ra[j];
+          nw = rw[j] + (wmax - rw[j]) * ra[j];
         }
-        const T cl = clipw(rw[j], wmin, wmax);
-        if (KAHAN && cl != rw[j]) rc[j] = (T)0;
+        const T cl = clipw(nw, wmin, wmax);
+        if (KAHAN && cl != nw) nc = (T)0;
         rw[j] = cl;
+        rc[j] = nc;
       }
       // ---- reset
 #pragma unroll
       for (int r = 0; r < M; ++r)
         if (r == d.reset_var) s[r] = vr;
-      if (lane == 0 && d.raster != nullptr) {
+      if (threadIdx.x == 0 && d.raster != nullptr) {
         const unsigned long long pos = atomicAdd(d.raster_count, 1ull);
         if (pos < (unsigned long long)d.raster_cap) d.raster[pos] = ((uint64_t)n << 32) | (uint64_t)inst;
       }
     }
   }
 #pragma unroll
-  for (int j = 0; j < SB_NJ; ++j) {
-    const int i = j * 64 + lane;
-    if (i < nin) {
+  for (int j = 0; j < NJ; ++j) {
+    const int i = first + j * 64 + lane;
+    if (valid[j]) {
       w[i] = rw[j];
       ap[i] = ra[j];
       if (KAHAN) wc[i] = rc[j];
     }
   }
-  if (lane == 0) {
+  if (threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < M; ++q) static_cast<T *>(d.S)[(int64_t)inst * M + q] = s[q];
     static_cast<T *>(d.apost)[inst] = apost;
   }
 }
 
+// Layouts (wavefronts per instance NW, input slots per lane NJ) compiled:
+// NJ in {1, 2, 4}, NW a power of two, NW * NJ <= 16.
+inline bool stdp_layout_ok(int nw, int nj, int nin) {
+  if (!(nj == 1 || nj == 2 || nj == 4)) return false;
+  if (!(nw == 1 || nw == 2 || nw == 4 || nw == 8 || nw == 16)) return false;
+  return nw * nj <= 16 && nw * nj * 64 >= nin;
+}
+
+// Default layout: four input slots per lane (no register spills, ~45
+// VGPRs), as many wavefronts as the inputs need.
+inline void stdp_default_layout(int nin, int *nw, int *nj) {
+  const int slots = (nin + 63) / 64;
+  *nj = slots >= 4 ? 4 : (slots >= 2 ? 2 : 1);
+  int w = 1;
+  while (w * *nj < slots) w *= 2;
+  *nw = w;
+}
+
+template <typename T, int M, bool KAHAN>
+void launch_stdp_m(const StdpBatchDev &d, int nw, int nj, int64_t n0, int nsteps, hipStream_t st) {
+  const dim3 g(d.ninst);
+#define BM_SB(NW_, NJ_)                                                                        \
+  if (nw == NW_ && nj == NJ_) {                                                                \
+    hipLaunchKernelGGL((k_stdp_batch<T, M, KAHAN, NW_, NJ_>), g, dim3(64 * NW_), 0, st, d, n0, \
+                       nsteps);                                                                \
+    return;                                                                                    \
+  }
+  BM_SB(1, 1) BM_SB(2, 1) BM_SB(4, 1) BM_SB(8, 1) BM_SB(16, 1)
+  BM_SB(1, 2) BM_SB(2, 2) BM_SB(4, 2) BM_SB(8, 2)
+  BM_SB(1, 4) BM_SB(2, 4) BM_SB(4, 4)
+#undef BM_SB
+  throw Error(BM_EINVAL, "unsupported STDP batch layout");
+}
+
 template <typename T, bool KAHAN>
-void launch_stdp_batch(const StdpBatchDev &d, int64_t n0, int nsteps, hipStream_t st) {
+void launch_stdp_batch(const StdpBatchDev &d, int nw, int nj, int64_t n0, int nsteps, hipStream_t st) {
   switch (d.M) {
-    case 1: hipLaunchKernelGGL((k_stdp_batch<T, 1, KAHAN>), dim3(d.ninst), dim3(64), 0, st, d, n0, nsteps); break;
-    case 2: hipLaunchKernelGGL((k_stdp_batch<T, 2, KAHAN>), dim3(d.ninst), dim3(64), 0, st, d, n0, nsteps); break;
-    case 3: hipLaunchKernelGGL((k_stdp_batch<T, 3, KAHAN>), dim3(d.ninst), dim3(64), 0, st, d, n0, nsteps); break;
-    default: hipLaunchKernelGGL((k_stdp_batch<T, 4, KAHAN>), dim3(d.ninst), dim3(64), 0, st, d, n0, nsteps); break;
+    case 1: launch_stdp_m<T, 1, KAHAN>(d, nw, nj, n0, nsteps, st); break;
+    case 2: launch_stdp_m<T, 2, KAHAN>(d, nw, nj, n0, nsteps, st); break;
+    case 3: launch_stdp_m<T, 3, KAHAN>(d, nw, nj, n0, nsteps, st); break;
+    default: launch_stdp_m<T, 4, KAHAN>(d, nw, nj, n0, nsteps, st); break;
   }
 }
 
@@ -221,6 +289,7 @@ struct bm_stdp_batch {
   DevBuf w, wc, apre, S, apost, raster, raster_count;
   hipStream_t stream = nullptr;
   int64_t step = 0;
+  int waves = 1, slots = 4;  // wavefronts per instance, input slots per lane
   // optional kernel timing with events on the handle's stream
   bool profiling = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -261,7 +330,7 @@ int bm_stdp_batch_create(int precision, double dt, int ninst, int nin, int nvars
                          bm_stdp_batch **out) {
   return guard([&] {
     BM_REQUIRE(precision == 32 || precision == 64, "precision must be 32 or 64");
-    BM_REQUIRE(ninst > 0 && nin > 0 && nin <= 64 * SB_NJ, "inputs per instance must be in [1, 1024]");
+    BM_REQUIRE(ninst > 0 && nin > 0 && nin <= SB_MAXIN, "inputs per instance must be in [1, 1024]");
     BM_REQUIRE(nvars >= 1 && nvars <= SB_MAXM, "nvars must be in [1, 4]");
     BM_REQUIRE(th_var >= 0 && th_var < nvars && reset_var >= 0 && reset_var < nvars &&
                    tgt_var >= 0 && tgt_var < nvars, "variable index out of range");
@@ -269,6 +338,7 @@ int bm_stdp_batch_create(int precision, double dt, int ninst, int nin, int nvars
     BM_REQUIRE(interactions >= 0 && interactions <= 3, "interactions out of range");
     auto h = std::make_unique<bm_stdp_batch>();
     h->precision = precision;
+    stdp_default_layout(nin, &h->waves, &h->slots);
     BM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     StdpBatchDev &d = h->dev;
     d.ninst = ninst;
@@ -342,8 +412,10 @@ int bm_stdp_batch_run(bm_stdp_batch *h, int64_t nsteps, int steps_per_launch) {
     for (int64_t done = 0; done < nsteps; done += chunk) {
       const int n = (int)std::min<int64_t>(chunk, nsteps - done);
       if (h->profiling) BM_HIP(hipEventRecord(h->ev0, h->stream));
-      if (h->precision == 64) launch_stdp_batch<double, false>(h->dev, h->step + done, n, h->stream);
-      else launch_stdp_batch<float, true>(h->dev, h->step + done, n, h->stream);
+      if (h->precision == 64)
+        launch_stdp_batch<double, false>(h->dev, h->waves, h->slots, h->step + done, n, h->stream);
+      else
+        launch_stdp_batch<float, true>(h->dev, h->waves, h->slots, h->step + done, n, h->stream);
       BM_HIP(hipGetLastError());
       if (h->profiling) {
         BM_HIP(hipEventRecord(h->ev1, h->stream));
@@ -355,6 +427,25 @@ int bm_stdp_batch_run(bm_stdp_batch *h, int64_t nsteps, int steps_per_launch) {
       }
     }
     h->step += nsteps;
+  });
+}
+
+int bm_stdp_batch_set_layout(bm_stdp_batch *h, int waves_per_instance, int slots_per_lane) {
+  return guard([&] {
+    int nw = waves_per_instance, nj = slots_per_lane;
+    if (nw == 0 && nj == 0) stdp_default_layout(h->dev.nin, &nw, &nj);
+    BM_REQUIRE(stdp_layout_ok(nw, nj, h->dev.nin),
+               "layout must have slots_per_lane in {1, 2, 4}, a power-of-two waves_per_instance, "
+               "waves * slots <= 16 and waves * slots * 64 >= inputs per instance");
+    h->waves = nw;
+    h->slots = nj;
+  });
+}
+
+int bm_stdp_batch_get_layout(bm_stdp_batch *h, int *waves_per_instance, int *slots_per_lane) {
+  return guard([&] {
+    *waves_per_instance = h->waves;
+    *slots_per_lane = h->slots;
   });
 }
 

@@ -388,6 +388,14 @@ int bm_stdp_batch_sync(bm_stdp_batch *h);
  * waited for while on); get_profile returns launches and total ms since on. */
 int bm_stdp_batch_set_profiling(bm_stdp_batch *h, int on);
 int bm_stdp_batch_get_profile(bm_stdp_batch *h, int64_t *launches, double *total_ms);
+/* Launch layout (no counterpart in the reference: how the instances map onto
+ * the GPU).  Each instance runs on waves_per_instance wavefronts holding
+ * slots_per_lane inputs per lane: slots in {1, 2, 4}, waves a power of two,
+ * waves * slots <= 16 and waves * slots * 64 >= inputs per instance; (0, 0)
+ * restores the default (4 slots, as many wavefronts as the inputs need).
+ * Results do not depend on the layout. */
+int bm_stdp_batch_set_layout(bm_stdp_batch *h, int waves_per_instance, int slots_per_lane);
+int bm_stdp_batch_get_layout(bm_stdp_batch *h, int *waves_per_instance, int *slots_per_lane);
 /* what: 0 weights [ninst][nin], 1 A_pre [ninst][nin], 2 A_post [ninst], 3 state [ninst][nvars]. */
 int bm_stdp_batch_get(bm_stdp_batch *h, int what, double *out);
 /* Post-synaptic spikes sorted by (step, instance); steps == NULL only counts. */
@@ -446,8 +454,9 @@ int bm_fit_get_spikes(bm_fit *h, int32_t *steps, int32_t *cand, int64_t cap, int
 /* ------------------------------------------------------- learned proposal
  * MLP nin -> nh -> nh -> 2*nu (ReLU), Gaussian NLL over nu normalised
  * parameters (outputs: nu means, nu log-sigmas).  BF16 MFMA products with
- * FP32 accumulation (gfx950 32x32x16); FP32 master weights and Adam; nin
- * <= 128 with nh = 256 (the row kernel's activations fit 64 KiB of LDS).  No counterpart in the
+ * FP32 accumulation (gfx950 32x32x16); FP32 master weights and Adam; nh a
+ * multiple of 64 up to 256, nin <= 128 with nh = 256 (the row kernel's
+ * activations fit 64 KiB of LDS).  No counterpart in the
  * reference (its fitting starts from uniform random particles); it seeds the
  * particle swarm of the fitting loop.  params: flat FP32 W1[nin][nh] b1[nh]
  * W2[nh][nh] b2[nh] W3[nh][32] b3[32] (output layer padded to 32). */

@@ -29,13 +29,13 @@ def stdp_consts():
     return A, C, dp, dm, Ap, Am
 
 
-def make_batch(precision, ninst, nin, seed, wseed=5):
+def make_batch(precision, ninst, nin, seed, wseed=5, rate=RATE):
     A, C, dp, dm, Ap, Am = stdp_consts()
     rng = np.random.default_rng(wseed)
     w0 = rng.random((ninst, nin)) * gmax
     S0 = np.zeros((ninst, 2))
     S0[:, 0] = vr
-    h = StdpBatchHandle(precision, DT, ninst, nin, A, C, S0, 0, vt, 0, vr, 1, w0, RATE, seed,
+    h = StdpBatchHandle(precision, DT, ninst, nin, A, C, S0, 0, vt, 0, vr, 1, w0, rate, seed,
                         dp, dm, Ap, Am, 0.0, gmax)
     return h, w0
 
@@ -72,6 +72,42 @@ def test_stdp_batch_fp64_matches_oracle():
     st = h.state()
     np.testing.assert_allclose(st[:, 0], net.groups[gid].S[0], rtol=1e-12)
     assert np.all((h.weights() >= 0) & (h.weights() <= gmax))
+
+
+@pytest.mark.parametrize("nin", [1000, 300, 70])
+def test_stdp_batch_layouts_identical(nin):
+    """Every launch layout (wavefronts per instance x input slots per lane)
+    gives the same bits as the default layout, in FP64 and FP32-Kahan."""
+    ninst, seed, steps = 12, 5, 3000
+    layouts = [(nw, nj) for nj in (1, 2, 4) for nw in (1, 2, 4, 8, 16)
+               if nw * nj <= 16 and nw * nj * 64 >= nin]
+    for prec in (64, 32):
+        ref = None
+        for lay in [(0, 0)] + layouts:
+            # the same total input drive as 1000 inputs at 15 Hz
+            h, _ = make_batch(prec, ninst, nin, seed, rate=RATE * 1000 / nin)
+            h.set_layout(*lay)
+            if lay == (0, 0):
+                nw, nj = h.layout()
+                assert (nw, nj) in layouts
+            h.run(1200, steps_per_launch=500)
+            h.run(steps - 1200)
+            out = (h.weights(), h.a_pre(), h.a_post(), h.state(), *h.spikes())
+            h.close()
+            if ref is None:
+                ref = out
+                assert len(out[4]) > 10
+                continue
+            for a, b in zip(ref, out):
+                np.testing.assert_array_equal(a, b, err_msg=f"layout {lay} precision {prec}")
+    h, _ = make_batch(32, 2, nin, seed)
+    for bad in ((3, 4), (16, 4), (1, 3)):
+        with pytest.raises(Exception):
+            h.set_layout(*bad)
+    if nin > 64:
+        with pytest.raises(Exception):
+            h.set_layout(1, 1)
+    h.close()
 
 
 def test_stdp_batch_c3_fp32_vs_fp64_histogram():
