@@ -422,9 +422,7 @@ def bench_uneven(args):
     def build32(lay):
         return build_uneven(d, 32, window=args.window, npt=args.npt, record=False, tuning=lay)[0]
     tuning, tune_report = autotuned(build32, args.no_autotune, args.tuning)
-    from brian_amd import autotune as AT
     res = {}
-    layout, ltimes = (0, 0), {}
     for prec in (32, 64):
         if prec == 64 and args.no_fp64:
             continue

@@ -25,7 +25,8 @@ Pi64 = C.POINTER(C.c_int64)
 
 # bm_net_set_tuning keys (brianmi.h)
 TUNE = {"window": 1, "neurons_per_thread": 2, "wave_max": 3, "row_split": 4, "ring_bits": 5,
-        "ring_pack": 6, "use_graph": 7, "window_max": 8, "specialize": 9, "defer": 10, "lean": 11}
+        "ring_pack": 6, "use_graph": 7, "window_max": 8, "specialize": 9, "defer": 10, "lean": 11,
+        "persist": 12, "persist_chunk": 13}
 
 
 class BrianMIError(RuntimeError):
@@ -76,6 +77,7 @@ _SIGS = {
     "bm_net_get_connection": [P, C.c_int, Pi64, Pi32, Pi32],
     "bm_net_record": [P, C.c_int, i64],
     "bm_net_prepare": [P, C.c_int, C.c_int, C.c_int],
+    "bm_net_set_partition": [P, C.c_int, C.c_int, Pi64],
     "bm_net_run": [P, i64],
     "bm_net_sync": [P],
     "bm_net_current_step": [P, Pi64],
@@ -436,6 +438,12 @@ class NetHandle:
 
     def record(self, gid, capacity):
         call("bm_net_record", self.h, gid, int(capacity))
+
+    def set_partition(self, gid, offsets):
+        """The group as independent networks back to back: network p holds
+        neurons [offsets[p], offsets[p+1])."""
+        o = np.ascontiguousarray(offsets, dtype=np.int64)
+        call("bm_net_set_partition", self.h, int(gid), len(o) - 1, o.ctypes.data_as(Pi64))
 
     def prepare(self, window=0, neurons_per_thread=0, use_graph=True):
         call("bm_net_prepare", self.h, int(window), int(neurons_per_thread), 1 if use_graph else 0)

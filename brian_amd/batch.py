@@ -63,7 +63,7 @@ def _prepare(h, tuning, window, npt, use_graph):
             h.set_tuning(key, tuning[key])
     h.prepare(window=tuning.get("window", window), neurons_per_thread=tuning.get("neurons_per_thread", npt),
               use_graph=use_graph)
-    for key in ("wave_max", "row_split", "lean"):
+    for key in ("wave_max", "row_split", "lean", "persist_chunk", "persist"):
         if key in tuning:
             h.set_tuning(key, tuning[key])
 
@@ -91,5 +91,7 @@ def build_uneven(d, precision=32, window=0, npt=0, use_graph=True, record=True, 
     if record:
         # raster capacity for a mean rate of 40 Hz over 0.5 s unless given
         h.record(g, capacity or max(1 << 20, int(d["N"] * 40 * 0.5)))
+    # the networks of the batch, for persistent launches (tuning key "persist")
+    h.set_partition(g, d["offsets"])
     _prepare(h, tuning, window, npt, use_graph)
     return h, g, cids[0], cids[1]

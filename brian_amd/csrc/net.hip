@@ -94,6 +94,15 @@ struct HostGroup {
   JitKernel jit;          // specialised lean sweep for jit_K neurons per thread
   int jit_K = 0;
   bool jit_lean = false;  // jit holds k_lean (else the table-driven k_sweep)
+  // independent networks laid back to back (bm_net_set_partition): network
+  // p holds neurons [part_off[p], part_off[p+1]); the persistent kernel
+  // gives each network one workgroup, largest first (part_order)
+  std::vector<int64_t> part_off;
+  DevBuf part_off_d, part_order_d;
+  bool part_checked = false;
+  int64_t part_max = 0;
+  JitKernel pjit;         // k_persist<pjit_K>
+  int pjit_K = 0;
   struct Mon {
     int var = 0, timestep = 1, mode = 0;
     int64_t cap = 0;
@@ -161,6 +170,8 @@ struct bm_net {
   int Wcap = 1;           // largest window the rings are sized for
   bool specialize = true; // lean sweep compiled at run time for each group's shape
   bool lean_kernel = true; // specialised groups use k_lean (the model as code) rather than k_sweep tables
+  bool persist = false;   // batches of independent networks: one persistent launch per run (k_persist)
+  int persist_chunk = 0;  // steps per persistent launch, 0: the whole run
   bool use_graph = true;
   std::vector<std::pair<int, int>> mon_ids;   // (group, monitor within group)
   // tiling knobs (bm_net_set_tuning)
@@ -359,7 +370,7 @@ std::string lit(double v, bool f32) {
 // with every constant written out.  FP64 keeps the reference's arithmetic
 // (count loops, left-fold rows with no contraction); FP32 uses one fused
 // multiply-add per delivery and per non-zero entry of A.
-std::string lean_prelude(bm_net *net, const HostGroup &g) {
+std::string lean_prelude(bm_net *net, const HostGroup &g, const char *kernel_header = "net_lean.hpp") {
   const bool f32 = net->precision == 32;
   const int M = g.M;
   const int nring = (int)g.ring_desc.size();
@@ -415,9 +426,11 @@ std::string lean_prelude(bm_net *net, const HostGroup &g) {
   d << "__device__ inline bool lean_spike(const lean_t (&x)[LEAN_M]) { return x[" << g.th_var << "] > ";
   if (g.th_kind == TH_VAR) d << "x[" << g.th_vtvar << "]";
   else d << lit(g.vt, f32);
-  d << "; }\n}\n#include \"net_lean.hpp\"\n";
+  d << "; }\n}\n#include \"" << kernel_header << "\"\n";
   return d.str();
 }
+
+JitKernel jit_compile(const std::string &src, const std::string &name);
 
 JitKernel jit_sweep(bm_net *net, const HostGroup &g, int K) {
   std::vector<int> sh, var, dly, rv, ro, rvv;
@@ -450,6 +463,12 @@ JitKernel jit_sweep(bm_net *net, const HostGroup &g, int K) {
                                 : std::string("bm::k_sweep<") + (net->precision == 64 ? "double" : "float") + ", " +
                                       std::to_string(g.M) + ", " + std::to_string(K) + ", 0>";
   const std::string src = (lean ? lean_prelude(net, g) : d.str()) + "// " + name + "\n";
+  return jit_compile(src, name);
+}
+
+// Compiles a kernel source with the kernel headers (or takes it from the
+// cache) and loads the kernel `name`.
+JitKernel jit_compile(const std::string &src, const std::string &name) {
   std::lock_guard<std::mutex> lk(g_jit_mu);
   auto it = g_jit.find(src);
   if (it != g_jit.end()) return it->second;
@@ -500,6 +519,112 @@ void ensure_jit(bm_net *net) {
     g.jit_K = K;
     g.jit_lean = net->lean_kernel;
   }
+}
+
+// ---- persistent launches for batches of independent networks (k_persist)
+constexpr int PERSIST_BLOCK = 1024;    // PBLOCK of net_persist.hpp
+// neurons per thread of the largest body of k_persist: networks of up to
+// 16,384 neurons in FP32, 8,192 in FP64
+int persist_kmax(const bm_net *net) { return net->precision == 64 ? 8 : 16; }
+
+// Why a network cannot run persistently, or "" if it can.
+std::string persist_blocker(const bm_net *net) {
+  if (!net->prepared) return "the network is not prepared";
+  if (!net->specialize || !net->lean_kernel) return "persistent launches need the specialised lean kernels";
+  if (!net->stdps.empty()) return "STDP is not supported in persistent launches";
+  for (const auto &cp : net->conns) {
+    const HostConn &c = *cp;
+    if (late_mod(c)) return "delayed modulated connections are not supported in persistent launches";
+    if (c.src != c.tgt) return "every connection must stay within one group";
+    if (c.min_delay < 1) return "persistent launches need synaptic delays of at least one step";
+    // the ordering argument of net_persist.hpp: deliveries of three
+    // consecutive steps never land in the slots being read and zeroed
+    if (net->groups[c.tgt]->D < c.max_delay + 3) return "the ring is too shallow for persistent launches";
+  }
+  for (const auto &gp : net->groups) {
+    const HostGroup &g = *gp;
+    if (g.kind != GK_NEURON) return "generator and Poisson groups are not supported in persistent launches";
+    if (!fast_eligible(g) || g.defer || g.need_hist) return "a group's model is outside the lean kernel's";
+    if (g.part_off.empty()) return "every group needs a partition into networks (bm_net_set_partition)";
+    if (g.part_max > (int64_t)PERSIST_BLOCK * persist_kmax(net))
+      return net->precision == 64 ? "a network exceeds 8,192 neurons (FP64 persistent launches)"
+                                  : "a network exceeds 16,384 neurons (persistent launches)";
+  }
+  return "";
+}
+
+// Checks on the device, once per partition, that no synapse joins two of
+// its networks.
+void check_partitions(bm_net *net) {
+  for (auto &gp : net->groups) {
+    HostGroup &g = *gp;
+    if (g.part_checked) continue;
+    DevBuf bad;
+    bad.alloc(4);
+    BM_HIP(hipMemsetAsync(bad.p, 0, 4, net->stream));
+    const int nparts = (int)g.part_off.size() - 1;
+    for (int o = 0; o < g.dev.nout; ++o) {
+      const OutDev &od = g.dev.outs[o];
+      launch_check_partition(od.row_ptr, od.col, od.src_n, od.src_off, od.tgt_off, g.part_off_d.as<int64_t>(),
+                             nparts, bad.as<int>(), net->stream);
+      BM_HIP(hipGetLastError());
+    }
+    int h = 0;
+    BM_HIP(hipMemcpyAsync(&h, bad.p, 4, hipMemcpyDeviceToHost, net->stream));
+    BM_HIP(hipStreamSynchronize(net->stream));
+    BM_REQUIRE(h == 0, "a synapse joins two networks of the partition");
+    g.part_checked = true;
+  }
+}
+
+void ensure_persist_jit(bm_net *net) {
+  for (auto &gp : net->groups) {
+    HostGroup &g = *gp;
+    const int K = persist_kmax(net);
+    if (g.pjit.fn && g.pjit_K == K) continue;
+    const std::string name = "bm::k_persist<" + std::to_string(K) + ">";
+    g.pjit = jit_compile(lean_prelude(net, g, "net_persist.hpp") + "// " + name + "\n", name);
+    g.pjit_K = K;
+  }
+}
+
+// One run of nsteps as persistent launches (chunks of persist_chunk steps);
+// each launch applies the tail of its last step, as run() does.
+void run_persist(bm_net *net, int64_t nsteps) {
+  const std::string why = persist_blocker(net);
+  BM_REQUIRE(why.empty(), why);
+  check_partitions(net);
+  ensure_persist_jit(net);
+  const int64_t chunk = net->persist_chunk > 0 ? net->persist_chunk : nsteps;
+  for (int64_t done = 0; done < nsteps; done += chunk) {
+    const int n = (int)std::min<int64_t>(chunk, nsteps - done);
+    hipEvent_t a = nullptr;
+    if (net->profiling) {
+      BM_HIP(hipEventCreate(&a));
+      net->ev.push_back(a);
+      BM_HIP(hipEventRecord(a, net->stream));
+    }
+    for (auto &gp : net->groups) {
+      HostGroup &g = *gp;
+      const int nparts = (int)g.part_off.size() - 1;
+      const int64_t *po = g.part_off_d.as<int64_t>();
+      const int32_t *ord = g.part_order_d.as<int32_t>();
+      int64_t n0 = net->step + done;
+      int ns = n, skip = 1;
+      void *args[] = {(void *)&g.dev, (void *)&po, (void *)&ord, (void *)&nparts, (void *)&n0, (void *)&ns,
+                      (void *)&skip};
+      BM_HIP(hipModuleLaunchKernel(g.pjit.fn, (unsigned)nparts, 1, 1, PERSIST_BLOCK, 1, 1, 0, net->stream, args,
+                                   nullptr));
+      BM_HIP(hipGetLastError());
+    }
+    if (net->profiling) {
+      hipEvent_t b;
+      BM_HIP(hipEventCreate(&b));
+      net->ev.push_back(b);
+      BM_HIP(hipEventRecord(b, net->stream));
+    }
+  }
+  net->step += nsteps;
 }
 
 void launch_group(bm_net *net, HostGroup &g, int64_t rel0, int W, int skip_first_tail,
@@ -1197,6 +1322,10 @@ void run(bm_net *net, int64_t nsteps) {
   BM_REQUIRE(net->prepared, "bm_net_prepare must be called before bm_net_run");
   BM_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
   if (nsteps == 0) return;
+  if (net->persist) {
+    run_persist(net, nsteps);
+    return;
+  }
   ensure_jit(net);
   const int W = net->W;
   int64_t done = 0;
@@ -2035,6 +2164,37 @@ int bm_net_record(bm_net *net, int gid, int64_t capacity) {
   });
 }
 
+int bm_net_set_partition(bm_net *net, int gid, int nparts, const int64_t *offsets) {
+  return guard([&] {
+    HostGroup &g = grp(net, gid);
+    BM_REQUIRE(g.kind == GK_NEURON, "only neuron groups are partitioned");
+    BM_REQUIRE(nparts >= 1 && offsets != nullptr, "nparts must be >= 1");
+    BM_REQUIRE(offsets[0] == 0 && offsets[nparts] == g.N, "offsets must run from 0 to the group's size");
+    int64_t mx = 0;
+    for (int p = 0; p < nparts; ++p) {
+      BM_REQUIRE(offsets[p + 1] > offsets[p], "offsets must be strictly increasing");
+      mx = std::max(mx, offsets[p + 1] - offsets[p]);
+    }
+    g.part_off.assign(offsets, offsets + nparts + 1);
+    g.part_max = mx;
+    // largest networks first, so the longest workgroups start first
+    std::vector<int32_t> order(nparts);
+    for (int p = 0; p < nparts; ++p) order[p] = p;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+      return offsets[a + 1] - offsets[a] > offsets[b + 1] - offsets[b];
+    });
+    upload_raw(g.part_off_d, g.part_off.data(), g.part_off.size(), net->stream);
+    upload_raw(g.part_order_d, order.data(), order.size(), net->stream);
+    BM_HIP(hipStreamSynchronize(net->stream));
+    g.part_checked = false;
+    if (net->persist) {
+      const std::string why = persist_blocker(net);
+      BM_REQUIRE(why.empty(), why);
+      check_partitions(net);
+    }
+  });
+}
+
 int bm_net_prepare(bm_net *net, int window, int neurons_per_thread, int use_graph) {
   return guard([&] { prepare(net, window, neurons_per_thread, use_graph); });
 }
@@ -2103,6 +2263,7 @@ int bm_net_set_tuning(bm_net *net, int key, int value) {
         break;
       case BM_TUNE_SPECIALIZE:
         BM_REQUIRE(net->defer_W == 0 || value != 0, "deferred propagation needs the specialised kernels");
+        BM_REQUIRE(!net->persist || value != 0, "persistent launches need the specialised kernels");
         net->specialize = value != 0;
         break;
       case BM_TUNE_DEFER:
@@ -2110,7 +2271,20 @@ int bm_net_set_tuning(bm_net *net, int key, int value) {
         net->defer_req = value != 0;
         break;
       case BM_TUNE_LEAN:
+        BM_REQUIRE(!net->persist || value != 0, "persistent launches need the lean kernels");
         net->lean_kernel = value != 0;
+        break;
+      case BM_TUNE_PERSIST:
+        if (value != 0) {
+          const std::string why = persist_blocker(net);
+          BM_REQUIRE(why.empty(), why);
+          check_partitions(net);
+        }
+        net->persist = value != 0;
+        break;
+      case BM_TUNE_PERSIST_CHUNK:
+        BM_REQUIRE(value >= 0, "persist_chunk must be >= 0 (0: the whole run in one launch)");
+        net->persist_chunk = value;
         break;
       default:
         throw Error(BM_EINVAL, "unknown tuning key");
@@ -2139,6 +2313,8 @@ int bm_net_get_tuning(bm_net *net, int key, int *value) {
       case BM_TUNE_SPECIALIZE: *value = net->specialize ? 1 : 0; break;
       case BM_TUNE_DEFER: *value = net->prepared ? (net->defer_W > 0 ? 1 : 0) : net->defer_req; break;
       case BM_TUNE_LEAN: *value = net->lean_kernel ? 1 : 0; break;
+      case BM_TUNE_PERSIST: *value = net->persist ? 1 : 0; break;
+      case BM_TUNE_PERSIST_CHUNK: *value = net->persist_chunk; break;
       default: throw Error(BM_EINVAL, "unknown tuning key");
     }
   });
@@ -2365,7 +2541,7 @@ int bm_net_get_profile(bm_net *net, int64_t *launches, double *total_ms, double 
   return guard([&] {
     *launches = net->prof_launches;
     *total_ms = net->prof_ms;
-    *window_steps = net->W;
+    *window_steps = net->persist ? (double)net->persist_chunk : net->W;
   });
 }
 

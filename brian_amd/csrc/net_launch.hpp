@@ -27,6 +27,8 @@ void launch_late_mod(const LateModDev &lm, hipStream_t st, const int64_t *d_base
 void launch_hist_clear(unsigned int *const *counts, const int *H, int ngroups, const int64_t *d_base,
                        int64_t rel_next, hipStream_t st);
 void launch_advance(int64_t *d_base, int64_t by, hipStream_t st);
+void launch_check_partition(const int64_t *row_ptr, const int32_t *col, int64_t nrows, int64_t src_off,
+                            int64_t tgt_off, const int64_t *part_off, int nparts, int *bad, hipStream_t st);
 
 // block-diagonal random connectivity built on the device (conngen.hip)
 struct GenSpec {

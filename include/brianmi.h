@@ -294,6 +294,14 @@ int bm_net_read_state_monitor(bm_net *net, int mid, double *out, int64_t max_sam
  * neurons_per_thread 0 = automatic. */
 int bm_net_prepare(bm_net *net, int window, int neurons_per_thread, int use_graph);
 
+/* The group as nparts independent networks laid back to back: network p
+ * holds neurons [offsets[p], offsets[p+1]), offsets[0] = 0, offsets[nparts]
+ * = N.  No counterpart in the reference (each of its networks is its own
+ * Network object, brian/network.py:313-586); it is the batch layout the
+ * persistent launch (BM_TUNE_PERSIST) works on.  Whether any synapse joins
+ * two networks is checked on the device before the first persistent run. */
+int bm_net_set_partition(bm_net *net, int gid, int nparts, const int64_t *offsets);
+
 /* Launch-shape knobs (no counterpart in the reference, whose step is a
  * Python loop; they are what the autotuner in brian_amd/autotune.py
  * searches).  None changes results: count rings make deliveries
@@ -324,7 +332,18 @@ int bm_net_prepare(bm_net *net, int window, int neurons_per_thread, int use_grap
  *                              its model compiled in (propagator, weights,
  *                              reset and threshold values as literals; spikes
  *                              kept as per-step wavefront ballots until the end
- *                              of the window); 0: the table-driven form */
+ *                              of the window); 0: the table-driven form
+ *   BM_TUNE_PERSIST            (after prepare) 1: each run is one persistent
+ *                              launch per group in which every network of the
+ *                              group's partition (bm_net_set_partition) is
+ *                              advanced by one workgroup through all the run's
+ *                              steps, state in registers, one workgroup
+ *                              barrier per step.  Needs lean groups with count
+ *                              rings, connections within a group, delays of
+ *                              at least one step, no STDP, networks of at
+ *                              most 16,384 neurons (8,192 in FP64); refused
+ *                              otherwise
+ *   BM_TUNE_PERSIST_CHUNK      steps per persistent launch (0: the whole run) */
 enum {
   BM_TUNE_WINDOW = 1,
   BM_TUNE_NEURONS_PER_THREAD = 2,
@@ -336,7 +355,9 @@ enum {
   BM_TUNE_WINDOW_MAX = 8,
   BM_TUNE_SPECIALIZE = 9,
   BM_TUNE_DEFER = 10,
-  BM_TUNE_LEAN = 11
+  BM_TUNE_LEAN = 11,
+  BM_TUNE_PERSIST = 12,
+  BM_TUNE_PERSIST_CHUNK = 13
 };
 int bm_net_set_tuning(bm_net *net, int key, int value);
 int bm_net_get_tuning(bm_net *net, int key, int *value);

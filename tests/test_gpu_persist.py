@@ -1,0 +1,104 @@
+"""Persistent per-network launches (BM_TUNE_PERSIST) on uneven batches.
+
+The persistent kernel runs the same per-neuron arithmetic as the windowed
+lean kernel (the model compiled in from the same source), and count-ring
+deliveries are order-independent, so its rasters and states must equal the
+windowed kernel's bit for bit, in FP32 and FP64, whatever the run lengths
+and the steps per launch.  The FP64 windowed path itself equals the oracle
+(tests/test_gpu_uneven.py); here the FP64 persistent path is also compared
+with the oracle directly.
+"""
+import numpy as np
+import pytest
+
+from brian_amd import workloads as W
+from tests.netbuild import device_uneven, first_divergence, host_uneven_csr, oracle_uneven
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(d, prec, persist, chunks, persist_chunk=0):
+    h, g, _, _ = device_uneven(d, precision=prec)
+    if persist:
+        h.set_tuning("persist_chunk", persist_chunk)
+        h.set_tuning("persist", 1)
+    for c in chunks:
+        h.run(c)
+    h.sync()
+    out = (h.get_spikes(g), [h.get_state(g, v, d["N"]) for v in range(3)])
+    h.close()
+    return out
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_persist_equals_windowed(prec):
+    # sizes up to 5000 neurons: the 1-, 2-, 4- and 8-slot bodies all run
+    d = W.uneven_batch(nnet=24, seed=11, lo=300, hi=5000 if prec == 64 else 9000)
+    chunks = (1, 700, 1299)
+    (ws, wi), wst = _run(d, prec, False, chunks)
+    (ps, pi), pst = _run(d, prec, True, chunks, persist_chunk=333)
+    print(f"FP{prec}: {len(ws)} spikes windowed, {len(ps)} persistent", flush=True)
+    assert len(ws) > 1000
+    np.testing.assert_array_equal(ps, ws)
+    np.testing.assert_array_equal(pi, wi)
+    for v in range(3):
+        np.testing.assert_array_equal(pst[v], wst[v])
+
+
+def test_persist_fp64_matches_oracle():
+    d = W.uneven_batch(nnet=8, seed=5, lo=500, hi=2000)
+    h, g, _, _ = device_uneven(d, precision=64)
+    h.set_tuning("persist", 1)
+    net, gid = oracle_uneven(d, host_uneven_csr(d))
+    for chunk in (1, 499, 1500):
+        h.run(chunk)
+        net.run(chunk)
+    s, i = h.get_spikes(g)
+    rs, ri = net.raster(gid)
+    assert len(rs) > 1000
+    assert first_divergence(s, i, rs, ri) is None
+    for var in range(3):
+        got = h.get_state(g, var, d["N"])
+        ref = net.groups[gid].S[var]
+        err = np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-3))
+        assert err <= 1e-12, (var, err)
+    h.close()
+
+
+def test_persist_then_windowed_continues():
+    """Switching between persistent and windowed runs keeps the simulation
+    identical to an all-windowed run."""
+    d = W.uneven_batch(nnet=12, seed=2, lo=400, hi=3000)
+    (ws, wi), wst = _run(d, 32, False, (900, 900))
+    h, g, _, _ = device_uneven(d, precision=32)
+    h.set_tuning("persist", 1)
+    h.run(900)
+    h.set_tuning("persist", 0)
+    h.run(500)
+    h.set_tuning("persist", 1)
+    h.run(400)
+    h.sync()
+    s, i = h.get_spikes(g)
+    np.testing.assert_array_equal(s, ws)
+    np.testing.assert_array_equal(i, wi)
+    for v in range(3):
+        np.testing.assert_array_equal(h.get_state(g, v, d["N"]), wst[v])
+    h.close()
+
+
+def test_persist_refusals():
+    # a network beyond the largest body
+    d = W.uneven_batch(nnet=2, seed=1, lo=17000, hi=17500)
+    h, g, _, _ = device_uneven(d, precision=32, record=False)
+    with pytest.raises(Exception, match="exceeds 16,384"):
+        h.set_tuning("persist", 1)
+    h.close()
+    # a partition that a connection crosses
+    d = W.uneven_batch(nnet=4, seed=1, lo=500, hi=900)
+    h, g, _, _ = device_uneven(d, precision=32, record=False)
+    off = d["offsets"].copy()
+    off[2] += 10
+    h.set_partition(g, off)
+    with pytest.raises(Exception, match="joins two networks"):
+        h.set_tuning("persist", 1)
+    h.close()
