@@ -523,9 +523,16 @@ void ensure_jit(bm_net *net) {
 
 // ---- persistent launches for batches of independent networks (k_persist)
 constexpr int PERSIST_BLOCK = 1024;    // PBLOCK of net_persist.hpp
-// neurons per thread of the largest body of k_persist: networks of up to
-// 16,384 neurons in FP32, 8,192 in FP64
-int persist_kmax(const bm_net *net) { return net->precision == 64 ? 8 : 16; }
+// neurons per thread of the largest body of k_persist (net_persist.hpp):
+// 20 in FP32, the last 4 slots' state in LDS (models of up to 3 variables,
+// 48 KiB), 16 for larger FP32 models, 8 in FP64
+int persist_kmax(const bm_net *net, const HostGroup &g) {
+  if (net->precision == 64) return 8;
+  return g.M <= 3 ? 20 : 16;
+}
+size_t persist_shmem(const bm_net *net, const HostGroup &g) {
+  return persist_kmax(net, g) == 20 ? (size_t)4 * g.M * PERSIST_BLOCK * net->tsize() : 0;
+}
 
 // Why a network cannot run persistently, or "" if it can.
 std::string persist_blocker(const bm_net *net) {
@@ -546,9 +553,9 @@ std::string persist_blocker(const bm_net *net) {
     if (g.kind != GK_NEURON) return "generator and Poisson groups are not supported in persistent launches";
     if (!fast_eligible(g) || g.defer || g.need_hist) return "a group's model is outside the lean kernel's";
     if (g.part_off.empty()) return "every group needs a partition into networks (bm_net_set_partition)";
-    if (g.part_max > (int64_t)PERSIST_BLOCK * persist_kmax(net))
-      return net->precision == 64 ? "a network exceeds 8,192 neurons (FP64 persistent launches)"
-                                  : "a network exceeds 16,384 neurons (persistent launches)";
+    if (g.part_max > (int64_t)PERSIST_BLOCK * persist_kmax(net, g))
+      return "a network exceeds " + std::to_string(PERSIST_BLOCK * persist_kmax(net, g)) +
+             " neurons, the largest a persistent workgroup holds";
   }
   return "";
 }
@@ -580,7 +587,7 @@ void check_partitions(bm_net *net) {
 void ensure_persist_jit(bm_net *net) {
   for (auto &gp : net->groups) {
     HostGroup &g = *gp;
-    const int K = persist_kmax(net);
+    const int K = persist_kmax(net, g);
     if (g.pjit.fn && g.pjit_K == K) continue;
     const std::string name = "bm::k_persist<" + std::to_string(K) + ">";
     g.pjit = jit_compile(lean_prelude(net, g, "net_persist.hpp") + "// " + name + "\n", name);
@@ -613,8 +620,8 @@ void run_persist(bm_net *net, int64_t nsteps) {
       int ns = n, skip = 1;
       void *args[] = {(void *)&g.dev, (void *)&po, (void *)&ord, (void *)&nparts, (void *)&n0, (void *)&ns,
                       (void *)&skip};
-      BM_HIP(hipModuleLaunchKernel(g.pjit.fn, (unsigned)nparts, 1, 1, PERSIST_BLOCK, 1, 1, 0, net->stream, args,
-                                   nullptr));
+      BM_HIP(hipModuleLaunchKernel(g.pjit.fn, (unsigned)nparts, 1, 1, PERSIST_BLOCK, 1, 1,
+                                   (unsigned)persist_shmem(net, g), net->stream, args, nullptr));
       BM_HIP(hipGetLastError());
     }
     if (net->profiling) {

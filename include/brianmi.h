@@ -341,8 +341,9 @@ int bm_net_set_partition(bm_net *net, int gid, int nparts, const int64_t *offset
  *                              barrier per step.  Needs lean groups with count
  *                              rings, connections within a group, delays of
  *                              at least one step, no STDP, networks of at
- *                              most 16,384 neurons (8,192 in FP64); refused
- *                              otherwise
+ *                              most 20,480 neurons in FP32 (16,384 for models
+ *                              of more than 3 variables, 8,192 in FP64);
+ *                              refused otherwise
  *   BM_TUNE_PERSIST_CHUNK      steps per persistent launch (0: the whole run) */
 enum {
   BM_TUNE_WINDOW = 1,

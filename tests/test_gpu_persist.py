@@ -32,8 +32,9 @@ def _run(d, prec, persist, chunks, persist_chunk=0):
 
 @pytest.mark.parametrize("prec", [32, 64])
 def test_persist_equals_windowed(prec):
-    # sizes up to 5000 neurons: the 1-, 2-, 4- and 8-slot bodies all run
-    d = W.uneven_batch(nnet=24, seed=11, lo=300, hi=5000 if prec == 64 else 9000)
+    # FP32 sizes up to 20,000 neurons: every body runs, the 20-slot one with
+    # 4 slots in LDS; FP64 up to 8,000 (its largest body)
+    d = W.uneven_batch(nnet=40 if prec == 32 else 24, seed=11, lo=300, hi=20000 if prec == 32 else 8000)
     chunks = (1, 700, 1299)
     (ws, wi), wst = _run(d, prec, False, chunks)
     (ps, pi), pst = _run(d, prec, True, chunks, persist_chunk=333)
@@ -88,9 +89,9 @@ def test_persist_then_windowed_continues():
 
 def test_persist_refusals():
     # a network beyond the largest body
-    d = W.uneven_batch(nnet=2, seed=1, lo=17000, hi=17500)
+    d = W.uneven_batch(nnet=2, seed=1, lo=21000, hi=21500)
     h, g, _, _ = device_uneven(d, precision=32, record=False)
-    with pytest.raises(Exception, match="exceeds 16,384"):
+    with pytest.raises(Exception, match="exceeds 20480"):
         h.set_tuning("persist", 1)
     h.close()
     # a partition that a connection crosses
