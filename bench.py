@@ -65,6 +65,14 @@ def sweep_bytes_per_neuron(prec, window, ring_sw):
     return 2 * STATE_VARS * (prec // 8) + BYTES_LAST + 2 * ring_sw * window
 
 
+# A persistent launch of S steps (one workgroup per network, the state on the
+# chip for the whole launch) moves per neuron the state in and out once and
+# the ring read + zero per step; the last spike step is read once and
+# written per spike (+4 B on the spike's bytes).
+def persist_bytes_per_neuron(prec, steps, ring_sw):
+    return 2 * STATE_VARS * (prec // 8) + 4 + 2 * ring_sw * steps
+
+
 def ring_sw(tuning, nch=2):
     """Bytes per neuron of the packed count ring the tuning selects: 8-bit
     counters (16 with ring_bits 16) for the nch incoming channels."""
@@ -213,6 +221,9 @@ def autotuned(build, no_autotune=False, fixed=None):
     if no_autotune:
         return {}, None
     layout, lay_res = autotune.tune_layout(lambda lay: build(lay))
+    if layout.get("persist"):
+        # the launch-shape knobs do not apply to persistent launches
+        return dict(layout), {"chosen": dict(layout), "layouts_us_per_step": [(l, t * 1e6) for l, t in lay_res]}
     h = build(layout)
     log = []
     best, used = autotune.tune(h, log=log)
@@ -463,6 +474,7 @@ def bench_uneven(args):
             launches, prof_ms, window = h.get_profile()
             sp = h.get_spikes(g)
             out["prof"] = (launches, prof_ms, window, len(sp[0]), events(sp[1]))
+            out["prof_steps"] = prof_steps
             h.set_profiling(False)
         h.close()
         res[prec] = out
@@ -472,14 +484,25 @@ def bench_uneven(args):
     r32 = res[32]
     launches, prof_ms, window, psp, pev = r32["prof"]
     avg_launch_s = prof_ms / 1e3 / max(launches, 1)
+    persist = bool(tuning.get("persist"))
     # 16-bit counters (source blocks of up to 20,000 neurons), two channels;
     # per-synapse delay byte on every event
-    alg_bytes = (d["N"] * sweep_bytes_per_neuron(32, window, 4) + psp / launches * BYTES_SPIKE
-                 + pev / launches * (BYTES_EVENT + 1))
-    roof = roofline(alg_bytes, avg_launch_s, hbm_traffic("sweep_uneven_pmc"),
-                    kernel="k_sweep<float,3,K,0> on the 21.4M-neuron block-diagonal batch",
-                    steps_per_launch=window, avg_launch_us=avg_launch_s * 1e6,
-                    alg_bytes_per_launch=alg_bytes, launches_timed=launches)
+    if persist:
+        steps_launch = r32["prof_steps"] / max(launches, 1)
+        alg_bytes = (d["N"] * persist_bytes_per_neuron(32, steps_launch, 4) + psp / launches * (BYTES_SPIKE + 4)
+                     + pev / launches * (BYTES_EVENT + 1))
+        roof = roofline(alg_bytes, avg_launch_s, hbm_traffic("persist_uneven_pmc"),
+                        kernel="k_persist (one workgroup per network; 1024-thread kernel for networks above "
+                               "4096 neurons, 256-thread kernel beside it for the rest)",
+                        steps_per_launch=steps_launch, avg_launch_us=avg_launch_s * 1e6,
+                        alg_bytes_per_launch=alg_bytes, launches_timed=launches)
+    else:
+        alg_bytes = (d["N"] * sweep_bytes_per_neuron(32, window, 4) + psp / launches * BYTES_SPIKE
+                     + pev / launches * (BYTES_EVENT + 1))
+        roof = roofline(alg_bytes, avg_launch_s, hbm_traffic("sweep_uneven_pmc"),
+                        kernel="k_lean (windowed sweep) on the 21.4M-neuron block-diagonal batch",
+                        steps_per_launch=window, avg_launch_us=avg_launch_s * 1e6,
+                        alg_bytes_per_launch=alg_bytes, launches_timed=launches)
     check = None
     if 64 in res:
         a, b = res[32]["raster"], res[64]["raster"]

@@ -107,10 +107,13 @@ def tune(h, trial_steps=256, reps=2, sweeps=2, knobs=KNOB_ORDER, log=None, max_s
     return best, used
 
 
+# the last: persistent per-network launches, for batches of independent
+# networks (refused, and skipped, for a network without a partition)
 LAYOUTS = ({"ring_bits": 0, "ring_pack": 1}, {"ring_bits": 16, "ring_pack": 1},
            {"ring_bits": 0, "ring_pack": 0},
            {"ring_bits": 0, "ring_pack": 1, "defer": 1, "neurons_per_thread": 1},
-           {"ring_bits": 0, "ring_pack": 1, "defer": 1, "neurons_per_thread": 2})
+           {"ring_bits": 0, "ring_pack": 1, "defer": 1, "neurons_per_thread": 2},
+           {"ring_bits": 0, "ring_pack": 1, "persist": 1})
 
 
 def tune_layout(build, layouts=LAYOUTS, trial_steps=256, reps=2):

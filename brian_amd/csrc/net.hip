@@ -99,10 +99,11 @@ struct HostGroup {
   // gives each network one workgroup, largest first (part_order)
   std::vector<int64_t> part_off;
   DevBuf part_off_d, part_order_d;
+  std::vector<int32_t> part_order;
   bool part_checked = false;
   int64_t part_max = 0;
-  JitKernel pjit;         // k_persist<pjit_K>
-  int pjit_K = 0;
+  JitKernel pjit, pjit_small;  // k_persist<pjit_K, 1024>, k_persist<pjit_K_small, 256>
+  int pjit_K = 0, pjit_K_small = 0;
   struct Mon {
     int var = 0, timestep = 1, mode = 0;
     int64_t cap = 0;
@@ -172,6 +173,8 @@ struct bm_net {
   bool lean_kernel = true; // specialised groups use k_lean (the model as code) rather than k_sweep tables
   bool persist = false;   // batches of independent networks: one persistent launch per run (k_persist)
   int persist_chunk = 0;  // steps per persistent launch, 0: the whole run
+  hipStream_t stream2 = nullptr;              // the small networks' persistent launches
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool use_graph = true;
   std::vector<std::pair<int, int>> mon_ids;   // (group, monitor within group)
   // tiling knobs (bm_net_set_tuning)
@@ -199,6 +202,9 @@ struct bm_net {
   ~bm_net() {
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     for (auto e : ev) hipEventDestroy(e);
+    if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_join) hipEventDestroy(ev_join);
+    if (stream2) hipStreamDestroy(stream2);
     if (stream) hipStreamDestroy(stream);
   }
 };
@@ -522,7 +528,8 @@ void ensure_jit(bm_net *net) {
 }
 
 // ---- persistent launches for batches of independent networks (k_persist)
-constexpr int PERSIST_BLOCK = 1024;    // PBLOCK of net_persist.hpp
+constexpr int PERSIST_BLOCK = 1024;       // PBLOCK of the large networks' kernel
+constexpr int PERSIST_BLOCK_SMALL = 256;  // PBLOCK of the small networks' kernel
 // neurons per thread of the largest body of k_persist (net_persist.hpp):
 // 20 in FP32, the last 4 slots' state in LDS (models of up to 3 variables,
 // 48 KiB), 16 for larger FP32 models, 8 in FP64
@@ -530,6 +537,10 @@ int persist_kmax(const bm_net *net, const HostGroup &g) {
   if (net->precision == 64) return 8;
   return g.M <= 3 ? 20 : 16;
 }
+// the small networks' kernel: 256 threads and up to 16 slots (8 in FP64),
+// several workgroups to a compute unit
+int persist_kmax_small(const bm_net *net) { return net->precision == 64 ? 8 : 16; }
+int64_t persist_small_cap(const bm_net *net) { return (int64_t)PERSIST_BLOCK_SMALL * persist_kmax_small(net); }
 size_t persist_shmem(const bm_net *net, const HostGroup &g) {
   return persist_kmax(net, g) == 20 ? (size_t)4 * g.M * PERSIST_BLOCK * net->tsize() : 0;
 }
@@ -587,11 +598,18 @@ void check_partitions(bm_net *net) {
 void ensure_persist_jit(bm_net *net) {
   for (auto &gp : net->groups) {
     HostGroup &g = *gp;
-    const int K = persist_kmax(net, g);
-    if (g.pjit.fn && g.pjit_K == K) continue;
-    const std::string name = "bm::k_persist<" + std::to_string(K) + ">";
-    g.pjit = jit_compile(lean_prelude(net, g, "net_persist.hpp") + "// " + name + "\n", name);
-    g.pjit_K = K;
+    const int K = persist_kmax(net, g), Ks = persist_kmax_small(net);
+    if (!(g.pjit.fn && g.pjit_K == K)) {
+      const std::string name = "bm::k_persist<" + std::to_string(K) + ", " + std::to_string(PERSIST_BLOCK) + ">";
+      g.pjit = jit_compile(lean_prelude(net, g, "net_persist.hpp") + "// " + name + "\n", name);
+      g.pjit_K = K;
+    }
+    if (!(g.pjit_small.fn && g.pjit_K_small == Ks)) {
+      const std::string name =
+          "bm::k_persist<" + std::to_string(Ks) + ", " + std::to_string(PERSIST_BLOCK_SMALL) + ">";
+      g.pjit_small = jit_compile(lean_prelude(net, g, "net_persist.hpp") + "// " + name + "\n", name);
+      g.pjit_K_small = Ks;
+    }
   }
 }
 
@@ -613,16 +631,42 @@ void run_persist(bm_net *net, int64_t nsteps) {
     }
     for (auto &gp : net->groups) {
       HostGroup &g = *gp;
+      // the order lists the networks largest first: those above the small
+      // kernel's capacity lead, and run on the 1024-thread kernel
       const int nparts = (int)g.part_off.size() - 1;
+      int nbig = 0;
+      while (nbig < nparts &&
+             g.part_off[g.part_order[nbig] + 1] - g.part_off[g.part_order[nbig]] > persist_small_cap(net))
+        ++nbig;
+      const int nsmall = nparts - nbig;
       const int64_t *po = g.part_off_d.as<int64_t>();
-      const int32_t *ord = g.part_order_d.as<int32_t>();
       int64_t n0 = net->step + done;
       int ns = n, skip = 1;
-      void *args[] = {(void *)&g.dev, (void *)&po, (void *)&ord, (void *)&nparts, (void *)&n0, (void *)&ns,
-                      (void *)&skip};
-      BM_HIP(hipModuleLaunchKernel(g.pjit.fn, (unsigned)nparts, 1, 1, PERSIST_BLOCK, 1, 1,
-                                   (unsigned)persist_shmem(net, g), net->stream, args, nullptr));
-      BM_HIP(hipGetLastError());
+      auto launch = [&](const JitKernel &k, const int32_t *ord, int cnt, int block, size_t shmem, hipStream_t st) {
+        void *args[] = {(void *)&g.dev, (void *)&po, (void *)&ord, (void *)&cnt, (void *)&n0, (void *)&ns,
+                        (void *)&skip};
+        BM_HIP(hipModuleLaunchKernel(k.fn, (unsigned)cnt, 1, 1, block, 1, 1, (unsigned)shmem, st, args, nullptr));
+        BM_HIP(hipGetLastError());
+      };
+      const int32_t *ord = g.part_order_d.as<int32_t>();
+      if (nbig > 0 && nsmall > 0) {
+        // both kernels at once: the small networks on a second stream
+        if (!net->stream2) {
+          BM_HIP(hipStreamCreateWithFlags(&net->stream2, hipStreamNonBlocking));
+          BM_HIP(hipEventCreateWithFlags(&net->ev_fork, hipEventDisableTiming));
+          BM_HIP(hipEventCreateWithFlags(&net->ev_join, hipEventDisableTiming));
+        }
+        BM_HIP(hipEventRecord(net->ev_fork, net->stream));
+        BM_HIP(hipStreamWaitEvent(net->stream2, net->ev_fork, 0));
+        launch(g.pjit_small, ord + nbig, nsmall, PERSIST_BLOCK_SMALL, 0, net->stream2);
+        launch(g.pjit, ord, nbig, PERSIST_BLOCK, persist_shmem(net, g), net->stream);
+        BM_HIP(hipEventRecord(net->ev_join, net->stream2));
+        BM_HIP(hipStreamWaitEvent(net->stream, net->ev_join, 0));
+      } else if (nbig > 0) {
+        launch(g.pjit, ord, nbig, PERSIST_BLOCK, persist_shmem(net, g), net->stream);
+      } else {
+        launch(g.pjit_small, ord, nsmall, PERSIST_BLOCK_SMALL, 0, net->stream);
+      }
     }
     if (net->profiling) {
       hipEvent_t b;
@@ -2192,6 +2236,7 @@ int bm_net_set_partition(bm_net *net, int gid, int nparts, const int64_t *offset
     });
     upload_raw(g.part_off_d, g.part_off.data(), g.part_off.size(), net->stream);
     upload_raw(g.part_order_d, order.data(), order.size(), net->stream);
+    g.part_order = order;
     BM_HIP(hipStreamSynchronize(net->stream));
     g.part_checked = false;
     if (net->persist) {

@@ -44,7 +44,6 @@
 
 namespace bm {
 
-constexpr int PBLOCK = 1024;
 constexpr int PL = 4;   // synapses per lane staged per delivery pass
 
 // Ring slices and last spike steps go through buffer instructions: a
@@ -174,7 +173,7 @@ __device__ __attribute__((always_inline)) inline void persist_emit(const GroupDe
 // thread, the last KL of them in LDS (sh_state [KL][M][PBLOCK]); sh_mask
 // holds [wavefront][slot] threshold ballots of a step, sh_spk [wavefront][64]
 // a wavefront's spike list.
-template <int K, int KL>
+template <int K, int KL, int PBLOCK>
 __device__ __attribute__((always_inline)) inline void persist_network(const GroupDev &g, int64_t off, int np, int64_t n0, int nsteps, int skip_first_tail,
                                 unsigned long long *sh_mask, int32_t *sh_spk_all, lean_t *sh_state) {
   using T = lean_t;
@@ -325,10 +324,13 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
 
 // Workgroup b advances network order[b] (largest first) through the run; the
 // network's slots per thread pick the body, so a small network does not pay
-// for the largest one's slots.  KMAX: the largest body (20 in FP32, 4 of its
-// slots in LDS; 8 in FP64, where a slot costs twice the registers).  The
-// dynamic LDS holds the LDS slots: KL x LEAN_M x PBLOCK values.
-template <int KMAX>
+// for the largest one's slots.  Two shapes are compiled: PBLOCK = 1024 for
+// large networks (bodies up to KMAX slots: 20 in FP32, its last 4 slots'
+// state in LDS, given as dynamic LDS of 4 x LEAN_M x PBLOCK values; 8 in
+// FP64, where a slot costs twice the registers) and PBLOCK = 256 for
+// networks of up to 256 x KMAX neurons, several workgroups to a compute
+// unit, so that one network's memory latency overlaps another's work.
+template <int KMAX, int PBLOCK>
 __global__ __launch_bounds__(PBLOCK) void k_persist(const GroupDev g, const int64_t *__restrict__ part_off,
                                                     const int32_t *__restrict__ order, int nparts, int64_t n0,
                                                     int nsteps, int skip_first_tail) {
@@ -341,15 +343,19 @@ __global__ __launch_bounds__(PBLOCK) void k_persist(const GroupDev g, const int6
   const int64_t off = part_off[part];
   const int np = (int)(part_off[part + 1] - off);
   const int nk = (np + PBLOCK - 1) / PBLOCK;
-  if (nk <= 1) persist_network<1, 0>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_spk, sh_state);
-  else if (nk <= 2) persist_network<2, 0>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_spk, sh_state);
-  else if (nk <= 4) persist_network<4, 0>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_spk, sh_state);
-  else if (nk <= 8) persist_network<8, 0>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_spk, sh_state);
-  else if constexpr (KMAX >= 20) {
-    if (nk <= 12) persist_network<12, 0>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_spk, sh_state);
-    else if (nk <= 16) persist_network<16, 0>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_spk, sh_state);
-    else if (nk <= 20) persist_network<20, 4>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_spk, sh_state);
+#define PERSIST_BODY(K_, KL_) persist_network<K_, KL_, PBLOCK>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_spk, sh_state)
+  if (nk <= 1) PERSIST_BODY(1, 0);
+  else if (nk <= 2) PERSIST_BODY(2, 0);
+  else if (nk <= 4) PERSIST_BODY(4, 0);
+  else if (nk <= 8) PERSIST_BODY(8, 0);
+  else if constexpr (KMAX >= 16) {
+    if (nk <= 12) PERSIST_BODY(12, 0);
+    else if (nk <= 16) PERSIST_BODY(16, 0);
+    else if constexpr (KMAX >= 20) {
+      if (nk <= 20) PERSIST_BODY(20, 4);
+    }
   }
+#undef PERSIST_BODY
 }
 
 }  // namespace bm
