@@ -220,13 +220,15 @@ def autotuned(build, no_autotune=False, fixed=None):
         return dict(fixed), {"chosen": dict(fixed), "searched": False}
     if no_autotune:
         return {}, None
-    layout, lay_res = autotune.tune_layout(lambda lay: build(lay))
+    # trial segments of ~1000 steps: shorter ones are dominated by the host's
+    # launch and synchronisation overhead, and mis-rank shapes a few % apart
+    layout, lay_res = autotune.tune_layout(lambda lay: build(lay), trial_steps=1024, reps=2)
     if layout.get("persist"):
         # the launch-shape knobs do not apply to persistent launches
         return dict(layout), {"chosen": dict(layout), "layouts_us_per_step": [(l, t * 1e6) for l, t in lay_res]}
     h = build(layout)
     log = []
-    best, used = autotune.tune(h, log=log)
+    best, used = autotune.tune(h, trial_steps=1024, reps=3, log=log)
     h.close()
     cfg = dict(layout)
     cfg.update(best)
@@ -903,6 +905,37 @@ def cpu_baseline_fit(budget_s):
                       f"1 thread of {host_cores()} host cores"}
 
 
+def other_configs(args):
+    """Short legs of the batched-STDP and uneven-batch configurations after
+    the headline, each with its FP64 check, so that one default run records
+    all three network configurations.  Each leg is self-contained (its own
+    handles, closed before the next); a leg that fails is reported, not
+    raised."""
+    out = {}
+    legs = (
+        ("batched_stdp", bench_stdp, dict(steps=200000, warmup=0, no_cpu=True, no_fp64=False, tuning=None,
+                                          no_autotune=False)),
+        ("uneven_batch", bench_uneven, dict(steps=5000, warmup=0, no_cpu=True, no_fp64=False, window=0, npt=0,
+                                            no_autotune=False,
+                                            tuning={"ring_bits": 0, "ring_pack": 1, "persist": 1})),
+    )
+    for name, fn, kw in legs:
+        t0 = time.perf_counter()
+        try:
+            r = fn(argparse.Namespace(**kw))
+            keep = ("value", "unit", "ms_per_step", "steps", "config", "fp64_check", "fp32_over_fp64_plain",
+                    "synaptic_events_per_s", "neuron_updates_per_s")
+            leg = {k: r[k] for k in keep if k in r}
+            leg["roofline"] = {k: r["roofline"].get(k) for k in ("bound", "achieved", "peak", "unit", "frac",
+                                                                  "traffic", "kernel", "avg_launch_us")}
+        except Exception as e:  # reported in the line rather than hiding the headline
+            leg = {"error": f"{type(e).__name__}: {e}"}
+        leg["wall_s"] = time.perf_counter() - t0
+        progress(f"{name}: {leg.get('ms_per_step', 'n/a')} ms/step in {leg['wall_s']:.1f} s")
+        out[name] = leg
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -927,6 +960,8 @@ def main():
                     help='fixed launch shape as JSON, e.g. \'{"window": 11, "neurons_per_thread": 4}\'')
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="only run the byte-counter calibration copy of this many bytes")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="cuba1m: skip the short batched-STDP and uneven-batch legs after the headline")
     args = ap.parse_args()
     if args.probe_copy:
         print(json.dumps(probe_copy(args.probe_copy)))
@@ -937,7 +972,10 @@ def main():
         sys.exit(2)
     fn = {"cuba1m": bench_cuba1m, "uneven": bench_uneven, "stdp": bench_stdp, "fit": bench_fit,
           "mlp": bench_mlp}[args.workload]
-    print(json.dumps(fn(args)), flush=True)
+    out = fn(args)
+    if args.workload == "cuba1m" and not args.no_other_configs:
+        out["other_configs"] = other_configs(args)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

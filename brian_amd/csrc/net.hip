@@ -2331,6 +2331,7 @@ int bm_net_set_tuning(bm_net *net, int key, int value) {
           const std::string why = persist_blocker(net);
           BM_REQUIRE(why.empty(), why);
           check_partitions(net);
+          ensure_persist_jit(net);  // compiled here rather than inside the first run
         }
         net->persist = value != 0;
         break;
