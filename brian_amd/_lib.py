@@ -26,7 +26,7 @@ Pi64 = C.POINTER(C.c_int64)
 # bm_net_set_tuning keys (brianmi.h)
 TUNE = {"window": 1, "neurons_per_thread": 2, "wave_max": 3, "row_split": 4, "ring_bits": 5,
         "ring_pack": 6, "use_graph": 7, "window_max": 8, "specialize": 9, "defer": 10, "lean": 11,
-        "persist": 12, "persist_chunk": 13}
+        "persist": 12, "persist_chunk": 13, "persist_queue": 14}
 
 
 class BrianMIError(RuntimeError):

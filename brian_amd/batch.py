@@ -63,7 +63,7 @@ def _prepare(h, tuning, window, npt, use_graph):
             h.set_tuning(key, tuning[key])
     h.prepare(window=tuning.get("window", window), neurons_per_thread=tuning.get("neurons_per_thread", npt),
               use_graph=use_graph)
-    for key in ("wave_max", "row_split", "lean", "persist_chunk", "persist"):
+    for key in ("wave_max", "row_split", "lean", "persist_chunk", "persist_queue", "persist"):
         if key in tuning:
             h.set_tuning(key, tuning[key])
 

@@ -104,6 +104,10 @@ struct HostGroup {
   int64_t part_max = 0;
   JitKernel pjit, pjit_small;  // k_persist<pjit_K, 1024>, k_persist<pjit_K_small, 256>
   int pjit_K = 0, pjit_K_small = 0;
+  // the persistent kernel's event queues: D slots of persist_queue entries
+  // per 16 neurons (pqueue_q), allocated at the first persistent run
+  DevBuf pqueue;
+  int pqueue_q = -1;
   struct Mon {
     int var = 0, timestep = 1, mode = 0;
     int64_t cap = 0;
@@ -173,6 +177,7 @@ struct bm_net {
   bool lean_kernel = true; // specialised groups use k_lean (the model as code) rather than k_sweep tables
   bool persist = false;   // batches of independent networks: one persistent launch per run (k_persist)
   int persist_chunk = 0;  // steps per persistent launch, 0: the whole run
+  int persist_queue = 16; // persistent event-queue entries per ring slot per 16 neurons (0: deliveries through the ring)
   hipStream_t stream2 = nullptr;              // the small networks' persistent launches
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool use_graph = true;
@@ -541,8 +546,25 @@ int persist_kmax(const bm_net *net, const HostGroup &g) {
 // several workgroups to a compute unit
 int persist_kmax_small(const bm_net *net) { return net->precision == 64 ? 8 : 16; }
 int64_t persist_small_cap(const bm_net *net) { return (int64_t)PERSIST_BLOCK_SMALL * persist_kmax_small(net); }
-size_t persist_shmem(const bm_net *net, const HostGroup &g) {
-  return persist_kmax(net, g) == 20 ? (size_t)4 * g.M * PERSIST_BLOCK * net->tsize() : 0;
+// dynamic LDS of k_persist (net_persist.hpp): the LDS state slots of the
+// 20-slot body, a count word per neuron, the staged queue entries, a fill
+// count and a dirty bit per ring slot
+size_t persist_shmem(const bm_net *net, const HostGroup &g, bool small) {
+  const int K = small ? persist_kmax_small(net) : persist_kmax(net, g);
+  const int block = small ? PERSIST_BLOCK_SMALL : PERSIST_BLOCK;
+  const size_t state = K == 20 ? (size_t)4 * g.M * block * net->tsize() : 0;
+  const int pq = small ? 2 : 4;  // queue entries per thread staged in LDS
+  return state + (size_t)4 * (K + pq) * block + (size_t)4 * g.D + (size_t)4 * ((g.D + 31) / 32);
+}
+// static LDS of k_persist: the threshold ballots and the spike lists
+size_t persist_static_lds(int kmax, int block) { return (size_t)8 * (block / 64) * kmax + (size_t)4 * block; }
+// LDS a workgroup may use (160 KiB on gfx950)
+size_t lds_per_block() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+    return (size_t)v;
+  return (size_t)160 * 1024;
 }
 
 // Why a network cannot run persistently, or "" if it can.
@@ -567,6 +589,11 @@ std::string persist_blocker(const bm_net *net) {
     if (g.part_max > (int64_t)PERSIST_BLOCK * persist_kmax(net, g))
       return "a network exceeds " + std::to_string(PERSIST_BLOCK * persist_kmax(net, g)) +
              " neurons, the largest a persistent workgroup holds";
+    if (persist_shmem(net, g, false) + persist_static_lds(persist_kmax(net, g), PERSIST_BLOCK) > lds_per_block())
+      return "the ring is too deep for persistent launches (" + std::to_string(g.D) + " slots)";
+    // a network's queues are addressed with 32-bit byte offsets
+    if ((double)g.D * (double)(((g.part_max * net->persist_queue) >> 4) + 1) * 4.0 >= 4294967296.0)
+      return "persist_queue is too large for this ring depth and network size";
   }
   return "";
 }
@@ -620,6 +647,16 @@ void run_persist(bm_net *net, int64_t nsteps) {
   BM_REQUIRE(why.empty(), why);
   check_partitions(net);
   ensure_persist_jit(net);
+  for (auto &gp : net->groups) {
+    HostGroup &g = *gp;
+    if (g.pqueue_q == net->persist_queue) continue;
+    // every launch leaves the queues empty, so they can be resized between runs
+    BM_HIP(hipStreamSynchronize(net->stream));
+    g.pqueue.release();
+    const int64_t entries = (int64_t)g.D * ((g.N * net->persist_queue) >> 4);
+    if (entries > 0) g.pqueue.alloc((size_t)entries * 4);
+    g.pqueue_q = net->persist_queue;
+  }
   const int64_t chunk = net->persist_chunk > 0 ? net->persist_chunk : nsteps;
   for (int64_t done = 0; done < nsteps; done += chunk) {
     const int n = (int)std::min<int64_t>(chunk, nsteps - done);
@@ -642,9 +679,11 @@ void run_persist(bm_net *net, int64_t nsteps) {
       const int64_t *po = g.part_off_d.as<int64_t>();
       int64_t n0 = net->step + done;
       int ns = n, skip = 1;
+      uint32_t *queue = g.pqueue.as<uint32_t>();
+      int qnum = queue ? g.pqueue_q : 0;
       auto launch = [&](const JitKernel &k, const int32_t *ord, int cnt, int block, size_t shmem, hipStream_t st) {
-        void *args[] = {(void *)&g.dev, (void *)&po, (void *)&ord, (void *)&cnt, (void *)&n0, (void *)&ns,
-                        (void *)&skip};
+        void *args[] = {(void *)&g.dev, (void *)&po,   (void *)&ord,   (void *)&cnt,  (void *)&n0,
+                        (void *)&ns,    (void *)&skip, (void *)&queue, (void *)&qnum};
         BM_HIP(hipModuleLaunchKernel(k.fn, (unsigned)cnt, 1, 1, block, 1, 1, (unsigned)shmem, st, args, nullptr));
         BM_HIP(hipGetLastError());
       };
@@ -658,14 +697,14 @@ void run_persist(bm_net *net, int64_t nsteps) {
         }
         BM_HIP(hipEventRecord(net->ev_fork, net->stream));
         BM_HIP(hipStreamWaitEvent(net->stream2, net->ev_fork, 0));
-        launch(g.pjit_small, ord + nbig, nsmall, PERSIST_BLOCK_SMALL, 0, net->stream2);
-        launch(g.pjit, ord, nbig, PERSIST_BLOCK, persist_shmem(net, g), net->stream);
+        launch(g.pjit_small, ord + nbig, nsmall, PERSIST_BLOCK_SMALL, persist_shmem(net, g, true), net->stream2);
+        launch(g.pjit, ord, nbig, PERSIST_BLOCK, persist_shmem(net, g, false), net->stream);
         BM_HIP(hipEventRecord(net->ev_join, net->stream2));
         BM_HIP(hipStreamWaitEvent(net->stream, net->ev_join, 0));
       } else if (nbig > 0) {
-        launch(g.pjit, ord, nbig, PERSIST_BLOCK, persist_shmem(net, g), net->stream);
+        launch(g.pjit, ord, nbig, PERSIST_BLOCK, persist_shmem(net, g, false), net->stream);
       } else {
-        launch(g.pjit_small, ord, nsmall, PERSIST_BLOCK_SMALL, 0, net->stream);
+        launch(g.pjit_small, ord, nsmall, PERSIST_BLOCK_SMALL, persist_shmem(net, g, true), net->stream);
       }
     }
     if (net->profiling) {
@@ -2339,6 +2378,19 @@ int bm_net_set_tuning(bm_net *net, int key, int value) {
         BM_REQUIRE(value >= 0, "persist_chunk must be >= 0 (0: the whole run in one launch)");
         net->persist_chunk = value;
         break;
+      case BM_TUNE_PERSIST_QUEUE:
+        BM_REQUIRE(value >= 0 && value <= 256, "persist_queue must be in 0..256 (queue entries per slot per 16 neurons)");
+        if (net->persist) {
+          const int old = net->persist_queue;
+          net->persist_queue = value;
+          const std::string why = persist_blocker(net);
+          if (!why.empty()) {
+            net->persist_queue = old;
+            throw Error(BM_EINVAL, why);
+          }
+        }
+        net->persist_queue = value;
+        break;
       default:
         throw Error(BM_EINVAL, "unknown tuning key");
     }
@@ -2368,6 +2420,7 @@ int bm_net_get_tuning(bm_net *net, int key, int *value) {
       case BM_TUNE_LEAN: *value = net->lean_kernel ? 1 : 0; break;
       case BM_TUNE_PERSIST: *value = net->persist ? 1 : 0; break;
       case BM_TUNE_PERSIST_CHUNK: *value = net->persist_chunk; break;
+      case BM_TUNE_PERSIST_QUEUE: *value = net->persist_queue; break;
       default: throw Error(BM_EINVAL, "unknown tuning key");
     }
   });

@@ -14,31 +14,53 @@
 // within the workgroup (__syncthreads is a workgroup-scope release and
 // acquire for global memory as well as LDS).
 //
+// Deliveries go through event queues instead of the group's count ring.
+// Each network has, per ring slot s, a queue of Q entries in HBM (entry:
+// target neuron within the network << 5 | the channel's bit offset in the
+// neuron's count word) whose fill count sh_qcnt[s] lives in LDS; the counts
+// of the slot being applied are summed in LDS (sh_cnt, one count word per
+// neuron in the ring's packed layout).  A spike's synapse takes a queue
+// position with an LDS atomic and writes its entry with a plain store, so a
+// step costs no global atomics and no pass over the ring: HBM sees the
+// entries written and read once.  A synapse whose queue is full falls back
+// to the count ring (a global atomic, as the windowed kernel delivers) and
+// marks its slot dirty (sh_dirty, one bit per slot); a dirty slot's ring
+// words are read, zeroed and added when it is applied.  Every slot is dirty
+// at the start of a launch (the ring may hold a windowed run's or the
+// previous launch's pending counts), and the entries still queued at the end
+// of a launch are added to the ring, so that between launches the ring
+// holds every pending delivery, as the windowed kernel expects.  Counts are
+// order-independent sums, so rasters and states equal the windowed kernel's.
+//
 // Step n of the loop, in the order of k_lean:
-//   tail of n-1: the ring words of slot (n-1) mod D, loaded and zeroed one
-//     step earlier, are delivered, and a neuron that spiked at n-1 is reset
-//     (skipped at the first step of a run: the previous run applied it;
-//     applied once more after the last step);
-//   prefetch:    the words of slot n mod D (the next tail's) are loaded
-//     and zeroed;
+//   tail of n-1: the counts of slot (n-1) mod D, summed in sh_cnt during
+//     step n-1, are delivered (and zeroed), and a neuron that spiked at n-1
+//     is reset (skipped at the first step of a run: the previous run applied
+//     it; applied once more after the last step);
+//   queue:       the fill count of slot n mod D and its first entries are
+//     loaded (the loads have the head and the delivery to arrive);
 //   head of n:   update and threshold; the threshold ballots go to LDS;
-//   delivery:    each wavefront appends its spikes to the raster and
-//     propagates them: count atomics into slots (n + d) mod D, d >= 1;
+//   LDS barrier: every tail has read sh_cnt, and every thread has read the
+//     slot's fill count and dirty bit;
+//   delivery:    each wavefront appends its spikes to the raster and queues
+//     their synapses into slots (n + d) mod D, d >= 1;
+//   sum:         slot n's entries (and its ring words if dirty) are added
+//     into sh_cnt by LDS atomics; its fill count and dirty bit are cleared;
 //   barrier.
 // Ordering: every wavefront is in the same step between two barriers.  Slot
-// n mod D only receives spikes of steps <= n-1, all issued before the
-// barrier that ends step n-1, so it is complete when step n prefetches it;
-// the spikes of step n land in slots (n+1) .. (n+dmax) mod D, never in the
-// slot step n zeroes; and the next deliveries into a zeroed slot come from
-// spikes of later steps, after the barrier that follows the zeroing.  Memory
+// n mod D only receives synapses of spikes of steps <= n-1, queued before the
+// barrier that ends step n-1, so it is complete when step n reads it; the
+// spikes of step n go to slots (n+1) .. (n+dmax) mod D, never to slot n mod
+// D (D >= dmax + 3); and the next entries of a cleared slot come from spikes
+// of later steps, after the barrier that follows the clearing.  Memory
 // operations of one workgroup reach a given address in the order the
-// barriers impose (all its waves run on one compute unit).
+// barriers impose (all its waves run on one compute unit, which shares one
+// vector L1).
 //
 // Vector memory counters complete in issue order, so the delivery issues
 // every load it needs (row bounds, columns, delays of up to 64*PL synapses)
-// before any of its atomics: waiting for a load never waits for an atomic
-// of the same pass; and the prefetch, issued before the head, is behind
-// only the previous step's atomics.
+// before any of its stores: waiting for a load never waits for a store of
+// the same pass.
 #pragma once
 #include "net_kernels.hpp"
 
@@ -68,15 +90,55 @@ __device__ inline void persist_zero(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
   else __builtin_amdgcn_raw_buffer_store_b32(0u, r, (int)voff, 0, 0);
 }
 
+// State words of type T through a buffer resource (the state is loaded once
+// and stored once per launch; the lane offset is shared by every slot).
+template <typename T>
+__device__ inline T persist_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+  else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+template <typename T>
+__device__ inline void persist_st(T v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(T) == 4) __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+  else {
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)voff, (int)soff, 0);
+  }
+}
+
+// A workgroup barrier for LDS only: global loads in flight stay in flight.
+__device__ __attribute__((always_inline)) inline void persist_lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The network's queues: Q entries per slot, slot s at base + s*Q.
+struct PersistQueue {
+  uint32_t *base;
+  uint32_t Q;
+  uint32_t *cnt;     // LDS [D] fill counts (may exceed Q: the rest went to the ring)
+  uint32_t *dirty;   // LDS [(D+31)/32] slots with ring words to add
+  uint32_t *stage;   // LDS [PQ*PBLOCK] the first entries of the slot being summed
+};
+
+// Count-ring fallback of one synapse (the windowed kernel's delivery).
+__device__ inline void persist_ring_add(const OutDev &od, int ds, int64_t tgt) {
+  unsigned char *dslot = static_cast<unsigned char *>(od.ring) + (int64_t)ds * od.slot_bytes;
+  const int64_t pos = tgt * (od.nch * od.bits) + od.ch * od.bits;
+  atomicAdd(reinterpret_cast<unsigned int *>(dslot) + (pos >> 5), 1u << (int)(pos & 31));
+}
+
 // The spikes listed in sh_spk[0..ns) (neuron indices of the group, at
-// most 64, all from one wavefront) of step n: raster append and count
-// atomics.  Per connection, lane j < ns looks up spike j's row, an
+// most 64, all from one wavefront) of step n: raster append and queue
+// entries.  Per connection, lane j < ns looks up spike j's row, an
 // inclusive scan over lanes spreads the rows' synapses over the wavefront,
-// and each pass stages up to 64*PL (column, delay) pairs before issuing
-// their atomics.  The raster position is reserved first and used last, so
-// its atomic's round trip overlaps the loads.
+// and each pass stages up to 64*PL (column, delay) pairs before queueing
+// them.  The raster position is reserved first and used last, so its
+// atomic's round trip overlaps the loads.
 __device__ __attribute__((always_inline)) inline void persist_emit(const GroupDev &g, const int32_t *sh_spk, int ns,
-                                                                 int64_t n, int sb, int lane) {
+                                                                 int64_t n, int sb, int lane, int64_t off,
+                                                                 const PersistQueue &pq) {
   const int shard = blockIdx.x & (RSHARD - 1);
   unsigned long long rb = 0;
   if (g.raster != nullptr && lane == 0) rb = atomicAdd(g.raster_count + shard * RSTRIDE, (unsigned long long)ns);
@@ -101,7 +163,10 @@ __device__ __attribute__((always_inline)) inline void persist_emit(const GroupDe
     }
     const int total = __shfl(inc, 63);
     const int pref = inc - len;  // spike j's row starts at synapse pref_j of the wavefront's list
-    const int pbits = od.nch * od.bits;
+    const uint32_t chbits = (uint32_t)(od.ch * od.bits);
+    // the network's queues as one resource: D*Q entries, 32-bit offsets
+    const __amdgpu_buffer_rsrc_t qrs = persist_rsrc(pq.base, (int64_t)od.D * pq.Q * 4);
+    const int64_t loc0 = od.tgt_off - off;  // target column -> neuron within the network
     for (int base = 0; base < total; base += 64 * PL) {
       int32_t cols[PL];
       int dls[PL];
@@ -154,9 +219,14 @@ __device__ __attribute__((always_inline)) inline void persist_emit(const GroupDe
         if (cols[u] >= 0) {
           int ds = sb + dls[u];
           if (ds >= od.D) ds -= od.D;  // sb < D and d < D
-          unsigned char *dslot = static_cast<unsigned char *>(od.ring) + (int64_t)ds * od.slot_bytes;
-          const int64_t pos = (od.tgt_off + cols[u]) * pbits + od.ch * od.bits;
-          atomicAdd(reinterpret_cast<unsigned int *>(dslot) + (pos >> 5), 1u << (int)(pos & 31));
+          const uint32_t p = atomicAdd(pq.cnt + ds, 1u);
+          const uint32_t loc = (uint32_t)(loc0 + cols[u]);
+          if (p < pq.Q) {
+            __builtin_amdgcn_raw_buffer_store_b32((loc << 5) | chbits, qrs, (int)(((uint32_t)ds * pq.Q + p) * 4u), 0, 0);
+          } else {
+            persist_ring_add(od, ds, off + loc);
+            atomicOr(pq.dirty + (ds >> 5), 1u << (ds & 31));
+          }
         }
       }
     }
@@ -172,16 +242,19 @@ __device__ __attribute__((always_inline)) inline void persist_emit(const GroupDe
 // One network of np neurons (offset off) for nsteps steps with K slots per
 // thread, the last KL of them in LDS (sh_state [KL][M][PBLOCK]); sh_mask
 // holds [wavefront][slot] threshold ballots of a step, sh_spk [wavefront][64]
-// a wavefront's spike list.
-template <int K, int KL, int PBLOCK>
+// a wavefront's spike list, sh_cnt the count words of the slot being summed
+// ([KMAX*PBLOCK], zero on entry); PQ queue entries per thread are loaded
+// into LDS ahead of the sum.
+template <int K, int KL, int PBLOCK, int PQ>
 __device__ __attribute__((always_inline)) inline void persist_network(const GroupDev &g, int64_t off, int np, int64_t n0, int nsteps, int skip_first_tail,
-                                unsigned long long *sh_mask, int32_t *sh_spk_all, lean_t *sh_state) {
+                                unsigned long long *sh_mask, int32_t *sh_spk_all, lean_t *sh_state,
+                                uint32_t *sh_cnt, PersistQueue pq) {
   using T = lean_t;
   constexpr int M = LEAN_M;
   constexpr int KR = K - KL;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wv = tid >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wavefront index, uniform
   int32_t *sh_spk = sh_spk_all + wv * 64;
   const int64_t N = g.N;
   const int nk = (np + PBLOCK - 1) / PBLOCK;  // slots in use, uniform
@@ -201,47 +274,55 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
     if ((k) < KR) s[k][j] = x[j];                                                              \
     else sh_state[(((k) - KR) * M + j) * PBLOCK + tid] = x[j];                                 \
   }
+  // neurons of slot k in this network (0..PBLOCK), uniform
+#define slot_n(k) min(max(np - (k) * PBLOCK, 0), PBLOCK)
+  // state variable j of slot k through a buffer resource whose range ends
+  // at the network's end (lanes past it load 0 and store nothing; the
+  // range check covers the lane offset, so every bound sits in the resource)
+#define PERSIST_SRSRC(k, j, o) \
+  persist_rsrc(static_cast<T *>(g.S) + (int64_t)(j) * N + (o) + (k) * PBLOCK, (int64_t)slot_n(k) * sizeof(T))
+  const uint32_t state_voff = (uint32_t)(tid * sizeof(T));
+  const uint32_t last_voff = (uint32_t)(tid * 4);
+  const __amdgpu_buffer_rsrc_t last_rsrc = persist_rsrc(g.last + off, (int64_t)np * 4);
   uint32_t prev = 0u;  // bit k: the neuron of slot k spiked at the previous step
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int loc = k * PBLOCK + tid;
     const bool v = (valid >> k) & 1u;
-    const int64_t i = off + (v ? loc : 0);
-    if (v && g.last[i] == (int32_t)(n0 - 1)) prev |= 1u << k;
+    const __amdgpu_buffer_rsrc_t lr = persist_rsrc(g.last + off + k * PBLOCK, (int64_t)slot_n(k) * 4);
+    const int32_t lst = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(lr, (int)last_voff, 0, 0);
+    if (v && lst == (int32_t)(n0 - 1)) prev |= 1u << k;
     T x[M];
 #pragma unroll
-    for (int j = 0; j < M; ++j) x[j] = v ? static_cast<const T *>(g.S)[(int64_t)j * N + i] : (T)0;
+    for (int j = 0; j < M; ++j) x[j] = persist_ld<T>(PERSIST_SRSRC(k, j, off), state_voff, 0u);
     PERSIST_PUT(k, x)
   }
   const int D = g.D;
   unsigned char *ring = LEAN_SW > 0 ? static_cast<unsigned char *>(g.rings[0].base) : nullptr;
   const int64_t slot_bytes = LEAN_SW > 0 ? g.rings[0].slot_bytes : 0;
   constexpr int SW = LEAN_SW > 0 ? LEAN_SW : 1;
-  constexpr int PR = 4 / SW;  // ring words packed per register
-  const uint32_t ring_voff = (uint32_t)(tid * SW);
-  const uint32_t last_voff = (uint32_t)(tid * 4);
-  const __amdgpu_buffer_rsrc_t last_rsrc = persist_rsrc(g.last + off, (int64_t)np * 4);
-  // neurons of slot k in this network (0..PBLOCK), uniform
-#define slot_n(k) min(max(np - (k) * PBLOCK, 0), PBLOCK)
-  int sl = (int)((n0 - 1) % D);  // ring slot of the tail being applied
-  if (sl < 0) sl += D;
-
-  uint32_t cw[(K + PR - 1) / PR];  // the next tail's ring words, PR slots per register
-  // loads the words of ring slot rs into cw and zeroes them
-#define PERSIST_FETCH(rs)                                                                        \
+  // adds the ring words of slot rs of this thread's neurons into sh_cnt and
+  // zeroes them (lanes past the network's end load 0 and store nothing); a
+  // rare path (the first D steps of a launch, full queues), kept rolled so
+  // that it holds no registers
+#define PERSIST_RING_SUM(rs)                                                                     \
   {                                                                                              \
-    unsigned char *fb = ring + (int64_t)(rs) * slot_bytes + off * SW;                            \
-    _Pragma("unroll") for (int q = 0; q < (K + PR - 1) / PR; ++q) cw[q] = 0u;                    \
-    _Pragma("unroll") for (int k = 0; k < K; ++k) {                                              \
-      const __amdgpu_buffer_rsrc_t rr = persist_rsrc(fb + (int64_t)k * PBLOCK * SW, (int64_t)slot_n(k) * SW); \
-      cw[k / PR] |= persist_load<SW>(rr, ring_voff) << (8 * SW * (k % PR));                       \
-      persist_zero<SW>(rr, ring_voff);                                                            \
+    const __amdgpu_buffer_rsrc_t rr = persist_rsrc(ring + (int64_t)(rs) * slot_bytes + off * SW, (int64_t)np * SW); \
+    _Pragma("unroll 1") for (int k = 0; k < nk; ++k) {                                           \
+      const uint32_t loc = (uint32_t)(k * PBLOCK + tid);                                         \
+      const uint32_t w = persist_load<SW>(rr, loc * SW);                                         \
+      persist_zero<SW>(rr, loc * SW);                                                             \
+      if (w != 0u) atomicAdd(sh_cnt + loc, w);                                                   \
     }                                                                                            \
   }
-#pragma unroll
-  for (int q = 0; q < (K + PR - 1) / PR; ++q) cw[q] = 0u;
-  if constexpr (LEAN_SW > 0)
-    if (!skip_first_tail) PERSIST_FETCH(sl)
+  int sl = (int)((n0 - 1) % D);  // ring slot of the tail being applied
+  if (sl < 0) sl += D;
+  if constexpr (LEAN_SW > 0) {
+    // every slot dirty, no entries queued; the first tail's counts from the ring
+    for (int q = tid; q < D; q += PBLOCK) pq.cnt[q] = 0u;
+    for (int q = tid; q < (D + 31) / 32; q += PBLOCK) pq.dirty[q] = 0xFFFFFFFFu;
+    if (!skip_first_tail) PERSIST_RING_SUM(sl)
+    __syncthreads();
+  }
 
   for (int st = 0; st <= nsteps; ++st) {
     const int64_t n = n0 + st;
@@ -253,8 +334,9 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
         PERSIST_GET(k, x)
         // lanes without a neuron deliver zero counts and never reset
         if constexpr (LEAN_SW > 0) {
-          constexpr uint32_t wmask = SW == 4 ? 0xFFFFFFFFu : ((1u << (8 * SW)) - 1u);
-          lean_deliver(x, (cw[k / PR] >> (8 * SW * (k % PR))) & wmask);
+          const uint32_t w = sh_cnt[k * PBLOCK + tid];
+          if (w != 0u) sh_cnt[k * PBLOCK + tid] = 0u;
+          lean_deliver(x, w);
         }
         T r[M];
 #pragma unroll
@@ -268,10 +350,27 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
     }
     if (++sl == D) sl = 0;
     if (st == nsteps) break;
-    // ---- prefetch of the tail of step n: slot n mod D is complete, its
-    // deliveries having come from spikes of steps <= n-1, issued before the
-    // last barrier; the loads have this whole step to arrive
-    if constexpr (LEAN_SW > 0) PERSIST_FETCH(sl)
+    // ---- slot n mod D: complete, its entries having come from spikes of
+    // steps <= n-1, queued before the last barrier
+    uint32_t ce = 0u;
+    bool dirty = false;
+    const uint32_t *qslot = pq.base + (int64_t)sl * pq.Q;
+    if constexpr (LEAN_SW > 0) {
+      ce = min(pq.cnt[sl], pq.Q);
+      dirty = (pq.dirty[sl >> 5] >> (sl & 31)) & 1u;
+      // the first PQ*PBLOCK entries go straight to LDS (entry i at
+      // pq.stage[i]: each wavefront's lanes fill 64 consecutive words), no
+      // registers held while they arrive; each resource ends at the slot's
+      // last entry (lanes past it load zeros, which the sum skips)
+#pragma unroll
+      for (int j = 0; j < PQ; ++j)
+        if ((uint32_t)(j * PBLOCK) < ce) {
+          const __amdgpu_buffer_rsrc_t qr =
+              persist_rsrc(const_cast<uint32_t *>(qslot) + j * PBLOCK, (int64_t)(ce - (uint32_t)(j * PBLOCK)) * 4);
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(qr, (__attribute__((address_space(3))) void *)(pq.stage + (j * PBLOCK + wv * 64)),
+                                                   4, (uint32_t)(tid * 4), 0, 0, 0);
+        }
+    }
     // ---- head of step n
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -286,6 +385,7 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
         __builtin_amdgcn_raw_buffer_store_b32((uint32_t)n, last_rsrc, (int)last_voff, k * PBLOCK * 4, 0);
       if (lane == 0) sh_mask[wv * K + k] = mask;
     }
+    if constexpr (LEAN_SW > 0) persist_lds_barrier();
     // ---- this wavefront's spikes of step n, in lists of up to 64
     const int sb = (int)(n % D);
     int ns = 0;
@@ -295,30 +395,77 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
       if (mask == 0ull) continue;
       const int c = __popcll(mask);
       if (ns + c > 64) {
-        persist_emit(g, sh_spk, ns, n, sb, lane);
+        persist_emit(g, sh_spk, ns, n, sb, lane, off, pq);
         ns = 0;
       }
       if ((mask >> lane) & 1ull)
         sh_spk[ns + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)(off + k * PBLOCK + tid);
       ns += c;
     }
-    if (ns > 0) persist_emit(g, sh_spk, ns, n, sb, lane);
+    if (ns > 0) persist_emit(g, sh_spk, ns, n, sb, lane, off, pq);
+    // ---- slot n's counts into sh_cnt
+    if constexpr (LEAN_SW > 0) {
+      // this wavefront's staged entries have arrived once its vector memory
+      // counter is zero (they were loaded by its own lanes)
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      // this lane's staging address, formed here so that the compiler does
+      // not keep one address per entry live across the step loop
+      typedef __attribute__((address_space(3))) const uint32_t lds_u32;
+      uint32_t t = (uint32_t)tid;
+      asm volatile("" : "+v"(t));
+      lds_u32 *stg = (lds_u32 *)pq.stage + t;
+#pragma unroll
+      for (int j = 0; j < PQ; ++j) {
+        if ((uint32_t)(j * PBLOCK + tid) < ce) {
+          const uint32_t e = stg[j * PBLOCK];
+          atomicAdd(sh_cnt + (e >> 5), 1u << (e & 31u));
+        }
+      }
+      for (uint32_t i = (uint32_t)(PQ * PBLOCK + tid); i < ce; i += PBLOCK) {
+        const uint32_t e = qslot[i];
+        atomicAdd(sh_cnt + (e >> 5), 1u << (e & 31u));
+      }
+      if (dirty) PERSIST_RING_SUM(sl)
+      if (tid == 0) {
+        pq.cnt[sl] = 0u;
+        if (dirty) atomicAnd(pq.dirty + (sl >> 5), ~(1u << (sl & 31)));
+      }
+    }
     __syncthreads();
   }
 
-  // ---- write back
+  // ---- write back (the offset passes through an empty asm statement so
+  // that the prologue's resources are rebuilt here, not kept in scalar
+  // registers through the step loop)
+  int64_t offw = off;
+  asm volatile("" : "+s"(offw));
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    if (!((valid >> k) & 1u)) continue;
-    const int64_t i = off + k * PBLOCK + tid;
     T x[M];
     PERSIST_GET(k, x)
 #pragma unroll
-    for (int j = 0; j < M; ++j) static_cast<T *>(g.S)[(int64_t)j * N + i] = x[j];
+    for (int j = 0; j < M; ++j) persist_st<T>(x[j], PERSIST_SRSRC(k, j, offw), state_voff, 0u);
+  }
+  // ---- entries still queued go to the ring (the barrier that ended the
+  // last step ordered every entry and fill count before these reads)
+  if constexpr (LEAN_SW > 0) {
+    const RingDev &rd = g.rings[0];
+    const int pbits = rd.nch * rd.bits;
+    for (int q = 0; q < D; ++q) {
+      const uint32_t c = min(pq.cnt[q], pq.Q);
+      const uint32_t *qs = pq.base + (int64_t)q * pq.Q;
+      unsigned int *rw = reinterpret_cast<unsigned int *>(ring + (int64_t)q * slot_bytes);
+      for (uint32_t i = (uint32_t)tid; i < c; i += PBLOCK) {
+        const uint32_t e = qs[i];
+        const int64_t pos = (off + (e >> 5)) * pbits + (e & 31u);
+        atomicAdd(rw + (pos >> 5), 1u << (int)(pos & 31));
+      }
+    }
   }
 #undef PERSIST_GET
 #undef PERSIST_PUT
-#undef PERSIST_FETCH
+#undef PERSIST_RING_SUM
+#undef PERSIST_SRSRC
 #undef slot_n
 }
 
@@ -326,24 +473,41 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
 // network's slots per thread pick the body, so a small network does not pay
 // for the largest one's slots.  Two shapes are compiled: PBLOCK = 1024 for
 // large networks (bodies up to KMAX slots: 20 in FP32, its last 4 slots'
-// state in LDS, given as dynamic LDS of 4 x LEAN_M x PBLOCK values; 8 in
-// FP64, where a slot costs twice the registers) and PBLOCK = 256 for
-// networks of up to 256 x KMAX neurons, several workgroups to a compute
-// unit, so that one network's memory latency overlaps another's work.
+// state in LDS; 8 in FP64, where a slot costs twice the registers) and
+// PBLOCK = 256 for networks of up to 256 x KMAX neurons, several workgroups
+// to a compute unit, so that one network's memory latency overlaps another's
+// work.  Dynamic LDS, in this order: the LDS state slots (4 x LEAN_M x PBLOCK
+// values when KMAX is 20), the count words (KMAX x PBLOCK), the staged queue
+// entries (PQ x PBLOCK words: 4 x 1024, 2 x 256), the fill counts (D) and
+// the dirty bits ((D+31)/32 words).  The queues: qnum entries per
+// slot per 16 neurons, network [off, off+np) at queue + D*(off*qnum/16).
 template <int KMAX, int PBLOCK>
 __global__ __launch_bounds__(PBLOCK) void k_persist(const GroupDev g, const int64_t *__restrict__ part_off,
                                                     const int32_t *__restrict__ order, int nparts, int64_t n0,
-                                                    int nsteps, int skip_first_tail) {
+                                                    int nsteps, int skip_first_tail, uint32_t *queue, int qnum) {
+  constexpr int KLMAX = KMAX == 20 ? 4 : 0;
+  constexpr int PQ = PBLOCK >= 1024 ? 4 : 2;
   __shared__ unsigned long long sh_mask[(PBLOCK / 64) * KMAX];
   __shared__ int32_t sh_spk[PBLOCK];
   extern __shared__ __align__(16) unsigned char smem[];
   lean_t *sh_state = reinterpret_cast<lean_t *>(smem);
+  uint32_t *sh_cnt = reinterpret_cast<uint32_t *>(smem + (size_t)KLMAX * LEAN_M * PBLOCK * sizeof(lean_t));
   if ((int)blockIdx.x >= nparts) return;  // uniform over the workgroup
   const int part = order[blockIdx.x];
   const int64_t off = part_off[part];
   const int np = (int)(part_off[part + 1] - off);
   const int nk = (np + PBLOCK - 1) / PBLOCK;
-#define PERSIST_BODY(K_, KL_) persist_network<K_, KL_, PBLOCK>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_spk, sh_state)
+  PersistQueue pq;
+  const int64_t qlo = (off * qnum) >> 4;
+  pq.Q = (uint32_t)((((off + np) * qnum) >> 4) - qlo);
+  pq.base = queue + (int64_t)g.D * qlo;
+  pq.stage = sh_cnt + KMAX * PBLOCK;
+  pq.cnt = pq.stage + PQ * PBLOCK;
+  pq.dirty = pq.cnt + g.D;
+  // the count words start at zero and every tail leaves them at zero
+  for (int q = threadIdx.x; q < KMAX * PBLOCK; q += PBLOCK) sh_cnt[q] = 0u;
+#define PERSIST_BODY(K_, KL_) \
+  persist_network<K_, KL_, PBLOCK, PQ>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_spk, sh_state, sh_cnt, pq)
   if (nk <= 1) PERSIST_BODY(1, 0);
   else if (nk <= 2) PERSIST_BODY(2, 0);
   else if (nk <= 4) PERSIST_BODY(4, 0);

@@ -344,7 +344,14 @@ int bm_net_set_partition(bm_net *net, int gid, int nparts, const int64_t *offset
  *                              most 20,480 neurons in FP32 (16,384 for models
  *                              of more than 3 variables, 8,192 in FP64);
  *                              refused otherwise
- *   BM_TUNE_PERSIST_CHUNK      steps per persistent launch (0: the whole run) */
+ *   BM_TUNE_PERSIST_CHUNK      steps per persistent launch (0: the whole run)
+ *   BM_TUNE_PERSIST_QUEUE      persistent deliveries: event-queue entries per
+ *                              ring slot per 16 neurons of a network (default
+ *                              16, one per neuron; 0..256).  A synapse whose
+ *                              queue is full is delivered through the count
+ *                              ring instead, so results never depend on it;
+ *                              the queues take D x N x value / 16 x 4 bytes,
+ *                              allocated at the first persistent run */
 enum {
   BM_TUNE_WINDOW = 1,
   BM_TUNE_NEURONS_PER_THREAD = 2,
@@ -358,7 +365,8 @@ enum {
   BM_TUNE_DEFER = 10,
   BM_TUNE_LEAN = 11,
   BM_TUNE_PERSIST = 12,
-  BM_TUNE_PERSIST_CHUNK = 13
+  BM_TUNE_PERSIST_CHUNK = 13,
+  BM_TUNE_PERSIST_QUEUE = 14
 };
 int bm_net_set_tuning(bm_net *net, int key, int value);
 int bm_net_get_tuning(bm_net *net, int key, int *value);
@@ -393,7 +401,8 @@ int bm_net_get_profile(bm_net *net, int64_t *launches, double *total_ms, double 
  * driving one linear neuron (S <- A.S + C, nvars <= 4) through a plastic
  * Connection onto S[tgt_var], with ExponentialSTDP (brian/stdp.py:452-539,
  * updaters :371-393, order connection -> pre -> post) and Reset(vr).
- * One wavefront per instance; FP32 weights carry Kahan compensation.
+ * One workgroup per instance (layout below); FP32 weights carry Kahan
+ * compensation.
  * w0 is [ninst][nin], S0 is [ninst][nvars]. */
 typedef struct bm_stdp_batch bm_stdp_batch;
 int bm_stdp_batch_create(int precision, double dt, int ninst, int nin, int nvars, const double *A,

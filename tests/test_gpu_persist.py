@@ -1,10 +1,12 @@
 """Persistent per-network launches (BM_TUNE_PERSIST) on uneven batches.
 
 The persistent kernel runs the same per-neuron arithmetic as the windowed
-lean kernel (the model compiled in from the same source), and count-ring
-deliveries are order-independent, so its rasters and states must equal the
-windowed kernel's bit for bit, in FP32 and FP64, whatever the run lengths
-and the steps per launch.  The FP64 windowed path itself equals the oracle
+lean kernel (the model compiled in from the same source), and its
+deliveries (per-network event queues summed in LDS, the count ring for
+synapses whose queue is full) add the same integer counts as the windowed
+kernel's count ring, so its rasters and states must equal the windowed
+kernel's bit for bit, in FP32 and FP64, whatever the run lengths, the steps
+per launch and the queue sizes.  The FP64 windowed path itself equals the oracle
 (tests/test_gpu_uneven.py); here the FP64 persistent path is also compared
 with the oracle directly.
 """
@@ -17,10 +19,12 @@ from tests.netbuild import device_uneven, first_divergence, host_uneven_csr, ora
 pytestmark = pytest.mark.gpu
 
 
-def _run(d, prec, persist, chunks, persist_chunk=0):
+def _run(d, prec, persist, chunks, persist_chunk=0, queue=None):
     h, g, _, _ = device_uneven(d, precision=prec)
     if persist:
         h.set_tuning("persist_chunk", persist_chunk)
+        if queue is not None:
+            h.set_tuning("persist_queue", queue)
         h.set_tuning("persist", 1)
     for c in chunks:
         h.run(c)
@@ -39,6 +43,24 @@ def test_persist_equals_windowed(prec):
     (ws, wi), wst = _run(d, prec, False, chunks)
     (ps, pi), pst = _run(d, prec, True, chunks, persist_chunk=333)
     print(f"FP{prec}: {len(ws)} spikes windowed, {len(ps)} persistent", flush=True)
+    assert len(ws) > 1000
+    np.testing.assert_array_equal(ps, ws)
+    np.testing.assert_array_equal(pi, wi)
+    for v in range(3):
+        np.testing.assert_array_equal(pst[v], wst[v])
+
+
+@pytest.mark.parametrize("queue", [0, 1])
+def test_persist_queue_overflow_equals_windowed(queue):
+    """Deliveries that find their slot's queue full go through the count
+    ring: with no queues (0) every synapse does, with one entry per slot per
+    16 neurons (1) a good part of them; rasters and states stay equal to the
+    windowed kernel's."""
+    d = W.uneven_batch(nnet=16, seed=13, lo=300, hi=12000)
+    chunks = (1, 600, 900)
+    (ws, wi), wst = _run(d, 32, False, chunks)
+    (ps, pi), pst = _run(d, 32, True, chunks, persist_chunk=400, queue=queue)
+    print(f"queue {queue}: {len(ws)} spikes windowed, {len(ps)} persistent", flush=True)
     assert len(ws) > 1000
     np.testing.assert_array_equal(ps, ws)
     np.testing.assert_array_equal(pi, wi)
@@ -102,4 +124,10 @@ def test_persist_refusals():
     h.set_partition(g, off)
     with pytest.raises(Exception, match="joins two networks"):
         h.set_tuning("persist", 1)
+    h.close()
+    # queue sizes outside 0..256 entries per slot per 16 neurons
+    d = W.uneven_batch(nnet=2, seed=1, lo=500, hi=900)
+    h, g, _, _ = device_uneven(d, precision=32, record=False)
+    with pytest.raises(Exception, match="persist_queue"):
+        h.set_tuning("persist_queue", 257)
     h.close()
