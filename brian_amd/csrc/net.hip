@@ -102,12 +102,13 @@ struct HostGroup {
   std::vector<int32_t> part_order;
   bool part_checked = false;
   int64_t part_max = 0;
-  JitKernel pjit, pjit_small;  // k_persist<pjit_K, 1024>, k_persist<pjit_K_small, 256>
-  int pjit_K = 0, pjit_K_small = 0;
+  JitKernel pjit[3];  // k_persist<pjit_K[t], PERSIST_TIER_BLOCK[t]>
+  int pjit_K[3] = {0, 0, 0};
   // the persistent kernel's event queues: D slots of persist_queue entries
   // per 16 neurons (pqueue_q), allocated at the first persistent run
   DevBuf pqueue;
   int pqueue_q = -1;
+  DevBuf prloc;  // PERSIST_RLOC staged raster entries per network
   struct Mon {
     int var = 0, timestep = 1, mode = 0;
     int64_t cap = 0;
@@ -178,8 +179,8 @@ struct bm_net {
   bool persist = false;   // batches of independent networks: one persistent launch per run (k_persist)
   int persist_chunk = 0;  // steps per persistent launch, 0: the whole run
   int persist_queue = 16; // persistent event-queue entries per ring slot per 16 neurons (0: deliveries through the ring)
-  hipStream_t stream2 = nullptr;              // the small networks' persistent launches
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t pstream[2] = {nullptr, nullptr};  // the smaller workgroup shapes' persistent launches
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
   bool use_graph = true;
   std::vector<std::pair<int, int>> mon_ids;   // (group, monitor within group)
   // tiling knobs (bm_net_set_tuning)
@@ -208,8 +209,10 @@ struct bm_net {
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     for (auto e : ev) hipEventDestroy(e);
     if (ev_fork) hipEventDestroy(ev_fork);
-    if (ev_join) hipEventDestroy(ev_join);
-    if (stream2) hipStreamDestroy(stream2);
+    for (int t = 0; t < 2; ++t) {
+      if (ev_join[t]) hipEventDestroy(ev_join[t]);
+      if (pstream[t]) hipStreamDestroy(pstream[t]);
+    }
     if (stream) hipStreamDestroy(stream);
   }
 };
@@ -533,31 +536,36 @@ void ensure_jit(bm_net *net) {
 }
 
 // ---- persistent launches for batches of independent networks (k_persist)
-constexpr int PERSIST_BLOCK = 1024;       // PBLOCK of the large networks' kernel
-constexpr int PERSIST_BLOCK_SMALL = 256;  // PBLOCK of the small networks' kernel
+// Three workgroup shapes; a network runs on the smallest that holds it
+// (persist_kmax neurons per thread), so that mid-size and small networks
+// run two or four workgroups to a compute unit and one network's memory
+// latency overlaps another's work.
+constexpr int PERSIST_TIERS = 3;
+constexpr int PERSIST_TIER_BLOCK[PERSIST_TIERS] = {1024, 512, 256};
+constexpr int PERSIST_BLOCK = 1024;  // the largest shape
 // neurons per thread of the largest body of k_persist (net_persist.hpp):
-// 20 in FP32, the last 4 slots' state in LDS (models of up to 3 variables,
-// 48 KiB), 16 for larger FP32 models, 8 in FP64
+// 20 in FP32, the last 4 slots' state in LDS (models of up to 3 variables),
+// 16 for larger FP32 models, 8 in FP64
 int persist_kmax(const bm_net *net, const HostGroup &g) {
   if (net->precision == 64) return 8;
   return g.M <= 3 ? 20 : 16;
 }
-// the small networks' kernel: 256 threads and up to 16 slots (8 in FP64),
-// several workgroups to a compute unit
-int persist_kmax_small(const bm_net *net) { return net->precision == 64 ? 8 : 16; }
-int64_t persist_small_cap(const bm_net *net) { return (int64_t)PERSIST_BLOCK_SMALL * persist_kmax_small(net); }
+// the shape of a network of np neurons
+int persist_tier(const bm_net *net, const HostGroup &g, int64_t np) {
+  const int64_t K = persist_kmax(net, g);
+  return np > 512 * K ? 0 : (np > 256 * K ? 1 : 2);
+}
 // dynamic LDS of k_persist (net_persist.hpp): the LDS state slots of the
 // 20-slot body, a count word per neuron, the staged queue entries, a fill
 // count and a dirty bit per ring slot
-size_t persist_shmem(const bm_net *net, const HostGroup &g, bool small) {
-  const int K = small ? persist_kmax_small(net) : persist_kmax(net, g);
-  const int block = small ? PERSIST_BLOCK_SMALL : PERSIST_BLOCK;
+size_t persist_shmem(const bm_net *net, const HostGroup &g, int block) {
+  const int K = persist_kmax(net, g);
   const size_t state = K == 20 ? (size_t)4 * g.M * block * net->tsize() : 0;
-  const int pq = small ? 2 : 4;  // queue entries per thread staged in LDS
+  const int pq = block >= 512 ? 4 : 2;  // queue entries per thread staged in LDS
   return state + (size_t)4 * (K + pq) * block + (size_t)4 * g.D + (size_t)4 * ((g.D + 31) / 32);
 }
-// static LDS of k_persist: the threshold ballots and the spike lists
-size_t persist_static_lds(int kmax, int block) { return (size_t)8 * (block / 64) * kmax + (size_t)4 * block; }
+// static LDS of k_persist: the threshold ballots and the wavefronts' spike counts
+size_t persist_static_lds(int kmax, int block) { return (size_t)8 * (block / 64) * kmax + (size_t)4 * (block / 64); }
 // LDS a workgroup may use (160 KiB on gfx950)
 size_t lds_per_block() {
   int dev = 0, v = 0;
@@ -589,7 +597,7 @@ std::string persist_blocker(const bm_net *net) {
     if (g.part_max > (int64_t)PERSIST_BLOCK * persist_kmax(net, g))
       return "a network exceeds " + std::to_string(PERSIST_BLOCK * persist_kmax(net, g)) +
              " neurons, the largest a persistent workgroup holds";
-    if (persist_shmem(net, g, false) + persist_static_lds(persist_kmax(net, g), PERSIST_BLOCK) > lds_per_block())
+    if (persist_shmem(net, g, PERSIST_BLOCK) + persist_static_lds(persist_kmax(net, g), PERSIST_BLOCK) > lds_per_block())
       return "the ring is too deep for persistent launches (" + std::to_string(g.D) + " slots)";
     // a network's queues are addressed with 32-bit byte offsets
     if ((double)g.D * (double)(((g.part_max * net->persist_queue) >> 4) + 1) * 4.0 >= 4294967296.0)
@@ -625,17 +633,13 @@ void check_partitions(bm_net *net) {
 void ensure_persist_jit(bm_net *net) {
   for (auto &gp : net->groups) {
     HostGroup &g = *gp;
-    const int K = persist_kmax(net, g), Ks = persist_kmax_small(net);
-    if (!(g.pjit.fn && g.pjit_K == K)) {
-      const std::string name = "bm::k_persist<" + std::to_string(K) + ", " + std::to_string(PERSIST_BLOCK) + ">";
-      g.pjit = jit_compile(lean_prelude(net, g, "net_persist.hpp") + "// " + name + "\n", name);
-      g.pjit_K = K;
-    }
-    if (!(g.pjit_small.fn && g.pjit_K_small == Ks)) {
+    const int K = persist_kmax(net, g);
+    for (int t = 0; t < PERSIST_TIERS; ++t) {
+      if (g.pjit[t].fn && g.pjit_K[t] == K) continue;
       const std::string name =
-          "bm::k_persist<" + std::to_string(Ks) + ", " + std::to_string(PERSIST_BLOCK_SMALL) + ">";
-      g.pjit_small = jit_compile(lean_prelude(net, g, "net_persist.hpp") + "// " + name + "\n", name);
-      g.pjit_K_small = Ks;
+          "bm::k_persist<" + std::to_string(K) + ", " + std::to_string(PERSIST_TIER_BLOCK[t]) + ">";
+      g.pjit[t] = jit_compile(lean_prelude(net, g, "net_persist.hpp") + "// " + name + "\n", name);
+      g.pjit_K[t] = K;
     }
   }
 }
@@ -657,6 +661,14 @@ void run_persist(bm_net *net, int64_t nsteps) {
     if (entries > 0) g.pqueue.alloc((size_t)entries * 4);
     g.pqueue_q = net->persist_queue;
   }
+  for (auto &gp : net->groups) {
+    HostGroup &g = *gp;
+    const size_t want = g.dev.raster ? (g.part_off.size() - 1) * (size_t)PERSIST_RLOC * 8 : 0;
+    if (g.prloc.bytes != want) {
+      BM_HIP(hipStreamSynchronize(net->stream));
+      g.prloc.alloc(want);
+    }
+  }
   const int64_t chunk = net->persist_chunk > 0 ? net->persist_chunk : nsteps;
   for (int64_t done = 0; done < nsteps; done += chunk) {
     const int n = (int)std::min<int64_t>(chunk, nsteps - done);
@@ -668,43 +680,57 @@ void run_persist(bm_net *net, int64_t nsteps) {
     }
     for (auto &gp : net->groups) {
       HostGroup &g = *gp;
-      // the order lists the networks largest first: those above the small
-      // kernel's capacity lead, and run on the 1024-thread kernel
+      // the order lists the networks largest first, so each shape's
+      // networks are a range of it
       const int nparts = (int)g.part_off.size() - 1;
-      int nbig = 0;
-      while (nbig < nparts &&
-             g.part_off[g.part_order[nbig] + 1] - g.part_off[g.part_order[nbig]] > persist_small_cap(net))
-        ++nbig;
-      const int nsmall = nparts - nbig;
+      int first[PERSIST_TIERS + 1];
+      first[0] = 0;
+      for (int t = 0, i = 0; t < PERSIST_TIERS; ++t) {
+        while (i < nparts && persist_tier(net, g, g.part_off[g.part_order[i] + 1] - g.part_off[g.part_order[i]]) == t)
+          ++i;
+        first[t + 1] = i;
+      }
       const int64_t *po = g.part_off_d.as<int64_t>();
       int64_t n0 = net->step + done;
       int ns = n, skip = 1;
       uint32_t *queue = g.pqueue.as<uint32_t>();
       int qnum = queue ? g.pqueue_q : 0;
-      auto launch = [&](const JitKernel &k, const int32_t *ord, int cnt, int block, size_t shmem, hipStream_t st) {
-        void *args[] = {(void *)&g.dev, (void *)&po,   (void *)&ord,   (void *)&cnt,  (void *)&n0,
-                        (void *)&ns,    (void *)&skip, (void *)&queue, (void *)&qnum};
-        BM_HIP(hipModuleLaunchKernel(k.fn, (unsigned)cnt, 1, 1, block, 1, 1, (unsigned)shmem, st, args, nullptr));
+      const int32_t *ord0 = g.part_order_d.as<int32_t>();
+      // the networks of shape t (workgroup b: network order[first[t] + b],
+      // staged raster entries at rloc + (first[t] + b) * PERSIST_RLOC)
+      auto launch = [&](int t, hipStream_t st) {
+        const int32_t *ord = ord0 + first[t];
+        int cnt = first[t + 1] - first[t];
+        unsigned long long *rloc =
+            g.prloc.p ? g.prloc.as<unsigned long long>() + (size_t)first[t] * PERSIST_RLOC : nullptr;
+        void *args[] = {(void *)&g.dev, (void *)&po,    (void *)&ord,  (void *)&cnt,   (void *)&n0, (void *)&ns,
+                        (void *)&skip,  (void *)&queue, (void *)&qnum, (void *)&rloc};
+        const int block = PERSIST_TIER_BLOCK[t];
+        BM_HIP(hipModuleLaunchKernel(g.pjit[t].fn, (unsigned)cnt, 1, 1, block, 1, 1,
+                                     (unsigned)persist_shmem(net, g, block), st, args, nullptr));
         BM_HIP(hipGetLastError());
       };
-      const int32_t *ord = g.part_order_d.as<int32_t>();
-      if (nbig > 0 && nsmall > 0) {
-        // both kernels at once: the small networks on a second stream
-        if (!net->stream2) {
-          BM_HIP(hipStreamCreateWithFlags(&net->stream2, hipStreamNonBlocking));
-          BM_HIP(hipEventCreateWithFlags(&net->ev_fork, hipEventDisableTiming));
-          BM_HIP(hipEventCreateWithFlags(&net->ev_join, hipEventDisableTiming));
-        }
+      // the shapes in use run at once: the first on the handle's stream, the
+      // others on streams of their own that join it afterwards
+      int used[PERSIST_TIERS], nused = 0;
+      for (int t = 0; t < PERSIST_TIERS; ++t)
+        if (first[t + 1] > first[t]) used[nused++] = t;
+      if (nused > 1) {
+        if (!net->ev_fork) BM_HIP(hipEventCreateWithFlags(&net->ev_fork, hipEventDisableTiming));
         BM_HIP(hipEventRecord(net->ev_fork, net->stream));
-        BM_HIP(hipStreamWaitEvent(net->stream2, net->ev_fork, 0));
-        launch(g.pjit_small, ord + nbig, nsmall, PERSIST_BLOCK_SMALL, persist_shmem(net, g, true), net->stream2);
-        launch(g.pjit, ord, nbig, PERSIST_BLOCK, persist_shmem(net, g, false), net->stream);
-        BM_HIP(hipEventRecord(net->ev_join, net->stream2));
-        BM_HIP(hipStreamWaitEvent(net->stream, net->ev_join, 0));
-      } else if (nbig > 0) {
-        launch(g.pjit, ord, nbig, PERSIST_BLOCK, persist_shmem(net, g, false), net->stream);
-      } else {
-        launch(g.pjit_small, ord, nsmall, PERSIST_BLOCK_SMALL, persist_shmem(net, g, true), net->stream);
+      }
+      for (int u = 1; u < nused; ++u) {
+        if (!net->pstream[u - 1]) {
+          BM_HIP(hipStreamCreateWithFlags(&net->pstream[u - 1], hipStreamNonBlocking));
+          BM_HIP(hipEventCreateWithFlags(&net->ev_join[u - 1], hipEventDisableTiming));
+        }
+        BM_HIP(hipStreamWaitEvent(net->pstream[u - 1], net->ev_fork, 0));
+        launch(used[u], net->pstream[u - 1]);
+      }
+      if (nused > 0) launch(used[0], net->stream);
+      for (int u = 1; u < nused; ++u) {
+        BM_HIP(hipEventRecord(net->ev_join[u - 1], net->pstream[u - 1]));
+        BM_HIP(hipStreamWaitEvent(net->stream, net->ev_join[u - 1], 0));
       }
     }
     if (net->profiling) {

@@ -129,20 +129,45 @@ __device__ inline void persist_ring_add(const OutDev &od, int ds, int64_t tgt) {
   atomicAdd(reinterpret_cast<unsigned int *>(dslot) + (pos >> 5), 1u << (int)(pos & 31));
 }
 
-// The spikes listed in sh_spk[0..ns) (neuron indices of the group, at
-// most 64, all from one wavefront) of step n: raster append and queue
-// entries.  Per connection, lane j < ns looks up spike j's row, an
-// inclusive scan over lanes spreads the rows' synapses over the wavefront,
-// and each pass stages up to 64*PL (column, delay) pairs before queueing
-// them.  The raster position is reserved first and used last, so its
-// atomic's round trip overlaps the loads.
-__device__ __attribute__((always_inline)) inline void persist_emit(const GroupDev &g, const int32_t *sh_spk, int ns,
-                                                                 int64_t n, int sb, int lane, int64_t off,
-                                                                 const PersistQueue &pq) {
+// The workgroup's raster entries go first to its own staging buffer (plain
+// stores by wavefront 0, which keeps the fill count); a full buffer, and the
+// rest at the end of the launch, is appended to the group's raster region
+// blockIdx % RSHARD with one atomic reservation, so no step waits for a
+// contended atomic's round trip.
+struct PersistRaster {
+  unsigned long long *buf;  // [PERSIST_RLOC] this workgroup's staged entries
+  int cnt;                  // staged entries (wavefront 0)
+};
+
+// Appends the staged entries to the raster (SpikeMonitor, brian/monitor.py:
+// 156-166) and empties the buffer; wavefront 0 only (it wrote them).
+__device__ inline void persist_raster_flush(const GroupDev &g, PersistRaster &pr, int lane) {
+  if (pr.cnt == 0) return;
   const int shard = blockIdx.x & (RSHARD - 1);
   unsigned long long rb = 0;
-  if (g.raster != nullptr && lane == 0) rb = atomicAdd(g.raster_count + shard * RSTRIDE, (unsigned long long)ns);
-  const int32_t mine = lane < ns ? sh_spk[lane] : 0;
+  if (lane == 0) rb = atomicAdd(g.raster_count + shard * RSTRIDE, (unsigned long long)pr.cnt);
+  rb = __shfl(rb, 0);
+  for (int i = lane; i < pr.cnt; i += 64)
+    if (rb + i < (unsigned long long)g.raster_cap) g.raster[(int64_t)shard * g.raster_cap + (int64_t)(rb + i)] = pr.buf[i];
+  pr.cnt = 0;
+}
+
+// Up to 64 spikes of step n of the workgroup, spike j's neuron (index in the
+// group) in lane j < ns of every wavefront: queue entries for their
+// synapses, shared by the NW wavefronts, and the raster entries (wavefront
+// 0).  Per connection, lane j looks up spike j's row, an inclusive scan over
+// lanes lays the rows' synapses end to end, and wavefront wv takes passes
+// wv, wv + NW, ... of 64*PL synapses, staging each pass's (column, delay)
+// pairs before queueing them.
+template <int NW>
+__device__ __attribute__((always_inline)) inline void persist_emit(const GroupDev &g, int32_t mine, int ns, int64_t n,
+                                                                 int sb, int lane, int wv, int64_t off,
+                                                                 const PersistQueue &pq, PersistRaster &pr) {
+  if (g.raster != nullptr && wv == 0) {
+    if (pr.cnt + ns > PERSIST_RLOC) persist_raster_flush(g, pr, lane);
+    if (lane < ns) pr.buf[pr.cnt + lane] = ((uint64_t)n << 32) | (uint64_t)(uint32_t)mine;
+    pr.cnt += ns;
+  }
   for (int o = 0; o < g.nout; ++o) {
     const OutDev &od = g.outs[o];
     int64_t lo = 0;
@@ -167,7 +192,7 @@ __device__ __attribute__((always_inline)) inline void persist_emit(const GroupDe
     // the network's queues as one resource: D*Q entries, 32-bit offsets
     const __amdgpu_buffer_rsrc_t qrs = persist_rsrc(pq.base, (int64_t)od.D * pq.Q * 4);
     const int64_t loc0 = od.tgt_off - off;  // target column -> neuron within the network
-    for (int base = 0; base < total; base += 64 * PL) {
+    for (int base = wv * 64 * PL; base < total; base += NW * 64 * PL) {
       int32_t cols[PL];
       int dls[PL];
 #pragma unroll
@@ -231,31 +256,25 @@ __device__ __attribute__((always_inline)) inline void persist_emit(const GroupDe
       }
     }
   }
-  // raster append (SpikeMonitor, brian/monitor.py:156-166)
-  if (g.raster != nullptr) {
-    rb = __shfl(rb, 0);
-    if (lane < ns && rb + lane < (unsigned long long)g.raster_cap)
-      g.raster[(int64_t)shard * g.raster_cap + (int64_t)(rb + lane)] = ((uint64_t)n << 32) | (uint64_t)(uint32_t)mine;
-  }
 }
 
 // One network of np neurons (offset off) for nsteps steps with K slots per
 // thread, the last KL of them in LDS (sh_state [KL][M][PBLOCK]); sh_mask
-// holds [wavefront][slot] threshold ballots of a step, sh_spk [wavefront][64]
-// a wavefront's spike list, sh_cnt the count words of the slot being summed
-// ([KMAX*PBLOCK], zero on entry); PQ queue entries per thread are loaded
-// into LDS ahead of the sum.
+// holds [wavefront][slot] threshold ballots of a step and sh_nsp
+// [wavefront] their spike counts, sh_cnt the count words of the slot being
+// summed ([KMAX*PBLOCK], zero on entry); PQ queue entries per thread are
+// loaded into LDS ahead of the sum.
 template <int K, int KL, int PBLOCK, int PQ>
 __device__ __attribute__((always_inline)) inline void persist_network(const GroupDev &g, int64_t off, int np, int64_t n0, int nsteps, int skip_first_tail,
-                                unsigned long long *sh_mask, int32_t *sh_spk_all, lean_t *sh_state,
-                                uint32_t *sh_cnt, PersistQueue pq) {
+                                unsigned long long *sh_mask, int32_t *sh_nsp, lean_t *sh_state,
+                                uint32_t *sh_cnt, PersistQueue pq, PersistRaster pr) {
+  constexpr int NW = PBLOCK / 64;  // wavefronts
   using T = lean_t;
   constexpr int M = LEAN_M;
   constexpr int KR = K - KL;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wavefront index, uniform
-  int32_t *sh_spk = sh_spk_all + wv * 64;
   const int64_t N = g.N;
   const int nk = (np + PBLOCK - 1) / PBLOCK;  // slots in use, uniform
   uint32_t valid = 0u;  // bit k: this lane holds a neuron in slot k
@@ -324,32 +343,23 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
     __syncthreads();
   }
 
-  for (int st = 0; st <= nsteps; ++st) {
+#define PERSIST_TAIL(k, x)                                                                       \
+  {                                                                                              \
+    /* lanes without a neuron deliver zero counts and never reset */                             \
+    if constexpr (LEAN_SW > 0) {                                                                 \
+      const uint32_t w = sh_cnt[(k) * PBLOCK + tid];                                             \
+      if (w != 0u) sh_cnt[(k) * PBLOCK + tid] = 0u;                                              \
+      lean_deliver(x, w);                                                                        \
+    }                                                                                            \
+    T r[M];                                                                                      \
+    _Pragma("unroll") for (int j = 0; j < M; ++j) r[j] = x[j];                                   \
+    lean_reset(r);                                                                               \
+    const bool rs = (prev >> (k)) & 1u;                                                          \
+    _Pragma("unroll") for (int j = 0; j < M; ++j) x[j] = rs ? r[j] : x[j];                       \
+  }
+  for (int st = 0; st < nsteps; ++st) {
     const int64_t n = n0 + st;
-    // ---- tail of step n-1
-    if (st > 0 || !skip_first_tail) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        T x[M];
-        PERSIST_GET(k, x)
-        // lanes without a neuron deliver zero counts and never reset
-        if constexpr (LEAN_SW > 0) {
-          const uint32_t w = sh_cnt[k * PBLOCK + tid];
-          if (w != 0u) sh_cnt[k * PBLOCK + tid] = 0u;
-          lean_deliver(x, w);
-        }
-        T r[M];
-#pragma unroll
-        for (int j = 0; j < M; ++j) r[j] = x[j];
-        lean_reset(r);
-        const bool rs = (prev >> k) & 1u;
-#pragma unroll
-        for (int j = 0; j < M; ++j) x[j] = rs ? r[j] : x[j];
-        PERSIST_PUT(k, x)
-      }
-    }
     if (++sl == D) sl = 0;
-    if (st == nsteps) break;
     // ---- slot n mod D: complete, its entries having come from spikes of
     // steps <= n-1, queued before the last barrier
     uint32_t ce = 0u;
@@ -371,11 +381,15 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
                                                    4, (uint32_t)(tid * 4), 0, 0, 0);
         }
     }
-    // ---- head of step n
+    // ---- tail of step n-1 (skipped at the first step of a run: the
+    // previous run applied it) and head of step n, slot by slot
+    const bool tail = st > 0 || !skip_first_tail;
+    int nsw = 0;  // this wavefront's spikes of step n
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       T x[M];
       PERSIST_GET(k, x)
+      if (tail) PERSIST_TAIL(k, x)
       lean_update(x);
       PERSIST_PUT(k, x)
       const bool sp = ((valid >> k) & 1u) && lean_spike(x);
@@ -384,25 +398,58 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
       if (sp)  // SpikeContainer's last spike step (only lanes holding a neuron store)
         __builtin_amdgcn_raw_buffer_store_b32((uint32_t)n, last_rsrc, (int)last_voff, k * PBLOCK * 4, 0);
       if (lane == 0) sh_mask[wv * K + k] = mask;
+      nsw += __popcll(mask);
     }
-    if constexpr (LEAN_SW > 0) persist_lds_barrier();
-    // ---- this wavefront's spikes of step n, in lists of up to 64
+    if (lane == 0) sh_nsp[wv] = nsw;
+    persist_lds_barrier();
+    // ---- the workgroup's spikes of step n (wavefront, then slot, then lane
+    // order), 64 at a time: every wavefront finds the same 64 neurons, and
+    // their synapses are shared out over the wavefronts
     const int sb = (int)(n % D);
-    int ns = 0;
-#pragma unroll 1
-    for (int k = 0; k < nk; ++k) {
-      const unsigned long long mask = sh_mask[wv * K + k];
-      if (mask == 0ull) continue;
-      const int c = __popcll(mask);
-      if (ns + c > 64) {
-        persist_emit(g, sh_spk, ns, n, sb, lane, off, pq);
-        ns = 0;
-      }
-      if ((mask >> lane) & 1ull)
-        sh_spk[ns + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)(off + k * PBLOCK + tid);
-      ns += c;
+    const int cw = lane < NW ? sh_nsp[lane] : 0;
+    int incw = cw;
+#pragma unroll
+    for (int d = 1; d < NW; d <<= 1) {
+      const int t = __shfl_up(incw, d);
+      if (lane >= d) incw += t;
     }
-    if (ns > 0) persist_emit(g, sh_spk, ns, n, sb, lane, off, pq);
+    const int S = __shfl(incw, NW - 1);
+    const int exw = incw - cw;
+#pragma unroll 1
+    for (int c0 = 0; c0 < S; c0 += 64) {
+      // spike c0 + lane (a lane past the end searches for the last spike):
+      // its wavefront w, the first with incw[w] > s, by a search that every
+      // lane runs
+      const int ss = min(c0 + lane, S - 1);
+      int a = -1;
+#pragma unroll
+      for (int step = NW / 2; step > 0; step >>= 1) {
+        const int m = a + step;
+        if (__shfl(incw, m) <= ss) a = m;
+      }
+      const int w = a + 1;
+      int r = ss - __shfl(exw, w);  // its rank among wavefront w's spikes
+      int32_t mine = 0;
+      if (c0 + lane < S) {
+        int k = 0;
+        unsigned long long m = 0ull;
+        for (; k < nk; ++k) {
+          m = sh_mask[w * K + k];
+          const int c = __popcll(m);
+          if (r < c) break;
+          r -= c;
+        }
+        // the r-th set bit of m
+        int pos = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+          const unsigned long long low = pos + step >= 64 ? ~0ull : ((1ull << (pos + step)) - 1ull);
+          if (__popcll(m & low) <= r) pos += step;
+        }
+        mine = (int32_t)(off + k * PBLOCK + w * 64 + pos);
+      }
+      persist_emit<NW>(g, mine, min(64, S - c0), n, sb, lane, wv, off, pq, pr);
+    }
     // ---- slot n's counts into sh_cnt
     if constexpr (LEAN_SW > 0) {
       // this wavefront's staged entries have arrived once its vector memory
@@ -432,6 +479,17 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
       }
     }
     __syncthreads();
+  }
+  if (g.raster != nullptr && wv == 0) persist_raster_flush(g, pr, lane);
+  // ---- the tail of the last step (of step n0-1 for an empty run)
+  if (nsteps > 0 || !skip_first_tail) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      T x[M];
+      PERSIST_GET(k, x)
+      PERSIST_TAIL(k, x)
+      PERSIST_PUT(k, x)
+    }
   }
 
   // ---- write back (the offset passes through an empty asm statement so
@@ -465,30 +523,35 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
 #undef PERSIST_GET
 #undef PERSIST_PUT
 #undef PERSIST_RING_SUM
+#undef PERSIST_TAIL
 #undef PERSIST_SRSRC
 #undef slot_n
 }
 
 // Workgroup b advances network order[b] (largest first) through the run; the
 // network's slots per thread pick the body, so a small network does not pay
-// for the largest one's slots.  Two shapes are compiled: PBLOCK = 1024 for
-// large networks (bodies up to KMAX slots: 20 in FP32, its last 4 slots'
-// state in LDS; 8 in FP64, where a slot costs twice the registers) and
-// PBLOCK = 256 for networks of up to 256 x KMAX neurons, several workgroups
-// to a compute unit, so that one network's memory latency overlaps another's
-// work.  Dynamic LDS, in this order: the LDS state slots (4 x LEAN_M x PBLOCK
-// values when KMAX is 20), the count words (KMAX x PBLOCK), the staged queue
-// entries (PQ x PBLOCK words: 4 x 1024, 2 x 256), the fill counts (D) and
-// the dirty bits ((D+31)/32 words).  The queues: qnum entries per
+// for the largest one's slots.  Bodies go up to KMAX slots: 20 in FP32, the
+// last 4 slots' state in LDS; 8 in FP64, where a slot costs twice the
+// registers.  The host compiles PBLOCK = 1024, 512 and 256 and gives each
+// network the smallest that holds it, so that smaller networks run two or
+// four workgroups to a compute unit and one network's memory latency
+// overlaps another's work.  Dynamic LDS, in this order: the LDS state slots
+// (4 x LEAN_M x PBLOCK values when KMAX is 20), the count words (KMAX x
+// PBLOCK), the staged queue entries (PQ x PBLOCK words: PQ = 4 from 512
+// threads, 2 below), the fill counts (D) and the dirty bits ((D+31)/32
+// words).  The queues: qnum entries per
 // slot per 16 neurons, network [off, off+np) at queue + D*(off*qnum/16).
+// rloc: PERSIST_RLOC staged raster entries per workgroup (null without a
+// raster).
 template <int KMAX, int PBLOCK>
 __global__ __launch_bounds__(PBLOCK) void k_persist(const GroupDev g, const int64_t *__restrict__ part_off,
                                                     const int32_t *__restrict__ order, int nparts, int64_t n0,
-                                                    int nsteps, int skip_first_tail, uint32_t *queue, int qnum) {
+                                                    int nsteps, int skip_first_tail, uint32_t *queue, int qnum,
+                                                    unsigned long long *rloc) {
   constexpr int KLMAX = KMAX == 20 ? 4 : 0;
-  constexpr int PQ = PBLOCK >= 1024 ? 4 : 2;
+  constexpr int PQ = PBLOCK >= 512 ? 4 : 2;
   __shared__ unsigned long long sh_mask[(PBLOCK / 64) * KMAX];
-  __shared__ int32_t sh_spk[PBLOCK];
+  __shared__ int32_t sh_nsp[PBLOCK / 64];
   extern __shared__ __align__(16) unsigned char smem[];
   lean_t *sh_state = reinterpret_cast<lean_t *>(smem);
   uint32_t *sh_cnt = reinterpret_cast<uint32_t *>(smem + (size_t)KLMAX * LEAN_M * PBLOCK * sizeof(lean_t));
@@ -504,10 +567,13 @@ __global__ __launch_bounds__(PBLOCK) void k_persist(const GroupDev g, const int6
   pq.stage = sh_cnt + KMAX * PBLOCK;
   pq.cnt = pq.stage + PQ * PBLOCK;
   pq.dirty = pq.cnt + g.D;
+  PersistRaster pr;
+  pr.buf = rloc + (int64_t)blockIdx.x * PERSIST_RLOC;
+  pr.cnt = 0;
   // the count words start at zero and every tail leaves them at zero
   for (int q = threadIdx.x; q < KMAX * PBLOCK; q += PBLOCK) sh_cnt[q] = 0u;
 #define PERSIST_BODY(K_, KL_) \
-  persist_network<K_, KL_, PBLOCK, PQ>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_spk, sh_state, sh_cnt, pq)
+  persist_network<K_, KL_, PBLOCK, PQ>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_nsp, sh_state, sh_cnt, pq, pr)
   if (nk <= 1) PERSIST_BODY(1, 0);
   else if (nk <= 2) PERSIST_BODY(2, 0);
   else if (nk <= 4) PERSIST_BODY(4, 0);

@@ -18,6 +18,8 @@ constexpr int MAXMON = 4;
 // atomic; a workgroup appends to region blockIdx % RSHARD
 constexpr int RSHARD = 32;
 constexpr int RSTRIDE = 8;     // counters are RSTRIDE uint64 apart
+// raster entries a persistent workgroup stages before appending them
+constexpr int PERSIST_RLOC = 2048;
 constexpr int BLOCK = 256;
 constexpr int WMAX = 16;
 constexpr int32_t NEVER = -2147483647 - 1;   // INT32_MIN
