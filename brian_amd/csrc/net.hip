@@ -564,8 +564,9 @@ size_t persist_shmem(const bm_net *net, const HostGroup &g, int block) {
   const int pq = block >= 512 ? 4 : 2;  // queue entries per thread staged in LDS
   return state + (size_t)4 * (K + pq) * block + (size_t)4 * g.D + (size_t)4 * ((g.D + 31) / 32);
 }
-// static LDS of k_persist: the threshold ballots and the wavefronts' spike counts
-size_t persist_static_lds(int kmax, int block) { return (size_t)8 * (block / 64) * kmax + (size_t)4 * (block / 64); }
+// static LDS of k_persist: the threshold ballots, the wavefronts' spike
+// counts and slots with spikes
+size_t persist_static_lds(int kmax, int block) { return (size_t)8 * (block / 64) * kmax + (size_t)8 * (block / 64); }
 // LDS a workgroup may use (160 KiB on gfx950)
 size_t lds_per_block() {
   int dev = 0, v = 0;

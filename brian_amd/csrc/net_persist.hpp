@@ -260,13 +260,14 @@ __device__ __attribute__((always_inline)) inline void persist_emit(const GroupDe
 
 // One network of np neurons (offset off) for nsteps steps with K slots per
 // thread, the last KL of them in LDS (sh_state [KL][M][PBLOCK]); sh_mask
-// holds [wavefront][slot] threshold ballots of a step and sh_nsp
-// [wavefront] their spike counts, sh_cnt the count words of the slot being
+// holds [wavefront][slot] threshold ballots of a step (stored for the slots
+// with spikes only), sh_nsp [wavefront] their spike counts and sh_nz
+// [wavefront] the slots with spikes, sh_cnt the count words of the slot being
 // summed ([KMAX*PBLOCK], zero on entry); PQ queue entries per thread are
 // loaded into LDS ahead of the sum.
 template <int K, int KL, int PBLOCK, int PQ>
 __device__ __attribute__((always_inline)) inline void persist_network(const GroupDev &g, int64_t off, int np, int64_t n0, int nsteps, int skip_first_tail,
-                                unsigned long long *sh_mask, int32_t *sh_nsp, lean_t *sh_state,
+                                unsigned long long *sh_mask, int32_t *sh_nsp, uint32_t *sh_nz, lean_t *sh_state,
                                 uint32_t *sh_cnt, PersistQueue pq, PersistRaster pr) {
   constexpr int NW = PBLOCK / 64;  // wavefronts
   using T = lean_t;
@@ -348,7 +349,7 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
     /* lanes without a neuron deliver zero counts and never reset */                             \
     if constexpr (LEAN_SW > 0) {                                                                 \
       const uint32_t w = sh_cnt[(k) * PBLOCK + tid];                                             \
-      if (w != 0u) sh_cnt[(k) * PBLOCK + tid] = 0u;                                              \
+      sh_cnt[(k) * PBLOCK + tid] = 0u;                                                           \
       lean_deliver(x, w);                                                                        \
     }                                                                                            \
     T r[M];                                                                                      \
@@ -382,25 +383,37 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
         }
     }
     // ---- tail of step n-1 (skipped at the first step of a run: the
-    // previous run applied it) and head of step n, slot by slot
-    const bool tail = st > 0 || !skip_first_tail;
-    int nsw = 0;  // this wavefront's spikes of step n
+    // previous run applied it) and head of step n, slot by slot; a slot's
+    // ballot is stored only when it holds spikes, and nzk marks those slots
+    int nsw = 0;        // this wavefront's spikes of step n
+    uint32_t nzk = 0u;  // its slots with spikes
+#define PERSIST_HEAD(k, x)                                                                       \
+  {                                                                                              \
+    lean_update(x);                                                                              \
+    PERSIST_PUT(k, x)                                                                            \
+    const bool sp = ((valid >> (k)) & 1u) && lean_spike(x);                                      \
+    const unsigned long long mask = __ballot(sp);                                                \
+    prev = (prev & ~(1u << (k))) | ((uint32_t)sp << (k));                                        \
+    if (mask != 0ull) {                                                                          \
+      if (sp) /* SpikeContainer's last spike step (lanes holding a neuron) */                    \
+        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)n, last_rsrc, (int)last_voff, (k) * PBLOCK * 4, 0); \
+      if (lane == 0) sh_mask[wv * K + (k)] = mask;                                               \
+      nzk |= 1u << (k);                                                                          \
+      nsw += __popcll(mask);                                                                     \
+    }                                                                                            \
+  }
+    const bool tail = st > 0 || !skip_first_tail;  // uniform
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       T x[M];
       PERSIST_GET(k, x)
       if (tail) PERSIST_TAIL(k, x)
-      lean_update(x);
-      PERSIST_PUT(k, x)
-      const bool sp = ((valid >> k) & 1u) && lean_spike(x);
-      const unsigned long long mask = __ballot(sp);
-      prev = sp ? (prev | (1u << k)) : (prev & ~(1u << k));
-      if (sp)  // SpikeContainer's last spike step (only lanes holding a neuron store)
-        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)n, last_rsrc, (int)last_voff, k * PBLOCK * 4, 0);
-      if (lane == 0) sh_mask[wv * K + k] = mask;
-      nsw += __popcll(mask);
+      PERSIST_HEAD(k, x)
     }
-    if (lane == 0) sh_nsp[wv] = nsw;
+    if (lane == 0) {
+      sh_nsp[wv] = nsw;
+      sh_nz[wv] = nzk;
+    }
     persist_lds_barrier();
     // ---- the workgroup's spikes of step n (wavefront, then slot, then lane
     // order), 64 at a time: every wavefront finds the same 64 neurons, and
@@ -433,7 +446,8 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
       if (c0 + lane < S) {
         int k = 0;
         unsigned long long m = 0ull;
-        for (; k < nk; ++k) {
+        for (uint32_t bits = sh_nz[w]; bits != 0u; bits &= bits - 1u) {
+          k = __builtin_ctz(bits);
           m = sh_mask[w * K + k];
           const int c = __popcll(m);
           if (r < c) break;
@@ -524,6 +538,7 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
 #undef PERSIST_PUT
 #undef PERSIST_RING_SUM
 #undef PERSIST_TAIL
+#undef PERSIST_HEAD
 #undef PERSIST_SRSRC
 #undef slot_n
 }
@@ -552,6 +567,7 @@ __global__ __launch_bounds__(PBLOCK) void k_persist(const GroupDev g, const int6
   constexpr int PQ = PBLOCK >= 512 ? 4 : 2;
   __shared__ unsigned long long sh_mask[(PBLOCK / 64) * KMAX];
   __shared__ int32_t sh_nsp[PBLOCK / 64];
+  __shared__ uint32_t sh_nz[PBLOCK / 64];
   extern __shared__ __align__(16) unsigned char smem[];
   lean_t *sh_state = reinterpret_cast<lean_t *>(smem);
   uint32_t *sh_cnt = reinterpret_cast<uint32_t *>(smem + (size_t)KLMAX * LEAN_M * PBLOCK * sizeof(lean_t));
@@ -573,7 +589,7 @@ __global__ __launch_bounds__(PBLOCK) void k_persist(const GroupDev g, const int6
   // the count words start at zero and every tail leaves them at zero
   for (int q = threadIdx.x; q < KMAX * PBLOCK; q += PBLOCK) sh_cnt[q] = 0u;
 #define PERSIST_BODY(K_, KL_) \
-  persist_network<K_, KL_, PBLOCK, PQ>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_nsp, sh_state, sh_cnt, pq, pr)
+  persist_network<K_, KL_, PBLOCK, PQ>(g, off, np, n0, nsteps, skip_first_tail, sh_mask, sh_nsp, sh_nz, sh_state, sh_cnt, pq, pr)
   if (nk <= 1) PERSIST_BODY(1, 0);
   else if (nk <= 2) PERSIST_BODY(2, 0);
   else if (nk <= 4) PERSIST_BODY(4, 0);
