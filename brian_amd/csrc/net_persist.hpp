@@ -179,15 +179,17 @@ __device__ __attribute__((always_inline)) inline void persist_emit(const GroupDe
         len = (int)(od.row_ptr[r + 1] - lo);
       }
     }
-    // inclusive scan of len over the wavefront
+    // inclusive scan of len over the first ns lanes (the others hold 0)
     int inc = len;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
+    for (int d = 1; d < ns; d <<= 1) {
       const int t = __shfl_up(inc, d);
       if (lane >= d) inc += t;
     }
-    const int total = __shfl(inc, 63);
-    const int pref = inc - len;  // spike j's row starts at synapse pref_j of the wavefront's list
+    const int total = __shfl(inc, ns - 1);
+    const int pref = inc - len;  // spike j's row starts at synapse pref_j of the list
+    const int64_t lmp = lo - pref;  // synapse q of spike j is lmp_j + q
+    // the largest power of two below ns: the owner search's first step
+    const int hb = ns > 1 ? 1 << (31 - __builtin_clz(ns - 1)) : 0;
     const uint32_t chbits = (uint32_t)(od.ch * od.bits);
     // the network's queues as one resource: D*Q entries, 32-bit offsets
     const __amdgpu_buffer_rsrc_t qrs = persist_rsrc(pq.base, (int64_t)od.D * pq.Q * 4);
@@ -198,39 +200,18 @@ __device__ __attribute__((always_inline)) inline void persist_emit(const GroupDe
 #pragma unroll
       for (int u = 0; u < PL; ++u) {
         const int q = base + u * 64 + lane;
-        // the spike owning synapse q: the last j < ns with pref_j <= q.
-        // Every lane runs the search (a lane past the list's end searches
-        // for the last synapse and loads nothing), so the cross-lane reads
-        // only ever read active lanes: a linear scan of scalar reads for a
-        // few spikes, a six-step search over shuffles for more
+        // the spike owning synapse q: the last j < ns with pref_j <= q, by
+        // a binary search of log2(ns) steps (none for one spike).  Every
+        // lane runs it (a lane past the list's end searches for the last
+        // synapse and loads nothing), so the shuffles only read active lanes
         const int qq = q < total ? q : total - 1;
-        int64_t loa;
-        int pa;
-        if (ns <= 8) {
-          loa = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)lo, 0) |
-                ((int64_t)__builtin_amdgcn_readlane((int)(lo >> 32), 0) << 32);
-          pa = 0;
-          for (int j = 1; j < ns; ++j) {
-            const int pj = __builtin_amdgcn_readlane(pref, j);
-            const int64_t lj = (int64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)lo, j) |
-                               ((int64_t)__builtin_amdgcn_readlane((int)(lo >> 32), j) << 32);
-            if (pj <= qq) {
-              pa = pj;
-              loa = lj;
-            }
-          }
-        } else {
-          int a = 0;
-#pragma unroll
-          for (int step = 32; step > 0; step >>= 1) {
-            const int m = a + step;
-            const int pm = __shfl(pref, m & 63);
-            if (m < ns && pm <= qq) a = m;
-          }
-          loa = __shfl(lo, a);
-          pa = __shfl(pref, a);
+        int a = 0;
+        for (int step = hb; step > 0; step >>= 1) {
+          const int m = a + step;
+          const int pm = __shfl(pref, m & 63);
+          if (m < ns && pm <= qq) a = m;
         }
-        const int64_t syn = loa + (qq - pa);
+        const int64_t syn = __shfl(lmp, a) + qq;
         cols[u] = -1;
         dls[u] = od.hom_delay;
         if (q < total) {
