@@ -65,12 +65,18 @@ def sweep_bytes_per_neuron(prec, window, ring_sw):
     return 2 * STATE_VARS * (prec // 8) + BYTES_LAST + 2 * ring_sw * window
 
 
-# A persistent launch of S steps (one workgroup per network, the state on the
-# chip for the whole launch) moves per neuron the state in and out once and
-# the ring read + zero per step; the last spike step is read once and
-# written per spike (+4 B on the spike's bytes).
-def persist_bytes_per_neuron(prec, steps, ring_sw):
-    return 2 * STATE_VARS * (prec // 8) + 4 + 2 * ring_sw * steps
+# A persistent launch (one workgroup per network, the state on the chip for
+# the whole launch, deliveries through per-network event queues) moves per
+# neuron the state in and out once, its last spike step in once, and the
+# count ring read + zeroed over the first D steps (the slots that may hold
+# counts from before the launch); per spike the row pointer pair 16 B, the
+# raster entry 8 B and the last spike step 4 B; per synaptic event the
+# column 4 B and delay 1 B read and the queue entry written and read 8 B.
+BYTES_EVENT_QUEUE = 13
+
+
+def persist_bytes_per_neuron(prec, ring_depth, ring_sw):
+    return 2 * STATE_VARS * (prec // 8) + 4 + 2 * ring_sw * ring_depth
 
 
 def ring_sw(tuning, nch=2):
@@ -477,6 +483,8 @@ def bench_uneven(args):
             sp = h.get_spikes(g)
             out["prof"] = (launches, prof_ms, window, len(sp[0]), events(sp[1]))
             out["prof_steps"] = prof_steps
+            # the ring's slots: the largest window + the longest delay + 1
+            out["ring_depth"] = h.get_tuning("window_max") + d["max_delay_steps"] + 1
             h.set_profiling(False)
         h.close()
         res[prec] = out
@@ -491,11 +499,11 @@ def bench_uneven(args):
     # per-synapse delay byte on every event
     if persist:
         steps_launch = r32["prof_steps"] / max(launches, 1)
-        alg_bytes = (d["N"] * persist_bytes_per_neuron(32, steps_launch, 4) + psp / launches * (BYTES_SPIKE + 4)
-                     + pev / launches * (BYTES_EVENT + 1))
+        alg_bytes = (d["N"] * persist_bytes_per_neuron(32, r32["ring_depth"], 4) + psp / launches * (BYTES_SPIKE + 4)
+                     + pev / launches * BYTES_EVENT_QUEUE)
         roof = roofline(alg_bytes, avg_launch_s, hbm_traffic("persist_uneven_pmc"),
-                        kernel="k_persist (one workgroup per network; 1024-thread kernel for networks above "
-                               "4096 neurons, 256-thread kernel beside it for the rest)",
+                        kernel="k_persist (one workgroup per network, on the smallest of 1024, 512 and 256 "
+                               "threads that holds it; three kernels at once)",
                         steps_per_launch=steps_launch, avg_launch_us=avg_launch_s * 1e6,
                         alg_bytes_per_launch=alg_bytes, launches_timed=launches)
     else:
