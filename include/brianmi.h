@@ -336,9 +336,10 @@ int bm_net_set_partition(bm_net *net, int gid, int nparts, const int64_t *offset
  *   BM_TUNE_PERSIST            (after prepare) 1: each run is one persistent
  *                              launch per group in which every network of the
  *                              group's partition (bm_net_set_partition) is
- *                              advanced by one workgroup through all the run's
- *                              steps, state in registers, one workgroup
- *                              barrier per step.  Needs lean groups with count
+ *                              advanced by one workgroup (1024, 512 or 256
+ *                              threads, the smallest that holds it) through
+ *                              all the run's steps, state in registers, one
+ *                              workgroup barrier per step.  Needs lean groups with count
  *                              rings, connections within a group, delays of
  *                              at least one step, no STDP, networks of at
  *                              most 20,480 neurons in FP32 (16,384 for models
