@@ -1244,14 +1244,15 @@ void prepare(bm_net *net, int window, int K, int use_graph) {
       // workgroups fills min(B, RSHARD) regions, the busiest taking the
       // spikes of ceil(B / RSHARD) workgroups (the generator kernel's grid
       // is 64 workgroups).  Neurons per thread may be retuned after prepare,
-      // so the regions are sized for the busiest share over K = 1, 2, 4 and
-      // counted for K = 1 (the most workgroups); each holds 1.5x that share,
-      // so an uneven spread over workgroups fits.
-      int64_t R = RSHARD, region = 0;
+      // so the regions are sized for the busiest share over K = 1, 2, 4; each
+      // holds 1.5x that share, so an uneven spread over workgroups fits.  All
+      // RSHARD regions exist whatever B: persistent launches give every
+      // network a workgroup, and may use them all.
+      const int64_t R = RSHARD;
+      int64_t region = 0;
       for (int K = 1; K <= 4; K *= 2) {
         const int64_t B = g.kind == GK_GENERATOR ? 64 : (g.N + (int64_t)BLOCK * K - 1) / ((int64_t)BLOCK * K);
         const int64_t per = (B + RSHARD - 1) / RSHARD;
-        if (K == 1) R = std::min<int64_t>(B, RSHARD);
         region = std::max(region, (g.raster_cap * per + B - 1) / B * 3 / 2 + 1024);
       }
       g.raster_region = region;

@@ -130,17 +130,21 @@ def cuba(N=4000, seed=1, connectivity="bernoulli", p=0.02, indegree=80, delay_st
 
 
 
-def uneven_batch(nnet=4096, seed=0, lo=500, hi=20000, p=0.02, dmin=0.1 * ms, dmax=5 * ms):
+def uneven_batch(nnet=4096, seed=0, lo=500, hi=20000, p=0.02, dmin=0.1 * ms, dmax=5 * ms, sizes=None):
     """Many independent CUBA networks laid out back to back in one group.
 
-    Sizes are round(exp(U[ln lo, ln hi])); network k occupies neurons
-    [off[k], off[k] + N[k]), its first int(0.8 N) neurons excitatory.  The
-    connectivity is described by blocks for the device generator (every pair
-    inside a network present with probability p, delays uniform in
-    [dmin, dmax]); exc blocks connect onto ge, inh blocks onto gi.
+    Sizes are round(exp(U[ln lo, ln hi])) (or the given sizes); network k
+    occupies neurons [off[k], off[k] + N[k]), its first int(0.8 N) neurons
+    excitatory.  The connectivity is described by blocks for the device
+    generator (every pair inside a network present with probability p,
+    delays uniform in [dmin, dmax]); exc blocks connect onto ge, inh blocks
+    onto gi.
     """
     rng = np.random.Generator(np.random.PCG64(seed))
     N = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), nnet))).astype(np.int64)
+    if sizes is not None:
+        N = np.asarray(sizes, dtype=np.int64)
+        nnet = len(N)
     Ne = (0.8 * N).astype(np.int64)
     off = np.zeros(nnet + 1, dtype=np.int64)
     np.cumsum(N, out=off[1:])

@@ -68,6 +68,60 @@ def test_persist_queue_overflow_equals_windowed(queue):
         np.testing.assert_array_equal(pst[v], wst[v])
 
 
+@pytest.mark.parametrize("prec", [32, 64])
+def test_persist_shape_boundaries(prec):
+    """Networks at the edges of the three workgroup shapes (20 neurons per
+    thread in FP32, 8 in FP64: 256, 512, 1024 threads), networks smaller
+    than a wavefront, and empty runs between the chunks."""
+    K = 20 if prec == 32 else 8
+    sizes = [1024 * K, 512 * K + 1, 512 * K, 256 * K + 1, 256 * K, 1, 7, 63, 64, 65, 256, 257, 1000]
+    d = W.uneven_batch(seed=21, sizes=sizes)
+    chunks = (1, 0, 523, 0, 876)
+    (ws, wi), wst = _run(d, prec, False, chunks)
+    (ps, pi), pst = _run(d, prec, True, chunks, persist_chunk=300)
+    print(f"FP{prec}: {len(ws)} spikes windowed, {len(ps)} persistent", flush=True)
+    assert len(ws) > 1000
+    np.testing.assert_array_equal(ps, ws)
+    np.testing.assert_array_equal(pi, wi)
+    for v in range(3):
+        np.testing.assert_array_equal(pst[v], wst[v])
+
+
+def test_persist_raster_overflow_reported():
+    """A raster too small for the run is reported as an overflow, as in the
+    windowed path (the persistent kernel stages entries per workgroup and
+    appends them in blocks)."""
+    # ~100k neurons at ~4 Hz for 0.3 s: ~130k spikes, far beyond the
+    # smallest raster (32 regions of at least 1024 entries)
+    d = W.uneven_batch(nnet=12, seed=4, lo=6000, hi=12000)
+    for persist in (False, True):
+        h, g, _, _ = device_uneven(d, precision=32, capacity=64)
+        if persist:
+            h.set_tuning("persist", 1)
+        h.run(3000)
+        h.sync()
+        with pytest.raises(Exception, match="overflow"):
+            h.get_spikes(g)
+        h.close()
+
+
+def test_persist_many_small_networks():
+    """More networks than a windowed sweep has workgroups (300 networks of
+    10-25 neurons, about 4,800 neurons: 19 workgroups of 256, fewer than the
+    32 raster regions): every region a persistent workgroup appends to
+    exists."""
+    d = W.uneven_batch(nnet=300, seed=8, lo=10, hi=25, p=0.2)
+    chunks = (700, 700)
+    (ws, wi), wst = _run(d, 32, False, chunks)
+    (ps, pi), pst = _run(d, 32, True, chunks)
+    print(f"{len(ws)} spikes windowed, {len(ps)} persistent", flush=True)
+    assert len(ws) > 1000
+    np.testing.assert_array_equal(ps, ws)
+    np.testing.assert_array_equal(pi, wi)
+    for v in range(3):
+        np.testing.assert_array_equal(pst[v], wst[v])
+
+
 def test_persist_fp64_matches_oracle():
     d = W.uneven_batch(nnet=8, seed=5, lo=500, hi=2000)
     h, g, _, _ = device_uneven(d, precision=64)
