@@ -156,13 +156,16 @@ __device__ inline void persist_raster_flush(const GroupDev &g, PersistRaster &pr
 // group) in lane j < ns of every wavefront: queue entries for their
 // synapses, shared by the NW wavefronts, and the raster entries (wavefront
 // 0).  Per connection, lane j looks up spike j's row, an inclusive scan over
-// lanes lays the rows' synapses end to end, and wavefront wv takes passes
-// wv, wv + NW, ... of 64*PL synapses, staging each pass's (column, delay)
+// lanes lays the rows' synapses end to end in passes of 64*PL, and the
+// passes of the step (numbered on from npass over connections and calls)
+// are dealt round-robin to the wavefronts, so that no wavefront takes the
+// first pass of every connection; each pass stages its (column, delay)
 // pairs before queueing them.
 template <int NW>
 __device__ __attribute__((always_inline)) inline void persist_emit(const GroupDev &g, int32_t mine, int ns, int64_t n,
                                                                  int sb, int lane, int wv, int64_t off,
-                                                                 const PersistQueue &pq, PersistRaster &pr) {
+                                                                 const PersistQueue &pq, PersistRaster &pr,
+                                                                 int &npass) {
   if (g.raster != nullptr && wv == 0) {
     if (pr.cnt + ns > PERSIST_RLOC) persist_raster_flush(g, pr, lane);
     if (lane < ns) pr.buf[pr.cnt + lane] = ((uint64_t)n << 32) | (uint64_t)(uint32_t)mine;
@@ -194,7 +197,10 @@ __device__ __attribute__((always_inline)) inline void persist_emit(const GroupDe
     // the network's queues as one resource: D*Q entries, 32-bit offsets
     const __amdgpu_buffer_rsrc_t qrs = persist_rsrc(pq.base, (int64_t)od.D * pq.Q * 4);
     const int64_t loc0 = od.tgt_off - off;  // target column -> neuron within the network
-    for (int base = wv * 64 * PL; base < total; base += NW * 64 * PL) {
+    // this wavefront's passes of the connection: j = j0, j0 + NW, ...
+    const int j0 = (wv - npass % NW + NW) % NW;
+    npass += (total + 64 * PL - 1) / (64 * PL);
+    for (int base = j0 * 64 * PL; base < total; base += NW * 64 * PL) {
       int32_t cols[PL];
       int dls[PL];
 #pragma unroll
@@ -409,6 +415,7 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
     }
     const int S = __shfl(incw, NW - 1);
     const int exw = incw - cw;
+    int npass = 0;  // delivery passes dealt out so far this step
 #pragma unroll 1
     for (int c0 = 0; c0 < S; c0 += 64) {
       // spike c0 + lane (a lane past the end searches for the last spike):
@@ -443,7 +450,7 @@ __device__ __attribute__((always_inline)) inline void persist_network(const Grou
         }
         mine = (int32_t)(off + k * PBLOCK + w * 64 + pos);
       }
-      persist_emit<NW>(g, mine, min(64, S - c0), n, sb, lane, wv, off, pq, pr);
+      persist_emit<NW>(g, mine, min(64, S - c0), n, sb, lane, wv, off, pq, pr, npass);
     }
     // ---- slot n's counts into sh_cnt
     if constexpr (LEAN_SW > 0) {
