@@ -66,7 +66,7 @@
 
 namespace bm {
 
-constexpr int PL = 4;   // synapses per lane staged per delivery pass
+constexpr int PL = 2;   // synapses per lane staged per delivery pass
 
 // Ring slices and last spike steps go through buffer instructions: a
 // resource per slot (base at the slot's first neuron, size clamped to the
