@@ -914,11 +914,12 @@ def cpu_baseline_fit(budget_s):
 
 
 def other_configs(args):
-    """Short legs of the batched-STDP and uneven-batch configurations after
-    the headline, each with its FP64 check, so that one default run records
-    all three network configurations.  Each leg is self-contained (its own
-    handles, closed before the next); a leg that fails is reported, not
-    raised."""
+    """Short legs of the batched-STDP and uneven-batch configurations (each
+    with its FP64 check) and of model fitting (2048 targets, 2000 candidates
+    each, 2 s of input: plain PSO against the learned proposal) after the
+    headline, so that one default run records every configuration.  Each
+    leg is self-contained (its own handles, closed before the next); a leg
+    that fails is reported, not raised."""
     out = {}
     legs = (
         ("batched_stdp", bench_stdp, dict(steps=200000, warmup=0, no_cpu=True, no_fp64=False, tuning=None,
@@ -926,14 +927,20 @@ def other_configs(args):
         ("uneven_batch", bench_uneven, dict(steps=5000, warmup=0, no_cpu=True, no_fp64=False, window=0, npt=0,
                                             no_autotune=False,
                                             tuning={"ring_bits": 0, "ring_pack": 1, "persist": 1})),
+        ("model_fitting", bench_fit, dict(targets=2048, popsize=2000, steps=20000, plain_iters=6, learned_iters=2,
+                                          train_pairs=400000, epochs=6, no_cpu=True)),
     )
     for name, fn, kw in legs:
         t0 = time.perf_counter()
         try:
             r = fn(argparse.Namespace(**kw))
             keep = ("value", "unit", "ms_per_step", "steps", "config", "fp64_check", "fp32_over_fp64_plain",
-                    "synaptic_events_per_s", "neuron_updates_per_s")
+                    "synaptic_events_per_s", "neuron_updates_per_s", "candidate_steps_per_s",
+                    "learned_over_plain_candidate_seconds", "learned_over_plain_incl_training")
             leg = {k: r[k] for k in keep if k in r}
+            for part in ("plain", "learned"):
+                if part in r:
+                    leg[part] = {k: v for k, v in r[part].items() if k != "history"}
             leg["roofline"] = {k: r["roofline"].get(k) for k in ("bound", "achieved", "peak", "unit", "frac",
                                                                   "traffic", "kernel", "avg_launch_us")}
         except Exception as e:  # reported in the line rather than hiding the headline
@@ -969,7 +976,7 @@ def main():
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="only run the byte-counter calibration copy of this many bytes")
     ap.add_argument("--no-other-configs", action="store_true",
-                    help="cuba1m: skip the short batched-STDP and uneven-batch legs after the headline")
+                    help="cuba1m: skip the short batched-STDP, uneven-batch and model-fitting legs after the headline")
     args = ap.parse_args()
     if args.probe_copy:
         print(json.dumps(probe_copy(args.probe_copy)))
