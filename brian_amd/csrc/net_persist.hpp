@@ -32,18 +32,20 @@
 // holds every pending delivery, as the windowed kernel expects.  Counts are
 // order-independent sums, so rasters and states equal the windowed kernel's.
 //
-// Step n of the loop, in the order of k_lean:
-//   tail of n-1: the counts of slot (n-1) mod D, summed in sh_cnt during
-//     step n-1, are delivered (and zeroed), and a neuron that spiked at n-1
-//     is reset (skipped at the first step of a run: the previous run applied
-//     it; applied once more after the last step);
+// Step n of the loop:
 //   queue:       the fill count of slot n mod D and its first entries are
-//     loaded (the loads have the head and the delivery to arrive);
-//   head of n:   update and threshold; the threshold ballots go to LDS;
-//   LDS barrier: every tail has read sh_cnt, and every thread has read the
-//     slot's fill count and dirty bit;
-//   delivery:    each wavefront appends its spikes to the raster and queues
-//     their synapses into slots (n + d) mod D, d >= 1;
+//     loaded (straight into LDS; they have the rest of the step to arrive);
+//   tail of n-1 and head of n, slot by slot (k_lean's order for each
+//     neuron): the counts of slot (n-1) mod D, summed in sh_cnt during step
+//     n-1, are delivered (and zeroed) and a neuron that spiked at n-1 is
+//     reset (the tail is skipped at the first step of a run: the previous
+//     run applied it, and applied once more after the last step); then
+//     update and threshold, a slot's ballot stored when it has spikes;
+//   LDS barrier: every tail has read sh_cnt, every ballot is stored, and
+//     every thread has read the slot's fill count and dirty bit;
+//   delivery:    the workgroup's spikes, 64 at a time, are appended to the
+//     raster staging (wavefront 0) and their synapses queued into slots
+//     (n + d) mod D, d >= 1, in passes shared out over the wavefronts;
 //   sum:         slot n's entries (and its ring words if dirty) are added
 //     into sh_cnt by LDS atomics; its fill count and dirty bit are cleared;
 //   barrier.
