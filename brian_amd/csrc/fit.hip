@@ -27,6 +27,37 @@ namespace bm {
 
 static const char *kFitTemplate = R"KERNEL(
 typedef %SCALAR% T;
+// FP32 division by a divisor d fixed for the run, with y = 1/d correctly
+// rounded once per candidate: q0 = a*y, then two remainder corrections
+// q += (a - q*d)*y with the remainder exact in a fused multiply-add.  The
+// first correction leaves q within one ulp of a/d; from there one more gives
+// the correctly rounded quotient (Markstein, IBM J. Res. Dev. 34 (1990)
+// 111-119), i.e. the bits of a / d, provided nothing under- or overflows:
+// |d| in [2^-20, 2^20] and |a| in [2^-100, 2^101), a biased exponent of a in
+// [lo, lo + 200] with lo = 27 (lo = 4096, never met, when d is out of
+// range).  Zeros, subnormals, infinities, NaNs and everything else outside
+// take the division itself, in a branch the wavefront enters only when one
+// of its lanes needs it.
+__device__ __forceinline__ unsigned bm_divisor_lo(float d) {
+  const float ad = __builtin_fabsf(d);
+  return ad >= 0x1p-20f && ad <= 0x1p20f ? 27u : 4096u;
+}
+__device__ __forceinline__ float bm_qdiv(float a, float d, float y, unsigned lo) {
+  float q = a * y;
+  float r = __builtin_fmaf(-q, d, a);
+  q = __builtin_fmaf(r, y, q);
+  r = __builtin_fmaf(-q, d, a);
+  q = __builtin_fmaf(r, y, q);
+  const unsigned e = (__builtin_bit_cast(unsigned, a) >> 23) & 0xffu;
+  const bool in = e - lo <= 200u;
+  if (__ballot(!in) != 0ull) {
+    // an empty statement the compiler may not move: it keeps the division
+    // inside the branch rather than computing it on every call
+    asm volatile("" ::: "memory");
+    if (!in) q = a / d;
+  }
+  return q;
+}
 extern "C" __global__ __launch_bounds__(256) void fit_runsim(
     const T *__restrict__ I_arr, long long I_stride, const int *__restrict__ cand_target,
     T *state, const T *__restrict__ params, int *spikecount, int *ncoinc,

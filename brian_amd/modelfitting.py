@@ -35,12 +35,46 @@ _CFUNCS = {"exp": "exp", "log": "log", "sqrt": "sqrt", "abs": "fabs", "sin": "si
 
 
 class _CGen:
-    """Python expression -> C expression with the same evaluation order."""
+    """Python expression -> C expression with the same evaluation order.
 
-    def __init__(self, names, namespace, precision):
+    With ``fast_divide`` (FP32 only) a division whose divisor depends on
+    nothing that changes during a run (parameters, constants) becomes
+    ``bm_qdiv(a, d, r, lo)`` with the divisor ``d``, its reciprocal ``r`` and
+    a range bound ``lo`` computed once per candidate before the time loop
+    (``self.hoisted``): the kernel then divides with one multiply and four
+    fused multiply-adds, which gives the correctly rounded quotient, the bits
+    of ``a / d``, over the operand range ``bm_qdiv`` checks (the division
+    itself outside it).
+    """
+
+    def __init__(self, names, namespace, precision, invariant=(), fast_divide=False):
         self.names = set(names)
         self.ns = namespace
         self.T = "double" if precision == 64 else "float"
+        self.invariant = set(invariant)
+        self.fast_divide = fast_divide and precision == 32
+        self.hoisted = []
+        self._hoist_ids = {}
+
+    def _is_invariant(self, node):
+        """True when the expression reads only parameters and constants."""
+        for n in ast.walk(node):
+            if isinstance(n, ast.Name) and n.id not in self.invariant and n.id in self.names | {"t"}:
+                return False
+        return True
+
+    def _divisor(self, node, subst):
+        """Names of the hoisted divisor, its reciprocal and the lowest biased
+        exponent of a dividend the fast form takes (bm_qdiv)."""
+        c = self.expr(node, subst)
+        if c not in self._hoist_ids:
+            k = len(self._hoist_ids)
+            self._hoist_ids[c] = k
+            self.hoisted += [f"  const {self.T} bm_div{k} = {c};",
+                             f"  const {self.T} bm_rcp{k} = (({self.T})1.0) / bm_div{k};",
+                             f"  const unsigned bm_dlo{k} = bm_divisor_lo(bm_div{k});"]
+        k = self._hoist_ids[c]
+        return f"bm_div{k}", f"bm_rcp{k}", f"bm_dlo{k}"
 
     def lit(self, v):
         return f"(({self.T}){float(v)!r})"
@@ -82,6 +116,9 @@ class _CGen:
                 if e == -1.0:
                     return f"({self.lit(1.0)}/{a})"
                 return f"pow({a},{self.expr(node.right, subst)})"
+            if isinstance(node.op, ast.Div) and self.fast_divide and self._is_invariant(node.right):
+                d, r, ok = self._divisor(node.right, subst)
+                return f"bm_qdiv({a},{d},{r},{ok})"
             b = self.expr(node.right, subst)
             op = {ast.Add: "+", ast.Sub: "-", ast.Mult: "*", ast.Div: "/"}.get(type(node.op))
             if op is None:
@@ -100,7 +137,7 @@ class _CGen:
         raise ValueError(f"unsupported expression element {ast.dump(node)[:60]}")
 
 
-def fit_kernel_source(eqs, threshold, reset, input_var="I", method="Euler", precision=32):
+def fit_kernel_source(eqs, threshold, reset, input_var="I", method="Euler", precision=32, fast_divide=True):
     """Kernel snippets for ``bm_fit_create`` and the variable/parameter lists.
 
     State variables are the differential equations' variables; parameters are
@@ -108,12 +145,16 @@ def fit_kernel_source(eqs, threshold, reset, input_var="I", method="Euler", prec
     a constant (first variable > value) or an expression string; ``reset`` a
     constant (first variable = value) or statements separated by ';' or
     newlines (first statement = RESET1, the others = RESET2).
+    ``fast_divide`` (FP32): divisions by run-invariant expressions use a
+    reciprocal computed once per candidate and two remainder corrections,
+    which give the bits of the division itself (``_CGen``).
     """
     names = eqs.variables()
     state = [n for n in names if n in eqs._diffeq]
     params = [n for n in names if n not in eqs._diffeq and n != input_var]
     T = "double" if precision == 64 else "float"
-    g = _CGen(state + params + [input_var], eqs._namespace, precision)
+    g = _CGen(state + params + [input_var], eqs._namespace, precision, invariant=params,
+              fast_divide=fast_divide)
     load = "\n".join([f"  {T} {v} = state[{k} * ncand + c];" for k, v in enumerate(state)] +
                      [f"  const {T} {p} = params[{k} * ncand + c];" for k, p in enumerate(params)])
     store = "\n".join(f"  state[{k} * ncand + c] = {v};" for k, v in enumerate(state))
@@ -169,6 +210,8 @@ def fit_kernel_source(eqs, threshold, reset, input_var="I", method="Euler", prec
             raise ValueError(f"unsupported reset statement {s!r}")
     reset1 = cst[0] if cst else ""
     reset2 = "\n".join(cst[1:])
+    if g.hoisted:
+        load += "\n" + "\n".join(g.hoisted)
     return {"load": load, "update": update, "threshold": th, "reset1": reset1, "reset2": reset2,
             "store": store, "state": state, "params": params}
 
@@ -200,9 +243,9 @@ class FitEngine:
 
     def __init__(self, eqs, threshold, reset, I, targets, cand_target, dt, delta_steps,
                  onset_steps=0, input_var="I", method="Euler", precision=32, features=False,
-                 record=0):
+                 record=0, fast_divide=True):
         self._lib = _lib.load()
-        src = fit_kernel_source(eqs, threshold, reset, input_var, method, precision)
+        src = fit_kernel_source(eqs, threshold, reset, input_var, method, precision, fast_divide)
         self.state_names, self.param_names = src["state"], src["params"]
         ou = I if isinstance(I, OUInput) else None
         if ou is not None:

@@ -88,6 +88,39 @@ def test_fit_kernel_fp64_matches_oracle():
     assert abs(ns32.sum() - ns.sum()) / ns.sum() < 0.01
 
 
+def test_fp32_fast_division_equals_division():
+    """FP32 kernels divide by run-invariant divisors (tau, the constant
+    tau_t) with a reciprocal computed once and two remainder corrections:
+    spike counts, coincidences and the final state must equal those of the
+    same kernel with plain IEEE division bit for bit.  Vt starts at 0 and
+    returns to 0 only through underflow, so the per-call range check sends
+    the zero dividends to the division and the rest to the fast form."""
+    ntarget, pop, steps = 16, 512, 20000
+    I = W.ou_current(ntarget, steps, seed=13)
+    hidden = W.fit_hidden_params(ntarget, seed=14)
+    trains = oracle_trains(hidden, spike_steps(hidden["refractory"], DT), I, steps)
+    rng = np.random.default_rng(15)
+    cand_target = np.repeat(np.arange(ntarget), pop)
+    M = ntarget * pop
+    params = {k: W.log_uniform(rng, lo, hi, M) for k, (lo, hi) in W.FIT_BOUNDS.items()}
+    refr = spike_steps(params.pop("refractory"), DT)
+    out = []
+    for fast in (True, False):
+        eng = FitEngine(fit_eqs(), W.FIT_THRESHOLD, W.FIT_RESET, I, trains, cand_target, DT, 40,
+                        precision=32, fast_divide=fast)
+        assert ("bm_qdiv" in eng.src["update"]) == fast
+        eng.set_candidates(params, init_state={"V": 0.0, "Vt": 0.0}, refractory_steps=refr)
+        eng.run(0, steps, stepsize=7000)
+        nc, ns = eng.results()
+        out.append((nc, ns, eng.state()))
+        eng.close()
+    (nc1, ns1, s1), (nc0, ns0, s0) = out
+    assert ns0.sum() > 1000 and nc0.sum() > 100
+    np.testing.assert_array_equal(ns1, ns0)
+    np.testing.assert_array_equal(nc1, nc0)
+    np.testing.assert_array_equal(s1.view(np.uint64), s0.view(np.uint64))
+
+
 def test_modelfitting_recovers_hidden_neuron():
     ntarget, steps = 3, 10000
     I = W.ou_current(ntarget, steps, seed=7)
