@@ -101,20 +101,34 @@ extern "C" __global__ __launch_bounds__(256) void fit_runsim(
 %REC_SPIKE%
     }
     const int Tspike = T_ + spikedelay;
-    bool near_last = last_spike_time + delta >= Tspike;
-    bool near_next = next_spike_time - delta <= Tspike;
-    near_last = near_last && has_spiked;
-    near_next = near_next && has_spiked;
-    nc += ((near_last && last_spike_allowed) || (near_next && next_spike_allowed)) ? 1 : 0;
-    const bool near_both = (near_last && last_spike_allowed) && (near_next && next_spike_allowed);
-    last_spike_allowed = last_spike_allowed && !near_last;
-    next_spike_allowed = (next_spike_allowed && !near_next) || near_both;
-    if (Tspike >= next_spike_time) {
-      ++idx;
-      last_spike_time = next_spike_time;
-      next_spike_time = train[idx + 1];
-      last_spike_allowed = next_spike_allowed;
-      next_spike_allowed = true;
+    // The coincidence state changes only in a step where the candidate
+    // spiked, and the train advances only when its next spike is reached:
+    // both are rare, so each runs inside a branch the wavefront enters only
+    // when one of its lanes needs it (with the empty statement keeping the
+    // compiler from computing the branch's body on every step); a lane with
+    // nothing to do leaves its state as the expressions would.
+    if (__ballot(has_spiked) != 0ull) {
+      asm volatile("" ::: "memory");
+      bool near_last = last_spike_time + delta >= Tspike;
+      bool near_next = next_spike_time - delta <= Tspike;
+      near_last = near_last && has_spiked;
+      near_next = near_next && has_spiked;
+      nc += ((near_last && last_spike_allowed) || (near_next && next_spike_allowed)) ? 1 : 0;
+      const bool near_both = (near_last && last_spike_allowed) && (near_next && next_spike_allowed);
+      last_spike_allowed = last_spike_allowed && !near_last;
+      next_spike_allowed = (next_spike_allowed && !near_next) || near_both;
+    }
+    if (__ballot(Tspike >= next_spike_time) != 0ull) {
+      asm volatile("" ::: "memory");
+      // the padded train ends with a sentinel no step reaches, so an
+      // advancing lane reads at most that sentinel
+      const bool adv = Tspike >= next_spike_time;
+      const int following = train[idx + (adv ? 2 : 1)];
+      idx += adv ? 1 : 0;
+      last_spike_time = adv ? next_spike_time : last_spike_time;
+      next_spike_time = adv ? following : next_spike_time;
+      last_spike_allowed = adv ? next_spike_allowed : last_spike_allowed;
+      next_spike_allowed = adv || next_spike_allowed;
     }
   }
 %STORE%
