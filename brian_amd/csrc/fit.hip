@@ -27,6 +27,7 @@ namespace bm {
 
 static const char *kFitTemplate = R"KERNEL(
 typedef %SCALAR% T;
+#define FIT_BLOCK 8
 // FP32 division by a divisor d fixed for the run, with y = 1/d correctly
 // rounded once per candidate: q0 = a*y, then two remainder corrections
 // q += (a - q*d)*y with the remainder exact in a fused multiply-add.  The
@@ -76,47 +77,59 @@ extern "C" __global__ __launch_bounds__(256) void fit_runsim(
   int idx = st_index[c];
   int last_spike_time = train[idx];
   int next_spike_time = train[idx + 1];
-  bool last_spike_allowed = (allowed[c] & 1) != 0;
-  bool next_spike_allowed = (allowed[c] & 2) != 0;
+  // bit 0: last_spike_allowed, bit 1: next_spike_allowed (an integer kept in
+  // a vector register rather than two lane masks)
+  int allow = allowed[c] & 3;
   int nc = ncoinc[c], ns = spikecount[c];
   const int spikedelay = delay_arr[c];
   const int refractory = refr_arr[c];
   int next_allowed = next_allowed_arr[c];
 %FEAT_LOAD%
-  for (int T_ = Tstart; T_ < Tend; ++T_) {
+  // steps in blocks of FIT_BLOCK whose inputs are loaded together, so that
+  // a load's latency is waited for once per block rather than at every step
+  for (int T0 = Tstart; T0 < Tend; T0 += FIT_BLOCK) {
+  const int nb = Tend - T0 < FIT_BLOCK ? Tend - T0 : FIT_BLOCK;
+  T in_blk[FIT_BLOCK];
+#pragma unroll
+  for (int u = 0; u < FIT_BLOCK; ++u) in_blk[u] = I_t[T0 + (u < nb ? u : 0)];
+#pragma unroll
+  for (int u = 0; u < FIT_BLOCK; ++u) {
+    if (u >= nb) break;
+    const int T_ = T0 + u;
     const T t = (T)T_ * dt;
     (void)t;
-    const T %INPUT% = I_t[T_];
+    const T %INPUT% = in_blk[u];
 %UPDATE%
     const bool is_refractory = (T_ <= next_allowed);
     const bool has_spiked = (%THRESHOLD%) && !is_refractory;
-    ns += (has_spiked && (T_ >= onset)) ? 1 : 0;
     if (has_spiked || is_refractory) {
 %RESET1%
     }
-    if (has_spiked) {
-%RESET2%
-      next_allowed = T_ + refractory;
-%FEAT_SPIKE%
-%REC_SPIKE%
-    }
     const int Tspike = T_ + spikedelay;
-    // The coincidence state changes only in a step where the candidate
-    // spiked, and the train advances only when its next spike is reached:
-    // both are rare, so each runs inside a branch the wavefront enters only
-    // when one of its lanes needs it (with the empty statement keeping the
-    // compiler from computing the branch's body on every step); a lane with
-    // nothing to do leaves its state as the expressions would.
+    // What a spike changes (the spike count, the second reset statements,
+    // the refractory deadline, the coincidence state) and the advance along
+    // the target train, once its next spike is reached, are rare: each runs
+    // inside a branch the wavefront enters only when one of its lanes needs
+    // it (with the empty statement keeping the compiler from computing the
+    // branch's body on every step); a lane with nothing to do leaves its
+    // state as the expressions would.
     if (__ballot(has_spiked) != 0ull) {
       asm volatile("" ::: "memory");
-      bool near_last = last_spike_time + delta >= Tspike;
-      bool near_next = next_spike_time - delta <= Tspike;
-      near_last = near_last && has_spiked;
-      near_next = near_next && has_spiked;
-      nc += ((near_last && last_spike_allowed) || (near_next && next_spike_allowed)) ? 1 : 0;
-      const bool near_both = (near_last && last_spike_allowed) && (near_next && next_spike_allowed);
-      last_spike_allowed = last_spike_allowed && !near_last;
-      next_spike_allowed = (next_spike_allowed && !near_next) || near_both;
+      ns += (has_spiked && (T_ >= onset)) ? 1 : 0;
+      if (has_spiked) {
+%RESET2%
+        next_allowed = T_ + refractory;
+%FEAT_SPIKE%
+%REC_SPIKE%
+      }
+      const int near_last = (has_spiked && last_spike_time + delta >= Tspike) ? 1 : 0;
+      const int near_next = (has_spiked && next_spike_time - delta <= Tspike) ? 2 : 0;
+      // bit 0: near_last && last_spike_allowed, bit 1: near_next && next_spike_allowed
+      const int hit = allow & (near_last | near_next);
+      nc += hit != 0 ? 1 : 0;
+      // last_spike_allowed &= !near_last;
+      // next_spike_allowed = (next_spike_allowed && !near_next) || both hits
+      allow = (allow & ~(near_last | near_next)) | (hit == 3 ? 2 : 0);
     }
     if (__ballot(Tspike >= next_spike_time) != 0ull) {
       asm volatile("" ::: "memory");
@@ -127,14 +140,15 @@ extern "C" __global__ __launch_bounds__(256) void fit_runsim(
       idx += adv ? 1 : 0;
       last_spike_time = adv ? next_spike_time : last_spike_time;
       next_spike_time = adv ? following : next_spike_time;
-      last_spike_allowed = adv ? next_spike_allowed : last_spike_allowed;
-      next_spike_allowed = adv || next_spike_allowed;
+      // last_spike_allowed = next_spike_allowed; next_spike_allowed = true
+      allow = adv ? (((allow >> 1) & 1) | 2) : allow;
     }
+  }
   }
 %STORE%
 %FEAT_STORE%
   st_index[c] = idx;
-  allowed[c] = (unsigned char)((last_spike_allowed ? 1 : 0) | (next_spike_allowed ? 2 : 0));
+  allowed[c] = (unsigned char)allow;
   ncoinc[c] = nc;
   spikecount[c] = ns;
   next_allowed_arr[c] = next_allowed;
