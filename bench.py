@@ -651,8 +651,9 @@ def bench_stdp(args):
                            "weights and traces in registers)",
                     steps_per_launch=steps_per_launch, avg_launch_us=avg_launch_s * 1e6,
                     alg_bytes_per_launch=alg_bytes, launches_timed=launches,
-                    note="register-resident weights: HBM is not the bound; the per-step input "
-                         "generation and trace updates on the vector ALUs are")
+                    note="register-resident weights: HBM is not the bound; the latency of each "
+                         "step's chain (hashes, ballot, LDS spike lists, barrier, fold, neuron "
+                         "update) at four wavefronts per SIMD is")
     check = None
     if 64 in res:
         bins = np.linspace(0, d["gmax"], 21)

@@ -98,12 +98,19 @@ __global__ __launch_bounds__(64 * NW) void k_stdp_batch(const StdpBatchDev d, in
   const T dp = (T)d.decay_pre, dm = (T)d.decay_post, Ap = (T)d.Ap, Am = (T)d.Am;
   const T wmin = (T)d.wmin, wmax = (T)d.wmax, vt = (T)d.vt, vr = (T)d.vr;
 
+  // The inputs' spikes of a step depend on nothing else in the step, so each
+  // step computes the next step's while the other wavefronts' spike lists
+  // are read from LDS (the hashes fill that wait instead of lengthening the
+  // chain from the barrier to the neuron update).
+  bool spk_next[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) spk_next[j] = valid[j] && (uint64_t)weyl_u32(key[j], (uint32_t)n0) < d.thr;
   for (int st = 0; st < nsteps; ++st) {
     const uint32_t n = (uint32_t)(n0 + st);
     // ---- groups: inputs, neuron state update + threshold, trace decay
     bool spk[NJ];
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) spk[j] = valid[j] && (uint64_t)weyl_u32(key[j], n) < d.thr;
+    for (int j = 0; j < NJ; ++j) spk[j] = spk_next[j];
     {
       T tmp[M];
 #pragma unroll
@@ -139,6 +146,8 @@ __global__ __launch_bounds__(64 * NW) void k_stdp_batch(const StdpBatchDev d, in
           x = x + __shfl(rw[j], l);
         }
       }
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) spk_next[j] = valid[j] && (uint64_t)weyl_u32(key[j], n + 1u) < d.thr;
     } else {
       const int b = st & 1;
       T *list = sh_list + (b * NW + wv) * 64 * NJ;
@@ -153,9 +162,14 @@ __global__ __launch_bounds__(64 * NW) void k_stdp_batch(const StdpBatchDev d, in
       }
       if (lane == 0) sh_cnt[b * NW + wv] = cnt;
       __syncthreads();
+      int cnts[NW];
+#pragma unroll
+      for (int v = 0; v < NW; ++v) cnts[v] = sh_cnt[b * NW + v];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) spk_next[j] = valid[j] && (uint64_t)weyl_u32(key[j], n + 1u) < d.thr;
 #pragma unroll
       for (int v = 0; v < NW; ++v) {
-        const int c = __builtin_amdgcn_readfirstlane(sh_cnt[b * NW + v]);
+        const int c = __builtin_amdgcn_readfirstlane(cnts[v]);
         const T *lv = sh_list + (b * NW + v) * 64 * NJ;
         for (int k = 0; k < c; ++k) x = x + lv[k];
       }

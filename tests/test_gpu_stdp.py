@@ -40,10 +40,10 @@ def make_batch(precision, ninst, nin, seed, wseed=5, rate=RATE):
     return h, w0
 
 
-def oracle_batch(ninst, nin, seed, w0):
+def oracle_batch(ninst, nin, seed, w0, rate=RATE):
     A, C, dp, dm, Ap, Am = stdp_consts()
     net = O.Net(DT)
-    sid = net.add_group(O.SpikeSource(ninst * nin, poisson=(np.full(ninst * nin, RATE), seed, DT, "weyl")),
+    sid = net.add_group(O.SpikeSource(ninst * nin, poisson=(np.full(ninst * nin, rate), seed, DT, "weyl")),
                         record=False)
     S0 = np.zeros((2, ninst))
     S0[0] = vr
@@ -72,6 +72,25 @@ def test_stdp_batch_fp64_matches_oracle():
     st = h.state()
     np.testing.assert_allclose(st[:, 0], net.groups[gid].S[0], rtol=1e-12)
     assert np.all((h.weights() >= 0) & (h.weights() <= gmax))
+
+
+def test_stdp_batch_dense_input_lists_match_oracle():
+    """Inputs at 3 kHz: a wavefront of 4 x 64 inputs lists about 77 spikes
+    a step, beyond the 64 entries each wavefront's list is read with at once,
+    so the fold also takes its one-by-one path; FP64 against the oracle."""
+    ninst, nin, seed, steps, rate = 4, 1000, 91, 600, 3000.0
+    h, w0 = make_batch(64, ninst, nin, seed, rate=rate)
+    assert h.layout() == (4, 4)
+    h.run(steps, steps_per_launch=250)
+    net, gid, c = oracle_batch(ninst, nin, seed, w0, rate=rate)
+    net.run(steps)
+    s, i = h.spikes()
+    rs, ri = net.raster(gid)
+    assert len(rs) > 5
+    np.testing.assert_array_equal(s, rs)
+    np.testing.assert_array_equal(i, ri)
+    np.testing.assert_allclose(h.weights().reshape(-1), c.data, rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(h.state()[:, 0], net.groups[gid].S[0], rtol=1e-12)
 
 
 @pytest.mark.parametrize("nin", [1000, 300, 70])
