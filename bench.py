@@ -50,14 +50,15 @@ ACC_STEPS = 10000          # the accuracy leg always covers 1 s of simulated tim
 # Algorithmic bytes of one launch of the windowed sweep kernel (W steps, the
 # state held in registers for the whole window), per neuron:
 #   state read + write once per launch        2 * m * s   (m = 3 variables)
-#   last spike step read + write once         8
+#   last spike step read once                 4
 #   packed count ring: read + zero per step   2 * sw * W  (sw bytes per neuron)
-# per spike: row pointer pair 16 B + raster entry 8 B; per synaptic event:
-# column index 4 B + the 32-bit count word read and written by the atomic 8 B
-# (+ 1 B per-synapse delay for a DelayConnection).
+# per spike: row pointer pair 16 B + raster entry 8 B + the last spike step
+# written 4 B (only neurons that spiked in the window write it back); per
+# synaptic event: column index 4 B + the 32-bit count word read and written
+# by the atomic 8 B (+ 1 B per-synapse delay for a DelayConnection).
 STATE_VARS = 3
-BYTES_LAST = 8
-BYTES_SPIKE = 24
+BYTES_LAST = 4
+BYTES_SPIKE = 28
 BYTES_EVENT = 12
 
 
@@ -499,7 +500,7 @@ def bench_uneven(args):
     # per-synapse delay byte on every event
     if persist:
         steps_launch = r32["prof_steps"] / max(launches, 1)
-        alg_bytes = (d["N"] * persist_bytes_per_neuron(32, r32["ring_depth"], 4) + psp / launches * (BYTES_SPIKE + 4)
+        alg_bytes = (d["N"] * persist_bytes_per_neuron(32, r32["ring_depth"], 4) + psp / launches * BYTES_SPIKE
                      + pev / launches * BYTES_EVENT_QUEUE)
         roof = roofline(alg_bytes, avg_launch_s, hbm_traffic("persist_uneven_pmc"),
                         kernel="k_persist (one workgroup per network, on the smallest of 1024, 512 and 256 "

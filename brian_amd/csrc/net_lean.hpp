@@ -45,13 +45,14 @@ __global__ __launch_bounds__(BLOCK) void k_lean(const GroupDev g, const int64_t 
   const int nsteps = tail_only ? 1 : W;
 
   T s[K][M];
-  int32_t last[K];
+  int32_t last[K], last0[K];
   bool valid[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = blk0 + k * BLOCK + tid;
     valid[k] = i < N;
     last[k] = valid[k] ? g.last[i] : NEVER;
+    last0[k] = last[k];
 #pragma unroll
     for (int j = 0; j < M; ++j) s[k][j] = valid[k] ? static_cast<const T *>(g.S)[(int64_t)j * N + i] : (T)0;
   }
@@ -162,12 +163,13 @@ __global__ __launch_bounds__(BLOCK) void k_lean(const GroupDev g, const int64_t 
     propagate_block<T>(g, sh_list, nullptr, nsp, n0, blk0, tid);
   }
 
-  // ---------------- write back
+  // ---------------- write back (the last spike step only where it changed:
+  // a window's spikes touch a small fraction of the neurons)
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if (!valid[k]) continue;
     const int i = blk0 + k * BLOCK + tid;
-    g.last[i] = last[k];
+    if (last[k] != last0[k]) g.last[i] = last[k];
 #pragma unroll
     for (int j = 0; j < M; ++j) static_cast<T *>(g.S)[(int64_t)j * N + i] = s[k][j];
   }
