@@ -315,20 +315,37 @@ bool one_step(const bm_net *net) {
   return false;
 }
 
-bool fast_eligible(const HostGroup &g) {
+// the common shape every specialised kernel handles, refractoriness aside
+bool lean_shape(const HostGroup &g) {
   if (g.kind != GK_NEURON || g.upd != UPD_LINEAR || g.M < 1 || g.M > 4) return false;
   if (!g.mons.empty() || g.mod_var >= 0 || g.stp_conn >= 0 || !g.pins.empty() || g.coin) return false;
   if (!g.ssm_vars.empty()) return false;
-  if (g.refr != REFR_NONE || g.th_kind == TH_EMPIRICAL || g.th_kind == TH_POISSON || g.ti_nt > 0) return false;
+  if (g.th_kind == TH_EMPIRICAL || g.th_kind == TH_POISSON || g.ti_nt > 0) return false;
   if (g.ring_desc.size() > 1) return false;
   if (g.ring_desc.size() == 1 && g.ring_desc[0].kind != RING_COUNT) return false;
   return true;
 }
 
+bool fast_eligible(const HostGroup &g) { return lean_shape(g) && g.refr == REFR_NONE; }
+
+// Refractoriness with one period for every neuron and at most one reset
+// op, an assignment (Refractoriness clamps that variable: reset.py:311-332),
+// runs on the model-compiled lean kernel only (not on the table-driven,
+// deferred or persistent kernels).
+bool lean_refr_eligible(const bm_net *net, const HostGroup &g) {
+  if (!net->specialize || !net->lean_kernel || !lean_shape(g) || g.refr != REFR_CONST) return false;
+  return g.rs_var.empty() || (g.rs_var.size() == 1 && g.rs_op[0] == 0);
+}
+
+// the group runs a kernel compiled at run time (k_lean or the specialised k_sweep)
+bool jit_group(const bm_net *net, const HostGroup &g) {
+  return net->specialize && (fast_eligible(g) || lean_refr_eligible(net, g));
+}
+
 size_t sweep_shmem(bm_net *net, const HostGroup &g, int K, int W);
 
 int group_K(bm_net *net, const HostGroup &g) {
-  if (fast_eligible(g)) return net->K;
+  if (fast_eligible(g) || lean_refr_eligible(net, g)) return net->K;
   if (g.upd == UPD_PROGRAM) return 1;
   if (g.upd == UPD_PERNEURON && g.M > 3) return 1;
   // a captured value per listed spike: four neurons per thread only while
@@ -392,7 +409,7 @@ std::string lean_prelude(bm_net *net, const HostGroup &g, const char *kernel_hea
   std::ostringstream d;
   d << "#include \"net_kernels.hpp\"\nnamespace bm {\n"
     << "typedef " << (f32 ? "float" : "double") << " lean_t;\n#define LEAN_M " << M << "\n#define LEAN_SW " << sw
-    << "\n";
+    << "\n#define LEAN_REFR " << (g.refr == REFR_CONST ? 1 : 0) << "\n";
   // deliveries in the channel order (Connections, then DelayConnection flushes)
   d << "__device__ inline void lean_deliver(lean_t (&x)[LEAN_M], uint32_t word) {\n";
   const uint32_t mask = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
@@ -526,7 +543,7 @@ void ensure_jit(bm_net *net) {
   if (!net->specialize) return;
   for (auto &gp : net->groups) {
     HostGroup &g = *gp;
-    if (g.kind == GK_GENERATOR || !fast_eligible(g)) continue;
+    if (g.kind == GK_GENERATOR || !jit_group(net, g)) continue;
     const int K = group_K(net, g);
     if (g.jit_K == K && g.jit.fn && g.jit_lean == net->lean_kernel) continue;
     g.jit = jit_sweep(net, g, K);
@@ -759,7 +776,7 @@ void launch_group(bm_net *net, HostGroup &g, int64_t rel0, int W, int skip_first
   if (blocks == 0) return;
   const size_t shmem = sweep_shmem(net, g, K, W);
   const bool fast = fast_eligible(g);
-  if (fast && net->specialize) {
+  if (jit_group(net, g)) {
     BM_REQUIRE(g.jit_K == K && g.jit.fn, "specialised sweep kernel not prepared for this shape");
     void *args[] = {(void *)&g.dev, (void *)&dbase, (void *)&rel0, (void *)&W, (void *)&skip_first_tail,
                     (void *)&tail_only};

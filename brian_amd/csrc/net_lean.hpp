@@ -1,6 +1,7 @@
 // Lean sweep kernel compiled at run time (hipRTC) for one group of the
 // common shape: shared linear propagator, at most one packed count ring,
-// reset ops without refractoriness, constant or variable threshold.
+// reset ops, constant or variable threshold, and optionally
+// Refractoriness with one period for all neurons (LEAN_REFR).
 //
 // The source that includes this header first defines the group's model as
 // compile-time code (generated on the host, net.hip:lean_prelude):
@@ -10,6 +11,9 @@
 //   lean_reset(x)              the reset ops (brian/reset.py:139-147)
 //   lean_update(x)             S <- A.S + C with A and C as literals
 //   lean_spike(x)              the threshold condition (brian/threshold.py:160-191)
+//   LEAN_REFR                  1: Refractoriness (brian/reset.py:311-332), the
+//                              reset op held while step - last spike < period
+//                              and the threshold gated by last + refractory <= t
 // so that a step costs only the arithmetic of the model: zero entries of A
 // vanish in FP32, and there are no table lookups.
 //
@@ -23,6 +27,9 @@
 // window from those ballots.
 #pragma once
 #include "net_kernels.hpp"
+#ifndef LEAN_REFR
+#define LEAN_REFR 0
+#endif
 
 namespace bm {
 
@@ -46,6 +53,9 @@ __global__ __launch_bounds__(BLOCK) void k_lean(const GroupDev g, const int64_t 
 
   T s[K][M];
   int32_t last[K], last0[K];
+#if LEAN_REFR
+  int32_t lastw[K];  // last spike step, advanced step by step within the window
+#endif
   bool valid[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -53,6 +63,9 @@ __global__ __launch_bounds__(BLOCK) void k_lean(const GroupDev g, const int64_t 
     valid[k] = i < N;
     last[k] = valid[k] ? g.last[i] : NEVER;
     last0[k] = last[k];
+#if LEAN_REFR
+    lastw[k] = last[k];
+#endif
 #pragma unroll
     for (int j = 0; j < M; ++j) s[k][j] = valid[k] ? static_cast<const T *>(g.S)[(int64_t)j * N + i] : (T)0;
   }
@@ -106,14 +119,31 @@ __global__ __launch_bounds__(BLOCK) void k_lean(const GroupDev g, const int64_t 
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         lean_deliver(s[k], cw[w][k]);
+#if LEAN_REFR
+        // the tail of step n0 + w - 1: the reset holds for `period` steps
+        // from the spike's own step on
+        if (lastw[k] != NEVER && (n0 + w - 1) - (int64_t)lastw[k] < (int64_t)g.period) lean_reset(s[k]);
+#else
         if (prev[k]) lean_reset(s[k]);
+#endif
       }
     }
     if (tail_only) continue;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       lean_update(s[k]);
+#if LEAN_REFR
+      bool sp = valid[k] && lean_spike(s[k]);
+      if (sp && lastw[k] != NEVER) {
+        // next allowed spike time in seconds, as the generic sweep tests it
+        const double t = (double)(n0 + w) * g.dt + 0.0;
+        const double lt = (double)lastw[k] * g.dt + 0.0;
+        sp = lt + g.refr_time <= t;
+      }
+      if (sp) lastw[k] = (int32_t)(n0 + w);
+#else
       const bool sp = valid[k] && lean_spike(s[k]);
+#endif
       spk[w][k] = __ballot(sp);
       prev[k] = sp;
     }
