@@ -203,6 +203,39 @@ def cpp_reference(timeout_s=120):
             "synaptic_events_per_s": (int(ns.group(1)) * 80 / sec) if ns else None}
 
 
+def gpu_reference_workload(steps=25000, warmup=1000):
+    """The reference C++ CUBA program's own workload on the GPU, with Brian's
+    step semantics (dev/benchmarking/cuba/CUBA.cpp: N = 32000, Bernoulli
+    p = 80/N, we = 1.62 mV, wi = -9 mV, 5 ms refractoriness, no delay, dt
+    0.1 ms, 25000 steps), FP64 and FP32; one step per launch (no delay)."""
+    from brian_amd import workloads as W
+    from brian_amd._lib import NetHandle
+    from brian_amd.equations import Equations
+    n = 32000
+    d = W.cuba(n, seed=3, p=80.0 / n, delay_steps=0)
+    eqs = Equations(W.CUBA_EQS, El=W.El, taum=W.taum, taue=W.taue, taui=W.taui)
+    A, C = eqs.linear_propagator(1e-4)
+    out = {"workload": "CUBA.cpp's network: N=32000, Bernoulli p=80/N, 5 ms refractory, no delay, "
+                       f"dt 0.1 ms, {steps} steps after {warmup} warm-up steps"}
+    for prec in (64, 32):
+        h = NetHandle(1e-4, prec)
+        g = h.add_group(d["S0"])
+        h.set_linear(g, A, C)
+        h.set_threshold(g, 0, d["Vt"])
+        h.add_reset_op(g, 0, 0, d["Vr"])
+        h.set_refractory(g, int(5e-3 / 1e-4) + 1, 5e-3 - 0.5e-4)
+        h.add_connection(g, 0, d["Ne"], g, 0, n, 1, *d["exc"], delay_steps=0)
+        h.add_connection(g, d["Ne"], d["Ni"], g, 0, n, 2, *d["inh"], delay_steps=0)
+        h.record(g, 16 << 20)
+        h.prepare()
+        el = timed_run(h, steps, warmup)
+        nsp = h.spike_count(g)
+        h.close()
+        out[f"fp{prec}"] = {"neuron_updates_per_s": n * steps / el, "us_per_step": el / steps * 1e6,
+                            "mean_rate_hz": nsp / n / ((steps + warmup) * 1e-4)}
+    return out
+
+
 # ---------------------------------------------------------------- cuba1m
 def build_workload(seed=2):
     from brian_amd import workloads as W
@@ -404,6 +437,9 @@ def bench_cuba1m(args):
 
     cpu = None if args.no_cpu else cpu_baseline(d, args.cpu_budget)
     cpp = None if args.no_cpu else cpp_reference()
+    ref_wl = gpu_reference_workload()
+    if cpp:
+        ref_wl["fp64_over_reference_cpp"] = ref_wl["fp64"]["neuron_updates_per_s"] / cpp["value"]
 
     value = N * K / el32
     return {
@@ -428,6 +464,7 @@ def bench_cuba1m(args):
         "roofline": roof,
         "cpu_baseline": cpu,
         "cpu_reference_cpp": cpp,
+        "gpu_on_reference_cpp_workload": ref_wl,
     }
 
 
