@@ -863,7 +863,10 @@ void build_stdp(bm_net *net, HostStdp &s) {
 void prepare(bm_net *net, int window, int K, int use_graph) {
   BM_REQUIRE(!net->prepared, "already prepared");
   BM_REQUIRE(K == 0 || K == 1 || K == 2 || K == 4, "neurons_per_thread must be 0, 1, 2 or 4");
-  net->K = K == 0 ? 4 : K;
+  // automatic: one neuron per thread (on a 1M-neuron CUBA network with 11-step
+  // windows the lean kernel ran 3.54 us/step at 1, 3.81 at 4; the generic
+  // kernel 14.1 at 1 and 2, 26.9 at 4)
+  net->K = K == 0 ? 1 : K;
   net->use_graph = use_graph != 0;
   // ---- window
   int dmin = 1 << 30;

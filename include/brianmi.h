@@ -291,7 +291,7 @@ int bm_net_read_state_monitor(bm_net *net, int mid, double *out, int64_t max_sam
 
 /* Freeze the structure, allocate rings, choose the step window.
  * window 0 = automatic (min synaptic delay + 1, capped), else forced (>=1).
- * neurons_per_thread 0 = automatic. */
+ * neurons_per_thread 0 = automatic (one neuron per thread). */
 int bm_net_prepare(bm_net *net, int window, int neurons_per_thread, int use_graph);
 
 /* The group as nparts independent networks laid back to back: network p
