@@ -328,12 +328,13 @@ bool lean_shape(const HostGroup &g) {
 
 bool fast_eligible(const HostGroup &g) { return lean_shape(g) && g.refr == REFR_NONE; }
 
-// Refractoriness with one period for every neuron and at most one reset
-// op, an assignment (Refractoriness clamps that variable: reset.py:311-332),
-// runs on the model-compiled lean kernel only (not on the table-driven,
-// deferred or persistent kernels).
+// Refractoriness (one period for every neuron, or one per neuron) with at
+// most one reset op, an assignment (Refractoriness clamps that variable:
+// reset.py:311-332), runs on the model-compiled lean kernel only (not on
+// the table-driven, deferred or persistent kernels).
 bool lean_refr_eligible(const bm_net *net, const HostGroup &g) {
-  if (!net->specialize || !net->lean_kernel || !lean_shape(g) || g.refr != REFR_CONST) return false;
+  if (!net->specialize || !net->lean_kernel || !lean_shape(g)) return false;
+  if (g.refr != REFR_CONST && g.refr != REFR_ARRAY) return false;
   return g.rs_var.empty() || (g.rs_var.size() == 1 && g.rs_op[0] == 0);
 }
 
@@ -409,7 +410,7 @@ std::string lean_prelude(bm_net *net, const HostGroup &g, const char *kernel_hea
   std::ostringstream d;
   d << "#include \"net_kernels.hpp\"\nnamespace bm {\n"
     << "typedef " << (f32 ? "float" : "double") << " lean_t;\n#define LEAN_M " << M << "\n#define LEAN_SW " << sw
-    << "\n#define LEAN_REFR " << (g.refr == REFR_CONST ? 1 : 0) << "\n";
+    << "\n#define LEAN_REFR " << (g.refr == REFR_CONST ? 1 : (g.refr == REFR_ARRAY ? 2 : 0)) << "\n";
   // deliveries in the channel order (Connections, then DelayConnection flushes)
   d << "__device__ inline void lean_deliver(lean_t (&x)[LEAN_M], uint32_t word) {\n";
   const uint32_t mask = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);

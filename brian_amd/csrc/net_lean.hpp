@@ -13,7 +13,10 @@
 //   lean_spike(x)              the threshold condition (brian/threshold.py:160-191)
 //   LEAN_REFR                  1: Refractoriness (brian/reset.py:311-332), the
 //                              reset op held while step - last spike < period
-//                              and the threshold gated by last + refractory <= t
+//                              and the threshold gated by last + refractory <= t;
+//                              2: the same with a refractory time per neuron
+//                              (neurongroup.py:368-429), the hold also ending
+//                              once last + refractory <= t - dt/4
 // so that a step costs only the arithmetic of the model: zero entries of A
 // vanish in FP32, and there are no table lookups.
 //
@@ -56,6 +59,9 @@ __global__ __launch_bounds__(BLOCK) void k_lean(const GroupDev g, const int64_t 
 #if LEAN_REFR
   int32_t lastw[K];  // last spike step, advanced step by step within the window
 #endif
+#if LEAN_REFR == 2
+  double rtime[K];   // refractory time of each neuron (seconds)
+#endif
   bool valid[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -65,6 +71,9 @@ __global__ __launch_bounds__(BLOCK) void k_lean(const GroupDev g, const int64_t 
     last0[k] = last[k];
 #if LEAN_REFR
     lastw[k] = last[k];
+#endif
+#if LEAN_REFR == 2
+    rtime[k] = valid[k] ? g.refr_arr[i] : 0.0;
 #endif
 #pragma unroll
     for (int j = 0; j < M; ++j) s[k][j] = valid[k] ? static_cast<const T *>(g.S)[(int64_t)j * N + i] : (T)0;
@@ -122,7 +131,14 @@ __global__ __launch_bounds__(BLOCK) void k_lean(const GroupDev g, const int64_t 
 #if LEAN_REFR
         // the tail of step n0 + w - 1: the reset holds for `period` steps
         // from the spike's own step on
-        if (lastw[k] != NEVER && (n0 + w - 1) - (int64_t)lastw[k] < (int64_t)g.period) lean_reset(s[k]);
+        if (lastw[k] != NEVER && (n0 + w - 1) - (int64_t)lastw[k] < (int64_t)g.period) {
+#if LEAN_REFR == 2
+          const double na = (double)lastw[k] * g.dt + rtime[k];
+          if (na > ((double)(n0 + w - 1) * g.dt - g.dt * 0.25)) lean_reset(s[k]);
+#else
+          lean_reset(s[k]);
+#endif
+        }
 #else
         if (prev[k]) lean_reset(s[k]);
 #endif
@@ -138,7 +154,11 @@ __global__ __launch_bounds__(BLOCK) void k_lean(const GroupDev g, const int64_t 
         // next allowed spike time in seconds, as the generic sweep tests it
         const double t = (double)(n0 + w) * g.dt + 0.0;
         const double lt = (double)lastw[k] * g.dt + 0.0;
+#if LEAN_REFR == 2
+        sp = lt + rtime[k] <= t;
+#else
         sp = lt + g.refr_time <= t;
+#endif
       }
       if (sp) lastw[k] = (int32_t)(n0 + w);
 #else

@@ -460,6 +460,54 @@ def test_lean_kernel_refractory_fp64_bit_exact(npt):
     assert first_divergence(*rasters["no refractoriness"], s, i) is not None
 
 
+def test_lean_kernel_per_neuron_refractory_fp64_bit_exact():
+    """A refractory time per neuron (uniform in [1, 8] ms) on the lean
+    kernel: the hold also ends once last + refractory <= t - dt/4 and the
+    threshold is gated per neuron.  FP64 raster bit-exact and state within
+    1e-12 against the oracle over several windows; the generic kernel gives
+    the same raster."""
+    d = W.cuba(4000, seed=37, delay_steps=10)
+    refr = np.random.default_rng(4).uniform(1e-3, 8e-3, 4000)
+    net, gid = oracle_cuba(d, refractory=refr)
+
+    def build(lean):
+        from brian_amd._lib import NetHandle
+        from tests.netbuild import cuba_propagator
+        A, C = cuba_propagator()
+        h = NetHandle(1e-4, 64)
+        g = h.add_group(d["S0"])
+        h.set_linear(g, A, C)
+        h.set_threshold(g, 0, d["Vt"])
+        h.add_reset_op(g, 0, 0, d["Vr"])
+        h.set_refractory(g, int(refr.max() / 1e-4) + 1, refr.max() - 0.5e-4, per_neuron=refr)
+        for (ip, ix, w), off, n, var in ((d["exc"], 0, d["Ne"], 1), (d["inh"], d["Ne"], d["Ni"], 2)):
+            h.add_connection(g, off, n, g, 0, 4000, var, ip, ix, w, delay_steps=d["delay_steps"])
+        h.record(g, 1 << 22)
+        h.prepare(neurons_per_thread=1)
+        h.set_tuning("lean", lean)
+        return h, g
+
+    h, g = build(1)
+    total = 0
+    for chunk, window in ((137, 11), (500, 11), (333, 7), (1200, 11)):
+        h.set_tuning("window", window)
+        h.run(chunk)
+        net.run(chunk)
+        total += chunk
+        _state_close(h, g, net, gid, 4000)
+    s, i = h.get_spikes(g)
+    rs, ri = net.raster(gid)
+    h.close()
+    assert len(rs) > 1000
+    assert first_divergence(s, i, rs, ri) is None
+    h2, g2 = build(0)
+    h2.run(total)
+    s2, i2 = h2.get_spikes(g2)
+    h2.close()
+    np.testing.assert_array_equal(s2, s)
+    np.testing.assert_array_equal(i2, i)
+
+
 def test_lean_kernel_refractory_fp32_rate():
     """Refractoriness on the lean kernel in FP32: population rate within 5%
     of the FP64 oracle."""
